@@ -1,0 +1,15 @@
+"""fluidsim_amd -- MI355X-native uniform-MAC projection backend (HIP, gfx950).
+
+The compute path is libfluidsim_hip.so (C ABI in include/fluidsim_hip.h).  `FluidGridHIP` mirrors the
+reference's FluidGridBase method set over that ABI.
+"""
+from ._abi import (FS_PRECOND_DIAGONAL, FS_PRECOND_MULTIGRID, FS_TRANSPORT_NONE, FS_TRANSPORT_REFERENCE,
+                   FS_TRANSPORT_SEMI_LAGRANGIAN, FluidSimError, lib)
+from .grid import (Context, FluidGridHIP, FluidGridState, FluidMixture, MixtureComponent, ProjectionResult,
+                   StepTimings, default_context)
+
+__all__ = [
+    "FS_PRECOND_DIAGONAL", "FS_PRECOND_MULTIGRID", "FS_TRANSPORT_NONE", "FS_TRANSPORT_REFERENCE",
+    "FS_TRANSPORT_SEMI_LAGRANGIAN", "FluidSimError", "lib", "Context", "FluidGridHIP", "FluidGridState",
+    "FluidMixture", "MixtureComponent", "ProjectionResult", "StepTimings", "default_context",
+]

@@ -1,0 +1,149 @@
+"""ctypes binding of libfluidsim_hip.so (declared in include/fluidsim_hip.h).
+
+The product path: every call goes to the HIP library.  There is no CPU fallback; if the library is
+missing or cannot be loaded, `lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfluidsim_hip.so")
+
+FS_OK = 0
+FS_E_INVALID_ARGUMENT = 1
+FS_E_LOGIC = 2
+FS_E_UNSUPPORTED = 3
+FS_E_DEVICE = 4
+FS_E_RUNTIME = 5
+
+FS_TRANSPORT_REFERENCE = 0
+FS_TRANSPORT_NONE = 1
+FS_TRANSPORT_SEMI_LAGRANGIAN = 2
+
+FS_PRECOND_DIAGONAL = 0
+FS_PRECOND_MULTIGRID = 1
+
+
+class FsProjectionResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("max_divergence", C.c_double),
+        ("converged", C.c_int),
+        ("physical_steps", C.c_int),
+        ("transport_steps", C.c_int),
+        ("relaxed", C.c_int),
+        ("max_volume_error", C.c_double),
+        ("rms_volume_error", C.c_double),
+        ("viscosity_iterations", C.c_int),
+        ("precision_limited", C.c_int),
+        ("restarts", C.c_int),
+        ("initial_residual", C.c_double),
+        ("final_residual", C.c_double),
+        ("rhs_max", C.c_double),
+    ]
+
+
+class FsStepTimings(C.Structure):
+    _fields_ = [
+        ("transport_ms", C.c_double),
+        ("viscosity_ms", C.c_double),
+        ("forces_ms", C.c_double),
+        ("pressure_ms", C.c_double),
+        ("selection_ms", C.c_double),
+        ("regrid_ms", C.c_double),
+    ]
+
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+# name -> argtypes (every function returns fs_status except the two string getters)
+SIGNATURES = {
+    "fs_context_create": [C.c_int, C.POINTER(_vp)],
+    "fs_context_destroy": [_vp],
+    "fs_context_synchronize": [_vp],
+    "fs_grid_create": [_vp, C.c_int, C.c_int, C.c_double, C.POINTER(_vp)],
+    "fs_grid_destroy": [_vp],
+    "fs_grid_shape": [_vp, _ip, _ip, _dp],
+    "fs_set_origin": [_vp, C.c_double, C.c_double],
+    "fs_get_origin": [_vp, _dp, _dp],
+    "fs_configure_materials": [_vp, C.c_int, _dp, _dp, C.c_int, _dp, _dp],
+    "fs_set_gravity": [_vp, C.c_double],
+    "fs_get_gravity": [_vp, _dp],
+    "fs_set_surface_tension": [_vp, C.c_double],
+    "fs_capillary_step_limit": [_vp, _dp],
+    "fs_set_preconditioner": [_vp, C.c_int],
+    "fs_set_transport": [_vp, C.c_int],
+    "fs_upload": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "fs_download": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "fs_invalidate_caches": [_vp],
+    "fs_step": [_vp, C.c_double, C.c_int, C.c_double, C.POINTER(FsStepTimings), C.POINTER(FsProjectionResult)],
+    "fs_project": [_vp, C.c_double, C.c_int, C.c_double, C.POINTER(FsProjectionResult)],
+    "fs_apply_gravity": [_vp, C.c_double],
+    "fs_add_vortex": [_vp, C.c_double, C.c_double, C.c_double, C.c_double],
+    "fs_max_divergence": [_vp, _dp],
+    "fs_advect_semi_lagrangian": [_vp, C.c_double],
+    "fs_snapshot_save": [_vp],
+    "fs_snapshot_restore": [_vp],
+    "fs_mg_levels": [_vp, _ip],
+    "fs_mg_level_shape": [_vp, C.c_int, _ip, _ip],
+    "fs_mg_level_centers": [_vp, C.c_int, _dp, _dp],
+    "fs_mg_level_weights": [_vp, C.c_int, _dp, _dp, _fp],
+    "fs_mg_level_prolong": [_vp, C.c_int, C.c_int, _ip, _ip, _dp],
+    "fs_mg_level_restrict": [_vp, C.c_int, C.c_int, _ip, _ip, _dp],
+    "fs_mg_coarse_factor": [_vp, _dp, _ip],
+    "fs_mg_apply": [_vp, _dp, C.c_double, _dp],
+    "fs_face_weights": [_vp, _dp, _dp],
+    "fs_time_kernel": [_vp, C.c_int, C.c_int, _dp],
+    "fs_grid_device_bytes": [_vp, C.POINTER(C.c_size_t)],
+}
+
+_lib = None
+
+
+class FluidSimError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[fs_status {code}] {message}")
+        self.code = code
+
+
+def lib():
+    """Loads the HIP library; raises if it is not built.  No fallback exists."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m fluidsim_amd.build` "
+            "(the HIP backend has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.fs_last_error.restype = C.c_char_p
+    L.fs_last_error.argtypes = []
+    L.fs_version.restype = C.c_char_p
+    L.fs_version.argtypes = []
+    _lib = L
+    return L
+
+
+def check(status, what=""):
+    if status != FS_OK:
+        msg = lib().fs_last_error().decode(errors="replace")
+        if status == FS_E_INVALID_ARGUMENT:
+            raise ValueError(f"{what}: {msg}")
+        if status == FS_E_LOGIC:
+            raise RuntimeError(f"{what}: {msg}")
+        if status == FS_E_UNSUPPORTED:
+            raise NotImplementedError(f"{what}: {msg}")
+        raise FluidSimError(status, f"{what}: {msg}")
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)

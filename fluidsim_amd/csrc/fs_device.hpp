@@ -1,0 +1,189 @@
+// fs_device.hpp -- device-side structures and helpers shared by the HIP kernels and the host driver.
+//
+// Memory layout (all resident in HBM, one set per grid):
+//   * Cell fields use a padded row-major layout: cell (x, y) lives at (y + 1) * stride + (x + 1) with
+//     stride = W + 2 and H + 2 rows.  The one-cell halo is never written and stays zero, so every
+//     5-point stencil runs without branches; wall faces carry zero weight.
+//   * Face fields (u, v) keep the reference layout: u is (W+1) x H, v is W x (H+1), row-major.
+//   * FP64: physical state (fractions, masses, u, v, accumulated pressure), face weights and the
+//     multigrid conductances (built bit-exactly like the reference), the pressure rhs.
+//   * FP32: PCG working vectors (x correction, r, z, d, q) and multigrid smoothing data.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsim {
+
+// Solver status codes kept on the device.
+enum PcgStatus : int { kRunning = 0, kConverged = 1, kStopped = 2 };
+
+// Device-resident PCG scalar block.  Kernels of one iteration communicate only through this block,
+// so the host never reads a scalar inside the loop (it polls `status` once per batch).
+struct PcgScalars {
+    double rz;          // r . z of the current direction (reference: residual_dot_correction)
+    double alpha;
+    double beta;
+    double r_mean;      // mean of the residual handed to the preconditioner
+    double z_mean;      // mean of the preconditioned residual (removed before use)
+    double tol;         // residual tolerance, max-norm, in rhs units
+    double rmax;        // last recurrence residual max-norm
+    double true_rmax;   // last FP64 true residual max-norm
+    double bmax;        // max |rhs| after mean removal
+    double b_mean;      // rhs mean removed from the divergence
+    double p_mean;      // pressure mean (removed)
+    double dq;          // last d . A d
+    double div_max;     // last max |div u| (1/s)
+    double r_sum;       // last residual sum
+    double initial_rmax;  // max |b - A p| of the predicted pressure
+    int iterations;
+    int max_iterations;
+    int status;
+    int verify;         // recurrence says converged: check the true residual this iteration
+    int restart;        // next beta is zero (true residual replaced the recurrence)
+    int restarts;
+    int mg_failed;      // coarse factorisation failed (the hierarchy must be rebuilt next time)
+    int pad1;
+};
+
+// One multigrid level as the FP32 smoothing kernels see it.
+struct LevelView {
+    int w, h, s;             // interior width/height, padded stride (w + 2)
+    const float* right;      // conductance of the face right of each cell (padded)
+    const float* down;       // conductance of the face below each cell (padded)
+    const float* invd;       // 1 / diagonal (padded)
+    float* rhs;              // padded
+    float* sol[2];           // ping-pong solution buffers (padded)
+    float* res;              // residual scratch (padded)
+};
+
+// Per-axis prolongation map of a fine level (first/second coarse index, weight of second).
+struct AxisProlong {
+    const int* first;
+    const int* second;
+    const float* weight;
+};
+
+// Per-axis restriction stencil of a coarse index (<= 4 fine taps, exact transpose of prolongation).
+struct AxisRestrict {
+    const int* count;   // [n]
+    const int* index;   // [n * 4]
+    const float* weight;  // [n * 4]
+};
+
+// Reduction scratch for one kernel: per-block partials plus the arrival counter of the last-block finish.
+struct ReduceSlot {
+    double* partials;
+    unsigned* counter;
+};
+
+__device__ __forceinline__ int pidx(int x, int y, int s) { return (y + 1) * s + (x + 1); }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Block-wide sum in a fixed order; result valid in thread 0.  `lds` holds blockDim/64 doubles.
+template <int NT> __device__ __forceinline__ double block_sum(double v, double* lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i)
+            r += lds[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+template <int NT> __device__ __forceinline__ double block_max(double v, double* lds)
+{
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i)
+            r = fmax(r, lds[i]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Publish this block's partials (written by thread 0 before the call) and report whether this
+// block arrived last.  The last block then reads every partial in block order, so the result does
+// not depend on arrival order.
+__device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned nblocks)
+{
+    __shared__ int is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned ticket = atomicAdd(counter, 1u);
+        is_last = ticket == nblocks - 1;
+    }
+    __syncthreads();
+    if (is_last)
+        __threadfence();
+    return is_last != 0;
+}
+
+// Reads a partial written by another block in this launch (bypasses the non-coherent L1).
+__device__ __forceinline__ double load_partial(const double* p, int i)
+{
+    return __hip_atomic_load(const_cast<double*>(p) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of `n` partials with stride `k` starting at `offset`, in a fixed order; every thread returns it.
+template <int NT>
+__device__ __forceinline__ double finish_sum(const double* partials, int n, int k, int offset, double* lds)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT)
+        v += load_partial(partials, i * k + offset);
+    double r = block_sum<NT>(v, lds);
+    __shared__ double out;
+    if (threadIdx.x == 0)
+        out = r;
+    __syncthreads();
+    r = out;
+    __syncthreads();
+    return r;
+}
+
+template <int NT>
+__device__ __forceinline__ double finish_max(const double* partials, int n, int k, int offset, double* lds)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT)
+        v = fmax(v, load_partial(partials, i * k + offset));
+    double r = block_max<NT>(v, lds);
+    __shared__ double out;
+    if (threadIdx.x == 0)
+        out = r;
+    __syncthreads();
+    r = out;
+    __syncthreads();
+    return r;
+}
+
+}  // namespace fsim

@@ -1,0 +1,1287 @@
+// fs_host.cpp -- host side of libfluidsim_hip.so: device buffers, multigrid hierarchy, the projection
+// driver and the extern "C" ABI declared in include/fluidsim_hip.h.
+//
+// One fs_grid owns every buffer of one simulation on one device.  A projection is a fixed sequence of
+// kernels on the context's stream; the PCG iteration body is captured once into a hipGraph and
+// replayed.  All loop decisions (alpha, beta, convergence, restarts, iteration budget) live in a
+// device scalar block, so the host only polls a status word between batches of iterations.
+#include "fs_kernels.hpp"
+#include "../../include/fluidsim_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace fsim {
+
+namespace {
+
+struct StatusError : std::runtime_error {
+    fs_status code;
+    StatusError(fs_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(fs_status code, const std::string& message) { throw StatusError(code, message); }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        fail(FS_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPC(call) hip_check((call), #call)
+
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Allocates and zero-fills (halos of padded fields must start, and stay, zero).
+    size_t alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count == 0)
+            return 0;
+        HIPC(hipMalloc(&p, count * sizeof(T)));
+        HIPC(hipMemset(p, 0, count * sizeof(T)));
+        return count * sizeof(T);
+    }
+};
+
+struct Level {
+    int w = 0, h = 0, s = 0;
+    std::vector<double> cx, cy;
+    // transfers to the next coarser level (empty on the last level)
+    std::vector<int> pxf, pxs, pyf, pys;
+    std::vector<double> pxw, pyw;
+    std::vector<int> rxc, rxi, ryc, ryi;
+    std::vector<double> rxw, ryw;
+
+    DevBuf<double> d_cx, d_cy, right64, down64;
+    DevBuf<float> right32, down32, invd32, rhs, sol0, sol1, res;
+    DevBuf<int> d_pxf, d_pxs, d_pyf, d_pys, d_rxc, d_rxi, d_ryc, d_ryi;
+    DevBuf<float> d_pxw, d_pyw, d_rxw, d_ryw;
+
+    size_t padded() const { return (size_t)s * (h + 2); }
+};
+
+// buildTransfers for one axis (PressureMultigrid.cpp:64-98): bilinear prolongation on cell centres and
+// restriction as its exact transpose.
+void build_axis(const std::vector<double>& fc, const std::vector<double>& cc, std::vector<int>& first,
+    std::vector<int>& second, std::vector<double>& weight, std::vector<int>& count, std::vector<int>& index,
+    std::vector<double>& rweight)
+{
+    const int nf = (int)fc.size(), nc = (int)cc.size();
+    first.assign(nf, 0);
+    second.assign(nf, 0);
+    weight.assign(nf, 0.0);
+    count.assign(nc, 0);
+    index.assign((size_t)nc * 4, -1);
+    rweight.assign((size_t)nc * 4, 0.0);
+    for (int f = 0; f < nf; ++f) {
+        const int parent = f / 2;
+        const int a = std::max(0, parent - int(fc[f] < cc[parent]));
+        const int b = std::min(a + 1, nc - 1);
+        const double wgt = a == b ? 0.0 : std::clamp((fc[f] - cc[a]) / (cc[b] - cc[a]), 0.0, 1.0);
+        first[f] = a;
+        second[f] = b;
+        weight[f] = wgt;
+        auto append = [&](int c, double value) {
+            if (value == 0.0)
+                return;
+            if (count[c] >= 4)
+                fail(FS_E_RUNTIME, "multigrid restriction stencil exceeds four taps");
+            const int k = count[c]++;
+            index[(size_t)c * 4 + k] = f;
+            rweight[(size_t)c * 4 + k] = value;
+        };
+        append(a, 1.0 - wgt);
+        append(b, wgt);
+    }
+}
+
+template <class T> void upload_vec(DevBuf<T>& d, const std::vector<T>& h, size_t& bytes)
+{
+    bytes += d.alloc(h.size());
+    if (!h.empty())
+        HIPC(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+std::vector<float> to_float(const std::vector<double>& v)
+{
+    std::vector<float> r(v.size());
+    for (size_t i = 0; i < v.size(); ++i)
+        r[i] = (float)v[i];
+    return r;
+}
+
+thread_local std::string g_last_error;
+
+}  // namespace
+
+}  // namespace fsim
+
+using namespace fsim;
+
+struct fs_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    PcgScalars* pinned = nullptr;   // host-pinned mirror of a grid's scalar block
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct fs_grid {
+    fs_context* ctx = nullptr;
+    int W = 0, H = 0;
+    double h = 1.0;
+    double ox = 0.0, oy = 0.0;
+    double gravity_y = 100.0;  // FluidGridCPU.h:37
+    double sigma = 0.0;
+    std::vector<double> rho_l, rho_g, mu_l, mu_g;
+    fs_preconditioner precond = FS_PRECOND_MULTIGRID;
+    fs_transport transport = FS_TRANSPORT_REFERENCE;
+    bool uploaded = false;
+    bool coeff_valid = false;
+    bool mg_valid = false;
+    bool has_snapshot = false;
+    size_t bytes = 0;
+    int predicted = 8;
+
+    DevBuf<double> fraction, u, v, lmass, gmass, density, p, p_prev, b, tmp64;
+    DevBuf<double> snap_fraction, snap_u, snap_v, snap_p, snap_p_prev, snap_lmass, snap_gmass;
+    DevBuf<double> sl_u, sl_v, sl_cells;
+    DevBuf<float> x, r, d, q, zdiag;
+    DevBuf<PcgScalars> S;
+    DevBuf<double> partials;
+    DevBuf<unsigned> counter;
+    DevBuf<double> factor;
+    int coarse_n = 0;
+    std::vector<Level> levels;
+    int tail_start = 0;
+    TailDesc tail{};
+    DevBuf<TailDesc> tail_dev;
+
+    hipGraphExec_t iter_exec = nullptr;
+    hipGraph_t iter_graph = nullptr;
+    fs_preconditioner graph_precond = FS_PRECOND_MULTIGRID;
+
+    ~fs_grid() { drop_graph(); }
+
+    size_t N() const { return (size_t)W * H; }
+    size_t NP() const { return (size_t)(W + 2) * (H + 2); }
+    size_t NU() const { return (size_t)(W + 1) * H; }
+    size_t NV() const { return (size_t)W * (H + 1); }
+    hipStream_t st() const { return ctx->stream; }
+
+    void drop_graph()
+    {
+        if (iter_exec)
+            (void)hipGraphExecDestroy(iter_exec);
+        if (iter_graph)
+            (void)hipGraphDestroy(iter_graph);
+        iter_exec = nullptr;
+        iter_graph = nullptr;
+    }
+
+    GridDev dev()
+    {
+        GridDev g{};
+        g.W = W;
+        g.H = H;
+        g.s = W + 2;
+        g.h = h;
+        g.nl = (int)rho_l.size();
+        g.ng = (int)rho_g.size();
+        g.fraction = fraction.p;
+        g.u = u.p;
+        g.v = v.p;
+        g.lmass = lmass.p;
+        g.gmass = gmass.p;
+        g.density = density.p;
+        g.p = p.p;
+        g.p_prev = p_prev.p;
+        g.b = b.p;
+        g.right64 = levels[0].right64.p;
+        g.down64 = levels[0].down64.p;
+        g.right32 = levels[0].right32.p;
+        g.down32 = levels[0].down32.p;
+        g.invd32 = levels[0].invd32.p;
+        g.x = x.p;
+        g.r = r.p;
+        g.d = d.p;
+        g.q = q.p;
+        g.zdiag = zdiag.p;
+        g.S = S.p;
+        g.partials = partials.p;
+        g.counter = counter.p;
+        return g;
+    }
+
+    LevelView view(int l)
+    {
+        Level& L = levels[l];
+        LevelView v{};
+        v.w = L.w;
+        v.h = L.h;
+        v.s = L.s;
+        v.right = L.right32.p;
+        v.down = L.down32.p;
+        v.invd = L.invd32.p;
+        v.rhs = l == 0 ? r.p : L.rhs.p;
+        v.sol[0] = L.sol0.p;
+        v.sol[1] = L.sol1.p;
+        v.res = L.res.p;
+        return v;
+    }
+
+    const int* status_ptr() const { return &S.p->status; }
+    const double* rmean_ptr() const { return &S.p->r_mean; }
+
+    // ---------------------------------------------------------------------------------------
+    void allocate()
+    {
+        const size_t NPd = NP();
+        bytes += fraction.alloc(N());
+        bytes += u.alloc(NU());
+        bytes += v.alloc(NV());
+        bytes += density.alloc(N());
+        bytes += p.alloc(NPd);
+        bytes += p_prev.alloc(NPd);
+        bytes += b.alloc(NPd);
+        bytes += tmp64.alloc(N());
+        bytes += x.alloc(NPd);
+        bytes += r.alloc(NPd);
+        bytes += d.alloc(NPd);
+        bytes += q.alloc(NPd);
+        bytes += zdiag.alloc(NPd);
+        bytes += S.alloc(1);
+        bytes += counter.alloc(1);
+        const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
+        bytes += partials.alloc(std::max<size_t>(4 * blocks, 64));
+        build_hierarchy();
+    }
+
+    // PressureMultigrid constructor (PressureMultigrid.cpp:38-62): halve until w*h <= 64.
+    void build_hierarchy()
+    {
+        levels.clear();
+        int w = W, hh = H, spacing = 1;
+        for (;;) {
+            levels.emplace_back();
+            Level& L = levels.back();
+            L.w = w;
+            L.h = hh;
+            L.s = w + 2;
+            L.cx.resize(w);
+            L.cy.resize(hh);
+            for (int i = 0; i < w; ++i)
+                L.cx[i] = 0.5 * (i * spacing + std::min((i + 1) * spacing, W));
+            for (int i = 0; i < hh; ++i)
+                L.cy[i] = 0.5 * (i * spacing + std::min((i + 1) * spacing, H));
+            if (w * hh <= 64)
+                break;
+            w = (w + 1) / 2;
+            hh = (hh + 1) / 2;
+            spacing *= 2;
+        }
+        for (size_t l = 0; l + 1 < levels.size(); ++l) {
+            Level& F = levels[l];
+            Level& Cl = levels[l + 1];
+            build_axis(F.cx, Cl.cx, F.pxf, F.pxs, F.pxw, F.rxc, F.rxi, F.rxw);
+            build_axis(F.cy, Cl.cy, F.pyf, F.pys, F.pyw, F.ryc, F.ryi, F.ryw);
+        }
+        for (size_t l = 0; l < levels.size(); ++l) {
+            Level& L = levels[l];
+            const size_t np = L.padded();
+            upload_vec(L.d_cx, L.cx, bytes);
+            upload_vec(L.d_cy, L.cy, bytes);
+            bytes += L.right64.alloc(np);
+            bytes += L.down64.alloc(np);
+            bytes += L.right32.alloc(np);
+            bytes += L.down32.alloc(np);
+            bytes += L.invd32.alloc(np);
+            if (l > 0)
+                bytes += L.rhs.alloc(np);
+            bytes += L.sol0.alloc(np);
+            bytes += L.sol1.alloc(np);
+            bytes += L.res.alloc(np);
+            if (l + 1 < levels.size()) {
+                upload_vec(L.d_pxf, L.pxf, bytes);
+                upload_vec(L.d_pxs, L.pxs, bytes);
+                upload_vec(L.d_pyf, L.pyf, bytes);
+                upload_vec(L.d_pys, L.pys, bytes);
+                upload_vec(L.d_rxc, L.rxc, bytes);
+                upload_vec(L.d_rxi, L.rxi, bytes);
+                upload_vec(L.d_ryc, L.ryc, bytes);
+                upload_vec(L.d_ryi, L.ryi, bytes);
+                upload_vec(L.d_pxw, to_float(L.pxw), bytes);
+                upload_vec(L.d_pyw, to_float(L.pyw), bytes);
+                upload_vec(L.d_rxw, to_float(L.rxw), bytes);
+                upload_vec(L.d_ryw, to_float(L.ryw), bytes);
+            }
+        }
+        const Level& last = levels.back();
+        coarse_n = last.w * last.h - 1;
+        bytes += factor.alloc((size_t)std::max(coarse_n, 1) * std::max(coarse_n, 1));
+
+        // Coarse tail: the deepest levels whose LDS footprint fits one workgroup.
+        const int nlev = (int)levels.size();
+        const size_t budget = 48 * 1024 / sizeof(float);
+        size_t floats = 6 * levels[nlev - 1].padded();
+        tail_start = nlev - 1;
+        for (int l = nlev - 2; l >= 0; --l) {
+            const size_t f = 6 * levels[l].padded();
+            if (floats + f > budget || nlev - l > kMaxTailLevels)
+                break;
+            floats += f;
+            tail_start = l;
+        }
+        std::memset(&tail, 0, sizeof(tail));
+        tail.levels = nlev - tail_start;
+        tail.coarse_n = coarse_n;
+        tail.factor = factor.p;
+        tail.top_rhs = tail_start == 0 ? r.p : levels[tail_start].rhs.p;
+        tail.top_sol[0] = levels[tail_start].sol0.p;
+        tail.top_sol[1] = levels[tail_start].sol1.p;
+        int off = 0;
+        for (int i = 0; i < tail.levels; ++i) {
+            Level& L = levels[tail_start + i];
+            TailLevel& T = tail.lev[i];
+            T.w = L.w;
+            T.h = L.h;
+            T.s = L.s;
+            T.right = L.right32.p;
+            T.down = L.down32.p;
+            T.invd = L.invd32.p;
+            T.px_first = L.d_pxf.p; T.px_second = L.d_pxs.p; T.px_w = L.d_pxw.p;
+            T.py_first = L.d_pyf.p; T.py_second = L.d_pys.p; T.py_w = L.d_pyw.p;
+            T.rx_count = L.d_rxc.p; T.rx_index = L.d_rxi.p; T.rx_w = L.d_rxw.p;
+            T.ry_count = L.d_ryc.p; T.ry_index = L.d_ryi.p; T.ry_w = L.d_ryw.p;
+            tail.lds_offset[i] = off;
+            off += (int)(6 * L.padded());
+        }
+        bytes += tail_dev.alloc(1);
+        HIPC(hipMemcpy(tail_dev.p, &tail, sizeof(TailDesc), hipMemcpyHostToDevice));
+    }
+
+    // ---------------------------------------------------------------------------------------
+    void rebuild_coefficients()
+    {
+        GridDev g = dev();
+        launch_density(st(), g);
+        launch_face_weights(st(), g);
+        Level& L0 = levels[0];
+        launch_level_diag(st(), W, H, L0.right64.p, L0.down64.p, L0.right32.p, L0.down32.p, L0.invd32.p);
+        coeff_valid = true;
+        mg_valid = false;
+    }
+
+    // PressureMultigrid::rebuild (PressureMultigrid.cpp:119-140) on the device, FP64.
+    void rebuild_hierarchy()
+    {
+        for (size_t l = 0; l + 1 < levels.size(); ++l) {
+            Level& F = levels[l];
+            Level& Cl = levels[l + 1];
+            launch_coarsen(st(), F.w, F.h, F.d_cx.p, F.d_cy.p, F.right64.p, F.down64.p, Cl.w, Cl.h, Cl.d_cx.p,
+                Cl.d_cy.p, Cl.right64.p, Cl.down64.p);
+            launch_level_diag(st(), Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p, Cl.invd32.p);
+        }
+        const Level& last = levels.back();
+        launch_cholesky(st(), last.w, last.h, last.right64.p, last.down64.p, factor.p, coarse_n, S.p);
+        mg_valid = true;
+    }
+
+    // One V(4,4) cycle on the residual r (minus its mean) into levels[0].sol0.
+    void vcycle(const int* status)
+    {
+        const int T = tail_start;
+        for (int l = 0; l < T; ++l) {
+            LevelView L = view(l);
+            const float* rhs = l == 0 ? r.p : L.rhs;
+            const double* shift = l == 0 ? rmean_ptr() : nullptr;
+            launch_rb_sweep(st(), L, 0, rhs, shift, 0, 1, status);
+            launch_rb_sweep(st(), L, 1, rhs, shift, 0, 0, status);
+            launch_rb_sweep(st(), L, 0, rhs, shift, 0, 0, status);
+            launch_rb_sweep(st(), L, 1, rhs, shift, 0, 0, status);
+            launch_mg_residual(st(), L, 0, rhs, shift, status);
+            LevelView Cl = view(l + 1);
+            const Level& F = levels[l];
+            AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
+            AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+            launch_mg_restrict(st(), L, Cl, rx, ry, status);
+        }
+        launch_mg_tail(st(), tail_dev.p, tail, 0, T == 0 ? rmean_ptr() : nullptr, status);
+        for (int l = T - 1; l >= 0; --l) {
+            LevelView L = view(l);
+            LevelView Cl = view(l + 1);
+            const Level& F = levels[l];
+            AxisProlong px{F.d_pxf.p, F.d_pxs.p, F.d_pxw.p};
+            AxisProlong py{F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
+            launch_mg_prolong(st(), L, 0, Cl, 0, px, py, status);
+            const float* rhs = l == 0 ? r.p : L.rhs;
+            const double* shift = l == 0 ? rmean_ptr() : nullptr;
+            launch_rb_sweep(st(), L, 0, rhs, shift, 1, 0, status);
+            launch_rb_sweep(st(), L, 1, rhs, shift, 1, 0, status);
+            launch_rb_sweep(st(), L, 0, rhs, shift, 1, 0, status);
+            launch_rb_sweep(st(), L, 1, rhs, shift, 1, 0, status);
+        }
+    }
+
+    const float* z_ptr() const { return precond == FS_PRECOND_MULTIGRID ? levels[0].sol0.p : zdiag.p; }
+
+    void precondition(const int* status)
+    {
+        if (precond == FS_PRECOND_MULTIGRID)
+            vcycle(status);
+        else
+            launch_precond_diag(st(), dev());
+    }
+
+    // The loop body (FluidGridCPU.cpp:1779-1819), every decision taken on the device.
+    void iteration_kernels()
+    {
+        GridDev g = dev();
+        launch_pcg_ad(st(), g);
+        launch_pcg_update(st(), g);
+        launch_flush(st(), g);
+        launch_true_residual(st(), g, 1);
+        precondition(status_ptr());
+        launch_pcg_rz(st(), g, z_ptr());
+        launch_pcg_direction(st(), g, z_ptr());
+    }
+
+    void ensure_graph()
+    {
+        if (iter_exec && graph_precond == precond)
+            return;
+        drop_graph();
+        HIPC(hipStreamBeginCapture(st(), hipStreamCaptureModeRelaxed));
+        iteration_kernels();
+        HIPC(hipStreamEndCapture(st(), &iter_graph));
+        HIPC(hipGraphInstantiate(&iter_exec, iter_graph, nullptr, nullptr, 0));
+        graph_precond = precond;
+    }
+
+    PcgScalars read_scalars()
+    {
+        HIPC(hipMemcpyAsync(ctx->pinned, S.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, st()));
+        HIPC(hipStreamSynchronize(st()));
+        return *ctx->pinned;
+    }
+
+    // FluidGridCPU::project (FluidGridCPU.cpp:1030-1124)
+    fs_projection_result project(double dt, int max_iterations, double tol)
+    {
+        fs_projection_result res{};
+        res.max_divergence = std::numeric_limits<double>::infinity();
+        res.physical_steps = 1;
+        if (!uploaded)
+            fail(FS_E_LOGIC, "Upload the grid state before stepping or projecting");
+        if (dt <= 0.0 || max_iterations < 0)
+            return res;
+        const double scale = h * h / dt;
+        GridDev g = dev();
+        ensure_graph();
+        launch_init_scalars(st(), S.p, tol * scale, max_iterations);
+        if (!coeff_valid)
+            rebuild_coefficients();
+        g = dev();
+        launch_rhs(st(), g, scale);
+        launch_rhs_center(st(), g);
+        launch_zero_pressure(st(), g);
+        launch_predict(st(), g);
+        launch_sub_mean_p(st(), g, 0);
+        launch_true_residual(st(), g, 0);
+        if (precond == FS_PRECOND_MULTIGRID && !mg_valid)
+            rebuild_hierarchy();
+        precondition(status_ptr());
+        launch_pcg_rz(st(), g, z_ptr());
+        launch_pcg_direction(st(), g, z_ptr());
+        PcgScalars s = read_scalars();
+        if (s.status == kRunning) {
+            int batch = std::max(1, std::min(predicted, 256));
+            for (;;) {
+                for (int i = 0; i < batch; ++i)
+                    HIPC(hipGraphLaunch(iter_exec, st()));
+                s = read_scalars();
+                if (s.status != kRunning)
+                    break;
+                batch = 2;
+            }
+            predicted = std::max(1, s.iterations);
+        }
+        launch_final_flush_sum(st(), g);
+        launch_sub_mean_p(st(), g, 1);
+        launch_correct_velocity(st(), g, dt);
+        launch_max_divergence(st(), g);
+        s = read_scalars();
+        if (s.mg_failed)
+            mg_valid = false;
+        if (s.status == kFailed)
+            return res;
+        res.iterations = s.iterations;
+        res.max_divergence = s.div_max;
+        res.converged = s.div_max <= tol;
+        res.restarts = s.restarts;
+        res.initial_residual = s.initial_rmax;
+        res.final_residual = s.true_rmax;
+        res.rhs_max = s.bmax;
+        return res;
+    }
+
+    double max_divergence()
+    {
+        launch_max_divergence(st(), dev());
+        return read_scalars().div_max;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+template <class F> fs_status guard(F&& f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return FS_OK;
+    } catch (const StatusError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return FS_E_RUNTIME;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FS_E_RUNTIME;
+    }
+}
+
+void need(bool cond, fs_status code, const char* message)
+{
+    if (!cond)
+        fail(code, message);
+}
+
+void set_device(fs_context* ctx) { HIPC(hipSetDevice(ctx->device)); }
+
+}  // namespace
+
+extern "C" {
+
+const char* fs_version(void) { return "fluidsim_hip 0.1 (gfx950)"; }
+
+const char* fs_last_error(void) { return g_last_error.c_str(); }
+
+fs_status fs_context_create(int device, fs_context** out)
+{
+    return guard([&] {
+        need(out != nullptr, FS_E_INVALID_ARGUMENT, "fs_context_create: null output pointer");
+        int count = 0;
+        HIPC(hipGetDeviceCount(&count));
+        need(device >= 0 && device < count, FS_E_INVALID_ARGUMENT, "fs_context_create: no such HIP device");
+        auto ctx = std::make_unique<fs_context>();
+        ctx->device = device;
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamCreate(&ctx->stream));
+        HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
+        HIPC(hipEventCreate(&ctx->ev0));
+        HIPC(hipEventCreate(&ctx->ev1));
+        *out = ctx.release();
+    });
+}
+
+fs_status fs_context_destroy(fs_context* ctx)
+{
+    return guard([&] {
+        if (!ctx)
+            return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream)
+            (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->ev0)
+            (void)hipEventDestroy(ctx->ev0);
+        if (ctx->ev1)
+            (void)hipEventDestroy(ctx->ev1);
+        if (ctx->pinned)
+            (void)hipHostFree(ctx->pinned);
+        if (ctx->stream)
+            (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+fs_status fs_context_synchronize(fs_context* ctx)
+{
+    return guard([&] {
+        need(ctx != nullptr, FS_E_INVALID_ARGUMENT, "null context");
+        set_device(ctx);
+        HIPC(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+fs_status fs_grid_create(fs_context* ctx, int width, int height, double cell_size, fs_grid** out)
+{
+    return guard([&] {
+        need(ctx != nullptr && out != nullptr, FS_E_INVALID_ARGUMENT, "fs_grid_create: null pointer");
+        need(width > 0 && height > 0, FS_E_INVALID_ARGUMENT, "fs_grid_create: grid dimensions must be positive");
+        need((long long)width * height <= (1LL << 28), FS_E_INVALID_ARGUMENT, "fs_grid_create: grid too large");
+        need(std::isfinite(cell_size) && cell_size > 0.0, FS_E_INVALID_ARGUMENT,
+            "fs_grid_create: cell size must be positive");
+        set_device(ctx);
+        auto g = std::make_unique<fs_grid>();
+        g->ctx = ctx;
+        g->W = width;
+        g->H = height;
+        g->h = cell_size;
+        g->allocate();
+        *out = g.release();
+    });
+}
+
+fs_status fs_grid_destroy(fs_grid* grid)
+{
+    return guard([&] {
+        if (!grid)
+            return;
+        set_device(grid->ctx);
+        (void)hipStreamSynchronize(grid->st());
+        delete grid;
+    });
+}
+
+fs_status fs_grid_shape(const fs_grid* grid, int* width, int* height, double* cell_size)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        if (width) *width = grid->W;
+        if (height) *height = grid->H;
+        if (cell_size) *cell_size = grid->h;
+    });
+}
+
+fs_status fs_set_origin(fs_grid* grid, double x, double y)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        grid->ox = x;
+        grid->oy = y;
+    });
+}
+
+fs_status fs_get_origin(const fs_grid* grid, double* x, double* y)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        if (x) *x = grid->ox;
+        if (y) *y = grid->oy;
+    });
+}
+
+fs_status fs_configure_materials(fs_grid* grid, int n_liquid, const double* liquid_density,
+    const double* liquid_viscosity, int n_gas, const double* gas_density, const double* gas_viscosity)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(n_liquid >= 0 && n_gas >= 0, FS_E_INVALID_ARGUMENT, "component counts must be non-negative");
+        need((n_liquid == 0 || liquid_density) && (n_gas == 0 || gas_density), FS_E_INVALID_ARGUMENT,
+            "component densities are required");
+        grid->rho_l.assign(liquid_density, liquid_density + n_liquid);
+        grid->rho_g.assign(gas_density, gas_density + n_gas);
+        grid->mu_l.assign(n_liquid, 0.0);
+        grid->mu_g.assign(n_gas, 0.0);
+        for (int k = 0; k < n_liquid && liquid_viscosity; ++k)
+            grid->mu_l[k] = liquid_viscosity[k];
+        for (int k = 0; k < n_gas && gas_viscosity; ++k)
+            grid->mu_g[k] = gas_viscosity[k];
+        // Changing mixtures requires another upload (FluidGridBase.h:41).
+        grid->uploaded = false;
+        grid->has_snapshot = false;
+    });
+}
+
+fs_status fs_set_gravity(fs_grid* grid, double gravity_y)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        grid->gravity_y = gravity_y;
+    });
+}
+
+fs_status fs_get_gravity(const fs_grid* grid, double* gravity_y)
+{
+    return guard([&] {
+        need(grid != nullptr && gravity_y != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *gravity_y = grid->gravity_y;
+    });
+}
+
+fs_status fs_set_surface_tension(fs_grid* grid, double coefficient)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        if (coefficient != 0.0)
+            fail(FS_E_UNSUPPORTED, "Surface tension is not implemented by the HIP backend yet");
+        grid->sigma = 0.0;
+    });
+}
+
+fs_status fs_capillary_step_limit(const fs_grid* grid, double* limit)
+{
+    return guard([&] {
+        need(grid != nullptr && limit != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *limit = std::numeric_limits<double>::infinity();
+    });
+}
+
+fs_status fs_set_preconditioner(fs_grid* grid, fs_preconditioner kind)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(kind == FS_PRECOND_DIAGONAL || kind == FS_PRECOND_MULTIGRID, FS_E_INVALID_ARGUMENT,
+            "unknown preconditioner");
+        grid->precond = kind;
+    });
+}
+
+fs_status fs_set_transport(fs_grid* grid, fs_transport mode)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(mode == FS_TRANSPORT_REFERENCE || mode == FS_TRANSPORT_NONE || mode == FS_TRANSPORT_SEMI_LAGRANGIAN,
+            FS_E_INVALID_ARGUMENT, "unknown transport mode");
+        grid->transport = mode;
+    });
+}
+
+fs_status fs_upload(fs_grid* grid, const double* liquid_fraction, const double* u, const double* v,
+    const double* pressure, const double* liquid_mass, const double* gas_mass)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        for (double mu : grid->mu_l)
+            if (mu != 0.0)
+                fail(FS_E_UNSUPPORTED, "Viscosity is not implemented by the HIP backend yet");
+        for (double mu : grid->mu_g)
+            if (mu != 0.0)
+                fail(FS_E_UNSUPPORTED, "Viscosity is not implemented by the HIP backend yet");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        const size_t N = g.N();
+        const size_t nl = g.rho_l.size(), ng = g.rho_g.size();
+        if (g.lmass.n != nl * N)
+            g.bytes += g.lmass.alloc(nl * N);
+        if (g.gmass.n != ng * N)
+            g.bytes += g.gmass.alloc(ng * N);
+        auto put = [&](DevBuf<double>& dst, const double* src, size_t n) {
+            if (n == 0)
+                return;
+            if (src)
+                HIPC(hipMemcpyAsync(dst.p, src, n * sizeof(double), hipMemcpyHostToDevice, g.st()));
+            else
+                HIPC(hipMemsetAsync(dst.p, 0, n * sizeof(double), g.st()));
+        };
+        put(g.fraction, liquid_fraction, N);
+        put(g.u, u, g.NU());
+        put(g.v, v, g.NV());
+        put(g.lmass, liquid_mass, nl * N);
+        put(g.gmass, gas_mass, ng * N);
+        if (pressure) {
+            HIPC(hipMemcpyAsync(g.tmp64.p, pressure, N * sizeof(double), hipMemcpyHostToDevice, g.st()));
+            launch_pad_copy(g.st(), g.tmp64.p, g.p.p, g.W, g.H, 1);
+        } else {
+            HIPC(hipMemsetAsync(g.p.p, 0, g.NP() * sizeof(double), g.st()));
+        }
+        // upload resets the pressure history (FluidGridCPU.cpp:249)
+        HIPC(hipMemcpyAsync(g.p_prev.p, g.p.p, g.NP() * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+        HIPC(hipStreamSynchronize(g.st()));
+        g.uploaded = true;
+        g.coeff_valid = false;
+        g.mg_valid = false;
+    });
+}
+
+fs_status fs_download(fs_grid* grid, double* liquid_fraction, double* u, double* v, double* pressure,
+    double* liquid_mass, double* gas_mass)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before downloading it");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        const size_t N = g.N();
+        auto get = [&](double* dst, const DevBuf<double>& src, size_t n) {
+            if (dst && n)
+                HIPC(hipMemcpyAsync(dst, src.p, n * sizeof(double), hipMemcpyDeviceToHost, g.st()));
+        };
+        get(liquid_fraction, g.fraction, N);
+        get(u, g.u, g.NU());
+        get(v, g.v, g.NV());
+        get(liquid_mass, g.lmass, g.rho_l.size() * N);
+        get(gas_mass, g.gmass, g.rho_g.size() * N);
+        if (pressure) {
+            launch_pad_copy(g.st(), g.p.p, g.tmp64.p, g.W, g.H, 0);
+            HIPC(hipMemcpyAsync(pressure, g.tmp64.p, N * sizeof(double), hipMemcpyDeviceToHost, g.st()));
+        }
+        HIPC(hipStreamSynchronize(g.st()));
+    });
+}
+
+fs_status fs_invalidate_caches(fs_grid* grid)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        grid->coeff_valid = false;
+        grid->mg_valid = false;
+    });
+}
+
+fs_status fs_project(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
+    fs_projection_result* result)
+{
+    return guard([&] {
+        need(grid != nullptr && result != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        set_device(grid->ctx);
+        *result = grid->project(dt, max_iterations, divergence_tolerance);
+    });
+}
+
+fs_status fs_apply_gravity(fs_grid* grid, double dt)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before applying forces");
+        set_device(grid->ctx);
+        launch_gravity(grid->st(), grid->dev(), grid->gravity_y, dt);
+        HIPC(hipStreamSynchronize(grid->st()));
+    });
+}
+
+fs_status fs_add_vortex(fs_grid* grid, double cx, double cy, double radius, double strength)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before editing velocities");
+        set_device(grid->ctx);
+        launch_add_vortex(grid->st(), grid->dev(), grid->ox, grid->oy, cx, cy, radius, strength);
+        HIPC(hipStreamSynchronize(grid->st()));
+    });
+}
+
+fs_status fs_max_divergence(fs_grid* grid, double* out)
+{
+    return guard([&] {
+        need(grid != nullptr && out != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before measuring divergence");
+        set_device(grid->ctx);
+        *out = grid->max_divergence();
+    });
+}
+
+static void advect_sl(fs_grid& g, double dt)
+{
+    const size_t N = g.N();
+    const size_t fields = 1 + g.rho_l.size() + g.rho_g.size();
+    if (g.sl_u.n != g.NU())
+        g.bytes += g.sl_u.alloc(g.NU());
+    if (g.sl_v.n != g.NV())
+        g.bytes += g.sl_v.alloc(g.NV());
+    if (g.sl_cells.n != fields * N)
+        g.bytes += g.sl_cells.alloc(fields * N);
+    launch_semi_lagrangian(g.st(), g.dev(), g.ox, g.oy, dt, g.sl_u.p, g.sl_v.p, g.sl_cells.p);
+    HIPC(hipMemcpyAsync(g.u.p, g.sl_u.p, g.NU() * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+    HIPC(hipMemcpyAsync(g.v.p, g.sl_v.p, g.NV() * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+    HIPC(hipMemcpyAsync(g.fraction.p, g.sl_cells.p, N * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+    if (g.rho_l.size())
+        HIPC(hipMemcpyAsync(g.lmass.p, g.sl_cells.p + N, g.rho_l.size() * N * sizeof(double),
+            hipMemcpyDeviceToDevice, g.st()));
+    if (g.rho_g.size())
+        HIPC(hipMemcpyAsync(g.gmass.p, g.sl_cells.p + (1 + g.rho_l.size()) * N, g.rho_g.size() * N * sizeof(double),
+            hipMemcpyDeviceToDevice, g.st()));
+    g.coeff_valid = false;
+    g.mg_valid = false;
+}
+
+fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before transport");
+        set_device(grid->ctx);
+        advect_sl(*grid, dt);
+        HIPC(hipStreamSynchronize(grid->st()));
+    });
+}
+
+fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
+    fs_step_timings* timings, fs_projection_result* result)
+{
+    return guard([&] {
+        need(grid != nullptr && result != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        fs_grid& g = *grid;
+        if (timings)
+            *timings = fs_step_timings{};
+        fs_projection_result res{};
+        res.max_divergence = std::numeric_limits<double>::infinity();
+        // FluidGridCPU::step returns {0, inf, false, 0, 0} for dt <= 0 (FluidGridCPU.cpp:306-307)
+        if (dt <= 0.0) {
+            res.physical_steps = 0;
+            *result = res;
+            return;
+        }
+        need(g.uploaded, FS_E_LOGIC, "Upload the grid state before stepping");
+        if (g.transport == FS_TRANSPORT_REFERENCE)
+            fail(FS_E_UNSUPPORTED,
+                "Reference PLIC/VOF transport is not implemented by the HIP backend yet; "
+                "select FS_TRANSPORT_NONE or FS_TRANSPORT_SEMI_LAGRANGIAN");
+        set_device(g.ctx);
+        using Clock = std::chrono::steady_clock;
+        auto t0 = Clock::now();
+        auto lap = [&](double* slot) {
+            if (!slot)
+                return;
+            HIPC(hipStreamSynchronize(g.st()));
+            const auto now = Clock::now();
+            *slot += std::chrono::duration<double, std::milli>(now - t0).count();
+            t0 = now;
+        };
+        int transport_steps = 0;
+        if (g.transport == FS_TRANSPORT_SEMI_LAGRANGIAN) {
+            advect_sl(g, dt);
+            transport_steps = 1;
+        }
+        lap(timings ? &timings->transport_ms : nullptr);
+        launch_gravity(g.st(), g.dev(), g.gravity_y, dt);
+        lap(timings ? &timings->forces_ms : nullptr);
+        const fs_projection_result proj = g.project(dt, max_iterations, divergence_tolerance);
+        lap(timings ? &timings->pressure_ms : nullptr);
+        res = proj;
+        res.physical_steps = 1;
+        res.transport_steps = transport_steps;
+        *result = res;
+    });
+}
+
+fs_status fs_snapshot_save(fs_grid* grid)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before taking a snapshot");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        auto copy = [&](DevBuf<double>& dst, const DevBuf<double>& src) {
+            if (dst.n != src.n)
+                g.bytes += dst.alloc(src.n);
+            if (src.n)
+                HIPC(hipMemcpyAsync(dst.p, src.p, src.n * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+        };
+        copy(g.snap_fraction, g.fraction);
+        copy(g.snap_u, g.u);
+        copy(g.snap_v, g.v);
+        copy(g.snap_p, g.p);
+        copy(g.snap_p_prev, g.p_prev);
+        copy(g.snap_lmass, g.lmass);
+        copy(g.snap_gmass, g.gmass);
+        HIPC(hipStreamSynchronize(g.st()));
+        g.has_snapshot = true;
+    });
+}
+
+fs_status fs_snapshot_restore(fs_grid* grid)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->has_snapshot, FS_E_LOGIC, "No snapshot to restore");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        auto copy = [&](DevBuf<double>& dst, const DevBuf<double>& src) {
+            if (src.n)
+                HIPC(hipMemcpyAsync(dst.p, src.p, src.n * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+        };
+        copy(g.fraction, g.snap_fraction);
+        copy(g.u, g.snap_u);
+        copy(g.v, g.snap_v);
+        copy(g.p, g.snap_p);
+        copy(g.p_prev, g.snap_p_prev);
+        copy(g.lmass, g.snap_lmass);
+        copy(g.gmass, g.snap_gmass);
+        g.coeff_valid = false;
+        g.mg_valid = false;
+        // no synchronisation: restores are stream-ordered before the next solve
+    });
+}
+
+fs_status fs_mg_levels(fs_grid* grid, int* levels)
+{
+    return guard([&] {
+        need(grid != nullptr && levels != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *levels = (int)grid->levels.size();
+    });
+}
+
+fs_status fs_mg_level_shape(fs_grid* grid, int level, int* width, int* height)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "level out of range");
+        if (width) *width = grid->levels[level].w;
+        if (height) *height = grid->levels[level].h;
+    });
+}
+
+fs_status fs_mg_level_centers(fs_grid* grid, int level, double* cx, double* cy)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "level out of range");
+        const Level& L = grid->levels[level];
+        set_device(grid->ctx);
+        // read back the device copies, which the coarsening kernels use
+        if (cx)
+            HIPC(hipMemcpy(cx, L.d_cx.p, L.w * sizeof(double), hipMemcpyDeviceToHost));
+        if (cy)
+            HIPC(hipMemcpy(cy, L.d_cy.p, L.h * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+static void ensure_hierarchy(fs_grid& g)
+{
+    need(g.uploaded, FS_E_LOGIC, "Upload the grid state before inspecting the hierarchy");
+    if (!g.coeff_valid)
+        g.rebuild_coefficients();
+    if (!g.mg_valid) {
+        launch_init_scalars(g.st(), g.S.p, 0.0, 0);
+        g.rebuild_hierarchy();
+        const PcgScalars s = g.read_scalars();
+        if (s.mg_failed || s.status == kFailed) {
+            g.mg_valid = false;
+            fail(FS_E_RUNTIME, "multigrid coarse factorization failed");
+        }
+    }
+}
+
+static std::vector<double> read_padded_interior(const double* dptr, int w, int h)
+{
+    const int s = w + 2;
+    std::vector<double> padded((size_t)s * (h + 2));
+    HIPC(hipMemcpy(padded.data(), dptr, padded.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> out((size_t)w * h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            out[(size_t)y * w + x] = padded[(size_t)(y + 1) * s + x + 1];
+    return out;
+}
+
+fs_status fs_mg_level_weights(fs_grid* grid, int level, double* right, double* down, float* inv_diag)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "level out of range");
+        set_device(grid->ctx);
+        ensure_hierarchy(*grid);
+        HIPC(hipStreamSynchronize(grid->st()));
+        const Level& L = grid->levels[level];
+        if (right) {
+            auto v = read_padded_interior(L.right64.p, L.w, L.h);
+            std::memcpy(right, v.data(), v.size() * sizeof(double));
+        }
+        if (down) {
+            auto v = read_padded_interior(L.down64.p, L.w, L.h);
+            std::memcpy(down, v.data(), v.size() * sizeof(double));
+        }
+        if (inv_diag) {
+            const int s = L.w + 2;
+            std::vector<float> padded((size_t)s * (L.h + 2));
+            HIPC(hipMemcpy(padded.data(), L.invd32.p, padded.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int y = 0; y < L.h; ++y)
+                for (int x = 0; x < L.w; ++x)
+                    inv_diag[(size_t)y * L.w + x] = padded[(size_t)(y + 1) * s + x + 1];
+        }
+    });
+}
+
+fs_status fs_mg_level_prolong(fs_grid* grid, int level, int axis, int* first, int* second, double* weight)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level + 1 < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "level out of range");
+        need(axis == 0 || axis == 1, FS_E_INVALID_ARGUMENT, "axis must be 0 or 1");
+        const Level& L = grid->levels[level];
+        const auto& f = axis == 0 ? L.pxf : L.pyf;
+        const auto& s = axis == 0 ? L.pxs : L.pys;
+        const auto& w = axis == 0 ? L.pxw : L.pyw;
+        for (size_t i = 0; i < f.size(); ++i) {
+            if (first) first[i] = f[i];
+            if (second) second[i] = s[i];
+            if (weight) weight[i] = w[i];
+        }
+    });
+}
+
+fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, int* index, double* weight)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level + 1 < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "level out of range");
+        need(axis == 0 || axis == 1, FS_E_INVALID_ARGUMENT, "axis must be 0 or 1");
+        const Level& L = grid->levels[level];
+        const auto& c = axis == 0 ? L.rxc : L.ryc;
+        const auto& ix = axis == 0 ? L.rxi : L.ryi;
+        const auto& w = axis == 0 ? L.rxw : L.ryw;
+        for (size_t i = 0; i < c.size(); ++i)
+            if (count) count[i] = c[i];
+        for (size_t i = 0; i < ix.size(); ++i) {
+            if (index) index[i] = ix[i];
+            if (weight) weight[i] = w[i];
+        }
+    });
+}
+
+fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        set_device(grid->ctx);
+        ensure_hierarchy(*grid);
+        HIPC(hipStreamSynchronize(grid->st()));
+        if (n)
+            *n = grid->coarse_n;
+        if (factor && grid->coarse_n > 0)
+            HIPC(hipMemcpy(factor, grid->factor.p, (size_t)grid->coarse_n * grid->coarse_n * sizeof(double),
+                hipMemcpyDeviceToHost));
+    });
+}
+
+fs_status fs_mg_apply(fs_grid* grid, const double* residual, double residual_mean, double* correction)
+{
+    return guard([&] {
+        need(grid != nullptr && residual != nullptr && correction != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        ensure_hierarchy(g);
+        const int W = g.W, H = g.H, s = W + 2;
+        std::vector<float> padded(g.NP(), 0.0f);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+                padded[(size_t)(y + 1) * s + x + 1] = (float)residual[(size_t)y * W + x];
+        HIPC(hipMemcpy(g.r.p, padded.data(), padded.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPC(hipStreamSynchronize(g.st()));
+        PcgScalars init{};
+        init.r_mean = residual_mean;
+        init.status = kRunning;
+        HIPC(hipMemcpy(g.S.p, &init, sizeof(init), hipMemcpyHostToDevice));
+        g.vcycle(nullptr);
+        HIPC(hipStreamSynchronize(g.st()));
+        HIPC(hipMemcpy(padded.data(), g.levels[0].sol0.p, padded.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+                correction[(size_t)y * W + x] = padded[(size_t)(y + 1) * s + x + 1];
+    });
+}
+
+fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_density_v)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state first");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        if (!g.coeff_valid)
+            g.rebuild_coefficients();
+        HIPC(hipStreamSynchronize(g.st()));
+        const int W = g.W, H = g.H;
+        auto right = read_padded_interior(g.levels[0].right64.p, W, H);
+        auto down = read_padded_interior(g.levels[0].down64.p, W, H);
+        if (inv_density_u) {
+            for (int y = 0; y < H; ++y) {
+                inv_density_u[(size_t)y * (W + 1)] = 0.0;
+                for (int x = 0; x < W; ++x)
+                    inv_density_u[(size_t)y * (W + 1) + x + 1] = right[(size_t)y * W + x];
+            }
+        }
+        if (inv_density_v) {
+            for (int x = 0; x < W; ++x)
+                inv_density_v[x] = 0.0;
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x)
+                    inv_density_v[(size_t)(y + 1) * W + x] = down[(size_t)y * W + x];
+        }
+    });
+}
+
+fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch)
+{
+    return guard([&] {
+        need(grid != nullptr && ms_per_launch != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        need(reps > 0, FS_E_INVALID_ARGUMENT, "reps must be positive");
+        need(kernel >= 0 && kernel <= 3, FS_E_INVALID_ARGUMENT, "unknown kernel id");
+        fs_grid& g = *grid;
+        need(g.uploaded, FS_E_LOGIC, "Upload the grid state first");
+        set_device(g.ctx);
+        if (!g.coeff_valid)
+            g.rebuild_coefficients();
+        launch_init_scalars(g.st(), g.S.p, 0.0, 1 << 30);
+        LevelView L0 = g.view(0);
+        GridDev gd = g.dev();
+        auto launch_once = [&](int i) {
+            switch (kernel) {
+            case 0: launch_rb_sweep(g.st(), L0, i & 1, g.r.p, nullptr, 0, 0, nullptr); break;
+            case 1: launch_pcg_ad(g.st(), gd); break;
+            case 2: launch_pcg_update(g.st(), gd); break;
+            default: launch_copy_f32(g.st(), g.r.p, g.q.p, g.NP()); break;
+            }
+        };
+        launch_once(0);
+        HIPC(hipEventRecord(g.ctx->ev0, g.st()));
+        for (int i = 0; i < reps; ++i)
+            launch_once(i);
+        HIPC(hipEventRecord(g.ctx->ev1, g.st()));
+        HIPC(hipEventSynchronize(g.ctx->ev1));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, g.ctx->ev0, g.ctx->ev1));
+        *ms_per_launch = (double)ms / reps;
+    });
+}
+
+fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes)
+{
+    return guard([&] {
+        need(grid != nullptr && bytes != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *bytes = grid->bytes;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,396 @@
+"""Host-side mirror of the reference backend interface over the HIP C ABI.
+
+`FluidGridHIP` implements the method set of fluid::FluidGridBase (FluidSim/FluidGridBase.h:20-147)
+with the reference's names, argument meaning and error behaviour, so code written against
+FluidGridCPU (FluidSim/UniformCPU/FluidGridCPU.h:15-225) reads the same here.  Every call goes
+through libfluidsim_hip.so; nothing is computed on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import FsProjectionResult, FsStepTimings, call, lib
+
+
+# ---- value types mirroring FluidGridTypes.h / FluidMixture.h ----
+
+@dataclass
+class FluidGridState:
+    """FluidGridState<double> (FluidGridTypes.h:10-20); empty arrays mean zero."""
+    liquid_fraction: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    u: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    v: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    pressure: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    liquid_mass: List[np.ndarray] = field(default_factory=list)
+    gas_mass: List[np.ndarray] = field(default_factory=list)
+
+
+@dataclass
+class ProjectionResult:
+    """FluidProjectionResult (FluidGridTypes.h:59-79) plus solver diagnostics."""
+    iterations: int = 0
+    max_divergence: float = math.inf
+    converged: bool = False
+    physical_steps: int = 1
+    transport_steps: int = 0
+    relaxed: bool = False
+    max_volume_error: float = 0.0
+    rms_volume_error: float = 0.0
+    viscosity_iterations: int = 0
+    precision_limited: bool = False
+    restarts: int = 0
+    initial_residual: float = 0.0
+    final_residual: float = 0.0
+    rhs_max: float = 0.0
+
+    def accepted(self) -> bool:
+        return bool(self.converged or self.relaxed or self.precision_limited)
+
+    @classmethod
+    def from_c(cls, r: FsProjectionResult) -> "ProjectionResult":
+        return cls(r.iterations, r.max_divergence, bool(r.converged), r.physical_steps, r.transport_steps,
+                   bool(r.relaxed), r.max_volume_error, r.rms_volume_error, r.viscosity_iterations,
+                   bool(r.precision_limited), r.restarts, r.initial_residual, r.final_residual, r.rhs_max)
+
+
+@dataclass
+class StepTimings:
+    """FluidStepTimings (FluidGridTypes.h:49-57), milliseconds."""
+    transport_ms: float = 0.0
+    viscosity_ms: float = 0.0
+    forces_ms: float = 0.0
+    pressure_ms: float = 0.0
+    selection_ms: float = 0.0
+    regrid_ms: float = 0.0
+
+
+@dataclass
+class MixtureComponent:
+    """MixtureComponent (FluidMixture.h:14-23)."""
+    substance: str
+    density: float
+    dynamic_viscosity: float = 0.0
+
+
+@dataclass
+class FluidMixture:
+    """FluidMixture (FluidMixture.h:25-39)."""
+    components: List[MixtureComponent] = field(default_factory=list)
+
+    def hasViscosity(self) -> bool:
+        return any(c.dynamic_viscosity != 0.0 for c in self.components)
+
+
+# ---- devices ----
+
+class Context:
+    """One HIP device and one stream (fs_context)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        call("fs_context_create", int(device), C.byref(h))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        call("fs_context_synchronize", self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_contexts = {}
+
+
+def default_context(device: int = 0) -> Context:
+    ctx = _default_contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _default_contexts[device] = ctx
+    return ctx
+
+
+def _dp(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class FluidGridHIP:
+    """Uniform MAC grid backend on one MI355X (the FluidGridBase method set)."""
+
+    Diagonal = _abi.FS_PRECOND_DIAGONAL
+    Multigrid = _abi.FS_PRECOND_MULTIGRID
+
+    def __init__(self, width: int, height: int, cell_size: float, context: Optional[Context] = None):
+        self.context = context or default_context(0)
+        h = C.c_void_p()
+        call("fs_grid_create", self.context.handle, int(width), int(height), float(cell_size), C.byref(h))
+        self._h = h
+        self.width, self.height, self.cell_size = int(width), int(height), float(cell_size)
+        self.liquid_mixture = FluidMixture()
+        self.gas_mixture = FluidMixture()
+        self.substances: list = []
+        self._surface_tension = 0.0
+
+    # -- lifetime --
+    def destroy(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_grid_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- geometry (FluidGridBase.h:31-39) --
+    def getWidth(self) -> int:
+        return self.width
+
+    def getHeight(self) -> int:
+        return self.height
+
+    def getCellSize(self) -> float:
+        return self.cell_size
+
+    def getOrigin(self):
+        x, y = C.c_double(), C.c_double()
+        call("fs_get_origin", self._h, C.byref(x), C.byref(y))
+        return (x.value, y.value)
+
+    def setOrigin(self, origin):
+        call("fs_set_origin", self._h, float(origin[0]), float(origin[1]))
+
+    def requiresRenderThread(self) -> bool:
+        return False
+
+    # -- materials (FluidGridBase.h:42-54) --
+    def configureMaterials(self, substances, liquid: FluidMixture, gas: FluidMixture):
+        self.substances = list(substances)
+        self.liquid_mixture, self.gas_mixture = liquid, gas
+        ld = np.array([c.density for c in liquid.components], dtype=np.float64)
+        lv = np.array([c.dynamic_viscosity for c in liquid.components], dtype=np.float64)
+        gd = np.array([c.density for c in gas.components], dtype=np.float64)
+        gv = np.array([c.dynamic_viscosity for c in gas.components], dtype=np.float64)
+        call("fs_configure_materials", self._h, ld.size, _dp(ld), _dp(lv), gd.size, _dp(gd), _dp(gv))
+
+    def getSubstances(self):
+        return self.substances
+
+    def getLiquidMixture(self) -> FluidMixture:
+        return self.liquid_mixture
+
+    def getGasMixture(self) -> FluidMixture:
+        return self.gas_mixture
+
+    def hasViscosity(self) -> bool:
+        return self.liquid_mixture.hasViscosity() or self.gas_mixture.hasViscosity()
+
+    def setComponentViscosity(self, phase: str, index: int, coefficient: float):
+        if coefficient != 0.0:
+            raise NotImplementedError("Viscosity is not implemented by the HIP backend yet")
+
+    # -- forces (FluidGridBase.h:70-84) --
+    def getGravityY(self) -> float:
+        g = C.c_double()
+        call("fs_get_gravity", self._h, C.byref(g))
+        return g.value
+
+    def setGravityY(self, acceleration: float):
+        call("fs_set_gravity", self._h, float(acceleration))
+
+    def getSurfaceTension(self) -> float:
+        return self._surface_tension
+
+    def setSurfaceTension(self, coefficient: float):
+        call("fs_set_surface_tension", self._h, float(coefficient))
+        self._surface_tension = float(coefficient)
+
+    def capillaryStepLimit(self) -> float:
+        v = C.c_double()
+        call("fs_capillary_step_limit", self._h, C.byref(v))
+        return v.value
+
+    # -- solver options (FluidGridCPU.h:39-44) --
+    def setPressurePreconditioner(self, kind: int):
+        call("fs_set_preconditioner", self._h, int(kind))
+
+    def setTransport(self, mode: int):
+        call("fs_set_transport", self._h, int(mode))
+
+    # -- state (FluidGridBase.h:118-119) --
+    def _sizes(self):
+        W, H = self.width, self.height
+        return W * H, (W + 1) * H, W * (H + 1)
+
+    def upload(self, state: FluidGridState):
+        N, NU, NV = self._sizes()
+        nl, ng = len(self.liquid_mixture.components), len(self.gas_mixture.components)
+
+        def field_(a, n):
+            if a is None or len(a) == 0:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            if a.size != n:
+                raise ValueError("FluidGridHIP upload dimensions do not match the grid")
+            return a
+
+        def comps(lst, k):
+            if lst is None or len(lst) == 0:
+                return None
+            if len(lst) != k:
+                raise ValueError("FluidGridHIP upload component count does not match the mixture")
+            out = np.zeros(k * N)
+            for i, a in enumerate(lst):
+                f = field_(a, N)
+                if f is not None:
+                    out[i * N:(i + 1) * N] = f
+            return out
+
+        arrays = [field_(state.liquid_fraction, N), field_(state.u, NU), field_(state.v, NV),
+                  field_(state.pressure, N), comps(state.liquid_mass, nl), comps(state.gas_mass, ng)]
+        self._keep = arrays
+        call("fs_upload", self._h, *[_dp(a) for a in arrays])
+
+    def download(self) -> FluidGridState:
+        N, NU, NV = self._sizes()
+        nl, ng = len(self.liquid_mixture.components), len(self.gas_mixture.components)
+        f, u, v, p = np.zeros(N), np.zeros(NU), np.zeros(NV), np.zeros(N)
+        lm, gm = np.zeros(nl * N), np.zeros(ng * N)
+        call("fs_download", self._h, _dp(f), _dp(u), _dp(v), _dp(p), _dp(lm) if nl else None,
+             _dp(gm) if ng else None)
+        return FluidGridState(f, u, v, p, [lm[i * N:(i + 1) * N] for i in range(nl)],
+                              [gm[i * N:(i + 1) * N] for i in range(ng)])
+
+    def invalidateCaches(self):
+        call("fs_invalidate_caches", self._h)
+
+    # -- simulation (FluidGridBase.h:133-135, FluidGridCPU.h:117-126) --
+    def step(self, dt: float, max_iterations: int = 200, divergence_tolerance: float = 1e-6,
+             timings: Optional[StepTimings] = None) -> ProjectionResult:
+        res = FsProjectionResult()
+        t = FsStepTimings()
+        call("fs_step", self._h, float(dt), int(max_iterations), float(divergence_tolerance),
+             C.byref(t) if timings is not None else None, C.byref(res))
+        if timings is not None:
+            for name, _ in FsStepTimings._fields_:
+                setattr(timings, name, getattr(t, name))
+        return ProjectionResult.from_c(res)
+
+    def project(self, dt: float, max_iterations: int = 200, divergence_tolerance: float = 1e-6) -> ProjectionResult:
+        res = FsProjectionResult()
+        call("fs_project", self._h, float(dt), int(max_iterations), float(divergence_tolerance), C.byref(res))
+        return ProjectionResult.from_c(res)
+
+    def applyGravity(self, dt: float):
+        call("fs_apply_gravity", self._h, float(dt))
+
+    def addVortex(self, center, radius: float, strength: float):
+        call("fs_add_vortex", self._h, float(center[0]), float(center[1]), float(radius), float(strength))
+
+    def advectSemiLagrangian(self, dt: float):
+        call("fs_advect_semi_lagrangian", self._h, float(dt))
+
+    def maxDivergence(self) -> float:
+        v = C.c_double()
+        call("fs_max_divergence", self._h, C.byref(v))
+        return v.value
+
+    # -- device-resident snapshots --
+    def snapshotSave(self):
+        call("fs_snapshot_save", self._h)
+
+    def snapshotRestore(self):
+        call("fs_snapshot_restore", self._h)
+
+    # -- hierarchy introspection --
+    def multigridLevels(self) -> int:
+        n = C.c_int()
+        call("fs_mg_levels", self._h, C.byref(n))
+        return n.value
+
+    def multigridShape(self, level: int):
+        w, h = C.c_int(), C.c_int()
+        call("fs_mg_level_shape", self._h, int(level), C.byref(w), C.byref(h))
+        return w.value, h.value
+
+    def multigridCenters(self, level: int):
+        w, h = self.multigridShape(level)
+        cx, cy = np.zeros(w), np.zeros(h)
+        call("fs_mg_level_centers", self._h, int(level), _dp(cx), _dp(cy))
+        return cx, cy
+
+    def multigridWeights(self, level: int):
+        w, h = self.multigridShape(level)
+        r, d, i = np.zeros(w * h), np.zeros(w * h), np.zeros(w * h, dtype=np.float32)
+        call("fs_mg_level_weights", self._h, int(level), _dp(r), _dp(d), i.ctypes.data_as(C.POINTER(C.c_float)))
+        return r.reshape(h, w), d.reshape(h, w), i.reshape(h, w)
+
+    def multigridProlong(self, level: int, axis: int):
+        w, h = self.multigridShape(level)
+        n = w if axis == 0 else h
+        f, s, wt = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        call("fs_mg_level_prolong", self._h, int(level), int(axis), f.ctypes.data_as(C.POINTER(C.c_int)),
+             s.ctypes.data_as(C.POINTER(C.c_int)), _dp(wt))
+        return f, s, wt
+
+    def multigridRestrict(self, level: int, axis: int):
+        cw, ch = self.multigridShape(level + 1)
+        n = cw if axis == 0 else ch
+        cnt, idx, wt = np.zeros(n, np.int32), np.zeros(4 * n, np.int32), np.zeros(4 * n)
+        call("fs_mg_level_restrict", self._h, int(level), int(axis), cnt.ctypes.data_as(C.POINTER(C.c_int)),
+             idx.ctypes.data_as(C.POINTER(C.c_int)), _dp(wt))
+        return cnt, idx.reshape(n, 4), wt.reshape(n, 4)
+
+    def multigridCoarseFactor(self):
+        n = C.c_int()
+        call("fs_mg_coarse_factor", self._h, None, C.byref(n))
+        f = np.zeros(max(n.value, 0) ** 2)
+        call("fs_mg_coarse_factor", self._h, _dp(f), C.byref(n))
+        return f.reshape(n.value, n.value)
+
+    def multigridApply(self, residual, residual_mean: float = 0.0):
+        r = np.ascontiguousarray(residual, dtype=np.float64).ravel()
+        out = np.zeros(self.width * self.height)
+        call("fs_mg_apply", self._h, _dp(r), float(residual_mean), _dp(out))
+        return out
+
+    def faceWeights(self):
+        N, NU, NV = self._sizes()
+        iu, iv = np.zeros(NU), np.zeros(NV)
+        call("fs_face_weights", self._h, _dp(iu), _dp(iv))
+        return iu, iv
+
+    def timeKernel(self, kernel: int, reps: int = 100) -> float:
+        ms = C.c_double()
+        call("fs_time_kernel", self._h, int(kernel), int(reps), C.byref(ms))
+        return ms.value
+
+    def deviceBytes(self) -> int:
+        b = C.c_size_t()
+        call("fs_grid_device_bytes", self._h, C.byref(b))
+        return b.value

@@ -1,0 +1,181 @@
+"""Seeded synthetic inputs for the benchmark and parity workloads.
+
+Random numbers come from SplitMix64 (Steele, Lea & Flood 2014) mapped to doubles through the top
+53 bits, with Box-Muller for normals, so every input is reproducible bit for bit on any machine.
+Layouts follow the reference (cells W*H, u (W+1)*H, v W*(H+1), y downward).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def next_u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            steps = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + steps * _GAMMA
+            self.state = self.state + np.uint64(n) * _GAMMA
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            return z ^ (z >> np.uint64(31))
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        u = (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+    def normal(self, n: int) -> np.ndarray:
+        u1 = self.uniform(n)
+        u2 = self.uniform(n)
+        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+
+
+@dataclass
+class Workload:
+    """A complete upload: grid shape, materials and state (host FP64 arrays)."""
+    name: str
+    width: int
+    height: int
+    cell_size: float
+    liquid_density: List[float]
+    gas_density: List[float]
+    gravity_y: float
+    dt: float
+    fraction: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    liquid_mass: np.ndarray  # component-major, k * N
+    gas_mass: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+
+def two_phase_masses(fraction: np.ndarray, cell_size: float, rho_l: float = 100.0, rho_g: float = 0.1):
+    """Water 100 / air 0.1 masses = density * phase fraction * h^2 (tests/FluidTestSetup.h:20-29)."""
+    volume = cell_size * cell_size
+    return rho_l * fraction * volume, rho_g * (1.0 - fraction) * volume
+
+
+def zero_walls(u: np.ndarray, v: np.ndarray, W: int, H: int):
+    u2 = u.reshape(H, W + 1)
+    u2[:, 0] = 0.0
+    u2[:, W] = 0.0
+    v2 = v.reshape(H + 1, W)
+    v2[0, :] = 0.0
+    v2[H, :] = 0.0
+
+
+def add_gaussian(u, v, W, H, h, cx, cy, radius, strength, kind="vortex", window=10.0):
+    """FluidGridCPU::addVortex (FluidGridCPU.cpp:998-1028) at world positions (origin 0).
+
+    kind="vortex" adds (-dy, dx) * strength * falloff; kind="source" adds (dx, dy) * ... (a divergent
+    Gaussian).  Faces farther than `window` radii from the centre are skipped: their falloff is below
+    exp(-50) ~ 2e-22, which cannot change an FP64 sum of O(1e-2) terms.
+    """
+    reach = window * radius
+    u2 = u.reshape(H, W + 1)
+    v2 = v.reshape(H + 1, W)
+    x0 = max(0, int(math.floor((cx - reach) / h)) - 1)
+    x1 = min(W, int(math.ceil((cx + reach) / h)) + 1)
+    y0 = max(0, int(math.floor((cy - reach) / h)) - 1)
+    y1 = min(H, int(math.ceil((cy + reach) / h)) + 1)
+    if x1 <= x0 or y1 <= y0:
+        return
+    # u faces: (x h, (y + 0.5) h), x in [x0, x1], y in [y0, y1)
+    xs = np.arange(x0, min(x1, W) + 1)
+    ys = np.arange(y0, min(y1, H))
+    px = xs[None, :] * h
+    py = (ys[:, None] + 0.5) * h
+    dx, dy = px - cx, py - cy
+    falloff = np.exp(-(dx * dx + dy * dy) / (2.0 * radius * radius))
+    if kind == "vortex":
+        u2[np.ix_(ys, xs)] += -dy * strength * falloff
+    else:
+        u2[np.ix_(ys, xs)] += dx * strength * falloff
+    # v faces: ((x + 0.5) h, y h), x in [x0, x1), y in [y0, y1]
+    xs = np.arange(x0, min(x1, W))
+    ys = np.arange(y0, min(y1, H) + 1)
+    px = (xs[None, :] + 0.5) * h
+    py = ys[:, None] * h
+    dx, dy = px - cx, py - cy
+    falloff = np.exp(-(dx * dx + dy * dy) / (2.0 * radius * radius))
+    if kind == "vortex":
+        v2[np.ix_(ys, xs)] += dx * strength * falloff
+    else:
+        v2[np.ix_(ys, xs)] += dy * strength * falloff
+
+
+def dam_break(W: int = 128, H: int = 128, seed: int = 1) -> Workload:
+    """Dam break: liquid where x_idx < 0.4 W and y_idx >= 0.4 H (the bottom 60%, y downward), water
+    100 / air 0.1, interior face velocities U(-1e-3, 1e-3), gravity +9.81 (downward), dt = 1/120."""
+    h = 1.0 / W
+    xs = np.arange(W)[None, :]
+    ys = np.arange(H)[:, None]
+    fraction = ((xs < 0.4 * W) & (ys >= 0.4 * H)).astype(np.float64).ravel()
+    rng = SplitMix64(seed)
+    u = rng.uniform((W + 1) * H, -1e-3, 1e-3)
+    v = rng.uniform(W * (H + 1), -1e-3, 1e-3)
+    zero_walls(u, v, W, H)
+    lm, gm = two_phase_masses(fraction, h)
+    return Workload("dam_break", W, H, h, [100.0], [0.1], 9.81, 1.0 / 120.0, fraction, u, v, lm, gm,
+                    {"seed": seed})
+
+
+def vortex_box(W: int = 1024, H: int = 1024, seed: int = 2, vortices: int = 64, sources: int = 16) -> Workload:
+    """Fully liquid box with `vortices` Gaussian vortices (centre U[0,1]^2, radius U[0.01,0.05],
+    strength N(0,1)) plus `sources` divergent Gaussians of the same law; walls closed."""
+    h = 1.0 / W
+    rng = SplitMix64(seed)
+    n = vortices + sources
+    cx = rng.uniform(n)
+    cy = rng.uniform(n) * (H * h)
+    cx = cx * (W * h)
+    radius = rng.uniform(n, 0.01, 0.05)
+    strength = rng.normal(n)
+    u = np.zeros((W + 1) * H)
+    v = np.zeros(W * (H + 1))
+    for k in range(n):
+        add_gaussian(u, v, W, H, h, cx[k], cy[k], radius[k], strength[k],
+                     "vortex" if k < vortices else "source")
+    zero_walls(u, v, W, H)
+    fraction = np.ones(W * H)
+    lm, gm = two_phase_masses(fraction, h)
+    return Workload("vortex_box", W, H, h, [100.0], [0.1], 9.81, 1.0 / 120.0, fraction, u, v, lm, gm,
+                    {"seed": seed, "vortices": vortices, "sources": sources})
+
+
+def sloped_pool(W: int, H: int, cell_size: float = 4.0, seed: int = 0) -> Workload:
+    """The reference tests' sloped pool fraction clamp(y + 1 - (0.35 x + 0.2 H), 0, 1)
+    (tests/FluidGridTests.cpp:629-641) with u = 3 sin(0.07 i), v = 4 cos(0.11 i)."""
+    xs = np.arange(W)[None, :].astype(np.float64)
+    ys = np.arange(H)[:, None].astype(np.float64)
+    fraction = np.clip(ys + 1.0 - (0.35 * xs + 0.2 * H), 0.0, 1.0).ravel()
+    u = 3.0 * np.sin(np.arange((W + 1) * H) * 0.07)
+    v = 4.0 * np.cos(np.arange(W * (H + 1)) * 0.11)
+    lm, gm = two_phase_masses(fraction, cell_size)
+    return Workload("sloped_pool", W, H, cell_size, [100.0], [0.1], 100.0, 1.0 / 120.0, fraction, u, v, lm, gm,
+                    {"seed": seed})
+
+
+def load_into(grid, wl: Workload):
+    """Configure materials and upload a workload into any object with the FluidGridBase method set."""
+    from .grid import FluidGridState, FluidMixture, MixtureComponent
+
+    liquid = FluidMixture([MixtureComponent(f"liquid{k}", d) for k, d in enumerate(wl.liquid_density)])
+    gas = FluidMixture([MixtureComponent(f"gas{k}", d) for k, d in enumerate(wl.gas_density)])
+    grid.configureMaterials([], liquid, gas)
+    grid.setGravityY(wl.gravity_y)
+    N = wl.width * wl.height
+    nl, ng = len(wl.liquid_density), len(wl.gas_density)
+    grid.upload(FluidGridState(wl.fraction, wl.u, wl.v, np.zeros(0),
+                               [wl.liquid_mass[k * N:(k + 1) * N] for k in range(nl)],
+                               [wl.gas_mass[k * N:(k + 1) * N] for k in range(ng)]))

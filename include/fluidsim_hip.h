@@ -1,0 +1,187 @@
+/*
+ * fluidsim_hip.h -- C ABI of libfluidsim_hip.so, the MI355X (gfx950) uniform-MAC backend.
+ *
+ * The ABI is the drop-in boundary for the reference's uniform grid backends: a C++ adapter class
+ * implementing fluid::FluidGridBase (FluidSim/FluidGridBase.h:20-147) forwards each virtual method
+ * to one call below, the way FluidGridWebGPU forwards to FluidGridGPU
+ * (FluidSim/UniformWebGPU/FluidGridWebGPU.cpp:134-184).  Only plain C types cross it: opaque
+ * handles, ints, doubles, pointers to caller-owned host arrays and POD structs.
+ *
+ * Conventions (mirroring the reference's error contract, FluidGridBase.h:48-116,
+ * FluidGridCPU.cpp:198-251, tests/FluidGridAPITests.cpp:186-229):
+ *   - every call returns an fs_status; FS_OK is 0.  FS_E_INVALID_ARGUMENT maps to
+ *     std::invalid_argument, FS_E_LOGIC to std::logic_error, FS_E_UNSUPPORTED to the base-class
+ *     "requires uniform WebGPU" throws, FS_E_DEVICE / FS_E_RUNTIME to std::runtime_error;
+ *   - numerical non-convergence is NOT an error: it is reported in fs_projection_result;
+ *   - fs_last_error() returns the message of the last failing call on this thread;
+ *   - a grid is owned by one thread (FluidGridBase.h:37-39); distinct grids may be used from
+ *     distinct threads.
+ *
+ * Layouts are the reference's (FluidGridCPU.h:128-130, FluidGridTypes.h:10-20): cells W*H
+ * row-major with y growing downward, u (W+1)*H, v W*(H+1), component masses component-major
+ * (k*W*H + cell).  All host arrays are FP64 like FluidGridState<double>.
+ */
+#ifndef FLUIDSIM_HIP_H
+#define FLUIDSIM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fs_status {
+    FS_OK = 0,
+    FS_E_INVALID_ARGUMENT = 1,
+    FS_E_LOGIC = 2,
+    FS_E_UNSUPPORTED = 3,
+    FS_E_DEVICE = 4,
+    FS_E_RUNTIME = 5
+} fs_status;
+
+typedef struct fs_context fs_context; /* one HIP device + one stream */
+typedef struct fs_grid fs_grid;       /* one simulation instance (FluidGridBase object) */
+
+/* FluidProjectionResult (FluidGridTypes.h:59-79) plus solver diagnostics. */
+typedef struct fs_projection_result {
+    int iterations;
+    double max_divergence;
+    int converged;
+    int physical_steps;
+    int transport_steps;
+    int relaxed;
+    double max_volume_error;
+    double rms_volume_error;
+    int viscosity_iterations;
+    int precision_limited;
+    /* diagnostics (no reference counterpart) */
+    int restarts;               /* true-residual restarts inside PCG */
+    double initial_residual;    /* max |b - A p| of the predicted pressure */
+    double final_residual;      /* max |b - A p| at the last true-residual check */
+    double rhs_max;             /* max |b| (b = -div u* h^2/dt, mean removed) */
+} fs_projection_result;
+
+/* FluidStepTimings (FluidGridTypes.h:49-57), milliseconds. */
+typedef struct fs_step_timings {
+    double transport_ms;
+    double viscosity_ms;
+    double forces_ms;
+    double pressure_ms;
+    double selection_ms;
+    double regrid_ms;
+} fs_step_timings;
+
+/* Transport applied by fs_step.  The reference step transports by PLIC/VOF
+ * (FluidGridCPU.cpp:362-463); FS_TRANSPORT_REFERENCE is reserved for it and currently returns
+ * FS_E_UNSUPPORTED.  FS_TRANSPORT_NONE runs forces + projection (identical to the reference
+ * step on a state at rest).  FS_TRANSPORT_SEMI_LAGRANGIAN is the build's own mode. */
+typedef enum fs_transport {
+    FS_TRANSPORT_REFERENCE = 0,
+    FS_TRANSPORT_NONE = 1,
+    FS_TRANSPORT_SEMI_LAGRANGIAN = 2
+} fs_transport;
+
+/* FluidGridCPU::PressurePreconditioner (FluidGridCPU.h:39-44). */
+typedef enum fs_preconditioner { FS_PRECOND_DIAGONAL = 0, FS_PRECOND_MULTIGRID = 1 } fs_preconditioner;
+
+/* ---- library / context ---- */
+const char* fs_version(void);
+/* Message of the last failed call on this thread ("" if none).  No reference counterpart: the C
+ * ABI cannot carry C++ exceptions, so it keeps their what() text here. */
+const char* fs_last_error(void);
+/* Opens HIP device `device` and one stream. */
+fs_status fs_context_create(int device, fs_context** out);
+fs_status fs_context_destroy(fs_context* ctx);
+fs_status fs_context_synchronize(fs_context* ctx);
+
+/* ---- grid lifetime: FluidGridCPU(int, int, double) (FluidGridCPU.h:63), destroy() (FluidGridBase.h:129) ---- */
+fs_status fs_grid_create(fs_context* ctx, int width, int height, double cell_size, fs_grid** out);
+fs_status fs_grid_destroy(fs_grid* grid);
+
+/* getWidth/getHeight/getCellSize/getOrigin/setOrigin (FluidGridBase.h:31-35) */
+fs_status fs_grid_shape(const fs_grid* grid, int* width, int* height, double* cell_size);
+fs_status fs_set_origin(fs_grid* grid, double x, double y);
+fs_status fs_get_origin(const fs_grid* grid, double* x, double* y);
+
+/* configureMaterials(SubstanceCatalogue, FluidMixture liquid, FluidMixture gas) (FluidGridBase.h:42):
+ * component densities and dynamic viscosities in mixture order.  Viscosity must be zero (the
+ * reference's uniform CPU engine rejects it, FluidGridCPU.cpp:200-201): non-zero returns
+ * FS_E_UNSUPPORTED at upload, like the reference. */
+fs_status fs_configure_materials(fs_grid* grid, int n_liquid, const double* liquid_density,
+    const double* liquid_viscosity, int n_gas, const double* gas_density, const double* gas_viscosity);
+
+/* getGravityY/setGravityY (FluidGridBase.h:70-71) */
+fs_status fs_set_gravity(fs_grid* grid, double gravity_y);
+fs_status fs_get_gravity(const fs_grid* grid, double* gravity_y);
+/* setSurfaceTension (FluidGridBase.h:76-80): only 0 is accepted for now (FS_E_UNSUPPORTED otherwise). */
+fs_status fs_set_surface_tension(fs_grid* grid, double coefficient);
+/* capillaryStepLimit (FluidGridBase.h:84): +inf without surface tension. */
+fs_status fs_capillary_step_limit(const fs_grid* grid, double* limit);
+/* FluidGridCPU::pressure_preconditioner (FluidGridCPU.h:39-44) */
+fs_status fs_set_preconditioner(fs_grid* grid, fs_preconditioner kind);
+fs_status fs_set_transport(fs_grid* grid, fs_transport mode);
+
+/* upload(const State&) (FluidGridBase.h:118, FluidGridCPU.cpp:198-251).  NULL means an empty
+ * vector (zeros).  Sizes are implied by the grid and the configured mixtures.  Resets the pressure
+ * history. */
+fs_status fs_upload(fs_grid* grid, const double* liquid_fraction, const double* u, const double* v,
+    const double* pressure, const double* liquid_mass, const double* gas_mass);
+/* download() (FluidGridBase.h:119).  NULL skips a field. */
+fs_status fs_download(fs_grid* grid, double* liquid_fraction, double* u, double* v, double* pressure,
+    double* liquid_mass, double* gas_mass);
+/* invalidateCaches() (FluidGridCPU.h:111) */
+fs_status fs_invalidate_caches(fs_grid* grid);
+
+/* step(dt, max_iterations, divergence_tolerance, timings) (FluidGridBase.h:133-134,
+ * FluidGridCPU.cpp:300-360).  `timings` may be NULL. */
+fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
+    fs_step_timings* timings, fs_projection_result* result);
+/* FluidGridCPU::project(dt, max_iterations, divergence_tolerance) (FluidGridCPU.cpp:1030-1124) */
+fs_status fs_project(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
+    fs_projection_result* result);
+/* FluidGridCPU::applyGravity(dt) (FluidGridCPU.cpp:988-996) */
+fs_status fs_apply_gravity(fs_grid* grid, double dt);
+/* FluidGridCPU::addVortex(center, radius, strength) (FluidGridCPU.cpp:998-1028) */
+fs_status fs_add_vortex(fs_grid* grid, double center_x, double center_y, double radius, double strength);
+/* maxDivergence() (FluidGridBase.h:135, FluidGridCPU.cpp:1126-1152) */
+fs_status fs_max_divergence(fs_grid* grid, double* out);
+/* Semi-Lagrangian transport alone (build-defined mode, no reference counterpart). */
+fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt);
+
+/* ---- device-resident snapshots (no reference counterpart): keep the inputs in HBM and restore
+ * them between timed solves or serving requests without a host round trip. ---- */
+fs_status fs_snapshot_save(fs_grid* grid);
+fs_status fs_snapshot_restore(fs_grid* grid);
+
+/* ---- multigrid hierarchy introspection (PressureMultigrid.cpp:24-140), for parity checks ---- */
+fs_status fs_mg_levels(fs_grid* grid, int* levels);
+fs_status fs_mg_level_shape(fs_grid* grid, int level, int* width, int* height);
+/* FP64 cell centres of a level (lengths width, height). */
+fs_status fs_mg_level_centers(fs_grid* grid, int level, double* cx, double* cy);
+/* FP64 conductances (right, down) and FP32 1/diagonal as stored on the device, interior w*h each.
+ * Rebuilds coefficients and hierarchy from the current state first. */
+fs_status fs_mg_level_weights(fs_grid* grid, int level, double* right, double* down, float* inv_diag);
+/* Prolongation maps of a fine level (axis 0: x, 1: y), lengths = that level's width/height. */
+fs_status fs_mg_level_prolong(fs_grid* grid, int level, int axis, int* first, int* second, double* weight);
+/* Restriction stencils of the next coarser level: count[n], index[4n], weight[4n]. */
+fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, int* index, double* weight);
+/* Coarsest dense Cholesky factor, n*n row-major (lower triangle); *n receives the unknown count. */
+fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n);
+/* One V-cycle: correction = M^-1 (residual - mean), both W*H FP64 host arrays (converted to FP32 on
+ * the device). */
+fs_status fs_mg_apply(fs_grid* grid, const double* residual, double residual_mean, double* correction);
+/* FP64 face weights inv_density_u ((W+1)*H) and inv_density_v (W*(H+1)) (FluidGridCPU.cpp:1154-1205). */
+fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_density_v);
+
+/* ---- measurement (no reference counterpart) ---- */
+/* Times `reps` back-to-back launches of one hot kernel on the grid's stream with HIP events and
+ * returns the mean milliseconds per launch.  kernel: 0 = fine-level red-black sweep,
+ * 1 = operator + dot (A d), 2 = PCG update, 3 = FP32 copy of one padded vector (reference kernel). */
+fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch);
+/* Cumulative device bytes allocated by this grid. */
+fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1147 @@
+/*
+ * fs_oracle.c -- FP64 CPU restatement of the reference UniformCPU pressure projection.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker; never the product path).
+ *
+ * Every function runs the reference's single-thread order: the reference splits passes over
+ * worker batches only when a pass has >= 65536 items and more than one thread is allowed
+ * (FluidGridCPU.cpp:25-57, PressureMultigrid.cpp:13-21); with one thread each pass is one
+ * batch, so this file is the bitwise-reproducible mode.  Sums that the reference forms with
+ * std::reduce follow libstdc++'s order (groups of four, ((a0+a1)+(a2+a3)) added to the
+ * running total); 4-lane partial sums are reproduced where the reference writes them out.
+ *
+ * Build with -ffp-contract=off so no multiply-add is fused.
+ */
+#include "fs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------------------------ */
+/* helpers                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+static double dmax(double a, double b) { return a < b ? b : a; }   /* std::max(a, b) */
+static int imax(int a, int b) { return a < b ? b : a; }
+static int imin(int a, int b) { return b < a ? b : a; }
+static double dclamp(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
+
+/* std::reduce(first, last, 0.0) as libstdc++ evaluates it. */
+static double reduce_sum(const double* a, size_t n)
+{
+    double init = 0.0;
+    size_t i = 0;
+    for (; n - i >= 4; i += 4) {
+        const double v1 = a[i] + a[i + 1];
+        const double v2 = a[i + 2] + a[i + 3];
+        init = init + (v1 + v2);
+    }
+    for (; i < n; ++i)
+        init = init + a[i];
+    return init;
+}
+
+/* FluidGridCPU.cpp:1488-1505 maxAbs: non-finite maps to +inf. */
+static double max_abs(const double* a, size_t n)
+{
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double v = isfinite(a[i]) ? fabs(a[i]) : INFINITY;
+        m = dmax(m, v);
+    }
+    return m;
+}
+
+/* std::lerp as libstdc++ evaluates it (cmath __lerp). */
+static double lerp_std(double a, double b, double t)
+{
+    if ((a <= 0 && b >= 0) || (a >= 0 && b <= 0))
+        return t * b + (1 - t) * a;
+    if (t == 1)
+        return b;
+    const double x = a + t * (b - a);
+    return (t > 1) == (b > a) ? (b < x ? x : b) : (b > x ? x : b);
+}
+
+/* FluidGridCPU.cpp:67-90 sampleScalar: bilinear blend at fractional array indices, clamped. */
+static double sample_scalar(const double* data, int width, int height, double x, double y)
+{
+    x = dclamp(x, 0.0, (double)(width - 1));
+    y = dclamp(y, 0.0, (double)(height - 1));
+    const int x0 = (int)x;
+    const int y0 = (int)y;
+    const int x1 = imin(x0 + 1, width - 1);
+    const int y1 = imin(y0 + 1, height - 1);
+    const double tx = x - x0;
+    const double ty = y - y0;
+    const double a = lerp_std(data[y0 * width + x0], data[y0 * width + x1], tx);
+    const double b = lerp_std(data[y1 * width + x0], data[y1 * width + x1], tx);
+    return lerp_std(a, b, ty);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* PressureMultigrid (PressureMultigrid.cpp)                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct { int first, second; double weight; } orc_interp;
+typedef struct { int count; int idx[4]; double wt[4]; } orc_restr;
+
+typedef struct {
+    int w, h, stride;
+    double *cx, *cy;
+    double *right, *down, *inv_diag, *rhs, *sol, *res;
+    orc_interp *px, *py;   /* fine-level prolongation maps (this level -> from next coarser) */
+    orc_restr *rx, *ry;    /* restriction stencils indexed by coarse index */
+    int nrx, nry;
+} orc_level;
+
+struct orc_mg {
+    int nlev;
+    orc_level* lev;
+    int n;          /* coarse unknowns = w*h - 1 of the last level */
+    double* factor; /* n*n */
+    double* crhs;   /* n */
+};
+
+/* Level::Level, PressureMultigrid.cpp:24-36 */
+static void level_init(orc_level* L, int w, int h, int spacing, int dw, int dh)
+{
+    memset(L, 0, sizeof(*L));
+    L->w = w;
+    L->h = h;
+    L->stride = w + 2;
+    const size_t count = (size_t)L->stride * (h + 2);
+    L->cx = dalloc(w);
+    L->cy = dalloc(h);
+    L->right = dalloc(count);
+    L->down = dalloc(count);
+    L->inv_diag = dalloc(count);
+    L->rhs = dalloc(count);
+    L->sol = dalloc(count);
+    L->res = dalloc(count);
+    /* A shorter final cell keeps odd grid sizes inside the original domain. */
+    for (int x = 0; x < w; ++x)
+        L->cx[x] = 0.5 * (x * spacing + imin((x + 1) * spacing, dw));
+    for (int y = 0; y < h; ++y)
+        L->cy[y] = 0.5 * (y * spacing + imin((y + 1) * spacing, dh));
+}
+
+static void level_free(orc_level* L)
+{
+    free(L->cx); free(L->cy); free(L->right); free(L->down); free(L->inv_diag);
+    free(L->rhs); free(L->sol); free(L->res);
+    free(L->px); free(L->py); free(L->rx); free(L->ry);
+}
+
+/* buildTransfers axis, PressureMultigrid.cpp:64-98 */
+static void build_axis(const double* fc, int nf, const double* cc, int nc, orc_interp** pro, orc_restr** res)
+{
+    orc_interp* p = (orc_interp*)calloc((size_t)imax(nf, 1), sizeof(orc_interp));
+    orc_restr* r = (orc_restr*)calloc((size_t)imax(nc, 1), sizeof(orc_restr));
+    for (int f = 0; f < nf; ++f) {
+        const int parent = f / 2;
+        const int first = imax(0, parent - (int)(fc[f] < cc[parent]));
+        const int second = imin(first + 1, nc - 1);
+        const double weight = first == second ? 0.0
+            : dclamp((fc[f] - cc[first]) / (cc[second] - cc[first]), 0.0, 1.0);
+        p[f].first = first;
+        p[f].second = second;
+        p[f].weight = weight;
+        /* restriction is the exact transpose of prolongation */
+        const double a = 1.0 - weight;
+        if (a != 0.0) {
+            orc_restr* s = &r[first];
+            s->idx[s->count] = f;
+            s->wt[s->count++] = a;
+        }
+        if (weight != 0.0) {
+            orc_restr* s = &r[second];
+            s->idx[s->count] = f;
+            s->wt[s->count++] = weight;
+        }
+    }
+    *pro = p;
+    *res = r;
+}
+
+orc_mg* orc_mg_create(int width, int height)
+{
+    orc_mg* mg = (orc_mg*)calloc(1, sizeof(orc_mg));
+    const int dw = width, dh = height;
+    int spacing = 1;
+    int cap = 4;
+    mg->lev = (orc_level*)calloc((size_t)cap, sizeof(orc_level));
+    for (;;) {
+        if (mg->nlev == cap) {
+            cap *= 2;
+            mg->lev = (orc_level*)realloc(mg->lev, (size_t)cap * sizeof(orc_level));
+        }
+        level_init(&mg->lev[mg->nlev++], width, height, spacing, dw, dh);
+        if (width * height <= 64)
+            break;
+        width = (width + 1) / 2;
+        height = (height + 1) / 2;
+        spacing *= 2;
+    }
+    for (int i = 0; i + 1 < mg->nlev; ++i) {
+        orc_level* F = &mg->lev[i];
+        const orc_level* C = &mg->lev[i + 1];
+        build_axis(F->cx, F->w, C->cx, C->w, &F->px, &F->rx);
+        build_axis(F->cy, F->h, C->cy, C->h, &F->py, &F->ry);
+        F->nrx = C->w;
+        F->nry = C->h;
+    }
+    const orc_level* last = &mg->lev[mg->nlev - 1];
+    mg->n = last->w * last->h - 1;
+    mg->factor = dalloc((size_t)imax(mg->n, 0) * imax(mg->n, 0));
+    mg->crhs = dalloc((size_t)imax(mg->n, 0));
+    return mg;
+}
+
+void orc_mg_destroy(orc_mg* mg)
+{
+    if (!mg)
+        return;
+    for (int i = 0; i < mg->nlev; ++i)
+        level_free(&mg->lev[i]);
+    free(mg->lev);
+    free(mg->factor);
+    free(mg->crhs);
+    free(mg);
+}
+
+/* Level::rebuildDiagonal, PressureMultigrid.cpp:104-117 */
+static void rebuild_diagonal(orc_level* L)
+{
+    const int s = L->stride;
+    for (int y = 0; y < L->h; ++y) {
+        for (int x = 0, i = (y + 1) * s + 1; x < L->w; ++x, ++i) {
+            const double diagonal = L->right[i - 1] + L->right[i] + L->down[i - s] + L->down[i];
+            L->inv_diag[i] = diagonal > 0.0 ? 1.0 / diagonal : 0.0;
+        }
+    }
+}
+
+/* coarsen, PressureMultigrid.cpp:142-202: parallel paths add; series resistances across density jumps. */
+static void coarsen(const orc_level* F, orc_level* C)
+{
+    for (int y = 0; y < C->h; ++y) {
+        for (int x = 0; x + 1 < C->w; ++x) {
+            const int edge = 2 * x + 2;
+            const double left = (F->cx[edge - 1] - C->cx[x]) / (F->cx[edge - 1] - F->cx[edge - 2]);
+            const double right = edge + 1 < F->w
+                ? (C->cx[x + 1] - F->cx[edge]) / (F->cx[edge + 1] - F->cx[edge]) : 0.0;
+            double conductance = 0.0;
+            for (int fy = 2 * y; fy < imin(2 * y + 2, F->h); ++fy) {
+                const int i = (fy + 1) * F->stride + edge;
+                double resistance = left / F->right[i - 1] + 1.0 / F->right[i];
+                if (right != 0.0)
+                    resistance += right / F->right[i + 1];
+                conductance += 1.0 / resistance;
+            }
+            C->right[(y + 1) * C->stride + x + 1] = conductance;
+        }
+    }
+    for (int y = 0; y < C->h - 1; ++y) {
+        for (int x = 0; x < C->w; ++x) {
+            const int edge = 2 * y + 2;
+            const double up = (F->cy[edge - 1] - C->cy[y]) / (F->cy[edge - 1] - F->cy[edge - 2]);
+            const double down = edge + 1 < F->h
+                ? (C->cy[y + 1] - F->cy[edge]) / (F->cy[edge + 1] - F->cy[edge]) : 0.0;
+            double conductance = 0.0;
+            for (int fx = 2 * x; fx < imin(2 * x + 2, F->w); ++fx) {
+                const int i = edge * F->stride + fx + 1;
+                double resistance = up / F->down[i - F->stride] + 1.0 / F->down[i];
+                if (down != 0.0)
+                    resistance += down / F->down[i + F->stride];
+                conductance += 1.0 / resistance;
+            }
+            C->down[(y + 1) * C->stride + x + 1] = conductance;
+        }
+    }
+    rebuild_diagonal(C);
+}
+
+/* factorCoarsest, PressureMultigrid.cpp:300-345: first cell pinned, dense Cholesky. */
+static int factor_coarsest(orc_mg* mg)
+{
+    const orc_level* L = &mg->lev[mg->nlev - 1];
+    const int n = mg->n;
+    if (n > 0)
+        memset(mg->factor, 0, sizeof(double) * (size_t)n * n);
+    double* f = mg->factor;
+    for (int cell = 1; cell <= n; ++cell) {
+        const int x = cell % L->w;
+        const int y = cell / L->w;
+        const int i = (y + 1) * L->stride + x + 1;
+        const int row = cell - 1;
+        f[row * n + row] = L->right[i - 1] + L->right[i] + L->down[i - L->stride] + L->down[i];
+        if (x > 0 && cell - 1 > 0)
+            f[row * n + row - 1] = -L->right[i - 1];
+        if (y > 0 && cell - L->w > 0)
+            f[row * n + row - L->w] = -L->down[i - L->stride];
+    }
+    for (int row = 0; row < n; ++row) {
+        for (int col = 0; col <= row; ++col) {
+            double value = f[row * n + col];
+            for (int k = 0; k < col; ++k)
+                value -= f[row * n + k] * f[col * n + k];
+            if (row == col) {
+                if (value <= 0.0 || !isfinite(value))
+                    return 0;
+                f[row * n + col] = sqrt(value);
+            } else
+                f[row * n + col] = value / f[col * n + col];
+        }
+    }
+    return 1;
+}
+
+/* PressureMultigrid::rebuild, PressureMultigrid.cpp:119-140 */
+int orc_mg_rebuild(orc_mg* mg, const double* inv_u, const double* inv_v)
+{
+    orc_level* F = &mg->lev[0];
+    for (int y = 0; y < F->h; ++y) {
+        for (int x = 0, i = (y + 1) * F->stride + 1; x < F->w; ++x, ++i) {
+            F->right[i] = inv_u[y * (F->w + 1) + x + 1];
+            F->down[i] = inv_v[(y + 1) * F->w + x];
+        }
+    }
+    rebuild_diagonal(F);
+    for (int i = 0; i + 1 < mg->nlev; ++i)
+        coarsen(&mg->lev[i], &mg->lev[i + 1]);
+    return factor_coarsest(mg);
+}
+
+/* smooth, PressureMultigrid.cpp:204-231: red-black Gauss-Seidel, colour order reversed on the way up. */
+static void smooth(orc_level* L, int reverse, int from_zero)
+{
+    const int s = L->stride;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int color = reverse ? 1 - pass : pass;
+        for (int y = 0; y < L->h; ++y) {
+            for (int x = (color ^ (y & 1)), i = (y + 1) * s + x + 1; x < L->w; x += 2, i += 2) {
+                double value = L->rhs[i];
+                if (!(from_zero && pass == 0)) {
+                    value += (L->right[i - 1] * L->sol[i - 1] + L->right[i] * L->sol[i + 1]) +
+                             (L->down[i - s] * L->sol[i - s] + L->down[i] * L->sol[i + s]);
+                }
+                L->sol[i] = value * L->inv_diag[i];
+            }
+        }
+    }
+}
+
+/* restrictResidual, PressureMultigrid.cpp:233-276 */
+static void restrict_residual(orc_level* F, orc_level* C)
+{
+    const int s = F->stride;
+    for (int y = 0; y < F->h; ++y) {
+        for (int x = 0, i = (y + 1) * s + 1; x < F->w; ++x, ++i) {
+            const double center = F->sol[i];
+            const double product = (F->right[i - 1] * (center - F->sol[i - 1]) +
+                                       F->right[i] * (center - F->sol[i + 1])) +
+                                   (F->down[i - s] * (center - F->sol[i - s]) +
+                                       F->down[i] * (center - F->sol[i + s]));
+            F->res[i] = F->rhs[i] - product;
+        }
+    }
+    for (int y = 0; y < C->h; ++y) {
+        for (int x = 0; x < C->w; ++x) {
+            const orc_restr* rx = &F->rx[x];
+            const orc_restr* ry = &F->ry[y];
+            double value = 0.0;
+            for (int iy = 0; iy < ry->count; ++iy) {
+                const double* row = F->res + (ry->idx[iy] + 1) * s + 1;
+                double horizontal = 0.0;
+                for (int ix = 0; ix < rx->count; ++ix)
+                    horizontal += rx->wt[ix] * row[rx->idx[ix]];
+                value += ry->wt[iy] * horizontal;
+            }
+            C->rhs[(y + 1) * C->stride + x + 1] = value;
+        }
+    }
+}
+
+/* prolongCorrection, PressureMultigrid.cpp:278-298 */
+static void prolong_correction(const orc_level* C, orc_level* F)
+{
+    for (int y = 0; y < F->h; ++y) {
+        const orc_interp* py = &F->py[y];
+        const double* top = C->sol + (py->first + 1) * C->stride + 1;
+        const double* bottom = C->sol + (py->second + 1) * C->stride + 1;
+        for (int x = 0, i = (y + 1) * F->stride + 1; x < F->w; ++x, ++i) {
+            const orc_interp* px = &F->px[x];
+            const double a = (1.0 - px->weight) * top[px->first] + px->weight * top[px->second];
+            const double b = (1.0 - px->weight) * bottom[px->first] + px->weight * bottom[px->second];
+            F->sol[i] += (1.0 - py->weight) * a + py->weight * b;
+        }
+    }
+}
+
+/* solveCoarsest, PressureMultigrid.cpp:347-384 */
+static void solve_coarsest(orc_mg* mg)
+{
+    orc_level* L = &mg->lev[mg->nlev - 1];
+    const int n = mg->n;
+    const double* f = mg->factor;
+    double* c = mg->crhs;
+    double mean = 0.0;
+    for (int y = 0; y < L->h; ++y)
+        for (int x = 0; x < L->w; ++x)
+            mean += L->rhs[(y + 1) * L->stride + x + 1];
+    mean /= n + 1;
+    for (int row = 0; row < n; ++row) {
+        const int cell = row + 1;
+        double value = L->rhs[(cell / L->w + 1) * L->stride + cell % L->w + 1] - mean;
+        for (int col = 0; col < row; ++col)
+            value -= f[row * n + col] * c[col];
+        c[row] = value / f[row * n + row];
+    }
+    for (int row = n - 1; row >= 0; --row) {
+        double value = c[row];
+        for (int col = row + 1; col < n; ++col)
+            value -= f[col * n + row] * c[col];
+        c[row] = value / f[row * n + row];
+    }
+    /* std::accumulate: sequential */
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i)
+        acc = acc + c[i];
+    mean = acc / (n + 1);
+    L->sol[L->stride + 1] = -mean;
+    for (int cell = 1; cell <= n; ++cell)
+        L->sol[(cell / L->w + 1) * L->stride + cell % L->w + 1] = c[cell - 1] - mean;
+}
+
+/* cycle, PressureMultigrid.cpp:386-408: V(4,4) */
+static void cycle(orc_mg* mg, int index)
+{
+    if (index + 1 == mg->nlev) {
+        solve_coarsest(mg);
+        return;
+    }
+    orc_level* F = &mg->lev[index];
+    orc_level* C = &mg->lev[index + 1];
+    const int sweeps = 4;
+    smooth(F, 0, 1);
+    for (int k = 1; k < sweeps; ++k)
+        smooth(F, 0, 0);
+    restrict_residual(F, C);
+    cycle(mg, index + 1);
+    prolong_correction(C, F);
+    for (int k = 0; k < sweeps; ++k)
+        smooth(F, 1, 0);
+}
+
+/* apply, PressureMultigrid.cpp:410-434 */
+void orc_mg_apply(orc_mg* mg, const double* residual, double* correction, double residual_mean)
+{
+    orc_level* F = &mg->lev[0];
+    for (int y = 0; y < F->h; ++y)
+        for (int x = 0; x < F->w; ++x)
+            F->rhs[(y + 1) * F->stride + x + 1] = residual[y * F->w + x] - residual_mean;
+    cycle(mg, 0);
+    for (int y = 0; y < F->h; ++y)
+        memcpy(correction + (size_t)y * F->w, F->sol + (y + 1) * F->stride + 1, sizeof(double) * (size_t)F->w);
+}
+
+int orc_mg_levels(const orc_mg* mg) { return mg->nlev; }
+
+void orc_mg_level_shape(const orc_mg* mg, int level, int* width, int* height)
+{
+    *width = mg->lev[level].w;
+    *height = mg->lev[level].h;
+}
+
+void orc_mg_level_centers(const orc_mg* mg, int level, double* cx, double* cy)
+{
+    const orc_level* L = &mg->lev[level];
+    memcpy(cx, L->cx, sizeof(double) * (size_t)L->w);
+    memcpy(cy, L->cy, sizeof(double) * (size_t)L->h);
+}
+
+void orc_mg_level_weights(const orc_mg* mg, int level, double* right, double* down, double* inv_diag)
+{
+    const orc_level* L = &mg->lev[level];
+    for (int y = 0; y < L->h; ++y) {
+        for (int x = 0; x < L->w; ++x) {
+            const int i = (y + 1) * L->stride + x + 1;
+            if (right) right[y * L->w + x] = L->right[i];
+            if (down) down[y * L->w + x] = L->down[i];
+            if (inv_diag) inv_diag[y * L->w + x] = L->inv_diag[i];
+        }
+    }
+}
+
+void orc_mg_level_prolong(const orc_mg* mg, int level, int axis, int* first, int* second, double* weight)
+{
+    const orc_level* L = &mg->lev[level];
+    const orc_interp* p = axis == 0 ? L->px : L->py;
+    const int n = axis == 0 ? L->w : L->h;
+    for (int i = 0; i < n; ++i) {
+        first[i] = p[i].first;
+        second[i] = p[i].second;
+        weight[i] = p[i].weight;
+    }
+}
+
+void orc_mg_level_restrict(const orc_mg* mg, int level, int axis, int* count, int* idx, double* wt)
+{
+    const orc_level* L = &mg->lev[level];
+    const orc_restr* r = axis == 0 ? L->rx : L->ry;
+    const int n = axis == 0 ? L->nrx : L->nry;
+    for (int c = 0; c < n; ++c) {
+        count[c] = r[c].count;
+        for (int k = 0; k < 4; ++k) {
+            idx[c * 4 + k] = k < r[c].count ? r[c].idx[k] : -1;
+            wt[c * 4 + k] = k < r[c].count ? r[c].wt[k] : 0.0;
+        }
+    }
+}
+
+int orc_mg_coarse_factor(const orc_mg* mg, double* factor)
+{
+    if (factor && mg->n > 0)
+        memcpy(factor, mg->factor, sizeof(double) * (size_t)mg->n * mg->n);
+    return mg->n;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FluidGridCPU subset                                                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+struct orc_grid {
+    int W, H;
+    double h;
+    double ox, oy;
+    double gravity_y;
+    int preconditioner; /* 0 diagonal, 1 multigrid */
+    int nl, ng;
+    double *rho_l, *rho_g;
+    double *fraction, *u, *v, *p, *p_prev;
+    double *lmass, *gmass;
+    double *density, *inv_u, *inv_v, *inv_diag;
+    double *rhs, *res, *cor, *dir, *prod;
+    orc_mg* mg;
+    int density_valid, coeff_valid, mg_valid;
+};
+
+orc_grid* orc_grid_create(int W, int H, double cell_size)
+{
+    orc_grid* g = (orc_grid*)calloc(1, sizeof(orc_grid));
+    const size_t N = (size_t)W * H;
+    g->W = W;
+    g->H = H;
+    g->h = cell_size;
+    g->gravity_y = 100.0; /* FluidGridCPU.h:37 default */
+    g->preconditioner = 1;
+    g->fraction = dalloc(N);
+    g->u = dalloc((size_t)(W + 1) * H);
+    g->v = dalloc((size_t)W * (H + 1));
+    g->p = dalloc(N);
+    g->p_prev = dalloc(N);
+    g->density = dalloc(N);
+    g->inv_u = dalloc((size_t)(W + 1) * H);
+    g->inv_v = dalloc((size_t)W * (H + 1));
+    g->inv_diag = dalloc(N);
+    g->rhs = dalloc(N);
+    g->res = dalloc(N);
+    g->cor = dalloc(N);
+    g->dir = dalloc(N);
+    g->prod = dalloc(N);
+    g->mg = orc_mg_create(W, H);
+    return g;
+}
+
+void orc_grid_destroy(orc_grid* g)
+{
+    if (!g)
+        return;
+    free(g->rho_l); free(g->rho_g);
+    free(g->fraction); free(g->u); free(g->v); free(g->p); free(g->p_prev);
+    free(g->lmass); free(g->gmass);
+    free(g->density); free(g->inv_u); free(g->inv_v); free(g->inv_diag);
+    free(g->rhs); free(g->res); free(g->cor); free(g->dir); free(g->prod);
+    orc_mg_destroy(g->mg);
+    free(g);
+}
+
+void orc_set_origin(orc_grid* g, double x, double y) { g->ox = x; g->oy = y; }
+void orc_set_gravity(orc_grid* g, double gy) { g->gravity_y = gy; }
+void orc_set_preconditioner(orc_grid* g, int kind) { g->preconditioner = kind; }
+
+void orc_invalidate_caches(orc_grid* g)
+{
+    g->density_valid = 0;
+    g->coeff_valid = 0;
+    g->mg_valid = 0;
+}
+
+int orc_configure(orc_grid* g, int nl, const double* rl, int ng, const double* rg)
+{
+    const size_t N = (size_t)g->W * g->H;
+    free(g->rho_l); free(g->rho_g); free(g->lmass); free(g->gmass);
+    g->nl = nl;
+    g->ng = ng;
+    g->rho_l = dalloc((size_t)nl);
+    g->rho_g = dalloc((size_t)ng);
+    for (int k = 0; k < nl; ++k) g->rho_l[k] = rl[k];
+    for (int k = 0; k < ng; ++k) g->rho_g[k] = rg[k];
+    g->lmass = dalloc((size_t)nl * N);
+    g->gmass = dalloc((size_t)ng * N);
+    orc_invalidate_caches(g);
+    return 0;
+}
+
+/* upload, FluidGridCPU.cpp:198-251: empty means zero; the pressure history restarts. */
+int orc_upload(orc_grid* g, const double* fraction, const double* u, const double* v, const double* p,
+    const double* lm, const double* gm)
+{
+    const size_t N = (size_t)g->W * g->H;
+    const size_t NU = (size_t)(g->W + 1) * g->H, NV = (size_t)g->W * (g->H + 1);
+#define ORC_ASSIGN(dst, src, n) do { if (src) memcpy(dst, src, sizeof(double) * (n)); else memset(dst, 0, sizeof(double) * (n)); } while (0)
+    ORC_ASSIGN(g->fraction, fraction, N);
+    ORC_ASSIGN(g->u, u, NU);
+    ORC_ASSIGN(g->v, v, NV);
+    ORC_ASSIGN(g->p, p, N);
+    ORC_ASSIGN(g->lmass, lm, (size_t)g->nl * N);
+    ORC_ASSIGN(g->gmass, gm, (size_t)g->ng * N);
+#undef ORC_ASSIGN
+    memcpy(g->p_prev, g->p, sizeof(double) * N);
+    orc_invalidate_caches(g);
+    return 0;
+}
+
+void orc_download(const orc_grid* g, double* fraction, double* u, double* v, double* p, double* lm, double* gm)
+{
+    const size_t N = (size_t)g->W * g->H;
+    if (fraction) memcpy(fraction, g->fraction, sizeof(double) * N);
+    if (u) memcpy(u, g->u, sizeof(double) * (size_t)(g->W + 1) * g->H);
+    if (v) memcpy(v, g->v, sizeof(double) * (size_t)g->W * (g->H + 1));
+    if (p) memcpy(p, g->p, sizeof(double) * N);
+    if (lm) memcpy(lm, g->lmass, sizeof(double) * (size_t)g->nl * N);
+    if (gm) memcpy(gm, g->gmass, sizeof(double) * (size_t)g->ng * N);
+}
+
+/* rebuildDensity, FluidGridCPU.cpp:1399-1423 */
+static void rebuild_density(orc_grid* g)
+{
+    if (g->density_valid)
+        return;
+    const size_t N = (size_t)g->W * g->H;
+    const double inverse_cell_volume = 1.0 / (g->h * g->h);
+    for (size_t i = 0; i < N; ++i) {
+        double total = 0.0;
+        for (int k = 0; k < g->nl; ++k) total += g->lmass[(size_t)k * N + i];
+        for (int k = 0; k < g->ng; ++k) total += g->gmass[(size_t)k * N + i];
+        g->density[i] = total * inverse_cell_volume;
+    }
+    g->density_valid = 1;
+}
+
+/* rebuildPressureCoefficients, FluidGridCPU.cpp:1154-1205 */
+static void rebuild_coefficients(orc_grid* g)
+{
+    if (g->coeff_valid)
+        return;
+    rebuild_density(g);
+    const int W = g->W, H = g->H;
+    for (int y = 0; y < H; ++y) {
+        const double* row = g->density + (size_t)y * W;
+        double* hz = g->inv_u + (size_t)y * (W + 1);
+        double* vt = g->inv_v + (size_t)y * W;
+        hz[0] = hz[W] = 0.0;
+        for (int x = 1; x < W; ++x)
+            hz[x] = 2.0 / (row[x - 1] + row[x]);
+        if (y == 0) {
+            for (int x = 0; x < W; ++x) vt[x] = 0.0;
+        } else {
+            for (int x = 0; x < W; ++x)
+                vt[x] = 2.0 / (row[x - W] + row[x]);
+        }
+    }
+    for (int x = 0; x < W; ++x)
+        g->inv_v[(size_t)H * W + x] = 0.0;
+    for (int y = 0; y < H; ++y) {
+        const double* hz = g->inv_u + (size_t)y * (W + 1);
+        const double* vt = g->inv_v + (size_t)y * W;
+        double* d = g->inv_diag + (size_t)y * W;
+        for (int x = 0; x < W; ++x) {
+            const double sum = hz[x] + hz[x + 1] + vt[x] + vt[x + W];
+            d[x] = sum > 0.0 ? 1.0 / sum : 0.0;
+        }
+    }
+    g->coeff_valid = 1;
+}
+
+void orc_density(orc_grid* g, double* density)
+{
+    rebuild_density(g);
+    memcpy(density, g->density, sizeof(double) * (size_t)g->W * g->H);
+}
+
+void orc_face_weights(orc_grid* g, double* iu, double* iv)
+{
+    rebuild_coefficients(g);
+    if (iu) memcpy(iu, g->inv_u, sizeof(double) * (size_t)(g->W + 1) * g->H);
+    if (iv) memcpy(iv, g->inv_v, sizeof(double) * (size_t)g->W * (g->H + 1));
+}
+
+/* applyGravity, FluidGridCPU.cpp:988-996: interior v rows only. */
+void orc_apply_gravity(orc_grid* g, double dt)
+{
+    const int W = g->W, H = g->H;
+    for (int i = W; i < W * (H - 1) + W; ++i)
+        g->v[i] += g->gravity_y * dt;
+}
+
+/* addVortex, FluidGridCPU.cpp:998-1028 */
+void orc_add_vortex(orc_grid* g, double cx, double cy, double radius, double strength)
+{
+    const int W = g->W, H = g->H;
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x <= W; ++x) {
+            const double px = g->ox + x * g->h, py = g->oy + (y + 0.5) * g->h;
+            const double dx = px - cx, dy = py - cy;
+            const double r2 = dx * dx + dy * dy;
+            const double falloff = exp(-r2 / (2.0 * radius * radius));
+            g->u[y * (W + 1) + x] += -dy * strength * falloff;
+        }
+    }
+    for (int y = 0; y <= H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            const double px = g->ox + (x + 0.5) * g->h, py = g->oy + y * g->h;
+            const double dx = px - cx, dy = py - cy;
+            const double r2 = dx * dx + dy * dy;
+            const double falloff = exp(-r2 / (2.0 * radius * radius));
+            g->v[y * W + x] += dx * strength * falloff;
+        }
+    }
+}
+
+/* maxDivergence, FluidGridCPU.cpp:1126-1152 */
+double orc_max_divergence(const orc_grid* g)
+{
+    const int W = g->W, H = g->H;
+    double maximum = 0.0;
+    for (int y = 0; y < H; ++y) {
+        const double* hz = g->u + (size_t)y * (W + 1);
+        const double* vt = g->v + (size_t)y * W;
+        for (int x = 0; x < W; ++x) {
+            const double divergence = ((hz[x + 1] - hz[x]) + (vt[x + W] - vt[x])) / g->h;
+            if (!isfinite(divergence))
+                return INFINITY;
+            maximum = dmax(maximum, fabs(divergence));
+        }
+    }
+    return maximum;
+}
+
+/* removeMean, FluidGridCPU.cpp:1478-1486 */
+static void remove_mean(double* a, size_t n)
+{
+    const double mean = reduce_sum(a, n) / (double)n;
+    for (size_t i = 0; i < n; ++i)
+        a[i] -= mean;
+}
+
+/* applyPressureOperator, FluidGridCPU.cpp:1507-1598: q = A p, returns p.q in the reference's order. */
+static double apply_operator(const orc_grid* g, const double* in, double* out)
+{
+    const int W = g->W, H = g->H;
+    const double* iu = g->inv_u;
+    const double* iv = g->inv_v;
+#define ORC_CELL(x, y) ({ \
+        const int i_ = (y) * W + (x); \
+        const double c_ = in[i_]; \
+        double v_ = 0.0; \
+        if ((x) > 0) v_ += iu[(y) * (W + 1) + (x)] * (c_ - in[i_ - 1]); \
+        if ((x) + 1 < W) v_ += iu[(y) * (W + 1) + (x) + 1] * (c_ - in[i_ + 1]); \
+        if ((y) > 0) v_ += iv[i_] * (c_ - in[i_ - W]); \
+        if ((y) + 1 < H) v_ += iv[i_ + W] * (c_ - in[i_ + W]); \
+        out[i_] = v_; \
+        c_ * v_; })
+    if (W < 3 || H < 3) {
+        double dot = 0.0;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+                dot += ORC_CELL(x, y);
+        return dot;
+    }
+    const int stride = W;
+    double dots[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int y = 1; y < H - 1; ++y) {
+        const double* row = in + (size_t)y * stride;
+        const double* above = row - stride;
+        const double* below = row + stride;
+        const double* hz = iu + (size_t)y * (stride + 1);
+        const double* up = iv + (size_t)y * stride;
+        const double* dn = up + stride;
+        double* res = out + (size_t)y * stride;
+#define ORC_APPLY(xx) ({ \
+        const double c_ = row[xx]; \
+        res[xx] = (hz[xx] * (c_ - row[(xx) - 1]) + hz[(xx) + 1] * (c_ - row[(xx) + 1])) + \
+                  (up[xx] * (c_ - above[xx]) + dn[xx] * (c_ - below[xx])); \
+        c_ * res[xx]; })
+        int x = 1;
+        for (; x + 3 < stride - 1; x += 4) {
+            dots[0] += ORC_APPLY(x);
+            dots[1] += ORC_APPLY(x + 1);
+            dots[2] += ORC_APPLY(x + 2);
+            dots[3] += ORC_APPLY(x + 3);
+        }
+        for (; x < stride - 1; ++x)
+            dots[0] += ORC_APPLY(x);
+#undef ORC_APPLY
+    }
+    double dot = (dots[0] + dots[1]) + (dots[2] + dots[3]);
+    for (int x = 0; x < W; ++x) {
+        dot += ORC_CELL(x, 0);
+        dot += ORC_CELL(x, H - 1);
+    }
+    for (int y = 1; y < H - 1; ++y) {
+        dot += ORC_CELL(0, y);
+        dot += ORC_CELL(W - 1, y);
+    }
+#undef ORC_CELL
+    return dot;
+}
+
+double orc_apply_operator(orc_grid* g, const double* p, double* q)
+{
+    rebuild_coefficients(g);
+    return apply_operator(g, p, q);
+}
+
+typedef struct { double sum, maximum; } orc_stats;
+
+/* 4-lane (sum, max) accumulation shared by updatePressureAndResidual and rebuildPressureResidual. */
+#define ORC_LANE_STATS(N, BODY)                                                          \
+    do {                                                                                 \
+        double sums_[4] = {0, 0, 0, 0}, maxs_[4] = {0, 0, 0, 0};                         \
+        size_t i = 0;                                                                    \
+        for (; i + 3 < (N); i += 4) {                                                    \
+            for (int lane_ = 0; lane_ < 4; ++lane_) {                                    \
+                const size_t j = i + lane_;                                              \
+                const double value = (BODY);                                             \
+                sums_[lane_] += value;                                                   \
+                maxs_[lane_] = dmax(maxs_[lane_], fabs(value));                          \
+            }                                                                            \
+        }                                                                                \
+        st.sum = (sums_[0] + sums_[1]) + (sums_[2] + sums_[3]);                          \
+        st.maximum = dmax(dmax(maxs_[0], maxs_[1]), dmax(maxs_[2], maxs_[3]));           \
+        for (; i < (N); ++i) {                                                           \
+            const size_t j = i;                                                          \
+            const double value = (BODY);                                                 \
+            st.sum += value;                                                             \
+            st.maximum = dmax(st.maximum, fabs(value));                                  \
+        }                                                                                \
+    } while (0)
+
+/* updatePressureAndResidual, FluidGridCPU.cpp:1600-1641 */
+static orc_stats update_pressure_residual(orc_grid* g, double alpha)
+{
+    const size_t N = (size_t)g->W * g->H;
+    orc_stats st;
+    double* p = g->p;
+    double* r = g->res;
+    const double* d = g->dir;
+    const double* q = g->prod;
+    ORC_LANE_STATS(N, (p[j] += alpha * d[j], r[j] -= alpha * q[j]));
+    if (!isfinite(st.sum))
+        st.maximum = INFINITY;
+    return st;
+}
+
+/* rebuildPressureResidual, FluidGridCPU.cpp:1643-1683 */
+static orc_stats rebuild_residual(orc_grid* g)
+{
+    const size_t N = (size_t)g->W * g->H;
+    apply_operator(g, g->p, g->prod);
+    orc_stats st;
+    double* r = g->res;
+    const double* b = g->rhs;
+    const double* q = g->prod;
+    ORC_LANE_STATS(N, (r[j] = b[j] - q[j]));
+    if (!isfinite(st.sum))
+        st.maximum = INFINITY;
+    return st;
+}
+
+/* applyPreconditioner, FluidGridCPU.cpp:1685-1726 */
+static double apply_preconditioner(orc_grid* g, double residual_mean)
+{
+    const size_t N = (size_t)g->W * g->H;
+    if (g->preconditioner == 1)
+        orc_mg_apply(g->mg, g->res, g->cor, residual_mean);
+    else
+        for (size_t i = 0; i < N; ++i)
+            g->cor[i] = (g->res[i] - residual_mean) * g->inv_diag[i];
+    const double mean = reduce_sum(g->cor, N) / (double)N;
+    double dots[4] = {0, 0, 0, 0};
+    size_t i = 0;
+    for (; i + 3 < N; i += 4) {
+        for (int lane = 0; lane < 4; ++lane) {
+            g->cor[i + lane] -= mean;
+            dots[lane] += g->res[i + lane] * g->cor[i + lane];
+        }
+    }
+    double dot = (dots[0] + dots[1]) + (dots[2] + dots[3]);
+    for (; i < N; ++i) {
+        g->cor[i] -= mean;
+        dot += g->res[i] * g->cor[i];
+    }
+    return dot;
+}
+
+/* solvePressure, FluidGridCPU.cpp:1728-1824 */
+static int solve_pressure(orc_grid* g, int max_iterations, double tol, orc_result* info)
+{
+    const size_t N = (size_t)g->W * g->H;
+    if (max_abs(g->rhs, N) <= tol) {
+        memset(g->p, 0, sizeof(double) * N);
+        memset(g->p_prev, 0, sizeof(double) * N);
+        return 0;
+    }
+    for (size_t i = 0; i < N; ++i) {
+        const double latest = g->p[i];
+        g->p[i] = 2.0 * latest - g->p_prev[i];
+        g->p_prev[i] = latest;
+    }
+    remove_mean(g->p, N);
+    const orc_stats initial = rebuild_residual(g);
+    info->initial_residual = initial.maximum;
+    if (initial.maximum <= tol || max_iterations == 0)
+        return 0;
+    if (g->preconditioner == 1 && !g->mg_valid) {
+        g->mg_valid = orc_mg_rebuild(g->mg, g->inv_u, g->inv_v);
+        if (!g->mg_valid)
+            return -1;
+    }
+    double residual_mean = initial.sum / (double)N;
+    double rz = apply_preconditioner(g, residual_mean);
+    memcpy(g->dir, g->cor, sizeof(double) * N);
+    int iterations = 0;
+    while (iterations < max_iterations && rz > 0.0 && isfinite(rz)) {
+        const double denominator = apply_operator(g, g->dir, g->prod);
+        if (denominator <= 0.0 || !isfinite(denominator))
+            break;
+        const double alpha = rz / denominator;
+        if (!isfinite(alpha))
+            break;
+        const orc_stats st = update_pressure_residual(g, alpha);
+        residual_mean = st.sum / (double)N;
+        ++iterations;
+        int restart = 0;
+        if (st.maximum <= tol) {
+            const orc_stats actual = rebuild_residual(g);
+            if (actual.maximum <= tol)
+                break;
+            residual_mean = actual.sum / (double)N;
+            restart = 1;
+            info->restarts++;
+        }
+        const double next = apply_preconditioner(g, residual_mean);
+        const double beta = restart ? 0.0 : next / rz;
+        if (!isfinite(beta))
+            break;
+        for (size_t i = 0; i < N; ++i)
+            g->dir[i] = g->cor[i] + beta * g->dir[i];
+        rz = next;
+    }
+    remove_mean(g->p, N);
+    return iterations;
+}
+
+/* Form rhs = -div * h^2/dt and remove its mean (FluidGridCPU.cpp:1063-1090). */
+static void form_rhs(orc_grid* g, double dt)
+{
+    const int W = g->W, H = g->H;
+    const size_t N = (size_t)W * H;
+    const double scale = g->h * g->h / dt;
+    double sum = 0.0;
+    for (int y = 0; y < H; ++y) {
+        const double* hz = g->u + (size_t)y * (W + 1);
+        const double* vt = g->v + (size_t)y * W;
+        double* rhs = g->rhs + (size_t)y * W;
+        for (int x = 0; x < W; ++x) {
+            const double du = hz[x + 1] - hz[x];
+            const double dv = vt[x + W] - vt[x];
+            rhs[x] = -(du + dv) / g->h * scale;
+            sum += rhs[x];
+        }
+    }
+    const double mean = sum / (double)N;
+    for (size_t i = 0; i < N; ++i)
+        g->rhs[i] -= mean;
+}
+
+void orc_pressure_rhs(orc_grid* g, double dt, double* rhs)
+{
+    form_rhs(g, dt);
+    memcpy(rhs, g->rhs, sizeof(double) * (size_t)g->W * g->H);
+}
+
+/* project, FluidGridCPU.cpp:1030-1124 */
+orc_result orc_project(orc_grid* g, double dt, int max_iterations, double tol)
+{
+    orc_result result;
+    memset(&result, 0, sizeof(result));
+    result.max_divergence = INFINITY;
+    result.physical_steps = 1;
+    const int W = g->W, H = g->H;
+    if (dt <= 0.0 || W <= 0 || H <= 0 || max_iterations < 0)
+        return result;
+    const double scale = g->h * g->h / dt;
+    rebuild_coefficients(g);
+    for (int y = 0; y < H; ++y) {
+        g->u[y * (W + 1)] = 0.0;
+        g->u[y * (W + 1) + W] = 0.0;
+    }
+    for (int x = 0; x < W; ++x) {
+        g->v[x] = 0.0;
+        g->v[H * W + x] = 0.0;
+    }
+    form_rhs(g, dt);
+    result.iterations = solve_pressure(g, max_iterations, tol * scale, &result);
+    if (result.iterations < 0) {
+        orc_result failed;
+        memset(&failed, 0, sizeof(failed));
+        failed.max_divergence = INFINITY;
+        failed.physical_steps = 1;
+        return failed;
+    }
+    for (int y = 0; y < H; ++y) {
+        const double* row = g->p + (size_t)y * W;
+        double* hz = g->u + (size_t)y * (W + 1);
+        const double* hw = g->inv_u + (size_t)y * (W + 1);
+        for (int x = 1; x < W; ++x)
+            hz[x] -= ((row[x] - row[x - 1]) / g->h) * dt * hw[x];
+        if (y == 0)
+            continue;
+        double* vt = g->v + (size_t)y * W;
+        const double* vw = g->inv_v + (size_t)y * W;
+        for (int x = 0; x < W; ++x)
+            vt[x] -= ((row[x] - row[x - W]) / g->h) * dt * vw[x];
+    }
+    /* diagnostic: true residual of the returned pressure */
+    {
+        const size_t N = (size_t)W * H;
+        double* q = (double*)malloc(sizeof(double) * N);
+        apply_operator(g, g->p, q);
+        double m = 0.0;
+        for (size_t i = 0; i < N; ++i)
+            m = dmax(m, fabs(g->rhs[i] - q[i]));
+        result.final_residual = m;
+        free(q);
+    }
+    result.max_divergence = orc_max_divergence(g);
+    result.converged = result.max_divergence <= tol;
+    return result;
+}
+
+orc_result orc_force_project(orc_grid* g, double dt, int max_iterations, double tol)
+{
+    if (dt <= 0.0) {
+        orc_result r;
+        memset(&r, 0, sizeof(r));
+        r.max_divergence = INFINITY;
+        return r;
+    }
+    orc_apply_gravity(g, dt);
+    return orc_project(g, dt, max_iterations, tol);
+}
+
+/* sampleVelocity, FluidGridCPU.cpp:465-471 */
+void orc_sample_velocity(const orc_grid* g, double px, double py, double* vx, double* vy)
+{
+    const double x = (px - g->ox) / g->h;
+    const double y = (py - g->oy) / g->h;
+    *vx = sample_scalar(g->u, g->W + 1, g->H, x, y - 0.5);
+    *vy = sample_scalar(g->v, g->W, g->H + 1, x - 0.5, y);
+}
+
+/*
+ * Semi-Lagrangian transport mode.  The reference has no such step (its transport is
+ * PLIC/VOF); this is the build's own definition, built only from the reference's staggered
+ * sampler (sampleVelocity/sampleScalar).  Midpoint (RK2) backtrace in index space:
+ *   a = P - 0.5*dt*vel(P);  src = P - dt*vel(a);  new = sample(old field, src)
+ * for every interior u and v face and every cell-centred scalar (fraction, masses).
+ * Wall faces stay zero.
+ */
+static void backtrace(const orc_grid* g, double px, double py, double dt, double* sx, double* sy)
+{
+    double vx, vy;
+    orc_sample_velocity(g, px, py, &vx, &vy);
+    const double mx = px - 0.5 * dt * vx, my = py - 0.5 * dt * vy;
+    orc_sample_velocity(g, mx, my, &vx, &vy);
+    *sx = px - dt * vx;
+    *sy = py - dt * vy;
+}
+
+void orc_advect_semi_lagrangian(orc_grid* g, double dt)
+{
+    const int W = g->W, H = g->H;
+    const size_t N = (size_t)W * H;
+    const size_t NU = (size_t)(W + 1) * H, NV = (size_t)W * (H + 1);
+    double* nu = dalloc(NU);
+    double* nv = dalloc(NV);
+    double* tmp = dalloc(N);
+    const double inv_h = 1.0 / g->h;
+    (void)inv_h;
+    for (int y = 0; y < H; ++y) {
+        for (int x = 1; x < W; ++x) {
+            double sx, sy;
+            backtrace(g, g->ox + x * g->h, g->oy + (y + 0.5) * g->h, dt, &sx, &sy);
+            nu[y * (W + 1) + x] = sample_scalar(g->u, W + 1, H, (sx - g->ox) / g->h, (sy - g->oy) / g->h - 0.5);
+        }
+    }
+    for (int y = 1; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            double sx, sy;
+            backtrace(g, g->ox + (x + 0.5) * g->h, g->oy + y * g->h, dt, &sx, &sy);
+            nv[y * W + x] = sample_scalar(g->v, W, H + 1, (sx - g->ox) / g->h - 0.5, (sy - g->oy) / g->h);
+        }
+    }
+    /* cell-centred scalars share one backtrace per cell */
+    double* srcx = dalloc(N);
+    double* srcy = dalloc(N);
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            double sx, sy;
+            backtrace(g, g->ox + (x + 0.5) * g->h, g->oy + (y + 0.5) * g->h, dt, &sx, &sy);
+            srcx[y * W + x] = (sx - g->ox) / g->h - 0.5;
+            srcy[y * W + x] = (sy - g->oy) / g->h - 0.5;
+        }
+    }
+    const int fields = 1 + g->nl + g->ng;
+    for (int f = 0; f < fields; ++f) {
+        double* a = f == 0 ? g->fraction : (f <= g->nl ? g->lmass + (size_t)(f - 1) * N
+                                                        : g->gmass + (size_t)(f - 1 - g->nl) * N);
+        for (size_t i = 0; i < N; ++i)
+            tmp[i] = sample_scalar(a, W, H, srcx[i], srcy[i]);
+        memcpy(a, tmp, sizeof(double) * N);
+    }
+    memcpy(g->u, nu, sizeof(double) * NU);
+    memcpy(g->v, nv, sizeof(double) * NV);
+    free(nu); free(nv); free(tmp); free(srcx); free(srcy);
+    orc_invalidate_caches(g);
+}
+
+orc_result orc_step_semi_lagrangian(orc_grid* g, double dt, int max_iterations, double tol)
+{
+    if (dt <= 0.0) {
+        orc_result r;
+        memset(&r, 0, sizeof(r));
+        r.max_divergence = INFINITY;
+        return r;
+    }
+    orc_advect_semi_lagrangian(g, dt);
+    orc_result r = orc_force_project(g, dt, max_iterations, tol);
+    r.transport_steps = 1;
+    return r;
+}

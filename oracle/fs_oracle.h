@@ -1,0 +1,96 @@
+/*
+ * fs_oracle.h -- FP64 CPU restatement of the reference UniformCPU pressure projection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity oracle: only tests/, the smoke check
+ * and the CPU-baseline leg of bench.py may load it, and only as the checker.  The
+ * product path (fluidsim_amd, libfluidsim_hip.so) never links or calls it.
+ *
+ * Follows, single-threaded (the reference's bitwise-reproducible mode):
+ *   FluidSim/UniformCPU/FluidGridCPU.cpp      (project, solvePressure, operator, gravity, vortices)
+ *   FluidSim/UniformCPU/PressureMultigrid.cpp (hierarchy, coarsening, RB-GS V(4,4), Cholesky)
+ *
+ * Layout conventions are the reference's: u is (W+1) x H, v is W x (H+1), cells W x H,
+ * all row-major, y grows downward (FluidGridCPU.h:128-130).
+ */
+#ifndef FS_ORACLE_H
+#define FS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mg orc_mg;
+typedef struct orc_grid orc_grid;
+
+/* Mirrors FluidProjectionResult (FluidGridTypes.h:59-79), fields used on this path. */
+typedef struct orc_result {
+    int iterations;
+    double max_divergence;
+    int converged;
+    int physical_steps;
+    int transport_steps;
+    int restarts;            /* true-residual restarts inside the PCG loop (diagnostic) */
+    double initial_residual; /* max |b - A p| before the loop (diagnostic) */
+    double final_residual;   /* max |b - A p| at exit, recomputed in FP64 (diagnostic) */
+} orc_result;
+
+/* ---- PressureMultigrid (PressureMultigrid.h:9-69) ---- */
+orc_mg* orc_mg_create(int width, int height);
+void orc_mg_destroy(orc_mg* mg);
+/* inv_density_u: (W+1)*H face weights, inv_density_v: W*(H+1). Returns 1 on success, 0 on pivot failure. */
+int orc_mg_rebuild(orc_mg* mg, const double* inv_density_u, const double* inv_density_v);
+/* correction[W*H] = V-cycle(residual - residual_mean). */
+void orc_mg_apply(orc_mg* mg, const double* residual, double* correction, double residual_mean);
+int orc_mg_levels(const orc_mg* mg);
+/* Level shape; returns width, height via pointers. */
+void orc_mg_level_shape(const orc_mg* mg, int level, int* width, int* height);
+/* Copy out level data.  Arrays are interior-only (w*h) for right/down/inv_diag, w for cx, h for cy. */
+void orc_mg_level_centers(const orc_mg* mg, int level, double* cx, double* cy);
+void orc_mg_level_weights(const orc_mg* mg, int level, double* right, double* down, double* inv_diag);
+/* Prolongation per axis: first[n], second[n], weight[n] for the fine level's width (axis 0) or height (axis 1). */
+void orc_mg_level_prolong(const orc_mg* mg, int level, int axis, int* first, int* second, double* weight);
+/* Restriction per axis for coarse index c: count[nc], idx[nc*4], wt[nc*4]. */
+void orc_mg_level_restrict(const orc_mg* mg, int level, int axis, int* count, int* idx, double* wt);
+/* Coarsest dense Cholesky factor (n*n, lower triangle meaningful); returns n. */
+int orc_mg_coarse_factor(const orc_mg* mg, double* factor);
+
+/* ---- FluidGridCPU subset (FluidGridCPU.h:15-225) ---- */
+orc_grid* orc_grid_create(int width, int height, double cell_size);
+void orc_grid_destroy(orc_grid* g);
+void orc_set_origin(orc_grid* g, double x, double y);
+void orc_set_gravity(orc_grid* g, double gravity_y);
+/* 0 = Diagonal, 1 = Multigrid (FluidGridCPU.h:39-44). */
+void orc_set_preconditioner(orc_grid* g, int kind);
+/* Component densities, as configureMaterials (FluidMixture.h:14-39). */
+int orc_configure(orc_grid* g, int n_liquid, const double* liquid_density, int n_gas, const double* gas_density);
+/* Any pointer may be NULL (zero field).  Masses are component-major: k*N + cell. */
+int orc_upload(orc_grid* g, const double* fraction, const double* u, const double* v, const double* pressure,
+    const double* liquid_mass, const double* gas_mass);
+void orc_download(const orc_grid* g, double* fraction, double* u, double* v, double* pressure,
+    double* liquid_mass, double* gas_mass);
+void orc_invalidate_caches(orc_grid* g);
+void orc_apply_gravity(orc_grid* g, double dt);
+void orc_add_vortex(orc_grid* g, double cx, double cy, double radius, double strength);
+double orc_max_divergence(const orc_grid* g);
+/* project(dt, max_iterations, divergence_tolerance) -- FluidGridCPU.cpp:1030-1124. */
+orc_result orc_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
+/* Gravity followed by projection: step() with transport and surface tension off. */
+orc_result orc_force_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
+/* Semi-Lagrangian transport mode (build-defined; the reference has none), then gravity and projection. */
+void orc_advect_semi_lagrangian(orc_grid* g, double dt);
+orc_result orc_step_semi_lagrangian(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
+/* Staggered bilinear velocity sample at a world position (FluidGridCPU.cpp:465-471). */
+void orc_sample_velocity(const orc_grid* g, double px, double py, double* vx, double* vy);
+
+/* Derived fields (read-only views of cached data, rebuilt on demand). */
+void orc_density(orc_grid* g, double* density);
+void orc_face_weights(orc_grid* g, double* inv_density_u, double* inv_density_v);
+/* q = A p (FP64 operator, FluidGridCPU.cpp:1507-1598); returns p.q */
+double orc_apply_operator(orc_grid* g, const double* p, double* q);
+/* rhs the projection would form from the current velocities (before mean removal is applied: mean removed). */
+void orc_pressure_rhs(orc_grid* g, double dt, double* rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/fs_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and bench.py's CPU-baseline leg,
+always as the checker or the timed CPU baseline, never as the product path.
+
+The wrapped C code restates, single-threaded and in FP64, the reference's
+FluidSim/UniformCPU/FluidGridCPU.cpp projection and FluidSim/UniformCPU/PressureMultigrid.cpp.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("max_divergence", C.c_double),
+        ("converged", C.c_int),
+        ("physical_steps", C.c_int),
+        ("transport_steps", C.c_int),
+        ("restarts", C.c_int),
+        ("initial_residual", C.c_double),
+        ("final_residual", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "orc_mg_create": (vp, [C.c_int, C.c_int]),
+        "orc_mg_destroy": (None, [vp]),
+        "orc_mg_rebuild": (C.c_int, [vp, _dp, _dp]),
+        "orc_mg_apply": (None, [vp, _dp, _dp, C.c_double]),
+        "orc_mg_levels": (C.c_int, [vp]),
+        "orc_mg_level_shape": (None, [vp, C.c_int, _ip, _ip]),
+        "orc_mg_level_centers": (None, [vp, C.c_int, _dp, _dp]),
+        "orc_mg_level_weights": (None, [vp, C.c_int, _dp, _dp, _dp]),
+        "orc_mg_level_prolong": (None, [vp, C.c_int, C.c_int, _ip, _ip, _dp]),
+        "orc_mg_level_restrict": (None, [vp, C.c_int, C.c_int, _ip, _ip, _dp]),
+        "orc_mg_coarse_factor": (C.c_int, [vp, _dp]),
+        "orc_grid_create": (vp, [C.c_int, C.c_int, C.c_double]),
+        "orc_grid_destroy": (None, [vp]),
+        "orc_set_origin": (None, [vp, C.c_double, C.c_double]),
+        "orc_set_gravity": (None, [vp, C.c_double]),
+        "orc_set_preconditioner": (None, [vp, C.c_int]),
+        "orc_configure": (C.c_int, [vp, C.c_int, _dp, C.c_int, _dp]),
+        "orc_upload": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_download": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_invalidate_caches": (None, [vp]),
+        "orc_apply_gravity": (None, [vp, C.c_double]),
+        "orc_add_vortex": (None, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "orc_max_divergence": (C.c_double, [vp]),
+        "orc_project": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
+        "orc_force_project": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
+        "orc_advect_semi_lagrangian": (None, [vp, C.c_double]),
+        "orc_step_semi_lagrangian": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
+        "orc_sample_velocity": (None, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "orc_density": (None, [vp, _dp]),
+        "orc_face_weights": (None, [vp, _dp, _dp]),
+        "orc_apply_operator": (C.c_double, [vp, _dp, _dp]),
+        "orc_pressure_rhs": (None, [vp, C.c_double, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Multigrid:
+    """PressureMultigrid (PressureMultigrid.h:9-69) restated in FP64."""
+
+    def __init__(self, width, height):
+        self.w, self.h = int(width), int(height)
+        self._h = lib().orc_mg_create(self.w, self.h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_mg_destroy(self._h)
+            self._h = None
+
+    def rebuild(self, inv_u, inv_v):
+        inv_u, inv_v = _f64(inv_u), _f64(inv_v)
+        assert inv_u.size == (self.w + 1) * self.h and inv_v.size == self.w * (self.h + 1)
+        return bool(lib().orc_mg_rebuild(self._h, _ptr(inv_u), _ptr(inv_v)))
+
+    def apply(self, residual, residual_mean=0.0):
+        r = _f64(residual)
+        assert r.size == self.w * self.h
+        out = np.zeros(self.w * self.h)
+        lib().orc_mg_apply(self._h, _ptr(r), _ptr(out), float(residual_mean))
+        return out
+
+    @property
+    def levels(self):
+        return lib().orc_mg_levels(self._h)
+
+    def shape(self, level):
+        w, h = C.c_int(), C.c_int()
+        lib().orc_mg_level_shape(self._h, level, C.byref(w), C.byref(h))
+        return w.value, h.value
+
+    def centers(self, level):
+        w, h = self.shape(level)
+        cx, cy = np.zeros(w), np.zeros(h)
+        lib().orc_mg_level_centers(self._h, level, _ptr(cx), _ptr(cy))
+        return cx, cy
+
+    def weights(self, level):
+        w, h = self.shape(level)
+        r, d, i = np.zeros(w * h), np.zeros(w * h), np.zeros(w * h)
+        lib().orc_mg_level_weights(self._h, level, _ptr(r), _ptr(d), _ptr(i))
+        return r.reshape(h, w), d.reshape(h, w), i.reshape(h, w)
+
+    def prolong(self, level, axis):
+        w, h = self.shape(level)
+        n = w if axis == 0 else h
+        f, s, wt = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        lib().orc_mg_level_prolong(self._h, level, axis, f.ctypes.data_as(_ip), s.ctypes.data_as(_ip), _ptr(wt))
+        return f, s, wt
+
+    def restriction(self, level, axis):
+        cw, ch = self.shape(level + 1)
+        n = cw if axis == 0 else ch
+        cnt, idx, wt = np.zeros(n, np.int32), np.zeros(4 * n, np.int32), np.zeros(4 * n)
+        lib().orc_mg_level_restrict(self._h, level, axis, cnt.ctypes.data_as(_ip), idx.ctypes.data_as(_ip), _ptr(wt))
+        return cnt, idx.reshape(n, 4), wt.reshape(n, 4)
+
+    def coarse_factor(self):
+        n = lib().orc_mg_coarse_factor(self._h, None)
+        f = np.zeros(max(n, 0) * max(n, 0))
+        lib().orc_mg_coarse_factor(self._h, _ptr(f))
+        return f.reshape(max(n, 0), max(n, 0))
+
+
+class OracleGrid:
+    """FluidGridCPU subset (FluidGridCPU.h:15-225) restated in FP64, host arrays as numpy."""
+
+    def __init__(self, width, height, cell_size):
+        self.width, self.height, self.cell_size = int(width), int(height), float(cell_size)
+        self._h = lib().orc_grid_create(self.width, self.height, self.cell_size)
+        self.n_liquid = 0
+        self.n_gas = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_grid_destroy(self._h)
+            self._h = None
+
+    @property
+    def N(self):
+        return self.width * self.height
+
+    def set_origin(self, x, y):
+        lib().orc_set_origin(self._h, x, y)
+
+    def set_gravity(self, gy):
+        lib().orc_set_gravity(self._h, gy)
+
+    def set_preconditioner(self, kind):
+        lib().orc_set_preconditioner(self._h, {"diagonal": 0, "multigrid": 1}[kind] if isinstance(kind, str) else int(kind))
+
+    def configure(self, liquid_density, gas_density):
+        ld, gd = _f64(np.atleast_1d(liquid_density)), _f64(np.atleast_1d(gas_density))
+        self.n_liquid, self.n_gas = ld.size, gd.size
+        lib().orc_configure(self._h, ld.size, _ptr(ld), gd.size, _ptr(gd))
+
+    def upload(self, fraction=None, u=None, v=None, pressure=None, liquid_mass=None, gas_mass=None):
+        args = [_f64(a) for a in (fraction, u, v, pressure, liquid_mass, gas_mass)]
+        sizes = [self.N, (self.width + 1) * self.height, self.width * (self.height + 1), self.N,
+                 self.n_liquid * self.N, self.n_gas * self.N]
+        for a, s in zip(args, sizes):
+            if a is not None and a.size != s:
+                raise ValueError("upload dimensions do not match the grid")
+        self._keep = args
+        lib().orc_upload(self._h, *[_ptr(a) for a in args])
+
+    def download(self):
+        out = dict(fraction=np.zeros(self.N), u=np.zeros((self.width + 1) * self.height),
+                   v=np.zeros(self.width * (self.height + 1)), pressure=np.zeros(self.N),
+                   liquid_mass=np.zeros(self.n_liquid * self.N), gas_mass=np.zeros(self.n_gas * self.N))
+        lib().orc_download(self._h, *[_ptr(out[k]) for k in
+                                      ("fraction", "u", "v", "pressure", "liquid_mass", "gas_mass")])
+        return out
+
+    def invalidate_caches(self):
+        lib().orc_invalidate_caches(self._h)
+
+    def apply_gravity(self, dt):
+        lib().orc_apply_gravity(self._h, dt)
+
+    def add_vortex(self, cx, cy, radius, strength):
+        lib().orc_add_vortex(self._h, cx, cy, radius, strength)
+
+    def max_divergence(self):
+        return lib().orc_max_divergence(self._h)
+
+    def project(self, dt, max_iterations=200, divergence_tolerance=1e-6):
+        return lib().orc_project(self._h, dt, max_iterations, divergence_tolerance).as_dict()
+
+    def force_project(self, dt, max_iterations=200, divergence_tolerance=1e-6):
+        return lib().orc_force_project(self._h, dt, max_iterations, divergence_tolerance).as_dict()
+
+    def advect_semi_lagrangian(self, dt):
+        lib().orc_advect_semi_lagrangian(self._h, dt)
+
+    def step_semi_lagrangian(self, dt, max_iterations=200, divergence_tolerance=1e-6):
+        return lib().orc_step_semi_lagrangian(self._h, dt, max_iterations, divergence_tolerance).as_dict()
+
+    def sample_velocity(self, x, y):
+        vx, vy = C.c_double(), C.c_double()
+        lib().orc_sample_velocity(self._h, x, y, C.byref(vx), C.byref(vy))
+        return vx.value, vy.value
+
+    def density(self):
+        d = np.zeros(self.N)
+        lib().orc_density(self._h, _ptr(d))
+        return d
+
+    def face_weights(self):
+        iu = np.zeros((self.width + 1) * self.height)
+        iv = np.zeros(self.width * (self.height + 1))
+        lib().orc_face_weights(self._h, _ptr(iu), _ptr(iv))
+        return iu, iv
+
+    def apply_operator(self, p):
+        p = _f64(p)
+        q = np.zeros(self.N)
+        dot = lib().orc_apply_operator(self._h, _ptr(p), _ptr(q))
+        return q, dot
+
+    def pressure_rhs(self, dt):
+        r = np.zeros(self.N)
+        lib().orc_pressure_rhs(self._h, dt, _ptr(r))
+        return r
