@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfluidsim_hip.so")
 ARCH = os.environ.get("FLUIDSIM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fs_kernels.hip", "fs_host.cpp"]
+SOURCES = ["fs_kernels.hip", "fs_multigrid.hip", "fs_host.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -62,7 +62,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for obj in objs:
         os.remove(obj)
+    build_cpp_test(verbose)
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "grid_api_test.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "grid_api_test")
+
+
+def build_cpp_test(verbose: bool = False) -> str:
+    """The C++ adapter test (host code only): g++ against the C ABI library."""
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", f"-I{ROOT}", "-I/opt/rocm/include", CPP_TEST_SRC,
+           f"-L{HERE}", "-lfluidsim_hip", "-Wl,-rpath,$ORIGIN/../../fluidsim_amd", "-o", CPP_TEST_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError("g++ failed on the C++ adapter test")
+    return CPP_TEST_BIN
 
 
 if __name__ == "__main__":

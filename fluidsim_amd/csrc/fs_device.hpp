@@ -17,6 +17,10 @@ namespace fsim {
 // Solver status codes kept on the device.
 enum PcgStatus : int { kRunning = 0, kConverged = 1, kStopped = 2 };
 
+// The FP32 recurrence residual drifts from the true one; every kReplaceEvery iterations it is
+// replaced by the FP64 true residual (reliable updating, van der Vorst & Ye 2000) without a restart.
+constexpr int kReplaceEvery = 32;
+
 // Device-resident PCG scalar block.  Kernels of one iteration communicate only through this block,
 // so the host never reads a scalar inside the loop (it polls `status` once per batch).
 struct PcgScalars {
@@ -42,7 +46,7 @@ struct PcgScalars {
     int restart;        // next beta is zero (true residual replaced the recurrence)
     int restarts;
     int mg_failed;      // coarse factorisation failed (the hierarchy must be rebuilt next time)
-    int pad1;
+    int replace;        // periodic residual replacement: refresh r from FP64 without restarting
 };
 
 // One multigrid level as the FP32 smoothing kernels see it.
@@ -68,12 +72,6 @@ struct AxisRestrict {
     const int* count;   // [n]
     const int* index;   // [n * 4]
     const float* weight;  // [n * 4]
-};
-
-// Reduction scratch for one kernel: per-block partials plus the arrival counter of the last-block finish.
-struct ReduceSlot {
-    double* partials;
-    unsigned* counter;
 };
 
 __device__ __forceinline__ int pidx(int x, int y, int s) { return (y + 1) * s + (x + 1); }
@@ -129,29 +127,8 @@ template <int NT> __device__ __forceinline__ double block_max(double v, double* 
     return r;
 }
 
-// Publish this block's partials (written by thread 0 before the call) and report whether this
-// block arrived last.  The last block then reads every partial in block order, so the result does
-// not depend on arrival order.
-__device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned nblocks)
-{
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned ticket = atomicAdd(counter, 1u);
-        is_last = ticket == nblocks - 1;
-    }
-    __syncthreads();
-    if (is_last)
-        __threadfence();
-    return is_last != 0;
-}
-
-// Reads a partial written by another block in this launch (bypasses the non-coherent L1).
-__device__ __forceinline__ double load_partial(const double* p, int i)
-{
-    return __hip_atomic_load(const_cast<double*>(p) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Reads a partial written by another block of the previous launch.
+__device__ __forceinline__ double load_partial(const double* p, int i) { return p[i]; }
 
 // Sum of `n` partials with stride `k` starting at `offset`, in a fixed order; every thread returns it.
 template <int NT>

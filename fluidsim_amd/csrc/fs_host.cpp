@@ -155,6 +155,7 @@ using namespace fsim;
 
 struct fs_context {
     int device = 0;
+    size_t lds_per_block = 0;
     hipStream_t stream = nullptr;
     PcgScalars* pinned = nullptr;   // host-pinned mirror of a grid's scalar block
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -183,9 +184,10 @@ struct fs_grid {
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<PcgScalars> S;
     DevBuf<double> partials;
-    DevBuf<unsigned> counter;
     DevBuf<double> factor;
+    DevBuf<float> ginv;
     int coarse_n = 0;
+    size_t tail_lds = 0;
     std::vector<Level> levels;
     int tail_start = 0;
     TailDesc tail{};
@@ -243,7 +245,6 @@ struct fs_grid {
         g.zdiag = zdiag.p;
         g.S = S.p;
         g.partials = partials.p;
-        g.counter = counter.p;
         return g;
     }
 
@@ -285,10 +286,36 @@ struct fs_grid {
         bytes += q.alloc(NPd);
         bytes += zdiag.alloc(NPd);
         bytes += S.alloc(1);
-        bytes += counter.alloc(1);
         const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
         bytes += partials.alloc(std::max<size_t>(4 * blocks, 64));
         build_hierarchy();
+    }
+
+    void fill_tail(int start)
+    {
+        std::memset(&tail, 0, sizeof(tail));
+        const int nlev = (int)levels.size();
+        tail.levels = nlev - start;
+        tail.coarse_n = coarse_n;
+        tail.ginv = ginv.p;
+        tail.top_rhs = start == 0 ? r.p : levels[start].rhs.p;
+        tail.top_sol = levels[start].sol0.p;
+        for (int i = 0; i < tail.levels; ++i) {
+            Level& L = levels[start + i];
+            TailLevel& T = tail.lev[i];
+            T.w = L.w;
+            T.h = L.h;
+            T.s = L.s;
+            const bool last = start + i + 1 == nlev;
+            T.cw = last ? 0 : levels[start + i + 1].w;
+            T.ch = last ? 0 : levels[start + i + 1].h;
+            T.right = L.right32.p;
+            T.down = L.down32.p;
+            T.pxf = L.d_pxf.p; T.pxs = L.d_pxs.p; T.pxw = L.d_pxw.p;
+            T.pyf = L.d_pyf.p; T.pys = L.d_pys.p; T.pyw = L.d_pyw.p;
+            T.rxc = L.d_rxc.p; T.rxi = L.d_rxi.p; T.rxw = L.d_rxw.p;
+            T.ryc = L.d_ryc.p; T.ryi = L.d_ryi.p; T.ryw = L.d_ryw.p;
+        }
     }
 
     // PressureMultigrid constructor (PressureMultigrid.cpp:38-62): halve until w*h <= 64.
@@ -354,41 +381,29 @@ struct fs_grid {
         coarse_n = last.w * last.h - 1;
         bytes += factor.alloc((size_t)std::max(coarse_n, 1) * std::max(coarse_n, 1));
 
+        bytes += ginv.alloc((size_t)std::max(coarse_n, 1) * std::max(coarse_n, 1));
+
         // Coarse tail: the deepest levels whose LDS footprint fits one workgroup.
         const int nlev = (int)levels.size();
-        const size_t budget = 48 * 1024 / sizeof(float);
-        size_t floats = 6 * levels[nlev - 1].padded();
-        tail_start = nlev - 1;
-        for (int l = nlev - 2; l >= 0; --l) {
-            const size_t f = 6 * levels[l].padded();
-            if (floats + f > budget || nlev - l > kMaxTailLevels)
+        size_t limit = ctx->lds_per_block > 0 ? ctx->lds_per_block : 65536;
+        limit = std::min<size_t>(limit, 160 * 1024) - 4096;
+        for (;;) {
+            tail_start = nlev - 1;
+            for (int l = nlev - 2; l >= 0 && nlev - l <= kMaxTailLevels; --l) {
+                fill_tail(l);
+                size_t elements = 0;
+                for (int i = 0; i < tail.levels; ++i)
+                    elements += (size_t)tail.lev[i].s * (tail.lev[i].h + 2);
+                // the tail kernel stages at most 8 padded elements per thread (8192 in all)
+                if (tail_layout(tail) > limit || elements > 8 * 1024)
+                    break;
+                tail_start = l;
+            }
+            fill_tail(tail_start);
+            tail_lds = tail_layout(tail);
+            if (tail_lds <= 65536 || tail_allow_lds(tail_lds) || limit <= 65536)
                 break;
-            floats += f;
-            tail_start = l;
-        }
-        std::memset(&tail, 0, sizeof(tail));
-        tail.levels = nlev - tail_start;
-        tail.coarse_n = coarse_n;
-        tail.factor = factor.p;
-        tail.top_rhs = tail_start == 0 ? r.p : levels[tail_start].rhs.p;
-        tail.top_sol[0] = levels[tail_start].sol0.p;
-        tail.top_sol[1] = levels[tail_start].sol1.p;
-        int off = 0;
-        for (int i = 0; i < tail.levels; ++i) {
-            Level& L = levels[tail_start + i];
-            TailLevel& T = tail.lev[i];
-            T.w = L.w;
-            T.h = L.h;
-            T.s = L.s;
-            T.right = L.right32.p;
-            T.down = L.down32.p;
-            T.invd = L.invd32.p;
-            T.px_first = L.d_pxf.p; T.px_second = L.d_pxs.p; T.px_w = L.d_pxw.p;
-            T.py_first = L.d_pyf.p; T.py_second = L.d_pys.p; T.py_w = L.d_pyw.p;
-            T.rx_count = L.d_rxc.p; T.rx_index = L.d_rxi.p; T.rx_w = L.d_rxw.p;
-            T.ry_count = L.d_ryc.p; T.ry_index = L.d_ryi.p; T.ry_w = L.d_ryw.p;
-            tail.lds_offset[i] = off;
-            off += (int)(6 * L.padded());
+            limit = 65536 - 4096;  // the device refused a larger dynamic LDS request
         }
         bytes += tail_dev.alloc(1);
         HIPC(hipMemcpy(tail_dev.p, &tail, sizeof(TailDesc), hipMemcpyHostToDevice));
@@ -417,7 +432,7 @@ struct fs_grid {
             launch_level_diag(st(), Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p, Cl.invd32.p);
         }
         const Level& last = levels.back();
-        launch_cholesky(st(), last.w, last.h, last.right64.p, last.down64.p, factor.p, coarse_n, S.p);
+        launch_coarse_factor(st(), last.w, last.h, last.right64.p, last.down64.p, factor.p, ginv.p, coarse_n, S.p);
         mg_valid = true;
     }
 
@@ -425,35 +440,26 @@ struct fs_grid {
     void vcycle(const int* status)
     {
         const int T = tail_start;
+        const CoarseLink none{};
         for (int l = 0; l < T; ++l) {
             LevelView L = view(l);
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
-            launch_rb_sweep(st(), L, 0, rhs, shift, 0, 1, status);
-            launch_rb_sweep(st(), L, 1, rhs, shift, 0, 0, status);
-            launch_rb_sweep(st(), L, 0, rhs, shift, 0, 0, status);
-            launch_rb_sweep(st(), L, 1, rhs, shift, 0, 0, status);
-            launch_mg_residual(st(), L, 0, rhs, shift, status);
-            LevelView Cl = view(l + 1);
+            launch_rb_multi(st(), L, 0, rhs, shift, 0, 1, none, status, l == 0);
             const Level& F = levels[l];
             AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
             AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
-            launch_mg_restrict(st(), L, Cl, rx, ry, status);
+            launch_res_restrict(st(), L, 1, rhs, shift, view(l + 1), rx, ry, status);
         }
-        launch_mg_tail(st(), tail_dev.p, tail, 0, T == 0 ? rmean_ptr() : nullptr, status);
+        launch_mg_tail(st(), tail, tail_lds, T == 0 ? rmean_ptr() : nullptr, status);
         for (int l = T - 1; l >= 0; --l) {
             LevelView L = view(l);
-            LevelView Cl = view(l + 1);
             const Level& F = levels[l];
-            AxisProlong px{F.d_pxf.p, F.d_pxs.p, F.d_pxw.p};
-            AxisProlong py{F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
-            launch_mg_prolong(st(), L, 0, Cl, 0, px, py, status);
+            const Level& Cl = levels[l + 1];
+            CoarseLink link{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
-            launch_rb_sweep(st(), L, 0, rhs, shift, 1, 0, status);
-            launch_rb_sweep(st(), L, 1, rhs, shift, 1, 0, status);
-            launch_rb_sweep(st(), L, 0, rhs, shift, 1, 0, status);
-            launch_rb_sweep(st(), L, 1, rhs, shift, 1, 0, status);
+            launch_rb_multi(st(), L, 1, rhs, shift, 1, 0, link, status, l == 0);
         }
     }
 
@@ -616,6 +622,9 @@ fs_status fs_context_create(int device, fs_context** out)
         auto ctx = std::make_unique<fs_context>();
         ctx->device = device;
         HIPC(hipSetDevice(device));
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+            ctx->lds_per_block = (size_t)lds;
         HIPC(hipStreamCreate(&ctx->stream));
         HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
         HIPC(hipEventCreate(&ctx->ev0));
@@ -1247,21 +1256,48 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
     return guard([&] {
         need(grid != nullptr && ms_per_launch != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
         need(reps > 0, FS_E_INVALID_ARGUMENT, "reps must be positive");
-        need(kernel >= 0 && kernel <= 3, FS_E_INVALID_ARGUMENT, "unknown kernel id");
+        need(kernel >= 0 && kernel <= 9, FS_E_INVALID_ARGUMENT, "unknown kernel id");
         fs_grid& g = *grid;
         need(g.uploaded, FS_E_LOGIC, "Upload the grid state first");
         set_device(g.ctx);
         if (!g.coeff_valid)
             g.rebuild_coefficients();
         launch_init_scalars(g.st(), g.S.p, 0.0, 1 << 30);
+        if (kernel >= 4 && !g.mg_valid)
+            g.rebuild_hierarchy();
         LevelView L0 = g.view(0);
         GridDev gd = g.dev();
+        const int nlev = (int)g.levels.size();
+        const int l1 = std::min(1, g.tail_start);
         auto launch_once = [&](int i) {
             switch (kernel) {
-            case 0: launch_rb_sweep(g.st(), L0, i & 1, g.r.p, nullptr, 0, 0, nullptr); break;
+            case 0: launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 0, 0, CoarseLink{}, nullptr, 1); break;
             case 1: launch_pcg_ad(g.st(), gd); break;
             case 2: launch_pcg_update(g.st(), gd); break;
-            default: launch_copy_f32(g.st(), g.r.p, g.q.p, g.NP()); break;
+            case 3: launch_copy_f32(g.st(), g.r.p, g.q.p, g.NP()); break;
+            case 4:  // second level's smoother (or the finest, if the tail starts at level 1)
+                launch_rb_multi(g.st(), g.view(l1), i & 1, l1 == 0 ? g.r.p : g.view(l1).rhs,
+                    l1 == 0 ? g.rmean_ptr() : nullptr, 0, 0, CoarseLink{}, nullptr, l1 == 0);
+                break;
+            case 5: {
+                if (g.tail_start == 0 || nlev < 2)
+                    break;
+                const Level& F = g.levels[0];
+                AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
+                AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+                launch_res_restrict(g.st(), L0, 1, g.r.p, g.rmean_ptr(), g.view(1), rx, ry, nullptr);
+                break;
+            }
+            case 6: launch_mg_tail(g.st(), g.tail, g.tail_lds, g.tail_start == 0 ? g.rmean_ptr() : nullptr,
+                        nullptr); break;
+            case 7: {
+                const Level& last = g.levels.back();
+                launch_coarse_factor(g.st(), last.w, last.h, last.right64.p, last.down64.p, g.factor.p, g.ginv.p,
+                    g.coarse_n, nullptr);
+                break;
+            }
+            case 8: launch_pcg_direction(g.st(), gd, g.levels[0].sol0.p); break;
+            default: launch_pcg_rz(g.st(), gd, g.levels[0].sol0.p); break;
             }
         };
         launch_once(0);

@@ -30,37 +30,49 @@ struct GridDev {
     float* zdiag;             // padded FP32 diagonal-preconditioned residual
     PcgScalars* S;            // device scalar block
     double* partials;         // reduction partials (>= 4 * max blocks)
-    unsigned* counter;        // last-block arrival counter (kept at zero between kernels)
 };
 
 // Status values beyond PcgStatus used around the loop.
 enum : int { kZeroRhs = 3, kInitialOk = 4, kFailed = 5 };
 
-// Tail of the hierarchy solved by one workgroup in LDS.
+// Prolongation source of the fused post-smoothing kernel (csol == nullptr: no prolongation).
+struct CoarseLink {
+    const float* csol;   // coarse solution (padded)
+    int cs;              // coarse padded stride
+    const int* pxf; const int* pxs; const float* pxw;
+    const int* pyf; const int* pys; const float* pyw;
+};
+
+// Tail of the hierarchy solved by one workgroup entirely in LDS.
 struct TailLevel {
     int w, h, s;
-    const float* right;
+    int cw, ch;          // next coarser tail level (0 on the last)
+    const float* right;  // global padded conductances
     const float* down;
-    const float* invd;
-    // transfers to the next coarser tail level (unused on the last level)
-    const int* px_first; const int* px_second; const float* px_w;
-    const int* py_first; const int* py_second; const float* py_w;
-    const int* rx_count; const int* rx_index; const float* rx_w;
-    const int* ry_count; const int* ry_index; const float* ry_w;
+    const int* pxf; const int* pxs; const float* pxw;   // transfers to the next tail level
+    const int* pyf; const int* pys; const float* pyw;
+    const int* rxc; const int* rxi; const float* rxw;
+    const int* ryc; const int* ryi; const float* ryw;
+    int lds_arrays;      // word offset of right, down, rhs, sol, res (each s * (h + 2))
+    int lds_tables;      // word offset of the transfer tables (3w + 3h + 9cw + 9ch words)
 };
 
 constexpr int kMaxTailLevels = 8;
 struct TailDesc {
     int levels;
-    int coarse_n;                 // unknowns of the dense coarse solve (w*h - 1 of the last level)
-    const double* factor;         // coarse Cholesky factor (n*n)
-    const float* top_rhs;         // global rhs of the first tail level (the FP32 residual on level 0)
-    float* top_sol[2];            // global ping-pong solution of the first tail level
-    int lds_offset[kMaxTailLevels];  // float offset of each level's six LDS arrays
+    int coarse_n;              // unknowns of the pinned coarse system (w*h - 1 of the last level)
+    const float* ginv;         // inverse of the pinned coarse matrix, FP32 n*n
+    const float* top_rhs;      // global rhs of the first tail level (the FP32 residual on level 0)
+    float* top_sol;            // global solution of the first tail level
+    int lds_ginv;              // word offset of ginv in LDS
+    int lds_words;             // total LDS words
     TailLevel lev[kMaxTailLevels];
 };
 
-size_t tail_lds_bytes(const TailDesc& t);
+// Fills the LDS offsets of `t` and returns its LDS bytes.
+size_t tail_layout(TailDesc& t);
+// Raises the dynamic-LDS limit of the tail kernel; false if the device refuses.
+bool tail_allow_lds(size_t bytes);
 
 void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations);
 
@@ -72,8 +84,9 @@ void launch_level_diag(hipStream_t st, int w, int h, const double* right64, cons
 void launch_coarsen(hipStream_t st, int fw, int fh, const double* fcx, const double* fcy,
     const double* fright, const double* fdown, int cw, int ch, const double* ccx, const double* ccy,
     double* cright, double* cdown);
-void launch_cholesky(hipStream_t st, int w, int h, const double* right64, const double* down64, double* factor,
-    int n, PcgScalars* S);
+// Bit-exact Cholesky of the pinned coarse matrix (factorCoarsest) plus its FP32 inverse.
+void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, const double* down64,
+    double* factor, float* ginv, int n, PcgScalars* S);
 
 // ---- projection set-up and finish (FP64) ----
 void launch_rhs(hipStream_t st, const GridDev& g, double scale);
@@ -101,16 +114,14 @@ void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z);
 void launch_precond_diag(hipStream_t st, const GridDev& g);
 
 // ---- multigrid V-cycle pieces (FP32) ----
-void launch_rb_sweep(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
-    int first_color, int from_zero, const int* status);
-void launch_mg_residual(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
-    const int* status);
-void launch_mg_restrict(hipStream_t st, const LevelView& F, const LevelView& Cl, AxisRestrict rx, AxisRestrict ry,
-    const int* status);
-void launch_mg_prolong(hipStream_t st, const LevelView& F, int fcur, const LevelView& Cl, int ccur, AxisProlong px,
-    AxisProlong py, const int* status);
-void launch_mg_tail(hipStream_t st, const TailDesc* dev_desc, const TailDesc& host_desc, int cur,
-    const double* shift, const int* status);
+// Four red-black sweeps in one launch (reads sol[cur], writes sol[cur ^ 1]); with link.csol set the
+// coarse correction is prolonged and added first.  fine selects the finest-level symbol.
+void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
+    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine);
+// Residual of sol[cur] and its restriction into C.rhs in one launch.
+void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
+    const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status);
+void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status);
 
 // ---- bench helper: one operator application without reductions (for event timing) ----
 void launch_copy_f32(hipStream_t st, const float* src, float* dst, size_t n);

@@ -1,0 +1,736 @@
+// fs_multigrid.hip -- the multigrid preconditioner of PressureMultigrid.cpp on gfx950.
+//
+// One V(4,4) cycle per level is three launches:
+//   k_rb_multi      four red-black Gauss-Seidel sweeps on a (64+16) x (32+16) LDS tile (halo 8: each
+//                   half-sweep shrinks the valid region by one cell), from zero on the way down;
+//   k_res_restrict  residual of the smoothed solution and its transpose-of-bilinear restriction;
+//   k_rb_multi      prolongation of the coarse correction fused into the load, then four reversed sweeps.
+// The deepest levels (up to 64 x 64, as LDS allows) and the coarse solve run in k_mg_tail, one
+// 1024-thread workgroup with every array in LDS.  The pinned coarse system (PressureMultigrid.cpp:
+// 300-384) is factored bit-exactly by k_coarse_factor, which also forms its inverse so the per-cycle
+// coarse solve is a 63 x 63 matrix-vector product instead of two serial substitutions.
+//
+// Smoothers divide by the sum of the four face conductances formed in FP32 (one rcp); pre- and
+// post-smoothing use the same diagonal, so the V-cycle stays symmetric.
+#include "fs_kernels.hpp"
+
+#include <math.h>
+
+namespace fsim {
+
+namespace {
+
+constexpr int kNT = 256;
+
+__device__ __forceinline__ float inv_or_zero(float d) { return d > 0.f ? __builtin_amdgcn_rcpf(d) : 0.f; }
+
+// ---------------------------------------------------------------------------------------------
+// Four-sweep red-black Gauss-Seidel with optional fused prolongation.
+// ---------------------------------------------------------------------------------------------
+// Layout: the tile's solution lives in two colour arrays (red, black), each LX/2 wide, so a wavefront
+// updating 64 same-colour cells of a row touches 64 consecutive LDS words (no bank conflicts).  Each
+// thread owns fixed (row, pair) slots; the four conductances, rhs and 1/diagonal of its cells stay in
+// registers for all eight half-sweeps, so an update reads four neighbours from LDS and writes one value.
+template <int TX, int TY, int NS, int FINE>
+__global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift,
+    int c0, int from_zero, CoarseLink link, const int* status)
+{
+    constexpr int HALO = 2 * NS;
+    constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
+    constexpr int HX = LX / 2;                       // colour-array row width
+    constexpr int NE = LX * LY;
+    constexpr int SLOTS = HX * (LY - 2);             // rows 1 .. LY-2, every pair
+    constexpr int ROUNDS = (SLOTS + kNT - 1) / kNT;
+    static_assert(LX % 2 == 0 && (TX % 2) == 0 && (HALO % 2) == 0, "tile must keep colour parity");
+    __shared__ float s_col[2][LY * HX];
+    const int w = L.w, h = L.h, s = L.s;
+    const int x0 = blockIdx.x * TX - HALO, y0 = blockIdx.y * TY - HALO;  // x0 + y0 is even
+    const float* sol_in = L.sol[cur];
+    // Every global load below goes through a clamped (valid) interior index and is masked afterwards,
+    // so a thread issues all its loads before its first wait.
+    float cl[2][ROUNDS], cr[2][ROUNDS], cu[2][ROUNDS], cd[2][ROUNDS], cb[2][ROUNDS];
+    int cly[ROUNDS], cj[ROUNDS];
+    bool cin[2][ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int t = threadIdx.x + r * kNT;
+        const int ly = 1 + t / HX, jj = t - (t / HX) * HX;
+        cly[r] = t < SLOTS ? ly : -1;
+        cj[r] = jj;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int lx = 2 * jj + ((c + ly) & 1);
+            const int x = x0 + lx, y = y0 + ly;
+            cin[c][r] = t < SLOTS && x >= 0 && x < w && y >= 0 && y < h;
+            const int P = (min(max(y, 0), h - 1) + 1) * s + (min(max(x, 0), w - 1) + 1);
+            cl[c][r] = L.right[P - 1];
+            cr[c][r] = L.right[P];
+            cu[c][r] = L.down[P - s];
+            cd[c][r] = L.down[P];
+            cb[c][r] = rhs[P];
+        }
+    }
+    constexpr int LOADS = (NE + kNT - 1) / kNT;
+    float vs[LOADS];
+    bool vin[LOADS];
+#pragma unroll
+    for (int it = 0; it < LOADS; ++it) {
+        const int e = threadIdx.x + it * kNT;
+        const int ly = e / LX, lx = e - ly * LX;
+        const int x = x0 + lx, y = y0 + ly;
+        vin[it] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
+        const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
+        float v = from_zero ? 0.f : sol_in[(yc + 1) * s + (xc + 1)];
+        if (link.csol) {
+            const float wx = link.pxw[xc], wy = link.pyw[yc];
+            const int a0 = link.pxf[xc], a1 = link.pxs[xc];
+            const float* top = link.csol + (link.pyf[yc] + 1) * link.cs + 1;
+            const float* bot = link.csol + (link.pys[yc] + 1) * link.cs + 1;
+            const float a = (1.f - wx) * top[a0] + wx * top[a1];
+            const float b = (1.f - wx) * bot[a0] + wx * bot[a1];
+            v += (1.f - wy) * a + wy * b;
+        }
+        vs[it] = v;
+    }
+    if (status && *status != kRunning)
+        return;
+    const double sh = shift ? *shift : 0.0;
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (!cin[c][r]) {
+                cl[c][r] = cr[c][r] = cu[c][r] = cd[c][r] = cb[c][r] = 0.f;
+            } else if (shift) {
+                cb[c][r] = (float)((double)cb[c][r] - sh);
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < LOADS; ++it) {
+        const int e = threadIdx.x + it * kNT;
+        if (e < NE) {
+            const int ly = e / LX, lx = e - ly * LX;
+            s_col[(lx + ly) & 1][ly * HX + (lx >> 1)] = vin[it] ? vs[it] : 0.f;
+        }
+    }
+    __syncthreads();
+    // ---- eight half-sweeps, each region one cell smaller than the last ----
+    // A half-sweep reads only the other colour, so all neighbour values are gathered first and the
+    // updates written afterwards: one LDS latency per half-sweep instead of one per round.
+#pragma unroll 1
+    for (int k = 0; k < 2 * NS; ++k) {
+        const int color = c0 ^ (k & 1);
+        const int lo = k + 1;
+        const bool neighbours = !(from_zero && k == 0);
+        float* mine = s_col[color];
+        const float* other = s_col[color ^ 1];
+        float n0[ROUNDS], n1[ROUNDS], n2[ROUNDS], n3[ROUNDS];
+        bool act[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int ly = cly[r];
+            const int p = (color + ly) & 1;
+            const int jj = cj[r];
+            const int lx = 2 * jj + p;
+            const int x = x0 + lx, y = y0 + ly;
+            act[r] = ly >= lo && ly < LY - lo && lx >= lo && lx < LX - lo && x >= 0 && x < w && y >= 0 && y < h;
+            n0[r] = n1[r] = n2[r] = n3[r] = 0.f;
+            if (act[r] && neighbours) {
+                const int row = ly * HX;
+                n0[r] = other[row + jj - 1 + p];
+                n1[r] = other[row + jj + p];
+                n2[r] = other[row - HX + jj];
+                n3[r] = other[row + HX + jj];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            if (!act[r])
+                continue;
+            const float a = cl[color][r], b = cr[color][r], u = cu[color][r], d = cd[color][r];
+            float v = cb[color][r];
+            if (neighbours)
+                v += (a * n0[r] + b * n1[r]) + (u * n2[r] + d * n3[r]);
+            mine[cly[r] * HX + cj[r]] = v * inv_or_zero((a + b) + (u + d));
+        }
+        __syncthreads();
+    }
+    float* sol_out = L.sol[cur ^ 1];
+    for (int e = threadIdx.x; e < TX * TY; e += kNT) {
+        const int ly = HALO + e / TX, lx = HALO + e % TX;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x < w && y < h)
+            sol_out[(y + 1) * s + (x + 1)] = s_col[(lx + ly) & 1][ly * HX + (lx >> 1)];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Residual + restriction (restrictResidual, PressureMultigrid.cpp:233-276).  A coarse tile of
+// CTX x CTY cells gathers fine taps in [2c - 2, 2c + 3]; the fine residual is formed in LDS on
+// that window (plus a one-cell ring of solution values).
+// ---------------------------------------------------------------------------------------------
+template <int CTX, int CTY>
+__global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, const float* rhs, const double* shift,
+    LevelView Cl, AxisRestrict rx, AxisRestrict ry, const int* status)
+{
+    constexpr int LX = 2 * CTX + 6, LY = 2 * CTY + 6;
+    __shared__ float s_sol[LY * LX];
+    __shared__ float s_right[LY * LX];
+    __shared__ float s_down[LY * LX];
+    __shared__ float s_res[LY * LX];
+    __shared__ int s_rxc[CTX], s_rxi[CTX * 4], s_ryc[CTY], s_ryi[CTY * 4];
+    __shared__ float s_rxw[CTX * 4], s_ryw[CTY * 4];
+    const int w = F.w, h = F.h, s = F.s;
+    const int cx0 = blockIdx.x * CTX, cy0 = blockIdx.y * CTY;
+    const int fx0 = 2 * cx0 - 3, fy0 = 2 * cy0 - 3;
+    const float* sol = F.sol[cur];
+    constexpr int NE = LX * LY;
+    constexpr int LOADS = (NE + kNT - 1) / kNT;
+    float vs[LOADS], vr[LOADS], vd[LOADS], vb[LOADS];
+    bool vin[LOADS];
+#pragma unroll
+    for (int it = 0; it < LOADS; ++it) {
+        const int e = threadIdx.x + it * kNT;
+        const int ly = e / LX, lx = e - ly * LX;
+        const int x = fx0 + lx, y = fy0 + ly;
+        vin[it] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
+        const int P = (min(max(y, 0), h - 1) + 1) * s + (min(max(x, 0), w - 1) + 1);
+        vs[it] = sol[P];
+        vr[it] = F.right[P];
+        vd[it] = F.down[P];
+        vb[it] = rhs[P];
+    }
+    // restriction tables of this coarse tile (missing taps: weight 0 at a valid LDS cell)
+    int tc[2], ti[2][4];
+    float tw[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int e = threadIdx.x - a * CTX;  // threads [0, CTX) take x, [CTX, CTX + CTY) take y
+        const bool isx = a == 0;
+        const int n_axis = isx ? CTX : CTY;
+        const int c = (isx ? cx0 : cy0) + min(max(e, 0), n_axis - 1);
+        const int limit = isx ? Cl.w : Cl.h;
+        const AxisRestrict& R = isx ? rx : ry;
+        const int cc = min(c, limit - 1);
+        const int n = R.count[cc];
+        tc[a] = c < limit ? n : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ti[a][k] = R.index[cc * 4 + k];
+            tw[a][k] = R.weight[cc * 4 + k];
+        }
+    }
+    if (status && *status != kRunning)
+        return;
+    const double sh = shift ? *shift : 0.0;
+#pragma unroll
+    for (int it = 0; it < LOADS; ++it) {
+        const int e = threadIdx.x + it * kNT;
+        if (e < NE) {
+            s_sol[e] = vin[it] ? vs[it] : 0.f;
+            s_right[e] = vin[it] ? vr[it] : 0.f;
+            s_down[e] = vin[it] ? vd[it] : 0.f;
+            s_res[e] = vin[it] ? (shift ? (float)((double)vb[it] - sh) : vb[it]) : 0.f;
+        }
+    }
+    if (threadIdx.x < CTX) {
+        const int e = threadIdx.x;
+        s_rxc[e] = tc[0];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = k < tc[0];
+            s_rxi[e * 4 + k] = on ? ti[0][k] - fx0 : 1;
+            s_rxw[e * 4 + k] = on ? tw[0][k] : 0.f;
+        }
+    } else if (threadIdx.x < CTX + CTY) {
+        const int e = threadIdx.x - CTX;
+        s_ryc[e] = tc[1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = k < tc[1];
+            s_ryi[e * 4 + k] = on ? ti[1][k] - fy0 : 1;
+            s_ryw[e * 4 + k] = on ? tw[1][k] : 0.f;
+        }
+    }
+    __syncthreads();
+    // residual in place of the rhs on the inner window
+    for (int e = threadIdx.x; e < (LX - 2) * (LY - 2); e += kNT) {
+        const int ly = 1 + e / (LX - 2), lx = 1 + e % (LX - 2);
+        const int x = fx0 + lx, y = fy0 + ly;
+        if (x < 0 || x >= w || y < 0 || y >= h)
+            continue;
+        const int i = ly * LX + lx;
+        const float c = s_sol[i];
+        const float product = (s_right[i - 1] * (c - s_sol[i - 1]) + s_right[i] * (c - s_sol[i + 1])) +
+                              (s_down[i - LX] * (c - s_sol[i - LX]) + s_down[i] * (c - s_sol[i + LX]));
+        s_res[i] = s_res[i] - product;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < CTX * CTY; e += kNT) {
+        const int ly = e / CTX, lx = e - ly * CTX;
+        const int x = cx0 + lx, y = cy0 + ly;
+        if (x >= Cl.w || y >= Cl.h)
+            continue;
+        // four taps per axis (missing taps carry weight 0 and a valid index): 16 independent reads
+        float tap[4][4];
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            const float* row = s_res + s_ryi[ly * 4 + iy] * LX;
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+                tap[iy][ix] = row[s_rxi[lx * 4 + ix]];
+        }
+        float value = 0.f;
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            float horizontal = 0.f;
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+                horizontal += s_rxw[lx * 4 + ix] * tap[iy][ix];
+            value += s_ryw[ly * 4 + iy] * horizontal;
+        }
+        Cl.rhs[(y + 1) * Cl.s + x + 1] = value;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coarse factorisation (factorCoarsest, PressureMultigrid.cpp:300-345) and inverse.
+// Right-looking updates applied column by column: every entry receives its subtractions in the
+// same k order as the reference's row loop, so the factor is bit-identical to it.
+// ---------------------------------------------------------------------------------------------
+constexpr int kMaxCoarse = 63;
+constexpr int kFactorThreads = 256;
+
+__global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, const double* right,
+    const double* down, double* factor, float* ginv, int n, PcgScalars* S)
+{
+    __shared__ double a[kMaxCoarse * kMaxCoarse];   // Cholesky working matrix (lower triangle)
+    __shared__ double m[kMaxCoarse * kMaxCoarse];   // Gauss-Jordan working matrix
+    __shared__ double q[kMaxCoarse];
+    __shared__ int failed;
+    const int s = w + 2;
+    const int tid = threadIdx.x;
+    const int tc = tid & 63, tr = tid >> 6;
+    constexpr int RSTEP = kFactorThreads / 64;
+    for (int e = tid; e < n * n; e += kFactorThreads) {
+        a[e] = 0.0;
+        m[e] = 0.0;
+    }
+    if (tid == 0)
+        failed = 0;
+    __syncthreads();
+    for (int cell = 1 + tid; cell <= n; cell += kFactorThreads) {
+        const int x = cell % w, y = cell / w;
+        const int i = (y + 1) * s + x + 1;
+        const int row = cell - 1;
+        const double diag = right[i - 1] + right[i] + down[i - s] + down[i];
+        a[row * n + row] = diag;
+        m[row * n + row] = diag;
+        if (x > 0 && cell - 1 > 0) {
+            a[row * n + row - 1] = -right[i - 1];
+            m[row * n + row - 1] = -right[i - 1];
+            m[(row - 1) * n + row] = -right[i - 1];
+        }
+        if (y > 0 && cell - w > 0) {
+            a[row * n + row - w] = -down[i - s];
+            m[row * n + row - w] = -down[i - s];
+            m[(row - w) * n + row] = -down[i - s];
+        }
+    }
+    __syncthreads();
+    // Column k: pivot sqrt, quotients L[i][k] = a[i][k] / pivot into q (one barrier), then the trailing
+    // update a[i][c] -= L[i][k] L[c][k] (second barrier).  Each entry still receives its subtractions
+    // in increasing k, as the reference's row loop applies them.
+    for (int k = 0; k < n; ++k) {
+        const double v = a[k * n + k];
+        if (v <= 0.0 || !isfinite(v)) {
+            if (tid == 0)
+                failed = 1;
+            break;
+        }
+        const double pivot = sqrt(v);
+        if (tid > k && tid < n) {
+            const double l = a[tid * n + k] / pivot;
+            a[tid * n + k] = l;
+            q[tid] = l;
+        }
+        __syncthreads();
+        if (tid == 0)
+            a[k * n + k] = pivot;  // after the barrier: every thread has read the unfactored diagonal
+        {
+            // rows i = k+1+tr+4m; gather every operand first, then write (independent LDS accesses)
+            const int c = k + 1 + tc;
+            const double qc = c < n ? q[c] : 0.0;
+            constexpr int M = (kMaxCoarse + RSTEP - 1) / RSTEP;
+            double li[M], av[M];
+#pragma unroll
+            for (int mm = 0; mm < M; ++mm) {
+                const int i = k + 1 + tr + RSTEP * mm;
+                const bool on = i < n && c <= i;
+                li[mm] = on ? q[i] : 0.0;
+                av[mm] = on ? a[i * n + c] : 0.0;
+            }
+#pragma unroll
+            for (int mm = 0; mm < M; ++mm) {
+                const int i = k + 1 + tr + RSTEP * mm;
+                if (i < n && c <= i)
+                    a[i * n + c] = av[mm] - li[mm] * qc;
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (failed) {
+        if (tid == 0 && S) {
+            S->mg_failed = 1;
+            if (S->status == kRunning)
+                S->status = kFailed;
+        }
+        return;
+    }
+    for (int e = tid; e < n * n; e += kFactorThreads) {
+        const int r = e / n, c = e - r * n;
+        factor[e] = c <= r ? a[e] : 0.0;
+    }
+    // In-place Gauss-Jordan inversion of the (SPD, so pivot-free) pinned matrix.
+    for (int k = 0; k < n; ++k) {
+        const double p = m[k * n + k];
+        __syncthreads();  // every thread holds the pivot before row k is rewritten
+        if (tid < n) {
+            q[tid] = m[tid * n + k];
+            m[k * n + tid] = tid == k ? 1.0 / p : m[k * n + tid] / p;
+        }
+        __syncthreads();
+        {
+            constexpr int M = (kMaxCoarse + RSTEP - 1) / RSTEP;
+            const double pk = tc < n ? m[k * n + tc] : 0.0;
+            double fv[M], mv[M];
+#pragma unroll
+            for (int mm = 0; mm < M; ++mm) {
+                const int i = tr + RSTEP * mm;
+                const bool on = i < n && i != k && tc < n;
+                fv[mm] = on ? q[i] : 0.0;
+                mv[mm] = on ? m[i * n + tc] : 0.0;
+            }
+#pragma unroll
+            for (int mm = 0; mm < M; ++mm) {
+                const int i = tr + RSTEP * mm;
+                if (i < n && i != k && tc < n)
+                    m[i * n + tc] = tc == k ? -fv[mm] / p : mv[mm] - fv[mm] * pk;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < n * n; e += kFactorThreads)
+        ginv[e] = (float)m[e];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coarse tail: the deepest levels and the coarse solve in one workgroup.
+// ---------------------------------------------------------------------------------------------
+constexpr int kTailThreads = 1024;
+
+struct TailArrays {
+    float* right;
+    float* down;
+    float* rhs;
+    float* sol;
+    float* res;
+    int* pxf; int* pxs; float* pxw;
+    int* pyf; int* pys; float* pyw;
+    int* rxc; int* rxi; float* rxw;
+    int* ryc; int* ryi; float* ryw;
+};
+
+__device__ __forceinline__ TailArrays tail_arrays(float* lds, const TailLevel& T)
+{
+    TailArrays A;
+    const int ps = T.s * (T.h + 2);
+    float* p = lds + T.lds_arrays;
+    A.right = p;
+    A.down = p + ps;
+    A.rhs = p + 2 * ps;
+    A.sol = p + 3 * ps;
+    A.res = p + 4 * ps;
+    float* t = lds + T.lds_tables;
+    A.pxf = (int*)t; t += T.w;
+    A.pxs = (int*)t; t += T.w;
+    A.pxw = t; t += T.w;
+    A.pyf = (int*)t; t += T.h;
+    A.pys = (int*)t; t += T.h;
+    A.pyw = t; t += T.h;
+    A.rxc = (int*)t; t += T.cw;
+    A.rxi = (int*)t; t += 4 * T.cw;
+    A.rxw = t; t += 4 * T.cw;
+    A.ryc = (int*)t; t += T.ch;
+    A.ryi = (int*)t; t += 4 * T.ch;
+    A.ryw = t;
+    return A;
+}
+
+__device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
+{
+    const int w = T.w, h = T.h, s = T.s;
+    const int pairs = (w + 1) >> 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int color = reverse ? 1 - pass : pass;
+        const bool neighbours = !(from_zero && pass == 0);
+        for (int t = threadIdx.x; t < pairs * h; t += kTailThreads) {
+            const int y = t / pairs;
+            const int x = 2 * (t - y * pairs) + ((color ^ y) & 1);
+            if (x >= w)
+                continue;
+            const int i = (y + 1) * s + x + 1;
+            const float rl = A.right[i - 1], rr = A.right[i], du = A.down[i - s], dd = A.down[i];
+            float v = A.rhs[i];
+            if (neighbours)
+                v += (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
+            A.sol[i] = v * inv_or_zero((rl + rr) + (du + dd));
+        }
+        __syncthreads();
+    }
+}
+
+__device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, const TailLevel& Cn,
+    const TailArrays& Cd)
+{
+    const int w = T.w, h = T.h, s = T.s;
+    for (int e = threadIdx.x; e < w * h; e += kTailThreads) {
+        const int y = e / w, x = e - y * w;
+        const int i = (y + 1) * s + x + 1;
+        const float c = F.sol[i];
+        const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
+                              (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
+        F.res[i] = F.rhs[i] - product;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < Cn.w * Cn.h; e += kTailThreads) {
+        const int y = e / Cn.w, x = e - y * Cn.w;
+        const int nx = F.rxc[x], ny = F.ryc[y];
+        float value = 0.f;
+        for (int iy = 0; iy < ny; ++iy) {
+            const float* row = F.res + (F.ryi[y * 4 + iy] + 1) * s + 1;
+            float horizontal = 0.f;
+            for (int ix = 0; ix < nx; ++ix)
+                horizontal += F.rxw[x * 4 + ix] * row[F.rxi[x * 4 + ix]];
+            value += F.ryw[y * 4 + iy] * horizontal;
+        }
+        Cd.rhs[(y + 1) * Cn.s + x + 1] = value;
+    }
+    __syncthreads();
+}
+
+__device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const TailLevel& Cn, const TailArrays& Cd)
+{
+    const int w = T.w, h = T.h;
+    for (int e = threadIdx.x; e < w * h; e += kTailThreads) {
+        const int y = e / w, x = e - y * w;
+        const float wy = F.pyw[y];
+        const float* top = Cd.sol + (F.pyf[y] + 1) * Cn.s + 1;
+        const float* bottom = Cd.sol + (F.pys[y] + 1) * Cn.s + 1;
+        const int a0 = F.pxf[x], a1 = F.pxs[x];
+        const float wx = F.pxw[x];
+        const float a = (1.f - wx) * top[a0] + wx * top[a1];
+        const float b = (1.f - wx) * bottom[a0] + wx * bottom[a1];
+        F.sol[(y + 1) * T.s + x + 1] += (1.f - wy) * a + wy * b;
+    }
+    __syncthreads();
+}
+
+// solveCoarsest (PressureMultigrid.cpp:347-384): remove the rhs mean, solve the pinned system
+// (here with its precomputed inverse), shift to a mean-zero gauge.  Wavefront 0, one lane per row.
+__device__ void tail_coarse_solve(const TailLevel& T, const TailArrays& A, const float* G, int n)
+{
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int w = T.w, s = T.s;
+        const int cells = n + 1;
+        float b = 0.f;
+        if (lane < cells)
+            b = A.rhs[(lane / w + 1) * s + lane % w + 1];
+        const double mean = wave_sum((double)b) / cells;
+        // y_j = rhs(cell j + 1) - mean, held by lane j + 1
+        const float y = (float)((double)b - mean);
+        float c = 0.f;
+        for (int j = 0; j < n; ++j) {
+            const float yj = __shfl(y, j + 1, 64);
+            if (lane < n)
+                c += G[lane * n + j] * yj;
+        }
+        const double gauge = wave_sum(lane < n ? (double)c : 0.0) / cells;
+        if (lane == 0)
+            A.sol[s + 1] = (float)(-gauge);
+        if (lane < n) {
+            const int cell = lane + 1;
+            A.sol[(cell / w + 1) * s + cell % w + 1] = (float)((double)c - gauge);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, const double* shift, const int* status)
+{
+    extern __shared__ float lds[];
+    const int nl = D.levels;
+    // Flat index over every level's padded arrays; all loads issue before the LDS stores.
+    constexpr int TL = 8;
+    int total = 0;
+    for (int l = 0; l < nl; ++l)
+        total += D.lev[l].s * (D.lev[l].h + 2);
+    float vr[TL], vd[TL], vb[TL];
+    int lv[TL], li[TL];
+    bool bin[TL];
+#pragma unroll
+    for (int it = 0; it < TL; ++it) {
+        const int e0 = threadIdx.x + it * kTailThreads;
+        int e = min(e0, total - 1);
+        int l = 0;
+        while (e >= D.lev[l].s * (D.lev[l].h + 2)) {
+            e -= D.lev[l].s * (D.lev[l].h + 2);
+            ++l;
+        }
+        const TailLevel& T = D.lev[l];
+        lv[it] = e0 < total ? l : -1;
+        li[it] = e;
+        vr[it] = T.right[e];
+        vd[it] = T.down[e];
+        const int y = e / T.s - 1, x = e % T.s - 1;
+        const bool interior0 = l == 0 && x >= 0 && x < T.w && y >= 0 && y < T.h;
+        const float b = D.top_rhs[min(e, D.lev[0].s * (D.lev[0].h + 2) - 1)];
+        vb[it] = interior0 ? b : 0.f;
+        bin[it] = interior0;
+    }
+    if (status && *status != kRunning)
+        return;
+    const double sh = shift ? *shift : 0.0;
+#pragma unroll
+    for (int it = 0; it < TL; ++it) {
+        if (lv[it] < 0)
+            continue;
+        const TailArrays A = tail_arrays(lds, D.lev[lv[it]]);
+        const int e = li[it];
+        A.right[e] = vr[it];
+        A.down[e] = vd[it];
+        A.rhs[e] = (shift && bin[it]) ? (float)((double)vb[it] - sh) : vb[it];
+        A.sol[e] = 0.f;
+        A.res[e] = 0.f;
+    }
+    for (int l = 0; l + 1 < nl; ++l) {
+        const TailLevel& T = D.lev[l];
+        const TailArrays A = tail_arrays(lds, T);
+        for (int e = threadIdx.x; e < T.w; e += kTailThreads) {
+            A.pxf[e] = T.pxf[e];
+            A.pxs[e] = T.pxs[e];
+            A.pxw[e] = T.pxw[e];
+        }
+        for (int e = threadIdx.x; e < T.h; e += kTailThreads) {
+            A.pyf[e] = T.pyf[e];
+            A.pys[e] = T.pys[e];
+            A.pyw[e] = T.pyw[e];
+        }
+        for (int e = threadIdx.x; e < T.cw; e += kTailThreads)
+            A.rxc[e] = T.rxc[e];
+        for (int e = threadIdx.x; e < 4 * T.cw; e += kTailThreads) {
+            A.rxi[e] = T.rxi[e];
+            A.rxw[e] = T.rxw[e];
+        }
+        for (int e = threadIdx.x; e < T.ch; e += kTailThreads)
+            A.ryc[e] = T.ryc[e];
+        for (int e = threadIdx.x; e < 4 * T.ch; e += kTailThreads) {
+            A.ryi[e] = T.ryi[e];
+            A.ryw[e] = T.ryw[e];
+        }
+    }
+    float* G = lds + D.lds_ginv;
+    const int n = D.coarse_n;
+    for (int e = threadIdx.x; e < n * n; e += kTailThreads)
+        G[e] = D.ginv[e];
+    __syncthreads();
+    for (int l = 0; l + 1 < nl; ++l) {
+        const TailLevel& T = D.lev[l];
+        const TailArrays A = tail_arrays(lds, T);
+        tail_smooth(T, A, 0, 1);
+        for (int k = 1; k < 4; ++k)
+            tail_smooth(T, A, 0, 0);
+        tail_residual_restrict(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+    }
+    tail_coarse_solve(D.lev[nl - 1], tail_arrays(lds, D.lev[nl - 1]), G, n);
+    for (int l = nl - 2; l >= 0; --l) {
+        const TailLevel& T = D.lev[l];
+        const TailArrays A = tail_arrays(lds, T);
+        tail_prolong(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+        for (int k = 0; k < 4; ++k)
+            tail_smooth(T, A, 1, 0);
+    }
+    const TailLevel& T0 = D.lev[0];
+    const TailArrays A0 = tail_arrays(lds, T0);
+    for (int e = threadIdx.x; e < T0.w * T0.h; e += kTailThreads) {
+        const int y = e / T0.w, x = e - y * T0.w;
+        const int i = (y + 1) * T0.s + x + 1;
+        D.top_sol[i] = A0.sol[i];
+    }
+}
+
+constexpr int kMultiTX = 64, kMultiTY = 32, kMultiNS = 4;
+constexpr int kRestrictCX = 32, kRestrictCY = 16;
+
+}  // namespace
+
+size_t tail_layout(TailDesc& t)
+{
+    int words = 0;
+    for (int l = 0; l < t.levels; ++l) {
+        TailLevel& T = t.lev[l];
+        T.lds_arrays = words;
+        words += 5 * T.s * (T.h + 2);
+        T.lds_tables = words;
+        words += 3 * T.w + 3 * T.h + 9 * T.cw + 9 * T.ch;
+    }
+    t.lds_ginv = words;
+    words += t.coarse_n * t.coarse_n;
+    t.lds_words = words;
+    return (size_t)words * 4;
+}
+
+bool tail_allow_lds(size_t bytes)
+{
+    return hipFuncSetAttribute((const void*)k_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+           hipSuccess;
+}
+
+void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
+    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine)
+{
+    const dim3 grid((L.w + kMultiTX - 1) / kMultiTX, (L.h + kMultiTY - 1) / kMultiTY);
+    if (fine)
+        hipLaunchKernelGGL((k_rb_multi<kMultiTX, kMultiTY, kMultiNS, 1>), grid, dim3(kNT), 0, st, L, cur, rhs, shift,
+            first_color, from_zero, link, status);
+    else
+        hipLaunchKernelGGL((k_rb_multi<kMultiTX, kMultiTY, kMultiNS, 0>), grid, dim3(kNT), 0, st, L, cur, rhs, shift,
+            first_color, from_zero, link, status);
+}
+
+void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
+    const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status)
+{
+    const dim3 grid((C.w + kRestrictCX - 1) / kRestrictCX, (C.h + kRestrictCY - 1) / kRestrictCY);
+    hipLaunchKernelGGL((k_res_restrict<kRestrictCX, kRestrictCY>), grid, dim3(kNT), 0, st, F, cur, rhs, shift, C, rx,
+        ry, status);
+}
+
+void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, const double* down64,
+    double* factor, float* ginv, int n, PcgScalars* S)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(kFactorThreads), 0, st, w, h, right64, down64, factor, ginv, n,
+        S);
+}
+
+void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status)
+{
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), lds_bytes, st, desc, shift, status);
+}
+
+}  // namespace fsim

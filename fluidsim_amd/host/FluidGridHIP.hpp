@@ -1,0 +1,302 @@
+// FluidGridHIP.hpp -- C++ host adapter: the reference backend interface over the HIP C ABI.
+//
+// Implements the method set of fluid::FluidGridBase (FluidSim/FluidGridBase.h:20-147) with the same
+// names, signatures and error behaviour, forwarding every call to libfluidsim_hip.so
+// (include/fluidsim_hip.h).  The value types mirror FluidSim/FluidGridTypes.h:10-79 and
+// FluidSim/FluidMixture.h:14-39 field for field.  Inside the reference tree the same class derives
+// from fluid::FluidGridBase and is selected in FluidGrid::selectBackend (FluidSim/FluidGrid.cpp:37-90).
+//
+// Error mapping (FluidGridBase.h:48-116): FS_E_INVALID_ARGUMENT -> std::invalid_argument,
+// FS_E_LOGIC -> std::logic_error, FS_E_UNSUPPORTED -> std::invalid_argument (the base class's
+// "requires uniform WebGPU" contract), device/runtime failures -> std::runtime_error.
+// Non-convergence is returned in ProjectionResult, never thrown.
+#pragma once
+
+#include "../../include/fluidsim_hip.h"
+
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace fluid_hip {
+
+template <typename Scalar> struct FluidGridState {
+    std::vector<Scalar> liquid_fraction;
+    std::vector<Scalar> u;
+    std::vector<Scalar> v;
+    std::vector<Scalar> pressure;
+    std::vector<std::vector<Scalar>> liquid_mass;
+    std::vector<std::vector<Scalar>> gas_mass;
+};
+
+struct FluidStepTimings {
+    double transport_ms = 0;
+    double viscosity_ms = 0;
+    double forces_ms = 0;
+    double pressure_ms = 0;
+    double selection_ms = 0;
+    double regrid_ms = 0;
+};
+
+struct FluidProjectionResult {
+    int iterations = 0;
+    double max_divergence = std::numeric_limits<double>::infinity();
+    bool converged = false;
+    int physical_steps = 1;
+    int transport_steps = 0;
+    bool relaxed = false;
+    double max_volume_error = 0.0;
+    double rms_volume_error = 0.0;
+    int viscosity_iterations = 0;
+    bool precision_limited = false;
+
+    bool accepted() const { return converged || relaxed || precision_limited; }
+};
+
+enum class Phase { Liquid, Gas };
+
+struct MixtureComponent {
+    int substance = 0;
+    double density = 0.0;
+    double dynamic_viscosity = 0.0;
+};
+
+struct FluidMixture {
+    std::vector<MixtureComponent> components;
+    bool hasViscosity() const
+    {
+        for (const auto& c : components)
+            if (c.dynamic_viscosity != 0.0)
+                return true;
+        return false;
+    }
+};
+
+struct DVec2 {
+    double x = 0.0, y = 0.0;
+};
+
+inline void check(fs_status status)
+{
+    if (status == FS_OK)
+        return;
+    const std::string message = fs_last_error();
+    switch (status) {
+    case FS_E_INVALID_ARGUMENT:
+    case FS_E_UNSUPPORTED: throw std::invalid_argument(message);
+    case FS_E_LOGIC: throw std::logic_error(message);
+    default: throw std::runtime_error(message);
+    }
+}
+
+// One HIP device and one stream shared by the grids created on it.
+class HipContext {
+public:
+    explicit HipContext(int device = 0) { check(fs_context_create(device, &ctx_)); }
+    ~HipContext() { fs_context_destroy(ctx_); }
+    HipContext(const HipContext&) = delete;
+    HipContext& operator=(const HipContext&) = delete;
+    fs_context* get() const { return ctx_; }
+
+private:
+    fs_context* ctx_ = nullptr;
+};
+
+class FluidGridHIP {
+public:
+    using Scalar = double;
+    using State = FluidGridState<Scalar>;
+    using ProjectionResult = FluidProjectionResult;
+    using StepTimings = FluidStepTimings;
+
+    enum class PressurePreconditioner { Diagonal, Multigrid };
+
+    FluidGridHIP(HipContext& context, int width, int height, double cell_size)
+        : width_(width), height_(height), cell_size_(cell_size)
+    {
+        check(fs_grid_create(context.get(), width, height, cell_size, &grid_));
+    }
+    ~FluidGridHIP() { destroy(); }
+    FluidGridHIP(const FluidGridHIP&) = delete;
+    FluidGridHIP& operator=(const FluidGridHIP&) = delete;
+
+    int getWidth() const { return width_; }
+    int getHeight() const { return height_; }
+    double getCellSize() const { return cell_size_; }
+    DVec2 getOrigin() const
+    {
+        DVec2 o;
+        check(fs_get_origin(grid_, &o.x, &o.y));
+        return o;
+    }
+    void setOrigin(DVec2 origin) { check(fs_set_origin(grid_, origin.x, origin.y)); }
+    bool requiresRenderThread() const { return false; }
+
+    void configureMaterials(FluidMixture liquid, FluidMixture gas)
+    {
+        std::vector<double> ld, lv, gd, gv;
+        for (const auto& c : liquid.components) {
+            ld.push_back(c.density);
+            lv.push_back(c.dynamic_viscosity);
+        }
+        for (const auto& c : gas.components) {
+            gd.push_back(c.density);
+            gv.push_back(c.dynamic_viscosity);
+        }
+        check(fs_configure_materials(grid_, (int)ld.size(), ld.data(), lv.data(), (int)gd.size(), gd.data(),
+            gv.data()));
+        liquid_ = std::move(liquid);
+        gas_ = std::move(gas);
+    }
+    const FluidMixture& getLiquidMixture() const { return liquid_; }
+    const FluidMixture& getGasMixture() const { return gas_; }
+    bool hasViscosity() const { return liquid_.hasViscosity() || gas_.hasViscosity(); }
+    void setComponentViscosity(Phase, size_t, double coefficient)
+    {
+        if (coefficient != 0.0)
+            throw std::invalid_argument("Viscosity is not implemented by the HIP backend yet");
+    }
+
+    double getGravityY() const
+    {
+        double g = 0.0;
+        check(fs_get_gravity(grid_, &g));
+        return g;
+    }
+    void setGravityY(double acceleration) { check(fs_set_gravity(grid_, acceleration)); }
+    double getSurfaceTension() const { return 0.0; }
+    void setSurfaceTension(double coefficient) { check(fs_set_surface_tension(grid_, coefficient)); }
+    double capillaryStepLimit() const
+    {
+        double v = 0.0;
+        check(fs_capillary_step_limit(grid_, &v));
+        return v;
+    }
+
+    void setPressurePreconditioner(PressurePreconditioner kind)
+    {
+        check(fs_set_preconditioner(grid_, kind == PressurePreconditioner::Diagonal ? FS_PRECOND_DIAGONAL
+                                                                                   : FS_PRECOND_MULTIGRID));
+    }
+    void setTransport(fs_transport mode) { check(fs_set_transport(grid_, mode)); }
+
+    // FluidGridCPU::upload (FluidGridCPU.cpp:198-251): empty vectors mean zero; sizes must match.
+    void upload(const State& state)
+    {
+        const size_t n = (size_t)width_ * height_;
+        const size_t nu = (size_t)(width_ + 1) * height_, nv = (size_t)width_ * (height_ + 1);
+        auto field = [&](const std::vector<double>& v, size_t count) -> const double* {
+            if (v.empty())
+                return nullptr;
+            if (v.size() != count)
+                throw std::invalid_argument("FluidGridHIP upload dimensions do not match the grid");
+            return v.data();
+        };
+        auto components = [&](const std::vector<std::vector<double>>& src, size_t k, std::vector<double>& flat)
+            -> const double* {
+            if (src.empty())
+                return nullptr;
+            if (src.size() != k)
+                throw std::invalid_argument("FluidGridHIP upload component count does not match the mixture");
+            flat.assign(k * n, 0.0);
+            for (size_t c = 0; c < k; ++c) {
+                const double* p = field(src[c], n);
+                if (p)
+                    std::copy(p, p + n, flat.begin() + c * n);
+            }
+            return flat.data();
+        };
+        std::vector<double> lm, gm;
+        check(fs_upload(grid_, field(state.liquid_fraction, n), field(state.u, nu), field(state.v, nv),
+            field(state.pressure, n), components(state.liquid_mass, liquid_.components.size(), lm),
+            components(state.gas_mass, gas_.components.size(), gm)));
+    }
+
+    State download() const
+    {
+        const size_t n = (size_t)width_ * height_;
+        const size_t kl = liquid_.components.size(), kg = gas_.components.size();
+        State s;
+        s.liquid_fraction.resize(n);
+        s.u.resize((size_t)(width_ + 1) * height_);
+        s.v.resize((size_t)width_ * (height_ + 1));
+        s.pressure.resize(n);
+        std::vector<double> lm(kl * n), gm(kg * n);
+        check(fs_download(grid_, s.liquid_fraction.data(), s.u.data(), s.v.data(), s.pressure.data(),
+            kl ? lm.data() : nullptr, kg ? gm.data() : nullptr));
+        for (size_t c = 0; c < kl; ++c)
+            s.liquid_mass.emplace_back(lm.begin() + c * n, lm.begin() + (c + 1) * n);
+        for (size_t c = 0; c < kg; ++c)
+            s.gas_mass.emplace_back(gm.begin() + c * n, gm.begin() + (c + 1) * n);
+        return s;
+    }
+
+    void invalidateCaches() { check(fs_invalidate_caches(grid_)); }
+
+    void destroy()
+    {
+        if (grid_)
+            fs_grid_destroy(grid_);
+        grid_ = nullptr;
+    }
+
+    ProjectionResult step(double dt, int max_iterations = 200, double divergence_tolerance = 1e-6,
+        StepTimings* timings = nullptr)
+    {
+        fs_projection_result r{};
+        fs_step_timings t{};
+        check(fs_step(grid_, dt, max_iterations, divergence_tolerance, timings ? &t : nullptr, &r));
+        if (timings)
+            *timings = {t.transport_ms, t.viscosity_ms, t.forces_ms, t.pressure_ms, t.selection_ms, t.regrid_ms};
+        return convert(r);
+    }
+
+    ProjectionResult project(double dt, int max_iterations = 200, double divergence_tolerance = 1e-6)
+    {
+        fs_projection_result r{};
+        check(fs_project(grid_, dt, max_iterations, divergence_tolerance, &r));
+        return convert(r);
+    }
+
+    void applyGravity(double dt) { check(fs_apply_gravity(grid_, dt)); }
+    void addVortex(DVec2 center, double radius, double strength)
+    {
+        check(fs_add_vortex(grid_, center.x, center.y, radius, strength));
+    }
+
+    double maxDivergence() const
+    {
+        double v = 0.0;
+        check(fs_max_divergence(grid_, &v));
+        return v;
+    }
+
+    fs_grid* handle() const { return grid_; }
+
+private:
+    static ProjectionResult convert(const fs_projection_result& r)
+    {
+        ProjectionResult p;
+        p.iterations = r.iterations;
+        p.max_divergence = r.max_divergence;
+        p.converged = r.converged != 0;
+        p.physical_steps = r.physical_steps;
+        p.transport_steps = r.transport_steps;
+        p.relaxed = r.relaxed != 0;
+        p.max_volume_error = r.max_volume_error;
+        p.rms_volume_error = r.rms_volume_error;
+        p.viscosity_iterations = r.viscosity_iterations;
+        p.precision_limited = r.precision_limited != 0;
+        return p;
+    }
+
+    fs_grid* grid_ = nullptr;
+    int width_;
+    int height_;
+    double cell_size_;
+    FluidMixture liquid_, gas_;
+};
+
+}  // namespace fluid_hip

@@ -175,8 +175,10 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
 
 /* ---- measurement (no reference counterpart) ---- */
 /* Times `reps` back-to-back launches of one hot kernel on the grid's stream with HIP events and
- * returns the mean milliseconds per launch.  kernel: 0 = fine-level red-black sweep,
- * 1 = operator + dot (A d), 2 = PCG update, 3 = FP32 copy of one padded vector (reference kernel). */
+ * returns the mean milliseconds per launch.  kernel: 0 = finest-level four-sweep smoother,
+ * 1 = operator + dot (A d), 2 = PCG update, 3 = FP32 copy of one padded vector (reference kernel),
+ * 4 = second level's smoother, 5 = finest residual + restriction, 6 = coarse tail, 7 = coarse
+ * factorisation, 8 = direction update, 9 = r.z reduction. */
 fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch);
 /* Cumulative device bytes allocated by this grid. */
 fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);

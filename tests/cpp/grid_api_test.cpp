@@ -1,0 +1,170 @@
+// C++ boundary test of the HIP backend through the FluidGridHIP adapter (fluidsim_amd/host/).
+// Mirrors the contract checks of the reference's tests/FluidGridAPITests.cpp: deterministic exact
+// results with and without timings (:83-134), zero-time and zero-iteration steps reported, not thrown
+// (:186-229), upload validation, and the hydrostatic closed form of tests/FluidGridTests.cpp:140-160.
+// Needs a GPU; run by tests/test_cpp_adapter.py.
+#include "fluidsim_amd/host/FluidGridHIP.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <iostream>
+#include <stdexcept>
+
+using fluid_hip::FluidGridHIP;
+using fluid_hip::FluidMixture;
+using fluid_hip::HipContext;
+
+namespace {
+
+void require(bool condition, const char* message)
+{
+    if (!condition)
+        throw std::runtime_error(message);
+}
+
+bool sameState(const FluidGridHIP::State& a, const FluidGridHIP::State& b)
+{
+    return a.liquid_fraction == b.liquid_fraction && a.u == b.u && a.v == b.v && a.pressure == b.pressure &&
+           a.liquid_mass == b.liquid_mass && a.gas_mass == b.gas_mass;
+}
+
+FluidGridHIP::State configure(FluidGridHIP& grid)
+{
+    FluidMixture liquid, gas;
+    liquid.components = {{0, 100.0}, {1, 300.0}};
+    gas.components = {{2, 0.1}, {3, 0.1}};
+    grid.configureMaterials(liquid, gas);
+    grid.setOrigin({-7.0, 11.0});
+    grid.setGravityY(80.0);
+    const int width = grid.getWidth(), height = grid.getHeight();
+    const double area = grid.getCellSize() * grid.getCellSize();
+    FluidGridHIP::State state;
+    state.liquid_fraction.resize(width * height);
+    state.liquid_mass.assign(2, std::vector<double>(width * height));
+    state.gas_mass.assign(2, std::vector<double>(width * height));
+    for (int y = 0; y < height; ++y) {
+        for (int x = 0; x < width; ++x) {
+            const int i = y * width + x;
+            const double f = std::clamp(y + 1.0 - (x + height * 0.5), 0.0, 1.0);
+            const double red_share = x < width / 4 ? 1.0 : 0.0;
+            const double cloud_share = 0.15 + 0.1 * std::sin(0.2 * x);
+            state.liquid_fraction[i] = f;
+            state.liquid_mass[0][i] = 100.0 * f * area * (1.0 - red_share);
+            state.liquid_mass[1][i] = 300.0 * f * area * red_share;
+            state.gas_mass[0][i] = 0.1 * (1.0 - f) * area * (1.0 - cloud_share);
+            state.gas_mass[1][i] = 0.1 * (1.0 - f) * area * cloud_share;
+        }
+    }
+    grid.upload(state);
+    return state;
+}
+
+void timedAndUntimedStepsAreIdentical(HipContext& ctx)
+{
+    FluidGridHIP plain(ctx, 32, 32, 1.0), measured(ctx, 32, 32, 1.0);
+    for (auto* g : {&plain, &measured}) {
+        g->setTransport(FS_TRANSPORT_SEMI_LAGRANGIAN);
+        configure(*g);
+    }
+    FluidGridHIP::StepTimings timings;
+    for (int tick = 0; tick < 8; ++tick) {
+        const auto a = plain.step(0.01, 2000, 1e-3);
+        const auto b = measured.step(0.01, 2000, 1e-3, &timings);
+        require(a.converged && b.converged, "timed comparison did not converge");
+        require(a.iterations == b.iterations && a.max_divergence == b.max_divergence, "timings changed the result");
+        require(sameState(plain.download(), measured.download()), "timings changed the state");
+        require(timings.transport_ms > 0.0 && timings.pressure_ms > 0.0, "missing stage measurements");
+    }
+    measured.step(0.0, 2000, 1e-3, &timings);
+    require(timings.transport_ms == 0.0 && timings.forces_ms == 0.0 && timings.pressure_ms == 0.0,
+        "zero-time step retained stale timings");
+}
+
+void zeroIterationAndZeroTimeAreReported(HipContext& ctx)
+{
+    FluidGridHIP grid(ctx, 16, 12, 1.0);
+    grid.setTransport(FS_TRANSPORT_NONE);
+    configure(grid);
+    const auto before = grid.download();
+    const auto zero = grid.step(0.0);
+    require(!zero.converged && zero.physical_steps == 0, "zero timestep unexpectedly advanced");
+    require(sameState(before, grid.download()), "zero timestep changed the state");
+    const auto budget = grid.step(0.01, 0, 1e-8);
+    require(!budget.converged, "zero-iteration projection unexpectedly converged");
+}
+
+void uploadIsValidated(HipContext& ctx)
+{
+    FluidGridHIP grid(ctx, 8, 6, 1.0);
+    bool threw = false;
+    try {
+        grid.project(0.01);
+    } catch (const std::logic_error&) {
+        threw = true;
+    }
+    require(threw, "use before upload was not a logic error");
+    auto state = configure(grid);
+    state.u.assign(5, 0.0);
+    threw = false;
+    try {
+        grid.upload(state);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    require(threw, "mismatched upload was not rejected");
+    threw = false;
+    try {
+        grid.setSurfaceTension(0.07);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    require(threw, "unsupported surface tension was not reported");
+}
+
+void hydrostaticPressure(HipContext& ctx)
+{
+    FluidGridHIP grid(ctx, 12, 10, 2.0);
+    FluidMixture liquid, gas;
+    liquid.components = {{0, 100.0}};
+    gas.components = {{1, 0.1}};
+    grid.configureMaterials(liquid, gas);
+    grid.setGravityY(100.0);
+    grid.setTransport(FS_TRANSPORT_NONE);
+    FluidGridHIP::State s;
+    s.liquid_fraction.assign(120, 1.0);
+    s.liquid_mass.assign(1, std::vector<double>(120, 100.0 * 4.0));
+    s.gas_mass.assign(1, std::vector<double>(120, 0.0));
+    grid.upload(s);
+    const auto r = grid.step(1.0 / 120.0, 2000, 1e-8);
+    require(r.converged, "hydrostatic projection did not converge");
+    const auto out = grid.download();
+    for (double v : out.v)
+        require(std::abs(v) < 1e-7, "pressure did not cancel gravity");
+    const double rise = out.pressure[9 * 12] - out.pressure[0];
+    const double expected = 100.0 * 100.0 * 2.0 * 9;
+    require(std::abs(rise - expected) < expected * 1e-8, "incorrect hydrostatic pressure gradient");
+}
+
+}  // namespace
+
+int main()
+{
+    HipContext ctx(0);
+    const struct {
+        const char* name;
+        void (*run)(HipContext&);
+    } tests[] = {{"timed and untimed steps are identical", timedAndUntimedStepsAreIdentical},
+        {"zero time and zero iterations are reported", zeroIterationAndZeroTimeAreReported},
+        {"upload validation", uploadIsValidated}, {"hydrostatic pressure", hydrostaticPressure}};
+    int failures = 0;
+    for (const auto& t : tests) {
+        try {
+            t.run(ctx);
+            std::cout << "PASS " << t.name << '\n';
+        } catch (const std::exception& e) {
+            ++failures;
+            std::cerr << "FAIL " << t.name << ": " << e.what() << '\n';
+        }
+    }
+    return failures == 0 ? 0 : 1;
+}
