@@ -1145,3 +1145,515 @@ orc_result orc_step_semi_lagrangian(orc_grid* g, double dt, int max_iterations, 
     r.transport_steps = 1;
     return r;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Reference transport: CFL-split X/Y PLIC volume transfers with MYC normals, component masses  */
+/* and consistent momentum (FluidGridCPU.cpp:93-167, 362-463, 473-833, 1207-1476).             */
+/* ------------------------------------------------------------------------------------------ */
+
+static double dmin(double a, double b) { return b < a ? b : a; }   /* std::min(a, b) */
+
+/* liquidArea, FluidGridCPU.cpp:93-133: liquid area of the half plane n.p <= offset inside a
+ * rectangle of the unit cell. */
+static double liquid_area(double nx, double ny, double offset, double x0, double y0, double x1, double y1)
+{
+    const double width = x1 - x0;
+    const double height = y1 - y0;
+    const double area = width * height;
+    const double min_projection = nx * (nx >= 0.0 ? x0 : x1) + ny * (ny >= 0.0 ? y0 : y1);
+    const double cut = offset - min_projection;
+    const double span_x = fabs(nx) * width;
+    const double span_y = fabs(ny) * height;
+    const double small_span = dmin(span_x, span_y);
+    const double large_span = dmax(span_x, span_y);
+    if (cut <= 0.0)
+        return 0.0;
+    if (cut >= small_span + large_span)
+        return area;
+    if (small_span == 0.0)
+        return area * cut / large_span;
+    if (cut < small_span)
+        return area * cut * cut / (2.0 * small_span * large_span);
+    if (cut < large_span)
+        return area * (cut - 0.5 * small_span) / large_span;
+    const double dry_cut = small_span + large_span - cut;
+    return area * (1.0 - dry_cut * dry_cut / (2.0 * small_span * large_span));
+}
+
+/* interfaceOffset, FluidGridCPU.cpp:135-167: line offset that cuts `fraction` of the unit cell. */
+static double interface_offset(double nx, double ny, double fraction)
+{
+    const double min_projection = dmin(nx, 0.0) + dmin(ny, 0.0);
+    const double span_x = fabs(nx);
+    const double span_y = fabs(ny);
+    const double small_span = dmin(span_x, span_y);
+    const double large_span = dmax(span_x, span_y);
+    const double triangle_fraction = 0.5 * small_span / large_span;
+    double cut;
+    if (fraction < triangle_fraction)
+        cut = sqrt(2.0 * small_span * large_span * fraction);
+    else if (fraction <= 1.0 - triangle_fraction)
+        cut = large_span * fraction + 0.5 * small_span;
+    else
+        cut = small_span + large_span - sqrt(2.0 * small_span * large_span * (1.0 - fraction));
+    return min_projection + cut;
+}
+
+/* interfaceNormal, FluidGridCPU.cpp:473-521: mixed Youngs-centred (MYC) normal. */
+static void interface_normal(const double* f, int W, int H, int x, int y, double* nx, double* ny)
+{
+    const int lx = imax(x - 1, 0), rx = imin(x + 1, W - 1);
+    const double* above = f + (size_t)imax(y - 1, 0) * W;
+    const double* row = f + (size_t)y * W;
+    const double* below = f + (size_t)imin(y + 1, H - 1) * W;
+    const double tl = above[lx], t = above[x], tr = above[rx];
+    const double l = row[lx], r = row[rx];
+    const double bl = below[lx], b = below[x], br = below[rx];
+    const double left_sum = tl + l + bl;
+    const double right_sum = tr + r + br;
+    const double top_sum = tl + t + tr;
+    const double bottom_sum = bl + b + br;
+    double cx = 0.5 * (left_sum - right_sum), cy = 0.5 * (top_sum - bottom_sum);
+    const double yx = left_sum + l - right_sum - r, yy = top_sum + t - bottom_sum - b;
+    if (cx == 0.0 && cy == 0.0) {
+        if (yx == 0.0 && yy == 0.0) { *nx = 1.0; *ny = 0.0; }
+        else { *nx = yx; *ny = yy; }
+        return;
+    }
+    if (fabs(cx) <= fabs(cy)) {
+        cy = copysign(1.0, cy);
+        if (fabs(yx) > fabs(cx * yy)) { *nx = yx; *ny = yy; return; }
+    } else {
+        cx = copysign(1.0, cx);
+        if (fabs(yy) > fabs(cy * yx)) { *nx = yx; *ny = yy; return; }
+    }
+    *nx = cx;
+    *ny = cy;
+}
+
+typedef struct {
+    double *next_fraction, *next_u, *next_v;
+    double *nx, *ny, *offset;
+    unsigned char* split;
+    double *mcw, *lcw, *gcw;
+    double *lt_u, *lt_v, *mt_u, *mt_v, *mom_u, *mom_v, *ct;
+} orc_transport;
+
+static void transport_alloc(const orc_grid* g, orc_transport* T)
+{
+    const int W = g->W, H = g->H;
+    const size_t N = (size_t)W * H, NU = (size_t)(W + 1) * H, NV = (size_t)W * (H + 1);
+    T->next_fraction = dalloc(N);
+    T->next_u = dalloc(NU);
+    T->next_v = dalloc(NV);
+    T->nx = dalloc(N);
+    T->ny = dalloc(N);
+    T->offset = dalloc(N);
+    T->split = (unsigned char*)calloc(N, 1);
+    T->mcw = dalloc(N);
+    T->lcw = dalloc((size_t)g->nl * N);
+    T->gcw = dalloc((size_t)g->ng * N);
+    T->lt_u = dalloc(NU);
+    T->lt_v = dalloc(NV);
+    T->mt_u = dalloc(NU);
+    T->mt_v = dalloc(NV);
+    T->mom_u = dalloc(NU);
+    T->mom_v = dalloc(NV);
+    T->ct = dalloc(NU > NV ? NU : NV);
+}
+
+static void transport_free(orc_transport* T)
+{
+    free(T->next_fraction); free(T->next_u); free(T->next_v);
+    free(T->nx); free(T->ny); free(T->offset); free(T->split);
+    free(T->mcw); free(T->lcw); free(T->gcw);
+    free(T->lt_u); free(T->lt_v); free(T->mt_u); free(T->mt_v);
+    free(T->mom_u); free(T->mom_v); free(T->ct);
+}
+
+/* rebuildInterfaces, FluidGridCPU.cpp:523-541: only interface cells are rewritten. */
+static void rebuild_interfaces(const orc_grid* g, orc_transport* T)
+{
+    const int W = g->W, H = g->H;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const double fraction = g->fraction[i];
+            if (fraction <= 0.0 || fraction >= 1.0)
+                continue;
+            interface_normal(g->fraction, W, H, x, y, &T->nx[i], &T->ny[i]);
+            T->offset[i] = interface_offset(T->nx[i], T->ny[i], fraction);
+        }
+}
+
+/* liquidTransferX / liquidTransferY, FluidGridCPU.cpp:759-833.  `horizontal` selects the axis;
+ * `speed` is the face velocity, `donor` the upwind cell. */
+static double liquid_transfer(const orc_grid* g, const orc_transport* T, int horizontal, double speed,
+    size_t donor, double step)
+{
+    if (speed == 0.0)
+        return 0.0;
+    const double fraction = g->fraction[donor];
+    if (fraction <= 0.0)
+        return 0.0;
+    const double volume_transfer = speed * (step / g->h);
+    const double strip = fabs(volume_transfer);
+    if (fraction >= 1.0)
+        return volume_transfer;
+    double x0, y0, x1, y1;
+    if (horizontal) {
+        x0 = speed > 0.0 ? 1.0 - strip : 0.0;
+        x1 = speed > 0.0 ? 1.0 : strip;
+        y0 = 0.0;
+        y1 = 1.0;
+    } else {
+        x0 = 0.0;
+        x1 = 1.0;
+        y0 = speed > 0.0 ? 1.0 - strip : 0.0;
+        y1 = speed > 0.0 ? 1.0 : strip;
+    }
+    const double area = liquid_area(T->nx[donor], T->ny[donor], T->offset[donor], x0, y0, x1, y1);
+    const double bounded = dclamp(area, dmax(0.0, strip - (1.0 - fraction)), dmin(strip, fraction));
+    return speed > 0.0 ? bounded : -bounded;
+}
+
+/* rebuildTransfersX / rebuildTransfersY, FluidGridCPU.cpp:1207-1233 */
+static void rebuild_transfers(const orc_grid* g, orc_transport* T, int horizontal, double step)
+{
+    const int W = g->W, H = g->H;
+    if (horizontal) {
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x <= W; ++x) {
+                const size_t face = (size_t)y * (W + 1) + x;
+                double t = 0.0;
+                if (x != 0 && x != W) {
+                    const double speed = g->u[face];
+                    t = liquid_transfer(g, T, 1, speed, (size_t)y * W + (speed > 0.0 ? x - 1 : x), step);
+                }
+                T->lt_u[face] = t;
+            }
+    } else {
+        for (int y = 0; y <= H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t face = (size_t)y * W + x;
+                double t = 0.0;
+                if (y != 0 && y != H) {
+                    const double speed = g->v[face];
+                    t = liquid_transfer(g, T, 0, speed, (size_t)(speed > 0.0 ? y - 1 : y) * W + x, step);
+                }
+                T->lt_v[face] = t;
+            }
+    }
+}
+
+/* freezeComponentCorrections, FluidGridCPU.cpp:1307-1349 */
+static void freeze_corrections(const orc_grid* g, orc_transport* T)
+{
+    const size_t N = (size_t)g->W * g->H;
+    const double inverse_cell_volume = 1.0 / (g->h * g->h);
+    for (size_t i = 0; i < N; ++i) {
+        const double fraction = g->fraction[i];
+        const int selected = fraction > 0.5;
+        T->split[i] = (unsigned char)selected;
+        const double inverse_fraction = 1.0 / (selected ? fraction : 1.0 - fraction);
+        T->mcw[i] = 0.0;
+        for (int k = 0; k < g->nl; ++k) {
+            const double weight = selected ? g->lmass[(size_t)k * N + i] * inverse_fraction : 0.0;
+            T->lcw[(size_t)k * N + i] = weight;
+            T->mcw[i] += weight * inverse_cell_volume;
+        }
+        for (int k = 0; k < g->ng; ++k) {
+            const double weight = !selected ? g->gmass[(size_t)k * N + i] * inverse_fraction : 0.0;
+            T->gcw[(size_t)k * N + i] = weight;
+            T->mcw[i] += weight * inverse_cell_volume;
+        }
+    }
+}
+
+/* componentMassTransfer, FluidGridCPU.cpp:543-551 */
+static double component_mass_transfer(double volume_transfer, double donor_fraction, double donor_mass)
+{
+    if (donor_fraction <= 0.0)
+        return 0.0;
+    return donor_mass * (volume_transfer / donor_fraction);
+}
+
+/* advectComponents + rebuildComponentTransfersX/Y, FluidGridCPU.cpp:1235-1305, 1351-1397 */
+static void advect_components(orc_grid* g, orc_transport* T, double step, int horizontal)
+{
+    const int W = g->W, H = g->H;
+    const size_t N = (size_t)W * H;
+    const double scale = step / g->h;
+    const double inverse_cell_volume = 1.0 / (g->h * g->h);
+    const double* flow = horizontal ? g->u : g->v;
+    const size_t face_offset = horizontal ? 1 : (size_t)W;
+    double* total = horizontal ? T->mt_u : T->mt_v;
+    memset(total, 0, sizeof(double) * (horizontal ? (size_t)(W + 1) * H : (size_t)W * (H + 1)));
+    const double* lt = horizontal ? T->lt_u : T->lt_v;
+    for (int phase = 0; phase < 2; ++phase) {
+        const int is_liquid = phase == 0;
+        const int count = is_liquid ? g->nl : g->ng;
+        for (int k = 0; k < count; ++k) {
+            double* mass = (is_liquid ? g->lmass : g->gmass) + (size_t)k * N;
+            const double* weights = (is_liquid ? T->lcw : T->gcw) + (size_t)k * N;
+            if (horizontal) {
+                for (int y = 0; y < H; ++y)
+                    for (int x = 0; x <= W; ++x) {
+                        const size_t face = (size_t)y * (W + 1) + x;
+                        if (x == 0 || x == W) { T->ct[face] = 0.0; continue; }
+                        const size_t donor = (size_t)y * W + (g->u[face] > 0.0 ? x - 1 : x);
+                        const double all = g->u[face] * scale;
+                        const double liquid = lt[face];
+                        const double fraction = g->fraction[donor];
+                        T->ct[face] = component_mass_transfer(is_liquid ? liquid : all - liquid,
+                            is_liquid ? fraction : 1.0 - fraction, mass[donor]);
+                        total[face] += T->ct[face] * inverse_cell_volume;
+                    }
+            } else {
+                for (int y = 0; y <= H; ++y)
+                    for (int x = 0; x < W; ++x) {
+                        const size_t face = (size_t)y * W + x;
+                        if (y == 0 || y == H) { T->ct[face] = 0.0; continue; }
+                        const size_t donor = g->v[face] > 0.0 ? face - W : face;
+                        const double all = g->v[face] * scale;
+                        const double liquid = lt[face];
+                        const double fraction = g->fraction[donor];
+                        T->ct[face] = component_mass_transfer(is_liquid ? liquid : all - liquid,
+                            is_liquid ? fraction : 1.0 - fraction, mass[donor]);
+                        total[face] += T->ct[face] * inverse_cell_volume;
+                    }
+            }
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    const size_t i = (size_t)y * W + x;
+                    const size_t before = horizontal ? (size_t)y * (W + 1) + x : i;
+                    const size_t after = before + face_offset;
+                    const double net = scale * (flow[after] - flow[before]);
+                    mass[i] += T->ct[before] - T->ct[after] + weights[i] * net;
+                }
+        }
+    }
+    orc_invalidate_caches(g);
+}
+
+/* momentumTransfer, FluidGridCPU.cpp:553-558 */
+static double momentum_transfer(double mass_transfer, double before, double after)
+{
+    return mass_transfer * (mass_transfer >= 0.0 ? before : after);
+}
+
+/* advectMomentumX, FluidGridCPU.cpp:560-634 */
+static void advect_momentum_x(orc_grid* g, orc_transport* T, double step)
+{
+    const int W = g->W, H = g->H;
+    const double scale = step / g->h;
+    const double* flow = g->u;
+#define ORC_MC(cx, cy) (T->mcw[(size_t)(cy) * W + (cx)] * scale * \
+        (flow[(size_t)(cy) * (W + 1) + (cx) + 1] - flow[(size_t)(cy) * (W + 1) + (cx)]))
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x <= W; ++x) {
+            const size_t face = (size_t)y * (W + 1) + x;
+            if (x == 0 || x == W) { T->mom_u[face] = 0.0; continue; }
+            const double lm = 0.5 * (T->mt_u[face - 1] + T->mt_u[face]);
+            const double rm = 0.5 * (T->mt_u[face] + T->mt_u[face + 1]);
+            const double lmom = momentum_transfer(lm, T->next_u[face - 1], T->next_u[face]);
+            const double rmom = momentum_transfer(rm, T->next_u[face], T->next_u[face + 1]);
+            const double correction = 0.5 * (ORC_MC(x - 1, y) + ORC_MC(x, y));
+            T->mom_u[face] += lmom - rmom + g->u[face] * correction;
+        }
+    for (int y = 0; y <= H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t face = (size_t)y * W + x;
+            if (y == 0 || y == H) { T->mom_v[face] = 0.0; continue; }
+            const size_t above_left = (size_t)(y - 1) * (W + 1) + x;
+            const size_t below_left = (size_t)y * (W + 1) + x;
+            const double lm = 0.5 * (T->mt_u[above_left] + T->mt_u[below_left]);
+            const double rm = 0.5 * (T->mt_u[above_left + 1] + T->mt_u[below_left + 1]);
+            const double lmom = x > 0 ? momentum_transfer(lm, T->next_v[face - 1], T->next_v[face]) : 0.0;
+            const double rmom = x + 1 < W ? momentum_transfer(rm, T->next_v[face], T->next_v[face + 1]) : 0.0;
+            const double correction = 0.5 * (ORC_MC(x, y - 1) + ORC_MC(x, y));
+            T->mom_v[face] += lmom - rmom + g->v[face] * correction;
+        }
+#undef ORC_MC
+}
+
+/* advectMomentumY, FluidGridCPU.cpp:636-712 */
+static void advect_momentum_y(orc_grid* g, orc_transport* T, double step)
+{
+    const int W = g->W, H = g->H;
+    const double scale = step / g->h;
+    const double* flow = g->v;
+    const size_t us = (size_t)W + 1;
+#define ORC_MC(cx, cy) (T->mcw[(size_t)(cy) * W + (cx)] * scale * \
+        (flow[(size_t)(cy) * W + (cx) + W] - flow[(size_t)(cy) * W + (cx)]))
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x <= W; ++x) {
+            const size_t face = (size_t)y * us + x;
+            if (x == 0 || x == W) { T->mom_u[face] = 0.0; continue; }
+            const size_t left_top = (size_t)y * W + x - 1;
+            const size_t right_top = (size_t)y * W + x;
+            const double tm = 0.5 * (T->mt_v[left_top] + T->mt_v[right_top]);
+            const double bm = 0.5 * (T->mt_v[left_top + W] + T->mt_v[right_top + W]);
+            const double tmom = y > 0 ? momentum_transfer(tm, T->next_u[face - us], T->next_u[face]) : 0.0;
+            const double bmom = y + 1 < H ? momentum_transfer(bm, T->next_u[face], T->next_u[face + us]) : 0.0;
+            const double correction = 0.5 * (ORC_MC(x - 1, y) + ORC_MC(x, y));
+            T->mom_u[face] += tmom - bmom + g->u[face] * correction;
+        }
+    for (int y = 0; y <= H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t face = (size_t)y * W + x;
+            if (y == 0 || y == H) { T->mom_v[face] = 0.0; continue; }
+            const double tm = 0.5 * (T->mt_v[face - W] + T->mt_v[face]);
+            const double bm = 0.5 * (T->mt_v[face] + T->mt_v[face + W]);
+            const double tmom = momentum_transfer(tm, T->next_v[face - W], T->next_v[face]);
+            const double bmom = momentum_transfer(bm, T->next_v[face], T->next_v[face + W]);
+            const double correction = 0.5 * (ORC_MC(x, y - 1) + ORC_MC(x, y));
+            T->mom_v[face] += tmom - bmom + g->v[face] * correction;
+        }
+#undef ORC_MC
+}
+
+/* rebuildMomentum, FluidGridCPU.cpp:1425-1476 */
+static void rebuild_momentum(orc_grid* g, orc_transport* T)
+{
+    const int W = g->W, H = g->H;
+    rebuild_density(g);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x <= W; ++x) {
+            const size_t face = (size_t)y * (W + 1) + x;
+            if (x == 0 || x == W) { T->mom_u[face] = 0.0; T->next_u[face] = 0.0; continue; }
+            const size_t cell = (size_t)y * W + x;
+            const double face_density = 0.5 * (g->density[cell - 1] + g->density[cell]);
+            T->mom_u[face] = face_density * g->u[face];
+            T->next_u[face] = g->u[face];
+        }
+    for (int y = 0; y <= H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t face = (size_t)y * W + x;
+            if (y == 0 || y == H) { T->mom_v[face] = 0.0; T->next_v[face] = 0.0; continue; }
+            const double face_density = 0.5 * (g->density[face - W] + g->density[face]);
+            T->mom_v[face] = face_density * g->v[face];
+            T->next_v[face] = g->v[face];
+        }
+}
+
+/* recoverVelocity, FluidGridCPU.cpp:714-757 */
+static void recover_velocity(orc_grid* g, orc_transport* T)
+{
+    const int W = g->W, H = g->H;
+    rebuild_density(g);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x <= W; ++x) {
+            const size_t face = (size_t)y * (W + 1) + x;
+            if (x == 0 || x == W) { T->next_u[face] = 0.0; continue; }
+            const size_t cell = (size_t)y * W + x;
+            const double face_density = 0.5 * (g->density[cell - 1] + g->density[cell]);
+            T->next_u[face] = T->mom_u[face] / face_density;
+        }
+    for (int y = 0; y <= H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t face = (size_t)y * W + x;
+            if (y == 0 || y == H) { T->next_v[face] = 0.0; continue; }
+            const double face_density = 0.5 * (g->density[face - W] + g->density[face]);
+            T->next_v[face] = T->mom_v[face] / face_density;
+        }
+}
+
+/* New fractions after one directional pass (FluidGridCPU.cpp:407-428, 437-455). */
+static void update_fraction(orc_grid* g, orc_transport* T, double scale, int horizontal)
+{
+    const int W = g->W, H = g->H;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            double net_liquid, net_fluid;
+            if (horizontal) {
+                const size_t lf = (size_t)y * (W + 1) + x;
+                net_liquid = T->lt_u[lf + 1] - T->lt_u[lf];
+                net_fluid = scale * (g->u[lf + 1] - g->u[lf]);
+            } else {
+                net_liquid = T->lt_v[i + W] - T->lt_v[i];
+                net_fluid = scale * (g->v[i + W] - g->v[i]);
+            }
+            T->next_fraction[i] = g->fraction[i] - net_liquid + (double)T->split[i] * net_fluid;
+        }
+    double* swap = g->fraction;
+    g->fraction = T->next_fraction;
+    T->next_fraction = swap;
+}
+
+/* Transport CFL limit, FluidGridCPU.cpp:366-389: half a cell of outflow per substep. */
+double orc_transport_max_step(const orc_grid* g, double dt)
+{
+    const int W = g->W, H = g->H;
+    double maximum = 0.0;
+    for (int y = 0; y < H; ++y) {
+        const double* hz = g->u + (size_t)y * (W + 1);
+        const double* vt = g->v + (size_t)y * W;
+        for (int x = 0; x < W; ++x) {
+            const double outflow = dmax(hz[x + 1], 0.0) + dmax(-hz[x], 0.0) + dmax(vt[x + W], 0.0) +
+                                   dmax(-vt[x], 0.0);
+            maximum = dmax(maximum, outflow);
+        }
+    }
+    return maximum > 0.0 ? 0.5 * g->h / maximum : dt;
+}
+
+/* advect, FluidGridCPU.cpp:362-463.  Returns the number of transport substeps. */
+int orc_advect_reference(orc_grid* g, double dt)
+{
+    orc_transport T;
+    transport_alloc(g, &T);
+    rebuild_momentum(g, &T);
+    const double max_step = orc_transport_max_step(g, dt);
+    double remaining = dt;
+    int substeps = 0;
+    while (remaining > 0.0) {
+        const double step = dmin(remaining, max_step);
+        const double scale = step / g->h;
+        freeze_corrections(g, &T);
+
+        rebuild_interfaces(g, &T);
+        rebuild_transfers(g, &T, 1, step);
+        advect_components(g, &T, step, 1);
+        advect_momentum_x(g, &T, step);
+        update_fraction(g, &T, scale, 1);
+        recover_velocity(g, &T);
+
+        rebuild_interfaces(g, &T);
+        rebuild_transfers(g, &T, 0, step);
+        advect_components(g, &T, step, 0);
+        advect_momentum_y(g, &T, step);
+        update_fraction(g, &T, scale, 0);
+        recover_velocity(g, &T);
+
+        remaining -= step;
+        ++substeps;
+    }
+    double* swap = g->u;
+    g->u = T.next_u;
+    T.next_u = swap;
+    swap = g->v;
+    g->v = T.next_v;
+    T.next_v = swap;
+    transport_free(&T);
+    return substeps;
+}
+
+/* step, FluidGridCPU.cpp:300-360, without surface tension or an acceleration field. */
+orc_result orc_step(orc_grid* g, double dt, int max_iterations, double tol)
+{
+    if (dt <= 0.0) {
+        orc_result r;
+        memset(&r, 0, sizeof(r));
+        r.max_divergence = INFINITY;
+        return r;
+    }
+    const int substeps = orc_advect_reference(g, dt);
+    orc_apply_gravity(g, dt);
+    orc_result r = orc_project(g, dt, max_iterations, tol);
+    r.physical_steps = 1;
+    r.transport_steps = 0;
+    (void)substeps;
+    return r;
+}

@@ -6,7 +6,8 @@
  * product path (fluidsim_amd, libfluidsim_hip.so) never links or calls it.
  *
  * Follows, single-threaded (the reference's bitwise-reproducible mode):
- *   FluidSim/UniformCPU/FluidGridCPU.cpp      (project, solvePressure, operator, gravity, vortices)
+ *   FluidSim/UniformCPU/FluidGridCPU.cpp      (project, solvePressure, operator, gravity, vortices,
+ *                                             PLIC/VOF transport, step)
  *   FluidSim/UniformCPU/PressureMultigrid.cpp (hierarchy, coarsening, RB-GS V(4,4), Cholesky)
  *
  * Layout conventions are the reference's: u is (W+1) x H, v is W x (H+1), cells W x H,
@@ -79,6 +80,14 @@ orc_result orc_force_project(orc_grid* g, double dt, int max_iterations, double 
 /* Semi-Lagrangian transport mode (build-defined; the reference has none), then gravity and projection. */
 void orc_advect_semi_lagrangian(orc_grid* g, double dt);
 orc_result orc_step_semi_lagrangian(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
+/* advect(dt), FluidGridCPU.cpp:362-463: CFL-split PLIC transport of fractions, component masses and
+ * momentum.  Returns the number of transport substeps taken. */
+int orc_advect_reference(orc_grid* g, double dt);
+/* Largest transport substep for dt (FluidGridCPU.cpp:366-389). */
+double orc_transport_max_step(const orc_grid* g, double dt);
+/* step(dt, max_iterations, tol), FluidGridCPU.cpp:300-360, without surface tension: transport, gravity,
+ * projection. */
+orc_result orc_step(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
 /* Staggered bilinear velocity sample at a world position (FluidGridCPU.cpp:465-471). */
 void orc_sample_velocity(const orc_grid* g, double px, double py, double* vx, double* vy);
 

@@ -79,6 +79,9 @@ def lib():
         "orc_force_project": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
         "orc_advect_semi_lagrangian": (None, [vp, C.c_double]),
         "orc_step_semi_lagrangian": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
+        "orc_advect_reference": (C.c_int, [vp, C.c_double]),
+        "orc_transport_max_step": (C.c_double, [vp, C.c_double]),
+        "orc_step": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
         "orc_sample_velocity": (None, [vp, C.c_double, C.c_double, _dp, _dp]),
         "orc_density": (None, [vp, _dp]),
         "orc_face_weights": (None, [vp, _dp, _dp]),
@@ -244,6 +247,17 @@ class OracleGrid:
 
     def step_semi_lagrangian(self, dt, max_iterations=200, divergence_tolerance=1e-6):
         return lib().orc_step_semi_lagrangian(self._h, dt, max_iterations, divergence_tolerance).as_dict()
+
+    def advect(self, dt):
+        """advect(dt), FluidGridCPU.cpp:362-463; returns the number of CFL substeps."""
+        return lib().orc_advect_reference(self._h, dt)
+
+    def transport_max_step(self, dt):
+        return lib().orc_transport_max_step(self._h, dt)
+
+    def step(self, dt, max_iterations=200, divergence_tolerance=1e-6):
+        """step(), FluidGridCPU.cpp:300-360 (surface tension off)."""
+        return lib().orc_step(self._h, dt, max_iterations, divergence_tolerance).as_dict()
 
     def sample_velocity(self, x, y):
         vx, vy = C.c_double(), C.c_double()
