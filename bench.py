@@ -24,9 +24,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "MGPCG solves/s to 1e-6 rel. residual (1024², mixed batch); p50/p99 step latency"
-# Algorithmic bytes of one fine-level red-black sweep per cell: read solution, right and down
-# conductances, 1/diagonal and rhs (FP32), write the solution (FP32).
-SWEEP_BYTES_PER_CELL = 24
+# Algorithmic bytes per fine cell of the finest post-smoothing launch (prolongation + four red-black
+# sweeps in LDS): read solution, right and down conductances and rhs (4 x FP32), read the coarse
+# correction (one FP32 per four fine cells), write the solution (FP32).
+SMOOTH_BYTES_PER_CELL = 21
 HBM_PEAK_GBS = 8000.0
 
 
@@ -120,9 +121,9 @@ def main():
     rel = [r.final_residual / r.rhs_max if r.rhs_max > 0 else 0.0 for r in results]
     iters = [r.iterations for r in results]
 
-    # Dominant kernel: the finest-level red-black sweep, timed with HIP events on the grid's stream.
+    # Dominant kernel: the finest-level post-smoothing launch, timed with HIP events on the grid's stream.
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
-    sweep_bytes = SWEEP_BYTES_PER_CELL * wl.width * wl.height
+    sweep_bytes = SMOOTH_BYTES_PER_CELL * wl.width * wl.height
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
 
     out = {
@@ -150,7 +151,7 @@ def main():
         "converged_all": bool(ok),
         "max_rel_residual": float(max(rel)),
         "roofline": {
-            "kernel": "k_rb_sweep<64,16,1> (finest-level red-black Gauss-Seidel sweep)",
+            "kernel": "k_rb_multi<64,32,4,1,1> (finest-level prolongation + 4 red-black Gauss-Seidel sweeps)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -86,6 +86,7 @@ SIGNATURES = {
     "fs_apply_gravity": [_vp, C.c_double],
     "fs_add_vortex": [_vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fs_max_divergence": [_vp, _dp],
+    "fs_advect": [_vp, C.c_double, _ip],
     "fs_advect_semi_lagrangian": [_vp, C.c_double],
     "fs_snapshot_save": [_vp],
     "fs_snapshot_restore": [_vp],

@@ -58,6 +58,11 @@ template <class T> struct DevBuf {
         return *this;
     }
     ~DevBuf() { release(); }
+    void swap(DevBuf& o) noexcept
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
     void release()
     {
         if (p)
@@ -158,6 +163,7 @@ struct fs_context {
     size_t lds_per_block = 0;
     hipStream_t stream = nullptr;
     PcgScalars* pinned = nullptr;   // host-pinned mirror of a grid's scalar block
+    double* pinned_value = nullptr; // host-pinned landing slot of single device reductions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -181,6 +187,11 @@ struct fs_grid {
     DevBuf<double> fraction, u, v, lmass, gmass, density, p, p_prev, b, tmp64;
     DevBuf<double> snap_fraction, snap_u, snap_v, snap_p, snap_p_prev, snap_lmass, snap_gmass;
     DevBuf<double> sl_u, sl_v, sl_cells;
+    // reference transport: second fraction/mass buffers, transported velocities (ping-pong), momenta,
+    // per-pass face transfers, frozen corrections
+    DevBuf<double> tr_fraction, tr_lmass, tr_gmass, tr_nu[2], tr_nv[2], tr_mom_u, tr_mom_v, tr_lt, tr_mt;
+    DevBuf<double> tr_lcw, tr_gcw, tr_mcw, tr_out;
+    DevBuf<unsigned char> tr_split;
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<PcgScalars> S;
     DevBuf<double> partials;
@@ -192,6 +203,7 @@ struct fs_grid {
     int tail_start = 0;
     TailDesc tail{};
     DevBuf<TailDesc> tail_dev;
+    DevBuf<unsigned> tail_tables;
 
     hipGraphExec_t iter_exec = nullptr;
     hipGraph_t iter_graph = nullptr;
@@ -318,6 +330,47 @@ struct fs_grid {
         }
     }
 
+    // Every tail level's transfer tables in the kernel's LDS layout, as one word block.
+    void build_tail_tables()
+    {
+        std::vector<unsigned> words((size_t)std::max(tail.table_words, 1), 0u);
+        auto fbits = [](double v) {
+            const float f = (float)v;
+            unsigned u;
+            std::memcpy(&u, &f, 4);
+            return u;
+        };
+        for (int i = 0; i < tail.levels; ++i) {
+            const TailLevel& T = tail.lev[i];
+            const Level& L = levels[tail_start + i];
+            size_t o = (size_t)(T.lds_tables - tail.lds_tables_base);
+            auto put_i = [&](const std::vector<int>& v, size_t n) {
+                for (size_t k = 0; k < n; ++k)
+                    words[o + k] = k < v.size() ? (unsigned)v[k] : 0u;
+                o += n;
+            };
+            auto put_f = [&](const std::vector<double>& v, size_t n) {
+                for (size_t k = 0; k < n; ++k)
+                    words[o + k] = k < v.size() ? fbits(v[k]) : 0u;
+                o += n;
+            };
+            put_i(L.pxf, T.w);
+            put_i(L.pxs, T.w);
+            put_f(L.pxw, T.w);
+            put_i(L.pyf, T.h);
+            put_i(L.pys, T.h);
+            put_f(L.pyw, T.h);
+            put_i(L.rxc, T.cw);
+            put_i(L.rxi, 4 * (size_t)T.cw);
+            put_f(L.rxw, 4 * (size_t)T.cw);
+            put_i(L.ryc, T.ch);
+            put_i(L.ryi, 4 * (size_t)T.ch);
+            put_f(L.ryw, 4 * (size_t)T.ch);
+        }
+        upload_vec(tail_tables, words, bytes);
+        tail.tables = tail_tables.p;
+    }
+
     // PressureMultigrid constructor (PressureMultigrid.cpp:38-62): halve until w*h <= 64.
     void build_hierarchy()
     {
@@ -394,13 +447,19 @@ struct fs_grid {
                 size_t elements = 0;
                 for (int i = 0; i < tail.levels; ++i)
                     elements += (size_t)tail.lev[i].s * (tail.lev[i].h + 2);
-                // the tail kernel stages at most 8 padded elements per thread (8192 in all)
-                if (tail_layout(tail) > limit || elements > 8 * 1024)
+                // the tail kernel stages at most 8 padded elements and 2 table words per thread and
+                // gives each of its 1024 threads at most 4 cells (2 coarse cells) of a level
+                const Level& L = levels[l];
+                const Level& Cn = levels[l + 1];
+                const bool fits_threads = (long)L.w * L.h <= 4096 && (long)((L.w + 1) / 2) * L.h <= 4096 &&
+                                          (long)Cn.w * Cn.h <= 2048;
+                if (tail_layout(tail) > limit || elements > 8 * 1024 || tail.table_words > 2 * 1024 || !fits_threads)
                     break;
                 tail_start = l;
             }
             fill_tail(tail_start);
             tail_lds = tail_layout(tail);
+            build_tail_tables();
             if (tail_lds <= 65536 || tail_allow_lds(tail_lds) || limit <= 65536)
                 break;
             limit = 65536 - 4096;  // the device refused a larger dynamic LDS request
@@ -627,6 +686,7 @@ fs_status fs_context_create(int device, fs_context** out)
             ctx->lds_per_block = (size_t)lds;
         HIPC(hipStreamCreate(&ctx->stream));
         HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
+        HIPC(hipHostMalloc((void**)&ctx->pinned_value, sizeof(double), hipHostMallocDefault));
         HIPC(hipEventCreate(&ctx->ev0));
         HIPC(hipEventCreate(&ctx->ev1));
         *out = ctx.release();
@@ -647,6 +707,8 @@ fs_status fs_context_destroy(fs_context* ctx)
             (void)hipEventDestroy(ctx->ev1);
         if (ctx->pinned)
             (void)hipHostFree(ctx->pinned);
+        if (ctx->pinned_value)
+            (void)hipHostFree(ctx->pinned_value);
         if (ctx->stream)
             (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -944,6 +1006,122 @@ static void advect_sl(fs_grid& g, double dt)
     g.mg_valid = false;
 }
 
+// Most CFL substeps one transport call may take; beyond it the velocities are unphysical (the reference
+// would loop for as long as the ratio says, or forever at an infinite speed).
+constexpr double kMaxTransportSubsteps = 1e6;
+
+// FluidGridCPU::advect (FluidGridCPU.cpp:362-463).  Returns the number of CFL substeps.
+static int advect_reference(fs_grid& g, double dt)
+{
+    const size_t N = g.N(), NU = g.NU(), NV = g.NV();
+    const size_t nl = g.rho_l.size(), ng = g.rho_g.size();
+    auto ensure = [&](auto& b, size_t n) {
+        if (b.n != n || (n && !b.p))
+            g.bytes += b.alloc(n);
+    };
+    ensure(g.tr_fraction, N);
+    ensure(g.tr_lmass, nl * N);
+    ensure(g.tr_gmass, ng * N);
+    for (int k = 0; k < 2; ++k) {
+        ensure(g.tr_nu[k], NU);
+        ensure(g.tr_nv[k], NV);
+    }
+    ensure(g.tr_mom_u, NU);
+    ensure(g.tr_mom_v, NV);
+    ensure(g.tr_lt, std::max(NU, NV));
+    ensure(g.tr_mt, std::max(NU, NV));
+    ensure(g.tr_lcw, nl * N);
+    ensure(g.tr_gcw, ng * N);
+    ensure(g.tr_mcw, N);
+    ensure(g.tr_split, N);
+    ensure(g.tr_out, 1);
+
+    int cur = 0;   // tr_nu[cur] / tr_nv[cur] hold the transported velocities
+    auto view = [&]() {
+        TransportDev t{};
+        t.W = g.W;
+        t.H = g.H;
+        t.nl = (int)nl;
+        t.ng = (int)ng;
+        t.icv = 1.0 / (g.h * g.h);
+        t.u = g.u.p;
+        t.v = g.v.p;
+        t.f = g.fraction.p;
+        t.f_out = g.tr_fraction.p;
+        t.lm = g.lmass.p;
+        t.lm_out = g.tr_lmass.p;
+        t.gm = g.gmass.p;
+        t.gm_out = g.tr_gmass.p;
+        t.density = g.density.p;
+        t.nu_in = g.tr_nu[cur].p;
+        t.nu_out = g.tr_nu[cur ^ 1].p;
+        t.nv_in = g.tr_nv[cur].p;
+        t.nv_out = g.tr_nv[cur ^ 1].p;
+        t.mom_u = g.tr_mom_u.p;
+        t.mom_v = g.tr_mom_v.p;
+        t.lt = g.tr_lt.p;
+        t.mt = g.tr_mt.p;
+        t.lcw = g.tr_lcw.p;
+        t.gcw = g.tr_gcw.p;
+        t.mcw = g.tr_mcw.p;
+        t.split = g.tr_split.p;
+        t.partials = g.partials.p;
+        t.out = g.tr_out.p;
+        return t;
+    };
+    auto swap_state = [&]() {
+        g.fraction.swap(g.tr_fraction);
+        g.lmass.swap(g.tr_lmass);
+        g.gmass.swap(g.tr_gmass);
+        cur ^= 1;
+    };
+
+    // rebuildMomentum into tr_nu[0] / tr_nv[0], then the CFL limit
+    launch_density(g.st(), g.dev());
+    cur = 1;
+    launch_transport_begin(g.st(), view());
+    cur = 0;
+    HIPC(hipMemcpyAsync(g.ctx->pinned_value, g.tr_out.p, sizeof(double), hipMemcpyDeviceToHost, g.st()));
+    HIPC(hipStreamSynchronize(g.st()));
+    const double max_outflow = *g.ctx->pinned_value;
+    // no cell exports more than half its volume per substep (FluidGridCPU.cpp:388-389)
+    const double max_step = max_outflow > 0.0 ? 0.5 * g.h / max_outflow : dt;
+    if (!(max_step > 0.0) || dt / max_step > kMaxTransportSubsteps)
+        fail(FS_E_RUNTIME, "Transport would need more than 1e6 CFL substeps (non-finite or extreme velocity)");
+
+    double remaining = dt;
+    int substeps = 0;
+    while (remaining > 0.0) {
+        const double step = std::min(remaining, max_step);
+        const double scale = step / g.h;
+        launch_transport_freeze(g.st(), view());
+        launch_transport_pass(g.st(), view(), 1, scale);
+        swap_state();
+        launch_transport_pass(g.st(), view(), 0, scale);
+        swap_state();
+        remaining -= step;
+        ++substeps;
+    }
+    g.u.swap(g.tr_nu[cur]);
+    g.v.swap(g.tr_nv[cur]);
+    g.coeff_valid = false;
+    g.mg_valid = false;
+    return substeps;
+}
+
+fs_status fs_advect(fs_grid* grid, double dt, int* substeps)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before transport");
+        set_device(grid->ctx);
+        const int n = dt > 0.0 ? advect_reference(*grid, dt) : 0;
+        HIPC(hipStreamSynchronize(grid->st()));
+        if (substeps)
+            *substeps = n;
+    });
+}
+
 fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt)
 {
     return guard([&] {
@@ -972,10 +1150,6 @@ fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergenc
             return;
         }
         need(g.uploaded, FS_E_LOGIC, "Upload the grid state before stepping");
-        if (g.transport == FS_TRANSPORT_REFERENCE)
-            fail(FS_E_UNSUPPORTED,
-                "Reference PLIC/VOF transport is not implemented by the HIP backend yet; "
-                "select FS_TRANSPORT_NONE or FS_TRANSPORT_SEMI_LAGRANGIAN");
         set_device(g.ctx);
         using Clock = std::chrono::steady_clock;
         auto t0 = Clock::now();
@@ -988,7 +1162,9 @@ fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergenc
             t0 = now;
         };
         int transport_steps = 0;
-        if (g.transport == FS_TRANSPORT_SEMI_LAGRANGIAN) {
+        if (g.transport == FS_TRANSPORT_REFERENCE) {
+            advect_reference(g, dt);   // UniformCPU reports no transport subdivision count
+        } else if (g.transport == FS_TRANSPORT_SEMI_LAGRANGIAN) {
             advect_sl(g, dt);
             transport_steps = 1;
         }
@@ -1271,7 +1447,16 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
         const int l1 = std::min(1, g.tail_start);
         auto launch_once = [&](int i) {
             switch (kernel) {
-            case 0: launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 0, 0, CoarseLink{}, nullptr, 1); break;
+            case 0: {  // the finest post-smoothing launch of a V-cycle (prolongation + four sweeps)
+                CoarseLink link{};
+                if (g.tail_start > 0) {
+                    const Level& F = g.levels[0];
+                    const Level& Cl = g.levels[1];
+                    link = CoarseLink{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
+                }
+                launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 1, 0, link, nullptr, 1);
+                break;
+            }
             case 1: launch_pcg_ad(g.st(), gd); break;
             case 2: launch_pcg_update(g.st(), gd); break;
             case 3: launch_copy_f32(g.st(), g.r.p, g.q.p, g.NP()); break;

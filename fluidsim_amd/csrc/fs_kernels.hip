@@ -99,12 +99,29 @@ __global__ __launch_bounds__(kNT) void k_density(GridDev g)
 // Conductance of the face right of / below each cell: 2 / (rho_a + rho_b), zero at walls.
 __global__ __launch_bounds__(kNT) void k_face_weights(GridDev g)
 {
-    FS_TILE_BEGIN(g.W, g.H)
-    const int i = y * g.W + x;
-    const int P = pidx(x, y, g.s);
-    g.right64[P] = x + 1 < g.W ? 2.0 / (g.density[i] + g.density[i + 1]) : 0.0;
-    g.down64[P] = y + 1 < g.H ? 2.0 / (g.density[i] + g.density[i + g.W]) : 0.0;
-    FS_TILE_END
+    const int W = g.W, H = g.H;
+    const int x = blockIdx.x * kTX + (threadIdx.x & 63);
+    const int xc = min(x, W - 1), xr = min(x + 1, W - 1);
+    double c[4], r[4], d[4];
+    int yy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int y = blockIdx.y * kTY + (threadIdx.x >> 6) + 4 * k;
+        yy[k] = y;
+        const int yc = min(y, H - 1), yd = min(y + 1, H - 1);
+        c[k] = g.density[yc * W + xc];
+        r[k] = g.density[yc * W + xr];
+        d[k] = g.density[yd * W + xc];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int y = yy[k];
+        if (x < W && y < H) {
+            const int P = pidx(x, y, g.s);
+            g.right64[P] = x + 1 < W ? 2.0 / (c[k] + r[k]) : 0.0;
+            g.down64[P] = y + 1 < H ? 2.0 / (c[k] + d[k]) : 0.0;
+        }
+    }
 }
 
 // PressureMultigrid Level::rebuildDiagonal (PressureMultigrid.cpp:104-117) in FP64, then FP32 copies.
@@ -112,13 +129,26 @@ __global__ __launch_bounds__(kNT) void k_level_diag(int w, int h, const double* 
     float* right32, float* down32, float* invd32)
 {
     const int s = w + 2;
-    FS_TILE_BEGIN(w, h)
-    const int i = pidx(x, y, s);
-    const double diagonal = right64[i - 1] + right64[i] + down64[i - s] + down64[i];
-    invd32[i] = (float)(diagonal > 0.0 ? 1.0 / diagonal : 0.0);
-    right32[i] = (float)right64[i];
-    down32[i] = (float)down64[i];
-    FS_TILE_END
+    const Tile4 T = tile4(w, h, s);
+    double rl[4], rr[4], du[4], dd[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = T.P[k];
+        rl[k] = right64[i - 1];
+        rr[k] = right64[i];
+        du[k] = down64[i - s];
+        dd[k] = down64[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (T.in[k]) {
+            const int i = T.P[k];
+            const double diagonal = rl[k] + rr[k] + du[k] + dd[k];
+            invd32[i] = (float)(diagonal > 0.0 ? 1.0 / diagonal : 0.0);
+            right32[i] = (float)rr[k];
+            down32[i] = (float)dd[k];
+        }
+    }
 }
 
 // PressureMultigrid::coarsen (PressureMultigrid.cpp:142-202): parallel paths add, series resistances
@@ -172,27 +202,48 @@ __global__ __launch_bounds__(kNT) void k_coarsen(int fw, int fh, const double* f
 __global__ __launch_bounds__(kNT) void k_rhs(GridDev g, double scale)
 {
     __shared__ double lds[kNT / 64];
+    const int W = g.W, H = g.H;
+    const int x = blockIdx.x * kTX + (threadIdx.x & 63);
+    const int xc = min(x, W - 1);
+    double uL[4], uR[4], vT[4], vB[4];
+    int yy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int y = blockIdx.y * kTY + (threadIdx.x >> 6) + 4 * k;
+        yy[k] = y;
+        const int yc = min(y, H - 1);
+        const double* hz = g.u + (size_t)yc * (W + 1);
+        const double* vt = g.v + (size_t)yc * W;
+        uL[k] = hz[xc];
+        uR[k] = hz[xc + 1];
+        vT[k] = vt[xc];
+        vB[k] = vt[xc + W];
+    }
     if (g.S->status != kRunning)
         return;
     double sum = 0.0;
-    const int W = g.W, H = g.H;
-    FS_TILE_BEGIN(W, H)
-    double* hz = g.u + (size_t)y * (W + 1);
-    double* vt = g.v + (size_t)y * W;
-    const double uL = x == 0 ? 0.0 : hz[x];
-    const double uR = x + 1 == W ? 0.0 : hz[x + 1];
-    const double vT = y == 0 ? 0.0 : vt[x];
-    const double vB = y + 1 == H ? 0.0 : vt[x + W];
-    if (x == 0) hz[0] = 0.0;
-    if (x + 1 == W) hz[W] = 0.0;
-    if (y == 0) vt[x] = 0.0;
-    if (y + 1 == H) vt[x + W] = 0.0;
-    const double du = uR - uL;
-    const double dv = vB - vT;
-    const double rhs = -(du + dv) / g.h * scale;
-    g.b[pidx(x, y, g.s)] = rhs;
-    sum += rhs;
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int y = yy[k];
+        if (x < W && y < H) {
+            // walls are zeroed and read as zero (FluidGridCPU.cpp:1050-1061)
+            double* hz = g.u + (size_t)y * (W + 1);
+            double* vt = g.v + (size_t)y * W;
+            const double l = x == 0 ? 0.0 : uL[k];
+            const double r = x + 1 == W ? 0.0 : uR[k];
+            const double t = y == 0 ? 0.0 : vT[k];
+            const double b = y + 1 == H ? 0.0 : vB[k];
+            if (x == 0) hz[0] = 0.0;
+            if (x + 1 == W) hz[W] = 0.0;
+            if (y == 0) vt[x] = 0.0;
+            if (y + 1 == H) vt[x + W] = 0.0;
+            const double du = r - l;
+            const double dv = b - t;
+            const double rhs = -(du + dv) / g.h * scale;
+            g.b[pidx(x, y, g.s)] = rhs;
+            sum += rhs;
+        }
+    }
     const double bs = block_sum<kNT>(sum, lds);
     if (threadIdx.x == 0)
         g.partials[block_id()] = bs;
@@ -202,16 +253,23 @@ __global__ __launch_bounds__(kNT) void k_rhs(GridDev g, double scale)
 __global__ __launch_bounds__(kNT) void k_rhs_center(GridDev g)
 {
     __shared__ double lds[kNT / 64];
+    const Tile4 T = tile4(g.W, g.H, g.s);
+    double bv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        bv[k] = g.b[T.P[k]];
     if (g.S->status != kRunning)
         return;
     const double mean = g.S->b_mean;
     double m = 0.0;
-    FS_TILE_BEGIN(g.W, g.H)
-    const int P = pidx(x, y, g.s);
-    const double v = g.b[P] - mean;
-    g.b[P] = v;
-    m = fmax(m, abs_or_inf(v));
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (T.in[k]) {
+            const double v = bv[k] - mean;
+            g.b[T.P[k]] = v;
+            m = fmax(m, abs_or_inf(v));
+        }
+    }
     const double bm = block_max<kNT>(m, lds);
     if (threadIdx.x == 0)
         g.partials[block_id()] = bm;
@@ -233,17 +291,25 @@ __global__ __launch_bounds__(kNT) void k_zero_pressure(GridDev g)
 __global__ __launch_bounds__(kNT) void k_predict(GridDev g)
 {
     __shared__ double lds[kNT / 64];
+    const Tile4 T = tile4(g.W, g.H, g.s);
+    double pv[4], qv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        pv[k] = g.p[T.P[k]];
+        qv[k] = g.p_prev[T.P[k]];
+    }
     if (g.S->status != kRunning)
         return;
     double sum = 0.0;
-    FS_TILE_BEGIN(g.W, g.H)
-    const int P = pidx(x, y, g.s);
-    const double latest = g.p[P];
-    const double next = 2.0 * latest - g.p_prev[P];
-    g.p[P] = next;
-    g.p_prev[P] = latest;
-    sum += next;
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (T.in[k]) {
+            const double next = 2.0 * pv[k] - qv[k];
+            g.p[T.P[k]] = next;
+            g.p_prev[T.P[k]] = pv[k];
+            sum += next;
+        }
+    }
     const double bs = block_sum<kNT>(sum, lds);
     if (threadIdx.x == 0)
         g.partials[block_id()] = bs;
@@ -251,14 +317,19 @@ __global__ __launch_bounds__(kNT) void k_predict(GridDev g)
 
 __global__ __launch_bounds__(kNT) void k_sub_mean_p(GridDev g, int final_pass)
 {
+    const Tile4 T = tile4(g.W, g.H, g.s);
+    double pv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        pv[k] = g.p[T.P[k]];
     const int st = g.S->status;
     if (final_pass ? !(st == kConverged || st == kStopped) : st != kRunning)
         return;
     const double mean = g.S->p_mean;
-    FS_TILE_BEGIN(g.W, g.H)
-    const int P = pidx(x, y, g.s);
-    g.p[P] -= mean;
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (T.in[k])
+            g.p[T.P[k]] = pv[k] - mean;
 }
 
 // True residual r = b - A p in FP64 (FP64 weights), stored as the FP32 working residual.
@@ -326,17 +397,27 @@ __global__ __launch_bounds__(kNT) void k_flush(GridDev g)
 __global__ __launch_bounds__(kNT) void k_final_flush_sum(GridDev g)
 {
     __shared__ double lds[kNT / 64];
+    const Tile4 T = tile4(g.W, g.H, g.s);
+    double pv[4];
+    float xv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        pv[k] = g.p[T.P[k]];
+        xv[k] = g.x[T.P[k]];
+    }
     const int st = g.S->status;
     if (!(st == kConverged || st == kStopped))
         return;
     double sum = 0.0;
-    FS_TILE_BEGIN(g.W, g.H)
-    const int P = pidx(x, y, g.s);
-    const double v = g.p[P] + (double)g.x[P];
-    g.p[P] = v;
-    g.x[P] = 0.0f;
-    sum += v;
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (T.in[k]) {
+            const double v = pv[k] + (double)xv[k];
+            g.p[T.P[k]] = v;
+            g.x[T.P[k]] = 0.0f;
+            sum += v;
+        }
+    }
     const double bs = block_sum<kNT>(sum, lds);
     if (threadIdx.x == 0)
         g.partials[block_id()] = bs;
@@ -346,30 +427,64 @@ __global__ __launch_bounds__(kNT) void k_final_flush_sum(GridDev g)
 // u face and its top v face.
 __global__ __launch_bounds__(kNT) void k_correct_velocity(GridDev g, double dt)
 {
+    const int W = g.W, s = g.s;
+    const Tile4 T = tile4(W, g.H, s);
+    const int x = blockIdx.x * kTX + (threadIdx.x & 63);
+    const int xc = min(x, W - 1);
+    double pc[4], pl[4], pu[4], wl[4], wu[4], uv[4], vv[4];
+    int yy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int P = T.P[k];
+        const int y = blockIdx.y * kTY + (threadIdx.x >> 6) + 4 * k;
+        yy[k] = y;
+        const int yc = min(y, g.H - 1);
+        pc[k] = g.p[P];
+        pl[k] = g.p[P - 1];
+        pu[k] = g.p[P - s];
+        wl[k] = g.right64[P - 1];
+        wu[k] = g.down64[P - s];
+        uv[k] = g.u[(size_t)yc * (W + 1) + xc];
+        vv[k] = g.v[(size_t)yc * W + xc];
+    }
     if (g.S->status == kFailed)
         return;
-    const int W = g.W, s = g.s;
-    FS_TILE_BEGIN(W, g.H)
-    const int P = pidx(x, y, s);
-    if (x >= 1)
-        g.u[(size_t)y * (W + 1) + x] -= ((g.p[P] - g.p[P - 1]) / g.h) * dt * g.right64[P - 1];
-    if (y >= 1)
-        g.v[(size_t)y * W + x] -= ((g.p[P] - g.p[P - s]) / g.h) * dt * g.down64[P - s];
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!T.in[k])
+            continue;
+        const int y = yy[k];
+        // u -= ((p - p_left)/h) dt w ; v -= ((p - p_up)/h) dt w  (FluidGridCPU.cpp:1098-1118)
+        if (x >= 1)
+            g.u[(size_t)y * (W + 1) + x] = uv[k] - ((pc[k] - pl[k]) / g.h) * dt * wl[k];
+        if (y >= 1)
+            g.v[(size_t)y * W + x] = vv[k] - ((pc[k] - pu[k]) / g.h) * dt * wu[k];
+    }
 }
 
 // maxDivergence (FluidGridCPU.cpp:1126-1152): non-finite divergence reports infinity.
 __global__ __launch_bounds__(kNT) void k_max_divergence(GridDev g)
 {
     __shared__ double lds[kNT / 64];
-    const int W = g.W;
+    const int W = g.W, H = g.H;
+    const int x = blockIdx.x * kTX + (threadIdx.x & 63);
+    const int xc = min(x, W - 1);
+    double dv[4];
+    bool in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int y = blockIdx.y * kTY + (threadIdx.x >> 6) + 4 * k;
+        in[k] = x < W && y < H;
+        const int yc = min(y, H - 1);
+        const double* hz = g.u + (size_t)yc * (W + 1);
+        const double* vt = g.v + (size_t)yc * W;
+        dv[k] = ((hz[xc + 1] - hz[xc]) + (vt[xc + W] - vt[xc])) / g.h;
+    }
     double m = 0.0;
-    FS_TILE_BEGIN(W, g.H)
-    const double* hz = g.u + (size_t)y * (W + 1);
-    const double* vt = g.v + (size_t)y * W;
-    const double divergence = ((hz[x + 1] - hz[x]) + (vt[x + W] - vt[x])) / g.h;
-    m = fmax(m, abs_or_inf(divergence));
-    FS_TILE_END
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (in[k])
+            m = fmax(m, abs_or_inf(dv[k]));
     const double bm = block_max<kNT>(m, lds);
     if (threadIdx.x == 0)
         g.partials[block_id()] = bm;

@@ -54,7 +54,7 @@ struct TailLevel {
     const int* rxc; const int* rxi; const float* rxw;
     const int* ryc; const int* ryi; const float* ryw;
     int lds_arrays;      // word offset of right, down, rhs, sol, res (each s * (h + 2))
-    int lds_tables;      // word offset of the transfer tables (3w + 3h + 9cw + 9ch words)
+    int lds_tables;      // word offset of this level's transfer tables (3w + 3h + 9cw + 9ch words)
 };
 
 constexpr int kMaxTailLevels = 8;
@@ -65,6 +65,9 @@ struct TailDesc {
     const float* top_rhs;      // global rhs of the first tail level (the FP32 residual on level 0)
     float* top_sol;            // global solution of the first tail level
     int lds_ginv;              // word offset of ginv in LDS
+    const unsigned* tables;    // every level's transfer tables, in their LDS layout (host-built)
+    int table_words;
+    int lds_tables_base;       // word offset of the table block in LDS
     int lds_words;             // total LDS words
     TailLevel lev[kMaxTailLevels];
 };
@@ -122,6 +125,37 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
     const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status);
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status);
+
+// ---- reference PLIC/VOF transport (FP64, reference layouts, bit-exact) ----
+// One directional pass reads the frozen flow (u, v), the fractions and masses of the pass start and the
+// transported velocities nu_in/nv_in; it writes new fractions/masses/density into the *_out buffers, the
+// recovered velocities into nu_out/nv_out and updates the momenta in place.  The host swaps the pairs.
+struct TransportDev {
+    int W, H, nl, ng;
+    double icv;                            // 1 / h^2
+    const double* u; const double* v;      // flow, frozen for the whole transport call
+    const double* f; double* f_out;        // liquid fraction N
+    const double* lm; double* lm_out;      // liquid masses nl * N
+    const double* gm; double* gm_out;      // gas masses ng * N
+    double* density;                       // N, rebuilt from the new masses
+    const double* nu_in; double* nu_out;   // transported velocities (W+1)*H
+    const double* nv_in; double* nv_out;   // W*(H+1)
+    double* mom_u; double* mom_v;          // momenta
+    double* lt; double* mt;                // liquid volume and total mass transfer of the pass's faces
+    double* lcw; double* gcw; double* mcw; // frozen correction weights (nl*N, ng*N, N)
+    unsigned char* split;                  // frozen phase selection (fraction > 0.5)
+    double* partials;                      // >= one double per 64 x 16 cell tile
+    double* out;                           // one device double: max outflow
+};
+// Density, momenta and transported velocities from u/v (rebuildMomentum), then the largest cell
+// outflow into *out (FluidGridCPU.cpp:1425-1476, 366-386).
+void launch_transport_begin(hipStream_t st, const TransportDev& t);
+// freezeComponentCorrections (FluidGridCPU.cpp:1307-1349).
+void launch_transport_freeze(hipStream_t st, const TransportDev& t);
+// One X (horizontal != 0) or Y pass of one substep: interfaces + liquid and component transfers,
+// component/fraction update with density, momentum transfer and velocity recovery
+// (FluidGridCPU.cpp:401-456).
+void launch_transport_pass(hipStream_t st, const TransportDev& t, int horizontal, double scale);
 
 // ---- bench helper: one operator application without reductions (for event timing) ----
 void launch_copy_f32(hipStream_t st, const float* src, float* dst, size_t n);

@@ -31,10 +31,13 @@ __device__ __forceinline__ float inv_or_zero(float d) { return d > 0.f ? __built
 // updating 64 same-colour cells of a row touches 64 consecutive LDS words (no bank conflicts).  Each
 // thread owns fixed (row, pair) slots; the four conductances, rhs and 1/diagonal of its cells stay in
 // registers for all eight half-sweeps, so an update reads four neighbours from LDS and writes one value.
-template <int TX, int TY, int NS, int FINE>
+// MODE 0: pre-smoothing from a zero guess (no solution read); 1: post-smoothing, the prolonged coarse
+// correction added to the solution first; 2: plain sweeps of the stored solution.
+template <int TX, int TY, int NS, int FINE, int MODE>
 __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift,
-    int c0, int from_zero, CoarseLink link, const int* status)
+    int c0, CoarseLink link, const int* status)
 {
+    constexpr bool from_zero = MODE == 0;
     constexpr int HALO = 2 * NS;
     constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
     constexpr int HX = LX / 2;                       // colour-array row width
@@ -73,24 +76,39 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
     constexpr int LOADS = (NE + kNT - 1) / kNT;
     float vs[LOADS];
     bool vin[LOADS];
+    int ix[LOADS], iy[LOADS];
 #pragma unroll
     for (int it = 0; it < LOADS; ++it) {
         const int e = threadIdx.x + it * kNT;
         const int ly = e / LX, lx = e - ly * LX;
         const int x = x0 + lx, y = y0 + ly;
         vin[it] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
-        const int xc = min(max(x, 0), w - 1), yc = min(max(y, 0), h - 1);
-        float v = from_zero ? 0.f : sol_in[(yc + 1) * s + (xc + 1)];
-        if (link.csol) {
-            const float wx = link.pxw[xc], wy = link.pyw[yc];
-            const int a0 = link.pxf[xc], a1 = link.pxs[xc];
-            const float* top = link.csol + (link.pyf[yc] + 1) * link.cs + 1;
-            const float* bot = link.csol + (link.pys[yc] + 1) * link.cs + 1;
-            const float a = (1.f - wx) * top[a0] + wx * top[a1];
-            const float b = (1.f - wx) * bot[a0] + wx * bot[a1];
-            v += (1.f - wy) * a + wy * b;
+        ix[it] = min(max(x, 0), w - 1);
+        iy[it] = min(max(y, 0), h - 1);
+        vs[it] = MODE == 0 ? 0.f : sol_in[(iy[it] + 1) * s + (ix[it] + 1)];
+    }
+    if (MODE == 1) {
+        // two batched phases: transfer tables, then the four coarse values of every cell
+        int a0[LOADS], a1[LOADS], b0[LOADS], b1[LOADS];
+        float wx[LOADS], wy[LOADS];
+#pragma unroll
+        for (int it = 0; it < LOADS; ++it) {
+            wx[it] = link.pxw[ix[it]];
+            wy[it] = link.pyw[iy[it]];
+            a0[it] = link.pxf[ix[it]];
+            a1[it] = link.pxs[ix[it]];
+            b0[it] = link.pyf[iy[it]];
+            b1[it] = link.pys[iy[it]];
         }
-        vs[it] = v;
+#pragma unroll
+        for (int it = 0; it < LOADS; ++it) {
+            const float* top = link.csol + (b0[it] + 1) * link.cs + 1;
+            const float* bot = link.csol + (b1[it] + 1) * link.cs + 1;
+            const float t0 = top[a0[it]], t1 = top[a1[it]], u0 = bot[a0[it]], u1 = bot[a1[it]];
+            const float a = (1.f - wx[it]) * t0 + wx[it] * t1;
+            const float b = (1.f - wx[it]) * u0 + wx[it] * u1;
+            vs[it] += (1.f - wy[it]) * a + wy[it] * b;
+        }
     }
     if (status && *status != kRunning)
         return;
@@ -127,32 +145,32 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
         const float* other = s_col[color ^ 1];
         float n0[ROUNDS], n1[ROUNDS], n2[ROUNDS], n3[ROUNDS];
         bool act[ROUNDS];
+        int dst[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
-            const int ly = cly[r];
+            const int ly = max(cly[r], 1);        // unused slots read a valid row and are masked
             const int p = (color + ly) & 1;
             const int jj = cj[r];
             const int lx = 2 * jj + p;
             const int x = x0 + lx, y = y0 + ly;
-            act[r] = ly >= lo && ly < LY - lo && lx >= lo && lx < LX - lo && x >= 0 && x < w && y >= 0 && y < h;
-            n0[r] = n1[r] = n2[r] = n3[r] = 0.f;
-            if (act[r] && neighbours) {
-                const int row = ly * HX;
-                n0[r] = other[row + jj - 1 + p];
-                n1[r] = other[row + jj + p];
-                n2[r] = other[row - HX + jj];
-                n3[r] = other[row + HX + jj];
-            }
+            act[r] = cly[r] >= lo && cly[r] < LY - lo && lx >= lo && lx < LX - lo && x >= 0 && x < w && y >= 0 &&
+                     y < h;
+            const int row = ly * HX;
+            dst[r] = row + jj;
+            // unconditional LDS reads at clamped, always-valid indices
+            n0[r] = other[row + max(jj - 1 + p, 0)];
+            n1[r] = other[row + min(jj + p, HX - 1)];
+            n2[r] = other[row - HX + jj];
+            n3[r] = other[row + HX + jj];
         }
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
-            if (!act[r])
-                continue;
             const float a = cl[color][r], b = cr[color][r], u = cu[color][r], d = cd[color][r];
             float v = cb[color][r];
             if (neighbours)
                 v += (a * n0[r] + b * n1[r]) + (u * n2[r] + d * n3[r]);
-            mine[cly[r] * HX + cj[r]] = v * inv_or_zero((a + b) + (u + d));
+            if (act[r])
+                mine[dst[r]] = v * inv_or_zero((a + b) + (u + d));
         }
         __syncthreads();
     }
@@ -361,15 +379,15 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
         {
             // rows i = k+1+tr+4m; gather every operand first, then write (independent LDS accesses)
             const int c = k + 1 + tc;
-            const double qc = c < n ? q[c] : 0.0;
+            const int cc = min(c, n - 1);
+            const double qc = q[cc];
             constexpr int M = (kMaxCoarse + RSTEP - 1) / RSTEP;
             double li[M], av[M];
 #pragma unroll
             for (int mm = 0; mm < M; ++mm) {
-                const int i = k + 1 + tr + RSTEP * mm;
-                const bool on = i < n && c <= i;
-                li[mm] = on ? q[i] : 0.0;
-                av[mm] = on ? a[i * n + c] : 0.0;
+                const int ic = min(k + 1 + tr + RSTEP * mm, n - 1);
+                li[mm] = q[ic];
+                av[mm] = a[ic * n + cc];
             }
 #pragma unroll
             for (int mm = 0; mm < M; ++mm) {
@@ -404,14 +422,14 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
         __syncthreads();
         {
             constexpr int M = (kMaxCoarse + RSTEP - 1) / RSTEP;
-            const double pk = tc < n ? m[k * n + tc] : 0.0;
+            const int tcc = min(tc, n - 1);
+            const double pk = m[k * n + tcc];
             double fv[M], mv[M];
 #pragma unroll
             for (int mm = 0; mm < M; ++mm) {
-                const int i = tr + RSTEP * mm;
-                const bool on = i < n && i != k && tc < n;
-                fv[mm] = on ? q[i] : 0.0;
-                mv[mm] = on ? m[i * n + tc] : 0.0;
+                const int ic = min(tr + RSTEP * mm, n - 1);
+                fv[mm] = q[ic];
+                mv[mm] = m[ic * n + tcc];
             }
 #pragma unroll
             for (int mm = 0; mm < M; ++mm) {
@@ -469,25 +487,40 @@ __device__ __forceinline__ TailArrays tail_arrays(float* lds, const TailLevel& T
     return A;
 }
 
+// Tail loops: a thread owns at most kTailIter items of a level; every LDS read goes through a clamped
+// (valid) index and only the stores are predicated, so the reads of all items issue together.
+constexpr int kTailIter = 4;
+
 __device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
 {
     const int w = T.w, h = T.h, s = T.s;
     const int pairs = (w + 1) >> 1;
+    const int items = pairs * h;
     for (int pass = 0; pass < 2; ++pass) {
         const int color = reverse ? 1 - pass : pass;
         const bool neighbours = !(from_zero && pass == 0);
-        for (int t = threadIdx.x; t < pairs * h; t += kTailThreads) {
+        float v[kTailIter];
+        int idx[kTailIter];
+        bool on[kTailIter];
+#pragma unroll
+        for (int it = 0; it < kTailIter; ++it) {
+            const int t = min((int)threadIdx.x + it * kTailThreads, items - 1);
             const int y = t / pairs;
             const int x = 2 * (t - y * pairs) + ((color ^ y) & 1);
-            if (x >= w)
-                continue;
-            const int i = (y + 1) * s + x + 1;
+            on[it] = (int)threadIdx.x + it * kTailThreads < items && x < w;
+            const int i = (y + 1) * s + min(x, w - 1) + 1;
+            idx[it] = i;
             const float rl = A.right[i - 1], rr = A.right[i], du = A.down[i - s], dd = A.down[i];
-            float v = A.rhs[i];
+            float acc = A.rhs[i];
+            const float n = (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
             if (neighbours)
-                v += (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
-            A.sol[i] = v * inv_or_zero((rl + rr) + (du + dd));
+                acc += n;
+            v[it] = acc * inv_or_zero((rl + rr) + (du + dd));
         }
+#pragma unroll
+        for (int it = 0; it < kTailIter; ++it)
+            if (on[it])
+                A.sol[idx[it]] = v[it];
         __syncthreads();
     }
 }
@@ -496,27 +529,55 @@ __device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, 
     const TailArrays& Cd)
 {
     const int w = T.w, h = T.h, s = T.s;
-    for (int e = threadIdx.x; e < w * h; e += kTailThreads) {
-        const int y = e / w, x = e - y * w;
-        const int i = (y + 1) * s + x + 1;
-        const float c = F.sol[i];
-        const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
-                              (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
-        F.res[i] = F.rhs[i] - product;
+    const int cells = w * h;
+    {
+        float rv[kTailIter];
+        int idx[kTailIter];
+        bool on[kTailIter];
+#pragma unroll
+        for (int it = 0; it < kTailIter; ++it) {
+            const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
+            on[it] = (int)threadIdx.x + it * kTailThreads < cells;
+            const int y = e / w, x = e - y * w;
+            const int i = (y + 1) * s + x + 1;
+            idx[it] = i;
+            const float c = F.sol[i];
+            const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
+                                  (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
+            rv[it] = F.rhs[i] - product;
+        }
+#pragma unroll
+        for (int it = 0; it < kTailIter; ++it)
+            if (on[it])
+                F.res[idx[it]] = rv[it];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < Cn.w * Cn.h; e += kTailThreads) {
-        const int y = e / Cn.w, x = e - y * Cn.w;
-        const int nx = F.rxc[x], ny = F.ryc[y];
-        float value = 0.f;
-        for (int iy = 0; iy < ny; ++iy) {
-            const float* row = F.res + (F.ryi[y * 4 + iy] + 1) * s + 1;
-            float horizontal = 0.f;
-            for (int ix = 0; ix < nx; ++ix)
-                horizontal += F.rxw[x * 4 + ix] * row[F.rxi[x * 4 + ix]];
-            value += F.ryw[y * 4 + iy] * horizontal;
+    {
+        // missing taps carry weight 0 and index -1, which addresses the zero halo
+        const int cells_c = Cn.w * Cn.h;
+        float val[2];
+        int dst[2];
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int e = min((int)threadIdx.x + it * kTailThreads, cells_c - 1);
+            const int y = e / Cn.w, x = e - y * Cn.w;
+            float value = 0.f;
+#pragma unroll
+            for (int iy = 0; iy < 4; ++iy) {
+                const float* row = F.res + (F.ryi[y * 4 + iy] + 1) * s + 1;
+                float horizontal = 0.f;
+#pragma unroll
+                for (int ix = 0; ix < 4; ++ix)
+                    horizontal += F.rxw[x * 4 + ix] * row[F.rxi[x * 4 + ix]];
+                value += F.ryw[y * 4 + iy] * horizontal;
+            }
+            val[it] = value;
+            dst[it] = (y + 1) * Cn.s + x + 1;
         }
-        Cd.rhs[(y + 1) * Cn.s + x + 1] = value;
+#pragma unroll
+        for (int it = 0; it < 2; ++it)
+            if ((int)threadIdx.x + it * kTailThreads < cells_c)
+                Cd.rhs[dst[it]] = val[it];
     }
     __syncthreads();
 }
@@ -524,7 +585,14 @@ __device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, 
 __device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const TailLevel& Cn, const TailArrays& Cd)
 {
     const int w = T.w, h = T.h;
-    for (int e = threadIdx.x; e < w * h; e += kTailThreads) {
+    const int cells = w * h;
+    float add[kTailIter], cur[kTailIter];
+    int idx[kTailIter];
+    bool on[kTailIter];
+#pragma unroll
+    for (int it = 0; it < kTailIter; ++it) {
+        const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
+        on[it] = (int)threadIdx.x + it * kTailThreads < cells;
         const int y = e / w, x = e - y * w;
         const float wy = F.pyw[y];
         const float* top = Cd.sol + (F.pyf[y] + 1) * Cn.s + 1;
@@ -533,8 +601,14 @@ __device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const Tail
         const float wx = F.pxw[x];
         const float a = (1.f - wx) * top[a0] + wx * top[a1];
         const float b = (1.f - wx) * bottom[a0] + wx * bottom[a1];
-        F.sol[(y + 1) * T.s + x + 1] += (1.f - wy) * a + wy * b;
+        add[it] = (1.f - wy) * a + wy * b;
+        idx[it] = (y + 1) * T.s + x + 1;
+        cur[it] = F.sol[idx[it]];
     }
+#pragma unroll
+    for (int it = 0; it < kTailIter; ++it)
+        if (on[it])
+            F.sol[idx[it]] = cur[it] + add[it];
     __syncthreads();
 }
 
@@ -616,36 +690,37 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
         A.sol[e] = 0.f;
         A.res[e] = 0.f;
     }
-    for (int l = 0; l + 1 < nl; ++l) {
-        const TailLevel& T = D.lev[l];
-        const TailArrays A = tail_arrays(lds, T);
-        for (int e = threadIdx.x; e < T.w; e += kTailThreads) {
-            A.pxf[e] = T.pxf[e];
-            A.pxs[e] = T.pxs[e];
-            A.pxw[e] = T.pxw[e];
+    {
+        // transfer tables (host-built, already in LDS layout) and the coarse inverse: flat copies with
+        // clamped, unconditional loads
+        constexpr int TT = 2, TG = 4;
+        unsigned tv[TT];
+        float gv[TG];
+        const int n = D.coarse_n;
+        const int nn = n * n;
+#pragma unroll
+        for (int it = 0; it < TT; ++it)
+            tv[it] = D.tables[min(threadIdx.x + it * kTailThreads, (unsigned)max(D.table_words - 1, 0))];
+#pragma unroll
+        for (int it = 0; it < TG; ++it)
+            gv[it] = D.ginv[min(threadIdx.x + it * kTailThreads, (unsigned)max(nn - 1, 0))];
+        unsigned* tables = (unsigned*)(lds + D.lds_tables_base);
+#pragma unroll
+        for (int it = 0; it < TT; ++it) {
+            const int e = threadIdx.x + it * kTailThreads;
+            if (e < D.table_words)
+                tables[e] = tv[it];
         }
-        for (int e = threadIdx.x; e < T.h; e += kTailThreads) {
-            A.pyf[e] = T.pyf[e];
-            A.pys[e] = T.pys[e];
-            A.pyw[e] = T.pyw[e];
-        }
-        for (int e = threadIdx.x; e < T.cw; e += kTailThreads)
-            A.rxc[e] = T.rxc[e];
-        for (int e = threadIdx.x; e < 4 * T.cw; e += kTailThreads) {
-            A.rxi[e] = T.rxi[e];
-            A.rxw[e] = T.rxw[e];
-        }
-        for (int e = threadIdx.x; e < T.ch; e += kTailThreads)
-            A.ryc[e] = T.ryc[e];
-        for (int e = threadIdx.x; e < 4 * T.ch; e += kTailThreads) {
-            A.ryi[e] = T.ryi[e];
-            A.ryw[e] = T.ryw[e];
+        float* Gl = lds + D.lds_ginv;
+#pragma unroll
+        for (int it = 0; it < TG; ++it) {
+            const int e = threadIdx.x + it * kTailThreads;
+            if (e < nn)
+                Gl[e] = gv[it];
         }
     }
     float* G = lds + D.lds_ginv;
     const int n = D.coarse_n;
-    for (int e = threadIdx.x; e < n * n; e += kTailThreads)
-        G[e] = D.ginv[e];
     __syncthreads();
     for (int l = 0; l + 1 < nl; ++l) {
         const TailLevel& T = D.lev[l];
@@ -684,11 +759,18 @@ size_t tail_layout(TailDesc& t)
         TailLevel& T = t.lev[l];
         T.lds_arrays = words;
         words += 5 * T.s * (T.h + 2);
-        T.lds_tables = words;
-        words += 3 * T.w + 3 * T.h + 9 * T.cw + 9 * T.ch;
     }
     t.lds_ginv = words;
     words += t.coarse_n * t.coarse_n;
+    t.lds_tables_base = words;
+    int tw = 0;
+    for (int l = 0; l < t.levels; ++l) {
+        TailLevel& T = t.lev[l];
+        T.lds_tables = words + tw;
+        tw += 3 * T.w + 3 * T.h + 9 * T.cw + 9 * T.ch;
+    }
+    t.table_words = tw;
+    words += tw;
     t.lds_words = words;
     return (size_t)words * 4;
 }
@@ -703,12 +785,20 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
     int first_color, int from_zero, const CoarseLink& link, const int* status, int fine)
 {
     const dim3 grid((L.w + kMultiTX - 1) / kMultiTX, (L.h + kMultiTY - 1) / kMultiTY);
-    if (fine)
-        hipLaunchKernelGGL((k_rb_multi<kMultiTX, kMultiTY, kMultiNS, 1>), grid, dim3(kNT), 0, st, L, cur, rhs, shift,
-            first_color, from_zero, link, status);
-    else
-        hipLaunchKernelGGL((k_rb_multi<kMultiTX, kMultiTY, kMultiNS, 0>), grid, dim3(kNT), 0, st, L, cur, rhs, shift,
-            first_color, from_zero, link, status);
+    const int mode = from_zero ? 0 : (link.csol ? 1 : 2);
+#define FS_RB(F, M)                                                                                          \
+    hipLaunchKernelGGL((k_rb_multi<kMultiTX, kMultiTY, kMultiNS, F, M>), grid, dim3(kNT), 0, st, L, cur, rhs, \
+        shift, first_color, link, status)
+    if (fine) {
+        if (mode == 0) FS_RB(1, 0);
+        else if (mode == 1) FS_RB(1, 1);
+        else FS_RB(1, 2);
+    } else {
+        if (mode == 0) FS_RB(0, 0);
+        else if (mode == 1) FS_RB(0, 1);
+        else FS_RB(0, 2);
+    }
+#undef FS_RB
 }
 
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,

@@ -312,6 +312,13 @@ class FluidGridHIP:
     def addVortex(self, center, radius: float, strength: float):
         call("fs_add_vortex", self._h, float(center[0]), float(center[1]), float(radius), float(strength))
 
+    def advect(self, dt: float) -> int:
+        """FluidGridCPU::advect (FluidGridCPU.cpp:362-463): PLIC/VOF transport alone; returns the CFL
+        substep count."""
+        n = C.c_int()
+        call("fs_advect", self._h, float(dt), C.byref(n))
+        return n.value
+
     def advectSemiLagrangian(self, dt: float):
         call("fs_advect_semi_lagrangian", self._h, float(dt))
 

@@ -71,10 +71,10 @@ typedef struct fs_step_timings {
     double regrid_ms;
 } fs_step_timings;
 
-/* Transport applied by fs_step.  The reference step transports by PLIC/VOF
- * (FluidGridCPU.cpp:362-463); FS_TRANSPORT_REFERENCE is reserved for it and currently returns
- * FS_E_UNSUPPORTED.  FS_TRANSPORT_NONE runs forces + projection (identical to the reference
- * step on a state at rest).  FS_TRANSPORT_SEMI_LAGRANGIAN is the build's own mode. */
+/* Transport applied by fs_step.  FS_TRANSPORT_REFERENCE (the default) is the reference step's
+ * CFL-split PLIC/VOF transport of fractions, component masses and momentum (FluidGridCPU.cpp:362-463),
+ * bit-identical to UniformCPU.  FS_TRANSPORT_NONE runs forces + projection only.
+ * FS_TRANSPORT_SEMI_LAGRANGIAN is the build's own mode. */
 typedef enum fs_transport {
     FS_TRANSPORT_REFERENCE = 0,
     FS_TRANSPORT_NONE = 1,
@@ -145,6 +145,10 @@ fs_status fs_apply_gravity(fs_grid* grid, double dt);
 fs_status fs_add_vortex(fs_grid* grid, double center_x, double center_y, double radius, double strength);
 /* maxDivergence() (FluidGridBase.h:135, FluidGridCPU.cpp:1126-1152) */
 fs_status fs_max_divergence(fs_grid* grid, double* out);
+/* FluidGridCPU::advect(dt) (FluidGridCPU.h:121, FluidGridCPU.cpp:362-463): reference transport alone.
+ * `substeps` (may be NULL) receives the number of CFL substeps taken.  Velocities that would need more
+ * than 1e6 substeps return FS_E_RUNTIME instead of looping. */
+fs_status fs_advect(fs_grid* grid, double dt, int* substeps);
 /* Semi-Lagrangian transport alone (build-defined mode, no reference counterpart). */
 fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt);
 
@@ -175,7 +179,8 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
 
 /* ---- measurement (no reference counterpart) ---- */
 /* Times `reps` back-to-back launches of one hot kernel on the grid's stream with HIP events and
- * returns the mean milliseconds per launch.  kernel: 0 = finest-level four-sweep smoother,
+ * returns the mean milliseconds per launch.  kernel: 0 = finest-level post-smoothing (prolongation +
+ * four sweeps, as in the V-cycle),
  * 1 = operator + dot (A d), 2 = PCG update, 3 = FP32 copy of one padded vector (reference kernel),
  * 4 = second level's smoother, 5 = finest residual + restriction, 6 = coarse tail, 7 = coarse
  * factorisation, 8 = direction update, 9 = r.z reduction. */

@@ -180,14 +180,6 @@ def test_zero_dt_and_zero_budget():
     g.destroy()
 
 
-def test_reference_transport_is_reported_unsupported():
-    wl = WL.dam_break(8, 8, seed=3)
-    g = hip_grid(wl)
-    with pytest.raises(NotImplementedError):
-        g.step(0.01)
-    g.destroy()
-
-
 def test_divergence_free_field_needs_no_iterations():
     W, H, h = 24, 16, 0.5
     xs = np.arange(W + 1) * h
