@@ -491,7 +491,7 @@ struct fs_grid {
             launch_level_diag(st(), Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p, Cl.invd32.p);
         }
         const Level& last = levels.back();
-        launch_coarse_factor(st(), last.w, last.h, last.right64.p, last.down64.p, factor.p, ginv.p, coarse_n, S.p);
+        launch_coarse_inverse(st(), last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, S.p);
         mg_valid = true;
     }
 
@@ -1361,6 +1361,10 @@ fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n)
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
         set_device(grid->ctx);
         ensure_hierarchy(*grid);
+        // the bit-exact factorCoarsest factor is formed on demand (the solve uses the sweep inverse)
+        const Level& last = grid->levels.back();
+        launch_coarse_factor(grid->st(), last.w, last.h, last.right64.p, last.down64.p, grid->factor.p, nullptr,
+            grid->coarse_n, nullptr);
         HIPC(hipStreamSynchronize(grid->st()));
         if (n)
             *n = grid->coarse_n;
@@ -1477,8 +1481,8 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
                         nullptr); break;
             case 7: {
                 const Level& last = g.levels.back();
-                launch_coarse_factor(g.st(), last.w, last.h, last.right64.p, last.down64.p, g.factor.p, g.ginv.p,
-                    g.coarse_n, nullptr);
+                launch_coarse_inverse(g.st(), last.w, last.h, last.right64.p, last.down64.p, g.ginv.p, g.coarse_n,
+                    nullptr);
                 break;
             }
             case 8: launch_pcg_direction(g.st(), gd, g.levels[0].sol0.p); break;

@@ -87,9 +87,12 @@ void launch_level_diag(hipStream_t st, int w, int h, const double* right64, cons
 void launch_coarsen(hipStream_t st, int fw, int fh, const double* fcx, const double* fcy,
     const double* fright, const double* fdown, int cw, int ch, const double* ccx, const double* ccy,
     double* cright, double* cdown);
-// Bit-exact Cholesky of the pinned coarse matrix (factorCoarsest) plus its FP32 inverse.
+// Bit-exact Cholesky of the pinned coarse matrix (factorCoarsest), plus its FP32 inverse when ginv is set.
 void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, const double* down64,
     double* factor, float* ginv, int n, PcgScalars* S);
+// FP32 inverse of the pinned coarse matrix by the in-register sweep operator (the projection's path).
+void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, const double* down64, float* ginv,
+    int n, PcgScalars* S);
 
 // ---- projection set-up and finish (FP64) ----
 void launch_rhs(hipStream_t st, const GridDev& g, double scale);

@@ -411,6 +411,8 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
         const int r = e / n, c = e - r * n;
         factor[e] = c <= r ? a[e] : 0.0;
     }
+    if (!ginv)
+        return;
     // In-place Gauss-Jordan inversion of the (SPD, so pivot-free) pinned matrix.
     for (int k = 0; k < n; ++k) {
         const double p = m[k * n + k];
@@ -442,6 +444,97 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
     }
     for (int e = tid; e < n * n; e += kFactorThreads)
         ginv[e] = (float)m[e];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Inverse of the pinned coarse matrix for the per-cycle coarse solve (the hot path).  Goodnight's
+// sweep operator, in place, one pivot per step: the matrix lives in registers, column j in lane j
+// and rows split over the four wavefronts (16 per wavefront).  A step broadcasts the pivot row
+// through LDS (double-buffered, one barrier) and the pivot column inside each wavefront by lane
+// reads, so the 63 steps cost 63 barriers and no LDS matrix traffic.  Pivots are the Schur
+// complement diagonals (the squared Cholesky pivots); a non-positive or non-finite one marks the
+// hierarchy failed exactly where the Cholesky factorisation would.
+// ---------------------------------------------------------------------------------------------
+constexpr int kSweepRows = 16;   // rows per wavefront (4 x 16 >= kMaxCoarse)
+
+// v as held by lane k (k uniform across the wavefront).
+__device__ __forceinline__ double lane_value(double v, int k)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const double* right, const double* down,
+    float* ginv, int n, PcgScalars* S)
+{
+    __shared__ double rowbuf[2][64];
+    const int s = w + 2;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    // pinned matrix entry (i, j) of unknowns i, j (cells i + 1, j + 1)
+    double a[kSweepRows];
+    const int cj = lane + 1;
+    const int xj = cj % w, yj = cj / w;
+#pragma unroll
+    for (int r = 0; r < kSweepRows; ++r) {
+        const int ci = wv * kSweepRows + r + 1;
+        double v = 0.0;
+        if (ci <= n && cj <= n) {
+            const int xi = ci % w, yi = ci / w;
+            const int pi = (yi + 1) * s + xi + 1;
+            if (ci == cj) {
+                v = right[pi - 1] + right[pi] + down[pi - s] + down[pi];
+            } else if (yi == yj && (xi - xj == 1 || xj - xi == 1)) {
+                v = -right[(yi + 1) * s + min(xi, xj) + 1];
+            } else if (xi == xj && (yi - yj == 1 || yj - yi == 1)) {
+                v = -down[(min(yi, yj) + 1) * s + xi + 1];
+            }
+        }
+        a[r] = v;
+    }
+    bool failed = false;
+    for (int kb = 0; kb < 4 && !failed; ++kb) {
+#pragma unroll
+        for (int r0 = 0; r0 < kSweepRows; ++r0) {
+            const int k = kb * kSweepRows + r0;
+            if (k >= n || failed)
+                break;
+            double* row = rowbuf[k & 1];
+            if (wv == kb)
+                row[lane] = a[r0];
+            __syncthreads();
+            const double d = row[k];
+            if (!(d > 0.0) || !isfinite(d)) {
+                failed = true;   // d is the same in every thread: the whole block leaves together
+                break;
+            }
+            const double inv_d = 1.0 / d;
+            const double t = lane == k ? inv_d : row[lane] * inv_d;
+#pragma unroll
+            for (int r = 0; r < kSweepRows; ++r) {
+                const int i = wv * kSweepRows + r;
+                const double b = lane_value(a[r], k);
+                const double base = lane == k ? 0.0 : a[r];
+                a[r] = i == k ? t : base - b * t;
+            }
+        }
+    }
+    if (failed) {
+        if (threadIdx.x == 0 && S) {
+            S->mg_failed = 1;
+            if (S->status == kRunning)
+                S->status = kFailed;
+        }
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < kSweepRows; ++r) {
+        const int i = wv * kSweepRows + r;
+        if (i < n && lane < n)
+            ginv[i * n + lane] = (float)a[r];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -816,6 +909,14 @@ void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, c
         return;
     hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(kFactorThreads), 0, st, w, h, right64, down64, factor, ginv, n,
         S);
+}
+
+void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, const double* down64, float* ginv,
+    int n, PcgScalars* S)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, st, w, h, right64, down64, ginv, n, S);
 }
 
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status)
