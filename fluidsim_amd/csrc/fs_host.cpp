@@ -192,6 +192,7 @@ struct fs_grid {
     DevBuf<double> tr_fraction, tr_lmass, tr_gmass, tr_nu[2], tr_nv[2], tr_mom_u, tr_mom_v, tr_lt, tr_mt;
     DevBuf<double> tr_lcw, tr_gcw, tr_mcw, tr_out;
     DevBuf<unsigned char> tr_split;
+    DevBuf<float> calib_src, calib_dst;   // 64 MiB copy-bandwidth reference (allocated on first use)
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<PcgScalars> S;
     DevBuf<double> partials;
@@ -1431,12 +1432,16 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
     });
 }
 
+// Words of the copy-bandwidth reference: 64 MiB per buffer, twice the aggregate L2, so every launch
+// streams from and to HBM.
+constexpr size_t kCalibWords = (size_t)16 << 20;
+
 fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch)
 {
     return guard([&] {
         need(grid != nullptr && ms_per_launch != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
         need(reps > 0, FS_E_INVALID_ARGUMENT, "reps must be positive");
-        need(kernel >= 0 && kernel <= 9, FS_E_INVALID_ARGUMENT, "unknown kernel id");
+        need(kernel >= 0 && kernel <= 10, FS_E_INVALID_ARGUMENT, "unknown kernel id");
         fs_grid& g = *grid;
         need(g.uploaded, FS_E_LOGIC, "Upload the grid state first");
         set_device(g.ctx);
@@ -1449,8 +1454,13 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
         GridDev gd = g.dev();
         const int nlev = (int)g.levels.size();
         const int l1 = std::min(1, g.tail_start);
+        if (kernel == 10 && g.calib_src.n != kCalibWords) {
+            g.bytes += g.calib_src.alloc(kCalibWords);
+            g.bytes += g.calib_dst.alloc(kCalibWords);
+        }
         auto launch_once = [&](int i) {
             switch (kernel) {
+            case 10: launch_copy_f32(g.st(), g.calib_src.p, g.calib_dst.p, kCalibWords); break;
             case 0: {  // the finest post-smoothing launch of a V-cycle (prolongation + four sweeps)
                 CoarseLink link{};
                 if (g.tail_start > 0) {

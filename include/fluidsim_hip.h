@@ -183,7 +183,8 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
  * four sweeps, as in the V-cycle),
  * 1 = operator + dot (A d), 2 = PCG update, 3 = FP32 copy of one padded vector (reference kernel),
  * 4 = second level's smoother, 5 = finest residual + restriction, 6 = coarse tail, 7 = coarse
- * factorisation, 8 = direction update, 9 = r.z reduction. */
+ * inverse (sweep operator), 8 = direction update, 9 = r.z reduction, 10 = FP32 copy of a 64 MiB buffer
+ * into another (HBM copy-bandwidth reference; 128 MiB moved per launch). */
 fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch);
 /* Cumulative device bytes allocated by this grid. */
 fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);
