@@ -522,6 +522,7 @@ struct orc_grid {
     int preconditioner; /* 0 diagonal, 1 multigrid */
     int nl, ng;
     double *rho_l, *rho_g;
+    double *mu_l, *mu_g;   /* component dynamic viscosities */
     double *fraction, *u, *v, *p, *p_prev;
     double *lmass, *gmass;
     double *density, *inv_u, *inv_v, *inv_diag;
@@ -561,7 +562,7 @@ void orc_grid_destroy(orc_grid* g)
 {
     if (!g)
         return;
-    free(g->rho_l); free(g->rho_g);
+    free(g->rho_l); free(g->rho_g); free(g->mu_l); free(g->mu_g);
     free(g->fraction); free(g->u); free(g->v); free(g->p); free(g->p_prev);
     free(g->lmass); free(g->gmass);
     free(g->density); free(g->inv_u); free(g->inv_v); free(g->inv_diag);
@@ -584,11 +585,13 @@ void orc_invalidate_caches(orc_grid* g)
 int orc_configure(orc_grid* g, int nl, const double* rl, int ng, const double* rg)
 {
     const size_t N = (size_t)g->W * g->H;
-    free(g->rho_l); free(g->rho_g); free(g->lmass); free(g->gmass);
+    free(g->rho_l); free(g->rho_g); free(g->lmass); free(g->gmass); free(g->mu_l); free(g->mu_g);
     g->nl = nl;
     g->ng = ng;
     g->rho_l = dalloc((size_t)nl);
     g->rho_g = dalloc((size_t)ng);
+    g->mu_l = dalloc((size_t)nl);
+    g->mu_g = dalloc((size_t)ng);
     for (int k = 0; k < nl; ++k) g->rho_l[k] = rl[k];
     for (int k = 0; k < ng; ++k) g->rho_g[k] = rg[k];
     g->lmass = dalloc((size_t)nl * N);
@@ -1650,10 +1653,55 @@ orc_result orc_step(orc_grid* g, double dt, int max_iterations, double tol)
         return r;
     }
     const int substeps = orc_advect_reference(g, dt);
+    int viscosity_iterations = 0;
+    if (orc_has_viscosity(g) && orc_apply_viscosity(g, dt, &viscosity_iterations) != ORC_VISC_OK) {
+        orc_result r;
+        memset(&r, 0, sizeof(r));
+        r.max_divergence = INFINITY;
+        r.viscosity_iterations = viscosity_iterations;
+        return r;
+    }
     orc_apply_gravity(g, dt);
     orc_result r = orc_project(g, dt, max_iterations, tol);
+    r.viscosity_iterations = viscosity_iterations;
     r.physical_steps = 1;
     r.transport_steps = 0;
     (void)substeps;
     return r;
+}
+
+/* ---- grid-level implicit viscosity (the WebGPU step order: transport, viscosity, forces, projection,
+ * FluidGridGPU.cpp:1134-1158) ---- */
+int orc_has_viscosity(const orc_grid* g)
+{
+    for (int k = 0; k < g->nl; ++k)
+        if (g->mu_l[k] > 0.0)
+            return 1;
+    for (int k = 0; k < g->ng; ++k)
+        if (g->mu_g[k] > 0.0)
+            return 1;
+    return 0;
+}
+
+void orc_configure_viscosity(orc_grid* g, const double* mu_l, const double* mu_g)
+{
+    for (int k = 0; k < g->nl; ++k)
+        g->mu_l[k] = mu_l ? mu_l[k] : 0.0;
+    for (int k = 0; k < g->ng; ++k)
+        g->mu_g[k] = mu_g ? mu_g[k] : 0.0;
+}
+
+int orc_apply_viscosity(orc_grid* g, double dt, int* iterations)
+{
+    if (iterations)
+        *iterations = 0;
+    if (dt <= 0.0 || !orc_has_viscosity(g))
+        return ORC_VISC_OK;
+    const size_t N = (size_t)g->W * g->H;
+    double* material = dalloc(2 * N);
+    orc_viscosity_material((int)N, g->h, g->nl, g->rho_l, g->mu_l, g->lmass, g->ng, g->rho_g, g->mu_g, g->gmass,
+        material);
+    const int status = orc_viscosity_solve(g->W, g->H, g->h, material, g->u, g->v, dt, 1e-4, -1, iterations, NULL);
+    free(material);
+    return status;
 }

@@ -9,6 +9,7 @@
  *   FluidSim/UniformCPU/FluidGridCPU.cpp      (project, solvePressure, operator, gravity, vortices,
  *                                             PLIC/VOF transport, step)
  *   FluidSim/UniformCPU/PressureMultigrid.cpp (hierarchy, coarsening, RB-GS V(4,4), Cholesky)
+ *   FluidSim/UniformWebGPU/ViscositySolverGPU.cpp + ViscosityKernels.h (implicit viscosity, FP64)
  *
  * Layout conventions are the reference's: u is (W+1) x H, v is W x (H+1), cells W x H,
  * all row-major, y grows downward (FluidGridCPU.h:128-130).
@@ -33,6 +34,7 @@ typedef struct orc_result {
     int restarts;            /* true-residual restarts inside the PCG loop (diagnostic) */
     double initial_residual; /* max |b - A p| before the loop (diagnostic) */
     double final_residual;   /* max |b - A p| at exit, recomputed in FP64 (diagnostic) */
+    int viscosity_iterations;
 } orc_result;
 
 /* ---- PressureMultigrid (PressureMultigrid.h:9-69) ---- */
@@ -90,6 +92,24 @@ double orc_transport_max_step(const orc_grid* g, double dt);
 orc_result orc_step(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
 /* Staggered bilinear velocity sample at a world position (FluidGridCPU.cpp:465-471). */
 void orc_sample_velocity(const orc_grid* g, double px, double py, double* vx, double* vy);
+
+/* ---- implicit viscosity (WebGPU-only in the reference; fs_viscosity.c) ---- */
+enum { ORC_VISC_OK = 0, ORC_VISC_NONFINITE = 1, ORC_VISC_BREAKDOWN = 2, ORC_VISC_NOT_CONVERGED = 3 };
+/* ViscositySolverGPU::solve (ViscositySolverGPU.cpp:169-294): material is (rho, mu) per cell; u, v are
+ * updated in place only on success.  maximum < 0 means unbounded.  Returns an ORC_VISC_* status. */
+int orc_viscosity_solve(int width, int height, double cell_size, const double* material, double* u, double* v,
+    double dt, double tolerance, int maximum, int* iterations, double* relative_residual);
+int orc_viscosity_levels(int width, int height);
+/* Material blend of FluidGridGPU::applyViscosity (FluidGridGPU.cpp:173-218). */
+void orc_viscosity_material(int cells, double cell_size, int n_liquid, const double* liquid_density,
+    const double* liquid_viscosity, const double* liquid_mass, int n_gas, const double* gas_density,
+    const double* gas_viscosity, const double* gas_mass, double* material);
+/* Component dynamic viscosities (setComponentViscosity, FluidGridBase.h:48-52). */
+void orc_configure_viscosity(orc_grid* g, const double* liquid_viscosity, const double* gas_viscosity);
+/* FluidGridGPU::applyViscosity(dt) (FluidGridGPU.cpp:987-1046) with the reference tolerance 1e-4, unbounded.
+ * Returns an ORC_VISC_* status; *iterations receives the dispatched iteration count. */
+int orc_apply_viscosity(orc_grid* g, double dt, int* iterations);
+int orc_has_viscosity(const orc_grid* g);
 
 /* Derived fields (read-only views of cached data, rebuilt on demand). */
 void orc_density(orc_grid* g, double* density);

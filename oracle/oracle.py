@@ -4,7 +4,9 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and bench.py's CPU
 always as the checker or the timed CPU baseline, never as the product path.
 
 The wrapped C code restates, single-threaded and in FP64, the reference's
-FluidSim/UniformCPU/FluidGridCPU.cpp projection and FluidSim/UniformCPU/PressureMultigrid.cpp.
+FluidSim/UniformCPU/FluidGridCPU.cpp step (transport, forces, projection),
+FluidSim/UniformCPU/PressureMultigrid.cpp, and the WebGPU-only implicit viscosity solve
+(FluidSim/UniformWebGPU/ViscositySolverGPU.cpp, ViscosityKernels.h).
 """
 from __future__ import annotations
 
@@ -32,6 +34,7 @@ class OrcResult(C.Structure):
         ("restarts", C.c_int),
         ("initial_residual", C.c_double),
         ("final_residual", C.c_double),
+        ("viscosity_iterations", C.c_int),
     ]
 
     def as_dict(self):
@@ -83,6 +86,14 @@ def lib():
         "orc_transport_max_step": (C.c_double, [vp, C.c_double]),
         "orc_step": (OrcResult, [vp, C.c_double, C.c_int, C.c_double]),
         "orc_sample_velocity": (None, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "orc_viscosity_solve": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double,
+                                          C.c_int, _ip, _dp]),
+        "orc_viscosity_levels": (C.c_int, [C.c_int, C.c_int]),
+        "orc_viscosity_material": (None, [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                          _dp]),
+        "orc_configure_viscosity": (None, [vp, _dp, _dp]),
+        "orc_apply_viscosity": (C.c_int, [vp, C.c_double, _ip]),
+        "orc_has_viscosity": (C.c_int, [vp]),
         "orc_density": (None, [vp, _dp]),
         "orc_face_weights": (None, [vp, _dp, _dp]),
         "orc_apply_operator": (C.c_double, [vp, _dp, _dp]),
@@ -259,6 +270,16 @@ class OracleGrid:
         """step(), FluidGridCPU.cpp:300-360 (surface tension off)."""
         return lib().orc_step(self._h, dt, max_iterations, divergence_tolerance).as_dict()
 
+    def configure_viscosity(self, liquid_viscosity, gas_viscosity):
+        self._mu = (_f64(liquid_viscosity), _f64(gas_viscosity))
+        lib().orc_configure_viscosity(self._h, _ptr(self._mu[0]), _ptr(self._mu[1]))
+
+    def apply_viscosity(self, dt):
+        """FluidGridGPU::applyViscosity(dt); returns (status, iterations)."""
+        it = C.c_int()
+        st = lib().orc_apply_viscosity(self._h, dt, C.byref(it))
+        return st, it.value
+
     def sample_velocity(self, x, y):
         vx, vy = C.c_double(), C.c_double()
         lib().orc_sample_velocity(self._h, x, y, C.byref(vx), C.byref(vy))
@@ -285,3 +306,29 @@ class OracleGrid:
         r = np.zeros(self.N)
         lib().orc_pressure_rhs(self._h, dt, _ptr(r))
         return r
+
+
+VISC_OK, VISC_NONFINITE, VISC_BREAKDOWN, VISC_NOT_CONVERGED = 0, 1, 2, 3
+
+
+def viscosity_solve(width, height, cell_size, material, u, v, dt, tolerance=1e-4, maximum=-1):
+    """ViscositySolverGPU::solve on host arrays (FP64 restatement).  material: (rho, mu) per cell.
+    Returns (status, iterations, relative_residual, u, v); u and v change only on success."""
+    material = _f64(material)
+    u = _f64(u).copy()
+    v = _f64(v).copy()
+    it = C.c_int()
+    rel = C.c_double()
+    st = lib().orc_viscosity_solve(width, height, cell_size, _ptr(material), _ptr(u), _ptr(v), dt, tolerance,
+                                   maximum, C.byref(it), C.byref(rel))
+    return st, it.value, rel.value, u, v
+
+
+def viscosity_material(cell_size, liquid_density, liquid_viscosity, liquid_mass, gas_density, gas_viscosity,
+                       gas_mass, cells):
+    ld, lv, lm = _f64(liquid_density), _f64(liquid_viscosity), _f64(liquid_mass)
+    gd, gv, gm = _f64(gas_density), _f64(gas_viscosity), _f64(gas_mass)
+    out = np.zeros(2 * cells)
+    lib().orc_viscosity_material(cells, cell_size, ld.size, _ptr(ld), _ptr(lv), _ptr(lm), gd.size, _ptr(gd),
+                                 _ptr(gv), _ptr(gm), _ptr(out))
+    return out
