@@ -87,6 +87,10 @@ SIGNATURES = {
     "fs_add_vortex": [_vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fs_max_divergence": [_vp, _dp],
     "fs_advect": [_vp, C.c_double, _ip],
+    "fs_apply_viscosity": [_vp, C.c_double, _ip],
+    "fs_set_component_viscosity": [_vp, C.c_int, C.c_int, C.c_double],
+    "fs_viscosity_solve": [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _ip,
+                           _dp],
     "fs_advect_semi_lagrangian": [_vp, C.c_double],
     "fs_snapshot_save": [_vp],
     "fs_snapshot_restore": [_vp],

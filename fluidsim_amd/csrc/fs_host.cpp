@@ -193,6 +193,7 @@ struct fs_grid {
     DevBuf<double> tr_lcw, tr_gcw, tr_mcw, tr_out;
     DevBuf<unsigned char> tr_split;
     DevBuf<float> calib_src, calib_dst;   // 64 MiB copy-bandwidth reference (allocated on first use)
+    ViscositySolver* visc = nullptr;       // implicit viscosity workspace (created on first use)
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<PcgScalars> S;
     DevBuf<double> partials;
@@ -210,7 +211,22 @@ struct fs_grid {
     hipGraph_t iter_graph = nullptr;
     fs_preconditioner graph_precond = FS_PRECOND_MULTIGRID;
 
-    ~fs_grid() { drop_graph(); }
+    ~fs_grid()
+    {
+        drop_graph();
+        viscosity_destroy(visc);
+    }
+
+    bool has_viscosity() const
+    {
+        for (double mu : mu_l)
+            if (mu > 0.0)
+                return true;
+        for (double mu : mu_g)
+            if (mu > 0.0)
+                return true;
+        return false;
+    }
 
     size_t N() const { return (size_t)W * H; }
     size_t NP() const { return (size_t)(W + 2) * (H + 2); }
@@ -791,6 +807,14 @@ fs_status fs_configure_materials(fs_grid* grid, int n_liquid, const double* liqu
         need(n_liquid >= 0 && n_gas >= 0, FS_E_INVALID_ARGUMENT, "component counts must be non-negative");
         need((n_liquid == 0 || liquid_density) && (n_gas == 0 || gas_density), FS_E_INVALID_ARGUMENT,
             "component densities are required");
+        need(n_liquid <= kMaxComponents && n_gas <= kMaxComponents, FS_E_INVALID_ARGUMENT,
+            "at most 64 components per phase");
+        for (int k = 0; k < n_liquid && liquid_viscosity; ++k)
+            need(std::isfinite(liquid_viscosity[k]) && liquid_viscosity[k] >= 0.0, FS_E_INVALID_ARGUMENT,
+                "Material viscosity must be finite and nonnegative");
+        for (int k = 0; k < n_gas && gas_viscosity; ++k)
+            need(std::isfinite(gas_viscosity[k]) && gas_viscosity[k] >= 0.0, FS_E_INVALID_ARGUMENT,
+                "Material viscosity must be finite and nonnegative");
         grid->rho_l.assign(liquid_density, liquid_density + n_liquid);
         grid->rho_g.assign(gas_density, gas_density + n_gas);
         grid->mu_l.assign(n_liquid, 0.0);
@@ -864,12 +888,6 @@ fs_status fs_upload(fs_grid* grid, const double* liquid_fraction, const double* 
 {
     return guard([&] {
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
-        for (double mu : grid->mu_l)
-            if (mu != 0.0)
-                fail(FS_E_UNSUPPORTED, "Viscosity is not implemented by the HIP backend yet");
-        for (double mu : grid->mu_g)
-            if (mu != 0.0)
-                fail(FS_E_UNSUPPORTED, "Viscosity is not implemented by the HIP backend yet");
         set_device(grid->ctx);
         fs_grid& g = *grid;
         const size_t N = g.N();
@@ -1110,6 +1128,111 @@ static int advect_reference(fs_grid& g, double dt)
     return substeps;
 }
 
+// Iteration budget standing in for the reference's unbounded viscosity solve (maximum = -1).
+constexpr int kViscosityBudget = 100000;
+
+static void raise_viscosity(int status, double relative)
+{
+    if (status == 1)
+        fail(FS_E_RUNTIME, "Viscosity received nonfinite velocity/material state");
+    if (status == 2)
+        fail(FS_E_RUNTIME, "Implicit viscosity solver breakdown");
+    if (status == 3)
+        fail(FS_E_RUNTIME, "Implicit viscosity did not converge; relative residual " + std::to_string(relative));
+}
+
+// FluidGridGPU::applyViscosity (FluidGridGPU.cpp:987-1046): blend, then solve to 1e-4.  Returns the
+// dispatched iteration count.
+static int apply_viscosity(fs_grid& g, double dt)
+{
+    if (dt <= 0.0 || !g.has_viscosity())
+        return 0;
+    if (!g.visc) {
+        g.visc = viscosity_create(g.W, g.H);
+        g.bytes += viscosity_bytes(g.visc);
+    }
+    VisMixture mix{};
+    for (size_t k = 0; k < g.rho_l.size(); ++k) {
+        mix.density[k] = g.rho_l[k];
+        mix.viscosity[k] = g.mu_l[k];
+    }
+    for (size_t k = 0; k < g.rho_g.size(); ++k) {
+        mix.density[kMaxComponents + k] = g.rho_g[k];
+        mix.viscosity[kMaxComponents + k] = g.mu_g[k];
+    }
+    double* material = viscosity_material_buffer(g.visc);
+    launch_viscosity_blend(g.st(), (int)g.N(), g.h, (int)g.rho_l.size(), (int)g.rho_g.size(), g.lmass.p,
+        g.gmass.p, mix, material);
+    int iterations = 0;
+    double relative = 0.0;
+    const int status = viscosity_solve(g.visc, g.st(), g.u.p, g.v.p, material, dt, g.h, 1e-4, kViscosityBudget,
+        &iterations, &relative);
+    raise_viscosity(status, relative);
+    return iterations;
+}
+
+fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before applying viscosity");
+        set_device(grid->ctx);
+        const int n = apply_viscosity(*grid, dt);
+        HIPC(hipStreamSynchronize(grid->st()));
+        if (iterations)
+            *iterations = n;
+    });
+}
+
+fs_status fs_set_component_viscosity(fs_grid* grid, int phase, int component, double coefficient)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(std::isfinite(coefficient) && coefficient >= 0.0, FS_E_INVALID_ARGUMENT,
+            "Material viscosity must be finite and nonnegative");
+        need(phase == 0 || phase == 1, FS_E_INVALID_ARGUMENT, "phase must be 0 (liquid) or 1 (gas)");
+        std::vector<double>& mu = phase == 0 ? grid->mu_l : grid->mu_g;
+        need(component >= 0 && (size_t)component < mu.size(), FS_E_INVALID_ARGUMENT, "no such mixture component");
+        mu[(size_t)component] = coefficient;
+    });
+}
+
+fs_status fs_viscosity_solve(fs_context* ctx, int width, int height, double cell_size, const double* material,
+    double* u, double* v, double dt, double tolerance, int max_iterations, int* iterations,
+    double* relative_residual)
+{
+    return guard([&] {
+        need(ctx != nullptr && material != nullptr && u != nullptr && v != nullptr, FS_E_INVALID_ARGUMENT,
+            "null pointer");
+        need(width > 0 && height > 0 && cell_size > 0.0, FS_E_INVALID_ARGUMENT, "invalid grid shape");
+        need(std::isfinite(tolerance) && tolerance >= 0.0, FS_E_INVALID_ARGUMENT, "invalid tolerance");
+        set_device(ctx);
+        const size_t N = (size_t)width * height, NU = (size_t)(width + 1) * height, NV = (size_t)width * (height + 1);
+        std::unique_ptr<ViscositySolver, void (*)(ViscositySolver*)> solver(viscosity_create(width, height),
+            viscosity_destroy);
+        DevBuf<double> dm, du, dv;
+        dm.alloc(2 * N);
+        du.alloc(NU);
+        dv.alloc(NV);
+        HIPC(hipMemcpyAsync(dm.p, material, 2 * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(du.p, u, NU * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(dv.p, v, NV * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        int it = 0;
+        double rel = 0.0;
+        const int budget = max_iterations < 0 ? kViscosityBudget : max_iterations;
+        const int status = viscosity_solve(solver.get(), ctx->stream, du.p, dv.p, dm.p, dt, cell_size, tolerance,
+            budget, &it, &rel);
+        if (iterations)
+            *iterations = it;
+        if (relative_residual)
+            *relative_residual = rel;
+        raise_viscosity(status, rel);
+        HIPC(hipMemcpyAsync(u, du.p, NU * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(v, dv.p, NV * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 fs_status fs_advect(fs_grid* grid, double dt, int* substeps)
 {
     return guard([&] {
@@ -1170,6 +1293,10 @@ fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergenc
             transport_steps = 1;
         }
         lap(timings ? &timings->transport_ms : nullptr);
+        // the WebGPU step's order: transport, viscosity, forces, projection (FluidGridGPU.cpp:1134-1158)
+        const int viscosity_iterations = apply_viscosity(g, dt);
+        if (viscosity_iterations)
+            lap(timings ? &timings->viscosity_ms : nullptr);
         launch_gravity(g.st(), g.dev(), g.gravity_y, dt);
         lap(timings ? &timings->forces_ms : nullptr);
         const fs_projection_result proj = g.project(dt, max_iterations, divergence_tolerance);
@@ -1177,6 +1304,7 @@ fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergenc
         res = proj;
         res.physical_steps = 1;
         res.transport_steps = transport_steps;
+        res.viscosity_iterations = viscosity_iterations;
         *result = res;
     });
 }

@@ -160,6 +160,28 @@ void launch_transport_freeze(hipStream_t st, const TransportDev& t);
 // (FluidGridCPU.cpp:401-456).
 void launch_transport_pass(hipStream_t st, const TransportDev& t, int horizontal, double scale);
 
+// ---- implicit viscosity (WebGPU-only in the reference) ----
+constexpr int kMaxComponents = 64;   // per phase
+// Component densities and dynamic viscosities: liquids at [0, nl), gases at [kMaxComponents, +ng).
+struct VisMixture {
+    double density[2 * kMaxComponents];
+    double viscosity[2 * kMaxComponents];
+};
+// Material blend of FluidGridGPU::applyViscosity into (rho, mu) per cell, FP64.
+void launch_viscosity_blend(hipStream_t st, int N, double cell_size, int nl, int ng, const double* lm,
+    const double* gm, const VisMixture& mix, double* material);
+struct ViscositySolver;   // device workspace of one grid size
+ViscositySolver* viscosity_create(int width, int height);
+void viscosity_destroy(ViscositySolver* s);
+size_t viscosity_bytes(const ViscositySolver* s);
+int viscosity_levels(const ViscositySolver* s);
+double* viscosity_material_buffer(ViscositySolver* s);   // 2 * W * H doubles
+// Backward-Euler viscosity on device FP64 u ((W+1) x H) and v (W x (H+1)) with a device FP64 (rho, mu)
+// material; maximum < 0 is unbounded.  Returns 0 converged (or nothing to do), 1 non-finite input,
+// 2 breakdown, 3 not converged within `maximum`; u, v change only on 0.
+int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material, double dt,
+    double spacing, double tol, int maximum, int* iterations, double* relative_residual);
+
 // ---- bench helper: one operator application without reductions (for event timing) ----
 void launch_copy_f32(hipStream_t st, const float* src, float* dst, size_t n);
 

@@ -1,0 +1,828 @@
+// fs_viscosity.hip -- implicit viscosity on gfx950: backward Euler M (v - v0) + dt K v = 0 on the MAC faces.
+//
+// Algorithm of the reference's WebGPU solver (FluidSim/UniformWebGPU/ViscositySolverGPU.cpp:59-294,
+// shaders FluidSim/UniformWebGPU/ViscosityKernels.h:10-630): M = face mass 0.5 h^2 (rho_a + rho_b),
+// K = symmetric strain operator (normal stress 2 mu on cell centres, shear on interior vertices), CG
+// preconditioned by a V(2,2) damped-Jacobi (weight 1/2) multigrid with a 16-sweep coarse solve, stop
+// when the recomputed residual meets r M^-1 r <= tol^2 v0 M v0 + 1e-12 sum(M), checked every four
+// iterations; a recurrence that claims convergence before the recomputed residual restarts CG from it.
+//
+// This build's layout and numerics: every level stores (w+1) x (h+1) slots of float2 (u face left of
+// cell p, v face on top of cell p) and float4 coefficients (cell mu, vertex mu, u mass, v mass); reads
+// clamp to the slot field like the reference's accessors.  The solve runs on the correction
+// delta = v - v0 (FP32), with FP64 dot products, and the result is v0 + delta in FP64, so faces the
+// viscosity does not reach keep their FP64 velocities exactly.  Kernels read a device scalar block and
+// exit at once when the solve has stopped, so a batch of four iterations is launched without host
+// round trips; the host reads the block once per batch, as the reference does.
+#include "fs_kernels.hpp"
+
+#include <math.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <vector>
+
+namespace fsim {
+
+namespace {
+
+constexpr int kVX = 64;
+constexpr int kVY = 4;
+constexpr int kVT = 256;
+
+struct VisScalars {
+    double rz;        // r . z of the current direction
+    double alpha;
+    double beta;
+    double target;    // tol^2 v0 M v0 + 1e-12 sum(M)
+    double rnorm;     // recurrence r M^-1 r
+    double energy;    // v0 M v0
+    double mass_sum;
+    double truth;     // recomputed r M^-1 r
+    int steps;        // completed alpha steps since the last (re)start
+    int bad;          // breakdown or non-finite value
+    int pad[2];
+};
+
+struct VLev {
+    int w, h;
+    float4* coef;
+    float2* mat;    // cell material (rho, mu), w * h
+    float2* b;      // rhs (levels > 0)
+    float2* x;
+    float2* xn;
+    float2* r;
+};
+
+__device__ __forceinline__ int sidx(int w, int h, int x, int y)
+{
+    return min(max(y, 0), h) * (w + 1) + min(max(x, 0), w);
+}
+
+__device__ __forceinline__ bool valid_u(int w, int h, int x, int y) { return x > 0 && x < w && y < h; }
+__device__ __forceinline__ bool valid_v(int w, int h, int x, int y) { return y > 0 && y < h && x < w; }
+
+__device__ __forceinline__ bool stopped(const VisScalars* S) { return S->bad != 0 || S->rnorm <= S->target; }
+
+// shear and stressProduct (ViscosityKernels.h:26-44)
+__device__ __forceinline__ float shear(const float4* c, const float2* s, int w, int h, int x, int y)
+{
+    const float2 p = s[sidx(w, h, x, y)];
+    return c[sidx(w, h, x, y)].y * ((p.x - s[sidx(w, h, x, y - 1)].x) + (p.y - s[sidx(w, h, x - 1, y)].y));
+}
+
+// product (ViscosityKernels.h:53-57): M s + dt K s on valid faces.
+__device__ float2 product(const float4* c, const float2* s, int w, int h, int x, int y, float dt)
+{
+    const int p = sidx(w, h, x, y);
+    const float2 cv = s[p];
+    const float4 a = c[p];
+    const float t = shear(c, s, w, h, x, y);
+    const float nu = 2.0f * (c[sidx(w, h, x - 1, y)].x * (cv.x - s[sidx(w, h, x - 1, y)].x) +
+                             a.x * (cv.x - s[sidx(w, h, x + 1, y)].x));
+    const float nv = 2.0f * (c[sidx(w, h, x, y - 1)].x * (cv.y - s[sidx(w, h, x, y - 1)].y) +
+                             a.x * (cv.y - s[sidx(w, h, x, y + 1)].y));
+    const float ku = nu + (t - shear(c, s, w, h, x, y + 1));
+    const float kv = nv + (t - shear(c, s, w, h, x + 1, y));
+    float2 out;
+    out.x = valid_u(w, h, x, y) ? a.z * cv.x + dt * ku : 0.0f;
+    out.y = valid_v(w, h, x, y) ? a.w * cv.y + dt * kv : 0.0f;
+    return out;
+}
+
+// diagonal (ViscosityKernels.h:17-24)
+__device__ __forceinline__ float2 diagonal(const float4* c, int w, int h, int x, int y, float dt)
+{
+    const float4 a = c[sidx(w, h, x, y)];
+    float2 d;
+    d.x = a.z + dt * (2.0f * (a.x + c[sidx(w, h, x - 1, y)].x) + a.y + c[sidx(w, h, x, y + 1)].y);
+    d.y = a.w + dt * (2.0f * (a.x + c[sidx(w, h, x, y - 1)].x) + a.y + c[sidx(w, h, x + 1, y)].y);
+    return d;
+}
+
+#define VIS_SLOT(L)                                            \
+    const int x = blockIdx.x * kVX + (threadIdx.x & 63);       \
+    const int y = blockIdx.y * kVY + (threadIdx.x >> 6);       \
+    const bool inside = x <= (L).w && y <= (L).h;              \
+    const int p = inside ? y * ((L).w + 1) + x : 0;
+
+__device__ __forceinline__ unsigned vblock() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// ---- material and coefficients ----
+
+// FluidGridGPU's blend + pack (FluidGridGPU.cpp:173-218): rho = sum(m) / h^2,
+// mu = sum(m_k mu_k / rho_k) / sum(m_k / rho_k), FP64.
+__global__ __launch_bounds__(kVT) void k_vis_blend(int N, double area, int nl, int ng, const double* lm,
+    const double* gm, VisMixture mix, double* material)
+{
+    for (int i = blockIdx.x * kVT + threadIdx.x; i < N; i += gridDim.x * kVT) {
+        double mass = 0.0, volume = 0.0, mu_volume = 0.0;
+        for (int k = 0; k < nl; ++k) {
+            const double m = lm[(size_t)k * N + i];
+            mass += m;
+            volume += m / mix.density[k];
+            mu_volume += m * (mix.viscosity[k] / mix.density[k]);
+        }
+        for (int k = 0; k < ng; ++k) {
+            const double m = gm[(size_t)k * N + i];
+            mass += m;
+            volume += m / mix.density[kMaxComponents + k];
+            mu_volume += m * (mix.viscosity[kMaxComponents + k] / mix.density[kMaxComponents + k]);
+        }
+        material[2 * i] = mass / area;
+        material[2 * i + 1] = mu_volume / volume;
+    }
+}
+
+__global__ __launch_bounds__(kVT) void k_vis_material(int N, const double* material, float2* mat)
+{
+    for (int i = blockIdx.x * kVT + threadIdx.x; i < N; i += gridDim.x * kVT)
+        mat[i] = make_float2((float)material[2 * i], (float)material[2 * i + 1]);
+}
+
+// "viscosity coarse material" (ViscosityKernels.h:115-135)
+__global__ __launch_bounds__(kVT) void k_vis_coarsen(VLev F, VLev C)
+{
+    const int x = blockIdx.x * kVX + (threadIdx.x & 63);
+    const int y = blockIdx.y * kVY + (threadIdx.x >> 6);
+    if (x >= C.w || y >= C.h)
+        return;
+    float2 v = make_float2(0.0f, 0.0f);
+    float n = 0.0f;
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+            const int qx = 2 * x + dx, qy = 2 * y + dy;
+            if (qx < F.w && qy < F.h) {
+                const float2 m = F.mat[qy * F.w + qx];
+                v.x += m.x;
+                v.y += m.y;
+                n += 1.0f;
+            }
+        }
+    C.mat[y * C.w + x] = make_float2(v.x / n, v.y / n);
+}
+
+// "viscosity coefficients" (ViscosityKernels.h:137-152)
+__global__ __launch_bounds__(kVT) void k_vis_coef(VLev L, float spacing)
+{
+    VIS_SLOT(L)
+    if (!inside)
+        return;
+    auto m = [&](int cx, int cy) { return L.mat[min(max(cy, 0), L.h - 1) * L.w + min(max(cx, 0), L.w - 1)]; };
+    const float2 c = m(x, y);
+    float vertex = 0.0f;
+    if (x > 0 && y > 0 && x < L.w && y < L.h)
+        vertex = 0.25f * (c.y + m(x - 1, y).y + m(x, y - 1).y + m(x - 1, y - 1).y);
+    const float half_area = 0.5f * spacing * spacing;
+    L.coef[p] = make_float4(c.y, vertex, half_area * (c.x + m(x - 1, y).x), half_area * (c.x + m(x, y - 1).x));
+}
+
+// ---- set-up of the correction system ----
+
+// "viscosity initial velocity and RHS" (ViscosityKernels.h:154-163): v0 on valid faces (FP32).
+__global__ __launch_bounds__(kVT) void k_vis_pack(VLev L, const double* u, const double* v, float2* v0)
+{
+    VIS_SLOT(L)
+    if (!inside)
+        return;
+    const int W = L.w, H = L.h;
+    const double su = u[min(y, H - 1) * (W + 1) + x];
+    const double sv = v[y * W + min(x, W - 1)];
+    v0[p] = make_float2(valid_u(W, H, x, y) ? (float)su : 0.0f, valid_v(W, H, x, y) ? (float)sv : 0.0f);
+}
+
+// r0 = M v0 - A v0 = -dt K v0 (the rhs of the correction system) and the mode-0 reduction:
+// r0 M^-1 r0, v0 M v0, sum of masses, non-finite flag (ViscosityKernels.h:395-470).
+__global__ __launch_bounds__(kVT) void k_vis_initial(VLev L, const float2* v0, float2* r0, float dt, double* partials)
+{
+    __shared__ double lds[kVT / 64];
+    VIS_SLOT(L)
+    double a = 0.0, b = 0.0, m = 0.0, bad = 0.0;
+    if (inside) {
+        const float4 c = L.coef[p];
+        const float2 q = product(L.coef, v0, L.w, L.h, x, y, dt);
+        const float2 s = v0[p];
+        float2 r;
+        r.x = valid_u(L.w, L.h, x, y) ? c.z * s.x - q.x : 0.0f;
+        r.y = valid_v(L.w, L.h, x, y) ? c.w * s.y - q.y : 0.0f;
+        r0[p] = r;
+        a = (double)r.x / c.z * r.x + (double)r.y / c.w * r.y;
+        b = (double)c.z * s.x * s.x + (double)c.w * s.y * s.y;
+        m = (double)c.z + (double)c.w;
+        if (!isfinite(a) || !isfinite(b) || !isfinite(m)) {
+            a = b = m = 0.0;
+            bad = 1.0;
+        }
+    }
+    const unsigned nb = gridDim.x * gridDim.y;
+    const double sa = block_sum<kVT>(a, lds);
+    const double sb = block_sum<kVT>(b, lds);
+    const double sm = block_sum<kVT>(m, lds);
+    const double sf = block_max<kVT>(bad, lds);
+    if (threadIdx.x == 0) {
+        partials[vblock()] = sa;
+        partials[nb + vblock()] = sb;
+        partials[2 * nb + vblock()] = sm;
+        partials[3 * nb + vblock()] = sf;
+    }
+}
+
+// ---- V-cycle ----
+
+// "viscosity symmetric smoother" (ViscosityKernels.h:214-236), damped Jacobi with weight 1/2.
+__global__ __launch_bounds__(kVT) void k_vis_smooth(VLev L, const float2* b, int zero, float dt, const VisScalars* S)
+{
+    if (stopped(S))
+        return;
+    VIS_SLOT(L)
+    if (!inside)
+        return;
+    const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
+    const float2 rhs = b[p];
+    float2 v;
+    if (zero) {
+        v.x = 0.5f * rhs.x / d.x;
+        v.y = 0.5f * rhs.y / d.y;
+    } else {
+        const float2 q = product(L.coef, L.x, L.w, L.h, x, y, dt);
+        const float2 s = L.x[p];
+        v.x = s.x + 0.5f * (rhs.x - q.x) / d.x;
+        v.y = s.y + 0.5f * (rhs.y - q.y) / d.y;
+    }
+    L.xn[p] = make_float2(valid_u(L.w, L.h, x, y) ? v.x : 0.0f, valid_v(L.w, L.h, x, y) ? v.y : 0.0f);
+}
+
+// "viscosity true residual" on a level (ViscosityKernels.h:187-195)
+__global__ __launch_bounds__(kVT) void k_vis_residual(VLev L, const float2* b, float dt, const VisScalars* S)
+{
+    if (stopped(S))
+        return;
+    VIS_SLOT(L)
+    if (!inside)
+        return;
+    const float2 q = product(L.coef, L.x, L.w, L.h, x, y, dt);
+    const float2 rhs = b[p];
+    L.r[p] = make_float2(valid_u(L.w, L.h, x, y) ? rhs.x - q.x : 0.0f, valid_v(L.w, L.h, x, y) ? rhs.y - q.y : 0.0f);
+}
+
+// Face-location bilinear transfer (ViscosityKernels.h:286-316).
+__device__ __forceinline__ void transfer_location(int fx, int fy, int comp, int& bx, int& by, float& tx, float& ty)
+{
+    const float ox = comp == 1 ? 0.5f : 0.0f, oy = comp == 1 ? 0.0f : 0.5f;
+    const float lx = ((float)fx + ox) * 0.5f - ox, ly = ((float)fy + oy) * 0.5f - oy;
+    const float fx0 = floorf(lx), fy0 = floorf(ly);
+    bx = (int)fx0;
+    by = (int)fy0;
+    tx = lx - fx0;
+    ty = ly - fy0;
+}
+
+__device__ __forceinline__ void coarse_point(int x, int y, int comp, int cw, int ch, int& ox, int& oy)
+{
+    ox = min(max(x, 0), cw - (comp == 1 ? 1 : 0));
+    oy = min(max(y, 0), ch - (comp == 1 ? 0 : 1));
+}
+
+__device__ float transfer_weight(int fx, int fy, int cx, int cy, int comp, int cw, int ch)
+{
+    int bx, by;
+    float tx, ty;
+    transfer_location(fx, fy, comp, bx, by, tx, ty);
+    float result = 0.0f;
+    for (int yy = 0; yy < 2; ++yy)
+        for (int xx = 0; xx < 2; ++xx) {
+            int px, py;
+            coarse_point(bx + xx, by + yy, comp, cw, ch, px, py);
+            if (px == cx && py == cy)
+                result += (xx == 1 ? tx : 1.0f - tx) * (yy == 1 ? ty : 1.0f - ty);
+        }
+    return result;
+}
+
+// "viscosity transpose restriction" (ViscosityKernels.h:318-357): C.b = P^T F.r
+__global__ __launch_bounds__(kVT) void k_vis_restrict(VLev F, VLev C, const VisScalars* S)
+{
+    if (stopped(S))
+        return;
+    VIS_SLOT(C)
+    if (!inside)
+        return;
+    float val[2] = {0.0f, 0.0f};
+    for (int comp = 0; comp < 2; ++comp) {
+        if ((comp == 0 && (x == 0 || x == C.w || y == C.h)) || (comp == 1 && (y == 0 || y == C.h || x == C.w)))
+            continue;
+        const int ex = comp == 1 ? 2 : 1, ey = comp == 1 ? 1 : 2;
+        for (int dy = -1; dy <= ey; ++dy)
+            for (int dx = -1; dx <= ex; ++dx) {
+                const int qx = 2 * x + dx, qy = 2 * y + dy;
+                if (qx < 0 || qy < 0 || qx > F.w || qy > F.h)
+                    continue;
+                const float2 r = F.r[qy * (F.w + 1) + qx];
+                val[comp] += transfer_weight(qx, qy, x, y, comp, C.w, C.h) * (comp == 0 ? r.x : r.y);
+            }
+    }
+    C.b[p] = make_float2(val[0], val[1]);
+}
+
+// "viscosity prolongation" (ViscosityKernels.h:359-391): F.xn = F.x + P C.x on valid faces
+__global__ __launch_bounds__(kVT) void k_vis_prolong(VLev F, VLev C, const VisScalars* S)
+{
+    if (stopped(S))
+        return;
+    VIS_SLOT(F)
+    if (!inside)
+        return;
+    const float2 s = F.x[p];
+    float val[2] = {s.x, s.y};
+    for (int comp = 0; comp < 2; ++comp) {
+        int bx, by;
+        float tx, ty;
+        transfer_location(x, y, comp, bx, by, tx, ty);
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                int px, py;
+                coarse_point(bx + dx, by + dy, comp, C.w, C.h, px, py);
+                const float2 c = C.x[py * (C.w + 1) + px];
+                val[comp] += (dx == 1 ? tx : 1.0f - tx) * (dy == 1 ? ty : 1.0f - ty) * (comp == 0 ? c.x : c.y);
+            }
+    }
+    F.xn[p] = make_float2(valid_u(F.w, F.h, x, y) ? val[0] : 0.0f, valid_v(F.w, F.h, x, y) ? val[1] : 0.0f);
+}
+
+// "viscosity shared coarse solve" (ViscosityKernels.h:242-282): 16 Jacobi sweeps from zero in LDS.
+constexpr int kCoarseSlots = 256;
+__global__ __launch_bounds__(kCoarseSlots) void k_vis_coarse(VLev L, const float2* b, float dt, const VisScalars* S)
+{
+    __shared__ float2 vel[kCoarseSlots];
+    __shared__ float4 cof[kCoarseSlots];
+    if (stopped(S))
+        return;
+    const int lane = threadIdx.x;
+    const int stride = L.w + 1;
+    const int n = stride * (L.h + 1);
+    const int x = lane % stride, y = lane / stride;
+    const int src = lane < n ? lane : n - 1;
+    vel[lane] = make_float2(0.0f, 0.0f);
+    cof[lane] = L.coef[src];
+    const float2 rhs = b[src];
+    __syncthreads();
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (lane < n) {
+            const float2 d = diagonal(cof, L.w, L.h, x, y, dt);
+            const float2 q = product(cof, vel, L.w, L.h, x, y, dt);
+            const float2 s = vel[lane];
+            v.x = valid_u(L.w, L.h, x, y) ? s.x + 0.5f * (rhs.x - q.x) / d.x : 0.0f;
+            v.y = valid_v(L.w, L.h, x, y) ? s.y + 0.5f * (rhs.y - q.y) / d.y : 0.0f;
+        }
+        __syncthreads();
+        vel[lane] = v;
+        __syncthreads();
+    }
+    if (lane < n)
+        L.x[lane] = vel[lane];
+}
+
+// ---- CG ----
+
+// mode-1 reduction: r . z, r M^-1 r (ViscosityKernels.h:395-470)
+__global__ __launch_bounds__(kVT) void k_vis_rz(VLev L, const float2* r, const float2* z, const VisScalars* S,
+    double* partials)
+{
+    __shared__ double lds[kVT / 64];
+    if (stopped(S))
+        return;
+    VIS_SLOT(L)
+    double a = 0.0, b = 0.0, bad = 0.0;
+    if (inside) {
+        const float4 c = L.coef[p];
+        const float2 rr = r[p], zz = z[p];
+        a = (double)rr.x * zz.x + (double)rr.y * zz.y;
+        b = (double)rr.x / c.z * rr.x + (double)rr.y / c.w * rr.y;
+        if (!isfinite(a) || !isfinite(b)) {
+            a = b = 0.0;
+            bad = 1.0;
+        }
+    }
+    const unsigned nb = gridDim.x * gridDim.y;
+    const double sa = block_sum<kVT>(a, lds);
+    const double sb = block_sum<kVT>(b, lds);
+    const double sf = block_max<kVT>(bad, lds);
+    if (threadIdx.x == 0) {
+        partials[vblock()] = sa;
+        partials[nb + vblock()] = sb;
+        partials[3 * nb + vblock()] = sf;
+    }
+}
+
+__global__ __launch_bounds__(kVT) void k_vis_direction(VLev L, const float2* z, float2* d, const VisScalars* S)
+{
+    if (stopped(S))
+        return;
+    VIS_SLOT(L)
+    if (!inside)
+        return;
+    const float2 zz = z[p];
+    if (S->steps > 0) {
+        const float beta = (float)S->beta;
+        const float2 dd = d[p];
+        d[p] = make_float2(zz.x + beta * dd.x, zz.y + beta * dd.y);
+    } else {
+        d[p] = zz;
+    }
+}
+
+// q = A d and the mode-2 reduction d . q
+__global__ __launch_bounds__(kVT) void k_vis_apply(VLev L, const float2* d, float2* q, float dt, const VisScalars* S,
+    double* partials)
+{
+    __shared__ double lds[kVT / 64];
+    if (stopped(S))
+        return;
+    VIS_SLOT(L)
+    double a = 0.0, bad = 0.0;
+    if (inside) {
+        const float2 v = product(L.coef, d, L.w, L.h, x, y, dt);
+        q[p] = v;
+        const float2 dd = d[p];
+        a = (double)dd.x * v.x + (double)dd.y * v.y;
+        if (!isfinite(a)) {
+            a = 0.0;
+            bad = 1.0;
+        }
+    }
+    const unsigned nb = gridDim.x * gridDim.y;
+    const double sa = block_sum<kVT>(a, lds);
+    const double sf = block_max<kVT>(bad, lds);
+    if (threadIdx.x == 0) {
+        partials[vblock()] = sa;
+        partials[3 * nb + vblock()] = sf;
+    }
+}
+
+__global__ __launch_bounds__(kVT) void k_vis_update(VLev L, float2* xs, float2* r, const float2* d, const float2* q,
+    const VisScalars* S)
+{
+    if (stopped(S))
+        return;
+    VIS_SLOT(L)
+    if (!inside)
+        return;
+    const float alpha = (float)S->alpha;
+    const float2 dd = d[p], qq = q[p], xx = xs[p], rr = r[p];
+    xs[p] = make_float2(xx.x + alpha * dd.x, xx.y + alpha * dd.y);
+    r[p] = make_float2(rr.x - alpha * qq.x, rr.y - alpha * qq.y);
+}
+
+// recomputed residual t = r0 - A delta and t M^-1 t (mode 3)
+__global__ __launch_bounds__(kVT) void k_vis_truth(VLev L, const float2* r0, const float2* xs, float2* t, float dt,
+    double* partials)
+{
+    __shared__ double lds[kVT / 64];
+    VIS_SLOT(L)
+    double a = 0.0, bad = 0.0;
+    if (inside) {
+        const float4 c = L.coef[p];
+        const float2 q = product(L.coef, xs, L.w, L.h, x, y, dt);
+        const float2 b = r0[p];
+        float2 v;
+        v.x = valid_u(L.w, L.h, x, y) ? b.x - q.x : 0.0f;
+        v.y = valid_v(L.w, L.h, x, y) ? b.y - q.y : 0.0f;
+        t[p] = v;
+        a = (double)v.x / c.z * v.x + (double)v.y / c.w * v.y;
+        if (!isfinite(a)) {
+            a = 0.0;
+            bad = 1.0;
+        }
+    }
+    const unsigned nb = gridDim.x * gridDim.y;
+    const double sa = block_sum<kVT>(a, lds);
+    const double sf = block_max<kVT>(bad, lds);
+    if (threadIdx.x == 0) {
+        partials[vblock()] = sa;
+        partials[3 * nb + vblock()] = sf;
+    }
+}
+
+enum VisFinish : int { kVisInitial = 0, kVisRz, kVisDq, kVisTruth };
+
+// One workgroup reduces the partials in block order and updates the scalar block
+// (the reference's "viscosity CG scalars", ViscosityKernels.h:490-535).
+__global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double* partials, int nb, int op, double tol2)
+{
+    __shared__ double lds[kVT / 64];
+    if ((op == kVisRz || op == kVisDq) && stopped(S))
+        return;
+    const double a = finish_sum<kVT>(partials, nb, 1, 0, lds);
+    const double b = (op == kVisInitial || op == kVisRz) ? finish_sum<kVT>(partials + nb, nb, 1, 0, lds) : 0.0;
+    const double m = op == kVisInitial ? finish_sum<kVT>(partials + 2 * nb, nb, 1, 0, lds) : 0.0;
+    const double f = finish_max<kVT>(partials + 3 * nb, nb, 1, 0, lds);
+    if (threadIdx.x != 0)
+        return;
+    VisScalars s = *S;
+    const bool bad = f != 0.0;
+    switch (op) {
+    case kVisInitial:
+        s = VisScalars{};
+        s.energy = b;
+        s.mass_sum = m;
+        s.target = tol2 * b + 1e-12 * m;
+        s.rnorm = a;
+        s.truth = a;
+        s.bad = bad ? 1 : 0;
+        break;
+    case kVisRz:
+        s.bad |= bad ? 1 : 0;
+        s.rnorm = b;
+        if (s.rnorm > s.target && !s.bad) {
+            if (a <= 0.0 || !isfinite(a)) {
+                s.bad = 1;
+            } else {
+                s.beta = s.steps > 0 ? a / s.rz : 0.0;
+                s.rz = a;
+            }
+        }
+        break;
+    case kVisDq:
+        if (bad || a <= 0.0 || !isfinite(a)) {
+            s.bad = 1;
+        } else {
+            s.alpha = s.rz / a;
+            s.steps += 1;
+        }
+        break;
+    default:  // kVisTruth
+        s.truth = bad ? INFINITY : a;
+        if (bad)
+            s.bad = 1;
+        break;
+    }
+    *S = s;
+}
+
+// Restart from the recomputed residual ("viscosity refresh residual", ViscosityKernels.h:607-620).
+__global__ void k_vis_refresh(VisScalars* S)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        VisScalars s = *S;
+        s.rnorm = s.truth;
+        s.rz = s.alpha = s.beta = 0.0;
+        s.steps = 0;
+        *S = s;
+    }
+}
+
+// "viscosity staggered result" (ViscosityKernels.h:165-185) on the correction: u = valid ? u0 + delta : 0.
+__global__ __launch_bounds__(kVT) void k_vis_unpack(int W, int H, const float2* xs, double* u, double* v)
+{
+    const int x = blockIdx.x * kVX + (threadIdx.x & 63);
+    const int y = blockIdx.y * kVY + (threadIdx.x >> 6);
+    if (x <= W && y < H) {
+        const int i = y * (W + 1) + x;
+        u[i] = valid_u(W, H, x, y) ? u[i] + (double)xs[i].x : 0.0;
+    }
+    if (x < W && y <= H) {
+        const int i = y * W + x;
+        v[i] = valid_v(W, H, x, y) ? v[i] + (double)xs[y * (W + 1) + x].y : 0.0;
+    }
+}
+
+inline dim3 slot_grid(int w, int h) { return dim3((w + 1 + kVX - 1) / kVX, (h + 1 + kVY - 1) / kVY); }
+inline unsigned slot_blocks(int w, int h) { return ((w + 1 + kVX - 1) / kVX) * ((h + 1 + kVY - 1) / kVY); }
+
+void vhip(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T> struct VBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    VBuf() = default;
+    VBuf(const VBuf&) = delete;
+    VBuf& operator=(const VBuf&) = delete;
+    ~VBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (!count)
+            return 0;
+        vhip(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+        vhip(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+        return count * sizeof(T);
+    }
+};
+
+}  // namespace
+
+struct ViscositySolver {
+    struct Level {
+        int w, h;
+        VBuf<float4> coef;
+        VBuf<float2> mat, b, x, xn, r;
+    };
+    int W, H;
+    std::vector<Level*> levels;
+    VBuf<float2> v0, r0, r, d, q, t;
+    VBuf<double> material, partials;
+    VBuf<VisScalars> S;
+    VisScalars* pinned = nullptr;
+    size_t bytes = 0;
+
+    ViscositySolver(int w, int h) : W(w), H(h)
+    {
+        for (;;) {
+            Level* L = new Level();
+            L->w = w;
+            L->h = h;
+            const size_t ns = (size_t)(w + 1) * (h + 1);
+            bytes += L->coef.alloc(ns);
+            bytes += L->mat.alloc((size_t)w * h);
+            bytes += L->b.alloc(ns);
+            bytes += L->x.alloc(ns);
+            bytes += L->xn.alloc(ns);
+            bytes += L->r.alloc(ns);
+            levels.push_back(L);
+            if (w * h <= 64)
+                break;
+            w = (w + 1) / 2;
+            h = (h + 1) / 2;
+        }
+        const size_t ns = (size_t)(W + 1) * (H + 1);
+        bytes += v0.alloc(ns);
+        bytes += r0.alloc(ns);
+        bytes += r.alloc(ns);
+        bytes += d.alloc(ns);
+        bytes += q.alloc(ns);
+        bytes += t.alloc(ns);
+        bytes += material.alloc(2 * (size_t)W * H);
+        bytes += partials.alloc(4 * (size_t)std::max(slot_blocks(W, H), 1u));
+        bytes += S.alloc(1);
+        vhip(hipHostMalloc((void**)&pinned, sizeof(VisScalars), hipHostMallocDefault), "hipHostMalloc");
+    }
+
+    ~ViscositySolver()
+    {
+        for (Level* L : levels)
+            delete L;
+        if (pinned)
+            (void)hipHostFree(pinned);
+    }
+
+    VLev view(int l)
+    {
+        Level& L = *levels[l];
+        return VLev{L.w, L.h, L.coef.p, L.mat.p, L.b.p, L.x.p, L.xn.p, L.r.p};
+    }
+
+    void swap_x(int l) { std::swap(levels[l]->x.p, levels[l]->xn.p); }
+
+    VisScalars read(hipStream_t st)
+    {
+        vhip(hipMemcpyAsync(pinned, S.p, sizeof(VisScalars), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        vhip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return *pinned;
+    }
+
+    void finish(hipStream_t st, int nb, int op, double tol2)
+    {
+        hipLaunchKernelGGL(k_vis_finish, dim3(1), dim3(kVT), 0, st, S.p, partials.p, nb, op, tol2);
+    }
+
+    // cycle (ViscositySolverGPU.cpp:134-167); the result is levels[l]->x
+    void cycle(hipStream_t st, int l, const float2* b, float dt)
+    {
+        VLev L = view(l);
+        if (l + 1 == (int)levels.size()) {
+            hipLaunchKernelGGL(k_vis_coarse, dim3(1), dim3(kCoarseSlots), 0, st, L, b, dt, S.p);
+            return;
+        }
+        const dim3 g = slot_grid(L.w, L.h);
+        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 1, dt, S.p);
+        swap_x(l);
+        L = view(l);
+        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, dt, S.p);
+        swap_x(l);
+        L = view(l);
+        hipLaunchKernelGGL(k_vis_residual, g, dim3(kVT), 0, st, L, b, dt, S.p);
+        VLev C = view(l + 1);
+        hipLaunchKernelGGL(k_vis_restrict, slot_grid(C.w, C.h), dim3(kVT), 0, st, L, C, S.p);
+        cycle(st, l + 1, C.b, dt);
+        C = view(l + 1);
+        hipLaunchKernelGGL(k_vis_prolong, g, dim3(kVT), 0, st, L, C, S.p);
+        swap_x(l);
+        for (int k = 0; k < 2; ++k) {
+            L = view(l);
+            hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, dt, S.p);
+            swap_x(l);
+        }
+    }
+
+    // ViscositySolverGPU::Impl::solve (ViscositySolverGPU.cpp:169-294) on device FP64 u, v and an FP64
+    // (rho, mu) material; u, v change only on success.
+    int solve(hipStream_t st, double* u, double* v, const double* mat64, double dt, double spacing, double tol,
+        int maximum, int* iterations_out, double* relative_out)
+    {
+        *iterations_out = 0;
+        *relative_out = 0.0;
+        if (dt <= 0.0)
+            return 0;
+        const float fdt = (float)dt;
+        const int N = W * H;
+        hipLaunchKernelGGL(k_vis_material, dim3(std::min((N + kVT - 1) / kVT, 4096)), dim3(kVT), 0, st, N, mat64,
+            levels[0]->mat.p);
+        for (size_t l = 0; l < levels.size(); ++l) {
+            if (l > 0) {
+                VLev F = view((int)l - 1), C = view((int)l);
+                hipLaunchKernelGGL(k_vis_coarsen, dim3((C.w + kVX - 1) / kVX, (C.h + kVY - 1) / kVY), dim3(kVT), 0,
+                    st, F, C);
+            }
+            VLev L = view((int)l);
+            hipLaunchKernelGGL(k_vis_coef, slot_grid(L.w, L.h), dim3(kVT), 0, st, L, (float)ldexp(spacing, (int)l));
+        }
+        VLev L0 = view(0);
+        const dim3 g0 = slot_grid(W, H);
+        const int nb = (int)slot_blocks(W, H);
+        hipLaunchKernelGGL(k_vis_pack, g0, dim3(kVT), 0, st, L0, (const double*)u, (const double*)v, v0.p);
+        hipLaunchKernelGGL(k_vis_initial, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p, r0.p, fdt, partials.p);
+        finish(st, nb, kVisInitial, tol * tol);
+        VisScalars s = read(st);
+        const double denom = std::max(s.energy, 1e-30);
+        *relative_out = std::sqrt(std::max(s.rnorm, 0.0) / denom);
+        if (s.bad)
+            return 1;   // non-finite velocity or material
+        if (s.rnorm <= s.target)
+            return 0;
+        const size_t ns = (size_t)(W + 1) * (H + 1);
+        vhip(hipMemcpyAsync(r.p, r0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
+        vhip(hipMemsetAsync(t.p, 0, ns * sizeof(float2), st), "clear");   // t doubles as the correction
+        float2* delta = t.p;
+        VBuf<float2>& truth = q;   // q is free between iterations
+        int iterations = 0;
+        while (maximum < 0 || iterations < maximum) {
+            const int batch = maximum < 0 ? 4 : std::min(4, maximum - iterations);
+            for (int i = 0; i < batch; ++i) {
+                cycle(st, 0, r.p, fdt);
+                L0 = view(0);
+                hipLaunchKernelGGL(k_vis_rz, g0, dim3(kVT), 0, st, L0, (const float2*)r.p, (const float2*)L0.x, S.p,
+                    partials.p);
+                finish(st, nb, kVisRz, 0.0);
+                hipLaunchKernelGGL(k_vis_direction, g0, dim3(kVT), 0, st, L0, (const float2*)L0.x, d.p, S.p);
+                hipLaunchKernelGGL(k_vis_apply, g0, dim3(kVT), 0, st, L0, (const float2*)d.p, q.p, fdt, S.p,
+                    partials.p);
+                finish(st, nb, kVisDq, 0.0);
+                hipLaunchKernelGGL(k_vis_update, g0, dim3(kVT), 0, st, L0, delta, r.p, (const float2*)d.p,
+                    (const float2*)q.p, S.p);
+            }
+            iterations += batch;
+            hipLaunchKernelGGL(k_vis_truth, g0, dim3(kVT), 0, st, L0, (const float2*)r0.p, (const float2*)delta,
+                truth.p, fdt, partials.p);
+            finish(st, nb, kVisTruth, 0.0);
+            s = read(st);
+            *iterations_out = iterations;
+            *relative_out = std::sqrt(std::max(s.truth, 0.0) / denom);
+            if (s.bad)
+                return 2;   // breakdown
+            if (s.truth <= s.target) {
+                hipLaunchKernelGGL(k_vis_unpack, g0, dim3(kVT), 0, st, W, H, (const float2*)delta, u, v);
+                return 0;
+            }
+            if (s.rnorm <= s.target) {
+                // the recurrence claims convergence the recomputed residual denies: restart from it
+                vhip(hipMemcpyAsync(r.p, truth.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
+                hipLaunchKernelGGL(k_vis_refresh, dim3(1), dim3(64), 0, st, S.p);
+            }
+        }
+        return 3;   // not converged within the caller's budget
+    }
+};
+
+ViscositySolver* viscosity_create(int width, int height) { return new ViscositySolver(width, height); }
+void viscosity_destroy(ViscositySolver* s) { delete s; }
+size_t viscosity_bytes(const ViscositySolver* s) { return s ? s->bytes : 0; }
+int viscosity_levels(const ViscositySolver* s) { return s ? (int)s->levels.size() : 0; }
+double* viscosity_material_buffer(ViscositySolver* s) { return s->material.p; }
+
+int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material, double dt,
+    double spacing, double tol, int maximum, int* iterations, double* relative_residual)
+{
+    return s->solve(st, u, v, material, dt, spacing, tol, maximum, iterations, relative_residual);
+}
+
+void launch_viscosity_blend(hipStream_t st, int N, double cell_size, int nl, int ng, const double* lm,
+    const double* gm, const VisMixture& mix, double* material)
+{
+    hipLaunchKernelGGL(k_vis_blend, dim3(std::min((N + kVT - 1) / kVT, 4096)), dim3(kVT), 0, st, N,
+        cell_size * cell_size, nl, ng, lm, gm, mix, material);
+}
+
+}  // namespace fsim

@@ -134,6 +134,24 @@ def _dp(a: Optional[np.ndarray]):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def viscosity_solve(width: int, height: int, cell_size: float, material, u, v, dt: float, tolerance: float = 1e-4,
+                    maximum: int = -1, context: Optional[Context] = None):
+    """ViscositySolverGPU::solve (ViscositySolverGPU.cpp:169-294) on host arrays.  material holds (rho, mu)
+    per cell.  Returns (iterations, relative_residual, u, v); raises RuntimeError (inputs untouched) on
+    breakdown, non-finite input or an exhausted budget."""
+    ctx = context or default_context()
+    m = np.ascontiguousarray(material, dtype=np.float64).ravel()
+    uu = np.array(u, dtype=np.float64).ravel()
+    vv = np.array(v, dtype=np.float64).ravel()
+    if m.size != 2 * width * height or uu.size != (width + 1) * height or vv.size != width * (height + 1):
+        raise ValueError("viscosity_solve dimensions do not match the grid")
+    it = C.c_int()
+    rel = C.c_double()
+    call("fs_viscosity_solve", ctx.handle, int(width), int(height), float(cell_size), _dp(m), _dp(uu), _dp(vv),
+         float(dt), float(tolerance), int(maximum), C.byref(it), C.byref(rel))
+    return it.value, rel.value, uu, vv
+
+
 class FluidGridHIP:
     """Uniform MAC grid backend on one MI355X (the FluidGridBase method set)."""
 
@@ -210,9 +228,18 @@ class FluidGridHIP:
     def hasViscosity(self) -> bool:
         return self.liquid_mixture.hasViscosity() or self.gas_mixture.hasViscosity()
 
-    def setComponentViscosity(self, phase: str, index: int, coefficient: float):
-        if coefficient != 0.0:
-            raise NotImplementedError("Viscosity is not implemented by the HIP backend yet")
+    def setComponentViscosity(self, phase, index: int, coefficient: float):
+        """setComponentViscosity (FluidGridBase.h:48-52); phase 0/"liquid" or 1/"gas"."""
+        ph = {"liquid": 0, "gas": 1}.get(phase, phase)
+        call("fs_set_component_viscosity", self._h, int(ph), int(index), float(coefficient))
+        mixture = self.liquid_mixture if ph == 0 else self.gas_mixture
+        mixture.components[index].dynamic_viscosity = float(coefficient)
+
+    def applyViscosity(self, dt: float) -> int:
+        """FluidGridGPU::applyViscosity (FluidGridGPU.cpp:1039-1046); returns the CG iterations."""
+        n = C.c_int()
+        call("fs_apply_viscosity", self._h, float(dt), C.byref(n))
+        return n.value
 
     # -- forces (FluidGridBase.h:70-84) --
     def getGravityY(self) -> float:

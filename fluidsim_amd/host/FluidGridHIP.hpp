@@ -153,10 +153,19 @@ public:
     const FluidMixture& getLiquidMixture() const { return liquid_; }
     const FluidMixture& getGasMixture() const { return gas_; }
     bool hasViscosity() const { return liquid_.hasViscosity() || gas_.hasViscosity(); }
-    void setComponentViscosity(Phase, size_t, double coefficient)
+    // FluidGridBase.h:48-52; out-of-range components throw std::invalid_argument.
+    void setComponentViscosity(Phase phase, size_t component, double coefficient)
     {
-        if (coefficient != 0.0)
-            throw std::invalid_argument("Viscosity is not implemented by the HIP backend yet");
+        check(fs_set_component_viscosity(grid_, phase == Phase::Liquid ? 0 : 1, (int)component, coefficient));
+        auto& mixture = phase == Phase::Liquid ? liquid_ : gas_;
+        mixture.components.at(component).dynamic_viscosity = coefficient;
+    }
+    // FluidGridGPU::applyViscosity (FluidGridGPU.cpp:1039-1046): CG iterations dispatched.
+    int applyViscosity(double dt)
+    {
+        int iterations = 0;
+        check(fs_apply_viscosity(grid_, dt, &iterations));
+        return iterations;
     }
 
     double getGravityY() const

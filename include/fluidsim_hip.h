@@ -104,11 +104,16 @@ fs_status fs_set_origin(fs_grid* grid, double x, double y);
 fs_status fs_get_origin(const fs_grid* grid, double* x, double* y);
 
 /* configureMaterials(SubstanceCatalogue, FluidMixture liquid, FluidMixture gas) (FluidGridBase.h:42):
- * component densities and dynamic viscosities in mixture order.  Viscosity must be zero (the
- * reference's uniform CPU engine rejects it, FluidGridCPU.cpp:200-201): non-zero returns
- * FS_E_UNSUPPORTED at upload, like the reference. */
+ * component densities and dynamic viscosities in mixture order (at most 64 per phase).  Viscosities
+ * must be finite and non-negative.  Any positive viscosity makes fs_step run the implicit viscosity
+ * solve after transport, as the reference's WebGPU backend does (FluidGridGPU.cpp:1134-1145); the
+ * reference's CPU backend rejects viscosity (FluidGridCPU.cpp:200-201). */
 fs_status fs_configure_materials(fs_grid* grid, int n_liquid, const double* liquid_density,
     const double* liquid_viscosity, int n_gas, const double* gas_density, const double* gas_viscosity);
+
+/* setComponentViscosity(Phase, size_t component, double coefficient) (FluidGridBase.h:48-52,
+ * FluidGridGPU.cpp:976-985): phase 0 = liquid, 1 = gas. */
+fs_status fs_set_component_viscosity(fs_grid* grid, int phase, int component, double coefficient);
 
 /* getGravityY/setGravityY (FluidGridBase.h:70-71) */
 fs_status fs_set_gravity(fs_grid* grid, double gravity_y);
@@ -149,6 +154,19 @@ fs_status fs_max_divergence(fs_grid* grid, double* out);
  * `substeps` (may be NULL) receives the number of CFL substeps taken.  Velocities that would need more
  * than 1e6 substeps return FS_E_RUNTIME instead of looping. */
 fs_status fs_advect(fs_grid* grid, double dt, int* substeps);
+/* FluidGridGPU::applyViscosity(dt) (FluidGridGPU.cpp:987-1046): blend each cell's (rho, mu) from the
+ * component masses (mu = sum(m mu/rho) / sum(m/rho)), then solve M (v - v0) + dt K v = 0 to the reference
+ * tolerance 1e-4.  *iterations receives the dispatched CG iterations (a multiple of 4; 0 when dt <= 0 or
+ * no component is viscous).  Breakdown, non-finite input or non-convergence return FS_E_RUNTIME with the
+ * velocities unchanged, as the reference throws. */
+fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations);
+/* ViscositySolverGPU::solve(u, v, material, dt, cell_size, tolerance, maximum)
+ * (ViscositySolverGPU.cpp:169-294, .h:12-28) on host arrays: material is (rho, mu) per cell (2*W*H),
+ * u ((W+1)*H) and v (W*(H+1)) are updated in place on success only.  max_iterations < 0 is the
+ * reference's unbounded mode (capped at 100000 here).  Failures return FS_E_RUNTIME. */
+fs_status fs_viscosity_solve(fs_context* ctx, int width, int height, double cell_size, const double* material,
+    double* u, double* v, double dt, double tolerance, int max_iterations, int* iterations,
+    double* relative_residual);
 /* Semi-Lagrangian transport alone (build-defined mode, no reference counterpart). */
 fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt);
 

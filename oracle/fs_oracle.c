@@ -523,6 +523,7 @@ struct orc_grid {
     int nl, ng;
     double *rho_l, *rho_g;
     double *mu_l, *mu_g;   /* component dynamic viscosities */
+    double sigma;          /* surface tension coefficient */
     double *fraction, *u, *v, *p, *p_prev;
     double *lmass, *gmass;
     double *density, *inv_u, *inv_v, *inv_diag;
@@ -1643,31 +1644,97 @@ int orc_advect_reference(orc_grid* g, double dt)
     return substeps;
 }
 
-/* step, FluidGridCPU.cpp:300-360, without surface tension or an acceleration field. */
+/* setSurfaceTension (FluidGridCPU.cpp:835-841); returns 0, or -1 for a negative or non-finite value. */
+int orc_set_surface_tension(orc_grid* g, double coefficient)
+{
+    if (!isfinite(coefficient) || coefficient < 0.0)
+        return -1;
+    g->sigma = coefficient;
+    return 0;
+}
+
+/* capillaryStepLimit (FluidGridCPU.cpp:843-860) */
+double orc_capillary_step_limit(const orc_grid* g)
+{
+    if (g->sigma == 0.0 || g->nl == 0 || g->ng == 0)
+        return INFINITY;
+    double ml = g->rho_l[0], mg = g->rho_g[0];
+    for (int k = 0; k < g->nl; ++k)
+        ml = dmin(ml, g->rho_l[k]);
+    for (int k = 0; k < g->ng; ++k)
+        mg = dmin(mg, g->rho_g[k]);
+    const double pi = 3.14159265358979323846;
+    return 0.5 * sqrt((ml + mg) * g->h * g->h * g->h / (2.0 * pi * g->sigma));
+}
+
+/* applySurfaceTension (FluidGridCPU.cpp:862-926): capillary face force sigma kappa df dt w / h. */
+void orc_apply_surface_tension(orc_grid* g, double dt)
+{
+    const int W = g->W, H = g->H;
+    const size_t N = (size_t)W * H;
+    double* curvature = dalloc(2 * N);
+    orc_capillary_geometry(W, H, g->h, g->fraction, NULL, NULL, curvature);
+    rebuild_coefficients(g);
+    for (int y = 0; y < H; ++y) {
+        for (int x = 1; x < W; ++x) {
+            const size_t cell = (size_t)y * W + x;
+            const size_t face = (size_t)y * (W + 1) + x;
+            const double jump = g->sigma * orc_face_curvature(curvature, cell - 1, cell) *
+                                (g->fraction[cell] - g->fraction[cell - 1]);
+            g->u[face] += dt * g->inv_u[face] * jump / g->h;
+        }
+        if (y == 0)
+            continue;
+        for (int x = 0; x < W; ++x) {
+            const size_t cell = (size_t)y * W + x;
+            const double jump = g->sigma * orc_face_curvature(curvature, cell - W, cell) *
+                                (g->fraction[cell] - g->fraction[cell - W]);
+            g->v[cell] += dt * g->inv_v[cell] * jump / g->h;
+        }
+    }
+    free(curvature);
+}
+
+/* step, FluidGridCPU.cpp:300-360 (no acceleration field): capillary subdivision, then per substep
+ * transport, viscosity (the WebGPU engine's, when a component is viscous), gravity, surface tension
+ * and projection; stops after the first substep whose projection does not converge. */
 orc_result orc_step(orc_grid* g, double dt, int max_iterations, double tol)
 {
-    if (dt <= 0.0) {
-        orc_result r;
-        memset(&r, 0, sizeof(r));
-        r.max_divergence = INFINITY;
-        return r;
+    orc_result result;
+    memset(&result, 0, sizeof(result));
+    result.max_divergence = INFINITY;
+    if (dt <= 0.0)
+        return result;
+    const int tension = g->sigma > 0.0;
+    const double subdivisions = tension ? dmax(1.0, ceil(dt / orc_capillary_step_limit(g))) : 1.0;
+    if (subdivisions > 2147483647.0)
+        return result;
+    const int count = (int)subdivisions;
+    const double step = dt / count;
+    for (int i = 0; i < count; ++i) {
+        orc_advect_reference(g, step);
+        int viscosity_iterations = 0;
+        if (orc_has_viscosity(g) && orc_apply_viscosity(g, step, &viscosity_iterations) != ORC_VISC_OK) {
+            result.converged = 0;
+            result.viscosity_iterations += viscosity_iterations;
+            return result;
+        }
+        result.viscosity_iterations += viscosity_iterations;
+        orc_apply_gravity(g, step);
+        if (tension)
+            orc_apply_surface_tension(g, step);
+        const orc_result p = orc_project(g, step, max_iterations, tol);
+        result.iterations += p.iterations;
+        result.max_divergence = p.max_divergence;
+        result.converged = p.converged;
+        result.restarts = p.restarts;
+        result.initial_residual = p.initial_residual;
+        result.final_residual = p.final_residual;
+        ++result.physical_steps;
+        if (!result.converged)
+            break;
     }
-    const int substeps = orc_advect_reference(g, dt);
-    int viscosity_iterations = 0;
-    if (orc_has_viscosity(g) && orc_apply_viscosity(g, dt, &viscosity_iterations) != ORC_VISC_OK) {
-        orc_result r;
-        memset(&r, 0, sizeof(r));
-        r.max_divergence = INFINITY;
-        r.viscosity_iterations = viscosity_iterations;
-        return r;
-    }
-    orc_apply_gravity(g, dt);
-    orc_result r = orc_project(g, dt, max_iterations, tol);
-    r.viscosity_iterations = viscosity_iterations;
-    r.physical_steps = 1;
-    r.transport_steps = 0;
-    (void)substeps;
-    return r;
+    return result;
 }
 
 /* ---- grid-level implicit viscosity (the WebGPU step order: transport, viscosity, forces, projection,

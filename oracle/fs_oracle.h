@@ -10,12 +10,15 @@
  *                                             PLIC/VOF transport, step)
  *   FluidSim/UniformCPU/PressureMultigrid.cpp (hierarchy, coarsening, RB-GS V(4,4), Cholesky)
  *   FluidSim/UniformWebGPU/ViscositySolverGPU.cpp + ViscosityKernels.h (implicit viscosity, FP64)
+ *   FluidSim/UniformCPU/FluidSurfaceTensionCPU.h (capillary geometry)
  *
  * Layout conventions are the reference's: u is (W+1) x H, v is W x (H+1), cells W x H,
  * all row-major, y grows downward (FluidGridCPU.h:128-130).
  */
 #ifndef FS_ORACLE_H
 #define FS_ORACLE_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -110,6 +113,17 @@ void orc_configure_viscosity(orc_grid* g, const double* liquid_viscosity, const 
  * Returns an ORC_VISC_* status; *iterations receives the dispatched iteration count. */
 int orc_apply_viscosity(orc_grid* g, double dt, int* iterations);
 int orc_has_viscosity(const orc_grid* g);
+
+/* ---- surface tension (FluidGridCPU.cpp:835-926, FluidSurfaceTensionCPU.h; fs_tension.c) ---- */
+int orc_set_surface_tension(orc_grid* g, double coefficient);
+double orc_capillary_step_limit(const orc_grid* g);
+void orc_apply_surface_tension(orc_grid* g, double dt);
+/* Capillary geometry of a fraction field: heights (x, y, orientation_x, orientation_y per cell), raw
+ * and completed curvatures ((value, valid) per cell).  Any output may be NULL. */
+void orc_capillary_geometry(int width, int height, double cell_size, const double* fraction, double* heights,
+    double* raw, double* curvature);
+/* faceCurvature (FluidSurfaceTensionCPU.h:378-386) of two cells of a (value, valid) array. */
+double orc_face_curvature(const double* curvature, size_t left, size_t right);
 
 /* Derived fields (read-only views of cached data, rebuilt on demand). */
 void orc_density(orc_grid* g, double* density);

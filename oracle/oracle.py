@@ -94,6 +94,10 @@ def lib():
         "orc_configure_viscosity": (None, [vp, _dp, _dp]),
         "orc_apply_viscosity": (C.c_int, [vp, C.c_double, _ip]),
         "orc_has_viscosity": (C.c_int, [vp]),
+        "orc_set_surface_tension": (C.c_int, [vp, C.c_double]),
+        "orc_capillary_step_limit": (C.c_double, [vp]),
+        "orc_apply_surface_tension": (None, [vp, C.c_double]),
+        "orc_capillary_geometry": (None, [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
         "orc_density": (None, [vp, _dp]),
         "orc_face_weights": (None, [vp, _dp, _dp]),
         "orc_apply_operator": (C.c_double, [vp, _dp, _dp]),
@@ -280,6 +284,16 @@ class OracleGrid:
         st = lib().orc_apply_viscosity(self._h, dt, C.byref(it))
         return st, it.value
 
+    def set_surface_tension(self, coefficient):
+        if lib().orc_set_surface_tension(self._h, coefficient) != 0:
+            raise ValueError("Surface tension must be finite and nonnegative")
+
+    def capillary_step_limit(self):
+        return lib().orc_capillary_step_limit(self._h)
+
+    def apply_surface_tension(self, dt):
+        lib().orc_apply_surface_tension(self._h, dt)
+
     def sample_velocity(self, x, y):
         vx, vy = C.c_double(), C.c_double()
         lib().orc_sample_velocity(self._h, x, y, C.byref(vx), C.byref(vy))
@@ -332,3 +346,12 @@ def viscosity_material(cell_size, liquid_density, liquid_viscosity, liquid_mass,
     lib().orc_viscosity_material(cells, cell_size, ld.size, _ptr(ld), _ptr(lv), _ptr(lm), gd.size, _ptr(gd),
                                  _ptr(gv), _ptr(gm), _ptr(out))
     return out
+
+
+def capillary_geometry(width, height, cell_size, fraction):
+    """Heights (N x 4), raw and completed curvatures (N x 2: value, valid) of a fraction field."""
+    f = _f64(fraction)
+    n = width * height
+    heights, raw, curv = np.zeros(4 * n), np.zeros(2 * n), np.zeros(2 * n)
+    lib().orc_capillary_geometry(width, height, cell_size, _ptr(f), _ptr(heights), _ptr(raw), _ptr(curv))
+    return heights.reshape(n, 4), raw.reshape(n, 2), curv.reshape(n, 2)
