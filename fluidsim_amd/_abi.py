@@ -88,6 +88,8 @@ SIGNATURES = {
     "fs_max_divergence": [_vp, _dp],
     "fs_advect": [_vp, C.c_double, _ip],
     "fs_apply_viscosity": [_vp, C.c_double, _ip],
+    "fs_apply_surface_tension": [_vp, C.c_double],
+    "fs_capillary_geometry": [_vp, _dp, _dp],
     "fs_set_component_viscosity": [_vp, C.c_int, C.c_int, C.c_double],
     "fs_viscosity_solve": [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _ip,
                            _dp],

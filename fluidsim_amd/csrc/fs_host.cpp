@@ -194,6 +194,7 @@ struct fs_grid {
     DevBuf<unsigned char> tr_split;
     DevBuf<float> calib_src, calib_dst;   // 64 MiB copy-bandwidth reference (allocated on first use)
     ViscositySolver* visc = nullptr;       // implicit viscosity workspace (created on first use)
+    DevBuf<double2> st_heights, st_orient, st_raw, st_curv;   // capillary geometry (first use)
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<PcgScalars> S;
     DevBuf<double> partials;
@@ -215,6 +216,37 @@ struct fs_grid {
     {
         drop_graph();
         viscosity_destroy(visc);
+    }
+
+    // capillaryStepLimit (FluidGridCPU.cpp:843-860)
+    double capillary_step_limit() const
+    {
+        if (sigma == 0.0 || rho_l.empty() || rho_g.empty())
+            return std::numeric_limits<double>::infinity();
+        double ml = rho_l.front(), mg = rho_g.front();
+        for (double d : rho_l)
+            ml = std::min(ml, d);
+        for (double d : rho_g)
+            mg = std::min(mg, d);
+        constexpr double pi = 3.14159265358979323846;
+        return 0.5 * std::sqrt((ml + mg) * h * h * h / (2.0 * pi * sigma));
+    }
+
+    // applySurfaceTension (FluidGridCPU.cpp:862-926): geometry, then the face force with the FP64
+    // pressure face weights of the current density.
+    void apply_surface_tension(double dt)
+    {
+        const size_t n = N();
+        if (st_curv.n != n) {
+            bytes += st_heights.alloc(n);
+            bytes += st_orient.alloc(n);
+            bytes += st_raw.alloc(n);
+            bytes += st_curv.alloc(n);
+        }
+        launch_capillary_geometry(st(), W, H, h, fraction.p, st_heights.p, st_orient.p, st_raw.p, st_curv.p);
+        rebuild_coefficients();
+        const Level& L0 = levels[0];
+        launch_capillary_force(st(), W, H, h, fraction.p, st_curv.p, L0.right64.p, L0.down64.p, sigma, dt, u.p, v.p);
     }
 
     bool has_viscosity() const
@@ -849,9 +881,9 @@ fs_status fs_set_surface_tension(fs_grid* grid, double coefficient)
 {
     return guard([&] {
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
-        if (coefficient != 0.0)
-            fail(FS_E_UNSUPPORTED, "Surface tension is not implemented by the HIP backend yet");
-        grid->sigma = 0.0;
+        need(std::isfinite(coefficient) && coefficient >= 0.0, FS_E_INVALID_ARGUMENT,
+            "Surface tension must be finite and nonnegative");
+        grid->sigma = coefficient;
     });
 }
 
@@ -859,7 +891,7 @@ fs_status fs_capillary_step_limit(const fs_grid* grid, double* limit)
 {
     return guard([&] {
         need(grid != nullptr && limit != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
-        *limit = std::numeric_limits<double>::infinity();
+        *limit = grid->capillary_step_limit();
     });
 }
 
@@ -1285,27 +1317,94 @@ fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergenc
             *slot += std::chrono::duration<double, std::milli>(now - t0).count();
             t0 = now;
         };
-        int transport_steps = 0;
-        if (g.transport == FS_TRANSPORT_REFERENCE) {
-            advect_reference(g, dt);   // UniformCPU reports no transport subdivision count
-        } else if (g.transport == FS_TRANSPORT_SEMI_LAGRANGIAN) {
-            advect_sl(g, dt);
-            transport_steps = 1;
+        // capillary subdivision (FluidGridCPU.cpp:319-357): count substeps of dt / count, each one
+        // transport, viscosity (the WebGPU order, FluidGridGPU.cpp:1134-1158), gravity, surface
+        // tension, projection; stop after the first substep that does not converge.
+        const bool tension = g.sigma > 0.0;
+        const double subdivisions = tension ? std::max(1.0, std::ceil(dt / g.capillary_step_limit())) : 1.0;
+        if (subdivisions > (double)std::numeric_limits<int>::max())
+            fail(FS_E_RUNTIME, "Surface tension requires too many physical substeps for this timestep");
+        const int count = (int)subdivisions;
+        const double step = dt / count;
+        res.physical_steps = 0;
+        for (int i = 0; i < count; ++i) {
+            if (g.transport == FS_TRANSPORT_REFERENCE) {
+                advect_reference(g, step);   // UniformCPU reports no transport subdivision count
+            } else if (g.transport == FS_TRANSPORT_SEMI_LAGRANGIAN) {
+                advect_sl(g, step);
+                res.transport_steps += 1;
+            }
+            lap(timings ? &timings->transport_ms : nullptr);
+            const int viscosity_iterations = apply_viscosity(g, step);
+            res.viscosity_iterations += viscosity_iterations;
+            if (viscosity_iterations)
+                lap(timings ? &timings->viscosity_ms : nullptr);
+            launch_gravity(g.st(), g.dev(), g.gravity_y, step);
+            if (tension)
+                g.apply_surface_tension(step);
+            lap(timings ? &timings->forces_ms : nullptr);
+            const fs_projection_result proj = g.project(step, max_iterations, divergence_tolerance);
+            lap(timings ? &timings->pressure_ms : nullptr);
+            res.iterations += proj.iterations;
+            res.max_divergence = proj.max_divergence;
+            res.converged = proj.converged;
+            res.restarts = proj.restarts;
+            res.initial_residual = proj.initial_residual;
+            res.final_residual = proj.final_residual;
+            res.rhs_max = proj.rhs_max;
+            ++res.physical_steps;
+            if (!res.converged)
+                break;
         }
-        lap(timings ? &timings->transport_ms : nullptr);
-        // the WebGPU step's order: transport, viscosity, forces, projection (FluidGridGPU.cpp:1134-1158)
-        const int viscosity_iterations = apply_viscosity(g, dt);
-        if (viscosity_iterations)
-            lap(timings ? &timings->viscosity_ms : nullptr);
-        launch_gravity(g.st(), g.dev(), g.gravity_y, dt);
-        lap(timings ? &timings->forces_ms : nullptr);
-        const fs_projection_result proj = g.project(dt, max_iterations, divergence_tolerance);
-        lap(timings ? &timings->pressure_ms : nullptr);
-        res = proj;
-        res.physical_steps = 1;
-        res.transport_steps = transport_steps;
-        res.viscosity_iterations = viscosity_iterations;
         *result = res;
+    });
+}
+
+fs_status fs_apply_surface_tension(fs_grid* grid, double dt)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before applying surface tension");
+        set_device(grid->ctx);
+        if (dt > 0.0 && grid->sigma > 0.0)
+            grid->apply_surface_tension(dt);
+        HIPC(hipStreamSynchronize(grid->st()));
+    });
+}
+
+fs_status fs_capillary_geometry(fs_grid* grid, double* heights, double* curvature)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state first");
+        set_device(grid->ctx);
+        fs_grid& g = *grid;
+        const size_t n = g.N();
+        if (g.st_curv.n != n) {
+            g.bytes += g.st_heights.alloc(n);
+            g.bytes += g.st_orient.alloc(n);
+            g.bytes += g.st_raw.alloc(n);
+            g.bytes += g.st_curv.alloc(n);
+        }
+        launch_capillary_geometry(g.st(), g.W, g.H, g.h, g.fraction.p, g.st_heights.p, g.st_orient.p, g.st_raw.p,
+            g.st_curv.p);
+        std::vector<double2> hv(n), ov(n), cv(n);
+        HIPC(hipMemcpyAsync(hv.data(), g.st_heights.p, n * sizeof(double2), hipMemcpyDeviceToHost, g.st()));
+        HIPC(hipMemcpyAsync(ov.data(), g.st_orient.p, n * sizeof(double2), hipMemcpyDeviceToHost, g.st()));
+        HIPC(hipMemcpyAsync(cv.data(), g.st_curv.p, n * sizeof(double2), hipMemcpyDeviceToHost, g.st()));
+        HIPC(hipStreamSynchronize(g.st()));
+        for (size_t i = 0; i < n; ++i) {
+            if (heights) {
+                heights[4 * i] = hv[i].x;
+                heights[4 * i + 1] = hv[i].y;
+                heights[4 * i + 2] = ov[i].x;
+                heights[4 * i + 3] = ov[i].y;
+            }
+            if (curvature) {
+                curvature[2 * i] = cv[i].x;
+                curvature[2 * i + 1] = cv[i].y;
+            }
+        }
     });
 }
 

@@ -182,6 +182,15 @@ double* viscosity_material_buffer(ViscositySolver* s);   // 2 * W * H doubles
 int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material, double dt,
     double spacing, double tol, int maximum, int* iterations, double* relative_residual);
 
+// ---- surface tension (FluidGridCPU::applySurfaceTension, FluidSurfaceTensionCPU) ----
+// Heights (x, y) and orientations per cell, raw and completed curvature (value, valid) per cell.
+void launch_capillary_geometry(hipStream_t st, int W, int H, double cell_size, const double* fraction,
+    double2* heights, double2* orientations, double2* raw, double2* curvature);
+// u, v += dt w sigma kappa_face (f_R - f_L) / h on interior faces; w = level-0 right64 / down64.
+void launch_capillary_force(hipStream_t st, int W, int H, double cell_size, const double* fraction,
+    const double2* curvature, const double* right64, const double* down64, double sigma, double dt, double* u,
+    double* v);
+
 // ---- bench helper: one operator application without reductions (for event timing) ----
 void launch_copy_f32(hipStream_t st, const float* src, float* dst, size_t n);
 

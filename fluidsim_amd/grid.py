@@ -235,6 +235,18 @@ class FluidGridHIP:
         mixture = self.liquid_mixture if ph == 0 else self.gas_mixture
         mixture.components[index].dynamic_viscosity = float(coefficient)
 
+    def applySurfaceTension(self, dt: float):
+        """FluidGridCPU::applySurfaceTension (FluidGridCPU.cpp:862-926)."""
+        call("fs_apply_surface_tension", self._h, float(dt))
+
+    def capillaryGeometry(self):
+        """(heights N x 4, curvature N x 2) of the current fractions (FluidSurfaceTensionCPU)."""
+        n = self.width * self.height
+        hts = np.zeros(4 * n)
+        cur = np.zeros(2 * n)
+        call("fs_capillary_geometry", self._h, _dp(hts), _dp(cur))
+        return hts.reshape(n, 4), cur.reshape(n, 2)
+
     def applyViscosity(self, dt: float) -> int:
         """FluidGridGPU::applyViscosity (FluidGridGPU.cpp:1039-1046); returns the CG iterations."""
         n = C.c_int()

@@ -118,9 +118,12 @@ fs_status fs_set_component_viscosity(fs_grid* grid, int phase, int component, do
 /* getGravityY/setGravityY (FluidGridBase.h:70-71) */
 fs_status fs_set_gravity(fs_grid* grid, double gravity_y);
 fs_status fs_get_gravity(const fs_grid* grid, double* gravity_y);
-/* setSurfaceTension (FluidGridBase.h:76-80): only 0 is accepted for now (FS_E_UNSUPPORTED otherwise). */
+/* setSurfaceTension (FluidGridBase.h:76-80, FluidGridCPU.cpp:835-841): finite and non-negative, else
+ * FS_E_INVALID_ARGUMENT.  A positive coefficient makes fs_step subdivide dt into capillary substeps and
+ * add the capillary face force before each projection (FluidGridCPU.cpp:319-357, 862-926). */
 fs_status fs_set_surface_tension(fs_grid* grid, double coefficient);
-/* capillaryStepLimit (FluidGridBase.h:84): +inf without surface tension. */
+/* capillaryStepLimit (FluidGridBase.h:84, FluidGridCPU.cpp:843-860): 0.5 sqrt((rho_l,min + rho_g,min) h^3 /
+ * (2 pi sigma)), +inf without surface tension or without one of the phases. */
 fs_status fs_capillary_step_limit(const fs_grid* grid, double* limit);
 /* FluidGridCPU::pressure_preconditioner (FluidGridCPU.h:39-44) */
 fs_status fs_set_preconditioner(fs_grid* grid, fs_preconditioner kind);
@@ -167,6 +170,12 @@ fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations);
 fs_status fs_viscosity_solve(fs_context* ctx, int width, int height, double cell_size, const double* material,
     double* u, double* v, double dt, double tolerance, int max_iterations, int* iterations,
     double* relative_residual);
+/* FluidGridCPU::applySurfaceTension(dt) (FluidGridCPU.cpp:862-926; private in the reference, exposed for
+ * parity tests): capillary geometry of the current fractions, then u, v += dt w sigma kappa df / h. */
+fs_status fs_apply_surface_tension(fs_grid* grid, double dt);
+/* Capillary geometry of the current fractions (FluidSurfaceTensionCPU.h:134-376), for parity tests:
+ * heights[4N] = (height x, height y, orientation x, orientation y), curvature[2N] = (value, valid). */
+fs_status fs_capillary_geometry(fs_grid* grid, double* heights, double* curvature);
 /* Semi-Lagrangian transport alone (build-defined mode, no reference counterpart). */
 fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt);
 
