@@ -97,6 +97,9 @@ struct Level {
     DevBuf<int> d_pxf, d_pxs, d_pyf, d_pys, d_rxc, d_rxi, d_ryc, d_ryi;
     DevBuf<float> d_pxw, d_pyw, d_rxw, d_ryw;
 
+    int smooth_variant = 0;     // tile of k_rb_multi on this level (autotuned)
+    int restrict_variant = 0;   // tile of k_res_restrict from this level (autotuned)
+
     size_t padded() const { return (size_t)s * (h + 2); }
 };
 
@@ -350,6 +353,7 @@ struct fs_grid {
         const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
         bytes += partials.alloc(std::max<size_t>(4 * blocks, 64));
         build_hierarchy();
+        tune_tiles();
     }
 
     void fill_tail(int start)
@@ -553,11 +557,11 @@ struct fs_grid {
             LevelView L = view(l);
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
-            launch_rb_multi(st(), L, 0, rhs, shift, 0, 1, none, status, l == 0);
             const Level& F = levels[l];
+            launch_rb_multi(st(), L, 0, rhs, shift, 0, 1, none, status, l == 0, F.smooth_variant);
             AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
             AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
-            launch_res_restrict(st(), L, 1, rhs, shift, view(l + 1), rx, ry, status);
+            launch_res_restrict(st(), L, 1, rhs, shift, view(l + 1), rx, ry, status, F.restrict_variant);
         }
         launch_mg_tail(st(), tail, tail_lds, T == 0 ? rmean_ptr() : nullptr, status);
         for (int l = T - 1; l >= 0; --l) {
@@ -567,8 +571,52 @@ struct fs_grid {
             CoarseLink link{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
-            launch_rb_multi(st(), L, 1, rhs, shift, 1, 0, link, status, l == 0);
+            launch_rb_multi(st(), L, 1, rhs, shift, 1, 0, link, status, l == 0, F.smooth_variant);
         }
+    }
+
+    // Autotuner: times every tile variant of the level smoother (plain sweeps) and of the residual +
+    // restriction on each level's own buffers and keeps the fastest.  The fused kernels compute the same
+    // values with any tile, so the choice affects time only.
+    void tune_tiles()
+    {
+        const CoarseLink none{};
+        auto time = [&](auto&& launch) {
+            launch();
+            HIPC(hipEventRecord(ctx->ev0, st()));
+            for (int k = 0; k < 3; ++k)
+                launch();
+            HIPC(hipEventRecord(ctx->ev1, st()));
+            HIPC(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            return ms;
+        };
+        for (int l = 0; l < tail_start; ++l) {
+            Level& F = levels[l];
+            const LevelView L = view(l);
+            const float* rhs = l == 0 ? r.p : L.rhs;
+            float best = std::numeric_limits<float>::infinity();
+            for (int v = 0; v < kSmoothVariants; ++v) {
+                const float ms = time([&] { launch_rb_multi(st(), L, 0, rhs, nullptr, 0, 0, none, nullptr, l == 0, v); });
+                if (ms < best) {
+                    best = ms;
+                    F.smooth_variant = v;
+                }
+            }
+            const AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
+            const AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+            const LevelView C = view(l + 1);
+            best = std::numeric_limits<float>::infinity();
+            for (int v = 0; v < kRestrictVariants; ++v) {
+                const float ms = time([&] { launch_res_restrict(st(), L, 0, rhs, nullptr, C, rx, ry, nullptr, v); });
+                if (ms < best) {
+                    best = ms;
+                    F.restrict_variant = v;
+                }
+            }
+        }
+        drop_graph();   // launches captured before the choice keep the old tiles
     }
 
     const float* z_ptr() const { return precond == FS_PRECOND_MULTIGRID ? levels[0].sol0.p : zdiag.p; }
@@ -1695,7 +1743,8 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
                     const Level& Cl = g.levels[1];
                     link = CoarseLink{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
                 }
-                launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 1, 0, link, nullptr, 1);
+                launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 1, 0, link, nullptr, 1,
+                    g.levels[0].smooth_variant);
                 break;
             }
             case 1: launch_pcg_ad(g.st(), gd); break;
@@ -1703,7 +1752,8 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
             case 3: launch_copy_f32(g.st(), g.r.p, g.q.p, g.NP()); break;
             case 4:  // second level's smoother (or the finest, if the tail starts at level 1)
                 launch_rb_multi(g.st(), g.view(l1), i & 1, l1 == 0 ? g.r.p : g.view(l1).rhs,
-                    l1 == 0 ? g.rmean_ptr() : nullptr, 0, 0, CoarseLink{}, nullptr, l1 == 0);
+                    l1 == 0 ? g.rmean_ptr() : nullptr, 0, 0, CoarseLink{}, nullptr, l1 == 0,
+                    g.levels[l1].smooth_variant);
                 break;
             case 5: {
                 if (g.tail_start == 0 || nlev < 2)
@@ -1711,7 +1761,8 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
                 const Level& F = g.levels[0];
                 AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
                 AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
-                launch_res_restrict(g.st(), L0, 1, g.r.p, g.rmean_ptr(), g.view(1), rx, ry, nullptr);
+                launch_res_restrict(g.st(), L0, 1, g.r.p, g.rmean_ptr(), g.view(1), rx, ry, nullptr,
+                    F.restrict_variant);
                 break;
             }
             case 6: launch_mg_tail(g.st(), g.tail, g.tail_lds, g.tail_start == 0 ? g.rmean_ptr() : nullptr,

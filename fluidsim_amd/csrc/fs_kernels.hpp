@@ -122,11 +122,15 @@ void launch_precond_diag(hipStream_t st, const GridDev& g);
 // ---- multigrid V-cycle pieces (FP32) ----
 // Four red-black sweeps in one launch (reads sol[cur], writes sol[cur ^ 1]); with link.csol set the
 // coarse correction is prolonged and added first.  fine selects the finest-level symbol.
+// variant: interior tile 0 = 64 x 32, 1 = 32 x 32, 2 = 32 x 16, 3 = 16 x 16 (same results, different speed).
+constexpr int kSmoothVariants = 4;
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
-    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine);
-// Residual of sol[cur] and its restriction into C.rhs in one launch.
+    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant);
+// Residual of sol[cur] and its restriction into C.rhs in one launch; variant: coarse tile 0 = 32 x 16,
+// 1 = 16 x 16, 2 = 16 x 8, 3 = 8 x 8.
+constexpr int kRestrictVariants = 4;
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
-    const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status);
+    const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status, int variant);
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status);
 
 // ---- reference PLIC/VOF transport (FP64, reference layouts, bit-exact) ----

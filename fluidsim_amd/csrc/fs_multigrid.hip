@@ -580,10 +580,42 @@ __device__ __forceinline__ TailArrays tail_arrays(float* lds, const TailLevel& T
     return A;
 }
 
-// Tail loops: a thread owns at most kTailIter items of a level; every LDS read goes through a clamped
+// Tail loops: a thread owns at most ITER items of a level; every LDS read goes through a clamped
 // (valid) index and only the stores are predicated, so the reads of all items issue together.
+// NT threads take part: the whole workgroup (barrier between phases) for the larger tail levels, or
+// wavefront 0 alone (WAVE; no workgroup barrier, only ordering inside the wavefront) for levels of at
+// most 32 x 32 cells, where a workgroup barrier per phase would cost more than the work.
 constexpr int kTailIter = 4;
+constexpr int kWaveSmoothIter = 8;     // (w+1)/2 * h <= 512 half-sweep items on one wavefront
+constexpr int kWaveCellIter = 16;      // w * h <= 1024 cells on one wavefront
+constexpr int kTailChunk = 4;          // items whose reads issue together before their stores
 
+// Orders the LDS accesses of one wavefront: its LDS operations complete in issue order, so a compiler
+// barrier plus a wait for outstanding LDS operations is enough between phases.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool WAVE> __device__ __forceinline__ void tail_sync()
+{
+    if (WAVE)
+        wave_sync();
+    else
+        __syncthreads();
+}
+
+// A level fits one wavefront when its cells, its half-sweep items and its coarse cells fit the
+// per-lane item counts of the wavefront loops.
+__device__ __forceinline__ bool tail_wave_level(const TailLevel& T)
+{
+    return T.w * T.h <= 64 * kWaveCellIter && ((T.w + 1) >> 1) * T.h <= 64 * kWaveSmoothIter &&
+           ((T.w + 1) >> 1) * ((T.h + 1) >> 1) <= 64 * ((kWaveCellIter + 3) / 4);
+}
+
+template <int NT, int ITER, bool WAVE>
 __device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
 {
     const int w = T.w, h = T.h, s = T.s;
@@ -592,67 +624,75 @@ __device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse
     for (int pass = 0; pass < 2; ++pass) {
         const int color = reverse ? 1 - pass : pass;
         const bool neighbours = !(from_zero && pass == 0);
-        float v[kTailIter];
-        int idx[kTailIter];
-        bool on[kTailIter];
+#pragma unroll 1
+        for (int c0 = 0; c0 < ITER; c0 += kTailChunk) {
+        float v[kTailChunk];
+        int idx[kTailChunk];
+        bool on[kTailChunk];
 #pragma unroll
-        for (int it = 0; it < kTailIter; ++it) {
-            const int t = min((int)threadIdx.x + it * kTailThreads, items - 1);
+        for (int k = 0; k < kTailChunk; ++k) {
+            const int it = c0 + k;
+            const int t = min((int)threadIdx.x + it * NT, items - 1);
             const int y = t / pairs;
             const int x = 2 * (t - y * pairs) + ((color ^ y) & 1);
-            on[it] = (int)threadIdx.x + it * kTailThreads < items && x < w;
+            on[k] = (int)threadIdx.x + it * NT < items && x < w;
             const int i = (y + 1) * s + min(x, w - 1) + 1;
-            idx[it] = i;
+            idx[k] = i;
             const float rl = A.right[i - 1], rr = A.right[i], du = A.down[i - s], dd = A.down[i];
             float acc = A.rhs[i];
             const float n = (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
             if (neighbours)
                 acc += n;
-            v[it] = acc * inv_or_zero((rl + rr) + (du + dd));
+            v[k] = acc * inv_or_zero((rl + rr) + (du + dd));
         }
 #pragma unroll
-        for (int it = 0; it < kTailIter; ++it)
-            if (on[it])
-                A.sol[idx[it]] = v[it];
-        __syncthreads();
+        for (int k = 0; k < kTailChunk; ++k)
+            if (on[k])
+                A.sol[idx[k]] = v[k];
+        }
+        tail_sync<WAVE>();
     }
 }
 
+template <int NT, int ITER, bool WAVE>
 __device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, const TailLevel& Cn,
     const TailArrays& Cd)
 {
     const int w = T.w, h = T.h, s = T.s;
     const int cells = w * h;
-    {
-        float rv[kTailIter];
-        int idx[kTailIter];
-        bool on[kTailIter];
+#pragma unroll 1
+    for (int c0 = 0; c0 < ITER; c0 += kTailChunk) {
+        float rv[kTailChunk];
+        int idx[kTailChunk];
+        bool on[kTailChunk];
 #pragma unroll
-        for (int it = 0; it < kTailIter; ++it) {
-            const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
-            on[it] = (int)threadIdx.x + it * kTailThreads < cells;
+        for (int k = 0; k < kTailChunk; ++k) {
+            const int it = c0 + k;
+            const int e = min((int)threadIdx.x + it * NT, cells - 1);
+            on[k] = (int)threadIdx.x + it * NT < cells;
             const int y = e / w, x = e - y * w;
             const int i = (y + 1) * s + x + 1;
-            idx[it] = i;
+            idx[k] = i;
             const float c = F.sol[i];
             const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
                                   (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
-            rv[it] = F.rhs[i] - product;
+            rv[k] = F.rhs[i] - product;
         }
 #pragma unroll
-        for (int it = 0; it < kTailIter; ++it)
-            if (on[it])
-                F.res[idx[it]] = rv[it];
+        for (int k = 0; k < kTailChunk; ++k)
+            if (on[k])
+                F.res[idx[k]] = rv[k];
     }
-    __syncthreads();
+    tail_sync<WAVE>();
     {
         // missing taps carry weight 0 and index -1, which addresses the zero halo
+        constexpr int CITER = (ITER + 3) / 4 > 1 ? (ITER + 3) / 4 : 2;   // coarse cells are a quarter
         const int cells_c = Cn.w * Cn.h;
-        float val[2];
-        int dst[2];
+        float val[CITER];
+        int dst[CITER];
 #pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const int e = min((int)threadIdx.x + it * kTailThreads, cells_c - 1);
+        for (int it = 0; it < CITER; ++it) {
+            const int e = min((int)threadIdx.x + it * NT, cells_c - 1);
             const int y = e / Cn.w, x = e - y * Cn.w;
             float value = 0.f;
 #pragma unroll
@@ -668,24 +708,28 @@ __device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, 
             dst[it] = (y + 1) * Cn.s + x + 1;
         }
 #pragma unroll
-        for (int it = 0; it < 2; ++it)
-            if ((int)threadIdx.x + it * kTailThreads < cells_c)
+        for (int it = 0; it < CITER; ++it)
+            if ((int)threadIdx.x + it * NT < cells_c)
                 Cd.rhs[dst[it]] = val[it];
     }
-    __syncthreads();
+    tail_sync<WAVE>();
 }
 
+template <int NT, int ITER, bool WAVE>
 __device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const TailLevel& Cn, const TailArrays& Cd)
 {
     const int w = T.w, h = T.h;
     const int cells = w * h;
-    float add[kTailIter], cur[kTailIter];
-    int idx[kTailIter];
-    bool on[kTailIter];
+#pragma unroll 1
+    for (int c0 = 0; c0 < ITER; c0 += kTailChunk) {
+    float add[kTailChunk], cur[kTailChunk];
+    int idx[kTailChunk];
+    bool on[kTailChunk];
 #pragma unroll
-    for (int it = 0; it < kTailIter; ++it) {
-        const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
-        on[it] = (int)threadIdx.x + it * kTailThreads < cells;
+    for (int k = 0; k < kTailChunk; ++k) {
+        const int it = c0 + k;
+        const int e = min((int)threadIdx.x + it * NT, cells - 1);
+        on[k] = (int)threadIdx.x + it * NT < cells;
         const int y = e / w, x = e - y * w;
         const float wy = F.pyw[y];
         const float* top = Cd.sol + (F.pyf[y] + 1) * Cn.s + 1;
@@ -694,46 +738,45 @@ __device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const Tail
         const float wx = F.pxw[x];
         const float a = (1.f - wx) * top[a0] + wx * top[a1];
         const float b = (1.f - wx) * bottom[a0] + wx * bottom[a1];
-        add[it] = (1.f - wy) * a + wy * b;
-        idx[it] = (y + 1) * T.s + x + 1;
-        cur[it] = F.sol[idx[it]];
+        add[k] = (1.f - wy) * a + wy * b;
+        idx[k] = (y + 1) * T.s + x + 1;
+        cur[k] = F.sol[idx[k]];
     }
 #pragma unroll
-    for (int it = 0; it < kTailIter; ++it)
-        if (on[it])
-            F.sol[idx[it]] = cur[it] + add[it];
-    __syncthreads();
+    for (int k = 0; k < kTailChunk; ++k)
+        if (on[k])
+            F.sol[idx[k]] = cur[k] + add[k];
+    }
+    tail_sync<WAVE>();
 }
 
 // solveCoarsest (PressureMultigrid.cpp:347-384): remove the rhs mean, solve the pinned system
-// (here with its precomputed inverse), shift to a mean-zero gauge.  Wavefront 0, one lane per row.
+// (here with its precomputed inverse), shift to a mean-zero gauge.  Wavefront 0, one lane per row;
+// the caller orders the result for the other threads.
 __device__ void tail_coarse_solve(const TailLevel& T, const TailArrays& A, const float* G, int n)
 {
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        const int w = T.w, s = T.s;
-        const int cells = n + 1;
-        float b = 0.f;
-        if (lane < cells)
-            b = A.rhs[(lane / w + 1) * s + lane % w + 1];
-        const double mean = wave_sum((double)b) / cells;
-        // y_j = rhs(cell j + 1) - mean, held by lane j + 1
-        const float y = (float)((double)b - mean);
-        float c = 0.f;
-        for (int j = 0; j < n; ++j) {
-            const float yj = __shfl(y, j + 1, 64);
-            if (lane < n)
-                c += G[lane * n + j] * yj;
-        }
-        const double gauge = wave_sum(lane < n ? (double)c : 0.0) / cells;
-        if (lane == 0)
-            A.sol[s + 1] = (float)(-gauge);
-        if (lane < n) {
-            const int cell = lane + 1;
-            A.sol[(cell / w + 1) * s + cell % w + 1] = (float)((double)c - gauge);
-        }
+    const int lane = threadIdx.x;
+    const int w = T.w, s = T.s;
+    const int cells = n + 1;
+    float b = 0.f;
+    if (lane < cells)
+        b = A.rhs[(lane / w + 1) * s + lane % w + 1];
+    const double mean = wave_sum((double)b) / cells;
+    // y_j = rhs(cell j + 1) - mean, held by lane j + 1
+    const float y = (float)((double)b - mean);
+    float c = 0.f;
+    for (int j = 0; j < n; ++j) {
+        const float yj = __shfl(y, j + 1, 64);
+        if (lane < n)
+            c += G[lane * n + j] * yj;
     }
-    __syncthreads();
+    const double gauge = wave_sum(lane < n ? (double)c : 0.0) / cells;
+    if (lane == 0)
+        A.sol[s + 1] = (float)(-gauge);
+    if (lane < n) {
+        const int cell = lane + 1;
+        A.sol[(cell / w + 1) * s + cell % w + 1] = (float)((double)c - gauge);
+    }
 }
 
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, const double* shift, const int* status)
@@ -815,21 +858,44 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
     float* G = lds + D.lds_ginv;
     const int n = D.coarse_n;
     __syncthreads();
-    for (int l = 0; l + 1 < nl; ++l) {
+    // levels [0, ws) by the whole workgroup, [ws, nl) by wavefront 0 alone (sizes only shrink)
+    int ws = 0;
+    while (ws + 1 < nl && !tail_wave_level(D.lev[ws]))
+        ++ws;
+    for (int l = 0; l < ws; ++l) {
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
-        tail_smooth(T, A, 0, 1);
+        tail_smooth<kTailThreads, kTailIter, false>(T, A, 0, 1);
         for (int k = 1; k < 4; ++k)
-            tail_smooth(T, A, 0, 0);
-        tail_residual_restrict(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+            tail_smooth<kTailThreads, kTailIter, false>(T, A, 0, 0);
+        tail_residual_restrict<kTailThreads, kTailIter, false>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
     }
-    tail_coarse_solve(D.lev[nl - 1], tail_arrays(lds, D.lev[nl - 1]), G, n);
-    for (int l = nl - 2; l >= 0; --l) {
+    if (threadIdx.x < 64) {
+        for (int l = ws; l + 1 < nl; ++l) {
+            const TailLevel& T = D.lev[l];
+            const TailArrays A = tail_arrays(lds, T);
+            tail_smooth<64, kWaveSmoothIter, true>(T, A, 0, 1);
+            for (int k = 1; k < 4; ++k)
+                tail_smooth<64, kWaveSmoothIter, true>(T, A, 0, 0);
+            tail_residual_restrict<64, kWaveCellIter, true>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+        }
+        tail_coarse_solve(D.lev[nl - 1], tail_arrays(lds, D.lev[nl - 1]), G, n);
+        wave_sync();
+        for (int l = nl - 2; l >= ws; --l) {
+            const TailLevel& T = D.lev[l];
+            const TailArrays A = tail_arrays(lds, T);
+            tail_prolong<64, kWaveCellIter, true>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+            for (int k = 0; k < 4; ++k)
+                tail_smooth<64, kWaveSmoothIter, true>(T, A, 1, 0);
+        }
+    }
+    __syncthreads();
+    for (int l = ws - 1; l >= 0; --l) {
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
-        tail_prolong(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+        tail_prolong<kTailThreads, kTailIter, false>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
         for (int k = 0; k < 4; ++k)
-            tail_smooth(T, A, 1, 0);
+            tail_smooth<kTailThreads, kTailIter, false>(T, A, 1, 0);
     }
     const TailLevel& T0 = D.lev[0];
     const TailArrays A0 = tail_arrays(lds, T0);
@@ -840,8 +906,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
     }
 }
 
-constexpr int kMultiTX = 64, kMultiTY = 32, kMultiNS = 4;
-constexpr int kRestrictCX = 32, kRestrictCY = 16;
+constexpr int kMultiNS = 4;
 
 }  // namespace
 
@@ -874,32 +939,53 @@ bool tail_allow_lds(size_t bytes)
            hipSuccess;
 }
 
+// Tile variants (interior cells per workgroup).  The fused sweeps compute every interior cell exactly as
+// global sweeps would, whatever the tile, so the choice changes time only; small levels use small
+// tiles to spread over more compute units.
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
-    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine)
+    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant)
 {
-    const dim3 grid((L.w + kMultiTX - 1) / kMultiTX, (L.h + kMultiTY - 1) / kMultiTY);
     const int mode = from_zero ? 0 : (link.csol ? 1 : 2);
-#define FS_RB(F, M)                                                                                          \
-    hipLaunchKernelGGL((k_rb_multi<kMultiTX, kMultiTY, kMultiNS, F, M>), grid, dim3(kNT), 0, st, L, cur, rhs, \
-        shift, first_color, link, status)
-    if (fine) {
-        if (mode == 0) FS_RB(1, 0);
-        else if (mode == 1) FS_RB(1, 1);
-        else FS_RB(1, 2);
-    } else {
-        if (mode == 0) FS_RB(0, 0);
-        else if (mode == 1) FS_RB(0, 1);
-        else FS_RB(0, 2);
+#define FS_RB(TX, TY, F, M)                                                                                   \
+    hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, F, M>), dim3((L.w + TX - 1) / TX, (L.h + TY - 1) / TY),   \
+        dim3(kNT), 0, st, L, cur, rhs, shift, first_color, link, status)
+#define FS_RB_MODES(TX, TY, F)          \
+    do {                                \
+        if (mode == 0)                  \
+            FS_RB(TX, TY, F, 0);        \
+        else if (mode == 1)             \
+            FS_RB(TX, TY, F, 1);        \
+        else                            \
+            FS_RB(TX, TY, F, 2);        \
+    } while (0)
+    switch (variant) {
+    case 1: FS_RB_MODES(32, 32, 0); break;
+    case 2: FS_RB_MODES(32, 16, 0); break;
+    case 3: FS_RB_MODES(16, 16, 0); break;
+    default:
+        if (fine)
+            FS_RB_MODES(64, 32, 1);
+        else
+            FS_RB_MODES(64, 32, 0);
+        break;
     }
+#undef FS_RB_MODES
 #undef FS_RB
 }
 
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
-    const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status)
+    const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status, int variant)
 {
-    const dim3 grid((C.w + kRestrictCX - 1) / kRestrictCX, (C.h + kRestrictCY - 1) / kRestrictCY);
-    hipLaunchKernelGGL((k_res_restrict<kRestrictCX, kRestrictCY>), grid, dim3(kNT), 0, st, F, cur, rhs, shift, C, rx,
-        ry, status);
+#define FS_RR(CX, CY)                                                                                         \
+    hipLaunchKernelGGL((k_res_restrict<CX, CY>), dim3((C.w + CX - 1) / CX, (C.h + CY - 1) / CY), dim3(kNT), 0, st, \
+        F, cur, rhs, shift, C, rx, ry, status)
+    switch (variant) {
+    case 1: FS_RR(16, 16); break;
+    case 2: FS_RR(16, 8); break;
+    case 3: FS_RR(8, 8); break;
+    default: FS_RR(32, 16); break;
+    }
+#undef FS_RR
 }
 
 void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, const double* down64,
