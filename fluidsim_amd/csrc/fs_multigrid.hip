@@ -580,156 +580,153 @@ __device__ __forceinline__ TailArrays tail_arrays(float* lds, const TailLevel& T
     return A;
 }
 
-// Tail loops: a thread owns at most ITER items of a level; every LDS read goes through a clamped
-// (valid) index and only the stores are predicated, so the reads of all items issue together.
-// NT threads take part: the whole workgroup (barrier between phases) for the larger tail levels, or
-// wavefront 0 alone (WAVE; no workgroup barrier, only ordering inside the wavefront) for levels of at
-// most 32 x 32 cells, where a workgroup barrier per phase would cost more than the work.
-constexpr int kTailIter = 4;
-constexpr int kWaveSmoothIter = 8;     // (w+1)/2 * h <= 512 half-sweep items on one wavefront
-constexpr int kWaveCellIter = 16;      // w * h <= 1024 cells on one wavefront
-constexpr int kTailChunk = 4;          // items whose reads issue together before their stores
+// Tail loops: a thread owns at most ITER items of a level (ITER chosen per level from its size, so a
+// small level does not execute the unrolled work of a large one), and a wavefront whose first item is
+// past the end skips the loop body entirely.  Every LDS read goes through a clamped (valid) index and
+// only the stores are predicated, so the reads of all items issue together.
+constexpr int kTailIter = 4;   // at most 4 * 1024 items per phase (the tail selection guarantees it)
 
-// Orders the LDS accesses of one wavefront: its LDS operations complete in issue order, so a compiler
-// barrier plus a wait for outstanding LDS operations is enough between phases.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+__device__ __forceinline__ bool wave_has_items(int items) { return (int)(threadIdx.x & ~63u) < items; }
 
-template <bool WAVE> __device__ __forceinline__ void tail_sync()
-{
-    if (WAVE)
-        wave_sync();
-    else
-        __syncthreads();
-}
-
-// A level fits one wavefront when its cells, its half-sweep items and its coarse cells fit the
-// per-lane item counts of the wavefront loops.
-__device__ __forceinline__ bool tail_wave_level(const TailLevel& T)
-{
-    return T.w * T.h <= 64 * kWaveCellIter && ((T.w + 1) >> 1) * T.h <= 64 * kWaveSmoothIter &&
-           ((T.w + 1) >> 1) * ((T.h + 1) >> 1) <= 64 * ((kWaveCellIter + 3) / 4);
-}
-
-template <int NT, int ITER, bool WAVE>
-__device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
+template <int ITER> __device__ void tail_smooth_n(const TailLevel& T, const TailArrays& A, int color, bool neighbours)
 {
     const int w = T.w, h = T.h, s = T.s;
     const int pairs = (w + 1) >> 1;
     const int items = pairs * h;
+    if (!wave_has_items(items))
+        return;
+    float v[ITER];
+    int idx[ITER];
+    bool on[ITER];
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int t = min((int)threadIdx.x + it * kTailThreads, items - 1);
+        const int y = t / pairs;
+        const int x = 2 * (t - y * pairs) + ((color ^ y) & 1);
+        on[it] = (int)threadIdx.x + it * kTailThreads < items && x < w;
+        const int i = (y + 1) * s + min(x, w - 1) + 1;
+        idx[it] = i;
+        const float rl = A.right[i - 1], rr = A.right[i], du = A.down[i - s], dd = A.down[i];
+        float acc = A.rhs[i];
+        const float n = (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
+        if (neighbours)
+            acc += n;
+        v[it] = acc * inv_or_zero((rl + rr) + (du + dd));
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; ++it)
+        if (on[it])
+            A.sol[idx[it]] = v[it];
+}
+
+__device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
+{
+    const int items = ((T.w + 1) >> 1) * T.h;
     for (int pass = 0; pass < 2; ++pass) {
         const int color = reverse ? 1 - pass : pass;
         const bool neighbours = !(from_zero && pass == 0);
-#pragma unroll 1
-        for (int c0 = 0; c0 < ITER; c0 += kTailChunk) {
-        float v[kTailChunk];
-        int idx[kTailChunk];
-        bool on[kTailChunk];
-#pragma unroll
-        for (int k = 0; k < kTailChunk; ++k) {
-            const int it = c0 + k;
-            const int t = min((int)threadIdx.x + it * NT, items - 1);
-            const int y = t / pairs;
-            const int x = 2 * (t - y * pairs) + ((color ^ y) & 1);
-            on[k] = (int)threadIdx.x + it * NT < items && x < w;
-            const int i = (y + 1) * s + min(x, w - 1) + 1;
-            idx[k] = i;
-            const float rl = A.right[i - 1], rr = A.right[i], du = A.down[i - s], dd = A.down[i];
-            float acc = A.rhs[i];
-            const float n = (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
-            if (neighbours)
-                acc += n;
-            v[k] = acc * inv_or_zero((rl + rr) + (du + dd));
-        }
-#pragma unroll
-        for (int k = 0; k < kTailChunk; ++k)
-            if (on[k])
-                A.sol[idx[k]] = v[k];
-        }
-        tail_sync<WAVE>();
+        if (items <= kTailThreads)
+            tail_smooth_n<1>(T, A, color, neighbours);
+        else if (items <= 2 * kTailThreads)
+            tail_smooth_n<2>(T, A, color, neighbours);
+        else
+            tail_smooth_n<kTailIter>(T, A, color, neighbours);
+        __syncthreads();
     }
 }
 
-template <int NT, int ITER, bool WAVE>
-__device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, const TailLevel& Cn,
-    const TailArrays& Cd)
+template <int ITER> __device__ void tail_residual_n(const TailLevel& T, const TailArrays& F)
 {
     const int w = T.w, h = T.h, s = T.s;
     const int cells = w * h;
-#pragma unroll 1
-    for (int c0 = 0; c0 < ITER; c0 += kTailChunk) {
-        float rv[kTailChunk];
-        int idx[kTailChunk];
-        bool on[kTailChunk];
+    if (!wave_has_items(cells))
+        return;
+    float rv[ITER];
+    int idx[ITER];
+    bool on[ITER];
 #pragma unroll
-        for (int k = 0; k < kTailChunk; ++k) {
-            const int it = c0 + k;
-            const int e = min((int)threadIdx.x + it * NT, cells - 1);
-            on[k] = (int)threadIdx.x + it * NT < cells;
-            const int y = e / w, x = e - y * w;
-            const int i = (y + 1) * s + x + 1;
-            idx[k] = i;
-            const float c = F.sol[i];
-            const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
-                                  (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
-            rv[k] = F.rhs[i] - product;
-        }
-#pragma unroll
-        for (int k = 0; k < kTailChunk; ++k)
-            if (on[k])
-                F.res[idx[k]] = rv[k];
+    for (int it = 0; it < ITER; ++it) {
+        const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
+        on[it] = (int)threadIdx.x + it * kTailThreads < cells;
+        const int y = e / w, x = e - y * w;
+        const int i = (y + 1) * s + x + 1;
+        idx[it] = i;
+        const float c = F.sol[i];
+        const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
+                              (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
+        rv[it] = F.rhs[i] - product;
     }
-    tail_sync<WAVE>();
-    {
-        // missing taps carry weight 0 and index -1, which addresses the zero halo
-        constexpr int CITER = (ITER + 3) / 4 > 1 ? (ITER + 3) / 4 : 2;   // coarse cells are a quarter
-        const int cells_c = Cn.w * Cn.h;
-        float val[CITER];
-        int dst[CITER];
 #pragma unroll
-        for (int it = 0; it < CITER; ++it) {
-            const int e = min((int)threadIdx.x + it * NT, cells_c - 1);
-            const int y = e / Cn.w, x = e - y * Cn.w;
-            float value = 0.f;
-#pragma unroll
-            for (int iy = 0; iy < 4; ++iy) {
-                const float* row = F.res + (F.ryi[y * 4 + iy] + 1) * s + 1;
-                float horizontal = 0.f;
-#pragma unroll
-                for (int ix = 0; ix < 4; ++ix)
-                    horizontal += F.rxw[x * 4 + ix] * row[F.rxi[x * 4 + ix]];
-                value += F.ryw[y * 4 + iy] * horizontal;
-            }
-            val[it] = value;
-            dst[it] = (y + 1) * Cn.s + x + 1;
-        }
-#pragma unroll
-        for (int it = 0; it < CITER; ++it)
-            if ((int)threadIdx.x + it * NT < cells_c)
-                Cd.rhs[dst[it]] = val[it];
-    }
-    tail_sync<WAVE>();
+    for (int it = 0; it < ITER; ++it)
+        if (on[it])
+            F.res[idx[it]] = rv[it];
 }
 
-template <int NT, int ITER, bool WAVE>
-__device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const TailLevel& Cn, const TailArrays& Cd)
+template <int ITER> __device__ void tail_restrict_n(const TailLevel& T, const TailArrays& F, const TailLevel& Cn,
+    const TailArrays& Cd)
+{
+    // missing taps carry weight 0 and index -1, which addresses the zero halo
+    const int s = T.s;
+    const int cells_c = Cn.w * Cn.h;
+    if (!wave_has_items(cells_c))
+        return;
+    float val[ITER];
+    int dst[ITER];
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int e = min((int)threadIdx.x + it * kTailThreads, cells_c - 1);
+        const int y = e / Cn.w, x = e - y * Cn.w;
+        float value = 0.f;
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            const float* row = F.res + (F.ryi[y * 4 + iy] + 1) * s + 1;
+            float horizontal = 0.f;
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+                horizontal += F.rxw[x * 4 + ix] * row[F.rxi[x * 4 + ix]];
+            value += F.ryw[y * 4 + iy] * horizontal;
+        }
+        val[it] = value;
+        dst[it] = (y + 1) * Cn.s + x + 1;
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; ++it)
+        if ((int)threadIdx.x + it * kTailThreads < cells_c)
+            Cd.rhs[dst[it]] = val[it];
+}
+
+__device__ void tail_residual_restrict(const TailLevel& T, const TailArrays& F, const TailLevel& Cn,
+    const TailArrays& Cd)
+{
+    const int cells = T.w * T.h;
+    if (cells <= kTailThreads)
+        tail_residual_n<1>(T, F);
+    else if (cells <= 2 * kTailThreads)
+        tail_residual_n<2>(T, F);
+    else
+        tail_residual_n<kTailIter>(T, F);
+    __syncthreads();
+    if (Cn.w * Cn.h <= kTailThreads)
+        tail_restrict_n<1>(T, F, Cn, Cd);
+    else
+        tail_restrict_n<2>(T, F, Cn, Cd);
+    __syncthreads();
+}
+
+template <int ITER> __device__ void tail_prolong_n(const TailLevel& T, const TailArrays& F, const TailLevel& Cn,
+    const TailArrays& Cd)
 {
     const int w = T.w, h = T.h;
     const int cells = w * h;
-#pragma unroll 1
-    for (int c0 = 0; c0 < ITER; c0 += kTailChunk) {
-    float add[kTailChunk], cur[kTailChunk];
-    int idx[kTailChunk];
-    bool on[kTailChunk];
+    if (!wave_has_items(cells))
+        return;
+    float add[ITER], cur[ITER];
+    int idx[ITER];
+    bool on[ITER];
 #pragma unroll
-    for (int k = 0; k < kTailChunk; ++k) {
-        const int it = c0 + k;
-        const int e = min((int)threadIdx.x + it * NT, cells - 1);
-        on[k] = (int)threadIdx.x + it * NT < cells;
+    for (int it = 0; it < ITER; ++it) {
+        const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
+        on[it] = (int)threadIdx.x + it * kTailThreads < cells;
         const int y = e / w, x = e - y * w;
         const float wy = F.pyw[y];
         const float* top = Cd.sol + (F.pyf[y] + 1) * Cn.s + 1;
@@ -738,16 +735,26 @@ __device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const Tail
         const float wx = F.pxw[x];
         const float a = (1.f - wx) * top[a0] + wx * top[a1];
         const float b = (1.f - wx) * bottom[a0] + wx * bottom[a1];
-        add[k] = (1.f - wy) * a + wy * b;
-        idx[k] = (y + 1) * T.s + x + 1;
-        cur[k] = F.sol[idx[k]];
+        add[it] = (1.f - wy) * a + wy * b;
+        idx[it] = (y + 1) * T.s + x + 1;
+        cur[it] = F.sol[idx[it]];
     }
 #pragma unroll
-    for (int k = 0; k < kTailChunk; ++k)
-        if (on[k])
-            F.sol[idx[k]] = cur[k] + add[k];
-    }
-    tail_sync<WAVE>();
+    for (int it = 0; it < ITER; ++it)
+        if (on[it])
+            F.sol[idx[it]] = cur[it] + add[it];
+}
+
+__device__ void tail_prolong(const TailLevel& T, const TailArrays& F, const TailLevel& Cn, const TailArrays& Cd)
+{
+    const int cells = T.w * T.h;
+    if (cells <= kTailThreads)
+        tail_prolong_n<1>(T, F, Cn, Cd);
+    else if (cells <= 2 * kTailThreads)
+        tail_prolong_n<2>(T, F, Cn, Cd);
+    else
+        tail_prolong_n<kTailIter>(T, F, Cn, Cd);
+    __syncthreads();
 }
 
 // solveCoarsest (PressureMultigrid.cpp:347-384): remove the rhs mean, solve the pinned system
@@ -858,44 +865,23 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
     float* G = lds + D.lds_ginv;
     const int n = D.coarse_n;
     __syncthreads();
-    // levels [0, ws) by the whole workgroup, [ws, nl) by wavefront 0 alone (sizes only shrink)
-    int ws = 0;
-    while (ws + 1 < nl && !tail_wave_level(D.lev[ws]))
-        ++ws;
-    for (int l = 0; l < ws; ++l) {
+    for (int l = 0; l + 1 < nl; ++l) {
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
-        tail_smooth<kTailThreads, kTailIter, false>(T, A, 0, 1);
+        tail_smooth(T, A, 0, 1);
         for (int k = 1; k < 4; ++k)
-            tail_smooth<kTailThreads, kTailIter, false>(T, A, 0, 0);
-        tail_residual_restrict<kTailThreads, kTailIter, false>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+            tail_smooth(T, A, 0, 0);
+        tail_residual_restrict(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
     }
-    if (threadIdx.x < 64) {
-        for (int l = ws; l + 1 < nl; ++l) {
-            const TailLevel& T = D.lev[l];
-            const TailArrays A = tail_arrays(lds, T);
-            tail_smooth<64, kWaveSmoothIter, true>(T, A, 0, 1);
-            for (int k = 1; k < 4; ++k)
-                tail_smooth<64, kWaveSmoothIter, true>(T, A, 0, 0);
-            tail_residual_restrict<64, kWaveCellIter, true>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
-        }
+    if (threadIdx.x < 64)
         tail_coarse_solve(D.lev[nl - 1], tail_arrays(lds, D.lev[nl - 1]), G, n);
-        wave_sync();
-        for (int l = nl - 2; l >= ws; --l) {
-            const TailLevel& T = D.lev[l];
-            const TailArrays A = tail_arrays(lds, T);
-            tail_prolong<64, kWaveCellIter, true>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
-            for (int k = 0; k < 4; ++k)
-                tail_smooth<64, kWaveSmoothIter, true>(T, A, 1, 0);
-        }
-    }
     __syncthreads();
-    for (int l = ws - 1; l >= 0; --l) {
+    for (int l = nl - 2; l >= 0; --l) {
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
-        tail_prolong<kTailThreads, kTailIter, false>(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
+        tail_prolong(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
         for (int k = 0; k < 4; ++k)
-            tail_smooth<kTailThreads, kTailIter, false>(T, A, 1, 0);
+            tail_smooth(T, A, 1, 0);
     }
     const TailLevel& T0 = D.lev[0];
     const TailArrays A0 = tail_arrays(lds, T0);

@@ -9,9 +9,9 @@
 //
 // This build's layout and numerics: every level stores (w+1) x (h+1) slots of float2 (u face left of
 // cell p, v face on top of cell p) and float4 coefficients (cell mu, vertex mu, u mass, v mass); reads
-// clamp to the slot field like the reference's accessors.  The solve runs on the correction
-// delta = v - v0 (FP32), with FP64 dot products, and the result is v0 + delta in FP64, so faces the
-// viscosity does not reach keep their FP64 velocities exactly.  Kernels read a device scalar block and
+// clamp to the slot field like the reference's accessors.  The iterate x starts at v0 in FP32 (the
+// reference's formulation: iterating on v - v0 instead loses three digits to cancellation when dt K
+// dominates M) and dot products are FP64.  A solve that needs no iteration leaves u, v untouched.  Kernels read a device scalar block and
 // exit at once when the solve has stopped, so a batch of four iterations is launched without host
 // round trips; the host reads the block once per batch, as the reference does.
 #include "fs_kernels.hpp"
@@ -474,8 +474,8 @@ __global__ __launch_bounds__(kVT) void k_vis_update(VLev L, float2* xs, float2* 
     r[p] = make_float2(rr.x - alpha * qq.x, rr.y - alpha * qq.y);
 }
 
-// recomputed residual t = r0 - A delta and t M^-1 t (mode 3)
-__global__ __launch_bounds__(kVT) void k_vis_truth(VLev L, const float2* r0, const float2* xs, float2* t, float dt,
+// recomputed residual t = M v0 - A x and t M^-1 t (mode 3)
+__global__ __launch_bounds__(kVT) void k_vis_truth(VLev L, const float2* v0, const float2* xs, float2* t, float dt,
     double* partials)
 {
     __shared__ double lds[kVT / 64];
@@ -484,10 +484,10 @@ __global__ __launch_bounds__(kVT) void k_vis_truth(VLev L, const float2* r0, con
     if (inside) {
         const float4 c = L.coef[p];
         const float2 q = product(L.coef, xs, L.w, L.h, x, y, dt);
-        const float2 b = r0[p];
+        const float2 s0 = v0[p];
         float2 v;
-        v.x = valid_u(L.w, L.h, x, y) ? b.x - q.x : 0.0f;
-        v.y = valid_v(L.w, L.h, x, y) ? b.y - q.y : 0.0f;
+        v.x = valid_u(L.w, L.h, x, y) ? c.z * s0.x - q.x : 0.0f;
+        v.y = valid_v(L.w, L.h, x, y) ? c.w * s0.y - q.y : 0.0f;
         t[p] = v;
         a = (double)v.x / c.z * v.x + (double)v.y / c.w * v.y;
         if (!isfinite(a)) {
@@ -572,18 +572,19 @@ __global__ void k_vis_refresh(VisScalars* S)
     }
 }
 
-// "viscosity staggered result" (ViscosityKernels.h:165-185) on the correction: u = valid ? u0 + delta : 0.
+// "viscosity staggered result" (ViscosityKernels.h:165-185): every face of u and v takes the solution
+// (walls 0).  The output is the FP32 iterate, as in the reference: writing u0 + (x - v0) instead would
+// keep the FP32 rounding noise of u0, which the stiff operator dt K amplifies in the residual.
 __global__ __launch_bounds__(kVT) void k_vis_unpack(int W, int H, const float2* xs, double* u, double* v)
 {
     const int x = blockIdx.x * kVX + (threadIdx.x & 63);
     const int y = blockIdx.y * kVY + (threadIdx.x >> 6);
     if (x <= W && y < H) {
         const int i = y * (W + 1) + x;
-        u[i] = valid_u(W, H, x, y) ? u[i] + (double)xs[i].x : 0.0;
+        u[i] = valid_u(W, H, x, y) ? (double)xs[i].x : 0.0;
     }
     if (x < W && y <= H) {
-        const int i = y * W + x;
-        v[i] = valid_v(W, H, x, y) ? v[i] + (double)xs[y * (W + 1) + x].y : 0.0;
+        v[y * W + x] = valid_v(W, H, x, y) ? (double)xs[y * (W + 1) + x].y : 0.0;
     }
 }
 
@@ -764,8 +765,8 @@ struct ViscositySolver {
             return 0;
         const size_t ns = (size_t)(W + 1) * (H + 1);
         vhip(hipMemcpyAsync(r.p, r0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
-        vhip(hipMemsetAsync(t.p, 0, ns * sizeof(float2), st), "clear");   // t doubles as the correction
-        float2* delta = t.p;
+        vhip(hipMemcpyAsync(t.p, v0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
+        float2* delta = t.p;   // the iterate x, starting from v0 (the reference's formulation)
         VBuf<float2>& truth = q;   // q is free between iterations
         int iterations = 0;
         while (maximum < 0 || iterations < maximum) {
@@ -784,7 +785,7 @@ struct ViscositySolver {
                     (const float2*)q.p, S.p);
             }
             iterations += batch;
-            hipLaunchKernelGGL(k_vis_truth, g0, dim3(kVT), 0, st, L0, (const float2*)r0.p, (const float2*)delta,
+            hipLaunchKernelGGL(k_vis_truth, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p, (const float2*)delta,
                 truth.p, fdt, partials.p);
             finish(st, nb, kVisTruth, 0.0);
             s = read(st);

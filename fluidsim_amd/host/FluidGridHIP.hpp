@@ -175,8 +175,12 @@ public:
         return g;
     }
     void setGravityY(double acceleration) { check(fs_set_gravity(grid_, acceleration)); }
-    double getSurfaceTension() const { return 0.0; }
-    void setSurfaceTension(double coefficient) { check(fs_set_surface_tension(grid_, coefficient)); }
+    double getSurfaceTension() const { return surface_tension_; }
+    void setSurfaceTension(double coefficient)
+    {
+        check(fs_set_surface_tension(grid_, coefficient));
+        surface_tension_ = coefficient;
+    }
     double capillaryStepLimit() const
     {
         double v = 0.0;
@@ -302,6 +306,7 @@ private:
     }
 
     fs_grid* grid_ = nullptr;
+    double surface_tension_ = 0.0;
     int width_;
     int height_;
     double cell_size_;

@@ -114,11 +114,14 @@ void uploadIsValidated(HipContext& ctx)
     require(threw, "mismatched upload was not rejected");
     threw = false;
     try {
-        grid.setSurfaceTension(0.07);
+        grid.setSurfaceTension(-1.0);
     } catch (const std::invalid_argument&) {
         threw = true;
     }
-    require(threw, "unsupported surface tension was not reported");
+    require(threw, "negative surface tension was not rejected");
+    grid.setSurfaceTension(0.07);
+    require(grid.getSurfaceTension() == 0.07 && std::isfinite(grid.capillaryStepLimit()),
+        "surface tension was not stored");
 }
 
 void hydrostaticPressure(HipContext& ctx)
