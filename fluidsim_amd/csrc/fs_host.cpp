@@ -97,7 +97,8 @@ struct Level {
     DevBuf<int> d_pxf, d_pxs, d_pyf, d_pys, d_rxc, d_rxi, d_ryc, d_ryi;
     DevBuf<float> d_pxw, d_pyw, d_rxw, d_ryw;
 
-    int smooth_variant = 0;     // tile of k_rb_multi on this level (autotuned)
+    int smooth_variant = 0;     // tile of k_rb_multi pre-smoothing on this level (autotuned)
+    int post_variant = 0;       // tile of k_rb_multi prolongation + post-smoothing (autotuned)
     int restrict_variant = 0;   // tile of k_res_restrict from this level (autotuned)
 
     size_t padded() const { return (size_t)s * (h + 2); }
@@ -165,6 +166,8 @@ struct fs_context {
     int device = 0;
     size_t lds_per_block = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;          // second stream (default priority): hierarchy rebuild overlap
+    hipEvent_t fork = nullptr, join = nullptr;
     PcgScalars* pinned = nullptr;   // host-pinned mirror of a grid's scalar block
     double* pinned_value = nullptr; // host-pinned landing slot of single device reductions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -200,6 +203,7 @@ struct fs_grid {
     DevBuf<double2> st_heights, st_orient, st_raw, st_curv;   // capillary geometry (first use)
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<PcgScalars> S;
+    DevBuf<PcgScalars> S_aux;   // failure flag of a hierarchy rebuilt on the second stream
     DevBuf<double> partials;
     DevBuf<double> factor;
     DevBuf<float> ginv;
@@ -350,6 +354,7 @@ struct fs_grid {
         bytes += q.alloc(NPd);
         bytes += zdiag.alloc(NPd);
         bytes += S.alloc(1);
+        bytes += S_aux.alloc(1);
         const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
         bytes += partials.alloc(std::max<size_t>(4 * blocks, 64));
         build_hierarchy();
@@ -533,20 +538,23 @@ struct fs_grid {
         mg_valid = false;
     }
 
-    // PressureMultigrid::rebuild (PressureMultigrid.cpp:119-140) on the device, FP64.
-    void rebuild_hierarchy()
+    // PressureMultigrid::rebuild (PressureMultigrid.cpp:119-140) on the device, FP64, on `stream`; a
+    // failed coarse factorisation is flagged in `flag`.
+    void rebuild_hierarchy(hipStream_t stream, PcgScalars* flag)
     {
         for (size_t l = 0; l + 1 < levels.size(); ++l) {
             Level& F = levels[l];
             Level& Cl = levels[l + 1];
-            launch_coarsen(st(), F.w, F.h, F.d_cx.p, F.d_cy.p, F.right64.p, F.down64.p, Cl.w, Cl.h, Cl.d_cx.p,
+            launch_coarsen(stream, F.w, F.h, F.d_cx.p, F.d_cy.p, F.right64.p, F.down64.p, Cl.w, Cl.h, Cl.d_cx.p,
                 Cl.d_cy.p, Cl.right64.p, Cl.down64.p);
-            launch_level_diag(st(), Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p, Cl.invd32.p);
+            launch_level_diag(stream, Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p,
+                Cl.invd32.p);
         }
         const Level& last = levels.back();
-        launch_coarse_inverse(st(), last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, S.p);
+        launch_coarse_inverse(stream, last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, flag);
         mg_valid = true;
     }
+    void rebuild_hierarchy() { rebuild_hierarchy(st(), S.p); }
 
     // One V(4,4) cycle on the residual r (minus its mean) into levels[0].sol0.
     void vcycle(const int* status)
@@ -571,7 +579,7 @@ struct fs_grid {
             CoarseLink link{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
-            launch_rb_multi(st(), L, 1, rhs, shift, 1, 0, link, status, l == 0, F.smooth_variant);
+            launch_rb_multi(st(), L, 1, rhs, shift, 1, 0, link, status, l == 0, F.post_variant);
         }
     }
 
@@ -596,12 +604,22 @@ struct fs_grid {
             Level& F = levels[l];
             const LevelView L = view(l);
             const float* rhs = l == 0 ? r.p : L.rhs;
+            const Level& Cl = levels[l + 1];
+            const CoarseLink link{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
             float best = std::numeric_limits<float>::infinity();
             for (int v = 0; v < kSmoothVariants; ++v) {
-                const float ms = time([&] { launch_rb_multi(st(), L, 0, rhs, nullptr, 0, 0, none, nullptr, l == 0, v); });
+                const float ms = time([&] { launch_rb_multi(st(), L, 0, rhs, nullptr, 0, 1, none, nullptr, l == 0, v); });
                 if (ms < best) {
                     best = ms;
                     F.smooth_variant = v;
+                }
+            }
+            best = std::numeric_limits<float>::infinity();
+            for (int v = 0; v < kSmoothVariants; ++v) {
+                const float ms = time([&] { launch_rb_multi(st(), L, 1, rhs, nullptr, 1, 0, link, nullptr, l == 0, v); });
+                if (ms < best) {
+                    best = ms;
+                    F.post_variant = v;
                 }
             }
             const AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
@@ -678,14 +696,26 @@ struct fs_grid {
         if (!coeff_valid)
             rebuild_coefficients();
         g = dev();
+        // The hierarchy depends only on the face weights: rebuild it on the second stream while the main
+        // stream forms the rhs, the predicted pressure and its residual; join before the first V-cycle.
+        const bool rebuild = precond == FS_PRECOND_MULTIGRID && !mg_valid;
+        if (rebuild) {
+            HIPC(hipEventRecord(ctx->fork, st()));
+            HIPC(hipStreamWaitEvent(ctx->aux, ctx->fork, 0));
+            launch_init_scalars(ctx->aux, S_aux.p, 0.0, 0);
+            rebuild_hierarchy(ctx->aux, S_aux.p);
+            HIPC(hipEventRecord(ctx->join, ctx->aux));
+        }
         launch_rhs(st(), g, scale);
         launch_rhs_center(st(), g);
         launch_zero_pressure(st(), g);
         launch_predict(st(), g);
         launch_sub_mean_p(st(), g, 0);
         launch_true_residual(st(), g, 0);
-        if (precond == FS_PRECOND_MULTIGRID && !mg_valid)
-            rebuild_hierarchy();
+        if (rebuild) {
+            HIPC(hipStreamWaitEvent(st(), ctx->join, 0));
+            launch_merge_failure(st(), S_aux.p, S.p);
+        }
         precondition(status_ptr());
         launch_pcg_rz(st(), g, z_ptr());
         launch_pcg_direction(st(), g, z_ptr());
@@ -782,6 +812,9 @@ fs_status fs_context_create(int device, fs_context** out)
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
             ctx->lds_per_block = (size_t)lds;
         HIPC(hipStreamCreate(&ctx->stream));
+        HIPC(hipStreamCreate(&ctx->aux));
+        HIPC(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
         HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
         HIPC(hipHostMalloc((void**)&ctx->pinned_value, sizeof(double), hipHostMallocDefault));
         HIPC(hipEventCreate(&ctx->ev0));
@@ -808,6 +841,12 @@ fs_status fs_context_destroy(fs_context* ctx)
             (void)hipHostFree(ctx->pinned_value);
         if (ctx->stream)
             (void)hipStreamDestroy(ctx->stream);
+        if (ctx->aux)
+            (void)hipStreamDestroy(ctx->aux);
+        if (ctx->fork)
+            (void)hipEventDestroy(ctx->fork);
+        if (ctx->join)
+            (void)hipEventDestroy(ctx->join);
         delete ctx;
     });
 }
@@ -1631,6 +1670,22 @@ fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, i
     });
 }
 
+fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "no such level");
+        const Level& L = grid->levels[(size_t)level];
+        const bool fused = level < grid->tail_start;
+        if (pre)
+            *pre = fused ? L.smooth_variant : -1;
+        if (post)
+            *post = fused ? L.post_variant : -1;
+        if (restriction)
+            *restriction = fused ? L.restrict_variant : -1;
+    });
+}
+
 fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n)
 {
     return guard([&] {
@@ -1744,7 +1799,7 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
                     link = CoarseLink{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
                 }
                 launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 1, 0, link, nullptr, 1,
-                    g.levels[0].smooth_variant);
+                    g.levels[0].post_variant);
                 break;
             }
             case 1: launch_pcg_ad(g.st(), gd); break;

@@ -79,6 +79,15 @@ __global__ void k_init_scalars(PcgScalars* S, double tol, int max_iterations)
     }
 }
 
+__global__ void k_merge_failure(const PcgScalars* from, PcgScalars* S)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && from->mg_failed) {
+        S->mg_failed = 1;
+        if (S->status == kRunning)
+            S->status = kFailed;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Density and face weights (FluidGridCPU.cpp:1399-1423, :1154-1205), FP64, bit-exact.
 // ------------------------------------------------------------------------------------------
@@ -949,6 +958,11 @@ inline unsigned linear_blocks(size_t n)
 void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations)
 {
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S, tol, max_iterations);
+}
+
+void launch_merge_failure(hipStream_t st, const PcgScalars* from, PcgScalars* S)
+{
+    hipLaunchKernelGGL(k_merge_failure, dim3(1), dim3(64), 0, st, from, S);
 }
 
 void launch_density(hipStream_t st, const GridDev& g)

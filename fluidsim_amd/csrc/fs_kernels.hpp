@@ -78,6 +78,8 @@ size_t tail_layout(TailDesc& t);
 bool tail_allow_lds(size_t bytes);
 
 void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations);
+// Copies a coarse-factorisation failure flagged in `from` into the solve's scalar block.
+void launch_merge_failure(hipStream_t st, const PcgScalars* from, PcgScalars* S);
 
 // ---- fields, coefficients, hierarchy (FP64) ----
 void launch_density(hipStream_t st, const GridDev& g);
@@ -122,8 +124,9 @@ void launch_precond_diag(hipStream_t st, const GridDev& g);
 // ---- multigrid V-cycle pieces (FP32) ----
 // Four red-black sweeps in one launch (reads sol[cur], writes sol[cur ^ 1]); with link.csol set the
 // coarse correction is prolonged and added first.  fine selects the finest-level symbol.
-// variant: interior tile 0 = 64 x 32, 1 = 32 x 32, 2 = 32 x 16, 3 = 16 x 16 (same results, different speed).
-constexpr int kSmoothVariants = 4;
+// variant: interior tile and threads 0 = 64 x 32 / 256, 1 = 32 x 32 / 256, 2 = 32 x 16 / 256,
+// 3 = 16 x 16 / 256, 4 = 64 x 32 / 512, 5 = 32 x 32 / 512 (same results, different speed).
+constexpr int kSmoothVariants = 6;
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
     int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant);
 // Residual of sol[cur] and its restriction into C.rhs in one launch; variant: coarse tile 0 = 32 x 16,

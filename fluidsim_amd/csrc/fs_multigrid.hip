@@ -33,8 +33,8 @@ __device__ __forceinline__ float inv_or_zero(float d) { return d > 0.f ? __built
 // registers for all eight half-sweeps, so an update reads four neighbours from LDS and writes one value.
 // MODE 0: pre-smoothing from a zero guess (no solution read); 1: post-smoothing, the prolonged coarse
 // correction added to the solution first; 2: plain sweeps of the stored solution.
-template <int TX, int TY, int NS, int FINE, int MODE>
-__global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift,
+template <int TX, int TY, int NS, int FINE, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift,
     int c0, CoarseLink link, const int* status)
 {
     constexpr bool from_zero = MODE == 0;
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
     constexpr int HX = LX / 2;                       // colour-array row width
     constexpr int NE = LX * LY;
     constexpr int SLOTS = HX * (LY - 2);             // rows 1 .. LY-2, every pair
-    constexpr int ROUNDS = (SLOTS + kNT - 1) / kNT;
+    constexpr int ROUNDS = (SLOTS + NT - 1) / NT;
     static_assert(LX % 2 == 0 && (TX % 2) == 0 && (HALO % 2) == 0, "tile must keep colour parity");
     __shared__ float s_col[2][LY * HX];
     const int w = L.w, h = L.h, s = L.s;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
     bool cin[2][ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
-        const int t = threadIdx.x + r * kNT;
+        const int t = threadIdx.x + r * NT;
         const int ly = 1 + t / HX, jj = t - (t / HX) * HX;
         cly[r] = t < SLOTS ? ly : -1;
         cj[r] = jj;
@@ -73,13 +73,13 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
             cb[c][r] = rhs[P];
         }
     }
-    constexpr int LOADS = (NE + kNT - 1) / kNT;
+    constexpr int LOADS = (NE + NT - 1) / NT;
     float vs[LOADS];
     bool vin[LOADS];
     int ix[LOADS], iy[LOADS];
 #pragma unroll
     for (int it = 0; it < LOADS; ++it) {
-        const int e = threadIdx.x + it * kNT;
+        const int e = threadIdx.x + it * NT;
         const int ly = e / LX, lx = e - ly * LX;
         const int x = x0 + lx, y = y0 + ly;
         vin[it] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
@@ -88,26 +88,34 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
         vs[it] = MODE == 0 ? 0.f : sol_in[(iy[it] + 1) * s + (ix[it] + 1)];
     }
     if (MODE == 1) {
-        // two batched phases: transfer tables, then the four coarse values of every cell
-        int a0[LOADS], a1[LOADS], b0[LOADS], b1[LOADS];
-        float wx[LOADS], wy[LOADS];
+        // per chunk of PC cells, two batched phases: transfer tables, then the four coarse values (chunks
+        // bound the registers this prologue adds to the kernel)
+        constexpr int PC = 4;
 #pragma unroll
-        for (int it = 0; it < LOADS; ++it) {
-            wx[it] = link.pxw[ix[it]];
-            wy[it] = link.pyw[iy[it]];
-            a0[it] = link.pxf[ix[it]];
-            a1[it] = link.pxs[ix[it]];
-            b0[it] = link.pyf[iy[it]];
-            b1[it] = link.pys[iy[it]];
-        }
+        for (int c0 = 0; c0 < LOADS; c0 += PC) {
+            int a0[PC], a1[PC], b0[PC], b1[PC];
+            float wx[PC], wy[PC];
 #pragma unroll
-        for (int it = 0; it < LOADS; ++it) {
-            const float* top = link.csol + (b0[it] + 1) * link.cs + 1;
-            const float* bot = link.csol + (b1[it] + 1) * link.cs + 1;
-            const float t0 = top[a0[it]], t1 = top[a1[it]], u0 = bot[a0[it]], u1 = bot[a1[it]];
-            const float a = (1.f - wx[it]) * t0 + wx[it] * t1;
-            const float b = (1.f - wx[it]) * u0 + wx[it] * u1;
-            vs[it] += (1.f - wy[it]) * a + wy[it] * b;
+            for (int k = 0; k < PC; ++k) {
+                const int it = min(c0 + k, LOADS - 1);
+                wx[k] = link.pxw[ix[it]];
+                wy[k] = link.pyw[iy[it]];
+                a0[k] = link.pxf[ix[it]];
+                a1[k] = link.pxs[ix[it]];
+                b0[k] = link.pyf[iy[it]];
+                b1[k] = link.pys[iy[it]];
+            }
+#pragma unroll
+            for (int k = 0; k < PC; ++k) {
+                if (c0 + k >= LOADS)
+                    break;
+                const float* top = link.csol + (b0[k] + 1) * link.cs + 1;
+                const float* bot = link.csol + (b1[k] + 1) * link.cs + 1;
+                const float t0 = top[a0[k]], t1 = top[a1[k]], u0 = bot[a0[k]], u1 = bot[a1[k]];
+                const float a = (1.f - wx[k]) * t0 + wx[k] * t1;
+                const float b = (1.f - wx[k]) * u0 + wx[k] * u1;
+                vs[c0 + k] += (1.f - wy[k]) * a + wy[k] * b;
+            }
         }
     }
     if (status && *status != kRunning)
@@ -126,7 +134,7 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
     }
 #pragma unroll
     for (int it = 0; it < LOADS; ++it) {
-        const int e = threadIdx.x + it * kNT;
+        const int e = threadIdx.x + it * NT;
         if (e < NE) {
             const int ly = e / LX, lx = e - ly * LX;
             s_col[(lx + ly) & 1][ly * HX + (lx >> 1)] = vin[it] ? vs[it] : 0.f;
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(kNT) void k_rb_multi(LevelView L, int cur, const fl
         __syncthreads();
     }
     float* sol_out = L.sol[cur ^ 1];
-    for (int e = threadIdx.x; e < TX * TY; e += kNT) {
+    for (int e = threadIdx.x; e < TX * TY; e += NT) {
         const int ly = HALO + e / TX, lx = HALO + e % TX;
         const int x = x0 + lx, y = y0 + ly;
         if (x < w && y < h)
@@ -932,27 +940,34 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
     int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant)
 {
     const int mode = from_zero ? 0 : (link.csol ? 1 : 2);
-#define FS_RB(TX, TY, F, M)                                                                                   \
-    hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, F, M>), dim3((L.w + TX - 1) / TX, (L.h + TY - 1) / TY),   \
-        dim3(kNT), 0, st, L, cur, rhs, shift, first_color, link, status)
-#define FS_RB_MODES(TX, TY, F)          \
+#define FS_RB(TX, TY, F, M, NT)                                                                               \
+    hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, F, M, NT>), dim3((L.w + TX - 1) / TX, (L.h + TY - 1) / TY), \
+        dim3(NT), 0, st, L, cur, rhs, shift, first_color, link, status)
+#define FS_RB_MODES(TX, TY, F, NT)      \
     do {                                \
         if (mode == 0)                  \
-            FS_RB(TX, TY, F, 0);        \
+            FS_RB(TX, TY, F, 0, NT);    \
         else if (mode == 1)             \
-            FS_RB(TX, TY, F, 1);        \
+            FS_RB(TX, TY, F, 1, NT);    \
         else                            \
-            FS_RB(TX, TY, F, 2);        \
+            FS_RB(TX, TY, F, 2, NT);    \
     } while (0)
     switch (variant) {
-    case 1: FS_RB_MODES(32, 32, 0); break;
-    case 2: FS_RB_MODES(32, 16, 0); break;
-    case 3: FS_RB_MODES(16, 16, 0); break;
+    case 1: FS_RB_MODES(32, 32, 0, 256); break;
+    case 2: FS_RB_MODES(32, 16, 0, 256); break;
+    case 3: FS_RB_MODES(16, 16, 0, 256); break;
+    case 4:
+        if (fine)
+            FS_RB_MODES(64, 32, 1, 512);
+        else
+            FS_RB_MODES(64, 32, 0, 512);
+        break;
+    case 5: FS_RB_MODES(32, 32, 0, 512); break;
     default:
         if (fine)
-            FS_RB_MODES(64, 32, 1);
+            FS_RB_MODES(64, 32, 1, 256);
         else
-            FS_RB_MODES(64, 32, 0);
+            FS_RB_MODES(64, 32, 0, 256);
         break;
     }
 #undef FS_RB_MODES

@@ -235,6 +235,12 @@ class FluidGridHIP:
         mixture = self.liquid_mixture if ph == 0 else self.gas_mixture
         mixture.components[index].dynamic_viscosity = float(coefficient)
 
+    def multigridTiles(self, level: int):
+        """(pre, post, restriction) autotuned tile variants of a level; -1 inside the tail kernel."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("fs_mg_level_tiles", self._h, int(level), C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
     def applySurfaceTension(self, dt: float):
         """FluidGridCPU::applySurfaceTension (FluidGridCPU.cpp:862-926)."""
         call("fs_apply_surface_tension", self._h, float(dt))

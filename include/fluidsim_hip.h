@@ -196,6 +196,11 @@ fs_status fs_mg_level_weights(fs_grid* grid, int level, double* right, double* d
 fs_status fs_mg_level_prolong(fs_grid* grid, int level, int axis, int* first, int* second, double* weight);
 /* Restriction stencils of the next coarser level: count[n], index[4n], weight[4n]. */
 fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, int* index, double* weight);
+/* Autotuned kernel tiles of a level (build's own): pre- and post-smoothing variants of the fused
+ * four-sweep kernel (0 = 64x32/256 threads, 1 = 32x32/256, 2 = 32x16/256, 3 = 16x16/256, 4 = 64x32/512,
+ * 5 = 32x32/512) and the residual-restriction variant (0 = 32x16, 1 = 16x16, 2 = 16x8, 3 = 8x8 coarse
+ * cells); -1 for levels solved inside the single-workgroup tail. */
+fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction);
 /* Coarsest dense Cholesky factor, n*n row-major (lower triangle); *n receives the unknown count. */
 fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n);
 /* One V-cycle: correction = M^-1 (residual - mean), both W*H FP64 host arrays (converted to FP32 on
