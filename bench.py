@@ -138,6 +138,9 @@ def main():
     iters = [r.iterations for r in results]
 
     # Dominant kernel: the finest-level post-smoothing launch, timed with HIP events on the grid's stream.
+    variants = {0: (64, 32, 256), 1: (32, 32, 256), 2: (32, 16, 256), 3: (16, 16, 256), 4: (64, 32, 512),
+                5: (32, 32, 512)}
+    tx, ty, nt = variants[grid.multigridTiles(0)[1]]
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
     sweep_bytes = SMOOTH_BYTES_PER_CELL * wl.width * wl.height
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
@@ -170,7 +173,8 @@ def main():
         "converged_all": bool(ok),
         "max_rel_residual": float(max(rel)),
         "roofline": {
-            "kernel": "k_rb_multi<64,32,4,1,1> (finest-level prolongation + 4 red-black Gauss-Seidel sweeps)",
+            "kernel": f"k_rb_multi<{tx},{ty},4,{1 if (tx, ty) == (64, 32) else 0},1,{nt}> (finest-level "
+                      "prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -596,51 +596,82 @@ constexpr int kTailIter = 4;   // at most 4 * 1024 items per phase (the tail sel
 
 __device__ __forceinline__ bool wave_has_items(int items) { return (int)(threadIdx.x & ~63u) < items; }
 
-template <int ITER> __device__ void tail_smooth_n(const TailLevel& T, const TailArrays& A, int color, bool neighbours)
+// t / d for 0 <= t < 2^22 without an integer division: (t + 1/2) / d lies at least 1/(2d) from an
+// integer, and the float product's error is below (t / d) 2^-22 < 1/(2d).
+__device__ __forceinline__ int tail_div(int t, float inv_d) { return (int)(((float)t + 0.5f) * inv_d); }
+
+// Four red-black sweeps of one tail level with each thread's cells, conductances, rhs and 1/diagonal
+// held in registers across all eight half-sweeps: a half-sweep is four LDS reads, one write and a
+// barrier per item.
+template <int ITER>
+__device__ void tail_smooth4_n(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
 {
     const int w = T.w, h = T.h, s = T.s;
     const int pairs = (w + 1) >> 1;
     const int items = pairs * h;
-    if (!wave_has_items(items))
-        return;
-    float v[ITER];
-    int idx[ITER];
-    bool on[ITER];
+    const bool active = wave_has_items(items);
+    const float inv_pairs = 1.0f / (float)pairs;
+    int idx[2][ITER];
+    bool on[2][ITER];
+    float wl[2][ITER], wr[2][ITER], wu[2][ITER], wd[2][ITER], rb[2][ITER], inv[2][ITER];
 #pragma unroll
-    for (int it = 0; it < ITER; ++it) {
-        const int t = min((int)threadIdx.x + it * kTailThreads, items - 1);
-        const int y = t / pairs;
-        const int x = 2 * (t - y * pairs) + ((color ^ y) & 1);
-        on[it] = (int)threadIdx.x + it * kTailThreads < items && x < w;
-        const int i = (y + 1) * s + min(x, w - 1) + 1;
-        idx[it] = i;
-        const float rl = A.right[i - 1], rr = A.right[i], du = A.down[i - s], dd = A.down[i];
-        float acc = A.rhs[i];
-        const float n = (rl * A.sol[i - 1] + rr * A.sol[i + 1]) + (du * A.sol[i - s] + dd * A.sol[i + s]);
-        if (neighbours)
-            acc += n;
-        v[it] = acc * inv_or_zero((rl + rr) + (du + dd));
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int it = 0; it < ITER; ++it) {
+            const int t = min((int)threadIdx.x + it * kTailThreads, items - 1);
+            const int y = tail_div(t, inv_pairs);
+            const int x = 2 * (t - y * pairs) + ((c ^ y) & 1);
+            on[c][it] = active && (int)threadIdx.x + it * kTailThreads < items && x < w;
+            const int i = (y + 1) * s + min(x, w - 1) + 1;
+            idx[c][it] = i;
+            wl[c][it] = A.right[i - 1];
+            wr[c][it] = A.right[i];
+            wu[c][it] = A.down[i - s];
+            wd[c][it] = A.down[i];
+            rb[c][it] = A.rhs[i];
+        }
     }
 #pragma unroll
-    for (int it = 0; it < ITER; ++it)
-        if (on[it])
-            A.sol[idx[it]] = v[it];
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int it = 0; it < ITER; ++it)
+            inv[c][it] = inv_or_zero((wl[c][it] + wr[c][it]) + (wu[c][it] + wd[c][it]));
+    for (int k = 0; k < 4; ++k) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const int color = reverse ? 1 - pass : pass;
+            const bool neighbours = !(from_zero && k == 0 && pass == 0);
+            if (active) {
+                float v[ITER];
+#pragma unroll
+                for (int it = 0; it < ITER; ++it) {
+                    const int i = idx[color][it];
+                    float acc = rb[color][it];
+                    const float n = (wl[color][it] * A.sol[i - 1] + wr[color][it] * A.sol[i + 1]) +
+                                    (wu[color][it] * A.sol[i - s] + wd[color][it] * A.sol[i + s]);
+                    if (neighbours)
+                        acc += n;
+                    v[it] = acc * inv[color][it];
+                }
+#pragma unroll
+                for (int it = 0; it < ITER; ++it)
+                    if (on[color][it])
+                        A.sol[idx[color][it]] = v[it];
+            }
+            __syncthreads();
+        }
+    }
 }
 
-__device__ void tail_smooth(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
+// Four sweeps (PressureMultigrid.cpp:204-231 applied four times); reverse = post-smoothing colour order.
+__device__ void tail_smooth4(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
 {
     const int items = ((T.w + 1) >> 1) * T.h;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int color = reverse ? 1 - pass : pass;
-        const bool neighbours = !(from_zero && pass == 0);
-        if (items <= kTailThreads)
-            tail_smooth_n<1>(T, A, color, neighbours);
-        else if (items <= 2 * kTailThreads)
-            tail_smooth_n<2>(T, A, color, neighbours);
-        else
-            tail_smooth_n<kTailIter>(T, A, color, neighbours);
-        __syncthreads();
-    }
+    if (items <= kTailThreads)
+        tail_smooth4_n<1>(T, A, reverse, from_zero);
+    else if (items <= 2 * kTailThreads)
+        tail_smooth4_n<2>(T, A, reverse, from_zero);
+    else
+        tail_smooth4_n<kTailIter>(T, A, reverse, from_zero);
 }
 
 template <int ITER> __device__ void tail_residual_n(const TailLevel& T, const TailArrays& F)
@@ -649,6 +680,7 @@ template <int ITER> __device__ void tail_residual_n(const TailLevel& T, const Ta
     const int cells = w * h;
     if (!wave_has_items(cells))
         return;
+    const float inv_w = 1.0f / (float)w;
     float rv[ITER];
     int idx[ITER];
     bool on[ITER];
@@ -656,7 +688,7 @@ template <int ITER> __device__ void tail_residual_n(const TailLevel& T, const Ta
     for (int it = 0; it < ITER; ++it) {
         const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
         on[it] = (int)threadIdx.x + it * kTailThreads < cells;
-        const int y = e / w, x = e - y * w;
+        const int y = tail_div(e, inv_w), x = e - y * w;
         const int i = (y + 1) * s + x + 1;
         idx[it] = i;
         const float c = F.sol[i];
@@ -678,12 +710,13 @@ template <int ITER> __device__ void tail_restrict_n(const TailLevel& T, const Ta
     const int cells_c = Cn.w * Cn.h;
     if (!wave_has_items(cells_c))
         return;
+    const float inv_cw = 1.0f / (float)Cn.w;
     float val[ITER];
     int dst[ITER];
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
         const int e = min((int)threadIdx.x + it * kTailThreads, cells_c - 1);
-        const int y = e / Cn.w, x = e - y * Cn.w;
+        const int y = tail_div(e, inv_cw), x = e - y * Cn.w;
         float value = 0.f;
 #pragma unroll
         for (int iy = 0; iy < 4; ++iy) {
@@ -728,6 +761,7 @@ template <int ITER> __device__ void tail_prolong_n(const TailLevel& T, const Tai
     const int cells = w * h;
     if (!wave_has_items(cells))
         return;
+    const float inv_w = 1.0f / (float)w;
     float add[ITER], cur[ITER];
     int idx[ITER];
     bool on[ITER];
@@ -735,7 +769,7 @@ template <int ITER> __device__ void tail_prolong_n(const TailLevel& T, const Tai
     for (int it = 0; it < ITER; ++it) {
         const int e = min((int)threadIdx.x + it * kTailThreads, cells - 1);
         on[it] = (int)threadIdx.x + it * kTailThreads < cells;
-        const int y = e / w, x = e - y * w;
+        const int y = tail_div(e, inv_w), x = e - y * w;
         const float wy = F.pyw[y];
         const float* top = Cd.sol + (F.pyf[y] + 1) * Cn.s + 1;
         const float* bottom = Cd.sol + (F.pys[y] + 1) * Cn.s + 1;
@@ -876,9 +910,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
     for (int l = 0; l + 1 < nl; ++l) {
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
-        tail_smooth(T, A, 0, 1);
-        for (int k = 1; k < 4; ++k)
-            tail_smooth(T, A, 0, 0);
+        tail_smooth4(T, A, 0, 1);
         tail_residual_restrict(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
     }
     if (threadIdx.x < 64)
@@ -888,8 +920,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
         tail_prolong(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
-        for (int k = 0; k < 4; ++k)
-            tail_smooth(T, A, 1, 0);
+        tail_smooth4(T, A, 1, 0);
     }
     const TailLevel& T0 = D.lev[0];
     const TailArrays A0 = tail_arrays(lds, T0);
