@@ -24,6 +24,22 @@ constexpr int kNT = 256;
 
 __device__ __forceinline__ float inv_or_zero(float d) { return d > 0.f ? __builtin_amdgcn_rcpf(d) : 0.f; }
 
+// XCD-aware tile order.  Workgroups of a launch are dealt to the 8 XCDs round-robin (workgroup b runs
+// on XCD b % 8), each XCD with its own L2.  Mapping the workgroups an XCD receives onto a contiguous
+// run of row-major tiles keeps neighbouring tiles, whose 8-cell halos overlap, on the same L2.  The
+// launch is one-dimensional with exactly gx * gy workgroups; the map is a bijection onto the tiles.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_tile(int gx, int gy, int& bx, int& by)
+{
+    const int n = gx * gy;
+    const int b = blockIdx.x;
+    const int per = n / kXcds, rem = n % kXcds;
+    const int xcd = b % kXcds, q = b / kXcds;
+    const int t = xcd * per + min(xcd, rem) + q;
+    by = t / gx;
+    bx = t - by * gx;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Four-sweep red-black Gauss-Seidel with optional fused prolongation.
 // ---------------------------------------------------------------------------------------------
@@ -47,7 +63,9 @@ __global__ __launch_bounds__(NT) void k_rb_multi(LevelView L, int cur, const flo
     static_assert(LX % 2 == 0 && (TX % 2) == 0 && (HALO % 2) == 0, "tile must keep colour parity");
     __shared__ float s_col[2][LY * HX];
     const int w = L.w, h = L.h, s = L.s;
-    const int x0 = blockIdx.x * TX - HALO, y0 = blockIdx.y * TY - HALO;  // x0 + y0 is even
+    int bx, by;
+    xcd_tile((w + TX - 1) / TX, (h + TY - 1) / TY, bx, by);
+    const int x0 = bx * TX - HALO, y0 = by * TY - HALO;  // x0 + y0 is even
     const float* sol_in = L.sol[cur];
     // Every global load below goes through a clamped (valid) interior index and is masked afterwards,
     // so a thread issues all its loads before its first wait.
@@ -208,7 +226,9 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, cons
     __shared__ int s_rxc[CTX], s_rxi[CTX * 4], s_ryc[CTY], s_ryi[CTY * 4];
     __shared__ float s_rxw[CTX * 4], s_ryw[CTY * 4];
     const int w = F.w, h = F.h, s = F.s;
-    const int cx0 = blockIdx.x * CTX, cy0 = blockIdx.y * CTY;
+    int bx, by;
+    xcd_tile((Cl.w + CTX - 1) / CTX, (Cl.h + CTY - 1) / CTY, bx, by);
+    const int cx0 = bx * CTX, cy0 = by * CTY;
     const int fx0 = 2 * cx0 - 3, fy0 = 2 * cy0 - 3;
     const float* sol = F.sol[cur];
     constexpr int NE = LX * LY;
@@ -972,7 +992,7 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
 {
     const int mode = from_zero ? 0 : (link.csol ? 1 : 2);
 #define FS_RB(TX, TY, F, M, NT)                                                                               \
-    hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, F, M, NT>), dim3((L.w + TX - 1) / TX, (L.h + TY - 1) / TY), \
+    hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, F, M, NT>), dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)), \
         dim3(NT), 0, st, L, cur, rhs, shift, first_color, link, status)
 #define FS_RB_MODES(TX, TY, F, NT)      \
     do {                                \
@@ -1009,8 +1029,8 @@ void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const floa
     const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status, int variant)
 {
 #define FS_RR(CX, CY)                                                                                         \
-    hipLaunchKernelGGL((k_res_restrict<CX, CY>), dim3((C.w + CX - 1) / CX, (C.h + CY - 1) / CY), dim3(kNT), 0, st, \
-        F, cur, rhs, shift, C, rx, ry, status)
+    hipLaunchKernelGGL((k_res_restrict<CX, CY>), dim3(((C.w + CX - 1) / CX) * ((C.h + CY - 1) / CY)), dim3(kNT), 0, \
+        st, F, cur, rhs, shift, C, rx, ry, status)
     switch (variant) {
     case 1: FS_RR(16, 16); break;
     case 2: FS_RR(16, 8); break;
