@@ -56,6 +56,33 @@ class FsStepTimings(C.Structure):
     ]
 
 
+class FsFrameSettings(C.Structure):
+    _fields_ = [
+        ("frame_time", C.c_double),
+        ("steps_per_second", C.c_int),
+        ("projection_tolerance", C.c_double),
+        ("collect_timings", C.c_int),
+    ]
+
+
+class FsFrameStatistics(C.Structure):
+    _fields_ = [
+        ("fluid_ms", C.c_double),
+        ("stages", FsStepTimings),
+        ("substeps", C.c_int),
+        ("transport_steps", C.c_int),
+        ("pressure_iterations", C.c_int),
+        ("viscosity_iterations", C.c_int),
+        ("projection_tolerance", C.c_double),
+        ("max_volume_error", C.c_double),
+        ("rms_volume_error", C.c_double),
+        ("steps_scheduled", C.c_int),
+        ("steps_run", C.c_int),
+        ("accepted", C.c_int),
+        ("max_divergence", C.c_double),
+    ]
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -108,6 +135,15 @@ SIGNATURES = {
     "fs_face_weights": [_vp, _dp, _dp],
     "fs_time_kernel": [_vp, C.c_int, C.c_int, _dp],
     "fs_grid_device_bytes": [_vp, C.POINTER(C.c_size_t)],
+    "fs_get_surface_tension": [_vp, _dp],
+    "fs_get_preconditioner": [_vp, _ip],
+    "fs_get_transport": [_vp, _ip],
+    "fs_component_counts": [_vp, _ip, _ip],
+    "fs_get_component": [_vp, C.c_int, C.c_int, _dp, _dp],
+    "fs_checkpoint_size": [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "fs_checkpoint_save": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
+    "fs_checkpoint_load": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
+    "fs_advance_frame": [_vp, C.POINTER(FsFrameSettings), C.POINTER(FsFrameStatistics)],
 }
 
 _lib = None

@@ -5,6 +5,7 @@
 // kernels on the context's stream; the PCG iteration body is captured once into a hipGraph and
 // replayed.  All loop decisions (alpha, beta, convergence, restarts, iteration budget) live in a
 // device scalar block, so the host only polls a status word between batches of iterations.
+#include "fs_json.hpp"
 #include "fs_kernels.hpp"
 #include "../../include/fluidsim_hip.h"
 
@@ -172,6 +173,8 @@ struct fs_context {
     double* pinned_value = nullptr; // host-pinned landing slot of single device reductions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+constexpr int kPressureBudget = 100000;   // PCG iterations of an unbounded (negative) budget
 
 struct fs_grid {
     fs_context* ctx = nullptr;
@@ -687,8 +690,12 @@ struct fs_grid {
         res.physical_steps = 1;
         if (!uploaded)
             fail(FS_E_LOGIC, "Upload the grid state before stepping or projecting");
-        if (dt <= 0.0 || max_iterations < 0)
+        if (dt <= 0.0)
             return res;
+        // A negative budget runs to tolerance, the uniform WebGPU contract used by its scenes
+        // (FluidGridGPU.cpp:1326, FluidScene.inl:355-359), bounded so that a stagnating solve ends.
+        if (max_iterations < 0)
+            max_iterations = kPressureBudget;
         const double scale = h * h / dt;
         GridDev g = dev();
         ensure_graph();
@@ -789,6 +796,204 @@ void need(bool cond, fs_status code, const char* message)
 }
 
 void set_device(fs_context* ctx) { HIPC(hipSetDevice(ctx->device)); }
+
+// ---- resident-instance checkpoint (FluidGridGPU.cpp:566-695 save, 697-880 load) ----
+//
+// Metadata: one JSON object naming the format, the grid geometry, every physical and solver setting
+// and the solver history the next step depends on.  Payload: FP64 values in native byte order, in
+// the fixed order of kCheckpointFields.  Derived data (density, face weights, the multigrid
+// hierarchy, solver workspaces) is rebuilt by the next operation, as in the reference.
+
+constexpr const char* kCheckpointBackend = "hip-uniform";
+constexpr int kCheckpointVersion = 1;
+const char* const kCheckpointFields[] = {"liquid_fraction", "u", "v", "pressure", "previous_pressure",
+    "liquid_mass", "gas_mass"};
+constexpr int kCheckpointFieldCount = 7;
+
+const char* native_byte_order()
+{
+    const unsigned short probe = 1;
+    unsigned char first = 0;
+    std::memcpy(&first, &probe, 1);
+    return first == 1 ? "little" : "big";
+}
+
+size_t checkpoint_values(const fs_grid& g, size_t n_liquid, size_t n_gas)
+{
+    return (3 + n_liquid + n_gas) * g.N() + g.NU() + g.NV();
+}
+
+std::string checkpoint_metadata(const fs_grid& g)
+{
+    using namespace fsjson;
+    std::string out = "{";
+    bool first = true;
+    put_key(out, "backend", first);
+    put_string(out, kCheckpointBackend);
+    put_key(out, "version", first);
+    put_int(out, kCheckpointVersion);
+    put_key(out, "scalar", first);
+    put_string(out, "float64");
+    put_key(out, "byte_order", first);
+    put_string(out, native_byte_order());
+    put_key(out, "width", first);
+    put_int(out, g.W);
+    put_key(out, "height", first);
+    put_int(out, g.H);
+    put_key(out, "cell_size", first);
+    put_number(out, g.h);
+    put_key(out, "origin", first);
+    out += '[';
+    put_number(out, g.ox);
+    out += ',';
+    put_number(out, g.oy);
+    out += ']';
+    put_key(out, "gravity_y", first);
+    put_number(out, g.gravity_y);
+    put_key(out, "surface_tension", first);
+    put_number(out, g.sigma);
+    put_key(out, "preconditioner", first);
+    put_int(out, (int)g.precond);
+    put_key(out, "transport", first);
+    put_int(out, (int)g.transport);
+    auto mixture = [&](const char* key, const std::vector<double>& rho, const std::vector<double>& mu) {
+        put_key(out, key, first);
+        out += '[';
+        for (size_t k = 0; k < rho.size(); ++k) {
+            if (k)
+                out += ',';
+            bool inner = true;
+            out += '{';
+            put_key(out, "density", inner);
+            put_number(out, rho[k]);
+            // zero viscosity is omitted, as the reference does (FluidGridGPU.cpp:597-599)
+            if (mu[k] != 0.0) {
+                put_key(out, "dynamic_viscosity", inner);
+                put_number(out, mu[k]);
+            }
+            out += '}';
+        }
+        out += ']';
+    };
+    mixture("liquid_mixture", g.rho_l, g.mu_l);
+    mixture("gas_mixture", g.rho_g, g.mu_g);
+    put_key(out, "previous_solve_iterations", first);
+    put_int(out, g.predicted);
+    put_key(out, "payload", first);
+    out += '[';
+    for (int i = 0; i < kCheckpointFieldCount; ++i) {
+        if (i)
+            out += ',';
+        put_string(out, kCheckpointFields[i]);
+    }
+    out += "]}";
+    return out;
+}
+
+[[noreturn]] void bad_checkpoint(const std::string& what) { fail(FS_E_RUNTIME, "Checkpoint: " + what); }
+
+const fsjson::Value& member(const fsjson::Value& obj, const char* key)
+{
+    const fsjson::Value* v = obj.find(key);
+    if (!v)
+        bad_checkpoint(std::string("missing \"") + key + "\"");
+    return *v;
+}
+
+double number_of(const fsjson::Value& v, const char* what)
+{
+    if (v.kind != fsjson::Value::Number || !std::isfinite(v.number))
+        bad_checkpoint(std::string(what) + " must be a finite number");
+    return v.number;
+}
+
+long long integer_of(const fsjson::Value& v, const char* what, long long lo, long long hi)
+{
+    const double x = number_of(v, what);
+    if (x != std::floor(x) || x < (double)lo || x > (double)hi)
+        bad_checkpoint(std::string(what) + " must be an integer in range");
+    return (long long)x;
+}
+
+const std::string& string_of(const fsjson::Value& v, const char* what)
+{
+    if (v.kind != fsjson::Value::String)
+        bad_checkpoint(std::string(what) + " must be a string");
+    return v.text;
+}
+
+// Everything a load restores, fully validated before any live state changes.
+struct CheckpointConfig {
+    double ox = 0, oy = 0, gravity = 0, sigma = 0;
+    fs_preconditioner precond = FS_PRECOND_MULTIGRID;
+    fs_transport transport = FS_TRANSPORT_REFERENCE;
+    std::vector<double> rho_l, rho_g, mu_l, mu_g;
+    int predicted = 1;
+};
+
+CheckpointConfig parse_checkpoint(const fs_grid& g, const char* text, size_t length)
+{
+    const fsjson::Value meta = fsjson::parse(text, length);
+    if (meta.kind != fsjson::Value::Object)
+        bad_checkpoint("metadata must be a JSON object");
+    const long long version = integer_of(member(meta, "version"), "version", -(1LL << 40), 1LL << 40);
+    if (string_of(member(meta, "backend"), "backend") != kCheckpointBackend || version != kCheckpointVersion ||
+        string_of(member(meta, "scalar"), "scalar") != "float64" ||
+        string_of(member(meta, "byte_order"), "byte_order") != native_byte_order())
+        bad_checkpoint("incompatible uniform HIP checkpoint format");
+    const fsjson::Value& fields = member(meta, "payload");
+    bool same_fields = fields.kind == fsjson::Value::Array && fields.items.size() == kCheckpointFieldCount;
+    for (int i = 0; same_fields && i < kCheckpointFieldCount; ++i)
+        same_fields = fields.items[i].kind == fsjson::Value::String && fields.items[i].text == kCheckpointFields[i];
+    if (!same_fields)
+        bad_checkpoint("incompatible payload layout");
+    if (integer_of(member(meta, "width"), "width", 1, 1 << 30) != g.W ||
+        integer_of(member(meta, "height"), "height", 1, 1 << 30) != g.H ||
+        number_of(member(meta, "cell_size"), "cell_size") != g.h)
+        bad_checkpoint("geometry does not match the fluid grid");
+
+    CheckpointConfig c;
+    const fsjson::Value& origin = member(meta, "origin");
+    if (origin.kind != fsjson::Value::Array || origin.items.size() != 2)
+        bad_checkpoint("origin must be a pair of numbers");
+    c.ox = number_of(origin.items[0], "origin");
+    c.oy = number_of(origin.items[1], "origin");
+    c.gravity = number_of(member(meta, "gravity_y"), "gravity_y");
+    c.sigma = number_of(member(meta, "surface_tension"), "surface_tension");
+    if (c.sigma < 0.0)
+        bad_checkpoint("invalid saved surface tension");
+    const long long pc = integer_of(member(meta, "preconditioner"), "preconditioner", -1000, 1000);
+    if (pc != FS_PRECOND_DIAGONAL && pc != FS_PRECOND_MULTIGRID)
+        bad_checkpoint("unknown pressure preconditioner");
+    c.precond = (fs_preconditioner)pc;
+    const long long tr = integer_of(member(meta, "transport"), "transport", -1000, 1000);
+    if (tr != FS_TRANSPORT_REFERENCE && tr != FS_TRANSPORT_NONE && tr != FS_TRANSPORT_SEMI_LAGRANGIAN)
+        bad_checkpoint("unknown transport mode");
+    c.transport = (fs_transport)tr;
+    auto mixture = [&](const char* key, std::vector<double>& rho, std::vector<double>& mu) {
+        const fsjson::Value& list = member(meta, key);
+        if (list.kind != fsjson::Value::Array || list.items.size() > (size_t)kMaxComponents)
+            bad_checkpoint(std::string(key) + " must be a list of at most 64 components");
+        for (const fsjson::Value& e : list.items) {
+            if (e.kind != fsjson::Value::Object)
+                bad_checkpoint(std::string(key) + " entries must be objects");
+            const double d = number_of(member(e, "density"), "density");
+            if (!(d > 0.0))
+                bad_checkpoint("component density must be positive");
+            const fsjson::Value* m = e.find("dynamic_viscosity");
+            const double v = m ? number_of(*m, "dynamic_viscosity") : 0.0;
+            if (v < 0.0)
+                bad_checkpoint("invalid saved material viscosity");
+            rho.push_back(d);
+            mu.push_back(v);
+        }
+    };
+    mixture("liquid_mixture", c.rho_l, c.mu_l);
+    mixture("gas_mixture", c.rho_g, c.mu_g);
+    c.predicted = (int)std::max<long long>(
+        1, integer_of(member(meta, "previous_solve_iterations"), "previous_solve_iterations", 0, 1 << 30));
+    return c;
+}
 
 }  // namespace
 
@@ -1850,6 +2055,211 @@ fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes)
         need(grid != nullptr && bytes != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
         *bytes = grid->bytes;
     });
+}
+
+fs_status fs_get_surface_tension(const fs_grid* grid, double* coefficient)
+{
+    return guard([&] {
+        need(grid != nullptr && coefficient != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *coefficient = grid->sigma;
+    });
+}
+
+fs_status fs_get_preconditioner(const fs_grid* grid, fs_preconditioner* kind)
+{
+    return guard([&] {
+        need(grid != nullptr && kind != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *kind = grid->precond;
+    });
+}
+
+fs_status fs_get_transport(const fs_grid* grid, fs_transport* mode)
+{
+    return guard([&] {
+        need(grid != nullptr && mode != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *mode = grid->transport;
+    });
+}
+
+fs_status fs_component_counts(const fs_grid* grid, int* n_liquid, int* n_gas)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        if (n_liquid) *n_liquid = (int)grid->rho_l.size();
+        if (n_gas) *n_gas = (int)grid->rho_g.size();
+    });
+}
+
+fs_status fs_get_component(const fs_grid* grid, int phase, int component, double* density, double* viscosity)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(phase == 0 || phase == 1, FS_E_INVALID_ARGUMENT, "phase must be 0 (liquid) or 1 (gas)");
+        const std::vector<double>& rho = phase == 0 ? grid->rho_l : grid->rho_g;
+        const std::vector<double>& mu = phase == 0 ? grid->mu_l : grid->mu_g;
+        need(component >= 0 && (size_t)component < rho.size(), FS_E_INVALID_ARGUMENT, "Material component is out of range");
+        if (density) *density = rho[component];
+        if (viscosity) *viscosity = mu[component];
+    });
+}
+
+fs_status fs_checkpoint_size(const fs_grid* grid, size_t* metadata_bytes, size_t* payload_bytes)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload before saving a checkpoint");
+        if (metadata_bytes)
+            *metadata_bytes = checkpoint_metadata(*grid).size() + 1;
+        if (payload_bytes)
+            *payload_bytes = checkpoint_values(*grid, grid->rho_l.size(), grid->rho_g.size()) * sizeof(double);
+    });
+}
+
+fs_status fs_checkpoint_save(fs_grid* grid, char* metadata, size_t metadata_capacity, void* payload,
+    size_t payload_capacity)
+{
+    return guard([&] {
+        need(grid != nullptr && metadata != nullptr && payload != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        need(grid->uploaded, FS_E_LOGIC, "Upload before saving a checkpoint");
+        fs_grid& g = *grid;
+        const std::string text = checkpoint_metadata(g);
+        const size_t values = checkpoint_values(g, g.rho_l.size(), g.rho_g.size());
+        need(metadata_capacity >= text.size() + 1 && payload_capacity >= values * sizeof(double),
+            FS_E_INVALID_ARGUMENT, "checkpoint buffers are smaller than fs_checkpoint_size reports");
+        set_device(g.ctx);
+        unsigned char* out = static_cast<unsigned char*>(payload);
+        size_t offset = 0;
+        auto get = [&](const double* src, size_t n) {
+            if (n)
+                HIPC(hipMemcpyAsync(out + offset, src, n * sizeof(double), hipMemcpyDeviceToHost, g.st()));
+            offset += n * sizeof(double);
+        };
+        const size_t N = g.N();
+        get(g.fraction.p, N);
+        get(g.u.p, g.NU());
+        get(g.v.p, g.NV());
+        // Both pressure histories are stored without the halo; the staging buffer is reused in
+        // stream order.
+        launch_pad_copy(g.st(), g.p.p, g.tmp64.p, g.W, g.H, 0);
+        get(g.tmp64.p, N);
+        launch_pad_copy(g.st(), g.p_prev.p, g.tmp64.p, g.W, g.H, 0);
+        get(g.tmp64.p, N);
+        get(g.lmass.p, g.rho_l.size() * N);
+        get(g.gmass.p, g.rho_g.size() * N);
+        HIPC(hipStreamSynchronize(g.st()));
+        std::memcpy(metadata, text.c_str(), text.size() + 1);
+    });
+}
+
+fs_status fs_checkpoint_load(fs_grid* grid, const char* metadata, size_t metadata_length, const void* payload,
+    size_t payload_bytes)
+{
+    return guard([&] {
+        need(grid != nullptr && metadata != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        fs_grid& g = *grid;
+        const CheckpointConfig c = parse_checkpoint(g, metadata, metadata_length);
+        const size_t values = checkpoint_values(g, c.rho_l.size(), c.rho_g.size());
+        if (payload_bytes != values * sizeof(double) || (values && payload == nullptr))
+            bad_checkpoint("fluid payload has the wrong size");
+        // Validated: replace the live configuration and state.
+        set_device(g.ctx);
+        const size_t N = g.N();
+        if (g.lmass.n != c.rho_l.size() * N)
+            g.bytes += g.lmass.alloc(c.rho_l.size() * N);
+        if (g.gmass.n != c.rho_g.size() * N)
+            g.bytes += g.gmass.alloc(c.rho_g.size() * N);
+        g.rho_l = c.rho_l;
+        g.rho_g = c.rho_g;
+        g.mu_l = c.mu_l;
+        g.mu_g = c.mu_g;
+        g.ox = c.ox;
+        g.oy = c.oy;
+        g.gravity_y = c.gravity;
+        g.sigma = c.sigma;
+        g.precond = c.precond;
+        g.transport = c.transport;
+        g.predicted = c.predicted;
+        const unsigned char* in = static_cast<const unsigned char*>(payload);
+        size_t offset = 0;
+        auto put = [&](double* dst, size_t n) {
+            if (n)
+                HIPC(hipMemcpyAsync(dst, in + offset, n * sizeof(double), hipMemcpyHostToDevice, g.st()));
+            offset += n * sizeof(double);
+        };
+        put(g.fraction.p, N);
+        put(g.u.p, g.NU());
+        put(g.v.p, g.NV());
+        put(g.tmp64.p, N);
+        launch_pad_copy(g.st(), g.tmp64.p, g.p.p, g.W, g.H, 1);
+        put(g.tmp64.p, N);
+        launch_pad_copy(g.st(), g.tmp64.p, g.p_prev.p, g.W, g.H, 1);
+        put(g.lmass.p, c.rho_l.size() * N);
+        put(g.gmass.p, c.rho_g.size() * N);
+        HIPC(hipStreamSynchronize(g.st()));
+        g.uploaded = true;
+        g.coeff_valid = false;
+        g.mg_valid = false;
+        g.has_snapshot = false;
+    });
+}
+
+fs_status fs_advance_frame(fs_grid* grid, const fs_frame_settings* settings, fs_frame_statistics* statistics)
+{
+    fs_frame_statistics stats{};
+    const fs_status status = guard([&] {
+        need(grid != nullptr && settings != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        const fs_frame_settings& s = *settings;
+        need(std::isfinite(s.frame_time) && s.frame_time > 0.0, FS_E_INVALID_ARGUMENT,
+            "frame time must be finite and positive");
+        need(s.steps_per_second > 0, FS_E_INVALID_ARGUMENT, "steps per second must be positive");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before stepping");
+        stats.projection_tolerance = s.projection_tolerance;
+        // FluidScene::sceneProcess (FluidScene.inl:343-362): equal substeps at the scene's minimum rate,
+        // or one engine step when the capillary subdivision already reaches that rate; uniform GPU
+        // scenes solve to tolerance without an iteration cutoff.
+        const double wanted = std::ceil(s.frame_time * (double)s.steps_per_second);
+        need(wanted <= 1e6, FS_E_INVALID_ARGUMENT, "frame needs more than 1e6 steps");
+        int step_count = (int)wanted;
+        if (std::ceil(s.frame_time / grid->capillary_step_limit()) >= step_count)
+            step_count = 1;
+        const double step = s.frame_time / step_count;
+        stats.steps_scheduled = step_count;
+        stats.accepted = 1;
+        using Clock = std::chrono::steady_clock;
+        for (int k = 0; k < step_count; ++k) {
+            fs_step_timings t{};
+            fs_projection_result r{};
+            const auto started = Clock::now();
+            const fs_status st = fs_step(grid, step, -1, s.projection_tolerance, s.collect_timings ? &t : nullptr, &r);
+            stats.fluid_ms += std::chrono::duration<double, std::milli>(Clock::now() - started).count();
+            if (st != FS_OK) {
+                stats.accepted = 0;
+                fail(st, g_last_error);
+            }
+            stats.stages.transport_ms += t.transport_ms;
+            stats.stages.viscosity_ms += t.viscosity_ms;
+            stats.stages.forces_ms += t.forces_ms;
+            stats.stages.pressure_ms += t.pressure_ms;
+            stats.stages.selection_ms += t.selection_ms;
+            stats.stages.regrid_ms += t.regrid_ms;
+            stats.substeps += r.physical_steps;
+            stats.transport_steps += r.transport_steps;
+            stats.pressure_iterations += r.iterations;
+            stats.viscosity_iterations += r.viscosity_iterations;
+            stats.max_volume_error = std::max(stats.max_volume_error, r.max_volume_error);
+            stats.rms_volume_error = std::max(stats.rms_volume_error, r.rms_volume_error);
+            stats.max_divergence = r.max_divergence;
+            ++stats.steps_run;
+            // a projection that is not accepted ends the frame (FluidScene.inl:387-393)
+            if (!(r.converged || r.relaxed || r.precision_limited)) {
+                stats.accepted = 0;
+                break;
+            }
+        }
+    });
+    if (statistics)
+        *statistics = stats;
+    return status;
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@ through libfluidsim_hip.so; nothing is computed on the host.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -15,7 +16,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _abi
-from ._abi import FsProjectionResult, FsStepTimings, call, lib
+from ._abi import FsFrameSettings, FsFrameStatistics, FsProjectionResult, FsStepTimings, call, lib
 
 
 # ---- value types mirroring FluidGridTypes.h / FluidMixture.h ----
@@ -68,6 +69,31 @@ class StepTimings:
     pressure_ms: float = 0.0
     selection_ms: float = 0.0
     regrid_ms: float = 0.0
+
+
+@dataclass
+class FluidCheckpoint:
+    """FluidCheckpoint (FluidSim/FluidCheckpoint.h:9-15): backend-owned JSON metadata and native payload."""
+    metadata: dict = field(default_factory=dict)
+    data: bytes = b""
+
+
+@dataclass
+class FrameStatistics:
+    """FluidScene::FrameStatistics (FluidScene.h:162-178) plus the frame's schedule and outcome."""
+    fluid_ms: float = 0.0
+    stages: StepTimings = field(default_factory=StepTimings)
+    substeps: int = 0
+    transport_steps: int = 0
+    pressure_iterations: int = 0
+    viscosity_iterations: int = 0
+    projection_tolerance: float = 0.0
+    max_volume_error: float = 0.0
+    rms_volume_error: float = 0.0
+    steps_scheduled: int = 0
+    steps_run: int = 0
+    accepted: bool = False
+    max_divergence: float = math.inf
 
 
 @dataclass
@@ -150,6 +176,17 @@ def viscosity_solve(width: int, height: int, cell_size: float, material, u, v, d
     call("fs_viscosity_solve", ctx.handle, int(width), int(height), float(cell_size), _dp(m), _dp(uu), _dp(vv),
          float(dt), float(tolerance), int(maximum), C.byref(it), C.byref(rel))
     return it.value, rel.value, uu, vv
+
+
+def _finite_json(value) -> bool:
+    """True if every number in a JSON-like value is finite (JSON has no NaN or infinity)."""
+    if isinstance(value, float):
+        return math.isfinite(value)
+    if isinstance(value, dict):
+        return all(_finite_json(v) for v in value.values())
+    if isinstance(value, (list, tuple)):
+        return all(_finite_json(v) for v in value)
+    return True
 
 
 class FluidGridHIP:
@@ -371,6 +408,81 @@ class FluidGridHIP:
         v = C.c_double()
         call("fs_max_divergence", self._h, C.byref(v))
         return v.value
+
+    # -- checkpoints (FluidGridBase.h:123-125, FluidGridGPU.cpp:566-880) --
+    def saveCheckpoint(self, checkpoint: FluidCheckpoint) -> bool:
+        """Fills `checkpoint` with the library's metadata and FP64 payload; the adapter adds the substance
+        catalogue and each component's substance, which the library does not hold."""
+        mb, pb = C.c_size_t(), C.c_size_t()
+        call("fs_checkpoint_size", self._h, C.byref(mb), C.byref(pb))
+        text = C.create_string_buffer(mb.value)
+        payload = C.create_string_buffer(max(pb.value, 1))
+        call("fs_checkpoint_save", self._h, text, mb.value, payload, pb.value)
+        meta = json.loads(text.value.decode())
+        meta["substances"] = [s if isinstance(s, (str, int, float)) else str(s) for s in self.substances]
+        for key, mixture in (("liquid_mixture", self.liquid_mixture), ("gas_mixture", self.gas_mixture)):
+            for entry, component in zip(meta[key], mixture.components):
+                entry["substance"] = component.substance
+        checkpoint.metadata = meta
+        checkpoint.data = payload.raw[:pb.value]
+        return True
+
+    def loadCheckpoint(self, checkpoint: FluidCheckpoint) -> bool:
+        """Validates and restores a checkpoint; raises FluidSimError (RuntimeError) without changing the
+        grid if it is incompatible or corrupt, as FluidGridGPU::loadCheckpoint throws."""
+        meta = checkpoint.metadata
+        if not isinstance(meta, dict):
+            raise _abi.FluidSimError(_abi.FS_E_RUNTIME, "Checkpoint metadata must be a JSON object")
+        substances = meta.get("substances", [])
+        if not isinstance(substances, list):
+            raise _abi.FluidSimError(_abi.FS_E_RUNTIME, "Checkpoint substances must be a list")
+        names = {}
+        for key in ("liquid_mixture", "gas_mixture"):
+            entries = meta.get(key)
+            names[key] = []
+            for e in entries if isinstance(entries, list) else []:
+                sub = e.get("substance", "") if isinstance(e, dict) else ""
+                if isinstance(e, dict) and "substance" in e and substances and sub not in substances:
+                    raise _abi.FluidSimError(_abi.FS_E_RUNTIME, "Checkpoint mixture references a missing substance")
+                names[key].append(sub)
+        text = json.dumps(meta, allow_nan=False).encode() if _finite_json(meta) else b"null"
+        data = bytes(checkpoint.data)
+        buf = C.create_string_buffer(data, max(len(data), 1))
+        call("fs_checkpoint_load", self._h, text, len(text), buf, len(data))
+        nl, ng = C.c_int(), C.c_int()
+        call("fs_component_counts", self._h, C.byref(nl), C.byref(ng))
+
+        def mixture(phase, count, labels):
+            out = FluidMixture()
+            for k in range(count):
+                d, mu = C.c_double(), C.c_double()
+                call("fs_get_component", self._h, phase, k, C.byref(d), C.byref(mu))
+                out.components.append(MixtureComponent(labels[k] if k < len(labels) else "", d.value, mu.value))
+            return out
+
+        self.liquid_mixture = mixture(0, nl.value, names["liquid_mixture"])
+        self.gas_mixture = mixture(1, ng.value, names["gas_mixture"])
+        self.substances = list(substances)
+        sigma = C.c_double()
+        call("fs_get_surface_tension", self._h, C.byref(sigma))
+        self._surface_tension = sigma.value
+        return True
+
+    # -- scene frame (FluidScene.inl:328-410) --
+    def advanceFrame(self, frame_time: float = 0.02, steps_per_second: int = 60,
+                     projection_tolerance: float = 0.1, profile_stages: bool = False) -> FrameStatistics:
+        """One FluidScene frame: equal steps at the scene rate (one step when the capillary subdivision
+        already reaches it), each solved to tolerance; stops at the first step not accepted."""
+        settings = FsFrameSettings(float(frame_time), int(steps_per_second), float(projection_tolerance),
+                                   int(bool(profile_stages)))
+        st = FsFrameStatistics()
+        status = lib().fs_advance_frame(self._h, C.byref(settings), C.byref(st))
+        out = FrameStatistics(st.fluid_ms, StepTimings(*[getattr(st.stages, n) for n, _ in FsStepTimings._fields_]),
+                              st.substeps, st.transport_steps, st.pressure_iterations, st.viscosity_iterations,
+                              st.projection_tolerance, st.max_volume_error, st.rms_volume_error,
+                              st.steps_scheduled, st.steps_run, bool(st.accepted), st.max_divergence)
+        _abi.check(status, "fs_advance_frame")
+        return out
 
     # -- device-resident snapshots --
     def snapshotSave(self):

@@ -14,6 +14,7 @@
 
 #include "../../include/fluidsim_hip.h"
 
+#include <cstddef>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -72,6 +73,28 @@ struct FluidMixture {
                 return true;
         return false;
     }
+};
+
+// FluidCheckpoint (FluidSim/FluidCheckpoint.h:9-15) with the metadata kept as JSON text.
+struct FluidCheckpoint {
+    std::string metadata;
+    std::vector<std::byte> data;
+};
+
+// FluidScene::FrameStatistics (FluidSim/FluidScene.h:162-178) plus the frame's schedule and outcome.
+struct FrameStatistics {
+    double fluid_ms = 0;
+    FluidStepTimings stages;
+    int substeps = 0;
+    int transport_steps = 0;
+    int pressure_iterations = 0;
+    int viscosity_iterations = 0;
+    double projection_tolerance = 0;
+    double max_volume_error = 0;
+    double rms_volume_error = 0;
+    int steps_scheduled = 0;
+    int steps_run = 0;
+    bool accepted = false;
 };
 
 struct DVec2 {
@@ -284,6 +307,70 @@ public:
         double v = 0.0;
         check(fs_max_divergence(grid_, &v));
         return v;
+    }
+
+    // saveCheckpoint / loadCheckpoint (FluidGridBase.h:123-125, FluidGridGPU.cpp:566-880).  A rejected
+    // load throws std::runtime_error and leaves the grid unchanged.
+    bool saveCheckpoint(FluidCheckpoint& checkpoint) const
+    {
+        size_t meta = 0, payload = 0;
+        check(fs_checkpoint_size(grid_, &meta, &payload));
+        std::string text(meta, '\0');
+        std::vector<std::byte> data(payload);
+        check(fs_checkpoint_save(grid_, text.data(), text.size(), data.data(), data.size()));
+        text.resize(meta - 1);
+        checkpoint.metadata = std::move(text);
+        checkpoint.data = std::move(data);
+        return true;
+    }
+
+    bool loadCheckpoint(const FluidCheckpoint& checkpoint)
+    {
+        check(fs_checkpoint_load(grid_, checkpoint.metadata.data(), checkpoint.metadata.size(),
+            checkpoint.data.data(), checkpoint.data.size()));
+        // refresh the adapter's view of the restored mixtures (substance ids are kept by position)
+        int nl = 0, ng = 0;
+        check(fs_component_counts(grid_, &nl, &ng));
+        auto restore = [&](FluidMixture& mixture, int phase, int count) {
+            FluidMixture out;
+            for (int k = 0; k < count; ++k) {
+                MixtureComponent c;
+                c.substance = (size_t)k < mixture.components.size() ? mixture.components[k].substance : 0;
+                check(fs_get_component(grid_, phase, k, &c.density, &c.dynamic_viscosity));
+                out.components.push_back(c);
+            }
+            mixture = std::move(out);
+        };
+        restore(liquid_, 0, nl);
+        restore(gas_, 1, ng);
+        check(fs_get_surface_tension(grid_, &surface_tension_));
+        return true;
+    }
+
+    // One FluidScene frame (FluidScene.inl:328-410): equal steps at the scene's rate, or one step when
+    // the capillary subdivision reaches it; unbounded pressure budget; stops at a step not accepted.
+    FrameStatistics advanceFrame(double frame_time = 0.02, int steps_per_second = 60,
+        double projection_tolerance = 0.1, bool profile_stages = false)
+    {
+        fs_frame_settings settings{frame_time, steps_per_second, projection_tolerance, profile_stages ? 1 : 0};
+        fs_frame_statistics st{};
+        const fs_status status = fs_advance_frame(grid_, &settings, &st);
+        FrameStatistics out;
+        out.fluid_ms = st.fluid_ms;
+        out.stages = {st.stages.transport_ms, st.stages.viscosity_ms, st.stages.forces_ms, st.stages.pressure_ms,
+            st.stages.selection_ms, st.stages.regrid_ms};
+        out.substeps = st.substeps;
+        out.transport_steps = st.transport_steps;
+        out.pressure_iterations = st.pressure_iterations;
+        out.viscosity_iterations = st.viscosity_iterations;
+        out.projection_tolerance = st.projection_tolerance;
+        out.max_volume_error = st.max_volume_error;
+        out.rms_volume_error = st.rms_volume_error;
+        out.steps_scheduled = st.steps_scheduled;
+        out.steps_run = st.steps_run;
+        out.accepted = st.accepted != 0;
+        check(status);
+        return out;
     }
 
     fs_grid* handle() const { return grid_; }

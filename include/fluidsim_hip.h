@@ -141,7 +141,9 @@ fs_status fs_download(fs_grid* grid, double* liquid_fraction, double* u, double*
 fs_status fs_invalidate_caches(fs_grid* grid);
 
 /* step(dt, max_iterations, divergence_tolerance, timings) (FluidGridBase.h:133-134,
- * FluidGridCPU.cpp:300-360).  `timings` may be NULL. */
+ * FluidGridCPU.cpp:300-360).  `timings` may be NULL.  max_iterations < 0 solves each projection to
+ * tolerance (the uniform WebGPU backend's unbounded budget, FluidGridGPU.cpp:1326), stopping after
+ * 100000 PCG iterations; the reference's CPU backend returns an unconverged result instead. */
 fs_status fs_step(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
     fs_step_timings* timings, fs_projection_result* result);
 /* FluidGridCPU::project(dt, max_iterations, divergence_tolerance) (FluidGridCPU.cpp:1030-1124) */
@@ -220,6 +222,76 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
 fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch);
 /* Cumulative device bytes allocated by this grid. */
 fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);
+
+/* ---- configuration getters (FluidGridBase.h:43-46, 70-80; FluidGridCPU.h:39-44) ----
+ * getSurfaceTension, pressure_preconditioner, the transport mode, and the configured mixture
+ * (FluidMixture components: density and dynamic viscosity), so that an adapter can refresh its own
+ * view after fs_checkpoint_load.  phase 0 = liquid, 1 = gas. */
+fs_status fs_get_surface_tension(const fs_grid* grid, double* coefficient);
+fs_status fs_get_preconditioner(const fs_grid* grid, fs_preconditioner* kind);
+fs_status fs_get_transport(const fs_grid* grid, fs_transport* mode);
+fs_status fs_component_counts(const fs_grid* grid, int* n_liquid, int* n_gas);
+fs_status fs_get_component(const fs_grid* grid, int phase, int component, double* density, double* viscosity);
+
+/* ---- resident-instance checkpoint: saveCheckpoint / loadCheckpoint (FluidGridBase.h:123-125,
+ * FluidSim/FluidCheckpoint.h:9-15, FluidSim/UniformWebGPU/FluidGridGPU.cpp:566-695 and 697-880) ----
+ * A checkpoint is JSON metadata (backend "hip-uniform", version 1, scalar "float64", byte order,
+ * geometry, origin, gravity, surface tension, preconditioner, transport, both mixtures' densities and
+ * viscosities, the previous solve's iteration count, the payload layout) plus a native payload of FP64
+ * values in the fixed order liquid_fraction (W*H), u ((W+1)*H), v (W*(H+1)), pressure (W*H), previous
+ * pressure (W*H), liquid masses (n_liquid*W*H), gas masses (n_gas*W*H).  Both pressure histories are
+ * kept, so a loaded instance continues bit for bit as the saved one would have.
+ *
+ * fs_checkpoint_size: bytes of the metadata text including its terminating NUL, and of the payload.
+ * fs_checkpoint_save: writes both; FS_E_LOGIC before the first upload, FS_E_INVALID_ARGUMENT if a
+ *   buffer is smaller than fs_checkpoint_size reports.  Saving does not change the instance.
+ * fs_checkpoint_load: `metadata` is `metadata_length` bytes of JSON text (members it does not know are
+ *   ignored, so an adapter may add its own).  Every field is validated before any live state changes;
+ *   an incompatible format, another geometry, a missing or invalid member or a payload of the wrong
+ *   size returns FS_E_RUNTIME with the instance untouched (the reference throws std::runtime_error).
+ *   A load replaces the configuration and the state, resets derived caches and drops the
+ *   device-resident snapshot. */
+fs_status fs_checkpoint_size(const fs_grid* grid, size_t* metadata_bytes, size_t* payload_bytes);
+fs_status fs_checkpoint_save(fs_grid* grid, char* metadata, size_t metadata_capacity, void* payload,
+    size_t payload_capacity);
+fs_status fs_checkpoint_load(fs_grid* grid, const char* metadata, size_t metadata_length, const void* payload,
+    size_t payload_bytes);
+
+/* ---- scene frame: FluidScene::sceneProcess (FluidSim/FluidScene.inl:328-410) ----
+ * Settings mirror FluidSceneSettings (FluidScene.h:21-35): frame_time (default 0.02 s),
+ * steps_per_second (default 60), projection_tolerance (default 0.1); collect_timings asks each step for
+ * its stage timings (the scene's profile_stages). */
+typedef struct fs_frame_settings {
+    double frame_time;
+    int steps_per_second;
+    double projection_tolerance;
+    int collect_timings;
+} fs_frame_settings;
+
+/* FluidScene::FrameStatistics (FluidScene.h:162-178) plus the schedule and the outcome. */
+typedef struct fs_frame_statistics {
+    double fluid_ms;            /* host wall time of the engine steps */
+    fs_step_timings stages;     /* summed stage timings (zero unless collect_timings) */
+    int substeps;               /* physical steps (capillary substeps included) */
+    int transport_steps;
+    int pressure_iterations;
+    int viscosity_iterations;
+    double projection_tolerance;
+    double max_volume_error;
+    double rms_volume_error;
+    /* no reference counterpart */
+    int steps_scheduled;        /* engine steps the frame was split into */
+    int steps_run;              /* engine steps taken (fewer when one was not accepted) */
+    int accepted;               /* 0 once a projection was not accepted: the scene stops simulating */
+    double max_divergence;      /* of the last step */
+} fs_frame_statistics;
+
+/* One scene frame on a uniform grid of this backend: step_count = ceil(frame_time * steps_per_second)
+ * equal steps, or a single step when ceil(frame_time / capillaryStepLimit) already reaches step_count
+ * (the engine's capillary subdivision then provides the rate); each step runs fs_step with an unbounded
+ * pressure budget (-1: to tolerance, at most 100000 iterations) and the frame stops at the first step
+ * whose projection is not accepted.  `statistics` (may be NULL) is filled also when a step fails. */
+fs_status fs_advance_frame(fs_grid* grid, const fs_frame_settings* settings, fs_frame_statistics* statistics);
 
 #ifdef __cplusplus
 }

@@ -271,7 +271,8 @@ class OracleGrid:
         return lib().orc_transport_max_step(self._h, dt)
 
     def step(self, dt, max_iterations=200, divergence_tolerance=1e-6):
-        """step(), FluidGridCPU.cpp:300-360 (surface tension off)."""
+        """step(), FluidGridCPU.cpp:300-360: capillary substeps, transport, the WebGPU viscosity stage,
+        gravity, surface tension, projection."""
         return lib().orc_step(self._h, dt, max_iterations, divergence_tolerance).as_dict()
 
     def configure_viscosity(self, liquid_viscosity, gas_viscosity):
@@ -355,3 +356,38 @@ def capillary_geometry(width, height, cell_size, fraction):
     heights, raw, curv = np.zeros(4 * n), np.zeros(2 * n), np.zeros(2 * n)
     lib().orc_capillary_geometry(width, height, cell_size, _ptr(f), _ptr(heights), _ptr(raw), _ptr(curv))
     return heights.reshape(n, 4), raw.reshape(n, 2), curv.reshape(n, 2)
+
+
+# ---- scene frame (FluidSim/FluidScene.inl:328-410) ----
+
+UNBOUNDED_BUDGET = 100000   # iterations that stand for the scene's unbounded (-1) pressure budget
+
+
+def frame_schedule(frame_time, steps_per_second, capillary_step_limit):
+    """(step_count, step) of one scene frame on a uniform GPU grid (FluidScene.inl:343-353): equal steps at
+    the scene's minimum rate, or one step when the engine's capillary subdivision already reaches it."""
+    import math
+
+    count = int(math.ceil(frame_time * float(steps_per_second)))
+    if math.ceil(frame_time / capillary_step_limit) >= count:
+        count = 1
+    return count, frame_time / count
+
+
+def run_frame(grid, frame_time=0.02, steps_per_second=60, projection_tolerance=0.1):
+    """One frame on an OracleGrid: the schedule above, each step solved to tolerance, statistics summed as
+    FluidScene.inl:371-393 does, stopping at the first step that is not accepted."""
+    count, step = frame_schedule(frame_time, steps_per_second, grid.capillary_step_limit())
+    stats = dict(steps_scheduled=count, steps_run=0, substeps=0, transport_steps=0, pressure_iterations=0,
+                 viscosity_iterations=0, accepted=True)
+    for _ in range(count):
+        r = grid.step(step, UNBOUNDED_BUDGET, projection_tolerance)
+        stats["steps_run"] += 1
+        stats["substeps"] += r["physical_steps"]
+        stats["transport_steps"] += r.get("transport_steps", 0)
+        stats["pressure_iterations"] += r["iterations"]
+        stats["viscosity_iterations"] += r.get("viscosity_iterations", 0)
+        if not r["converged"]:
+            stats["accepted"] = False
+            break
+    return stats

@@ -1,7 +1,8 @@
 // C++ boundary test of the HIP backend through the FluidGridHIP adapter (fluidsim_amd/host/).
 // Mirrors the contract checks of the reference's tests/FluidGridAPITests.cpp: deterministic exact
 // results with and without timings (:83-134), zero-time and zero-iteration steps reported, not thrown
-// (:186-229), upload validation, and the hydrostatic closed form of tests/FluidGridTests.cpp:140-160.
+// (:186-229), upload validation, the hydrostatic closed form of tests/FluidGridTests.cpp:140-160,
+// checkpoint continuation (tests/FluidWebGPUAPITests.cpp:518-666) and the scene frame schedule.
 // Needs a GPU; run by tests/test_cpp_adapter.py.
 #include "fluidsim_amd/host/FluidGridHIP.hpp"
 
@@ -148,6 +149,70 @@ void hydrostaticPressure(HipContext& ctx)
     require(std::abs(rise - expected) < expected * 1e-8, "incorrect hydrostatic pressure gradient");
 }
 
+// tests/FluidWebGPUAPITests.cpp:518-666 through the C++ adapter: a restored instance continues
+// bit for bit, saving changes nothing, and a corrupt checkpoint is rejected without mutation.
+void checkpointContinuation(HipContext& ctx)
+{
+    FluidGridHIP original(ctx, 33, 25, 2.5), resumed(ctx, 33, 25, 2.5);
+    configure(original);
+    original.setComponentViscosity(fluid_hip::Phase::Liquid, 1, 40.0);
+    for (int tick = 0; tick < 20; ++tick)
+        require(original.step(0.004, 2000, 0.01).converged, "checkpoint warm-up failed");
+    const auto before = original.download();
+    fluid_hip::FluidCheckpoint saved;
+    require(original.saveCheckpoint(saved), "saving failed");
+    require(sameState(before, original.download()), "saving changed the state");
+    require(saved.data.size() == ((3 + 4) * 33 * 25 + 34 * 25 + 33 * 26) * sizeof(double), "payload size");
+    require(resumed.loadCheckpoint(saved), "loading failed");
+    require(sameState(before, resumed.download()), "loading changed the state");
+    require(resumed.getGravityY() == 80.0 && resumed.getOrigin().x == -7.0 &&
+                resumed.getLiquidMixture().components.size() == 2 &&
+                resumed.getLiquidMixture().components[1].dynamic_viscosity == 40.0,
+        "loading lost the configuration");
+    for (int tick = 0; tick < 20; ++tick) {
+        const auto a = original.step(0.004, 2000, 0.01);
+        const auto b = resumed.step(0.004, 2000, 0.01);
+        require(a.converged && b.converged && a.iterations == b.iterations &&
+                    a.viscosity_iterations == b.viscosity_iterations && a.max_divergence == b.max_divergence,
+            "resumed solve differs");
+    }
+    require(sameState(original.download(), resumed.download()), "resumed trajectory differs");
+    fluid_hip::FluidCheckpoint a, b;
+    original.saveCheckpoint(a);
+    resumed.saveCheckpoint(b);
+    require(a.metadata == b.metadata && a.data == b.data, "continued checkpoints differ");
+    auto corrupt = a;
+    corrupt.metadata.replace(corrupt.metadata.find("\"width\":33"), 10, "\"width\":34");
+    bool threw = false;
+    try {
+        resumed.loadCheckpoint(corrupt);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    fluid_hip::FluidCheckpoint after;
+    resumed.saveCheckpoint(after);
+    require(threw && after.metadata == b.metadata && after.data == b.data, "corrupt checkpoint was not rejected");
+}
+
+// FluidScene.inl:343-393: ceil(frame_time * rate) equal steps, statistics summed over them.
+void sceneFrame(HipContext& ctx)
+{
+    FluidGridHIP framed(ctx, 24, 20, 1.0), stepped(ctx, 24, 20, 1.0);
+    configure(framed);
+    configure(stepped);
+    const auto stats = framed.advanceFrame(0.02, 60, 0.1, true);
+    int iterations = 0;
+    for (int k = 0; k < 2; ++k) {
+        const auto r = stepped.step(0.01, -1, 0.1);
+        require(r.converged, "frame reference step failed");
+        iterations += r.iterations;
+    }
+    require(stats.accepted && stats.steps_scheduled == 2 && stats.steps_run == 2 && stats.substeps == 2,
+        "frame schedule");
+    require(stats.pressure_iterations == iterations && stats.stages.pressure_ms > 0.0, "frame statistics");
+    require(sameState(framed.download(), stepped.download()), "frame differs from its explicit steps");
+}
+
 }  // namespace
 
 int main()
@@ -158,7 +223,8 @@ int main()
         void (*run)(HipContext&);
     } tests[] = {{"timed and untimed steps are identical", timedAndUntimedStepsAreIdentical},
         {"zero time and zero iterations are reported", zeroIterationAndZeroTimeAreReported},
-        {"upload validation", uploadIsValidated}, {"hydrostatic pressure", hydrostaticPressure}};
+        {"upload validation", uploadIsValidated}, {"hydrostatic pressure", hydrostaticPressure},
+        {"checkpoint continuation", checkpointContinuation}, {"scene frame", sceneFrame}};
     int failures = 0;
     for (const auto& t : tests) {
         try {

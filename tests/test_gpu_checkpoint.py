@@ -1,0 +1,211 @@
+"""Resident-instance checkpoints on the GPU (FluidGridGPU::saveCheckpoint / loadCheckpoint,
+FluidGridGPU.cpp:566-880), with the reference's own contract (tests/FluidWebGPUAPITests.cpp:518-700,
+tests/FluidViscosityIntegrationTests.cpp:165-210, tests/FluidSurfaceTensionTests.cpp:296-316):
+
+  - saving changes nothing; the payload holds exactly the native fields;
+  - a fresh instance that loads the checkpoint continues bit for bit as the uninterrupted one (same
+    iterations, same divergence, same state, same later checkpoint), for the multigrid and diagonal
+    preconditioners, with viscosity and with surface tension;
+  - the metadata survives a JSON text round trip;
+  - every incompatible or corrupt checkpoint is rejected without changing the live instance.
+"""
+import copy
+import json
+
+import numpy as np
+import pytest
+
+from fluidsim_amd import (FS_PRECOND_DIAGONAL, FluidCheckpoint, FluidGridHIP, FluidGridState, FluidMixture,
+                          FluidSimError, MixtureComponent)
+
+from test_oracle_tension import disk
+
+pytestmark = pytest.mark.gpu
+
+W, H, CELL = 33, 25, 2.5
+
+
+def same(a, b):
+    return (np.array_equal(a.liquid_fraction, b.liquid_fraction) and np.array_equal(a.u, b.u)
+            and np.array_equal(a.v, b.v) and np.array_equal(a.pressure, b.pressure)
+            and all(np.array_equal(x, y) for x, y in zip(a.liquid_mass, b.liquid_mass))
+            and all(np.array_equal(x, y) for x, y in zip(a.gas_mass, b.gas_mass))
+            and len(a.liquid_mass) == len(b.liquid_mass) and len(a.gas_mass) == len(b.gas_mass))
+
+
+def make_grid(extra_gas=False, viscosity=0.0, sigma=0.0, precond=None, gravity=73.0):
+    """Two liquids and two (or three) gases over a sloped pool, as the reference fixture."""
+    g = FluidGridHIP(W, H, CELL)
+    gases = [MixtureComponent("air", 0.1), MixtureComponent("steam", 0.1)]
+    if extra_gas:
+        gases.append(MixtureComponent("extra gas", 0.1))
+    g.configureMaterials(["water", "oil", "air", "steam", "extra gas"],
+                         FluidMixture([MixtureComponent("water", 100.0, viscosity), MixtureComponent("oil", 300.0)]),
+                         FluidMixture(gases))
+    N = W * H
+    ys, xs = np.mgrid[0:H, 0:W]
+    f = np.clip(ys + 1.0 - (xs * 0.4 + H * 0.45), 0.0, 1.0).ravel()
+    if sigma > 0.0:
+        f = np.maximum(f, disk(W, H, 4.0, False, 12.0, 6.0))
+    area = CELL * CELL
+    red = (xs < W // 4).ravel().astype(float)
+    cloud = (0.15 + 0.1 * np.sin(0.2 * xs)).ravel()
+    lm = [100.0 * f * area * (1.0 - red), 300.0 * f * area * red]
+    gm = [0.1 * (1.0 - f) * area * (1.0 - cloud), 0.1 * (1.0 - f) * area * cloud]
+    if extra_gas:
+        gm = [gm[0], gm[1] * 0.6, gm[1] * 0.4]
+    g.upload(FluidGridState(f, np.zeros(0), np.zeros(0), np.zeros(0), lm, gm))
+    g.setGravityY(gravity)
+    g.setOrigin((-7.0, 4.0))
+    if sigma > 0.0:
+        g.setSurfaceTension(sigma)
+    if precond is not None:
+        g.setPressurePreconditioner(precond)
+    return g
+
+
+def saved(g):
+    c = FluidCheckpoint()
+    assert g.saveCheckpoint(c)
+    return c
+
+
+@pytest.mark.parametrize("variant", ["multigrid", "extra_gas", "diagonal", "viscous", "capillary"])
+def test_checkpoint_continues_bit_for_bit(variant):
+    kw = dict(extra_gas=variant == "extra_gas", viscosity=120.0 if variant == "viscous" else 0.0,
+              sigma=4000.0 if variant == "capillary" else 0.0,
+              precond=FS_PRECOND_DIAGONAL if variant == "diagonal" else None)
+    original = make_grid(**kw)
+    for tick in range(30):
+        assert original.step(0.004, 2000, 0.01).accepted(), f"warm-up step {tick}"
+    before = original.download()
+    c = saved(original)
+    assert same(before, original.download()), "saving changed the state"
+    groups = 2 + (3 if kw["extra_gas"] else 2)
+    assert len(c.data) == ((3 + groups) * W * H + (W + 1) * H + W * (H + 1)) * 8
+    c.metadata = json.loads(json.dumps(c.metadata))   # the on-disk text round trip
+    assert c.metadata["backend"] == "hip-uniform" and c.metadata["version"] == 1
+    assert c.metadata["scalar"] == "float64" and c.metadata["width"] == W and c.metadata["cell_size"] == CELL
+
+    resumed = FluidGridHIP(W, H, CELL)          # never configured or uploaded
+    assert resumed.loadCheckpoint(c)
+    assert same(before, resumed.download()), "loading changed the state"
+    assert resumed.getOrigin() == (-7.0, 4.0) and resumed.getGravityY() == 73.0
+    assert resumed.getSurfaceTension() == kw["sigma"]
+    assert [m.substance for m in resumed.getGasMixture().components] == \
+        [m.substance for m in original.getGasMixture().components]
+    assert resumed.getLiquidMixture().components[0].dynamic_viscosity == kw["viscosity"]
+    observed = saved(resumed)
+    assert observed.metadata == c.metadata and observed.data == c.data, "restore lost configuration or history"
+
+    for tick in range(40):
+        a = original.step(0.004, 2000, 0.01)
+        b = resumed.step(0.004, 2000, 0.01)
+        assert a.accepted() and b.accepted()
+        assert (a.iterations, a.max_divergence, a.viscosity_iterations, a.physical_steps) == \
+            (b.iterations, b.max_divergence, b.viscosity_iterations, b.physical_steps), f"tick {tick}"
+        assert same(original.download(), resumed.download()), f"trajectory differs at tick {tick}"
+    x, y = saved(original), saved(resumed)
+    assert x.metadata == y.metadata and x.data == y.data, "continued checkpoint history differs"
+    original.destroy()
+    resumed.destroy()
+
+
+def test_pressure_history_is_part_of_the_checkpoint():
+    """Re-uploading the downloaded state restarts the pressure prediction (FluidGridCPU.cpp:249), so it
+    diverges from the uninterrupted run; the checkpoint keeps both pressures and does not."""
+    original = make_grid()
+    for _ in range(20):
+        assert original.step(0.004, 2000, 0.01).converged
+    c = saved(original)
+    resumed = FluidGridHIP(W, H, CELL)
+    resumed.loadCheckpoint(c)
+    reuploaded = make_grid()
+    reuploaded.upload(original.download())
+    ra = original.step(0.004, 2000, 0.01)
+    rb = resumed.step(0.004, 2000, 0.01)
+    rc = reuploaded.step(0.004, 2000, 0.01)
+    assert ra.iterations == rb.iterations and same(original.download(), resumed.download())
+    assert not same(original.download(), reuploaded.download())
+    assert rc.converged
+
+
+def test_initial_state_checkpoint_with_diagonal_solver():
+    """tests/FluidWebGPUAPITests.cpp:670-700: saved before the first pressure solve."""
+    native = make_grid(precond=FS_PRECOND_DIAGONAL, gravity=11.0)
+    c = saved(native)
+    loaded = FluidGridHIP(W, H, CELL)
+    loaded.loadCheckpoint(c)
+    for tick in range(60):
+        a = native.step(0.004, 2000, 0.01)
+        b = loaded.step(0.004, 2000, 0.01)
+        assert a.converged and b.converged and a.iterations == b.iterations and a.max_divergence == b.max_divergence
+    assert same(native.download(), loaded.download())
+    assert c.metadata["preconditioner"] == FS_PRECOND_DIAGONAL and c.metadata["gravity_y"] == 11.0
+
+
+def _corruptions():
+    def meta(key, value):
+        return lambda c: c.metadata.__setitem__(key, value)
+
+    def drop(key):
+        return lambda c: c.metadata.pop(key)
+
+    def mix(key, field, value):
+        return lambda c: c.metadata[key][0].__setitem__(field, value)
+
+    return {
+        "version": meta("version", 999),
+        "backend": meta("backend", "another-engine"),
+        "width": meta("width", W + 1),
+        "height": meta("height", H - 1),
+        "cell_size": meta("cell_size", CELL * 2),
+        "scalar": meta("scalar", "float32"),
+        "byte_order": meta("byte_order", "unsupported"),
+        "payload_short": lambda c: setattr(c, "data", c.data[:-8]),
+        "payload_long": lambda c: setattr(c, "data", c.data + b"\0" * 8),
+        "missing_substance": mix("liquid_mixture", "substance", "unobtainium"),
+        "missing_history": drop("previous_solve_iterations"),
+        "negative_viscosity": mix("liquid_mixture", "dynamic_viscosity", -1.0),
+        "zero_density": mix("gas_mixture", "density", 0.0),
+        "negative_tension": meta("surface_tension", -1.0),
+        "preconditioner": meta("preconditioner", 7),
+        "transport": meta("transport", 9),
+        "fractional_width": meta("width", W + 0.5),
+        "origin": meta("origin", [1.0]),
+        "layout": meta("payload", ["u", "v"]),
+        "string_number": meta("gravity_y", "73"),
+        "too_many_components": lambda c: c.metadata.__setitem__("gas_mixture", [{"density": 1.0}] * 65),
+        "component_count": lambda c: c.metadata["gas_mixture"].append({"density": 0.2}),
+    }
+
+
+@pytest.mark.parametrize("case", sorted(_corruptions()))
+def test_corrupt_checkpoint_is_rejected_without_mutation(case):
+    g = make_grid(viscosity=10.0)
+    for _ in range(5):
+        assert g.step(0.004, 2000, 0.01).converged
+    good = saved(g)
+    bad = copy.deepcopy(good)
+    _corruptions()[case](bad)
+    target = make_grid(extra_gas=True, gravity=5.0)
+    target.step(0.004, 2000, 0.01)
+    before = saved(target)
+    with pytest.raises(FluidSimError):
+        target.loadCheckpoint(bad)
+    after = saved(target)
+    assert after.metadata == before.metadata and after.data == before.data, "rejected load mutated the grid"
+    assert len(target.getGasMixture().components) == 3
+    # the untouched checkpoint still loads into the same target
+    assert target.loadCheckpoint(good)
+    assert same(g.download(), target.download())
+
+
+def test_checkpoint_contract_errors():
+    g = FluidGridHIP(8, 6, 1.0)
+    with pytest.raises(RuntimeError):          # FS_E_LOGIC: nothing uploaded yet
+        g.saveCheckpoint(FluidCheckpoint())
+    with pytest.raises(FluidSimError):
+        g.loadCheckpoint(FluidCheckpoint({"backend": "hip-uniform"}, b""))
+    with pytest.raises(FluidSimError):         # the checkpoint of another geometry
+        g.loadCheckpoint(saved(make_grid()))
