@@ -83,6 +83,48 @@ class FsFrameStatistics(C.Structure):
     ]
 
 
+class FsServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("lanes", C.c_int), ("hbm_budget_bytes", C.c_size_t)]
+
+
+class FsRequest(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("cell_size", C.c_double), ("steps", C.c_int), ("dt", C.c_double),
+        ("max_iterations", C.c_int), ("divergence_tolerance", C.c_double), ("gravity_y", C.c_double),
+        ("surface_tension", C.c_double), ("transport", C.c_int), ("preconditioner", C.c_int),
+        ("n_liquid", C.c_int), ("n_gas", C.c_int),
+        ("liquid_density", C.POINTER(C.c_double)), ("liquid_viscosity", C.POINTER(C.c_double)),
+        ("gas_density", C.POINTER(C.c_double)), ("gas_viscosity", C.POINTER(C.c_double)),
+        ("liquid_fraction", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
+        ("liquid_mass", C.POINTER(C.c_double)), ("gas_mass", C.POINTER(C.c_double)),
+    ]
+
+
+FS_REQUEST_QUEUED, FS_REQUEST_RUNNING, FS_REQUEST_DONE, FS_REQUEST_FAILED = 0, 1, 2, 3
+
+FS_FEATURE_TRANSPORT, FS_FEATURE_VISCOSITY, FS_FEATURE_SURFACE_TENSION = 1, 2, 4
+
+
+class FsRequestReport(C.Structure):
+    _fields_ = [
+        ("state", C.c_int), ("lane", C.c_int), ("steps_done", C.c_int), ("pressure_iterations", C.c_int),
+        ("viscosity_iterations", C.c_int), ("physical_steps", C.c_int), ("max_divergence", C.c_double),
+        ("queue_ms", C.c_double), ("run_ms", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
+        ("step_max_ms", C.c_double), ("estimated_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
+        ("error", C.c_char * 256),
+    ]
+
+
+class FsServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_longlong), ("admitted", C.c_longlong), ("completed", C.c_longlong),
+        ("failed", C.c_longlong), ("steps", C.c_longlong), ("queued", C.c_int), ("resident", C.c_int),
+        ("budget_bytes", C.c_size_t), ("resident_bytes", C.c_size_t), ("peak_resident_bytes", C.c_size_t),
+        ("elapsed_s", C.c_double), ("steps_per_s", C.c_double), ("step_p50_ms", C.c_double),
+        ("step_p99_ms", C.c_double),
+    ]
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -144,6 +186,17 @@ SIGNATURES = {
     "fs_checkpoint_save": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_checkpoint_load": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_advance_frame": [_vp, C.POINTER(FsFrameSettings), C.POINTER(FsFrameStatistics)],
+    "fs_grid_estimate_bytes": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)],
+    "fs_server_create": [C.POINTER(FsServerConfig), C.POINTER(_vp)],
+    "fs_server_destroy": [_vp],
+    "fs_server_submit": [_vp, C.POINTER(FsRequest), C.POINTER(C.c_longlong)],
+    "fs_server_wait": [_vp, C.c_longlong, C.c_double, C.POINTER(FsRequestReport)],
+    "fs_server_query": [_vp, C.c_longlong, C.POINTER(FsRequestReport)],
+    "fs_server_result": [_vp, C.c_longlong, _dp, _dp, _dp, _dp, _dp, _dp],
+    "fs_server_release": [_vp, C.c_longlong],
+    "fs_server_drain": [_vp],
+    "fs_server_get_stats": [_vp, C.POINTER(FsServerStats)],
+    "fs_server_reset_stats": [_vp],
 }
 
 _lib = None

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfluidsim_hip.so")
 ARCH = os.environ.get("FLUIDSIM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fs_kernels.hip", "fs_multigrid.hip", "fs_transport.hip", "fs_viscosity.hip", "fs_tension.hip", "fs_host.cpp"]
+SOURCES = ["fs_kernels.hip", "fs_multigrid.hip", "fs_transport.hip", "fs_viscosity.hip", "fs_tension.hip", "fs_host.cpp",
+           "fs_server.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -5,6 +5,7 @@
 // kernels on the context's stream; the PCG iteration body is captured once into a hipGraph and
 // replayed.  All loop decisions (alpha, beta, convergence, restarts, iteration budget) live in a
 // device scalar block, so the host only polls a status word between batches of iterations.
+#include "fs_internal.hpp"
 #include "fs_json.hpp"
 #include "fs_kernels.hpp"
 #include "../../include/fluidsim_hip.h"
@@ -17,6 +18,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -39,6 +41,19 @@ void hip_check(hipError_t e, const char* what)
 }
 
 #define HIPC(call) hip_check((call), #call)
+
+}  // namespace
+
+// Serialises the calls that implicitly synchronise with every blocking stream (allocation, freeing,
+// synchronous copies and fills on the null stream) against hipGraph capture on another thread's
+// stream, so that grids may be created, captured and destroyed concurrently from several threads.
+std::mutex& device_setup_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+namespace {
 
 template <class T> struct DevBuf {
     T* p = nullptr;
@@ -66,8 +81,10 @@ template <class T> struct DevBuf {
     }
     void release()
     {
-        if (p)
+        if (p) {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
             (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -78,6 +95,7 @@ template <class T> struct DevBuf {
         n = count;
         if (count == 0)
             return 0;
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
         HIPC(hipMalloc(&p, count * sizeof(T)));
         HIPC(hipMemset(p, 0, count * sizeof(T)));
         return count * sizeof(T);
@@ -143,6 +161,7 @@ void build_axis(const std::vector<double>& fc, const std::vector<double>& cc, st
 template <class T> void upload_vec(DevBuf<T>& d, const std::vector<T>& h, size_t& bytes)
 {
     bytes += d.alloc(h.size());
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
     if (!h.empty())
         HIPC(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
 }
@@ -158,6 +177,8 @@ std::vector<float> to_float(const std::vector<double>& v)
 thread_local std::string g_last_error;
 
 }  // namespace
+
+void set_last_error(const std::string& message) { g_last_error = message; }
 
 }  // namespace fsim
 
@@ -526,7 +547,10 @@ struct fs_grid {
             limit = 65536 - 4096;  // the device refused a larger dynamic LDS request
         }
         bytes += tail_dev.alloc(1);
-        HIPC(hipMemcpy(tail_dev.p, &tail, sizeof(TailDesc), hipMemcpyHostToDevice));
+        {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            HIPC(hipMemcpy(tail_dev.p, &tail, sizeof(TailDesc), hipMemcpyHostToDevice));
+        }
     }
 
     // ---------------------------------------------------------------------------------------
@@ -668,6 +692,7 @@ struct fs_grid {
         if (iter_exec && graph_precond == precond)
             return;
         drop_graph();
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
         HIPC(hipStreamBeginCapture(st(), hipStreamCaptureModeRelaxed));
         iteration_kernels();
         HIPC(hipStreamEndCapture(st(), &iter_graph));
@@ -1013,6 +1038,7 @@ fs_status fs_context_create(int device, fs_context** out)
         auto ctx = std::make_unique<fs_context>();
         ctx->device = device;
         HIPC(hipSetDevice(device));
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
             ctx->lds_per_block = (size_t)lds;
@@ -1036,6 +1062,7 @@ fs_status fs_context_destroy(fs_context* ctx)
         (void)hipSetDevice(ctx->device);
         if (ctx->stream)
             (void)hipStreamSynchronize(ctx->stream);
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
         if (ctx->ev0)
             (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1)
@@ -2054,6 +2081,51 @@ fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes)
     return guard([&] {
         need(grid != nullptr && bytes != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
         *bytes = grid->bytes;
+    });
+}
+
+fs_status fs_grid_estimate_bytes(int width, int height, int n_liquid, int n_gas, int features, size_t* bytes)
+{
+    return guard([&] {
+        need(bytes != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        need(width > 0 && height > 0 && n_liquid >= 0 && n_gas >= 0, FS_E_INVALID_ARGUMENT,
+            "grid dimensions must be positive and component counts non-negative");
+        const size_t W = (size_t)width, H = (size_t)height, N = W * H, NU = (W + 1) * H, NV = W * (H + 1);
+        const size_t NP = (W + 2) * (H + 2), nl = (size_t)n_liquid, ng = (size_t)n_gas;
+        const size_t blocks = ((W + 63) / 64) * ((H + 15) / 16);
+        // state, pressure history, PCG vectors and reductions (fs_grid::allocate)
+        size_t b = 8 * (3 * N + NU + NV + 3 * NP) + 4 * 5 * NP + 8 * std::max<size_t>(4 * blocks, 64) +
+                   4 * sizeof(PcgScalars) + 8 * (nl + ng) * N;
+        // pressure hierarchy (build_hierarchy): FP64 + FP32 conductances, 1/diag, rhs, two solutions,
+        // residual, transfer tables; dense coarse factor and inverse; tail tables
+        size_t w = W, h = H;
+        for (int l = 0;; ++l) {
+            const size_t np = (w + 2) * (h + 2);
+            b += np * (16 + 12 + 12 + (l > 0 ? 4 : 0)) + 64 * (w + h);
+            if (w * h <= 64)
+                break;
+            w = (w + 1) / 2;
+            h = (h + 1) / 2;
+        }
+        const size_t coarse = std::max<size_t>(w * h, 2) - 1;
+        b += coarse * coarse * 12 + (64 << 10);
+        if (features & FS_FEATURE_TRANSPORT)   // advect_reference buffers
+            b += 8 * (N + 2 * (nl + ng) * N + 3 * NU + 3 * NV + 2 * std::max(NU, NV) + N) + N + 8;
+        if (features & FS_FEATURE_VISCOSITY) {   // ViscositySolver + blended material
+            size_t vw = W, vh = H;
+            for (;;) {
+                const size_t ns = (vw + 1) * (vh + 1);
+                b += ns * (16 + 4 * 8) + 8 * vw * vh;
+                if (vw * vh <= 64)
+                    break;
+                vw = (vw + 1) / 2;
+                vh = (vh + 1) / 2;
+            }
+            b += 6 * 8 * (W + 1) * (H + 1) + 2 * 16 * N + 32 * blocks + 4096;
+        }
+        if (features & FS_FEATURE_SURFACE_TENSION)
+            b += 4 * 16 * N;
+        *bytes = b;
     });
 }
 

@@ -14,11 +14,13 @@
 // dominates M) and dot products are FP64.  A solve that needs no iteration leaves u, v untouched.  Kernels read a device scalar block and
 // exit at once when the solve has stopped, so a batch of four iterations is launched without host
 // round trips; the host reads the block once per batch, as the reference does.
+#include "fs_internal.hpp"
 #include "fs_kernels.hpp"
 
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <vector>
 
@@ -606,8 +608,10 @@ template <class T> struct VBuf {
     ~VBuf() { release(); }
     void release()
     {
-        if (p)
+        if (p) {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
             (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -617,6 +621,7 @@ template <class T> struct VBuf {
         n = count;
         if (!count)
             return 0;
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
         vhip(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
         vhip(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
         return count * sizeof(T);
@@ -668,15 +673,20 @@ struct ViscositySolver {
         bytes += material.alloc(2 * (size_t)W * H);
         bytes += partials.alloc(4 * (size_t)std::max(slot_blocks(W, H), 1u));
         bytes += S.alloc(1);
-        vhip(hipHostMalloc((void**)&pinned, sizeof(VisScalars), hipHostMallocDefault), "hipHostMalloc");
+        {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            vhip(hipHostMalloc((void**)&pinned, sizeof(VisScalars), hipHostMallocDefault), "hipHostMalloc");
+        }
     }
 
     ~ViscositySolver()
     {
         for (Level* L : levels)
             delete L;
-        if (pinned)
+        if (pinned) {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
             (void)hipHostFree(pinned);
+        }
     }
 
     VLev view(int l)

@@ -179,3 +179,56 @@ def load_into(grid, wl: Workload):
     grid.upload(FluidGridState(wl.fraction, wl.u, wl.v, np.zeros(0),
                                [wl.liquid_mass[k * N:(k + 1) * N] for k in range(nl)],
                                [wl.gas_mass[k * N:(k + 1) * N] for k in range(ng)]))
+
+
+def pool_with_drop(W: int, H: int, h: float, fill: float, drop_radius: float, rng: SplitMix64):
+    """Liquid pool over the bottom `fill` of the box plus one circular drop above it (cell-centre
+    sampled fractions); returns the fraction field."""
+    ys = (np.arange(H)[:, None] + 0.5) / H
+    xs = (np.arange(W)[None, :] + 0.5) * h
+    pool = (ys >= 1.0 - fill).astype(np.float64)
+    cx = rng.uniform(1, 0.3, 0.7)[0] * W * h
+    cy = (1.0 - fill) * 0.5 * H * h
+    r = drop_radius * W * h
+    drop = (np.hypot(xs - cx, ys * H * h - cy) < r).astype(np.float64)
+    return np.maximum(pool, drop).ravel()
+
+
+def serving_stream(seed: int = 5, count: int = 24, min_size: int = 128, max_size: int = 2048,
+                   rate: float = 4.0, steps: int = 50):
+    """Poisson request stream: `count` simulations arriving at `rate` per second, each side
+    log-uniform in [min_size, max_size] (independent aspect), cell size 1/max(W, H), water 100 / air
+    0.1, liquid filling 30-90% plus a drop, random vortex velocity, gravity 9.81, kinematic viscosity
+    log-uniform in [1e-3, 1] (mu = nu rho per phase), surface tension U[0.01, 0.07], `steps` steps of
+    dt = half the capillary step limit (one physical substep per step), each projection to a maximum
+    divergence of 1e-4 (about 1e-6 of the initial divergence of these flows).  Returns [(arrival_s, kwargs)]
+    where kwargs build a `fluidsim_amd.serving.Request`."""
+    rng = SplitMix64(seed)
+    out = []
+    t = 0.0
+    lo, hi = math.log(min_size), math.log(max_size)
+    for k in range(count):
+        t += -math.log1p(-rng.uniform(1)[0]) / rate
+        W = int(round(math.exp(rng.uniform(1, lo, hi)[0])))
+        H = int(round(math.exp(rng.uniform(1, lo, hi)[0])))
+        h = 1.0 / max(W, H)
+        fill = rng.uniform(1, 0.3, 0.9)[0]
+        fraction = pool_with_drop(W, H, h, fill, 0.08, rng)
+        nu = math.exp(rng.uniform(1, math.log(1e-3), math.log(1.0))[0])
+        sigma = rng.uniform(1, 0.01, 0.07)[0]
+        u = np.zeros((W + 1) * H)
+        v = np.zeros(W * (H + 1))
+        nv = 8
+        cxs, cys = rng.uniform(nv, 0.0, W * h), rng.uniform(nv, 0.0, H * h)
+        radii, strengths = rng.uniform(nv, 0.02, 0.08), rng.normal(nv) * 0.05
+        for j in range(nv):
+            add_gaussian(u, v, W, H, h, cxs[j], cys[j], radii[j], strengths[j])
+        zero_walls(u, v, W, H)
+        lm, gm = two_phase_masses(fraction, h)
+        limit = 0.5 * math.sqrt((100.0 + 0.1) * h ** 3 / (2.0 * math.pi * sigma))
+        out.append((t, dict(width=W, height=H, cell_size=h, steps=steps, dt=0.5 * limit, max_iterations=-1,
+                            divergence_tolerance=1e-4,
+                            gravity_y=9.81, surface_tension=sigma, liquid_density=[100.0], gas_density=[0.1],
+                            liquid_viscosity=[nu * 100.0], gas_viscosity=[nu * 0.1], liquid_fraction=fraction,
+                            u=u, v=v, liquid_mass=lm, gas_mass=gm)))
+    return out

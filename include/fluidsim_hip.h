@@ -223,6 +223,17 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
 /* Cumulative device bytes allocated by this grid. */
 fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);
 
+/* ---- device footprint (no reference counterpart) ----
+ * Upper bound of the device bytes a grid of this shape allocates (fs_grid_device_bytes reads the actual
+ * sum once it exists): the always-present state, pressure solver and hierarchy, plus the stages named
+ * in `features` (allocated on their first use). */
+enum {
+    FS_FEATURE_TRANSPORT = 1,          /* reference PLIC/VOF transport */
+    FS_FEATURE_VISCOSITY = 2,          /* implicit viscosity solve */
+    FS_FEATURE_SURFACE_TENSION = 4     /* capillary geometry */
+};
+fs_status fs_grid_estimate_bytes(int width, int height, int n_liquid, int n_gas, int features, size_t* bytes);
+
 /* ---- configuration getters (FluidGridBase.h:43-46, 70-80; FluidGridCPU.h:39-44) ----
  * getSurfaceTension, pressure_preconditioner, the transport mode, and the configured mixture
  * (FluidMixture components: density and dynamic viscosity), so that an adapter can refresh its own
@@ -292,6 +303,95 @@ typedef struct fs_frame_statistics {
  * pressure budget (-1: to tolerance, at most 100000 iterations) and the frame stops at the first step
  * whose projection is not accepted.  `statistics` (may be NULL) is filled also when a step fails. */
 fs_status fs_advance_frame(fs_grid* grid, const fs_frame_settings* settings, fs_frame_statistics* statistics);
+
+/* ---- serving loop (no reference counterpart: the reference steps one scene per process,
+ * FluidScene.inl:328-410; several scenes at most rotate on one thread) ----
+ * A server keeps many simulations resident on one GPU and steps them concurrently.  `lanes` worker
+ * threads each own a HIP stream; an admitted request is placed on the lane with the fewest resident
+ * instances and stepped there in rotation with the lane's other instances (continuous batching: a
+ * request joins its lane's rotation at the next step boundary, and leaves it when done).  Requests are
+ * admitted in arrival order while the sum of the admitted instances' fs_grid_estimate_bytes stays within
+ * the HBM budget; finishing an instance frees its device memory and admits the next waiting request.
+ * Each step is fs_step(dt, max_iterations, divergence_tolerance) on the instance's own grid, so a
+ * request's results do not depend on what else is resident.  A step that fails, or whose projection is
+ * not accepted, ends that request as failed (the scene's simulation_failed rule) without affecting others.
+ * Step latency is the wall time of one step's execution on its lane. */
+typedef struct fs_server fs_server;
+
+typedef struct fs_server_config {
+    int device;
+    int lanes;                    /* concurrent worker streams, 1..16 */
+    size_t hbm_budget_bytes;      /* admission budget; 0 = 80% of the device memory free at creation */
+} fs_server_config;
+
+typedef struct fs_request {
+    int width, height;
+    double cell_size;
+    int steps;                    /* resident timesteps to run (>= 0) */
+    double dt;
+    int max_iterations;           /* pressure budget per projection, -1 = to tolerance */
+    double divergence_tolerance;
+    double gravity_y;
+    double surface_tension;
+    int transport;                /* fs_transport */
+    int preconditioner;           /* fs_preconditioner */
+    int n_liquid, n_gas;
+    const double* liquid_density;   /* n_liquid */
+    const double* liquid_viscosity; /* n_liquid, may be NULL */
+    const double* gas_density;      /* n_gas */
+    const double* gas_viscosity;    /* n_gas, may be NULL */
+    /* initial state in fs_upload's layout; NULL means zeros.  Copied by fs_server_submit. */
+    const double* liquid_fraction;
+    const double* u;
+    const double* v;
+    const double* liquid_mass;
+    const double* gas_mass;
+} fs_request;
+
+enum { FS_REQUEST_QUEUED = 0, FS_REQUEST_RUNNING = 1, FS_REQUEST_DONE = 2, FS_REQUEST_FAILED = 3 };
+
+typedef struct fs_request_report {
+    int state;                    /* FS_REQUEST_* */
+    int lane;                     /* -1 until admitted */
+    int steps_done;
+    int pressure_iterations;      /* summed over the steps */
+    int viscosity_iterations;
+    int physical_steps;           /* capillary substeps included */
+    double max_divergence;        /* of the last step */
+    double queue_ms;              /* submission -> admission */
+    double run_ms;                /* admission -> end */
+    double step_p50_ms, step_p99_ms, step_max_ms;
+    size_t estimated_bytes;       /* admission footprint */
+    size_t device_bytes;          /* allocated by the instance at its end */
+    char error[256];
+} fs_request_report;
+
+typedef struct fs_server_stats {
+    long long submitted, admitted, completed, failed, steps;
+    int queued, resident;
+    size_t budget_bytes, resident_bytes, peak_resident_bytes;
+    double elapsed_s;             /* since creation or the last fs_server_reset_stats */
+    double steps_per_s;           /* steps completed in that window / elapsed_s */
+    double step_p50_ms, step_p99_ms;   /* over the steps of that window */
+} fs_server_stats;
+
+fs_status fs_server_create(const fs_server_config* config, fs_server** out);
+/* Stops the lanes after their current step; unfinished requests end as failed. */
+fs_status fs_server_destroy(fs_server* server);
+/* Validates and copies a request; FS_E_INVALID_ARGUMENT if malformed or larger than the whole budget. */
+fs_status fs_server_submit(fs_server* server, const fs_request* request, long long* id);
+/* Waits up to timeout_s seconds (< 0: without limit) for the request to finish, then reports it. */
+fs_status fs_server_wait(fs_server* server, long long id, double timeout_s, fs_request_report* report);
+fs_status fs_server_query(fs_server* server, long long id, fs_request_report* report);
+/* Final state of a finished request (fs_download's layout; NULL skips a field); FS_E_LOGIC otherwise. */
+fs_status fs_server_result(fs_server* server, long long id, double* liquid_fraction, double* u, double* v,
+    double* pressure, double* liquid_mass, double* gas_mass);
+/* Forgets a finished request and its stored result. */
+fs_status fs_server_release(fs_server* server, long long id);
+/* Blocks until no request is queued or running. */
+fs_status fs_server_drain(fs_server* server);
+fs_status fs_server_get_stats(fs_server* server, fs_server_stats* stats);
+fs_status fs_server_reset_stats(fs_server* server);
 
 #ifdef __cplusplus
 }

@@ -55,3 +55,23 @@ def test_invalid_arguments_are_rejected_before_any_device_work():
     assert L.fs_project(None, 0.1, 10, 1e-6, None) == _abi.FS_E_INVALID_ARGUMENT
     with pytest.raises(ValueError):
         _abi.check(_abi.FS_E_INVALID_ARGUMENT, "probe")
+
+
+def test_server_and_estimate_validation_need_no_device():
+    import ctypes as C
+
+    from fluidsim_amd import _abi
+
+    L = _abi.lib()
+    cfg = _abi.FsServerConfig(0, 0, 1 << 30)
+    h = C.c_void_p()
+    assert L.fs_server_create(C.byref(cfg), C.byref(h)) == _abi.FS_E_INVALID_ARGUMENT
+    assert b"lanes" in L.fs_last_error()
+    b = C.c_size_t()
+    assert L.fs_grid_estimate_bytes(0, 8, 1, 1, 0, C.byref(b)) == _abi.FS_E_INVALID_ARGUMENT
+    sizes = []
+    for features in (0, 1, 2, 4, 7):
+        assert L.fs_grid_estimate_bytes(256, 128, 1, 1, features, C.byref(b)) == _abi.FS_OK
+        sizes.append(b.value)
+    assert sizes[0] < min(sizes[1:4]) and sizes[4] > max(sizes[1:4])
+    assert L.fs_grid_estimate_bytes(512, 256, 1, 1, 0, C.byref(b)) == _abi.FS_OK and b.value > 3 * sizes[0]
