@@ -88,6 +88,59 @@ def cpu_baseline(wl, tau, budget_s):
     }
 
 
+def serving_leg(count, rate, steps, lanes, seed=5, max_size=2048):
+    """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
+    [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each),
+    first served one request at a time on one stream (serial plain), then arriving at `rate` per second
+    into the serving loop with `lanes` concurrent streams and HBM-budget admission.  Reports aggregate
+    steps/s of both, the served per-step latency p50/p99 and the queueing delay."""
+    from fluidsim_amd import workloads as WL
+    from fluidsim_amd.serving import Request, Server
+
+    stream = [(t, Request(**kw)) for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps)]
+    total_steps = sum(r.steps for _, r in stream)
+    with Server(lanes=1) as srv:
+        t0 = time.perf_counter()
+        for _, r in stream:
+            rep = srv.wait(srv.submit(r))
+            if not rep.done:
+                raise RuntimeError(f"serial request failed: {rep.error}")
+        serial_s = time.perf_counter() - t0
+        serial = srv.stats()
+    with Server(lanes=lanes) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for t, r in stream:
+            delay = t0 + t - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(r))
+        srv.drain()
+        served_s = time.perf_counter() - t0
+        st = srv.stats()
+        reps = [srv.wait(i) for i in ids]
+    failed = [r.error for r in reps if not r.done]
+    if failed:
+        raise RuntimeError(f"served request failed: {failed[0]}")
+    queue = [r.queue_ms for r in reps]
+    return {
+        "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each "
+                    "(PLIC transport, viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to |div u| <= 1e-4), "
+                    f"Poisson arrivals at {rate}/s, seed {seed}",
+        "lanes": lanes,
+        "cells_mean": float(np.mean([r.width * r.height for _, r in stream])),
+        "steps": total_steps,
+        "serial_steps_per_s": total_steps / serial_s,
+        "served_steps_per_s": total_steps / served_s,
+        "speedup_vs_serial": serial_s / served_s,
+        "served_step_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
+        "serial_step_ms": {"p50": serial["step_p50_ms"], "p99": serial["step_p99_ms"]},
+        "queue_ms": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+        "peak_resident_bytes": st["peak_resident_bytes"],
+        "arrival_span_s": stream[-1][0],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +150,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--serving-requests", type=int, default=0, help="size of the C5 serving stream (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=4.0)
+    ap.add_argument("--serving-steps", type=int, default=50)
+    ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--serving-max-size", type=int, default=2048)
+    ap.add_argument("--serving-only", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -107,6 +166,11 @@ def main():
 
     from fluidsim_amd import FluidGridHIP
     from fluidsim_amd import workloads as WL
+
+    if args.serving_only:
+        print(json.dumps(serving_leg(args.serving_requests or 24, args.serving_rate, args.serving_steps,
+                                     args.serving_lanes, max_size=args.serving_max_size)), flush=True)
+        return
 
     wl = WL.vortex_box(args.size, args.size, seed=2)
     grid = FluidGridHIP(wl.width, wl.height, wl.cell_size)
@@ -188,6 +252,10 @@ def main():
     }
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+    if args.serving_requests > 0:
+        grid.destroy()
+        out["serving"] = serving_leg(args.serving_requests, args.serving_rate, args.serving_steps,
+                                     args.serving_lanes, max_size=args.serving_max_size)
     if rank == 0:
         print(json.dumps(out), flush=True)
     grid.destroy()

@@ -17,9 +17,17 @@ namespace fsim {
 // Solver status codes kept on the device.
 enum PcgStatus : int { kRunning = 0, kConverged = 1, kStopped = 2 };
 
-// The FP32 recurrence residual drifts from the true one; every kReplaceEvery iterations it is
-// replaced by the FP64 true residual (reliable updating, van der Vorst & Ye 2000) without a restart.
+// The FP32 recurrence residual drifts from the true one.  The FP64 true residual b - A p is checked
+// (after folding the FP32 correction into the FP64 pressure) when the recurrence reaches the tolerance,
+// when it has dropped kRefineDrop below the residual that started the current window (mixed-precision
+// iterative refinement: FP32 CG cannot resolve much more than that on 1000:1 density contrasts), and
+// every kReplaceEvery iterations.  A check that finds the true residual above the tolerance replaces r
+// with it; CG restarts (beta = 0) if the check was the tolerance one or the true residual exceeds the
+// recurrence by kDriftRatio, and otherwise continues on the refreshed residual (reliable updating,
+// van der Vorst & Ye 2000).
 constexpr int kReplaceEvery = 32;
+constexpr double kRefineDrop = 1e-3;
+constexpr double kDriftRatio = 2.0;
 
 // Device-resident PCG scalar block.  Kernels of one iteration communicate only through this block,
 // so the host never reads a scalar inside the loop (it polls `status` once per batch).
@@ -46,7 +54,9 @@ struct PcgScalars {
     int restart;        // next beta is zero (true residual replaced the recurrence)
     int restarts;
     int mg_failed;      // coarse factorisation failed (the hierarchy must be rebuilt next time)
-    int replace;        // periodic residual replacement: refresh r from FP64 without restarting
+    int replace;        // check not triggered by the tolerance: refresh r from FP64, restart only on drift
+    int pad;
+    double window_rmax; // true residual max-norm that started the current refinement window
 };
 
 // One multigrid level as the FP32 smoothing kernels see it.

@@ -850,6 +850,7 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
             if (op == kFinTrueRes0) {
                 S->rmax = maximum;
                 S->initial_rmax = maximum;
+                S->window_rmax = maximum;
                 S->r_mean = total / n;
                 if (maximum <= S->tol || S->max_iterations == 0)
                     S->status = kInitialOk;
@@ -858,10 +859,12 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
                     S->status = kConverged;
                 } else {
                     S->r_mean = total / n;
-                    if (!S->replace) {
+                    if (!S->replace || maximum > kDriftRatio * S->rmax) {
                         S->restart = 1;
                         S->restarts += 1;
                     }
+                    S->rmax = maximum;
+                    S->window_rmax = maximum;
                 }
                 S->verify = 0;
                 S->replace = 0;
@@ -902,7 +905,7 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
             S->r_mean = total / n;
             if (maximum <= S->tol) {
                 S->verify = 1;
-            } else if (S->iterations % kReplaceEvery == 0) {
+            } else if (maximum <= kRefineDrop * S->window_rmax || S->iterations % kReplaceEvery == 0) {
                 S->verify = 1;
                 S->replace = 1;
             }

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from fluidsim_amd import (FS_PRECOND_DIAGONAL, FS_TRANSPORT_NONE, FS_TRANSPORT_SEMI_LAGRANGIAN, FluidGridHIP,
-                          FluidGridState)
+                          FluidGridState, FluidMixture, MixtureComponent)
 from fluidsim_amd import workloads as WL
 
 pytestmark = pytest.mark.gpu
@@ -274,4 +274,31 @@ def test_gravity_and_vortex_edits_match_oracle(oracle):
     assert np.allclose(sh.u, so["u"], rtol=1e-14, atol=1e-15)
     assert np.allclose(sh.v, so["v"], rtol=1e-14, atol=1e-15)
     assert o.max_divergence() == pytest.approx(g.maxDivergence(), rel=1e-12)
+    g.destroy()
+
+
+@pytest.mark.parametrize("index", [3, 11, 2])
+def test_high_contrast_pool_with_drop_converges_like_the_oracle(oracle, index):
+    """A 1000:1 liquid pool plus a drop (the serving stream's requests 3, 11, 2: 314x132, 271x159,
+    343x254): the FP32 recurrence loses accuracy after a 1e3 reduction, so the FP64 refinement check must
+    restart CG from the true residual; the projection then converges like the FP64 oracle's."""
+    t, kw = WL.serving_stream(5, 16, 128, 2048, 4.0, 10)[index]
+    W, H, h, dt = kw["width"], kw["height"], kw["cell_size"], kw["dt"]
+    g = FluidGridHIP(W, H, h)
+    g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0)]), FluidMixture([MixtureComponent("air", 0.1)]))
+    g.setGravityY(9.81)
+    g.upload(FluidGridState(kw["liquid_fraction"], kw["u"], kw["v"], np.zeros(0), [kw["liquid_mass"]],
+                            [kw["gas_mass"]]))
+    o = oracle.OracleGrid(W, H, h)
+    o.configure([100.0], [0.1])
+    o.set_gravity(9.81)
+    o.upload(kw["liquid_fraction"], kw["u"], kw["v"], None, kw["liquid_mass"], kw["gas_mass"])
+    g.applyGravity(dt)
+    o.apply_gravity(dt)
+    rg = g.project(dt, 200, 1e-4)
+    ro = o.project(dt, 200, 1e-4)
+    assert ro["converged"] and rg.converged and rg.iterations <= 80, (rg, ro["iterations"])
+    sg, so = g.download(), o.download()
+    scale = max(np.abs(so["u"]).max(), np.abs(so["v"]).max())
+    assert np.abs(sg.u - so["u"]).max() < 1e-4 * scale and np.abs(sg.v - so["v"]).max() < 1e-4 * scale
     g.destroy()
