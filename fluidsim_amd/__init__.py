@@ -6,10 +6,11 @@ reference's FluidGridBase method set over that ABI.
 from ._abi import (FS_PRECOND_DIAGONAL, FS_PRECOND_MULTIGRID, FS_TRANSPORT_NONE, FS_TRANSPORT_REFERENCE,
                    FS_TRANSPORT_SEMI_LAGRANGIAN, FluidSimError, lib)
 from .grid import (Context, FluidCheckpoint, FluidGridHIP, FluidGridState, FluidMixture, FrameStatistics,
-                   MixtureComponent, ProjectionResult, StepTimings, default_context, viscosity_solve)
+                   MixtureComponent, ProjectionResult, StepTimings, batch_project, default_context,
+                   viscosity_solve)
 
 __all__ = [
     "FS_PRECOND_DIAGONAL", "FS_PRECOND_MULTIGRID", "FS_TRANSPORT_NONE", "FS_TRANSPORT_REFERENCE",
     "FS_TRANSPORT_SEMI_LAGRANGIAN", "FluidSimError", "lib", "Context", "FluidGridHIP", "FluidGridState",
-    "FluidMixture", "FluidCheckpoint", "FrameStatistics", "MixtureComponent", "ProjectionResult", "StepTimings", "default_context", "viscosity_solve",
+    "FluidMixture", "FluidCheckpoint", "FrameStatistics", "MixtureComponent", "ProjectionResult", "StepTimings", "batch_project", "default_context", "viscosity_solve",
 ]

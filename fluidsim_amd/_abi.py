@@ -133,6 +133,7 @@ _vp = C.c_void_p
 # name -> argtypes (every function returns fs_status except the two string getters)
 SIGNATURES = {
     "fs_context_create": [C.c_int, C.POINTER(_vp)],
+    "fs_context_create_ex": [C.c_int, C.c_int, C.POINTER(_vp)],
     "fs_context_destroy": [_vp],
     "fs_context_synchronize": [_vp],
     "fs_grid_create": [_vp, C.c_int, C.c_int, C.c_double, C.POINTER(_vp)],
@@ -186,6 +187,7 @@ SIGNATURES = {
     "fs_checkpoint_save": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_checkpoint_load": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_advance_frame": [_vp, C.POINTER(FsFrameSettings), C.POINTER(FsFrameStatistics)],
+    "fs_batch_project": [C.c_int, C.POINTER(_vp), C.c_double, C.c_int, _dp, C.POINTER(FsProjectionResult)],
     "fs_grid_estimate_bytes": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)],
     "fs_server_create": [C.POINTER(FsServerConfig), C.POINTER(_vp)],
     "fs_server_destroy": [_vp],

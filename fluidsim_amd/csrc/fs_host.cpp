@@ -193,6 +193,9 @@ struct fs_context {
     PcgScalars* pinned = nullptr;   // host-pinned mirror of a grid's scalar block
     double* pinned_value = nullptr; // host-pinned landing slot of single device reductions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf<BatchInstance> batch_desc;       // descriptors of the last batched solve
+    PcgScalars* batch_pinned = nullptr;     // host-pinned landing slots of a batch's scalar blocks
+    size_t batch_pinned_n = 0;
 };
 
 constexpr int kPressureBudget = 100000;   // PCG iterations of an unbounded (negative) budget
@@ -707,6 +710,80 @@ struct fs_grid {
         return *ctx->pinned;
     }
 
+    // FluidGridCPU::project (FluidGridCPU.cpp:1030-1124), first phase: scalar block, coefficients,
+    // hierarchy, rhs, predicted pressure, its residual and the first direction -- launches only.
+    void project_begin(double dt, int max_iterations, double tol)
+    {
+        const double scale = h * h / dt;
+        launch_init_scalars(st(), S.p, tol * scale, max_iterations);
+        if (!coeff_valid)
+            rebuild_coefficients();
+        GridDev g = dev();
+        // The hierarchy depends only on the face weights: rebuild it on the second stream while the main
+        // stream forms the rhs, the predicted pressure and its residual; join before the first V-cycle.
+        const bool rebuild = precond == FS_PRECOND_MULTIGRID && !mg_valid;
+        // A context without the helper stream rebuilds in line.
+        hipStream_t side = ctx->aux ? ctx->aux : st();
+        if (rebuild) {
+            if (ctx->aux) {
+                HIPC(hipEventRecord(ctx->fork, st()));
+                HIPC(hipStreamWaitEvent(side, ctx->fork, 0));
+            }
+            launch_init_scalars(side, S_aux.p, 0.0, 0);
+            rebuild_hierarchy(side, S_aux.p);
+            if (ctx->aux)
+                HIPC(hipEventRecord(ctx->join, side));
+        }
+        launch_rhs(st(), g, scale);
+        launch_rhs_center(st(), g);
+        launch_zero_pressure(st(), g);
+        launch_predict(st(), g);
+        launch_sub_mean_p(st(), g, 0);
+        launch_true_residual(st(), g, 0);
+        if (rebuild) {
+            if (ctx->aux)
+                HIPC(hipStreamWaitEvent(st(), ctx->join, 0));
+            launch_merge_failure(st(), S_aux.p, S.p);
+        }
+        precondition(status_ptr());
+        launch_pcg_rz(st(), g, z_ptr());
+        launch_pcg_direction(st(), g, z_ptr());
+    }
+
+    // Last phase: fold the correction into the pressure, remove its mean, correct the velocity and
+    // measure the divergence -- launches only.
+    void project_end(double dt)
+    {
+        GridDev g = dev();
+        launch_final_flush_sum(st(), g);
+        launch_sub_mean_p(st(), g, 1);
+        launch_correct_velocity(st(), g, dt);
+        launch_max_divergence(st(), g);
+    }
+
+    fs_projection_result project_result(const PcgScalars& s, double tol)
+    {
+        fs_projection_result res{};
+        res.max_divergence = std::numeric_limits<double>::infinity();
+        res.physical_steps = 1;
+        if (s.mg_failed)
+            mg_valid = false;
+        if (s.status == kFailed)
+            return res;
+        res.iterations = s.iterations;
+        res.max_divergence = s.div_max;
+        res.converged = s.div_max <= tol;
+        res.restarts = s.restarts;
+        res.initial_residual = s.initial_rmax;
+        res.final_residual = s.true_rmax;
+        res.rhs_max = s.bmax;
+        return res;
+    }
+
+    // A negative budget runs to tolerance, the uniform WebGPU contract used by its scenes
+    // (FluidGridGPU.cpp:1326, FluidScene.inl:355-359), bounded so that a stagnating solve ends.
+    static int budget(int max_iterations) { return max_iterations < 0 ? kPressureBudget : max_iterations; }
+
     // FluidGridCPU::project (FluidGridCPU.cpp:1030-1124)
     fs_projection_result project(double dt, int max_iterations, double tol)
     {
@@ -717,40 +794,8 @@ struct fs_grid {
             fail(FS_E_LOGIC, "Upload the grid state before stepping or projecting");
         if (dt <= 0.0)
             return res;
-        // A negative budget runs to tolerance, the uniform WebGPU contract used by its scenes
-        // (FluidGridGPU.cpp:1326, FluidScene.inl:355-359), bounded so that a stagnating solve ends.
-        if (max_iterations < 0)
-            max_iterations = kPressureBudget;
-        const double scale = h * h / dt;
-        GridDev g = dev();
         ensure_graph();
-        launch_init_scalars(st(), S.p, tol * scale, max_iterations);
-        if (!coeff_valid)
-            rebuild_coefficients();
-        g = dev();
-        // The hierarchy depends only on the face weights: rebuild it on the second stream while the main
-        // stream forms the rhs, the predicted pressure and its residual; join before the first V-cycle.
-        const bool rebuild = precond == FS_PRECOND_MULTIGRID && !mg_valid;
-        if (rebuild) {
-            HIPC(hipEventRecord(ctx->fork, st()));
-            HIPC(hipStreamWaitEvent(ctx->aux, ctx->fork, 0));
-            launch_init_scalars(ctx->aux, S_aux.p, 0.0, 0);
-            rebuild_hierarchy(ctx->aux, S_aux.p);
-            HIPC(hipEventRecord(ctx->join, ctx->aux));
-        }
-        launch_rhs(st(), g, scale);
-        launch_rhs_center(st(), g);
-        launch_zero_pressure(st(), g);
-        launch_predict(st(), g);
-        launch_sub_mean_p(st(), g, 0);
-        launch_true_residual(st(), g, 0);
-        if (rebuild) {
-            HIPC(hipStreamWaitEvent(st(), ctx->join, 0));
-            launch_merge_failure(st(), S_aux.p, S.p);
-        }
-        precondition(status_ptr());
-        launch_pcg_rz(st(), g, z_ptr());
-        launch_pcg_direction(st(), g, z_ptr());
+        project_begin(dt, budget(max_iterations), tol);
         PcgScalars s = read_scalars();
         if (s.status == kRunning) {
             int batch = std::max(1, std::min(predicted, 256));
@@ -764,23 +809,31 @@ struct fs_grid {
             }
             predicted = std::max(1, s.iterations);
         }
-        launch_final_flush_sum(st(), g);
-        launch_sub_mean_p(st(), g, 1);
-        launch_correct_velocity(st(), g, dt);
-        launch_max_divergence(st(), g);
-        s = read_scalars();
-        if (s.mg_failed)
-            mg_valid = false;
-        if (s.status == kFailed)
-            return res;
-        res.iterations = s.iterations;
-        res.max_divergence = s.div_max;
-        res.converged = s.div_max <= tol;
-        res.restarts = s.restarts;
-        res.initial_residual = s.initial_rmax;
-        res.final_residual = s.true_rmax;
-        res.rhs_max = s.bmax;
-        return res;
+        project_end(dt);
+        return project_result(read_scalars(), tol);
+    }
+
+    // This grid's entry of a batched solve (the iteration body of iteration_kernels()).
+    BatchInstance batch_instance()
+    {
+        if (tail_start > kMaxBatchLevels)
+            fail(FS_E_UNSUPPORTED, "grid has more multigrid levels than a batched solve supports");
+        BatchInstance b{};
+        b.g = dev();
+        b.multigrid = precond == FS_PRECOND_MULTIGRID;
+        b.levels = tail_start;
+        b.tail = tail_dev.p;
+        for (int l = 0; l < tail_start; ++l) {
+            const Level& F = levels[l];
+            const Level& Cl = levels[l + 1];
+            BatchLevel& B = b.lev[l];
+            B.L = view(l);
+            B.C = view(l + 1);
+            B.rx = AxisRestrict{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
+            B.ry = AxisRestrict{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+            B.link = CoarseLink{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
+        }
+        return b;
     }
 
     double max_divergence()
@@ -1028,9 +1081,12 @@ const char* fs_version(void) { return "fluidsim_hip 0.1 (gfx950)"; }
 
 const char* fs_last_error(void) { return g_last_error.c_str(); }
 
-fs_status fs_context_create(int device, fs_context** out)
+fs_status fs_context_create(int device, fs_context** out) { return fs_context_create_ex(device, 0, out); }
+
+fs_status fs_context_create_ex(int device, int flags, fs_context** out)
 {
     return guard([&] {
+        need((flags & ~FS_CONTEXT_SINGLE_STREAM) == 0, FS_E_INVALID_ARGUMENT, "fs_context_create_ex: unknown flags");
         need(out != nullptr, FS_E_INVALID_ARGUMENT, "fs_context_create: null output pointer");
         int count = 0;
         HIPC(hipGetDeviceCount(&count));
@@ -1043,9 +1099,11 @@ fs_status fs_context_create(int device, fs_context** out)
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
             ctx->lds_per_block = (size_t)lds;
         HIPC(hipStreamCreate(&ctx->stream));
-        HIPC(hipStreamCreate(&ctx->aux));
-        HIPC(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
-        HIPC(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+        if (!(flags & FS_CONTEXT_SINGLE_STREAM)) {
+            HIPC(hipStreamCreate(&ctx->aux));
+            HIPC(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+        }
         HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
         HIPC(hipHostMalloc((void**)&ctx->pinned_value, sizeof(double), hipHostMallocDefault));
         HIPC(hipEventCreate(&ctx->ev0));
@@ -1071,6 +1129,8 @@ fs_status fs_context_destroy(fs_context* ctx)
             (void)hipHostFree(ctx->pinned);
         if (ctx->pinned_value)
             (void)hipHostFree(ctx->pinned_value);
+        if (ctx->batch_pinned)
+            (void)hipHostFree(ctx->batch_pinned);
         if (ctx->stream)
             (void)hipStreamDestroy(ctx->stream);
         if (ctx->aux)
@@ -2081,6 +2141,70 @@ fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes)
     return guard([&] {
         need(grid != nullptr && bytes != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
         *bytes = grid->bytes;
+    });
+}
+
+fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_iterations,
+    const double* divergence_tolerances, fs_projection_result* results)
+{
+    return guard([&] {
+        need(count >= 0 && (count == 0 || (grids && divergence_tolerances && results)), FS_E_INVALID_ARGUMENT,
+            "null pointer");
+        if (count == 0)
+            return;
+        fs_context* ctx = grids[0] ? grids[0]->ctx : nullptr;
+        for (int i = 0; i < count; ++i) {
+            need(grids[i] != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+            need(grids[i]->ctx == ctx, FS_E_INVALID_ARGUMENT, "batched grids must share one context");
+            need(grids[i]->uploaded, FS_E_LOGIC, "Upload the grid state before stepping or projecting");
+            for (int j = 0; j < i; ++j)
+                need(grids[j] != grids[i], FS_E_INVALID_ARGUMENT, "a grid appears twice in the batch");
+        }
+        for (int i = 0; i < count; ++i) {
+            results[i] = fs_projection_result{};
+            results[i].max_divergence = std::numeric_limits<double>::infinity();
+            results[i].physical_steps = 1;
+        }
+        if (dt <= 0.0)
+            return;
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        std::vector<BatchInstance> desc((size_t)count);
+        size_t lds = 0;
+        for (int i = 0; i < count; ++i) {
+            fs_grid& g = *grids[i];
+            g.project_begin(dt, fs_grid::budget(max_iterations), divergence_tolerances[i]);
+            desc[(size_t)i] = g.batch_instance();
+            if (g.precond == FS_PRECOND_MULTIGRID)
+                lds = std::max(lds, g.tail_lds);
+        }
+        if (ctx->batch_desc.n < (size_t)count)
+            ctx->batch_desc.alloc((size_t)count);
+        HIPC(hipMemcpyAsync(ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance),
+            hipMemcpyHostToDevice, st));
+        if (lds > 65536)
+            need(batch_allow_lds(lds), FS_E_DEVICE, "the device refused the batched solve's LDS request");
+        launch_batch_pcg(st, ctx->batch_desc.p, count, lds);
+        HIPC(hipGetLastError());
+        for (int i = 0; i < count; ++i)
+            grids[i]->project_end(dt);
+        if (ctx->batch_pinned_n < (size_t)count) {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            if (ctx->batch_pinned)
+                (void)hipHostFree(ctx->batch_pinned);
+            ctx->batch_pinned = nullptr;
+            ctx->batch_pinned_n = 0;
+            HIPC(hipHostMalloc((void**)&ctx->batch_pinned, (size_t)count * sizeof(PcgScalars), hipHostMallocDefault));
+            ctx->batch_pinned_n = (size_t)count;
+        }
+        for (int i = 0; i < count; ++i)
+            HIPC(hipMemcpyAsync(ctx->batch_pinned + i, grids[i]->S.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        for (int i = 0; i < count; ++i) {
+            const PcgScalars sc = ctx->batch_pinned[i];
+            results[i] = grids[i]->project_result(sc, divergence_tolerances[i]);
+            grids[i]->predicted = std::max(1, sc.iterations);
+        }
     });
 }
 

@@ -136,6 +136,26 @@ void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const floa
     const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status, int variant);
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status);
 
+// ---- batched PCG: one 1024-thread workgroup runs the whole iteration loop of one instance ----
+constexpr int kMaxBatchLevels = 16;   // tiled levels above an instance's tail
+struct BatchLevel {
+    LevelView L;        // tiled level l (rhs: r on level 0)
+    LevelView C;        // level l + 1 (its rhs receives the restriction, its sol[0] the coarse correction)
+    AxisRestrict rx, ry;  // restriction l -> l + 1
+    CoarseLink link;    // prolongation l + 1 -> l
+};
+struct BatchInstance {
+    GridDev g;
+    int multigrid;      // 0: diagonal preconditioner
+    int levels;         // tiled levels (the tail starts at this level)
+    const TailDesc* tail;   // device copy of the instance's tail descriptor
+    BatchLevel lev[kMaxBatchLevels];
+};
+// Runs every instance's PCG iterations (the captured iteration body, looped on the device until the
+// instance's scalar block leaves kRunning) in one launch of `count` workgroups.
+void launch_batch_pcg(hipStream_t st, const BatchInstance* batch, int count, size_t lds_bytes);
+bool batch_allow_lds(size_t bytes);
+
 // ---- reference PLIC/VOF transport (FP64, reference layouts, bit-exact) ----
 // One directional pass reads the frozen flow (u, v), the fractions and masses of the pass start and the
 // transported velocities nu_in/nv_in; it writes new fractions/masses/density into the *_out buffers, the

@@ -848,9 +848,10 @@ __device__ void tail_coarse_solve(const TailLevel& T, const TailArrays& A, const
     }
 }
 
-__global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, const double* shift, const int* status)
+// The tail V-cycle of one instance on a 1024-thread workgroup: stage every tail level, its transfer
+// tables and the coarse inverse in `lds`, cycle, and write the first tail level's solution back.
+__device__ void tail_run(const TailDesc& D, const double* shift, const int* status, float* lds)
 {
-    extern __shared__ float lds[];
     const int nl = D.levels;
     // Flat index over every level's padded arrays; all loads issue before the LDS stores.
     constexpr int TL = 8;
@@ -951,7 +952,316 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, cons
     }
 }
 
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, const double* shift, const int* status)
+{
+    extern __shared__ float lds[];
+    tail_run(D, shift, status, lds);
+}
+
 constexpr int kMultiNS = 4;
+
+// ---------------------------------------------------------------------------------------------
+// Batched PCG: one 1024-thread workgroup per instance runs the instance's whole iteration loop (the
+// kernels of one captured PCG iteration become phases separated by workgroup barriers), so a batch of
+// small grids is one launch instead of a dozen launches per iteration per grid.  Vectors stay in the
+// instance's own HBM buffers; the scalar block is kept in LDS and written back at the end.  Every value
+// is computed with the expression of the corresponding single-instance kernel; only the FP64
+// reductions run in another order (thread-strided sums, then the 16 wavefronts in order).
+// ---------------------------------------------------------------------------------------------
+constexpr int kBT = kTailThreads;
+constexpr int kBW = kBT / 64;
+
+// Workgroup sum / max, valid in every thread.
+__device__ __forceinline__ double wg_sum(double v, double* slot)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0)
+        slot[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBW; ++i)
+        r += slot[i];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double wg_max(double v, double* slot)
+{
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0)
+        slot[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBW; ++i)
+        r = fmax(r, slot[i]);
+    __syncthreads();
+    return r;
+}
+
+// Eight red-black half-sweeps of one level in place on `sol` (k_rb_multi's update): colours from
+// `first`, alternating; the first half-sweep of a pre-smoothing ignores the (zero) neighbours.
+__device__ void wg_sweeps(const LevelView& L, float* sol, const float* rhs, const double* shift, int first,
+    bool from_zero)
+{
+    const int w = L.w, h = L.h, s = L.s;
+    const int pairs = (w + 1) >> 1;
+    const int items = pairs * h;
+    const double sh = shift ? *shift : 0.0;
+    for (int k = 0; k < 2 * kMultiNS; ++k) {
+        const int color = first ^ (k & 1);
+        const bool neighbours = !(from_zero && k == 0);
+        for (int t = threadIdx.x; t < items; t += kBT) {
+            const int y = t / pairs;
+            const int x = 2 * (t - y * pairs) + ((color + y) & 1);
+            if (x >= w)
+                continue;
+            const int i = (y + 1) * s + x + 1;
+            const float a = L.right[i - 1], b = L.right[i], u = L.down[i - s], d = L.down[i];
+            float v = shift ? (float)((double)rhs[i] - sh) : rhs[i];
+            if (neighbours)
+                v += (a * sol[i - 1] + b * sol[i + 1]) + (u * sol[i - s] + d * sol[i + s]);
+            sol[i] = v * inv_or_zero((a + b) + (u + d));
+        }
+        __syncthreads();
+    }
+}
+
+// Residual of level F's sol[1] into F.res, then its transpose-of-bilinear restriction into C.rhs
+// (k_res_restrict's arithmetic; missing taps have weight 0 and index -1, the zero halo).
+__device__ void wg_res_restrict(const LevelView& F, const float* rhs, const double* shift, const LevelView& C,
+    const AxisRestrict& rx, const AxisRestrict& ry)
+{
+    const int w = F.w, h = F.h, s = F.s;
+    const float* sol = F.sol[1];
+    const double sh = shift ? *shift : 0.0;
+    for (int e = threadIdx.x; e < w * h; e += kBT) {
+        const int y = e / w, x = e - y * w;
+        const int i = (y + 1) * s + x + 1;
+        const float c = sol[i];
+        const float product = (F.right[i - 1] * (c - sol[i - 1]) + F.right[i] * (c - sol[i + 1])) +
+                              (F.down[i - s] * (c - sol[i - s]) + F.down[i] * (c - sol[i + s]));
+        const float b = shift ? (float)((double)rhs[i] - sh) : rhs[i];
+        F.res[i] = b - product;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < C.w * C.h; e += kBT) {
+        const int y = e / C.w, x = e - y * C.w;
+        float value = 0.f;
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            const float* row = F.res + (ry.index[y * 4 + iy] + 1) * s + 1;
+            float horizontal = 0.f;
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+                horizontal += rx.weight[x * 4 + ix] * row[rx.index[x * 4 + ix]];
+            value += ry.weight[y * 4 + iy] * horizontal;
+        }
+        C.rhs[(y + 1) * C.s + x + 1] = value;
+    }
+    __syncthreads();
+}
+
+// sol[0] = sol[1] + bilinear prolongation of the coarse correction (k_rb_multi MODE 1's load).
+__device__ void wg_prolong(const LevelView& L, const CoarseLink& link)
+{
+    const int w = L.w, h = L.h, s = L.s;
+    for (int e = threadIdx.x; e < w * h; e += kBT) {
+        const int y = e / w, x = e - y * w;
+        const int i = (y + 1) * s + x + 1;
+        const float wx = link.pxw[x], wy = link.pyw[y];
+        const float* top = link.csol + (link.pyf[y] + 1) * link.cs + 1;
+        const float* bot = link.csol + (link.pys[y] + 1) * link.cs + 1;
+        const int a0 = link.pxf[x], a1 = link.pxs[x];
+        const float a = (1.f - wx) * top[a0] + wx * top[a1];
+        const float b = (1.f - wx) * bot[a0] + wx * bot[a1];
+        float v = L.sol[1][i];
+        v += (1.f - wy) * a + wy * b;
+        L.sol[0][i] = v;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
+{
+    extern __shared__ float lds[];
+    __shared__ PcgScalars S;
+    __shared__ double slot[3][kBW];
+    const BatchInstance& I = batch[blockIdx.x];
+    const GridDev& g = I.g;
+    const int W = g.W, H = g.H, s = g.s, N = W * H;
+    const double n = (double)N;
+    if (threadIdx.x == 0)
+        S = *g.S;
+    __syncthreads();
+    const float* z = I.multigrid ? (I.levels > 0 ? I.lev[0].L.sol[0] : I.tail->top_sol) : g.zdiag;
+    while (S.status == kRunning) {
+        // q = A d and d.q (k_pcg_ad), alpha (kFinAd)
+        double dot = 0.0;
+        for (int e = threadIdx.x; e < N; e += kBT) {
+            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+            const float c = g.d[i];
+            const float dl = g.d[i - 1], dr = g.d[i + 1], du = g.d[i - s], dd = g.d[i + s];
+            const float wl = g.right32[i - 1], wr = g.right32[i], wu = g.down32[i - s], wd = g.down32[i];
+            const float q = (wl * (c - dl) + wr * (c - dr)) + (wu * (c - du) + wd * (c - dd));
+            g.q[i] = q;
+            dot += (double)c * (double)q;
+        }
+        dot = wg_sum(dot, slot[0]);
+        if (threadIdx.x == 0) {
+            S.dq = dot;
+            if (dot <= 0.0 || !isfinite(dot)) {
+                S.status = kStopped;
+            } else {
+                const double alpha = S.rz / dot;
+                if (!isfinite(alpha))
+                    S.status = kStopped;
+                S.alpha = alpha;
+            }
+        }
+        __syncthreads();
+        if (S.status != kRunning)
+            break;
+        // x += alpha d, r -= alpha q (k_pcg_update); recurrence checks (kFinUpdate)
+        const float alpha = (float)S.alpha;
+        double sum = 0.0, m = 0.0;
+        for (int e = threadIdx.x; e < N; e += kBT) {
+            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+            g.x[i] = g.x[i] + alpha * g.d[i];
+            const float r = g.r[i] - alpha * g.q[i];
+            g.r[i] = r;
+            sum += (double)r;
+            m = fmax(m, (double)fabsf(r));
+        }
+        sum = wg_sum(sum, slot[0]);
+        m = wg_max(m, slot[1]);
+        if (threadIdx.x == 0) {
+            if (!isfinite(sum))
+                m = INFINITY;
+            S.iterations += 1;
+            S.rmax = m;
+            S.r_sum = sum;
+            S.r_mean = sum / n;
+            if (m <= S.tol) {
+                S.verify = 1;
+            } else if (m <= kRefineDrop * S.window_rmax || S.iterations % kReplaceEvery == 0) {
+                S.verify = 1;
+                S.replace = 1;
+            }
+        }
+        __syncthreads();
+        // flush and FP64 true residual (k_flush, k_true_residual mode 1, kFinTrueRes1)
+        if (S.verify) {
+            double tsum = 0.0, tmax = 0.0;
+            for (int e = threadIdx.x; e < N; e += kBT) {
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                g.p[i] = g.p[i] + (double)g.x[i];
+                g.x[i] = 0.0f;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < N; e += kBT) {
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const double c = g.p[i];
+                const double product = (g.right64[i - 1] * (c - g.p[i - 1]) + g.right64[i] * (c - g.p[i + 1])) +
+                                       (g.down64[i - s] * (c - g.p[i - s]) + g.down64[i] * (c - g.p[i + s]));
+                const double r = g.b[i] - product;
+                g.r[i] = (float)r;
+                tsum += r;
+                tmax = fmax(tmax, fabs(r));
+            }
+            tsum = wg_sum(tsum, slot[0]);
+            tmax = wg_max(tmax, slot[1]);
+            if (threadIdx.x == 0) {
+                if (!isfinite(tsum))
+                    tmax = INFINITY;
+                S.true_rmax = tmax;
+                S.r_sum = tsum;
+                if (tmax <= S.tol) {
+                    S.status = kConverged;
+                } else {
+                    S.r_mean = tsum / n;
+                    if (!S.replace || tmax > kDriftRatio * S.rmax) {
+                        S.restart = 1;
+                        S.restarts += 1;
+                    }
+                    S.rmax = tmax;
+                    S.window_rmax = tmax;
+                }
+                S.verify = 0;
+                S.replace = 0;
+            }
+            __syncthreads();
+            if (S.status != kRunning)
+                break;
+        }
+        // preconditioner: V(4,4) cycle of r - mean (vcycle) or the diagonal (k_precond_diag)
+        if (I.multigrid) {
+            for (int l = 0; l < I.levels; ++l) {
+                const BatchLevel& B = I.lev[l];
+                const double* shift = l == 0 ? &S.r_mean : nullptr;
+                wg_sweeps(B.L, B.L.sol[1], B.L.rhs, shift, 0, true);
+                wg_res_restrict(B.L, B.L.rhs, shift, B.C, B.rx, B.ry);
+            }
+            tail_run(*I.tail, I.levels == 0 ? &S.r_mean : nullptr, nullptr, lds);
+            __syncthreads();
+            for (int l = I.levels - 1; l >= 0; --l) {
+                const BatchLevel& B = I.lev[l];
+                wg_prolong(B.L, B.link);
+                wg_sweeps(B.L, B.L.sol[0], B.L.rhs, l == 0 ? &S.r_mean : nullptr, 1, false);
+            }
+        } else {
+            const double mean = S.r_mean;
+            for (int e = threadIdx.x; e < N; e += kBT) {
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                g.zdiag[i] = (float)(((double)g.r[i] - mean) * (double)g.invd32[i]);
+            }
+            __syncthreads();
+        }
+        // sum z, r.z, sum r (k_pcg_rz), beta (kFinRz)
+        double sz = 0.0, srz = 0.0, sr = 0.0;
+        for (int e = threadIdx.x; e < N; e += kBT) {
+            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+            const double zz = (double)z[i], rr = (double)g.r[i];
+            sz += zz;
+            srz += rr * zz;
+            sr += rr;
+        }
+        sz = wg_sum(sz, slot[0]);
+        srz = wg_sum(srz, slot[1]);
+        sr = wg_sum(sr, slot[2]);
+        if (threadIdx.x == 0) {
+            const double mean = sz / n;
+            const double next = srz - mean * sr;
+            S.z_mean = mean;
+            const double beta = S.restart ? 0.0 : next / S.rz;
+            S.beta = beta;
+            S.restart = 0;
+            if (!isfinite(beta)) {
+                S.status = kStopped;
+            } else {
+                S.rz = next;
+                if (!(S.iterations < S.max_iterations && next > 0.0 && isfinite(next)))
+                    S.status = kStopped;
+            }
+        }
+        __syncthreads();
+        if (S.status != kRunning)
+            break;
+        // d = (z - mean z) + beta d (k_pcg_direction)
+        const float zmean = (float)S.z_mean;
+        const double beta64 = S.beta;
+        const float beta = (float)beta64;
+        for (int e = threadIdx.x; e < N; e += kBT) {
+            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+            const float zz = z[i] - zmean;
+            g.d[i] = beta64 == 0.0 ? zz : zz + beta * g.d[i];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *g.S = S;
+}
 
 }  // namespace
 
@@ -1060,6 +1370,18 @@ void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, 
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status)
 {
     hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), lds_bytes, st, desc, shift, status);
+}
+
+bool batch_allow_lds(size_t bytes)
+{
+    return hipFuncSetAttribute((const void*)k_batch_pcg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+           hipSuccess;
+}
+
+void launch_batch_pcg(hipStream_t st, const BatchInstance* batch, int count, size_t lds_bytes)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_batch_pcg, dim3(count), dim3(kBT), lds_bytes, st, batch);
 }
 
 }  // namespace fsim

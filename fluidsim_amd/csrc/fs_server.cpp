@@ -1,7 +1,7 @@
 // fs_server.cpp -- the serving loop: many resident simulations stepped concurrently on one GPU.
 //
-// Built only on the grid ABI (include/fluidsim_hip.h).  `lanes` worker threads each own an fs_context
-// (one HIP stream plus its helper stream and pinned read-back words); a lane steps its resident
+// Built only on the grid ABI (include/fluidsim_hip.h).  `lanes` worker threads each own a single-stream
+// fs_context (one HIP stream -- one hardware queue -- and pinned read-back words); a lane steps its resident
 // instances in rotation, one fs_step each per round, so instances on different lanes overlap on the
 // device while each instance's kernels stay ordered on its lane's stream.  Admission is first come,
 // first served against an HBM budget counted with fs_grid_estimate_bytes; an instance's grid is created
@@ -326,7 +326,7 @@ fs_status fs_server_create(const fs_server_config* config, fs_server** out)
         for (int i = 0; i < config->lanes; ++i) {
             auto lane = std::make_unique<Lane>();
             lane->index = i;
-            if (fs_context_create(config->device, &lane->ctx) != FS_OK) {
+            if (fs_context_create_ex(config->device, FS_CONTEXT_SINGLE_STREAM, &lane->ctx) != FS_OK) {
                 const std::string message = fs_last_error();
                 for (auto& l : s->lanes)
                     fs_context_destroy(l->ctx);

@@ -178,6 +178,19 @@ def viscosity_solve(width: int, height: int, cell_size: float, material, u, v, d
     return it.value, rel.value, uu, vv
 
 
+def batch_project(grids, dt: float, max_iterations: int = 200, divergence_tolerance=1e-6):
+    """fs_batch_project: project every grid (all on one context) with one persistent launch running all
+    their PCG iterations.  `divergence_tolerance` is one value or one per grid.  Returns one
+    ProjectionResult per grid, with fs_project's contract."""
+    grids = list(grids)
+    n = len(grids)
+    tol = np.broadcast_to(np.asarray(divergence_tolerance, dtype=np.float64), (n,)).copy()
+    handles = (C.c_void_p * max(n, 1))(*[g.handle.value for g in grids])
+    res = (FsProjectionResult * max(n, 1))()
+    call("fs_batch_project", n, handles, float(dt), int(max_iterations), _dp(tol), res)
+    return [ProjectionResult.from_c(res[i]) for i in range(n)]
+
+
 def _finite_json(value) -> bool:
     """True if every number in a JSON-like value is finite (JSON has no NaN or infinity)."""
     if isinstance(value, float):

@@ -89,8 +89,14 @@ const char* fs_version(void);
 /* Message of the last failed call on this thread ("" if none).  No reference counterpart: the C
  * ABI cannot carry C++ exceptions, so it keeps their what() text here. */
 const char* fs_last_error(void);
-/* Opens HIP device `device` and one stream. */
+/* Opens HIP device `device` with its work stream and a helper stream (the multigrid hierarchy is rebuilt
+ * on the helper while the work stream forms the right-hand side). */
 fs_status fs_context_create(int device, fs_context** out);
+/* As fs_context_create; FS_CONTEXT_SINGLE_STREAM creates the work stream only (the rebuild then runs in
+ * line), so that N such contexts occupy N of the process's hardware queues (GPU_MAX_HW_QUEUES) -- the
+ * serving loop's lanes use it.  No reference counterpart. */
+enum { FS_CONTEXT_SINGLE_STREAM = 1 };
+fs_status fs_context_create_ex(int device, int flags, fs_context** out);
 fs_status fs_context_destroy(fs_context* ctx);
 fs_status fs_context_synchronize(fs_context* ctx);
 
@@ -222,6 +228,18 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
 fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_launch);
 /* Cumulative device bytes allocated by this grid. */
 fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);
+
+/* ---- batched projection (no reference counterpart: the reference projects one grid per call,
+ * FluidGridCPU.cpp:1030-1124) ----
+ * fs_project on `count` grids of one context at once: every grid's setup (coefficients, hierarchy, rhs,
+ * predicted pressure, first direction) is launched on the context's stream, then ONE persistent launch
+ * runs all grids' PCG iterations -- one 1024-thread workgroup per grid, looping on the device until that
+ * grid converges or stops -- and then every grid's velocity correction.  Each grid's result follows
+ * fs_project's contract (same stopping rule, same result fields) and does not depend on the other
+ * grids; the FP64 reductions run in another order than fs_project's, so values agree with it to
+ * rounding, not bit for bit.  tolerances[i] is grid i's divergence tolerance. */
+fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_iterations,
+    const double* divergence_tolerances, fs_projection_result* results);
 
 /* ---- device footprint (no reference counterpart) ----
  * Upper bound of the device bytes a grid of this shape allocates (fs_grid_device_bytes reads the actual

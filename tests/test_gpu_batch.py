@@ -1,0 +1,132 @@
+"""Batched projection on the GPU (fs_batch_project): one persistent launch runs the PCG iterations of
+many grids of different sizes, one workgroup per grid.
+
+Each grid's result must follow fs_project's contract and agree with the single-grid projection and with
+the FP64 oracle to the step tolerance (only the FP64 reduction order differs, so not bit for bit); a
+grid's result must be the same alone and in any batch, bit for bit (every workgroup reads only its own
+grid); diagonal and multigrid grids mix; malformed batches are rejected.
+"""
+import numpy as np
+import pytest
+
+from fluidsim_amd import (FS_PRECOND_DIAGONAL, FS_TRANSPORT_NONE, Context, FluidGridHIP, FluidGridState,
+                          FluidMixture, MixtureComponent, batch_project)
+from fluidsim_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(17, 13), (1, 129), (131, 1), (64, 64), (65, 37), (128, 96), (200, 150), (300, 200), (2, 67), (96, 257)]
+
+
+def pool_grid(W, H, seed, precond=None):
+    rng = WL.SplitMix64(seed)
+    h = 1.0 / max(W, H)
+    f = WL.pool_with_drop(W, H, h, 0.3 + 0.6 * rng.uniform(1)[0], 0.1, rng)
+    u = np.zeros((W + 1) * H)
+    v = np.zeros(W * (H + 1))
+    for _ in range(6):
+        WL.add_gaussian(u, v, W, H, h, rng.uniform(1, 0, W * h)[0], rng.uniform(1, 0, H * h)[0],
+                        rng.uniform(1, 0.02, 0.08)[0], 0.5 * rng.normal(1)[0])
+    WL.zero_walls(u, v, W, H)
+    lm, gm = WL.two_phase_masses(f, h)
+    g = FluidGridHIP(W, H, h)
+    g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0)]), FluidMixture([MixtureComponent("air", 0.1)]))
+    g.setGravityY(9.81)
+    g.setTransport(FS_TRANSPORT_NONE)
+    if precond is not None:
+        g.setPressurePreconditioner(precond)
+    state = FluidGridState(f, u, v, np.zeros(0), [lm], [gm])
+    g.upload(state)
+    g.applyGravity(0.002)
+    return g, state
+
+
+def tau_of(g):
+    return 1e-6 * g.maxDivergence()
+
+
+def test_batch_matches_single_grid_projections_and_the_oracle(oracle):
+    grids = [pool_grid(W, H, k)[0] for k, (W, H) in enumerate(SHAPES)]
+    singles = [pool_grid(W, H, k)[0] for k, (W, H) in enumerate(SHAPES)]
+    taus = [tau_of(g) for g in grids]
+    # velocity scale of each problem: the largest face speed before the projection
+    scales = []
+    for g in grids:
+        st = g.download()
+        scales.append(max(np.abs(st.u).max(), np.abs(st.v).max()))
+    res = batch_project(grids, 0.002, 2000, taus)
+    for k, (g, s, r, tau) in enumerate(zip(grids, singles, res, taus)):
+        rs = s.project(0.002, 2000, tau)
+        assert r.converged and rs.converged, (SHAPES[k], r, rs)
+        assert abs(r.iterations - rs.iterations) <= 3, (SHAPES[k], r.iterations, rs.iterations)
+        a, b = g.download(), s.download()
+        scale = scales[k]
+        assert np.abs(a.u - b.u).max() <= 1e-4 * scale and np.abs(a.v - b.v).max() <= 1e-4 * scale, SHAPES[k]
+        # the oracle on the same input
+        W, H = SHAPES[k]
+        _, st = pool_grid(W, H, k)
+        o = oracle.OracleGrid(W, H, 1.0 / max(W, H))
+        o.configure([100.0], [0.1])
+        o.set_gravity(9.81)
+        o.upload(st.liquid_fraction, st.u, st.v, None, st.liquid_mass[0], st.gas_mass[0])
+        o.apply_gravity(0.002)
+        ro = o.project(0.002, 2000, tau)
+        so = o.download()
+        assert ro["converged"]
+        assert np.abs(a.u - so["u"]).max() <= 1e-4 * scale and np.abs(a.v - so["v"]).max() <= 1e-4 * scale
+    for g in grids + singles:
+        g.destroy()
+
+
+def test_alone_equals_in_any_batch():
+    probe, _ = pool_grid(96, 64, 42)
+    ref = batch_project([probe], 0.002, 2000, tau_of(probe))[0]
+    ref_state = probe.download()
+    again, _ = pool_grid(96, 64, 42)
+    others = [pool_grid(W, H, 100 + k)[0] for k, (W, H) in enumerate(SHAPES)]
+    mixed = others[:4] + [again] + others[4:]
+    out = batch_project(mixed, 0.002, 2000, [tau_of(g) for g in mixed])
+    r = out[4]
+    assert (r.iterations, r.max_divergence, r.restarts) == (ref.iterations, ref.max_divergence, ref.restarts)
+    s = again.download()
+    for k in ("u", "v", "pressure"):
+        assert np.array_equal(getattr(s, k), getattr(ref_state, k)), k
+    for g in others + [probe, again]:
+        g.destroy()
+
+
+def test_diagonal_and_multigrid_grids_mix():
+    grids = [pool_grid(W, H, 7 + k, FS_PRECOND_DIAGONAL if k % 2 else None)[0] for k, (W, H) in
+             enumerate([(40, 30), (33, 20), (64, 48), (50, 50)])]
+    singles = [pool_grid(W, H, 7 + k, FS_PRECOND_DIAGONAL if k % 2 else None)[0] for k, (W, H) in
+               enumerate([(40, 30), (33, 20), (64, 48), (50, 50)])]
+    taus = [1e-5 * g.maxDivergence() for g in grids]
+    res = batch_project(grids, 0.002, 5000, taus)
+    for g, s, r, tau in zip(grids, singles, res, taus):
+        rs = s.project(0.002, 5000, tau)
+        assert r.converged and rs.converged
+        assert abs(r.iterations - rs.iterations) <= max(3, rs.iterations // 20)
+    for g in grids + singles:
+        g.destroy()
+
+
+def test_batch_contract_and_validation():
+    a, _ = pool_grid(20, 16, 1)
+    b, _ = pool_grid(24, 12, 2)
+    zero = batch_project([a, b], 0.0, 100, 1e-6)
+    assert all(not r.converged for r in zero)
+    budget = batch_project([a, b], 0.002, 0, 1e-12)
+    assert all(not r.converged for r in budget)
+    assert batch_project([], 0.002, 100, 1e-6) == []
+    with pytest.raises(ValueError):
+        batch_project([a, a], 0.002, 100, 1e-6)
+    other = FluidGridHIP(8, 8, 1.0, context=Context(0))
+    other.configureMaterials([], FluidMixture([MixtureComponent("w", 100.0)]), FluidMixture([MixtureComponent("a", 0.1)]))
+    other.upload(FluidGridState(np.ones(64)))
+    with pytest.raises(ValueError):
+        batch_project([a, other], 0.002, 100, 1e-6)
+    fresh = FluidGridHIP(8, 8, 1.0)
+    with pytest.raises(RuntimeError):
+        batch_project([a, fresh], 0.002, 100, 1e-6)
+    for g in (a, b, other, fresh):
+        g.destroy()
