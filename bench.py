@@ -88,56 +88,154 @@ def cpu_baseline(wl, tau, budget_s):
     }
 
 
-def serving_leg(count, rate, steps, lanes, seed=5, max_size=2048):
+def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048):
     """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
-    [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each),
-    first served one request at a time on one stream (serial plain), then arriving at `rate` per second
-    into the serving loop with `lanes` concurrent streams and HBM-budget admission.  Reports aggregate
-    steps/s of both, the served per-step latency p50/p99 and the queueing delay."""
+    [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each).
+    It is first served one request at a time on one stream (serial plain), then fed into the serving loop
+    (`lanes` concurrent streams, HBM-budget admission) at each arrival rate in `rates` (per second).
+    Reports aggregate steps/s, per-step latency p50/p99 and queueing delay of every run."""
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
 
-    stream = [(t, Request(**kw)) for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps)]
-    total_steps = sum(r.steps for _, r in stream)
+    def stream_at(rate):
+        return [(t, Request(**kw)) for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps)]
+
+    base = stream_at(rates[0])
+    total_steps = sum(r.steps for _, r in base)
     with Server(lanes=1) as srv:
         t0 = time.perf_counter()
-        for _, r in stream:
+        for _, r in base:
             rep = srv.wait(srv.submit(r))
             if not rep.done:
                 raise RuntimeError(f"serial request failed: {rep.error}")
         serial_s = time.perf_counter() - t0
         serial = srv.stats()
-    with Server(lanes=lanes) as srv:
-        ids = []
-        t0 = time.perf_counter()
-        for t, r in stream:
-            delay = t0 + t - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            ids.append(srv.submit(r))
-        srv.drain()
-        served_s = time.perf_counter() - t0
-        st = srv.stats()
-        reps = [srv.wait(i) for i in ids]
-    failed = [r.error for r in reps if not r.done]
-    if failed:
-        raise RuntimeError(f"served request failed: {failed[0]}")
-    queue = [r.queue_ms for r in reps]
+    runs = []
+    for rate in rates:
+        stream = stream_at(rate)
+        with Server(lanes=lanes) as srv:
+            ids = []
+            t0 = time.perf_counter()
+            for t, r in stream:
+                delay = t0 + t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(r))
+            srv.drain()
+            served_s = time.perf_counter() - t0
+            st = srv.stats()
+            reps = [srv.wait(i) for i in ids]
+        failed = [r.error for r in reps if not r.done]
+        if failed:
+            raise RuntimeError(f"served request failed: {failed[0]}")
+        queue = [r.queue_ms for r in reps]
+        runs.append({
+            "rate_per_s": rate,
+            "arrival_span_s": stream[-1][0],
+            "steps_per_s": total_steps / served_s,
+            "speedup_vs_serial": serial_s / served_s,
+            "step_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
+            "queue_ms": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
+            "peak_resident_bytes": st["peak_resident_bytes"],
+        })
     return {
-        "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each "
-                    "(PLIC transport, viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to |div u| <= 1e-4), "
-                    f"Poisson arrivals at {rate}/s, seed {seed}",
+        "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each (PLIC transport, "
+                    "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to |div u| <= 1e-4), "
+                    f"Poisson arrivals, seed {seed}",
         "lanes": lanes,
-        "cells_mean": float(np.mean([r.width * r.height for _, r in stream])),
+        "cells_mean": float(np.mean([r.width * r.height for _, r in base])),
         "steps": total_steps,
         "serial_steps_per_s": total_steps / serial_s,
-        "served_steps_per_s": total_steps / served_s,
-        "speedup_vs_serial": serial_s / served_s,
-        "served_step_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
         "serial_step_ms": {"p50": serial["step_p50_ms"], "p99": serial["step_p99_ms"]},
-        "queue_ms": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
-        "peak_resident_bytes": st["peak_resident_bytes"],
-        "arrival_span_s": stream[-1][0],
+        "served": runs,
+    }
+
+
+def batch_leg(count, seed=4, visc_sample=16):
+    """Config C4 on one GPU: `count` seeded instances (sides log-uniform 64..512, 30-90% liquid plus a
+    drop, vortex velocity), each projected once to 1e-6 of its own initial max divergence.  Timed twice
+    from the same device-resident inputs: one fs_project per grid in turn (the plain GPU MGPCG), then one
+    fs_batch_project over all of them (one persistent launch for all PCG iterations).  Each instance's
+    implicit viscosity solve (nu log-uniform 1e-3..1) is timed on its own, instance by instance."""
+    from fluidsim_amd import FluidGridHIP, FluidMixture, MixtureComponent, batch_project, FS_TRANSPORT_NONE
+    from fluidsim_amd import workloads as WL
+
+    t_start = time.perf_counter()
+
+    def note(msg):
+        print(f"[batch leg {time.perf_counter() - t_start:7.2f} s] {msg}", file=sys.stderr, flush=True)
+
+    wls = WL.batch_instances(seed, count)
+    note(f"generated {count} instances")
+    grids, taus = [], []
+    for wl in wls:
+        g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+        nu = wl.meta["nu"]
+        g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0, nu * 100.0)]),
+                             FluidMixture([MixtureComponent("air", 0.1, nu * 0.1)]))
+        from fluidsim_amd import FluidGridState
+        g.upload(FluidGridState(wl.fraction, wl.u, wl.v, np.zeros(0), [wl.liquid_mass], [wl.gas_mass]))
+        g.setTransport(FS_TRANSPORT_NONE)
+        taus.append(1e-6 * g.maxDivergence())
+        g.snapshotSave()
+        grids.append(g)
+        if len(grids) % 64 == 0:
+            note(f"created {len(grids)} grids")
+    dt = wls[0].dt
+
+    def restore():
+        for g in grids:
+            g.snapshotRestore()
+        grids[0].context.synchronize()
+
+    # warm-up of both paths (graph capture, tile tuning done at creation)
+    restore()
+    for g, tau in zip(grids, taus):
+        g.project(dt, 2000, tau)
+    note("serial warm-up done")
+    restore()
+    batch_project(grids, dt, 2000, taus)
+    note("batched warm-up done")
+    restore()
+    t0 = time.perf_counter()
+    serial = [g.project(dt, 2000, tau) for g, tau in zip(grids, taus)]
+    serial_s = time.perf_counter() - t0
+    note(f"serial {serial_s * 1e3:.1f} ms")
+    restore()
+    t0 = time.perf_counter()
+    batched = batch_project(grids, dt, 2000, taus)
+    batch_s = time.perf_counter() - t0
+    note(f"batched {batch_s * 1e3:.1f} ms")
+    ok = all(r.converged for r in serial) and all(r.converged for r in batched)
+    # viscosity: a bounded sample of the instances, one solve each
+    restore()
+    sample = grids[:visc_sample]
+    t0 = time.perf_counter()
+    vit = 0
+    for k, g in enumerate(sample):
+        vit += g.applyViscosity(dt)
+        if (k + 1) % 8 == 0:
+            note(f"viscosity {k + 1}/{len(sample)}: {vit} iterations so far")
+    visc_s = time.perf_counter() - t0
+    for k, g in enumerate(grids):
+        g.destroy()
+        if (k + 1) % 32 == 0:
+            note(f"released {k + 1} grids")
+    cells = [wl.width * wl.height for wl in wls]
+    return {
+        "workload": f"{count} instances, sides log-uniform 64..512, liquid 30-90% + drop, vortex velocity, "
+                    f"seed {seed}; one projection each to 1e-6 of its initial max divergence",
+        "cells_total": int(sum(cells)),
+        "serial_solves_per_s": count / serial_s,
+        "batched_solves_per_s": count / batch_s,
+        "speedup": serial_s / batch_s,
+        "batched_ms": batch_s * 1e3,
+        "converged_all": bool(ok),
+        "pcg_iterations": {"serial_mean": float(np.mean([r.iterations for r in serial])),
+                           "batched_mean": float(np.mean([r.iterations for r in batched]))},
+        "viscosity_sample": len(sample),
+        "viscosity_solves_per_s_serial": len(sample) / visc_s if sample else None,
+        "viscosity_iterations_mean": vit / max(len(sample), 1),
     }
 
 
@@ -151,11 +249,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--serving-requests", type=int, default=0, help="size of the C5 serving stream (0: skip)")
-    ap.add_argument("--serving-rate", type=float, default=4.0)
+    ap.add_argument("--serving-rates", type=str, default="4,16,64", help="arrival rates to sweep (per second)")
     ap.add_argument("--serving-steps", type=int, default=50)
     ap.add_argument("--serving-lanes", type=int, default=4)
     ap.add_argument("--serving-max-size", type=int, default=2048)
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=0, help="size of the C4 batched-solve leg (0: skip)")
+    ap.add_argument("--batch-only", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -167,8 +267,12 @@ def main():
     from fluidsim_amd import FluidGridHIP
     from fluidsim_amd import workloads as WL
 
+    rates = [float(x) for x in args.serving_rates.split(",") if x]
+    if args.batch_only:
+        print(json.dumps(batch_leg(args.batch_instances or 256)), flush=True)
+        return
     if args.serving_only:
-        print(json.dumps(serving_leg(args.serving_requests or 24, args.serving_rate, args.serving_steps,
+        print(json.dumps(serving_leg(args.serving_requests or 24, rates, args.serving_steps,
                                      args.serving_lanes, max_size=args.serving_max_size)), flush=True)
         return
 
@@ -252,9 +356,12 @@ def main():
     }
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+    if args.batch_instances > 0:
+        grid.destroy()
+        out["batched"] = batch_leg(args.batch_instances)
     if args.serving_requests > 0:
         grid.destroy()
-        out["serving"] = serving_leg(args.serving_requests, args.serving_rate, args.serving_steps,
+        out["serving"] = serving_leg(args.serving_requests, rates, args.serving_steps,
                                      args.serving_lanes, max_size=args.serving_max_size)
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -1120,25 +1120,28 @@ fs_status fs_context_destroy(fs_context* ctx)
         (void)hipSetDevice(ctx->device);
         if (ctx->stream)
             (void)hipStreamSynchronize(ctx->stream);
-        std::lock_guard<std::mutex> lock(device_setup_mutex());
-        if (ctx->ev0)
-            (void)hipEventDestroy(ctx->ev0);
-        if (ctx->ev1)
-            (void)hipEventDestroy(ctx->ev1);
-        if (ctx->pinned)
-            (void)hipHostFree(ctx->pinned);
-        if (ctx->pinned_value)
-            (void)hipHostFree(ctx->pinned_value);
-        if (ctx->batch_pinned)
-            (void)hipHostFree(ctx->batch_pinned);
-        if (ctx->stream)
-            (void)hipStreamDestroy(ctx->stream);
-        if (ctx->aux)
-            (void)hipStreamDestroy(ctx->aux);
-        if (ctx->fork)
-            (void)hipEventDestroy(ctx->fork);
-        if (ctx->join)
-            (void)hipEventDestroy(ctx->join);
+        {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            if (ctx->ev0)
+                (void)hipEventDestroy(ctx->ev0);
+            if (ctx->ev1)
+                (void)hipEventDestroy(ctx->ev1);
+            if (ctx->pinned)
+                (void)hipHostFree(ctx->pinned);
+            if (ctx->pinned_value)
+                (void)hipHostFree(ctx->pinned_value);
+            if (ctx->batch_pinned)
+                (void)hipHostFree(ctx->batch_pinned);
+            if (ctx->stream)
+                (void)hipStreamDestroy(ctx->stream);
+            if (ctx->aux)
+                (void)hipStreamDestroy(ctx->aux);
+            if (ctx->fork)
+                (void)hipEventDestroy(ctx->fork);
+            if (ctx->join)
+                (void)hipEventDestroy(ctx->join);
+        }
+        // outside the lock: the descriptor buffer's release takes it
         delete ctx;
     });
 }

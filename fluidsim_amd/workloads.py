@@ -232,3 +232,32 @@ def serving_stream(seed: int = 5, count: int = 24, min_size: int = 128, max_size
                             liquid_viscosity=[nu * 100.0], gas_viscosity=[nu * 0.1], liquid_fraction=fraction,
                             u=u, v=v, liquid_mass=lm, gas_mass=gm)))
     return out
+
+
+def batch_instances(seed: int = 4, count: int = 256, min_size: int = 64, max_size: int = 512):
+    """Batched-solve instances: `count` grids with each side log-uniform in [min_size, max_size]
+    (independent aspect), cell size 1/max(W, H), water 100 / air 0.1 with the liquid filling 30-90% of
+    the height plus a drop, random-vortex velocity (8 vortices), and a kinematic viscosity log-uniform in
+    [1e-3, 1] (mu = nu rho per phase).  Obstacle masks are not modelled (the reference has none).
+    Returns a list of Workload, one per instance, each from its own SplitMix64 substream."""
+    out = []
+    lo, hi = math.log(min_size), math.log(max_size)
+    for k in range(count):
+        rng = SplitMix64(seed * 1000003 + k)
+        W = int(round(math.exp(rng.uniform(1, lo, hi)[0])))
+        H = int(round(math.exp(rng.uniform(1, lo, hi)[0])))
+        h = 1.0 / max(W, H)
+        fill = rng.uniform(1, 0.3, 0.9)[0]
+        fraction = pool_with_drop(W, H, h, fill, 0.08, rng)
+        u = np.zeros((W + 1) * H)
+        v = np.zeros(W * (H + 1))
+        cxs, cys = rng.uniform(8, 0.0, W * h), rng.uniform(8, 0.0, H * h)
+        radii, strengths = rng.uniform(8, 0.02, 0.08), rng.normal(8)
+        for j in range(8):
+            add_gaussian(u, v, W, H, h, cxs[j], cys[j], radii[j], strengths[j])
+        zero_walls(u, v, W, H)
+        lm, gm = two_phase_masses(fraction, h)
+        nu = math.exp(rng.uniform(1, math.log(1e-3), math.log(1.0))[0])
+        out.append(Workload("batch_instance", W, H, h, [100.0], [0.1], 9.81, 1.0 / 240.0, fraction, u, v, lm, gm,
+                            {"seed": seed, "index": k, "nu": nu}))
+    return out

@@ -130,3 +130,18 @@ def test_batch_contract_and_validation():
         batch_project([a, fresh], 0.002, 100, 1e-6)
     for g in (a, b, other, fresh):
         g.destroy()
+
+
+def test_context_with_batch_buffers_is_released():
+    """A context that ran a batched solve (descriptor buffer, pinned result slots) is destroyed cleanly."""
+    ctx = Context(0)
+    g = FluidGridHIP(40, 24, 1.0 / 40, context=ctx)
+    g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0)]), FluidMixture([MixtureComponent("air", 0.1)]))
+    f = np.clip(np.arange(24)[:, None] - 10.0 + 0 * np.arange(40)[None, :], 0.0, 1.0).ravel()
+    lm, gm = WL.two_phase_masses(f, 1.0 / 40)
+    g.upload(FluidGridState(f, np.zeros(41 * 24), np.zeros(40 * 25), np.zeros(0), [lm], [gm]))
+    g.setGravityY(9.81)
+    g.applyGravity(0.002)
+    assert batch_project([g], 0.002, 500, 1e-6)[0].converged
+    g.destroy()
+    ctx.close()
