@@ -248,6 +248,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--fine-tiles", type=str, default="", help="force the finest level's tile variants "
+                    "PRE,POST,RESTRICT (profiling runs: the autotuner's choice can shift under a profiler)")
     ap.add_argument("--serving-requests", type=int, default=0, help="size of the C5 serving stream (0: skip)")
     ap.add_argument("--serving-rates", type=str, default="4,16,64", help="arrival rates to sweep (per second)")
     ap.add_argument("--serving-steps", type=int, default=50)
@@ -279,6 +281,8 @@ def main():
     wl = WL.vortex_box(args.size, args.size, seed=2)
     grid = FluidGridHIP(wl.width, wl.height, wl.cell_size)
     WL.load_into(grid, wl)
+    if args.fine_tiles:
+        grid.setMultigridTiles(0, *[int(v) for v in args.fine_tiles.split(",")])
     tau = 1e-6 * grid.maxDivergence()  # 1e-6 of max |div u*| = 1e-6 relative residual (max-norm)
     grid.snapshotSave()
 

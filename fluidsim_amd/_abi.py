@@ -168,6 +168,7 @@ SIGNATURES = {
     "fs_snapshot_restore": [_vp],
     "fs_mg_levels": [_vp, _ip],
     "fs_mg_level_tiles": [_vp, C.c_int, _ip, _ip, _ip],
+    "fs_mg_set_level_tiles": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
     "fs_mg_level_shape": [_vp, C.c_int, _ip, _ip],
     "fs_mg_level_centers": [_vp, C.c_int, _dp, _dp],
     "fs_mg_level_weights": [_vp, C.c_int, _dp, _dp, _fp],

@@ -1981,6 +1981,25 @@ fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* 
     });
 }
 
+fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int restriction)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(level >= 0 && level < grid->tail_start, FS_E_INVALID_ARGUMENT, "no tiled level with this index");
+        need(pre >= -1 && pre < kSmoothVariants && post >= -1 && post < kSmoothVariants && restriction >= -1 &&
+                 restriction < kRestrictVariants,
+            FS_E_INVALID_ARGUMENT, "unknown tile variant");
+        Level& L = grid->levels[(size_t)level];
+        if (pre >= 0)
+            L.smooth_variant = pre;
+        if (post >= 0)
+            L.post_variant = post;
+        if (restriction >= 0)
+            L.restrict_variant = restriction;
+        grid->drop_graph();   // the captured iteration holds the old tiles
+    });
+}
+
 fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n)
 {
     return guard([&] {

@@ -291,6 +291,10 @@ class FluidGridHIP:
         call("fs_mg_level_tiles", self._h, int(level), C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
 
+    def setMultigridTiles(self, level: int, pre: int = -1, post: int = -1, restriction: int = -1):
+        """Force tile variants of a tiled level (-1 keeps the autotuned one); values do not change."""
+        call("fs_mg_set_level_tiles", self._h, int(level), int(pre), int(post), int(restriction))
+
     def applySurfaceTension(self, dt: float):
         """FluidGridCPU::applySurfaceTension (FluidGridCPU.cpp:862-926)."""
         call("fs_apply_surface_tension", self._h, float(dt))

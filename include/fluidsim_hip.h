@@ -209,6 +209,9 @@ fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, i
  * 5 = 32x32/512) and the residual-restriction variant (0 = 32x16, 1 = 16x16, 2 = 16x8, 3 = 8x8 coarse
  * cells); -1 for levels solved inside the single-workgroup tail. */
 fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction);
+/* Overrides the autotuned variants of a tiled level (-1 keeps one): every variant computes the same
+ * values, so this changes time only -- for profiling one fixed configuration and for testing that claim. */
+fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int restriction);
 /* Coarsest dense Cholesky factor, n*n row-major (lower triangle); *n receives the unknown count. */
 fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n);
 /* One V-cycle: correction = M^-1 (residual - mean), both W*H FP64 host arrays (converted to FP32 on

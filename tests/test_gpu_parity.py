@@ -302,3 +302,34 @@ def test_high_contrast_pool_with_drop_converges_like_the_oracle(oracle, index):
     scale = max(np.abs(so["u"]).max(), np.abs(so["v"]).max())
     assert np.abs(sg.u - so["u"]).max() < 1e-4 * scale and np.abs(sg.v - so["v"]).max() < 1e-4 * scale
     g.destroy()
+
+
+def test_every_tile_variant_computes_the_same_values():
+    """The autotuner only picks among tilings of identical arithmetic: each smoother and restriction variant
+    gives a bit-identical V-cycle and projection."""
+    wl = WL.sloped_pool(300, 200, 4.0)
+    g = hip_grid(wl)
+    i = np.arange(300 * 200)
+    a = np.sin(i * 0.31) + np.cos(i * 0.073)
+    a -= a.mean()
+    levels = [l for l in range(g.multigridLevels()) if g.multigridTiles(l)[0] >= 0]
+    assert levels, "no tiled level"
+    g.setMultigridTiles(levels[0], 0, 0, 0)
+    ref = g.multigridApply(a)
+    for l in levels:
+        for v in range(6):
+            g.setMultigridTiles(l, v, v, v % 4)
+            assert g.multigridTiles(l) == (v, v, v % 4)
+            assert np.array_equal(g.multigridApply(a), ref), f"level {l} variant {v}"
+        g.setMultigridTiles(l, 0, 0, 0)
+    g.snapshotSave()
+    r0 = g.project(wl.dt, 2000, 1e-6)
+    p0 = g.download().pressure
+    g.snapshotRestore()
+    for l in levels:
+        g.setMultigridTiles(l, 5, 4, 3)
+    r1 = g.project(wl.dt, 2000, 1e-6)
+    assert r0.iterations == r1.iterations and np.array_equal(g.download().pressure, p0)
+    with pytest.raises(ValueError):
+        g.setMultigridTiles(levels[0], 6, 0, 0)
+    g.destroy()
