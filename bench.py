@@ -7,6 +7,11 @@ one complete projection -- rhs formation, coefficient and multigrid-hierarchy re
 CG to max|b - A p| <= 1e-6 max|b|, velocity correction and FP64 divergence check -- starting from the
 same device-resident input every time (restored from an HBM snapshot, pressure history reset).
 
+Two further legs are appended to the same JSON line: "batched" (config C4: 256 mixed-size grids
+projected one after another, then all at once with fs_batch_project) and "serving" (config C5: a
+Poisson stream of full simulations served one at a time, then through the serving loop at each
+arrival rate).  --batch-instances 0 / --serving-requests 0 skip them.
+
 Prints one JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
 """
 from __future__ import annotations
@@ -250,13 +255,13 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--fine-tiles", type=str, default="", help="force the finest level's tile variants "
                     "PRE,POST,RESTRICT (profiling runs: the autotuner's choice can shift under a profiler)")
-    ap.add_argument("--serving-requests", type=int, default=0, help="size of the C5 serving stream (0: skip)")
-    ap.add_argument("--serving-rates", type=str, default="4,16,64", help="arrival rates to sweep (per second)")
+    ap.add_argument("--serving-requests", type=int, default=24, help="size of the C5 serving stream (0: skip)")
+    ap.add_argument("--serving-rates", type=str, default="8,64", help="arrival rates to sweep (per second)")
     ap.add_argument("--serving-steps", type=int, default=50)
     ap.add_argument("--serving-lanes", type=int, default=4)
     ap.add_argument("--serving-max-size", type=int, default=2048)
     ap.add_argument("--serving-only", action="store_true")
-    ap.add_argument("--batch-instances", type=int, default=0, help="size of the C4 batched-solve leg (0: skip)")
+    ap.add_argument("--batch-instances", type=int, default=256, help="size of the C4 batched-solve leg (0: skip)")
     ap.add_argument("--batch-only", action="store_true")
     args = ap.parse_args()
 
