@@ -16,6 +16,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace fsim {
 
 namespace {
@@ -49,9 +51,13 @@ __device__ __forceinline__ void xcd_tile(int gx, int gy, int& bx, int& by)
 // registers for all eight half-sweeps, so an update reads four neighbours from LDS and writes one value.
 // MODE 0: pre-smoothing from a zero guess (no solution read); 1: post-smoothing, the prolonged coarse
 // correction added to the solution first; 2: plain sweeps of the stored solution.
+// 512-thread post-smoothing tiles are held to 128 VGPRs (four waves per SIMD, two workgroups per CU).
+template <int MODE, int NT> constexpr int rb_waves_per_eu() { return (MODE == 1 && NT == 512) ? 4 : 1; }
+
 template <int TX, int TY, int NS, int FINE, int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift,
-    int c0, CoarseLink link, const int* status)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(rb_waves_per_eu<MODE, NT>())))
+void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int c0, CoarseLink link,
+    const int* status)
 {
     constexpr bool from_zero = MODE == 0;
     constexpr int HALO = 2 * NS;
@@ -159,35 +165,47 @@ __global__ __launch_bounds__(NT) void k_rb_multi(LevelView L, int cur, const flo
         }
     }
     __syncthreads();
+    // Per-slot invariants of the eight half-sweeps, computed once: the slot's LDS index, the index of
+    // its left neighbour for each colour, 1/diagonal of each colour's cell, and (one byte per colour)
+    // the cell's distance to the tile edge plus one -- zero for a slot outside the grid -- so the
+    // half-sweep at depth k (region shrunk by k + 1 cells) updates the cell iff its byte exceeds k + 1.
+    int sdst[ROUNDS], sleft[2][ROUNDS], sreach[ROUNDS];
+    float sinv[2][ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int ly = max(cly[r], 1);        // unused slots address a valid row and stay inactive
+        const int jj = cj[r];
+        sdst[r] = ly * HX + jj;
+        int reach = 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int p = (c + ly) & 1;
+            sleft[c][r] = sdst[r] + p - 1;    // rows 1 .. LY-2: every neighbour index stays inside the array
+            const int lx = 2 * jj + p;
+            const int dist = min(min(ly, LY - 1 - ly), min(lx, LX - 1 - lx));
+            if (cin[c][r])
+                reach |= (dist + 1) << (8 * c);
+            sinv[c][r] = inv_or_zero((cl[c][r] + cr[c][r]) + (cu[c][r] + cd[c][r]));
+        }
+        sreach[r] = reach;
+    }
     // ---- eight half-sweeps, each region one cell smaller than the last ----
     // A half-sweep reads only the other colour, so all neighbour values are gathered first and the
-    // updates written afterwards: one LDS latency per half-sweep instead of one per round.
-#pragma unroll 1
-    for (int k = 0; k < 2 * NS; ++k) {
-        const int color = c0 ^ (k & 1);
+    // updates written afterwards: one LDS latency per half-sweep instead of one per round.  The colour
+    // is a compile-time constant inside a half-sweep, so per-colour registers are addressed directly.
+    auto half = [&](auto color_tag, int k) {
+        constexpr int color = decltype(color_tag)::value;
         const int lo = k + 1;
         const bool neighbours = !(from_zero && k == 0);
         float* mine = s_col[color];
         const float* other = s_col[color ^ 1];
         float n0[ROUNDS], n1[ROUNDS], n2[ROUNDS], n3[ROUNDS];
-        bool act[ROUNDS];
-        int dst[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
-            const int ly = max(cly[r], 1);        // unused slots read a valid row and are masked
-            const int p = (color + ly) & 1;
-            const int jj = cj[r];
-            const int lx = 2 * jj + p;
-            const int x = x0 + lx, y = y0 + ly;
-            act[r] = cly[r] >= lo && cly[r] < LY - lo && lx >= lo && lx < LX - lo && x >= 0 && x < w && y >= 0 &&
-                     y < h;
-            const int row = ly * HX;
-            dst[r] = row + jj;
-            // unconditional LDS reads at clamped, always-valid indices
-            n0[r] = other[row + max(jj - 1 + p, 0)];
-            n1[r] = other[row + min(jj + p, HX - 1)];
-            n2[r] = other[row - HX + jj];
-            n3[r] = other[row + HX + jj];
+            n0[r] = other[sleft[color][r]];
+            n1[r] = other[sleft[color][r] + 1];
+            n2[r] = other[sdst[r] - HX];
+            n3[r] = other[sdst[r] + HX];
         }
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
@@ -195,10 +213,20 @@ __global__ __launch_bounds__(NT) void k_rb_multi(LevelView L, int cur, const flo
             float v = cb[color][r];
             if (neighbours)
                 v += (a * n0[r] + b * n1[r]) + (u * n2[r] + d * n3[r]);
-            if (act[r])
-                mine[dst[r]] = v * inv_or_zero((a + b) + (u + d));
+            if (((sreach[r] >> (8 * color)) & 0xff) > lo)
+                mine[sdst[r]] = v * sinv[color][r];
         }
         __syncthreads();
+    };
+#pragma unroll 1
+    for (int k = 0; k < 2 * NS; k += 2) {
+        if (c0 == 0) {
+            half(std::integral_constant<int, 0>{}, k);
+            half(std::integral_constant<int, 1>{}, k + 1);
+        } else {
+            half(std::integral_constant<int, 1>{}, k);
+            half(std::integral_constant<int, 0>{}, k + 1);
+        }
     }
     float* sol_out = L.sol[cur ^ 1];
     for (int e = threadIdx.x; e < TX * TY; e += NT) {
