@@ -57,6 +57,10 @@ struct PcgScalars {
     int replace;        // check not triggered by the tolerance: refresh r from FP64, restart only on drift
     int pad;
     double window_rmax; // true residual max-norm that started the current refinement window
+    // r.z of the direction used by the iteration that starts at iteration count c, kept in slot c & 1:
+    // the direction kernel reads the previous slot and writes the next one while other workgroups of
+    // the same launch still read the previous value.
+    double rz_slot[2];
 };
 
 // One multigrid level as the FP32 smoothing kernels see it.

@@ -340,6 +340,7 @@ struct fs_grid {
         g.zdiag = zdiag.p;
         g.S = S.p;
         g.partials = partials.p;
+        g.partials_ad = partials.p + 4 * ((size_t)((W + 63) / 64) * ((H + 15) / 16));
         return g;
     }
 
@@ -383,7 +384,9 @@ struct fs_grid {
         bytes += S.alloc(1);
         bytes += S_aux.alloc(1);
         const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
-        bytes += partials.alloc(std::max<size_t>(4 * blocks, 64));
+        // region A: up to four partials per cell block; region B (after it): the operator's d.q partials,
+        // read by the update kernel while it writes region A
+        bytes += partials.alloc(5 * blocks + 64);
         build_hierarchy();
         tune_tiles();
     }
@@ -1822,9 +1825,11 @@ fs_status fs_snapshot_restore(fs_grid* grid)
         need(grid->has_snapshot, FS_E_LOGIC, "No snapshot to restore");
         set_device(grid->ctx);
         fs_grid& g = *grid;
+        // one launch for all seven fields
+        CopyList list{};
         auto copy = [&](DevBuf<double>& dst, const DevBuf<double>& src) {
             if (src.n)
-                HIPC(hipMemcpyAsync(dst.p, src.p, src.n * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+                list.item[list.count++] = CopyList::Item{src.p, dst.p, src.n};
         };
         copy(g.fraction, g.snap_fraction);
         copy(g.u, g.snap_u);
@@ -1833,6 +1838,8 @@ fs_status fs_snapshot_restore(fs_grid* grid)
         copy(g.p_prev, g.snap_p_prev);
         copy(g.lmass, g.snap_lmass);
         copy(g.gmass, g.snap_gmass);
+        launch_copy_many(g.st(), list);
+        HIPC(hipGetLastError());
         g.coeff_valid = false;
         g.mg_valid = false;
         // no synchronisation: restores are stream-ordered before the next solve

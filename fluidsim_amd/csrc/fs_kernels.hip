@@ -673,10 +673,13 @@ __global__ __launch_bounds__(kNT) void k_pcg_ad(GridDev g)
     }
     const double bs = block_sum<kNT>(dot, lds);
     if (threadIdx.x == 0)
-        g.partials[block_id()] = bs;
+        g.partials_ad[block_id()] = bs;
 }
 
 // x += alpha d ; r -= alpha q ; residual sum and max-norm; convergence test of the recurrence.
+// Every workgroup first reduces the operator's d.q partials itself (same order, same value) and forms
+// alpha; workgroup 0 records it, clears the restart flag the previous direction consumed, and stops
+// the solve on a non-positive d.q.
 __global__ __launch_bounds__(kNT) void k_pcg_update(GridDev g)
 {
     __shared__ double lds[kNT / 64];
@@ -692,7 +695,21 @@ __global__ __launch_bounds__(kNT) void k_pcg_update(GridDev g)
     }
     if (g.S->status != kRunning)
         return;
-    const float alpha = (float)g.S->alpha;
+    const double dq = finish_sum<kNT>(g.partials_ad, (int)block_count(), 1, 0, lds);
+    const double rz = g.S->rz_slot[g.S->iterations & 1];
+    bool stop = dq <= 0.0 || !isfinite(dq);
+    const double alpha64 = stop ? 0.0 : rz / dq;
+    stop = stop || !isfinite(alpha64);
+    if (block_id() == 0 && threadIdx.x == 0) {
+        g.S->dq = dq;
+        g.S->alpha = alpha64;
+        g.S->restart = 0;
+        if (stop)
+            g.S->status = kStopped;
+    }
+    if (stop)
+        return;
+    const float alpha = (float)alpha64;
     double sum = 0.0, m = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -765,9 +782,12 @@ __global__ __launch_bounds__(kNT) void k_pcg_rz(GridDev g, const float* z)
     }
 }
 
-// d = (z - mean z) + beta d
+// d = (z - mean z) + beta d.  Every workgroup reduces the r.z partials itself (same order, same
+// value): beta = next / previous r.z (zero after a restart); workgroup 0 records the new r.z in the
+// next slot and stops the solve at the iteration budget or on a non-positive r.z.
 __global__ __launch_bounds__(kNT) void k_pcg_direction(GridDev g, const float* z)
 {
+    __shared__ double lds[kNT / 64];
     const Tile4 T = tile4(g.W, g.H, g.s);
     float zv[4], dv[4];
 #pragma unroll
@@ -777,8 +797,27 @@ __global__ __launch_bounds__(kNT) void k_pcg_direction(GridDev g, const float* z
     }
     if (g.S->status != kRunning)
         return;
-    const float mean = (float)g.S->z_mean;
-    const double beta64 = g.S->beta;
+    const int nb = (int)block_count();
+    const double tz = finish_sum<kNT>(g.partials, nb, 3, 0, lds);
+    const double trz = finish_sum<kNT>(g.partials, nb, 3, 1, lds);
+    const double tr = finish_sum<kNT>(g.partials, nb, 3, 2, lds);
+    const double n = (double)g.W * g.H;
+    const double zmean = tz / n;
+    const double next = trz - zmean * tr;
+    const int c = g.S->iterations;
+    const double beta64 = g.S->restart ? 0.0 : next / g.S->rz_slot[(c + 1) & 1];
+    const bool stop = !isfinite(beta64) || !(c < g.S->max_iterations && next > 0.0 && isfinite(next));
+    if (block_id() == 0 && threadIdx.x == 0) {
+        g.S->z_mean = zmean;
+        g.S->beta = beta64;
+        if (isfinite(beta64))
+            g.S->rz_slot[c & 1] = next;
+        if (stop)
+            g.S->status = kStopped;
+    }
+    if (stop)
+        return;
+    const float mean = (float)zmean;
     const float beta = (float)beta64;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -938,6 +977,14 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
     }
 }
 
+// Several FP64 device-to-device copies in one launch (blockIdx.y selects the copy).
+__global__ __launch_bounds__(kNT) void k_copy_many(CopyList list)
+{
+    const CopyList::Item it = list.item[blockIdx.y];
+    for (size_t i = (size_t)blockIdx.x * kNT + threadIdx.x; i < it.n; i += (size_t)gridDim.x * kNT)
+        it.dst[i] = it.src[i];
+}
+
 __global__ void k_copy_f32(const float* src, float* dst, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1081,7 +1128,6 @@ void launch_pad_copy(hipStream_t st, const double* src, double* dst, int W, int 
 void launch_pcg_ad(hipStream_t st, const GridDev& g)
 {
     hipLaunchKernelGGL(k_pcg_ad, cell_grid(g.W, g.H), dim3(kNT), 0, st, g);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kNT), 0, st, g, (int)kFinAd, (int)cell_blocks(g.W, g.H));
 }
 
 void launch_pcg_update(hipStream_t st, const GridDev& g)
@@ -1093,7 +1139,6 @@ void launch_pcg_update(hipStream_t st, const GridDev& g)
 void launch_pcg_rz(hipStream_t st, const GridDev& g, const float* z)
 {
     hipLaunchKernelGGL(k_pcg_rz, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kNT), 0, st, g, (int)kFinRz, (int)cell_blocks(g.W, g.H));
 }
 
 void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z)
@@ -1104,6 +1149,19 @@ void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z)
 void launch_precond_diag(hipStream_t st, const GridDev& g)
 {
     hipLaunchKernelGGL(k_precond_diag, cell_grid(g.W, g.H), dim3(kNT), 0, st, g);
+}
+
+void launch_copy_many(hipStream_t st, const CopyList& list)
+{
+    if (list.count <= 0)
+        return;
+    size_t most = 0;
+    for (int k = 0; k < list.count; ++k)
+        most = std::max(most, list.item[k].n);
+    if (most == 0)
+        return;
+    const unsigned bx = (unsigned)std::min<size_t>((most + kNT - 1) / kNT, 2048);
+    hipLaunchKernelGGL(k_copy_many, dim3(bx, list.count), dim3(kNT), 0, st, list);
 }
 
 void launch_copy_f32(hipStream_t st, const float* src, float* dst, size_t n)

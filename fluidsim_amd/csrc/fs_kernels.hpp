@@ -30,6 +30,7 @@ struct GridDev {
     float* zdiag;             // padded FP32 diagonal-preconditioned residual
     PcgScalars* S;            // device scalar block
     double* partials;         // reduction partials (>= 4 * max blocks)
+    double* partials_ad;      // d.q partials of the operator kernel (one per block)
 };
 
 // Status values beyond PcgStatus used around the loop.
@@ -112,6 +113,18 @@ void launch_add_vortex(hipStream_t st, const GridDev& g, double ox, double oy, d
     double strength);
 void launch_semi_lagrangian(hipStream_t st, const GridDev& g, double ox, double oy, double dt, double* new_u,
     double* new_v, double* scratch_cells);
+// Up to eight FP64 device-to-device copies in one launch.
+struct CopyList {
+    struct Item {
+        const double* src;
+        double* dst;
+        size_t n;
+    };
+    int count;
+    Item item[8];
+};
+void launch_copy_many(hipStream_t st, const CopyList& list);
+
 void launch_pad_copy(hipStream_t st, const double* src, double* dst, int W, int H, int to_padded);
 
 // ---- PCG loop (FP32 vectors, FP64 reductions) ----

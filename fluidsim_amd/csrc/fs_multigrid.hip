@@ -1120,8 +1120,11 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
     const GridDev& g = I.g;
     const int W = g.W, H = g.H, s = g.s, N = W * H;
     const double n = (double)N;
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         S = *g.S;
+        S.rz = S.rz_slot[S.iterations & 1];   // r.z of the first direction (written by its kernel)
+        S.restart = 0;                         // consumed by that direction
+    }
     __syncthreads();
     const float* z = I.multigrid ? (I.levels > 0 ? I.lev[0].L.sol[0] : I.tail->top_sol) : g.zdiag;
     while (S.status == kRunning) {
