@@ -114,7 +114,7 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
     if (MODE == 1) {
         // per chunk of PC cells, two batched phases: transfer tables, then the four coarse values (chunks
         // bound the registers this prologue adds to the kernel)
-        constexpr int PC = 4;
+        constexpr int PC = NT == 512 ? 1 : 4;   // 512-thread tiles keep to 128 VGPRs without spilling
 #pragma unroll
         for (int c0 = 0; c0 < LOADS; c0 += PC) {
             int a0[PC], a1[PC], b0[PC], b1[PC];
