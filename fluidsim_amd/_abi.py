@@ -148,6 +148,8 @@ SIGNATURES = {
     "fs_capillary_step_limit": [_vp, _dp],
     "fs_set_preconditioner": [_vp, C.c_int],
     "fs_set_transport": [_vp, C.c_int],
+    "fs_set_solid_mask": [_vp, C.c_void_p],
+    "fs_solid_cells": [_vp, C.POINTER(C.c_longlong)],
     "fs_upload": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "fs_download": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "fs_invalidate_caches": [_vp],

@@ -26,6 +26,11 @@ enum PcgStatus : int { kRunning = 0, kConverged = 1, kStopped = 2 };
 // recurrence by kDriftRatio, and otherwise continues on the refreshed residual (reliable updating,
 // van der Vorst & Ye 2000).
 constexpr int kReplaceEvery = 32;
+// Solid-mask extension: a coarse pivot at or below this fraction of its row's assembled diagonal marks
+// the last unknown of a disconnected or fully blocked coarse region, which is then pinned at zero (the
+// reference pins its first cell the same way).  Grids without a mask never pin: every pivot of their
+// weighted Laplacians lies far above the bound, and a non-positive one fails as in the reference.
+constexpr double kPinRatio = 1e-9;
 constexpr double kRefineDrop = 1e-3;
 constexpr double kDriftRatio = 2.0;
 

@@ -229,6 +229,8 @@ struct fs_grid {
     ViscositySolver* visc = nullptr;       // implicit viscosity workspace (created on first use)
     DevBuf<double2> st_heights, st_orient, st_raw, st_curv;   // capillary geometry (first use)
     DevBuf<float> x, r, d, q, zdiag;
+    DevBuf<unsigned char> solid;   // solid-mask extension (empty: no mask)
+    long long solid_cells = 0;
     DevBuf<PcgScalars> S;
     DevBuf<PcgScalars> S_aux;   // failure flag of a hierarchy rebuilt on the second stream
     DevBuf<double> partials;
@@ -341,6 +343,7 @@ struct fs_grid {
         g.S = S.p;
         g.partials = partials.p;
         g.partials_ad = partials.p + 4 * ((size_t)((W + 63) / 64) * ((H + 15) / 16));
+        g.solid = solid.n ? solid.p : nullptr;
         return g;
     }
 
@@ -584,7 +587,8 @@ struct fs_grid {
                 Cl.invd32.p);
         }
         const Level& last = levels.back();
-        launch_coarse_inverse(stream, last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, flag);
+        launch_coarse_inverse(stream, last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, flag,
+            solid.n > 0);
         mg_valid = true;
     }
     void rebuild_hierarchy() { rebuild_hierarchy(st(), S.p); }
@@ -1269,6 +1273,74 @@ fs_status fs_set_surface_tension(fs_grid* grid, double coefficient)
         need(std::isfinite(coefficient) && coefficient >= 0.0, FS_E_INVALID_ARGUMENT,
             "Surface tension must be finite and nonnegative");
         grid->sigma = coefficient;
+    });
+}
+
+fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        const int W = grid->W, H = grid->H;
+        const size_t N = grid->N();
+        std::vector<unsigned char> m;
+        long long solids = 0;
+        if (mask) {
+            m.resize(N);
+            for (size_t i = 0; i < N; ++i) {
+                m[i] = mask[i] ? 1 : 0;
+                solids += m[i];
+            }
+            need(solids < (long long)N, FS_E_INVALID_ARGUMENT, "a solid mask must leave at least one fluid cell");
+            // the fluid cells must form one 4-connected region
+            std::vector<unsigned char> seen(N, 0);
+            std::vector<size_t> stack;
+            size_t start = 0;
+            while (m[start])
+                ++start;
+            stack.push_back(start);
+            seen[start] = 1;
+            size_t reached = 0;
+            while (!stack.empty()) {
+                const size_t i = stack.back();
+                stack.pop_back();
+                ++reached;
+                const int x = (int)(i % W), y = (int)(i / W);
+                auto visit = [&](size_t j) {
+                    if (!m[j] && !seen[j]) {
+                        seen[j] = 1;
+                        stack.push_back(j);
+                    }
+                };
+                if (x > 0) visit(i - 1);
+                if (x + 1 < W) visit(i + 1);
+                if (y > 0) visit(i - W);
+                if (y + 1 < H) visit(i + W);
+            }
+            need(reached == N - (size_t)solids, FS_E_INVALID_ARGUMENT,
+                "the fluid cells of a solid mask must form one 4-connected region");
+        }
+        set_device(grid->ctx);
+        if (mask) {
+            if (grid->solid.n != N)
+                grid->bytes += grid->solid.alloc(N);
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            HIPC(hipMemcpy(grid->solid.p, m.data(), N, hipMemcpyHostToDevice));
+        } else if (grid->solid.n) {
+            grid->bytes -= grid->solid.n;
+            grid->solid.release();
+        }
+        grid->solid_cells = solids;
+        grid->coeff_valid = false;
+        grid->mg_valid = false;
+        grid->drop_graph();
+    });
+}
+
+fs_status fs_solid_cells(const fs_grid* grid, long long* count)
+{
+    return guard([&] {
+        need(grid != nullptr && count != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *count = grid->solid_cells;
     });
 }
 
@@ -2016,7 +2088,7 @@ fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n)
         // the bit-exact factorCoarsest factor is formed on demand (the solve uses the sweep inverse)
         const Level& last = grid->levels.back();
         launch_coarse_factor(grid->st(), last.w, last.h, last.right64.p, last.down64.p, grid->factor.p, nullptr,
-            grid->coarse_n, nullptr);
+            grid->coarse_n, nullptr, grid->solid.n > 0);
         HIPC(hipStreamSynchronize(grid->st()));
         if (n)
             *n = grid->coarse_n;
@@ -2146,7 +2218,7 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
             case 7: {
                 const Level& last = g.levels.back();
                 launch_coarse_inverse(g.st(), last.w, last.h, last.right64.p, last.down64.p, g.ginv.p, g.coarse_n,
-                    nullptr);
+                    nullptr, g.solid.n > 0);
                 break;
             }
             case 8: launch_pcg_direction(g.st(), gd, g.levels[0].sol0.p); break;

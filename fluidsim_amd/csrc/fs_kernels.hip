@@ -105,13 +105,15 @@ __global__ __launch_bounds__(kNT) void k_density(GridDev g)
     }
 }
 
-// Conductance of the face right of / below each cell: 2 / (rho_a + rho_b), zero at walls.
+// Conductance of the face right of / below each cell: 2 / (rho_a + rho_b), zero at walls and, with a
+// solid mask, zero on every face that touches a solid cell.
 __global__ __launch_bounds__(kNT) void k_face_weights(GridDev g)
 {
     const int W = g.W, H = g.H;
     const int x = blockIdx.x * kTX + (threadIdx.x & 63);
     const int xc = min(x, W - 1), xr = min(x + 1, W - 1);
     double c[4], r[4], d[4];
+    bool open_r[4], open_d[4];
     int yy[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -121,14 +123,20 @@ __global__ __launch_bounds__(kNT) void k_face_weights(GridDev g)
         c[k] = g.density[yc * W + xc];
         r[k] = g.density[yc * W + xr];
         d[k] = g.density[yd * W + xc];
+        open_r[k] = open_d[k] = true;
+        if (g.solid) {
+            const bool sc = g.solid[yc * W + xc] != 0;
+            open_r[k] = !(sc || g.solid[yc * W + xr] != 0);
+            open_d[k] = !(sc || g.solid[yd * W + xc] != 0);
+        }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int y = yy[k];
         if (x < W && y < H) {
             const int P = pidx(x, y, g.s);
-            g.right64[P] = x + 1 < W ? 2.0 / (c[k] + r[k]) : 0.0;
-            g.down64[P] = y + 1 < H ? 2.0 / (c[k] + d[k]) : 0.0;
+            g.right64[P] = x + 1 < W && open_r[k] ? 2.0 / (c[k] + r[k]) : 0.0;
+            g.down64[P] = y + 1 < H && open_d[k] ? 2.0 / (c[k] + d[k]) : 0.0;
         }
     }
 }
@@ -215,6 +223,7 @@ __global__ __launch_bounds__(kNT) void k_rhs(GridDev g, double scale)
     const int x = blockIdx.x * kTX + (threadIdx.x & 63);
     const int xc = min(x, W - 1);
     double uL[4], uR[4], vT[4], vB[4];
+    unsigned closed[4];   // solid mask: bit 0 left, 1 right, 2 top, 3 bottom face touches a solid cell
     int yy[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -227,6 +236,13 @@ __global__ __launch_bounds__(kNT) void k_rhs(GridDev g, double scale)
         uR[k] = hz[xc + 1];
         vT[k] = vt[xc];
         vB[k] = vt[xc + W];
+        closed[k] = 0u;
+        if (g.solid) {
+            const unsigned char* m = g.solid + (size_t)yc * W;
+            const bool sc = m[xc] != 0;
+            closed[k] = (sc || m[max(xc - 1, 0)] ? 1u : 0u) | (sc || m[min(xc + 1, W - 1)] ? 2u : 0u) |
+                        (sc || m[xc - (yc > 0 ? W : 0)] ? 4u : 0u) | (sc || m[xc + (yc + 1 < H ? W : 0)] ? 8u : 0u);
+        }
     }
     if (g.S->status != kRunning)
         return;
@@ -235,17 +251,20 @@ __global__ __launch_bounds__(kNT) void k_rhs(GridDev g, double scale)
     for (int k = 0; k < 4; ++k) {
         const int y = yy[k];
         if (x < W && y < H) {
-            // walls are zeroed and read as zero (FluidGridCPU.cpp:1050-1061)
+            // walls are zeroed and read as zero (FluidGridCPU.cpp:1050-1061); faces touching a solid
+            // cell likewise (both cells of such a face write the same zero and neither reads it)
             double* hz = g.u + (size_t)y * (W + 1);
             double* vt = g.v + (size_t)y * W;
-            const double l = x == 0 ? 0.0 : uL[k];
-            const double r = x + 1 == W ? 0.0 : uR[k];
-            const double t = y == 0 ? 0.0 : vT[k];
-            const double b = y + 1 == H ? 0.0 : vB[k];
-            if (x == 0) hz[0] = 0.0;
-            if (x + 1 == W) hz[W] = 0.0;
-            if (y == 0) vt[x] = 0.0;
-            if (y + 1 == H) vt[x + W] = 0.0;
+            const bool cl = x == 0 || (closed[k] & 1u), cr = x + 1 == W || (closed[k] & 2u);
+            const bool ct = y == 0 || (closed[k] & 4u), cb = y + 1 == H || (closed[k] & 8u);
+            const double l = cl ? 0.0 : uL[k];
+            const double r = cr ? 0.0 : uR[k];
+            const double t = ct ? 0.0 : vT[k];
+            const double b = cb ? 0.0 : vB[k];
+            if (cl) hz[x] = 0.0;
+            if (cr) hz[x + 1] = 0.0;
+            if (ct) vt[x] = 0.0;
+            if (cb) vt[x + W] = 0.0;
             const double du = r - l;
             const double dv = b - t;
             const double rhs = -(du + dv) / g.h * scale;

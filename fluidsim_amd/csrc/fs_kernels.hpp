@@ -31,6 +31,7 @@ struct GridDev {
     PcgScalars* S;            // device scalar block
     double* partials;         // reduction partials (>= 4 * max blocks)
     double* partials_ad;      // d.q partials of the operator kernel (one per block)
+    const unsigned char* solid;   // W * H, nonzero = solid cell (solid-mask extension); null: no mask
 };
 
 // Status values beyond PcgStatus used around the loop.
@@ -91,11 +92,12 @@ void launch_coarsen(hipStream_t st, int fw, int fh, const double* fcx, const dou
     const double* fright, const double* fdown, int cw, int ch, const double* ccx, const double* ccy,
     double* cright, double* cdown);
 // Bit-exact Cholesky of the pinned coarse matrix (factorCoarsest), plus its FP32 inverse when ginv is set.
+// pin != 0 (grids with a solid mask): a vanishing pivot pins its unknown at zero instead of failing.
 void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, const double* down64,
-    double* factor, float* ginv, int n, PcgScalars* S);
+    double* factor, float* ginv, int n, PcgScalars* S, int pin);
 // FP32 inverse of the pinned coarse matrix by the in-register sweep operator (the projection's path).
 void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, const double* down64, float* ginv,
-    int n, PcgScalars* S);
+    int n, PcgScalars* S, int pin);
 
 // ---- projection set-up and finish (FP64) ----
 void launch_rhs(hipStream_t st, const GridDev& g, double scale);

@@ -338,7 +338,9 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, cons
         const float c = s_sol[i];
         const float product = (s_right[i - 1] * (c - s_sol[i - 1]) + s_right[i] * (c - s_sol[i + 1])) +
                               (s_down[i - LX] * (c - s_sol[i - LX]) + s_down[i] * (c - s_sol[i + LX]));
-        s_res[i] = s_res[i] - product;
+        // a cell without conductance (solid masks only) is not restricted: the V-cycle stays symmetric
+        const bool active = (s_right[i - 1] + s_right[i]) + (s_down[i - LX] + s_down[i]) > 0.f;
+        s_res[i] = active ? s_res[i] - product : 0.f;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < CTX * CTY; e += kNT) {
@@ -377,11 +379,12 @@ constexpr int kMaxCoarse = 63;
 constexpr int kFactorThreads = 256;
 
 __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, const double* right,
-    const double* down, double* factor, float* ginv, int n, PcgScalars* S)
+    const double* down, double* factor, float* ginv, int n, PcgScalars* S, int pin)
 {
     __shared__ double a[kMaxCoarse * kMaxCoarse];   // Cholesky working matrix (lower triangle)
     __shared__ double m[kMaxCoarse * kMaxCoarse];   // Gauss-Jordan working matrix
     __shared__ double q[kMaxCoarse];
+    __shared__ double assembled[kMaxCoarse];        // diagonal before elimination (pinning bound)
     __shared__ int failed;
     const int s = w + 2;
     const int tid = threadIdx.x;
@@ -401,6 +404,7 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
         const double diag = right[i - 1] + right[i] + down[i - s] + down[i];
         a[row * n + row] = diag;
         m[row * n + row] = diag;
+        assembled[row] = diag;
         if (x > 0 && cell - 1 > 0) {
             a[row * n + row - 1] = -right[i - 1];
             m[row * n + row - 1] = -right[i - 1];
@@ -416,16 +420,21 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
     // Column k: pivot sqrt, quotients L[i][k] = a[i][k] / pivot into q (one barrier), then the trailing
     // update a[i][c] -= L[i][k] L[c][k] (second barrier).  Each entry still receives its subtractions
     // in increasing k, as the reference's row loop applies them.
+    unsigned long long pinned = 0ull;   // the same in every thread
     for (int k = 0; k < n; ++k) {
         const double v = a[k * n + k];
-        if (v <= 0.0 || !isfinite(v)) {
+        // a pinned unknown gets a zero column: the trailing update below then subtracts exact zeros
+        const bool pin_k = pin && isfinite(v) && fabs(v) <= kPinRatio * assembled[k];
+        if (!pin_k && (v <= 0.0 || !isfinite(v))) {
             if (tid == 0)
                 failed = 1;
             break;
         }
-        const double pivot = sqrt(v);
+        if (pin_k)
+            pinned |= 1ull << k;
+        const double pivot = pin_k ? 0.0 : sqrt(v);
         if (tid > k && tid < n) {
-            const double l = a[tid * n + k] / pivot;
+            const double l = pin_k ? 0.0 : a[tid * n + k] / pivot;
             a[tid * n + k] = l;
             q[tid] = l;
         }
@@ -465,10 +474,10 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
     }
     for (int e = tid; e < n * n; e += kFactorThreads) {
         const int r = e / n, c = e - r * n;
-        factor[e] = c <= r ? a[e] : 0.0;
+        factor[e] = c <= r && !((pinned >> r) & 1ull) ? a[e] : 0.0;
     }
-    if (!ginv)
-        return;
+    if (!ginv || pinned)
+        return;   // the Gauss-Jordan inverse below assumes no pinned unknown (callers pass no ginv with pin)
     // In-place Gauss-Jordan inversion of the (SPD, so pivot-free) pinned matrix.
     for (int k = 0; k < n; ++k) {
         const double p = m[k * n + k];
@@ -523,9 +532,10 @@ __device__ __forceinline__ double lane_value(double v, int k)
 }
 
 __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const double* right, const double* down,
-    float* ginv, int n, PcgScalars* S)
+    float* ginv, int n, PcgScalars* S, int pin)
 {
     __shared__ double rowbuf[2][64];
+    __shared__ double assembled[64];   // diagonal before elimination (pinning bound)
     const int s = w + 2;
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -542,6 +552,7 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const doub
             const int pi = (yi + 1) * s + xi + 1;
             if (ci == cj) {
                 v = right[pi - 1] + right[pi] + down[pi - s] + down[pi];
+                assembled[lane] = v;
             } else if (yi == yj && (xi - xj == 1 || xj - xi == 1)) {
                 v = -right[(yi + 1) * s + min(xi, xj) + 1];
             } else if (xi == xj && (yi - yj == 1 || yj - yi == 1)) {
@@ -551,6 +562,9 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const doub
         a[r] = v;
     }
     bool failed = false;
+    // Solid masks: an unknown whose Schur pivot vanishes is pinned -- never swept, and its row and
+    // column of the inverse are zero (the swept block is then the inverse of the other unknowns' matrix)
+    unsigned long long pinned = 0ull;   // the same in every thread
     for (int kb = 0; kb < 4 && !failed; ++kb) {
 #pragma unroll
         for (int r0 = 0; r0 < kSweepRows; ++r0) {
@@ -560,8 +574,12 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const doub
             double* row = rowbuf[k & 1];
             if (wv == kb)
                 row[lane] = a[r0];
-            __syncthreads();
+            __syncthreads();   // also orders the assembled[] writes before the first read
             const double d = row[k];
+            if (pin && isfinite(d) && fabs(d) <= kPinRatio * assembled[k]) {
+                pinned |= 1ull << k;
+                continue;
+            }
             if (!(d > 0.0) || !isfinite(d)) {
                 failed = true;   // d is the same in every thread: the whole block leaves together
                 break;
@@ -589,7 +607,7 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const doub
     for (int r = 0; r < kSweepRows; ++r) {
         const int i = wv * kSweepRows + r;
         if (i < n && lane < n)
-            ginv[i * n + lane] = (float)a[r];
+            ginv[i * n + lane] = ((pinned >> i) & 1ull) || ((pinned >> lane) & 1ull) ? 0.f : (float)a[r];
     }
 }
 
@@ -742,7 +760,8 @@ template <int ITER> __device__ void tail_residual_n(const TailLevel& T, const Ta
         const float c = F.sol[i];
         const float product = (F.right[i - 1] * (c - F.sol[i - 1]) + F.right[i] * (c - F.sol[i + 1])) +
                               (F.down[i - s] * (c - F.sol[i - s]) + F.down[i] * (c - F.sol[i + s]));
-        rv[it] = F.rhs[i] - product;
+        const bool active = (F.right[i - 1] + F.right[i]) + (F.down[i - s] + F.down[i]) > 0.f;
+        rv[it] = active ? F.rhs[i] - product : 0.f;   // cells without conductance are not restricted
     }
 #pragma unroll
     for (int it = 0; it < ITER; ++it)
@@ -1071,7 +1090,8 @@ __device__ void wg_res_restrict(const LevelView& F, const float* rhs, const doub
         const float product = (F.right[i - 1] * (c - sol[i - 1]) + F.right[i] * (c - sol[i + 1])) +
                               (F.down[i - s] * (c - sol[i - s]) + F.down[i] * (c - sol[i + s]));
         const float b = shift ? (float)((double)rhs[i] - sh) : rhs[i];
-        F.res[i] = b - product;
+        const bool active = (F.right[i - 1] + F.right[i]) + (F.down[i - s] + F.down[i]) > 0.f;
+        F.res[i] = active ? b - product : 0.f;   // cells without conductance are not restricted
     }
     __syncthreads();
     for (int e = threadIdx.x; e < C.w * C.h; e += kBT) {
@@ -1382,20 +1402,20 @@ void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const floa
 }
 
 void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, const double* down64,
-    double* factor, float* ginv, int n, PcgScalars* S)
+    double* factor, float* ginv, int n, PcgScalars* S, int pin)
 {
     if (n <= 0)
         return;
     hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(kFactorThreads), 0, st, w, h, right64, down64, factor, ginv, n,
-        S);
+        S, pin);
 }
 
 void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, const double* down64, float* ginv,
-    int n, PcgScalars* S)
+    int n, PcgScalars* S, int pin)
 {
     if (n <= 0)
         return;
-    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, st, w, h, right64, down64, ginv, n, S);
+    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, st, w, h, right64, down64, ginv, n, S, pin);
 }
 
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status)

@@ -341,6 +341,22 @@ class FluidGridHIP:
     def setTransport(self, mode: int):
         call("fs_set_transport", self._h, int(mode))
 
+    # -- solid obstacles (extension: the reference's only walls are the domain border) --
+    def setSolidMask(self, mask):
+        """W*H cells, nonzero = solid; None removes the mask (fs_set_solid_mask)."""
+        if mask is None:
+            call("fs_set_solid_mask", self._h, None)
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+        if m.size != self.width * self.height:
+            raise ValueError("solid mask dimensions do not match the grid")
+        call("fs_set_solid_mask", self._h, m.ctypes.data)
+
+    def solidCells(self) -> int:
+        n = C.c_longlong()
+        call("fs_solid_cells", self._h, C.byref(n))
+        return n.value
+
     # -- state (FluidGridBase.h:118-119) --
     def _sizes(self):
         W, H = self.width, self.height

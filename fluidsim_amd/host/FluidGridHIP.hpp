@@ -218,6 +218,20 @@ public:
     }
     void setTransport(fs_transport mode) { check(fs_set_transport(grid_, mode)); }
 
+    // Solid obstacles (no reference counterpart): W*H cells, nonzero = solid; an empty vector removes them.
+    void setSolidMask(const std::vector<unsigned char>& mask)
+    {
+        if (!mask.empty() && mask.size() != static_cast<size_t>(width_) * height_)
+            throw std::invalid_argument("solid mask dimensions do not match the grid");
+        check(fs_set_solid_mask(grid_, mask.empty() ? nullptr : mask.data()));
+    }
+    long long solidCells() const
+    {
+        long long n = 0;
+        check(fs_solid_cells(grid_, &n));
+        return n;
+    }
+
     // FluidGridCPU::upload (FluidGridCPU.cpp:198-251): empty vectors mean zero; sizes must match.
     void upload(const State& state)
     {

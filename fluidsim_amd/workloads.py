@@ -153,6 +153,81 @@ def vortex_box(W: int = 1024, H: int = 1024, seed: int = 2, vortices: int = 64, 
                     {"seed": seed, "vortices": vortices, "sources": sources})
 
 
+def obstacle_mask(W: int, H: int, rng: SplitMix64, shapes: int = 40, coverage: float = 0.15) -> np.ndarray:
+    """Seeded solid mask of `shapes` circles and axis-aligned boxes (alternating), centres uniform in
+    the domain, base radii / half-sides U[0.5, 1.5] of a common scale that is chosen (bisection) so the
+    union covers `coverage` of the cells; fluid pockets cut off from the largest fluid region are filled,
+    so the fluid stays one 4-connected region.  Returns W*H uint8 (1 = solid)."""
+    from scipy import ndimage
+
+    cx = rng.uniform(shapes) * W
+    cy = rng.uniform(shapes) * H
+    a = rng.uniform(shapes, 0.5, 1.5)
+    b = rng.uniform(shapes, 0.5, 1.5)
+    xs = np.arange(W)[None, :] + 0.5
+    ys = np.arange(H)[:, None] + 0.5
+
+    def union(scale):
+        m = np.zeros((H, W), dtype=bool)
+        for k in range(shapes):
+            ra, rb = a[k] * scale, b[k] * scale
+            x0, x1 = max(0, int(cx[k] - ra) - 1), min(W, int(cx[k] + ra) + 2)
+            y0, y1 = max(0, int(cy[k] - rb) - 1), min(H, int(cy[k] + rb) + 2)
+            if x1 <= x0 or y1 <= y0:
+                continue
+            dx = xs[:, x0:x1] - cx[k]
+            dy = ys[y0:y1, :] - cy[k]
+            if k % 2 == 0:
+                inside = dx * dx + dy * dy < ra * ra
+            else:
+                inside = (np.abs(dx) < ra) & (np.abs(dy) < rb)
+            m[y0:y1, x0:x1] |= inside
+        return m
+
+    lo, hi = 0.0, 0.5 * max(W, H)
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if union(mid).mean() < coverage:
+            lo = mid
+        else:
+            hi = mid
+    solid = union(hi)
+    labels, count = ndimage.label(~solid)
+    if count > 1:
+        sizes = np.bincount(labels.ravel())
+        sizes[0] = 0
+        solid |= labels != int(np.argmax(sizes))
+    return solid.astype(np.uint8).ravel()
+
+
+def obstacle_tank(W: int = 2048, H: int = 2048, seed: int = 3, shapes: int = 40, coverage: float = 0.15,
+                  fill: float = 0.7, vortices: int = 64, sources: int = 16) -> Workload:
+    """Tank with a seeded obstacle mask (`obstacle_mask`, about `coverage` of the cells solid), liquid
+    f = 1 in the bottom `fill` of the rows (water 100 / air 0.1: a free surface with a 1000:1 density
+    jump), and the vortex-box velocity law (Gaussian vortices and sources, walls closed); the mask is
+    in meta["solid"].  dt = 1/120."""
+    h = 1.0 / max(W, H)
+    rng = SplitMix64(seed)
+    solid = obstacle_mask(W, H, rng, shapes, coverage)
+    n = vortices + sources
+    cx = rng.uniform(n) * (W * h)
+    cy = rng.uniform(n) * (H * h)
+    radius = rng.uniform(n, 0.01, 0.05)
+    strength = rng.normal(n)
+    u = np.zeros((W + 1) * H)
+    v = np.zeros(W * (H + 1))
+    for k in range(n):
+        add_gaussian(u, v, W, H, h, cx[k], cy[k], radius[k], strength[k],
+                     "vortex" if k < vortices else "source")
+    zero_walls(u, v, W, H)
+    ys = np.arange(H)[:, None] + 0.5
+    fraction = np.broadcast_to((ys >= (1.0 - fill) * H).astype(np.float64), (H, W)).ravel().copy()
+    fraction[solid != 0] = 0.0
+    lm, gm = two_phase_masses(fraction, h)
+    return Workload("obstacle_tank", W, H, h, [100.0], [0.1], 9.81, 1.0 / 120.0, fraction, u, v, lm, gm,
+                    {"seed": seed, "solid": solid, "solid_fraction": float(solid.mean())})
+
+
 def sloped_pool(W: int, H: int, cell_size: float = 4.0, seed: int = 0) -> Workload:
     """The reference tests' sloped pool fraction clamp(y + 1 - (0.35 x + 0.2 H), 0, 1)
     (tests/FluidGridTests.cpp:629-641) with u = 3 sin(0.07 i), v = 4 cos(0.11 i)."""

@@ -135,6 +135,19 @@ fs_status fs_capillary_step_limit(const fs_grid* grid, double* limit);
 fs_status fs_set_preconditioner(fs_grid* grid, fs_preconditioner kind);
 fs_status fs_set_transport(fs_grid* grid, fs_transport mode);
 
+/* Solid obstacles.  No counterpart in the reference, whose only walls are the domain border
+ * (FluidGridCPU.cpp:1050-1061, 1170-1185); this generalises that wall rule.  `mask` holds W*H bytes,
+ * nonzero = solid cell; NULL removes the mask.  Every face with a solid cell on either side gets zero
+ * pressure face weight and its velocity is zeroed by the projection's wall step, so no flow enters a
+ * solid.  The fluid (non-solid) cells must be non-empty and form one 4-connected region, else
+ * FS_E_INVALID_ARGUMENT (an enclosed pocket would add a null space per pocket).  Coarse multigrid
+ * unknowns left without conductance, or cut off by coarsening, are pinned at zero by the coarse solve.
+ * Viscosity, transport and surface tension see the zeroed face velocities; the mask is configuration,
+ * not state: checkpoints do not carry it. */
+fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask);
+/* Number of solid cells of the current mask (0 without one). */
+fs_status fs_solid_cells(const fs_grid* grid, long long* count);
+
 /* upload(const State&) (FluidGridBase.h:118, FluidGridCPU.cpp:198-251).  NULL means an empty
  * vector (zeros).  Sizes are implied by the grid and the configured mixtures.  Resets the pressure
  * history. */

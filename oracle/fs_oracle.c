@@ -104,7 +104,15 @@ struct orc_mg {
     int n;          /* coarse unknowns = w*h - 1 of the last level */
     double* factor; /* n*n */
     double* crhs;   /* n */
+    int pin_singular;        /* solid-mask extension: pin singular coarse unknowns instead of failing */
+    unsigned char* pinned;   /* n: unknown held at zero by the coarse solve */
 };
+
+/* Solid-mask extension (no counterpart in the reference, whose grids have no solids): a coarse
+ * Cholesky pivot at or below this fraction of its row's assembled diagonal marks the unknown as the
+ * last one of a disconnected or fully blocked region; it is pinned to zero like the reference's
+ * first cell.  Without a mask every pivot of these weighted Laplacians is far above the bound. */
+#define ORC_PIN_RATIO 1e-9
 
 /* Level::Level, PressureMultigrid.cpp:24-36 */
 static void level_init(orc_level* L, int w, int h, int spacing, int dw, int dh)
@@ -198,6 +206,7 @@ orc_mg* orc_mg_create(int width, int height)
     mg->n = last->w * last->h - 1;
     mg->factor = dalloc((size_t)imax(mg->n, 0) * imax(mg->n, 0));
     mg->crhs = dalloc((size_t)imax(mg->n, 0));
+    mg->pinned = (unsigned char*)calloc((size_t)imax(mg->n, 1), 1);
     return mg;
 }
 
@@ -210,6 +219,7 @@ void orc_mg_destroy(orc_mg* mg)
     free(mg->lev);
     free(mg->factor);
     free(mg->crhs);
+    free(mg->pinned);
     free(mg);
 }
 
@@ -265,13 +275,16 @@ static void coarsen(const orc_level* F, orc_level* C)
     rebuild_diagonal(C);
 }
 
-/* factorCoarsest, PressureMultigrid.cpp:300-345: first cell pinned, dense Cholesky. */
+/* factorCoarsest, PressureMultigrid.cpp:300-345: first cell pinned, dense Cholesky.  With
+ * pin_singular (solid masks) a vanishing pivot pins its unknown: its row and column of the factor
+ * are zero and the solve holds it at zero. */
 static int factor_coarsest(orc_mg* mg)
 {
     const orc_level* L = &mg->lev[mg->nlev - 1];
     const int n = mg->n;
     if (n > 0)
         memset(mg->factor, 0, sizeof(double) * (size_t)n * n);
+    memset(mg->pinned, 0, (size_t)imax(n, 1));
     double* f = mg->factor;
     for (int cell = 1; cell <= n; ++cell) {
         const int x = cell % L->w;
@@ -287,14 +300,21 @@ static int factor_coarsest(orc_mg* mg)
     for (int row = 0; row < n; ++row) {
         for (int col = 0; col <= row; ++col) {
             double value = f[row * n + col];
+            const double assembled = value;
             for (int k = 0; k < col; ++k)
                 value -= f[row * n + k] * f[col * n + k];
             if (row == col) {
+                if (mg->pin_singular && isfinite(value) && fabs(value) <= ORC_PIN_RATIO * assembled) {
+                    mg->pinned[row] = 1;
+                    for (int k = 0; k <= row; ++k)
+                        f[row * n + k] = 0.0;
+                    continue;
+                }
                 if (value <= 0.0 || !isfinite(value))
                     return 0;
                 f[row * n + col] = sqrt(value);
             } else
-                f[row * n + col] = value / f[col * n + col];
+                f[row * n + col] = mg->pinned[col] ? 0.0 : value / f[col * n + col];
         }
     }
     return 1;
@@ -346,7 +366,9 @@ static void restrict_residual(orc_level* F, orc_level* C)
                                        F->right[i] * (center - F->sol[i + 1])) +
                                    (F->down[i - s] * (center - F->sol[i - s]) +
                                        F->down[i] * (center - F->sol[i + s]));
-            F->res[i] = F->rhs[i] - product;
+            /* a cell without conductance (only solid masks make one) has a zero row: its residual
+             * is not restricted, so the V-cycle neither reads nor writes it and stays symmetric */
+            F->res[i] = F->inv_diag[i] > 0.0 ? F->rhs[i] - product : 0.0;
         }
     }
     for (int y = 0; y < C->h; ++y) {
@@ -399,13 +421,13 @@ static void solve_coarsest(orc_mg* mg)
         double value = L->rhs[(cell / L->w + 1) * L->stride + cell % L->w + 1] - mean;
         for (int col = 0; col < row; ++col)
             value -= f[row * n + col] * c[col];
-        c[row] = value / f[row * n + row];
+        c[row] = mg->pinned[row] ? 0.0 : value / f[row * n + row];
     }
     for (int row = n - 1; row >= 0; --row) {
         double value = c[row];
         for (int col = row + 1; col < n; ++col)
             value -= f[col * n + row] * c[col];
-        c[row] = value / f[row * n + row];
+        c[row] = mg->pinned[row] ? 0.0 : value / f[row * n + row];
     }
     /* std::accumulate: sequential */
     double acc = 0.0;
@@ -503,6 +525,19 @@ void orc_mg_level_restrict(const orc_mg* mg, int level, int axis, int* count, in
     }
 }
 
+void orc_mg_set_pin(orc_mg* mg, int pin) { mg->pin_singular = pin != 0; }
+
+int orc_mg_coarse_pinned(const orc_mg* mg, unsigned char* pinned)
+{
+    int count = 0;
+    for (int i = 0; i < mg->n; ++i) {
+        count += mg->pinned[i];
+        if (pinned)
+            pinned[i] = mg->pinned[i];
+    }
+    return count;
+}
+
 int orc_mg_coarse_factor(const orc_mg* mg, double* factor)
 {
     if (factor && mg->n > 0)
@@ -528,6 +563,7 @@ struct orc_grid {
     double *lmass, *gmass;
     double *density, *inv_u, *inv_v, *inv_diag;
     double *rhs, *res, *cor, *dir, *prod;
+    unsigned char* solid;  /* solid-mask extension: W*H, nonzero = solid cell (NULL: none) */
     orc_mg* mg;
     int density_valid, coeff_valid, mg_valid;
 };
@@ -568,6 +604,7 @@ void orc_grid_destroy(orc_grid* g)
     free(g->lmass); free(g->gmass);
     free(g->density); free(g->inv_u); free(g->inv_v); free(g->inv_diag);
     free(g->rhs); free(g->res); free(g->cor); free(g->dir); free(g->prod);
+    free(g->solid);
     orc_mg_destroy(g->mg);
     free(g);
 }
@@ -575,6 +612,33 @@ void orc_grid_destroy(orc_grid* g)
 void orc_set_origin(orc_grid* g, double x, double y) { g->ox = x; g->oy = y; }
 void orc_set_gravity(orc_grid* g, double gy) { g->gravity_y = gy; }
 void orc_set_preconditioner(orc_grid* g, int kind) { g->preconditioner = kind; }
+
+/* Solid-mask extension.  A face with a solid cell on either side is closed like a domain wall: zero
+ * pressure face weight, zero velocity after the projection's wall step.  NULL removes the mask. */
+void orc_set_solid(orc_grid* g, const unsigned char* mask)
+{
+    free(g->solid);
+    g->solid = NULL;
+    if (mask) {
+        const size_t N = (size_t)g->W * g->H;
+        g->solid = (unsigned char*)malloc(N);
+        for (size_t i = 0; i < N; ++i)
+            g->solid[i] = mask[i] ? 1 : 0;
+    }
+    g->mg->pin_singular = mask != NULL;
+    orc_invalidate_caches(g);
+}
+
+/* Closed faces of the solid mask: the face left of / above cell (x, y). */
+static int solid_left(const orc_grid* g, int x, int y)
+{
+    return g->solid && x > 0 && x < g->W && (g->solid[(size_t)y * g->W + x] || g->solid[(size_t)y * g->W + x - 1]);
+}
+static int solid_up(const orc_grid* g, int x, int y)
+{
+    return g->solid && y > 0 && y < g->H &&
+           (g->solid[(size_t)y * g->W + x] || g->solid[(size_t)(y - 1) * g->W + x]);
+}
 
 void orc_invalidate_caches(orc_grid* g)
 {
@@ -670,6 +734,15 @@ static void rebuild_coefficients(orc_grid* g)
     }
     for (int x = 0; x < W; ++x)
         g->inv_v[(size_t)H * W + x] = 0.0;
+    if (g->solid) {
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                if (solid_left(g, x, y))
+                    g->inv_u[(size_t)y * (W + 1) + x] = 0.0;
+                if (solid_up(g, x, y))
+                    g->inv_v[(size_t)y * W + x] = 0.0;
+            }
+    }
     for (int y = 0; y < H; ++y) {
         const double* hz = g->inv_u + (size_t)y * (W + 1);
         const double* vt = g->inv_v + (size_t)y * W;
@@ -680,6 +753,14 @@ static void rebuild_coefficients(orc_grid* g)
         }
     }
     g->coeff_valid = 1;
+}
+
+orc_mg* orc_grid_multigrid(orc_grid* g)
+{
+    rebuild_coefficients(g);
+    if (!g->mg_valid)
+        g->mg_valid = orc_mg_rebuild(g->mg, g->inv_u, g->inv_v);
+    return g->mg_valid ? g->mg : NULL;
 }
 
 void orc_density(orc_grid* g, double* density)
@@ -1009,6 +1090,15 @@ orc_result orc_project(orc_grid* g, double dt, int max_iterations, double tol)
     for (int x = 0; x < W; ++x) {
         g->v[x] = 0.0;
         g->v[H * W + x] = 0.0;
+    }
+    if (g->solid) {
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                if (solid_left(g, x, y))
+                    g->u[(size_t)y * (W + 1) + x] = 0.0;
+                if (solid_up(g, x, y))
+                    g->v[(size_t)y * W + x] = 0.0;
+            }
     }
     form_rhs(g, dt);
     result.iterations = solve_pressure(g, max_iterations, tol * scale, &result);

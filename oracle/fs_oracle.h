@@ -59,6 +59,10 @@ void orc_mg_level_prolong(const orc_mg* mg, int level, int axis, int* first, int
 void orc_mg_level_restrict(const orc_mg* mg, int level, int axis, int* count, int* idx, double* wt);
 /* Coarsest dense Cholesky factor (n*n, lower triangle meaningful); returns n. */
 int orc_mg_coarse_factor(const orc_mg* mg, double* factor);
+/* Solid-mask extension: pin vanishing coarse pivots (set by orc_set_solid for a grid's hierarchy). */
+void orc_mg_set_pin(orc_mg* mg, int pin);
+/* Pinned coarse unknowns of the last factorisation (n flags, may be NULL); returns their count. */
+int orc_mg_coarse_pinned(const orc_mg* mg, unsigned char* pinned);
 
 /* ---- FluidGridCPU subset (FluidGridCPU.h:15-225) ---- */
 orc_grid* orc_grid_create(int width, int height, double cell_size);
@@ -75,6 +79,12 @@ int orc_upload(orc_grid* g, const double* fraction, const double* u, const doubl
 void orc_download(const orc_grid* g, double* fraction, double* u, double* v, double* pressure,
     double* liquid_mass, double* gas_mass);
 void orc_invalidate_caches(orc_grid* g);
+/* Solid-mask extension (no counterpart in the reference): W*H bytes, nonzero = solid; NULL clears.
+ * Faces touching a solid cell get zero pressure weight and zero velocity in project(); vanishing
+ * coarse pivots (disconnected or fully blocked coarse regions) are pinned instead of failing. */
+void orc_set_solid(orc_grid* g, const unsigned char* mask);
+/* The grid's hierarchy, rebuilt from the current coefficients if stale; NULL if the factorisation fails. */
+orc_mg* orc_grid_multigrid(orc_grid* g);
 void orc_apply_gravity(orc_grid* g, double dt);
 void orc_add_vortex(orc_grid* g, double cx, double cy, double radius, double strength);
 double orc_max_divergence(const orc_grid* g);

@@ -66,6 +66,9 @@ def lib():
         "orc_mg_level_prolong": (None, [vp, C.c_int, C.c_int, _ip, _ip, _dp]),
         "orc_mg_level_restrict": (None, [vp, C.c_int, C.c_int, _ip, _ip, _dp]),
         "orc_mg_coarse_factor": (C.c_int, [vp, _dp]),
+        "orc_mg_set_pin": (None, [vp, C.c_int]),
+        "orc_mg_coarse_pinned": (C.c_int, [vp, C.c_void_p]),
+        "orc_grid_multigrid": (vp, [vp]),
         "orc_grid_create": (vp, [C.c_int, C.c_int, C.c_double]),
         "orc_grid_destroy": (None, [vp]),
         "orc_set_origin": (None, [vp, C.c_double, C.c_double]),
@@ -75,6 +78,7 @@ def lib():
         "orc_upload": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_download": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_invalidate_caches": (None, [vp]),
+        "orc_set_solid": (None, [vp, C.c_void_p]),
         "orc_apply_gravity": (None, [vp, C.c_double]),
         "orc_add_vortex": (None, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
         "orc_max_divergence": (C.c_double, [vp]),
@@ -130,10 +134,27 @@ class Multigrid:
         self.w, self.h = int(width), int(height)
         self._h = lib().orc_mg_create(self.w, self.h)
 
+    @classmethod
+    def borrowed(cls, handle, width, height, owner):
+        """A view of a hierarchy owned by `owner` (an OracleGrid); never destroyed here."""
+        m = cls.__new__(cls)
+        m.w, m.h, m._h, m._owner = int(width), int(height), handle, owner
+        return m
+
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and not hasattr(self, "_owner"):
             lib().orc_mg_destroy(self._h)
             self._h = None
+
+    def set_pin(self, pin=True):
+        """Solid-mask extension: pin vanishing coarse pivots instead of failing."""
+        lib().orc_mg_set_pin(self._h, int(bool(pin)))
+
+    def pinned(self):
+        n = lib().orc_mg_coarse_factor(self._h, None)
+        flags = np.zeros(max(n, 1), np.uint8)
+        lib().orc_mg_coarse_pinned(self._h, flags.ctypes.data)
+        return flags[:max(n, 0)].astype(bool)
 
     def rebuild(self, inv_u, inv_v):
         inv_u, inv_v = _f64(inv_u), _f64(inv_v)
@@ -241,6 +262,21 @@ class OracleGrid:
 
     def invalidate_caches(self):
         lib().orc_invalidate_caches(self._h)
+
+    def multigrid(self):
+        """The grid's own hierarchy, rebuilt from the current coefficients; None if it fails."""
+        h = lib().orc_grid_multigrid(self._h)
+        return Multigrid.borrowed(h, self.width, self.height, self) if h else None
+
+    def set_solid(self, mask):
+        """Solid-mask extension: W*H cells, nonzero = solid; None removes the mask."""
+        if mask is None:
+            lib().orc_set_solid(self._h, None)
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+        if m.size != self.N:
+            raise ValueError("solid mask dimensions do not match the grid")
+        lib().orc_set_solid(self._h, m.ctypes.data)
 
     def apply_gravity(self, dt):
         lib().orc_apply_gravity(self._h, dt)
