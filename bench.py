@@ -7,10 +7,11 @@ one complete projection -- rhs formation, coefficient and multigrid-hierarchy re
 CG to max|b - A p| <= 1e-6 max|b|, velocity correction and FP64 divergence check -- starting from the
 same device-resident input every time (restored from an HBM snapshot, pressure history reset).
 
-Two further legs are appended to the same JSON line: "batched" (config C4: 256 mixed-size grids
-projected one after another, then all at once with fs_batch_project) and "serving" (config C5: a
-Poisson stream of full simulations served one at a time, then through the serving loop at each
-arrival rate).  --batch-instances 0 / --serving-requests 0 skip them.
+Three further legs are appended to the same JSON line: "mixed_precision" (config C3: the 2048 x 2048
+obstacle tank projected with FP32, binary16 and bfloat16 multigrid levels), "batched" (config C4: 256
+mixed-size grids projected one after another, then all at once with fs_batch_project) and "serving"
+(config C5: a Poisson stream of full simulations served one at a time, then through the serving loop
+at each arrival rate).  --precision-size 0 / --batch-instances 0 / --serving-requests 0 skip them.
 
 Prints one JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
 """
@@ -156,6 +157,78 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048):
     }
 
 
+def smoother_model_bytes(coef_bytes, coarse_bytes):
+    """Algorithmic bytes per fine cell of the finest post-smoothing launch: four sweeps of (read solution
+    and rhs, FP32 on the finest level, read two conductances, write the solution) plus the bilinear
+    prolongation (read a quarter coarse value, read and write the fine solution)."""
+    return 4 * (12 + 2 * coef_bytes) + 8 + 0.25 * coarse_bytes
+
+
+def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400):
+    """Config C3 on one GPU: the obstacle tank (40 seeded circles and boxes over 15% of the cells, liquid
+    in the bottom 70%, vortex velocity) projected to 1e-6 of its initial max divergence with each
+    multigrid storage mode, from the same device-resident input every solve.  Reports solves/s, PCG
+    iterations, the finest post-smoother's launch time and model bytes, and each result's distance from
+    the FP32 result (the FP64-oracle errors are checked by the GPU tests).  The bfloat16-vector mode gets
+    `stall_budget` iterations."""
+    from fluidsim_amd import FS_MG_BF16, FS_MG_BF16_VECTORS, FS_MG_FP16, FS_MG_FP32, FluidGridHIP
+    from fluidsim_amd import workloads as WL
+
+    t_start = time.perf_counter()
+
+    def note(msg):
+        print(f"[precision leg {time.perf_counter() - t_start:7.2f} s] {msg}", file=sys.stderr, flush=True)
+
+    wl = WL.obstacle_tank(size, size, seed=seed)
+    mask = wl.meta["solid"]
+    note(f"generated the {size}^2 obstacle tank ({mask.mean():.3f} solid)")
+    modes = [("fp32", FS_MG_FP32, 4, 4), ("fp16", FS_MG_FP16, 2, 4), ("bf16", FS_MG_BF16, 2, 4),
+             ("bf16_vectors", FS_MG_BF16_VECTORS, 2, 2)]
+    out = {"workload": f"obstacle_tank {size}x{size}: 40 circles/boxes over {mask.mean():.3f} of the cells, "
+                       f"liquid in the bottom 70% (1000:1), vortex velocity, seed {seed}; one projection per "
+                       "solve to 1e-6 of the initial max divergence", "modes": {}}
+    ref_u = None
+    for name, mode, coef_bytes, coarse_bytes in modes:
+        g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+        WL.load_into(g, wl)
+        g.setSolidMask(mask)
+        g.setMultigridPrecision(mode)
+        tau = 1e-6 * g.maxDivergence()
+        budget = stall_budget if mode == FS_MG_BF16_VECTORS else 2000
+        g.snapshotSave()
+        g.snapshotRestore()
+        g.project(wl.dt, budget, tau)        # warm-up: hierarchy, graph capture
+        results = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.snapshotRestore()
+            results.append(g.project(wl.dt, budget, tau))
+        elapsed = time.perf_counter() - t0
+        u = g.download().u
+        post_ms = g.timeKernel(0, kernel_reps)
+        model = smoother_model_bytes(coef_bytes, coarse_bytes)
+        r = results[-1]
+        entry = {
+            "solves_per_s": steps / elapsed,
+            "ms_per_solve": elapsed / steps * 1e3,
+            "pcg_iterations": r.iterations,
+            "converged": bool(all(x.converged for x in results)),
+            "max_divergence_over_tol": r.max_divergence / tau if tau > 0 else 0.0,
+            "fine_post_smoother_us": post_ms * 1e3,
+            "fine_post_smoother_model_bytes_per_cell": model,
+            "fine_post_smoother_model_gbs": model * size * size / (post_ms * 1e-3) / 1e9,
+        }
+        if ref_u is None:
+            ref_u = u
+        else:
+            entry["velocity_rel_l2_vs_fp32"] = float(np.linalg.norm(u - ref_u) / np.linalg.norm(ref_u))
+        out["modes"][name] = entry
+        note(f"{name}: {entry['ms_per_solve']:.1f} ms/solve, {r.iterations} iterations, "
+             f"converged {entry['converged']}, post-smoother {entry['fine_post_smoother_us']:.1f} us")
+        g.destroy()
+    return out
+
+
 def batch_leg(count, seed=4, visc_sample=16):
     """Config C4 on one GPU: `count` seeded instances (sides log-uniform 64..512, 30-90% liquid plus a
     drop, vortex velocity), each projected once to 1e-6 of its own initial max divergence.  Timed twice
@@ -263,6 +336,9 @@ def main():
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=256, help="size of the C4 batched-solve leg (0: skip)")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--precision-size", type=int, default=2048, help="side of the C3 mixed-precision leg (0: skip)")
+    ap.add_argument("--precision-steps", type=int, default=3)
+    ap.add_argument("--precision-only", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -277,6 +353,9 @@ def main():
     rates = [float(x) for x in args.serving_rates.split(",") if x]
     if args.batch_only:
         print(json.dumps(batch_leg(args.batch_instances or 256)), flush=True)
+        return
+    if args.precision_only:
+        print(json.dumps(precision_leg(args.precision_size or 2048, steps=args.precision_steps)), flush=True)
         return
     if args.serving_only:
         print(json.dumps(serving_leg(args.serving_requests or 24, rates, args.serving_steps,
@@ -365,6 +444,8 @@ def main():
     }
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+    if args.precision_size > 0:
+        out["mixed_precision"] = precision_leg(args.precision_size, steps=args.precision_steps)
     if args.batch_instances > 0:
         grid.destroy()
         out["batched"] = batch_leg(args.batch_instances)

@@ -3,14 +3,14 @@
 The compute path is libfluidsim_hip.so (C ABI in include/fluidsim_hip.h).  `FluidGridHIP` mirrors the
 reference's FluidGridBase method set over that ABI.
 """
-from ._abi import (FS_PRECOND_DIAGONAL, FS_PRECOND_MULTIGRID, FS_TRANSPORT_NONE, FS_TRANSPORT_REFERENCE,
-                   FS_TRANSPORT_SEMI_LAGRANGIAN, FluidSimError, lib)
+from ._abi import (FS_MG_BF16, FS_MG_BF16_VECTORS, FS_MG_FP16, FS_MG_FP32, FS_PRECOND_DIAGONAL, FS_PRECOND_MULTIGRID,
+                   FS_TRANSPORT_NONE, FS_TRANSPORT_REFERENCE, FS_TRANSPORT_SEMI_LAGRANGIAN, FluidSimError, lib)
 from .grid import (Context, FluidCheckpoint, FluidGridHIP, FluidGridState, FluidMixture, FrameStatistics,
                    MixtureComponent, ProjectionResult, StepTimings, batch_project, default_context,
                    viscosity_solve)
 
 __all__ = [
-    "FS_PRECOND_DIAGONAL", "FS_PRECOND_MULTIGRID", "FS_TRANSPORT_NONE", "FS_TRANSPORT_REFERENCE",
+    "FS_MG_BF16", "FS_MG_BF16_VECTORS", "FS_MG_FP16", "FS_MG_FP32", "FS_PRECOND_DIAGONAL", "FS_PRECOND_MULTIGRID", "FS_TRANSPORT_NONE", "FS_TRANSPORT_REFERENCE",
     "FS_TRANSPORT_SEMI_LAGRANGIAN", "FluidSimError", "lib", "Context", "FluidGridHIP", "FluidGridState",
     "FluidMixture", "FluidCheckpoint", "FrameStatistics", "MixtureComponent", "ProjectionResult", "StepTimings", "batch_project", "default_context", "viscosity_solve",
 ]

@@ -25,6 +25,11 @@ FS_TRANSPORT_SEMI_LAGRANGIAN = 2
 FS_PRECOND_DIAGONAL = 0
 FS_PRECOND_MULTIGRID = 1
 
+FS_MG_FP32 = 0
+FS_MG_FP16 = 1
+FS_MG_BF16 = 2
+FS_MG_BF16_VECTORS = 3
+
 
 class FsProjectionResult(C.Structure):
     _fields_ = [
@@ -149,6 +154,8 @@ SIGNATURES = {
     "fs_set_preconditioner": [_vp, C.c_int],
     "fs_set_transport": [_vp, C.c_int],
     "fs_set_solid_mask": [_vp, C.c_void_p],
+    "fs_set_multigrid_precision": [_vp, C.c_int],
+    "fs_get_multigrid_precision": [_vp, C.POINTER(C.c_int)],
     "fs_solid_cells": [_vp, C.POINTER(C.c_longlong)],
     "fs_upload": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "fs_download": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],

@@ -38,16 +38,24 @@ def hipcc():
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
     cc = hipcc()
-    for src in SOURCES:
+
+    def compile_one(src):
         path = os.path.join(CSRC, src)
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         cmd = [cc, *HIPCC_FLAGS, *lang, "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        proc = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    # one hipcc per source, a few at a time (each is single-threaded)
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "4")), os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        results = list(pool.map(compile_one, SOURCES))
+    objs = []
+    for src, obj, proc in results:
         if proc.returncode != 0:
             sys.stderr.write(proc.stdout + proc.stderr)
             raise RuntimeError(f"hipcc failed on {src}")

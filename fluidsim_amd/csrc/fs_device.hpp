@@ -68,16 +68,23 @@ struct PcgScalars {
     double rz_slot[2];
 };
 
-// One multigrid level as the FP32 smoothing kernels see it.
-struct LevelView {
+// 16-bit storage formats of multigrid levels (arithmetic stays FP32): IEEE binary16 and bfloat16
+// (the upper half of an FP32 word).  Plain structs, so host code can name them without device headers.
+struct half16 { unsigned short bits; };
+struct bf16 { unsigned short bits; };
+
+// One multigrid level as the smoothing kernels see it: conductances stored as C, the level's rhs and
+// solution vectors as V (FP32 everywhere by default).
+template <class C, class V> struct LevelViewT {
     int w, h, s;             // interior width/height, padded stride (w + 2)
-    const float* right;      // conductance of the face right of each cell (padded)
-    const float* down;       // conductance of the face below each cell (padded)
-    const float* invd;       // 1 / diagonal (padded)
-    float* rhs;              // padded
-    float* sol[2];           // ping-pong solution buffers (padded)
-    float* res;              // residual scratch (padded)
+    const C* right;          // conductance of the face right of each cell (padded)
+    const C* down;           // conductance of the face below each cell (padded)
+    const float* invd;       // 1 / diagonal (padded, FP32; unused by the 16-bit kernels)
+    V* rhs;                  // padded
+    V* sol[2];               // ping-pong solution buffers (padded)
+    V* res;                  // residual scratch (padded)
 };
+using LevelView = LevelViewT<float, float>;
 
 // Per-axis prolongation map of a fine level (first/second coarse index, weight of second).
 struct AxisProlong {

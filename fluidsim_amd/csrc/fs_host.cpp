@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -115,6 +116,9 @@ struct Level {
     DevBuf<float> right32, down32, invd32, rhs, sol0, sol1, res;
     DevBuf<int> d_pxf, d_pxs, d_pyf, d_pys, d_rxc, d_rxi, d_ryc, d_ryi;
     DevBuf<float> d_pxw, d_pyw, d_rxw, d_ryw;
+    // 16-bit copies (fs_set_multigrid_precision): conductances of a tiled level, and the bfloat16 rhs and
+    // solutions of levels 1 .. tail start
+    DevBuf<unsigned short> right16, down16, rhs16, sol16a, sol16b;
 
     int smooth_variant = 0;     // tile of k_rb_multi pre-smoothing on this level (autotuned)
     int post_variant = 0;       // tile of k_rb_multi prolongation + post-smoothing (autotuned)
@@ -210,6 +214,7 @@ struct fs_grid {
     std::vector<double> rho_l, rho_g, mu_l, mu_g;
     fs_preconditioner precond = FS_PRECOND_MULTIGRID;
     fs_transport transport = FS_TRANSPORT_REFERENCE;
+    fs_mg_precision mg_prec = FS_MG_FP32;
     bool uploaded = false;
     bool coeff_valid = false;
     bool mg_valid = false;
@@ -342,7 +347,7 @@ struct fs_grid {
         g.zdiag = zdiag.p;
         g.S = S.p;
         g.partials = partials.p;
-        g.partials_ad = partials.p + 4 * ((size_t)((W + 63) / 64) * ((H + 15) / 16));
+        g.partials_ad = partials.p + 5 * ((size_t)((W + 63) / 64) * ((H + 15) / 16));
         g.solid = solid.n ? solid.p : nullptr;
         return g;
     }
@@ -387,9 +392,9 @@ struct fs_grid {
         bytes += S.alloc(1);
         bytes += S_aux.alloc(1);
         const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
-        // region A: up to four partials per cell block; region B (after it): the operator's d.q partials,
+        // region A: up to five partials per cell block; region B (after it): the operator's d.q partials,
         // read by the update kernel while it writes region A
-        bytes += partials.alloc(5 * blocks + 64);
+        bytes += partials.alloc(6 * blocks + 64);
         build_hierarchy();
         tune_tiles();
     }
@@ -586,6 +591,7 @@ struct fs_grid {
             launch_level_diag(stream, Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p,
                 Cl.invd32.p);
         }
+        pack_levels(stream);
         const Level& last = levels.back();
         launch_coarse_inverse(stream, last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, flag,
             solid.n > 0);
@@ -593,9 +599,204 @@ struct fs_grid {
     }
     void rebuild_hierarchy() { rebuild_hierarchy(st(), S.p); }
 
+    // ---- 16-bit multigrid levels ----
+    bool active16() const { return mg_prec != FS_MG_FP32 && tail_start >= 1; }
+
+    // Allocates (or releases) the 16-bit arrays of the current mode and points the tail's top level at
+    // the storage its neighbouring tiled level uses.
+    void apply_precision()
+    {
+        const bool on = active16();
+        const bool vec16 = on && mg_prec == FS_MG_BF16_VECTORS;
+        for (int l = 0; l < (int)levels.size(); ++l) {
+            Level& L = levels[l];
+            const size_t np = L.padded();
+            auto want = [&](DevBuf<unsigned short>& b, bool need16) {
+                if (need16 && b.n != np)
+                    bytes += b.alloc(np);
+                else if (!need16 && b.n) {
+                    bytes -= b.n * sizeof(unsigned short);
+                    b.release();
+                }
+            };
+            want(L.right16, on && l < tail_start);
+            want(L.down16, on && l < tail_start);
+            const bool v = vec16 && l >= 1 && l <= tail_start;
+            want(L.rhs16, v);
+            want(L.sol16a, v);
+            want(L.sol16b, v && l < tail_start);
+        }
+        if (vec16) {
+            tail.top_rhs = levels[tail_start].rhs16.p;
+            tail.top_sol = levels[tail_start].sol16a.p;
+        } else {
+            tail.top_rhs = tail_start == 0 ? r.p : levels[tail_start].rhs.p;
+            tail.top_sol = levels[tail_start].sol0.p;
+        }
+        mg_valid = false;   // the 16-bit conductances are packed by the next hierarchy rebuild
+        drop_graph();
+    }
+
+    void pack_levels(hipStream_t stream)
+    {
+        for (int l = 0; l < tail_start && active16(); ++l) {
+            Level& L = levels[l];
+            if (mg_prec == FS_MG_FP16)
+                launch_level_pack(stream, L.w, L.h, L.right64.p, L.down64.p, (half16*)L.right16.p, (half16*)L.down16.p);
+            else
+                launch_level_pack(stream, L.w, L.h, L.right64.p, L.down64.p, (bf16*)L.right16.p, (bf16*)L.down16.p);
+        }
+    }
+
+    // Level l with C conductances and V vectors: FP32 vectors, or the bfloat16 ones of levels >= 1.
+    template <class C, class V> LevelViewT<C, V> view_t(int l)
+    {
+        Level& L = levels[l];
+        LevelViewT<C, V> v{};
+        v.w = L.w;
+        v.h = L.h;
+        v.s = L.s;
+        v.right = (const C*)L.right16.p;
+        v.down = (const C*)L.down16.p;
+        v.invd = L.invd32.p;
+        if constexpr (std::is_same<V, float>::value) {
+            v.rhs = l == 0 ? r.p : L.rhs.p;
+            v.sol[0] = L.sol0.p;
+            v.sol[1] = L.sol1.p;
+        } else {
+            v.rhs = (V*)L.rhs16.p;
+            v.sol[0] = (V*)L.sol16a.p;
+            v.sol[1] = (V*)L.sol16b.p;
+        }
+        v.res = nullptr;
+        return v;
+    }
+
+    template <class VC> CoarseLinkT<VC> link_t(int l)
+    {
+        const Level& F = levels[l];
+        const Level& Cl = levels[l + 1];
+        const VC* csol = std::is_same<VC, float>::value ? (const VC*)Cl.sol0.p : (const VC*)Cl.sol16a.p;
+        return CoarseLinkT<VC>{csol, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
+    }
+
+    // Pre-smoothing (from_zero) or prolongation + post-smoothing of tiled level l with 16-bit storage;
+    // VL is the vector type of levels >= 1.
+    template <class C, class VL> void smooth16(int l, int cur, int first_color, int from_zero, const int* status,
+        int variant)
+    {
+        const CoarseLinkT<VL> link = from_zero ? CoarseLinkT<VL>{} : link_t<VL>(l);
+        if (l == 0)
+            launch_rb_multi_t<C, float, VL>(st(), view_t<C, float>(0), cur, r.p, rmean_ptr(), first_color, from_zero,
+                link, status, variant);
+        else {
+            const LevelViewT<C, VL> L = view_t<C, VL>(l);
+            launch_rb_multi_t<C, VL, VL>(st(), L, cur, L.rhs, nullptr, first_color, from_zero, link, status, variant);
+        }
+    }
+
+    template <class C, class VL> void restrict16(int l, int cur, const int* status, int variant)
+    {
+        const Level& F = levels[l];
+        const AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
+        const AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+        if (l == 0)
+            launch_res_restrict_t<C, float, VL>(st(), view_t<C, float>(0), cur, r.p, rmean_ptr(), view_t<C, VL>(1), rx,
+                ry, status, variant);
+        else {
+            const LevelViewT<C, VL> L = view_t<C, VL>(l);
+            launch_res_restrict_t<C, VL, VL>(st(), L, cur, L.rhs, nullptr, view_t<C, VL>(l + 1), rx, ry, status,
+                variant);
+        }
+    }
+
+    template <class C, class VL> void vcycle16(const int* status)
+    {
+        const int T = tail_start;
+        for (int l = 0; l < T; ++l) {
+            smooth16<C, VL>(l, 0, 0, 1, status, levels[l].smooth_variant);
+            restrict16<C, VL>(l, 1, status, levels[l].restrict_variant);
+        }
+        launch_mg_tail(st(), tail, tail_lds, nullptr, status, std::is_same<VL, bf16>::value);
+        for (int l = T - 1; l >= 0; --l)
+            smooth16<C, VL>(l, 1, 1, 0, status, levels[l].post_variant);
+    }
+
+    template <class C, class VL> void tune_tiles16()
+    {
+        auto time = [&](auto&& launch) {
+            launch();
+            HIPC(hipEventRecord(ctx->ev0, st()));
+            for (int k = 0; k < 3; ++k)
+                launch();
+            HIPC(hipEventRecord(ctx->ev1, st()));
+            HIPC(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            return ms;
+        };
+        for (int l = 0; l < tail_start; ++l) {
+            Level& F = levels[l];
+            auto pick = [&](int count, auto&& launch) {
+                float best = std::numeric_limits<float>::infinity();
+                int choice = 0;
+                for (int v = 0; v < count; ++v) {
+                    const float ms = time([&] { launch(v); });
+                    if (ms < best) {
+                        best = ms;
+                        choice = v;
+                    }
+                }
+                return choice;
+            };
+            F.smooth_variant = pick(kSmoothVariants, [&](int v) { smooth16<C, VL>(l, 0, 0, 1, nullptr, v); });
+            F.post_variant = pick(kSmoothVariants, [&](int v) { smooth16<C, VL>(l, 1, 1, 0, nullptr, v); });
+            F.restrict_variant = pick(kRestrictVariants, [&](int v) { restrict16<C, VL>(l, 0, nullptr, v); });
+        }
+        drop_graph();
+    }
+
+    void tune_for_precision()
+    {
+        if (!active16())
+            tune_tiles();
+        else if (mg_prec == FS_MG_FP16)
+            tune_tiles16<half16, float>();
+        else if (mg_prec == FS_MG_BF16)
+            tune_tiles16<bf16, float>();
+        else
+            tune_tiles16<bf16, bf16>();
+    }
+
+    // The finest post-smoothing launch of a V-cycle in the grid's storage mode (kernel timing).
+    void fine_post_smooth(int cur)
+    {
+        if (mg_prec == FS_MG_FP16 && active16())
+            smooth16<half16, float>(0, cur, 1, 0, nullptr, levels[0].post_variant);
+        else if (mg_prec == FS_MG_BF16 && active16())
+            smooth16<bf16, float>(0, cur, 1, 0, nullptr, levels[0].post_variant);
+        else if (active16())
+            smooth16<bf16, bf16>(0, cur, 1, 0, nullptr, levels[0].post_variant);
+        else {
+            CoarseLink link{};
+            if (tail_start > 0)
+                link = link_t<float>(0);
+            launch_rb_multi(st(), view(0), cur, r.p, rmean_ptr(), 1, 0, link, nullptr, 1, levels[0].post_variant);
+        }
+    }
+
     // One V(4,4) cycle on the residual r (minus its mean) into levels[0].sol0.
     void vcycle(const int* status)
     {
+        if (active16()) {
+            if (mg_prec == FS_MG_FP16)
+                vcycle16<half16, float>(status);
+            else if (mg_prec == FS_MG_BF16)
+                vcycle16<bf16, float>(status);
+            else
+                vcycle16<bf16, bf16>(status);
+            return;
+        }
         const int T = tail_start;
         const CoarseLink none{};
         for (int l = 0; l < T; ++l) {
@@ -675,6 +876,12 @@ struct fs_grid {
     }
 
     const float* z_ptr() const { return precond == FS_PRECOND_MULTIGRID ? levels[0].sol0.p : zdiag.p; }
+    // bfloat16 level vectors round on every store, so the V-cycle is linear only to ~2^-9: the CG then
+    // takes the flexible (Polak-Ribiere) beta, which tolerates a varying preconditioner
+    int flexible() const
+    {
+        return precond == FS_PRECOND_MULTIGRID && mg_prec == FS_MG_BF16_VECTORS && active16();
+    }
 
     void precondition(const int* status)
     {
@@ -693,8 +900,8 @@ struct fs_grid {
         launch_flush(st(), g);
         launch_true_residual(st(), g, 1);
         precondition(status_ptr());
-        launch_pcg_rz(st(), g, z_ptr());
-        launch_pcg_direction(st(), g, z_ptr());
+        launch_pcg_rz(st(), g, z_ptr(), flexible());
+        launch_pcg_direction(st(), g, z_ptr(), flexible());
     }
 
     void ensure_graph()
@@ -753,8 +960,8 @@ struct fs_grid {
             launch_merge_failure(st(), S_aux.p, S.p);
         }
         precondition(status_ptr());
-        launch_pcg_rz(st(), g, z_ptr());
-        launch_pcg_direction(st(), g, z_ptr());
+        launch_pcg_rz(st(), g, z_ptr(), flexible());
+        launch_pcg_direction(st(), g, z_ptr(), flexible());
     }
 
     // Last phase: fold the correction into the pressure, remove its mean, correct the velocity and
@@ -1333,6 +1540,29 @@ fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask)
         grid->coeff_valid = false;
         grid->mg_valid = false;
         grid->drop_graph();
+    });
+}
+
+fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(mode == FS_MG_FP32 || mode == FS_MG_FP16 || mode == FS_MG_BF16 || mode == FS_MG_BF16_VECTORS,
+            FS_E_INVALID_ARGUMENT, "unknown multigrid precision");
+        if (mode == grid->mg_prec)
+            return;
+        set_device(grid->ctx);
+        grid->mg_prec = mode;
+        grid->apply_precision();
+        grid->tune_for_precision();
+    });
+}
+
+fs_status fs_get_multigrid_precision(const fs_grid* grid, fs_mg_precision* mode)
+{
+    return guard([&] {
+        need(grid != nullptr && mode != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *mode = grid->mg_prec;
     });
 }
 
@@ -2171,7 +2401,7 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
         if (!g.coeff_valid)
             g.rebuild_coefficients();
         launch_init_scalars(g.st(), g.S.p, 0.0, 1 << 30);
-        if (kernel >= 4 && !g.mg_valid)
+        if ((kernel == 0 || kernel >= 4) && !g.mg_valid)
             g.rebuild_hierarchy();
         LevelView L0 = g.view(0);
         GridDev gd = g.dev();
@@ -2184,17 +2414,9 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
         auto launch_once = [&](int i) {
             switch (kernel) {
             case 10: launch_copy_f32(g.st(), g.calib_src.p, g.calib_dst.p, kCalibWords); break;
-            case 0: {  // the finest post-smoothing launch of a V-cycle (prolongation + four sweeps)
-                CoarseLink link{};
-                if (g.tail_start > 0) {
-                    const Level& F = g.levels[0];
-                    const Level& Cl = g.levels[1];
-                    link = CoarseLink{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
-                }
-                launch_rb_multi(g.st(), L0, i & 1, g.r.p, g.rmean_ptr(), 1, 0, link, nullptr, 1,
-                    g.levels[0].post_variant);
+            case 0:  // the finest post-smoothing launch of a V-cycle (prolongation + four sweeps)
+                g.fine_post_smooth(i & 1);
                 break;
-            }
             case 1: launch_pcg_ad(g.st(), gd); break;
             case 2: launch_pcg_update(g.st(), gd); break;
             case 3: launch_copy_f32(g.st(), g.r.p, g.q.p, g.NP()); break;
@@ -2258,6 +2480,8 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             need(grids[i] != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
             need(grids[i]->ctx == ctx, FS_E_INVALID_ARGUMENT, "batched grids must share one context");
             need(grids[i]->uploaded, FS_E_LOGIC, "Upload the grid state before stepping or projecting");
+            need(!grids[i]->active16(), FS_E_UNSUPPORTED,
+                "batched solves run FP32 multigrid levels: set the grid's multigrid precision to FS_MG_FP32");
             for (int j = 0; j < i; ++j)
                 need(grids[j] != grids[i], FS_E_INVALID_ARGUMENT, "a grid appears twice in the batch");
         }
@@ -2319,7 +2543,7 @@ fs_status fs_grid_estimate_bytes(int width, int height, int n_liquid, int n_gas,
         const size_t NP = (W + 2) * (H + 2), nl = (size_t)n_liquid, ng = (size_t)n_gas;
         const size_t blocks = ((W + 63) / 64) * ((H + 15) / 16);
         // state, pressure history, PCG vectors and reductions (fs_grid::allocate)
-        size_t b = 8 * (3 * N + NU + NV + 3 * NP) + 4 * 5 * NP + 8 * std::max<size_t>(4 * blocks, 64) +
+        size_t b = 8 * (3 * N + NU + NV + 3 * NP) + 4 * 5 * NP + 8 * (6 * blocks + 64) +
                    4 * sizeof(PcgScalars) + 8 * (nl + ng) * N;
         // pressure hierarchy (build_hierarchy): FP64 + FP32 conductances, 1/diag, rhs, two solutions,
         // residual, transfer tables; dense coarse factor and inverse; tail tables

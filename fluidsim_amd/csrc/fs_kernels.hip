@@ -769,19 +769,24 @@ __global__ __launch_bounds__(kNT) void k_precond_diag(GridDev g)
 }
 
 // sum z, r.z and sum r in one pass; r.(z - mean z) = r.z - mean z * sum r; then beta.
-__global__ __launch_bounds__(kNT) void k_pcg_rz(GridDev g, const float* z)
+// FLEX (flexible CG, for a preconditioner that is not exactly linear): also sum z.q and q, from which
+// the direction kernel forms the Polak-Ribiere beta z_new.(r_new - r_old) / r_old.z_old with
+// r_new - r_old = -alpha q (Notay 2000).
+template <bool FLEX> __global__ __launch_bounds__(kNT) void k_pcg_rz(GridDev g, const float* z)
 {
     __shared__ double lds[kNT / 64];
+    constexpr int NP = FLEX ? 5 : 3;
     const Tile4 T = tile4(g.W, g.H, g.s);
-    float zv[4], rv[4];
+    float zv[4], rv[4], qv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         zv[k] = z[T.P[k]];
         rv[k] = g.r[T.P[k]];
+        qv[k] = FLEX ? g.q[T.P[k]] : 0.f;
     }
     if (g.S->status != kRunning)
         return;
-    double sz = 0.0, srz = 0.0, sr = 0.0;
+    double sz = 0.0, srz = 0.0, sr = 0.0, szq = 0.0, sq = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (T.in[k]) {
@@ -789,22 +794,33 @@ __global__ __launch_bounds__(kNT) void k_pcg_rz(GridDev g, const float* z)
             sz += zz;
             srz += rr * zz;
             sr += rr;
+            if (FLEX) {
+                szq += zz * (double)qv[k];
+                sq += (double)qv[k];
+            }
         }
     }
     const double b0 = block_sum<kNT>(sz, lds);
     const double b1 = block_sum<kNT>(srz, lds);
     const double b2 = block_sum<kNT>(sr, lds);
+    const double b3 = FLEX ? block_sum<kNT>(szq, lds) : 0.0;
+    const double b4 = FLEX ? block_sum<kNT>(sq, lds) : 0.0;
     if (threadIdx.x == 0) {
-        g.partials[3 * block_id()] = b0;
-        g.partials[3 * block_id() + 1] = b1;
-        g.partials[3 * block_id() + 2] = b2;
+        g.partials[NP * block_id()] = b0;
+        g.partials[NP * block_id() + 1] = b1;
+        g.partials[NP * block_id() + 2] = b2;
+        if (FLEX) {
+            g.partials[NP * block_id() + 3] = b3;
+            g.partials[NP * block_id() + 4] = b4;
+        }
     }
 }
 
 // d = (z - mean z) + beta d.  Every workgroup reduces the r.z partials itself (same order, same
-// value): beta = next / previous r.z (zero after a restart); workgroup 0 records the new r.z in the
-// next slot and stops the solve at the iteration budget or on a non-positive r.z.
-__global__ __launch_bounds__(kNT) void k_pcg_direction(GridDev g, const float* z)
+// value): beta = next / previous r.z (zero after a restart), or with FLEX the Polak-Ribiere
+// -alpha (z - mean z).q / previous r.z; workgroup 0 records the new r.z in the next slot and stops the
+// solve at the iteration budget or on a non-positive r.z.
+template <bool FLEX> __global__ __launch_bounds__(kNT) void k_pcg_direction(GridDev g, const float* z)
 {
     __shared__ double lds[kNT / 64];
     const Tile4 T = tile4(g.W, g.H, g.s);
@@ -817,14 +833,21 @@ __global__ __launch_bounds__(kNT) void k_pcg_direction(GridDev g, const float* z
     if (g.S->status != kRunning)
         return;
     const int nb = (int)block_count();
-    const double tz = finish_sum<kNT>(g.partials, nb, 3, 0, lds);
-    const double trz = finish_sum<kNT>(g.partials, nb, 3, 1, lds);
-    const double tr = finish_sum<kNT>(g.partials, nb, 3, 2, lds);
+    constexpr int NP = FLEX ? 5 : 3;
+    const double tz = finish_sum<kNT>(g.partials, nb, NP, 0, lds);
+    const double trz = finish_sum<kNT>(g.partials, nb, NP, 1, lds);
+    const double tr = finish_sum<kNT>(g.partials, nb, NP, 2, lds);
     const double n = (double)g.W * g.H;
     const double zmean = tz / n;
     const double next = trz - zmean * tr;
     const int c = g.S->iterations;
-    const double beta64 = g.S->restart ? 0.0 : next / g.S->rz_slot[(c + 1) & 1];
+    double numerator = next;
+    if (FLEX) {
+        const double tzq = finish_sum<kNT>(g.partials, nb, NP, 3, lds);
+        const double tq = finish_sum<kNT>(g.partials, nb, NP, 4, lds);
+        numerator = -g.S->alpha * (tzq - zmean * tq);
+    }
+    const double beta64 = g.S->restart ? 0.0 : numerator / g.S->rz_slot[(c + 1) & 1];
     const bool stop = !isfinite(beta64) || !(c < g.S->max_iterations && next > 0.0 && isfinite(next));
     if (block_id() == 0 && threadIdx.x == 0) {
         g.S->z_mean = zmean;
@@ -1155,14 +1178,20 @@ void launch_pcg_update(hipStream_t st, const GridDev& g)
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kNT), 0, st, g, (int)kFinUpdate, (int)cell_blocks(g.W, g.H));
 }
 
-void launch_pcg_rz(hipStream_t st, const GridDev& g, const float* z)
+void launch_pcg_rz(hipStream_t st, const GridDev& g, const float* z, int flexible)
 {
-    hipLaunchKernelGGL(k_pcg_rz, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
+    if (flexible)
+        hipLaunchKernelGGL(k_pcg_rz<true>, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
+    else
+        hipLaunchKernelGGL(k_pcg_rz<false>, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
 }
 
-void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z)
+void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z, int flexible)
 {
-    hipLaunchKernelGGL(k_pcg_direction, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
+    if (flexible)
+        hipLaunchKernelGGL(k_pcg_direction<true>, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
+    else
+        hipLaunchKernelGGL(k_pcg_direction<false>, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
 }
 
 void launch_precond_diag(hipStream_t st, const GridDev& g)

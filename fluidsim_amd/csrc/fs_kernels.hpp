@@ -37,13 +37,15 @@ struct GridDev {
 // Status values beyond PcgStatus used around the loop.
 enum : int { kZeroRhs = 3, kInitialOk = 4, kFailed = 5 };
 
-// Prolongation source of the fused post-smoothing kernel (csol == nullptr: no prolongation).
-struct CoarseLink {
-    const float* csol;   // coarse solution (padded)
+// Prolongation source of the fused post-smoothing kernel (csol == nullptr: no prolongation); the coarse
+// solution is stored as V.
+template <class V> struct CoarseLinkT {
+    const V* csol;       // coarse solution (padded)
     int cs;              // coarse padded stride
     const int* pxf; const int* pxs; const float* pxw;
     const int* pyf; const int* pys; const float* pyw;
 };
+using CoarseLink = CoarseLinkT<float>;
 
 // Tail of the hierarchy solved by one workgroup entirely in LDS.
 struct TailLevel {
@@ -64,8 +66,10 @@ struct TailDesc {
     int levels;
     int coarse_n;              // unknowns of the pinned coarse system (w*h - 1 of the last level)
     const float* ginv;         // inverse of the pinned coarse matrix, FP32 n*n
-    const float* top_rhs;      // global rhs of the first tail level (the FP32 residual on level 0)
-    float* top_sol;            // global solution of the first tail level
+    // global rhs and solution of the first tail level (the FP32 residual and z when the tail starts at
+    // level 0); stored as FP32, or as bfloat16 when the tail kernel runs with a bfloat16 top level
+    const void* top_rhs;
+    void* top_sol;
     int lds_ginv;              // word offset of ginv in LDS
     const unsigned* tables;    // every level's transfer tables, in their LDS layout (host-built)
     int table_words;
@@ -132,8 +136,10 @@ void launch_pad_copy(hipStream_t st, const double* src, double* dst, int W, int 
 // ---- PCG loop (FP32 vectors, FP64 reductions) ----
 void launch_pcg_ad(hipStream_t st, const GridDev& g);
 void launch_pcg_update(hipStream_t st, const GridDev& g);
-void launch_pcg_rz(hipStream_t st, const GridDev& g, const float* z);
-void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z);
+// flexible: Polak-Ribiere beta from z.q (for a V-cycle with bfloat16 vectors, not exactly linear); the
+// two kernels of one iteration must agree on it (it changes the partials' layout).
+void launch_pcg_rz(hipStream_t st, const GridDev& g, const float* z, int flexible = 0);
+void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z, int flexible = 0);
 void launch_precond_diag(hipStream_t st, const GridDev& g);
 
 // ---- multigrid V-cycle pieces (FP32) ----
@@ -149,7 +155,24 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
 constexpr int kRestrictVariants = 4;
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
     const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status, int variant);
-void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status);
+// bf16_top: the first tail level's rhs and solution are stored as bfloat16 (else FP32).
+void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status,
+    int bf16_top = 0);
+
+// ---- 16-bit multigrid levels: conductances as C (half16 or bf16), vectors as V ----
+// Conductances of one level rounded (FP64 -> FP32 -> C, each to nearest even) into 16-bit copies.
+template <class C> void launch_level_pack(hipStream_t st, int w, int h, const double* right64, const double* down64,
+    C* right, C* down);
+// The fused smoother and residual-restriction kernels of launch_rb_multi / launch_res_restrict with
+// C conductances, V vectors on this level (its rhs argument included) and VC vectors on the next
+// coarser level; MODE 0 and 1 only.  Instantiated for (half16, float, float), (bf16, float, float),
+// (bf16, float, bf16) and (bf16, bf16, bf16).
+template <class C, class V, class VC>
+void launch_rb_multi_t(hipStream_t st, const LevelViewT<C, V>& L, int cur, const V* rhs, const double* shift,
+    int first_color, int from_zero, const CoarseLinkT<VC>& link, const int* status, int variant);
+template <class C, class V, class VC>
+void launch_res_restrict_t(hipStream_t st, const LevelViewT<C, V>& F, int cur, const V* rhs, const double* shift,
+    const LevelViewT<C, VC>& Cl, AxisRestrict rx, AxisRestrict ry, const int* status, int variant);
 
 // ---- batched PCG: one 1024-thread workgroup runs the whole iteration loop of one instance ----
 constexpr int kMaxBatchLevels = 16;   // tiled levels above an instance's tail

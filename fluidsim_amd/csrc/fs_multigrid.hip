@@ -16,6 +16,7 @@
 
 #include <math.h>
 
+#include <algorithm>
 #include <type_traits>
 
 namespace fsim {
@@ -25,6 +26,23 @@ namespace {
 constexpr int kNT = 256;
 
 __device__ __forceinline__ float inv_or_zero(float d) { return d > 0.f ? __builtin_amdgcn_rcpf(d) : 0.f; }
+
+// Loads and stores of multigrid storage formats; arithmetic is FP32.  Stores round to nearest even.
+__device__ __forceinline__ float ldv(float v) { return v; }
+__device__ __forceinline__ float ldv(half16 v) { return (float)__builtin_bit_cast(_Float16, v.bits); }
+__device__ __forceinline__ float ldv(bf16 v) { return __uint_as_float((unsigned)v.bits << 16); }
+template <class T> __device__ __forceinline__ T stv(float f);
+template <> __device__ __forceinline__ float stv<float>(float f) { return f; }
+template <> __device__ __forceinline__ half16 stv<half16>(float f)
+{
+    return half16{__builtin_bit_cast(unsigned short, (_Float16)f)};
+}
+template <> __device__ __forceinline__ bf16 stv<bf16>(float f)
+{
+    // round to nearest even on the 16 bits dropped (finite inputs: the levels hold no NaN)
+    const unsigned u = __float_as_uint(f);
+    return bf16{(unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16)};
+}
 
 // XCD-aware tile order.  Workgroups of a launch are dealt to the 8 XCDs round-robin (workgroup b runs
 // on XCD b % 8), each XCD with its own L2.  Mapping the workgroups an XCD receives onto a contiguous
@@ -54,9 +72,10 @@ __device__ __forceinline__ void xcd_tile(int gx, int gy, int& bx, int& by)
 // 512-thread post-smoothing tiles are held to 128 VGPRs (four waves per SIMD, two workgroups per CU).
 template <int MODE, int NT> constexpr int rb_waves_per_eu() { return (MODE == 1 && NT == 512) ? 4 : 1; }
 
-template <int TX, int TY, int NS, int FINE, int MODE, int NT>
+// C: conductance storage, V: this level's vectors (rhs, solution), VC: the coarse solution (MODE 1).
+template <int TX, int TY, int NS, int FINE, int MODE, int NT, class C = float, class V = float, class VC = float>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(rb_waves_per_eu<MODE, NT>())))
-void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int c0, CoarseLink link,
+void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, int c0, CoarseLinkT<VC> link,
     const int* status)
 {
     constexpr bool from_zero = MODE == 0;
@@ -72,7 +91,7 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
     int bx, by;
     xcd_tile((w + TX - 1) / TX, (h + TY - 1) / TY, bx, by);
     const int x0 = bx * TX - HALO, y0 = by * TY - HALO;  // x0 + y0 is even
-    const float* sol_in = L.sol[cur];
+    const V* sol_in = L.sol[cur];
     // Every global load below goes through a clamped (valid) interior index and is masked afterwards,
     // so a thread issues all its loads before its first wait.
     float cl[2][ROUNDS], cr[2][ROUNDS], cu[2][ROUNDS], cd[2][ROUNDS], cb[2][ROUNDS];
@@ -90,11 +109,11 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
             const int x = x0 + lx, y = y0 + ly;
             cin[c][r] = t < SLOTS && x >= 0 && x < w && y >= 0 && y < h;
             const int P = (min(max(y, 0), h - 1) + 1) * s + (min(max(x, 0), w - 1) + 1);
-            cl[c][r] = L.right[P - 1];
-            cr[c][r] = L.right[P];
-            cu[c][r] = L.down[P - s];
-            cd[c][r] = L.down[P];
-            cb[c][r] = rhs[P];
+            cl[c][r] = ldv(L.right[P - 1]);
+            cr[c][r] = ldv(L.right[P]);
+            cu[c][r] = ldv(L.down[P - s]);
+            cd[c][r] = ldv(L.down[P]);
+            cb[c][r] = ldv(rhs[P]);
         }
     }
     constexpr int LOADS = (NE + NT - 1) / NT;
@@ -109,7 +128,7 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
         vin[it] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
         ix[it] = min(max(x, 0), w - 1);
         iy[it] = min(max(y, 0), h - 1);
-        vs[it] = MODE == 0 ? 0.f : sol_in[(iy[it] + 1) * s + (ix[it] + 1)];
+        vs[it] = MODE == 0 ? 0.f : ldv(sol_in[(iy[it] + 1) * s + (ix[it] + 1)]);
     }
     if (MODE == 1) {
         // per chunk of PC cells, two batched phases: transfer tables, then the four coarse values (chunks
@@ -133,9 +152,9 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
             for (int k = 0; k < PC; ++k) {
                 if (c0 + k >= LOADS)
                     break;
-                const float* top = link.csol + (b0[k] + 1) * link.cs + 1;
-                const float* bot = link.csol + (b1[k] + 1) * link.cs + 1;
-                const float t0 = top[a0[k]], t1 = top[a1[k]], u0 = bot[a0[k]], u1 = bot[a1[k]];
+                const VC* top = link.csol + (b0[k] + 1) * link.cs + 1;
+                const VC* bot = link.csol + (b1[k] + 1) * link.cs + 1;
+                const float t0 = ldv(top[a0[k]]), t1 = ldv(top[a1[k]]), u0 = ldv(bot[a0[k]]), u1 = ldv(bot[a1[k]]);
                 const float a = (1.f - wx[k]) * t0 + wx[k] * t1;
                 const float b = (1.f - wx[k]) * u0 + wx[k] * u1;
                 vs[c0 + k] += (1.f - wy[k]) * a + wy[k] * b;
@@ -228,12 +247,12 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
             half(std::integral_constant<int, 0>{}, k + 1);
         }
     }
-    float* sol_out = L.sol[cur ^ 1];
+    V* sol_out = L.sol[cur ^ 1];
     for (int e = threadIdx.x; e < TX * TY; e += NT) {
         const int ly = HALO + e / TX, lx = HALO + e % TX;
         const int x = x0 + lx, y = y0 + ly;
         if (x < w && y < h)
-            sol_out[(y + 1) * s + (x + 1)] = s_col[(lx + ly) & 1][ly * HX + (lx >> 1)];
+            sol_out[(y + 1) * s + (x + 1)] = stv<V>(s_col[(lx + ly) & 1][ly * HX + (lx >> 1)]);
     }
 }
 
@@ -242,9 +261,9 @@ void k_rb_multi(LevelView L, int cur, const float* rhs, const double* shift, int
 // CTX x CTY cells gathers fine taps in [2c - 2, 2c + 3]; the fine residual is formed in LDS on
 // that window (plus a one-cell ring of solution values).
 // ---------------------------------------------------------------------------------------------
-template <int CTX, int CTY>
-__global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, const float* rhs, const double* shift,
-    LevelView Cl, AxisRestrict rx, AxisRestrict ry, const int* status)
+template <int CTX, int CTY, class C = float, class V = float, class VC = float>
+__global__ __launch_bounds__(kNT) void k_res_restrict(LevelViewT<C, V> F, int cur, const V* rhs, const double* shift,
+    LevelViewT<C, VC> Cl, AxisRestrict rx, AxisRestrict ry, const int* status)
 {
     constexpr int LX = 2 * CTX + 6, LY = 2 * CTY + 6;
     __shared__ float s_sol[LY * LX];
@@ -258,7 +277,7 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, cons
     xcd_tile((Cl.w + CTX - 1) / CTX, (Cl.h + CTY - 1) / CTY, bx, by);
     const int cx0 = bx * CTX, cy0 = by * CTY;
     const int fx0 = 2 * cx0 - 3, fy0 = 2 * cy0 - 3;
-    const float* sol = F.sol[cur];
+    const V* sol = F.sol[cur];
     constexpr int NE = LX * LY;
     constexpr int LOADS = (NE + kNT - 1) / kNT;
     float vs[LOADS], vr[LOADS], vd[LOADS], vb[LOADS];
@@ -270,10 +289,10 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, cons
         const int x = fx0 + lx, y = fy0 + ly;
         vin[it] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
         const int P = (min(max(y, 0), h - 1) + 1) * s + (min(max(x, 0), w - 1) + 1);
-        vs[it] = sol[P];
-        vr[it] = F.right[P];
-        vd[it] = F.down[P];
-        vb[it] = rhs[P];
+        vs[it] = ldv(sol[P]);
+        vr[it] = ldv(F.right[P]);
+        vd[it] = ldv(F.down[P]);
+        vb[it] = ldv(rhs[P]);
     }
     // restriction tables of this coarse tile (missing taps: weight 0 at a valid LDS cell)
     int tc[2], ti[2][4];
@@ -366,7 +385,7 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelView F, int cur, cons
                 horizontal += s_rxw[lx * 4 + ix] * tap[iy][ix];
             value += s_ryw[ly * 4 + iy] * horizontal;
         }
-        Cl.rhs[(y + 1) * Cl.s + x + 1] = value;
+        Cl.rhs[(y + 1) * Cl.s + x + 1] = stv<VC>(value);
     }
 }
 
@@ -897,7 +916,9 @@ __device__ void tail_coarse_solve(const TailLevel& T, const TailArrays& A, const
 
 // The tail V-cycle of one instance on a 1024-thread workgroup: stage every tail level, its transfer
 // tables and the coarse inverse in `lds`, cycle, and write the first tail level's solution back.
-__device__ void tail_run(const TailDesc& D, const double* shift, const int* status, float* lds)
+// TV: storage of the first tail level's global rhs and solution (FP32, or bfloat16).
+template <class TV = float> __device__ void tail_run(const TailDesc& D, const double* shift, const int* status,
+    float* lds)
 {
     const int nl = D.levels;
     // Flat index over every level's padded arrays; all loads issue before the LDS stores.
@@ -924,7 +945,7 @@ __device__ void tail_run(const TailDesc& D, const double* shift, const int* stat
         vd[it] = T.down[e];
         const int y = e / T.s - 1, x = e % T.s - 1;
         const bool interior0 = l == 0 && x >= 0 && x < T.w && y >= 0 && y < T.h;
-        const float b = D.top_rhs[min(e, D.lev[0].s * (D.lev[0].h + 2) - 1)];
+        const float b = ldv(((const TV*)D.top_rhs)[min(e, D.lev[0].s * (D.lev[0].h + 2) - 1)]);
         vb[it] = interior0 ? b : 0.f;
         bin[it] = interior0;
     }
@@ -995,14 +1016,15 @@ __device__ void tail_run(const TailDesc& D, const double* shift, const int* stat
     for (int e = threadIdx.x; e < T0.w * T0.h; e += kTailThreads) {
         const int y = e / T0.w, x = e - y * T0.w;
         const int i = (y + 1) * T0.s + x + 1;
-        D.top_sol[i] = A0.sol[i];
+        ((TV*)D.top_sol)[i] = stv<TV>(A0.sol[i]);
     }
 }
 
+template <class TV>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailDesc D, const double* shift, const int* status)
 {
     extern __shared__ float lds[];
-    tail_run(D, shift, status, lds);
+    tail_run<TV>(D, shift, status, lds);
 }
 
 constexpr int kMultiNS = 4;
@@ -1146,7 +1168,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
         S.restart = 0;                         // consumed by that direction
     }
     __syncthreads();
-    const float* z = I.multigrid ? (I.levels > 0 ? I.lev[0].L.sol[0] : I.tail->top_sol) : g.zdiag;
+    const float* z = I.multigrid ? (I.levels > 0 ? I.lev[0].L.sol[0] : (const float*)I.tail->top_sol) : g.zdiag;
     while (S.status == kRunning) {
         // q = A d and d.q (k_pcg_ad), alpha (kFinAd)
         double dot = 0.0;
@@ -1254,7 +1276,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
                 wg_sweeps(B.L, B.L.sol[1], B.L.rhs, shift, 0, true);
                 wg_res_restrict(B.L, B.L.rhs, shift, B.C, B.rx, B.ry);
             }
-            tail_run(*I.tail, I.levels == 0 ? &S.r_mean : nullptr, nullptr, lds);
+            tail_run<float>(*I.tail, I.levels == 0 ? &S.r_mean : nullptr, nullptr, lds);
             __syncthreads();
             for (int l = I.levels - 1; l >= 0; --l) {
                 const BatchLevel& B = I.lev[l];
@@ -1341,8 +1363,10 @@ size_t tail_layout(TailDesc& t)
 
 bool tail_allow_lds(size_t bytes)
 {
-    return hipFuncSetAttribute((const void*)k_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
-           hipSuccess;
+    return hipFuncSetAttribute((const void*)k_mg_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+               (int)bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)k_mg_tail<bf16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+               (int)bytes) == hipSuccess;
 }
 
 // Tile variants (interior cells per workgroup).  The fused sweeps compute every interior cell exactly as
@@ -1418,10 +1442,95 @@ void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, 
     hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, st, w, h, right64, down64, ginv, n, S, pin);
 }
 
-void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status)
+void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status,
+    int bf16_top)
 {
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), lds_bytes, st, desc, shift, status);
+    if (bf16_top)
+        hipLaunchKernelGGL(k_mg_tail<bf16>, dim3(1), dim3(kTailThreads), lds_bytes, st, desc, shift, status);
+    else
+        hipLaunchKernelGGL(k_mg_tail<float>, dim3(1), dim3(kTailThreads), lds_bytes, st, desc, shift, status);
 }
+
+// ---- 16-bit multigrid levels ----
+namespace {
+
+template <class C>
+__global__ __launch_bounds__(kNT) void k_level_pack(int w, int h, const double* right64, const double* down64,
+    C* right, C* down)
+{
+    const int s = w + 2;
+    const int n = s * (h + 2);
+    for (int i = blockIdx.x * kNT + threadIdx.x; i < n; i += gridDim.x * kNT) {
+        right[i] = stv<C>((float)right64[i]);
+        down[i] = stv<C>((float)down64[i]);
+    }
+}
+
+}  // namespace
+
+template <class C> void launch_level_pack(hipStream_t st, int w, int h, const double* right64, const double* down64,
+    C* right, C* down)
+{
+    const int n = (w + 2) * (h + 2);
+    const int blocks = std::min((n + kNT - 1) / kNT, 4096);
+    hipLaunchKernelGGL(k_level_pack<C>, dim3(blocks), dim3(kNT), 0, st, w, h, right64, down64, right, down);
+}
+template void launch_level_pack<half16>(hipStream_t, int, int, const double*, const double*, half16*, half16*);
+template void launch_level_pack<bf16>(hipStream_t, int, int, const double*, const double*, bf16*, bf16*);
+
+template <class C, class V, class VC>
+void launch_rb_multi_t(hipStream_t st, const LevelViewT<C, V>& L, int cur, const V* rhs, const double* shift,
+    int first_color, int from_zero, const CoarseLinkT<VC>& link, const int* status, int variant)
+{
+#define FS_RBT(TX, TY, M, NT)                                                                                 \
+    hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, 0, M, NT, C, V, VC>),                                   \
+        dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)), dim3(NT), 0, st, L, cur, rhs, shift, first_color, link, \
+        status)
+#define FS_RBT_MODES(TX, TY, NT)      \
+    do {                              \
+        if (from_zero)                \
+            FS_RBT(TX, TY, 0, NT);    \
+        else                          \
+            FS_RBT(TX, TY, 1, NT);    \
+    } while (0)
+    switch (variant) {
+    case 1: FS_RBT_MODES(32, 32, 256); break;
+    case 2: FS_RBT_MODES(32, 16, 256); break;
+    case 3: FS_RBT_MODES(16, 16, 256); break;
+    case 4: FS_RBT_MODES(64, 32, 512); break;
+    case 5: FS_RBT_MODES(32, 32, 512); break;
+    default: FS_RBT_MODES(64, 32, 256); break;
+    }
+#undef FS_RBT_MODES
+#undef FS_RBT
+}
+
+template <class C, class V, class VC>
+void launch_res_restrict_t(hipStream_t st, const LevelViewT<C, V>& F, int cur, const V* rhs, const double* shift,
+    const LevelViewT<C, VC>& Cl, AxisRestrict rx, AxisRestrict ry, const int* status, int variant)
+{
+#define FS_RRT(CX, CY)                                                                                        \
+    hipLaunchKernelGGL((k_res_restrict<CX, CY, C, V, VC>), dim3(((Cl.w + CX - 1) / CX) * ((Cl.h + CY - 1) / CY)),  \
+        dim3(kNT), 0, st, F, cur, rhs, shift, Cl, rx, ry, status)
+    switch (variant) {
+    case 1: FS_RRT(16, 16); break;
+    case 2: FS_RRT(16, 8); break;
+    case 3: FS_RRT(8, 8); break;
+    default: FS_RRT(32, 16); break;
+    }
+#undef FS_RRT
+}
+
+#define FS_MG16(C, V, VC)                                                                                      \
+    template void launch_rb_multi_t<C, V, VC>(hipStream_t, const LevelViewT<C, V>&, int, const V*, const double*, \
+        int, int, const CoarseLinkT<VC>&, const int*, int);                                                    \
+    template void launch_res_restrict_t<C, V, VC>(hipStream_t, const LevelViewT<C, V>&, int, const V*,          \
+        const double*, const LevelViewT<C, VC>&, AxisRestrict, AxisRestrict, const int*, int);
+FS_MG16(half16, float, float)
+FS_MG16(bf16, float, float)
+FS_MG16(bf16, float, bf16)
+FS_MG16(bf16, bf16, bf16)
+#undef FS_MG16
 
 bool batch_allow_lds(size_t bytes)
 {

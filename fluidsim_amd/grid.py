@@ -352,6 +352,15 @@ class FluidGridHIP:
             raise ValueError("solid mask dimensions do not match the grid")
         call("fs_set_solid_mask", self._h, m.ctypes.data)
 
+    # -- multigrid level storage (extension: FS_MG_FP32 / FS_MG_FP16 / FS_MG_BF16) --
+    def setMultigridPrecision(self, mode: int):
+        call("fs_set_multigrid_precision", self._h, int(mode))
+
+    def getMultigridPrecision(self) -> int:
+        m = C.c_int()
+        call("fs_get_multigrid_precision", self._h, C.byref(m))
+        return m.value
+
     def solidCells(self) -> int:
         n = C.c_longlong()
         call("fs_solid_cells", self._h, C.byref(n))

@@ -225,6 +225,14 @@ public:
             throw std::invalid_argument("solid mask dimensions do not match the grid");
         check(fs_set_solid_mask(grid_, mask.empty() ? nullptr : mask.data()));
     }
+    // Multigrid level storage (no reference counterpart): FS_MG_FP32, FS_MG_FP16 or FS_MG_BF16.
+    void setMultigridPrecision(fs_mg_precision mode) { check(fs_set_multigrid_precision(grid_, mode)); }
+    fs_mg_precision getMultigridPrecision() const
+    {
+        fs_mg_precision m = FS_MG_FP32;
+        check(fs_get_multigrid_precision(grid_, &m));
+        return m;
+    }
     long long solidCells() const
     {
         long long n = 0;

@@ -84,6 +84,29 @@ typedef enum fs_transport {
 /* FluidGridCPU::PressurePreconditioner (FluidGridCPU.h:39-44). */
 typedef enum fs_preconditioner { FS_PRECOND_DIAGONAL = 0, FS_PRECOND_MULTIGRID = 1 } fs_preconditioner;
 
+/* Storage of the multigrid levels that the fused smoothers stream (no reference counterpart: the
+ * reference's UniformCPU hierarchy is FP64, PressureMultigrid.h:9-69).  Arithmetic is FP32 in every
+ * mode, the CG vectors are FP32 and the dot products, residual norms and accumulated pressure FP64.
+ *   FS_MG_FP32          conductances and level vectors FP32 (default);
+ *   FS_MG_FP16          conductances of the tiled levels as IEEE binary16, level vectors FP32;
+ *   FS_MG_BF16          conductances of the tiled levels as bfloat16, level vectors FP32;
+ *   FS_MG_BF16_VECTORS  conductances and the rhs/solution vectors of the coarser tiled levels as
+ *                       bfloat16 (the finest level's rhs and solution are the FP32 CG residual and z).
+ *                       Rounding every stored vector makes the V-cycle linear only to ~2^-9, so the CG
+ *                       takes the flexible (Polak-Ribiere) beta; hard problems (obstacles with 1000:1
+ *                       density jumps at 2048^2) can stall.  binary16 vectors are not offered: its range
+ *                       cannot hold a level's rhs and solution, which differ by ~cells/conductance.
+ * With 16-bit conductances the preconditioner is a fixed SPD operator (rounded once per hierarchy
+ * rebuild), so the CG keeps its Fletcher-Reeves beta.  The FP64 hierarchy, the coarse factor and the
+ * in-LDS tail are unchanged; a grid whose whole hierarchy fits the tail has no tiled level and runs
+ * FP32 whatever the mode. */
+typedef enum fs_mg_precision {
+    FS_MG_FP32 = 0,
+    FS_MG_FP16 = 1,
+    FS_MG_BF16 = 2,
+    FS_MG_BF16_VECTORS = 3
+} fs_mg_precision;
+
 /* ---- library / context ---- */
 const char* fs_version(void);
 /* Message of the last failed call on this thread ("" if none).  No reference counterpart: the C
@@ -145,6 +168,9 @@ fs_status fs_set_transport(fs_grid* grid, fs_transport mode);
  * Viscosity, transport and surface tension see the zeroed face velocities; the mask is configuration,
  * not state: checkpoints do not carry it. */
 fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask);
+/* Multigrid level storage (fs_mg_precision above).  FS_E_INVALID_ARGUMENT for an unknown mode. */
+fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode);
+fs_status fs_get_multigrid_precision(const fs_grid* grid, fs_mg_precision* mode);
 /* Number of solid cells of the current mask (0 without one). */
 fs_status fs_solid_cells(const fs_grid* grid, long long* count);
 

@@ -1,0 +1,183 @@
+"""16-bit multigrid level storage (fs_set_multigrid_precision) on the GPU.
+
+FS_MG_FP16 / FS_MG_BF16 round the tiled levels' conductances to binary16 / bfloat16; FS_MG_BF16_VECTORS
+also keeps the coarser tiled levels' rhs and solutions in bfloat16 (and runs the flexible CG beta);
+smoothing arithmetic stays FP32.  Bars:
+  - the V-cycle stays a valid preconditioner: positive, symmetric and linear -- to the FP32 bars of
+    tests/FluidGridWebGPUTests.cpp:428-535 (5e-4) with 16-bit conductances, whose rounding is fixed
+    at set-up, and to 2e-2 with bfloat16 vectors (8 significand bits, rounded on every store);
+  - it stays close to the FP32 V-cycle (2e-3 binary16, 1e-2 bfloat16 conductances, 3e-2 with bfloat16
+    vectors, relative L2);
+  - projections reach the same stopping rule as FP32 and match the FP64 oracle within 1e-5 relative
+    L2 -- precision moves iteration counts, not the answer;
+  - a grid with no tiled level computes exactly what FP32 computes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fluidsim_amd import FS_MG_BF16, FS_MG_BF16_VECTORS, FS_MG_FP16, FS_MG_FP32, FluidGridHIP, batch_project
+from fluidsim_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 1e-5
+MODES = {"fp16": FS_MG_FP16, "bf16": FS_MG_BF16, "bf16v": FS_MG_BF16_VECTORS}
+CLOSE = {"fp16": 2e-3, "bf16": 1e-2, "bf16v": 3e-2}
+LINEAR = {"fp16": 5e-4, "bf16": 5e-4, "bf16v": 2e-2}
+GROWTH = {"fp16": 1.5, "bf16": 1.5, "bf16v": 2.0}   # iterations at most GROWTH x FP32's, plus 2
+
+
+def rel_l2(a, b):
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
+
+
+def hip_grid(wl, mask=None, mode=FS_MG_FP32):
+    g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+    WL.load_into(g, wl)
+    if mask is not None:
+        g.setSolidMask(mask)
+    g.setMultigridPrecision(mode)
+    return g
+
+
+def oracle_grid(O, wl, mask=None):
+    g = O.OracleGrid(wl.width, wl.height, wl.cell_size)
+    g.configure(wl.liquid_density, wl.gas_density)
+    g.set_gravity(wl.gravity_y)
+    g.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+    if mask is not None:
+        g.set_solid(mask)
+    return g
+
+
+def test_mode_is_set_and_validated():
+    wl = WL.sloped_pool(200, 150, 4.0)
+    g = hip_grid(wl)
+    assert g.getMultigridPrecision() == FS_MG_FP32
+    for m in (FS_MG_FP16, FS_MG_BF16, FS_MG_BF16_VECTORS, FS_MG_FP32):
+        g.setMultigridPrecision(m)
+        assert g.getMultigridPrecision() == m
+    with pytest.raises(ValueError):
+        g.setMultigridPrecision(7)
+    g.destroy()
+
+
+def workload(case):
+    if case == "pool256x192":
+        return WL.sloped_pool(256, 192, 4.0), None
+    wl = WL.obstacle_tank(301, 203, seed=3)
+    return wl, wl.meta["solid"]
+
+
+@pytest.mark.parametrize("case", ["pool256x192", "tank301x203"])
+@pytest.mark.parametrize("mode", ["fp16", "bf16", "bf16v"])
+def test_16bit_vcycle_is_a_valid_preconditioner_close_to_fp32(case, mode):
+    wl, mask = workload(case)
+    W, H = wl.width, wl.height
+    g32 = hip_grid(wl, mask)
+    g16 = hip_grid(wl, mask, MODES[mode])
+    i = np.arange(W * H)
+    a = np.sin(i * 0.31) + np.cos(i * 0.073)
+    b = np.cos(i * 0.17) - np.sin(i * 0.019)
+    a -= a.mean()
+    b -= b.mean()
+    ma, mb = g16.multigridApply(a), g16.multigridApply(b)
+    mc = g16.multigridApply(1.3 * a - 0.7 * b)
+    assert np.array_equal(ma, g16.multigridApply(a)), "V-cycle is not history-free"
+    ma, mb, mc = ma - ma.mean(), mb - mb.mean(), mc - mc.mean()
+    assert np.abs(mc - (1.3 * ma - 0.7 * mb)).max() <= LINEAR[mode] * np.abs(mc).max()
+    ea, eb = a @ ma, b @ mb
+    assert ea > 0 and eb > 0
+    assert abs(a @ mb - b @ ma) <= LINEAR[mode] * math.sqrt(ea * eb)
+    ref = g32.multigridApply(a)
+    ref -= ref.mean()
+    assert rel_l2(ma, ref) < CLOSE[mode]
+    assert rel_l2(ma, ref) > 0.0, "the 16-bit levels were not used"
+    g32.destroy()
+    g16.destroy()
+
+
+@pytest.mark.parametrize("case", ["dam128", "vortex256", "tank301x203"])
+@pytest.mark.parametrize("mode", ["fp16", "bf16", "bf16v"])
+def test_16bit_projection_matches_oracle(oracle, case, mode):
+    mask = None
+    if case == "dam128":
+        wl = WL.dam_break(128, 128, seed=1)
+    elif case == "vortex256":
+        wl = WL.vortex_box(256, 256, seed=2)
+    else:
+        wl = WL.obstacle_tank(301, 203, seed=3)
+        mask = wl.meta["solid"]
+    o = oracle_grid(oracle, wl, mask)
+    tau = 1e-6 * o.max_divergence()
+    ro = o.project(wl.dt, 4000, tau)
+    assert ro["converged"]
+    so = o.download()
+    iterations = {}
+    for name, m in (("fp32", FS_MG_FP32), (mode, MODES[mode])):
+        g = hip_grid(wl, mask, m)
+        r = g.project(wl.dt, 4000, tau)
+        assert r.converged and r.max_divergence <= tau, (name, r)
+        s = g.download()
+        assert rel_l2(s.u, so["u"]) < REL_L2 and rel_l2(s.v, so["v"]) < REL_L2, name
+        fluid = np.ones(wl.width * wl.height, bool) if mask is None else mask == 0
+        ph, po = s.pressure[fluid], so["pressure"][fluid]
+        assert rel_l2(ph - ph.mean(), po - po.mean()) < REL_L2, name
+        iterations[name] = r.iterations
+        g.destroy()
+    print(f"{case}: iterations fp32 {iterations['fp32']} {mode} {iterations[mode]} oracle {ro['iterations']}")
+    assert iterations[mode] <= GROWTH[mode] * iterations["fp32"] + 2
+
+
+def test_grid_without_tiled_levels_is_fp32():
+    wl = WL.dam_break(48, 40, seed=3)
+    results = []
+    for m in (FS_MG_FP32, FS_MG_FP16, FS_MG_BF16, FS_MG_BF16_VECTORS):
+        g = hip_grid(wl, None, m)
+        g.applyGravity(wl.dt)
+        r = g.project(wl.dt, 200, 1e-6)
+        s = g.download()
+        results.append((r.iterations, s.u.copy(), s.v.copy(), s.pressure.copy()))
+        g.destroy()
+    for k in (1, 2, 3):
+        assert results[k][0] == results[0][0]
+        for a, b in zip(results[k][1:], results[0][1:]):
+            assert np.array_equal(a, b)
+
+
+def test_mode_switch_back_to_fp32_is_bit_identical():
+    wl = WL.vortex_box(256, 256, seed=2)
+    a = hip_grid(wl)
+    b = hip_grid(wl, None, FS_MG_BF16_VECTORS)
+    b.setMultigridPrecision(FS_MG_FP32)
+    ra, rb = a.project(wl.dt, 200, 1e-6), b.project(wl.dt, 200, 1e-6)
+    assert ra.iterations == rb.iterations
+    assert np.array_equal(a.download().pressure, b.download().pressure)
+    a.destroy()
+    b.destroy()
+
+
+def test_batched_solve_refuses_16bit_levels():
+    wl = WL.vortex_box(256, 256, seed=2)
+    g = hip_grid(wl, None, FS_MG_FP16)
+    with pytest.raises(NotImplementedError):
+        batch_project([g], wl.dt, 200, 1e-6)
+    g.destroy()
+
+
+def test_vortex_box_1024_reaches_relative_tolerance_in_every_mode():
+    wl = WL.vortex_box(1024, 1024, seed=2)
+    its = {}
+    for name, m in (("fp32", FS_MG_FP32), ("fp16", FS_MG_FP16), ("bf16", FS_MG_BF16),
+                    ("bf16v", FS_MG_BF16_VECTORS)):
+        g = hip_grid(wl, None, m)
+        tau = 1e-6 * g.maxDivergence()
+        r = g.project(wl.dt, 400, tau)
+        assert r.converged and r.max_divergence <= tau, name
+        its[name] = r.iterations
+        g.destroy()
+    print("vortex box 1024^2 iterations:", its)
+    assert its["fp16"] <= its["fp32"] + 2 and its["bf16"] <= its["fp32"] + 2 and its["bf16v"] <= 2 * its["fp32"] + 2

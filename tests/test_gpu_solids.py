@@ -205,23 +205,41 @@ def test_masked_steps_keep_solid_faces_closed(oracle):
     g.destroy()
 
 
-def test_obstacle_tank_2048_matches_oracle(oracle):
+_TANK_2048 = {}
+
+
+def tank_2048_oracle(O):
+    """The FP64 oracle's projection of the 2048 x 2048 tank, computed once for every precision."""
+    if not _TANK_2048:
+        wl = WL.obstacle_tank(2048, 2048, seed=3)
+        mask = wl.meta["solid"]
+        o = oracle_grid(O, wl, mask)
+        tau = 1e-6 * o.max_divergence()
+        ro = o.project(wl.dt, 2000, tau)
+        assert ro["converged"]
+        _TANK_2048.update(wl=wl, mask=mask, tau=tau, result=ro, state=o.download())
+    return _TANK_2048
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp16", "bf16"])
+def test_obstacle_tank_2048_matches_oracle(oracle, mode):
     """2048 x 2048, 40 seeded circles and boxes over 15% of the cells, liquid in the bottom 70%: one
-    projection to 1e-6 of the initial divergence, against the FP64 oracle on the same input."""
-    wl = WL.obstacle_tank(2048, 2048, seed=3)
-    mask = wl.meta["solid"]
-    print(f"obstacle tank 2048^2: solid fraction {mask.mean():.4f}", flush=True)
+    projection to 1e-6 of the initial divergence with FP32, binary16 or bfloat16 multigrid conductances,
+    against the FP64 oracle on the same input.  (With bfloat16 level vectors the solve stalls above
+    this tolerance on this problem; the benchmark reports that mode.)"""
+    from fluidsim_amd import FS_MG_BF16, FS_MG_FP16, FS_MG_FP32
+
+    ref = tank_2048_oracle(oracle)
+    wl, mask, tau = ref["wl"], ref["mask"], ref["tau"]
     g = hip_grid(wl, mask)
-    tau = 1e-6 * g.maxDivergence()
+    g.setMultigridPrecision({"fp32": FS_MG_FP32, "fp16": FS_MG_FP16, "bf16": FS_MG_BF16}[mode])
     rh = g.project(wl.dt, 2000, tau)
     assert rh.converged and rh.max_divergence <= tau
     sh = g.download()
     g.destroy()
-    o = oracle_grid(oracle, wl, mask)
-    ro = o.project(wl.dt, 2000, tau)
-    assert ro["converged"]
-    so = o.download()
-    print(f"iterations: gpu {rh.iterations} oracle {ro['iterations']}", flush=True)
+    so = ref["state"]
+    print(f"obstacle tank 2048^2 {mode}: iterations gpu {rh.iterations} oracle {ref['result']['iterations']}",
+          flush=True)
     assert rel_l2(sh.u, so["u"]) < REL_L2 and rel_l2(sh.v, so["v"]) < REL_L2
     fluid = mask == 0
     po, ph = so["pressure"][fluid], sh.pressure[fluid]
