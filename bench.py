@@ -231,7 +231,7 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400):
 
 def batch_leg(count, seed=4, visc_sample=16):
     """Config C4 on one GPU: `count` seeded instances (sides log-uniform 64..512, 30-90% liquid plus a
-    drop, vortex velocity), each projected once to 1e-6 of its own initial max divergence.  Timed twice
+    drop, vortex velocity, obstacles over 0-25% of the cells), each projected once to 1e-6 of its own initial max divergence.  Timed twice
     from the same device-resident inputs: one fs_project per grid in turn (the plain GPU MGPCG), then one
     fs_batch_project over all of them (one persistent launch for all PCG iterations).  Each instance's
     implicit viscosity solve (nu log-uniform 1e-3..1) is timed on its own, instance by instance."""
@@ -253,6 +253,8 @@ def batch_leg(count, seed=4, visc_sample=16):
                              FluidMixture([MixtureComponent("air", 0.1, nu * 0.1)]))
         from fluidsim_amd import FluidGridState
         g.upload(FluidGridState(wl.fraction, wl.u, wl.v, np.zeros(0), [wl.liquid_mass], [wl.gas_mass]))
+        if wl.meta["solid"] is not None:
+            g.setSolidMask(wl.meta["solid"])
         g.setTransport(FS_TRANSPORT_NONE)
         taus.append(1e-6 * g.maxDivergence())
         g.snapshotSave()
@@ -302,7 +304,9 @@ def batch_leg(count, seed=4, visc_sample=16):
     cells = [wl.width * wl.height for wl in wls]
     return {
         "workload": f"{count} instances, sides log-uniform 64..512, liquid 30-90% + drop, vortex velocity, "
-                    f"seed {seed}; one projection each to 1e-6 of its initial max divergence",
+                    f"obstacles over 0-25% of the cells, seed {seed}; one projection each to 1e-6 of its "
+                    "initial max divergence",
+        "masked_instances": int(sum(wl.meta["solid"] is not None for wl in wls)),
         "cells_total": int(sum(cells)),
         "serial_solves_per_s": count / serial_s,
         "batched_solves_per_s": count / batch_s,

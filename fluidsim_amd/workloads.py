@@ -185,7 +185,7 @@ def obstacle_mask(W: int, H: int, rng: SplitMix64, shapes: int = 40, coverage: f
         return m
 
     lo, hi = 0.0, 0.5 * max(W, H)
-    for _ in range(40):
+    for _ in range(30):
         mid = 0.5 * (lo + hi)
         if union(mid).mean() < coverage:
             lo = mid
@@ -312,9 +312,10 @@ def serving_stream(seed: int = 5, count: int = 24, min_size: int = 128, max_size
 def batch_instances(seed: int = 4, count: int = 256, min_size: int = 64, max_size: int = 512):
     """Batched-solve instances: `count` grids with each side log-uniform in [min_size, max_size]
     (independent aspect), cell size 1/max(W, H), water 100 / air 0.1 with the liquid filling 30-90% of
-    the height plus a drop, random-vortex velocity (8 vortices), and a kinematic viscosity log-uniform in
-    [1e-3, 1] (mu = nu rho per phase).  Obstacle masks are not modelled (the reference has none).
-    Returns a list of Workload, one per instance, each from its own SplitMix64 substream."""
+    the height plus a drop, random-vortex velocity (8 vortices), a kinematic viscosity log-uniform in
+    [1e-3, 1] (mu = nu rho per phase) and an obstacle mask of 8 circles and boxes covering a uniform
+    0-25% of the cells (`obstacle_mask`; meta["solid"] is None below 1%).  Returns a list of Workload,
+    one per instance, each from its own SplitMix64 substream."""
     out = []
     lo, hi = math.log(min_size), math.log(max_size)
     for k in range(count):
@@ -331,8 +332,12 @@ def batch_instances(seed: int = 4, count: int = 256, min_size: int = 64, max_siz
         for j in range(8):
             add_gaussian(u, v, W, H, h, cxs[j], cys[j], radii[j], strengths[j])
         zero_walls(u, v, W, H)
-        lm, gm = two_phase_masses(fraction, h)
         nu = math.exp(rng.uniform(1, math.log(1e-3), math.log(1.0))[0])
+        coverage = rng.uniform(1, 0.0, 0.25)[0]
+        solid = obstacle_mask(W, H, rng, 8, coverage) if coverage >= 0.01 else None
+        if solid is not None:
+            fraction[solid != 0] = 0.0
+        lm, gm = two_phase_masses(fraction, h)
         out.append(Workload("batch_instance", W, H, h, [100.0], [0.1], 9.81, 1.0 / 240.0, fraction, u, v, lm, gm,
-                            {"seed": seed, "index": k, "nu": nu}))
+                            {"seed": seed, "index": k, "nu": nu, "solid": solid, "coverage": coverage}))
     return out
