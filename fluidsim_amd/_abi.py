@@ -156,6 +156,7 @@ SIGNATURES = {
     "fs_set_solid_mask": [_vp, C.c_void_p],
     "fs_set_multigrid_precision": [_vp, C.c_int],
     "fs_get_multigrid_precision": [_vp, C.POINTER(C.c_int)],
+    "fs_set_flexible_cg": [_vp, C.c_int],
     "fs_solid_cells": [_vp, C.POINTER(C.c_longlong)],
     "fs_upload": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "fs_download": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],

@@ -215,6 +215,7 @@ struct fs_grid {
     fs_preconditioner precond = FS_PRECOND_MULTIGRID;
     fs_transport transport = FS_TRANSPORT_REFERENCE;
     fs_mg_precision mg_prec = FS_MG_FP32;
+    bool flexible_cg = false;   // fs_set_flexible_cg
     bool uploaded = false;
     bool coeff_valid = false;
     bool mg_valid = false;
@@ -880,7 +881,7 @@ struct fs_grid {
     // takes the flexible (Polak-Ribiere) beta, which tolerates a varying preconditioner
     int flexible() const
     {
-        return precond == FS_PRECOND_MULTIGRID && mg_prec == FS_MG_BF16_VECTORS && active16();
+        return flexible_cg || (precond == FS_PRECOND_MULTIGRID && mg_prec == FS_MG_BF16_VECTORS && active16());
     }
 
     void precondition(const int* status)
@@ -1555,6 +1556,17 @@ fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode)
         grid->mg_prec = mode;
         grid->apply_precision();
         grid->tune_for_precision();
+    });
+}
+
+fs_status fs_set_flexible_cg(fs_grid* grid, int enabled)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        if (grid->flexible_cg != (enabled != 0)) {
+            grid->flexible_cg = enabled != 0;
+            grid->drop_graph();
+        }
     });
 }
 

@@ -356,6 +356,10 @@ class FluidGridHIP:
     def setMultigridPrecision(self, mode: int):
         call("fs_set_multigrid_precision", self._h, int(mode))
 
+    def setFlexibleCG(self, enabled: bool):
+        """Polak-Ribiere (flexible) CG beta instead of the reference's Fletcher-Reeves one."""
+        call("fs_set_flexible_cg", self._h, int(bool(enabled)))
+
     def getMultigridPrecision(self) -> int:
         m = C.c_int()
         call("fs_get_multigrid_precision", self._h, C.byref(m))

@@ -171,6 +171,11 @@ fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask);
 /* Multigrid level storage (fs_mg_precision above).  FS_E_INVALID_ARGUMENT for an unknown mode. */
 fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode);
 fs_status fs_get_multigrid_precision(const fs_grid* grid, fs_mg_precision* mode);
+/* Flexible CG (no reference counterpart; the reference's solvePressure uses the Fletcher-Reeves beta,
+ * FluidGridCPU.cpp:1807-1816): nonzero makes the PCG use the Polak-Ribiere beta
+ * z_new.(r_new - r_old) / z_old.r_old (Notay 2000), which tolerates a preconditioner that is not exactly
+ * linear.  FS_MG_BF16_VECTORS turns it on by itself. */
+fs_status fs_set_flexible_cg(fs_grid* grid, int enabled);
 /* Number of solid cells of the current mask (0 without one). */
 fs_status fs_solid_cells(const fs_grid* grid, long long* count);
 
