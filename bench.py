@@ -164,7 +164,7 @@ def smoother_model_bytes(coef_bytes, coarse_bytes):
     return 4 * (12 + 2 * coef_bytes) + 8 + 0.25 * coarse_bytes
 
 
-def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400):
+def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, only=None):
     """Config C3 on one GPU: the obstacle tank (40 seeded circles and boxes over 15% of the cells, liquid
     in the bottom 70%, vortex velocity) projected to 1e-6 of its initial max divergence with each
     multigrid storage mode, from the same device-resident input every solve.  Reports solves/s, PCG
@@ -184,6 +184,8 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400):
     note(f"generated the {size}^2 obstacle tank ({mask.mean():.3f} solid)")
     modes = [("fp32", FS_MG_FP32, 4, 4), ("fp16", FS_MG_FP16, 2, 4), ("bf16", FS_MG_BF16, 2, 4),
              ("bf16_vectors", FS_MG_BF16_VECTORS, 2, 2)]
+    if only:
+        modes = [m for m in modes if m[0] in only]
     out = {"workload": f"obstacle_tank {size}x{size}: 40 circles/boxes over {mask.mean():.3f} of the cells, "
                        f"liquid in the bottom 70% (1000:1), vortex velocity, seed {seed}; one projection per "
                        "solve to 1e-6 of the initial max divergence", "modes": {}}
@@ -343,6 +345,7 @@ def main():
     ap.add_argument("--precision-size", type=int, default=2048, help="side of the C3 mixed-precision leg (0: skip)")
     ap.add_argument("--precision-steps", type=int, default=3)
     ap.add_argument("--precision-only", action="store_true")
+    ap.add_argument("--precision-modes", type=str, default="", help="subset of fp32,fp16,bf16,bf16_vectors")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -359,7 +362,9 @@ def main():
         print(json.dumps(batch_leg(args.batch_instances or 256)), flush=True)
         return
     if args.precision_only:
-        print(json.dumps(precision_leg(args.precision_size or 2048, steps=args.precision_steps)), flush=True)
+        only = [m for m in args.precision_modes.split(",") if m] or None
+        print(json.dumps(precision_leg(args.precision_size or 2048, steps=args.precision_steps, only=only)),
+              flush=True)
         return
     if args.serving_only:
         print(json.dumps(serving_leg(args.serving_requests or 24, rates, args.serving_steps,

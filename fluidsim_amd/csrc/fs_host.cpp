@@ -203,6 +203,7 @@ struct fs_context {
 };
 
 constexpr int kPressureBudget = 100000;   // PCG iterations of an unbounded (negative) budget
+constexpr size_t kFoldBlocks = 1536;      // cell blocks (64 x 16) beyond which PCG partials are pre-reduced
 
 struct fs_grid {
     fs_context* ctx = nullptr;
@@ -348,7 +349,15 @@ struct fs_grid {
         g.zdiag = zdiag.p;
         g.S = S.p;
         g.partials = partials.p;
-        g.partials_ad = partials.p + 5 * ((size_t)((W + 63) / 64) * ((H + 15) / 16));
+        const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
+        g.partials_ad = partials.p + 5 * blocks;
+        // every consumer workgroup reading every block's partial costs blocks^2 loads: past this many
+        // blocks (a 1024 x 1536 grid) one extra single-workgroup launch pre-reduces them
+        const bool fold = blocks > kFoldBlocks;
+        g.fold_ad = fold ? partials.p + 6 * blocks : g.partials_ad;
+        g.fold_ad_n = fold ? 1 : (int)blocks;
+        g.fold_rz = fold ? partials.p + 6 * blocks + 8 : partials.p;
+        g.fold_rz_n = fold ? 1 : (int)blocks;
         g.solid = solid.n ? solid.p : nullptr;
         return g;
     }
@@ -394,7 +403,7 @@ struct fs_grid {
         bytes += S_aux.alloc(1);
         const size_t blocks = (size_t)((W + 63) / 64) * ((H + 15) / 16);
         // region A: up to five partials per cell block; region B (after it): the operator's d.q partials,
-        // read by the update kernel while it writes region A
+        // read by the update kernel while it writes region A; then 16 slots of pre-reduced sums
         bytes += partials.alloc(6 * blocks + 64);
         build_hierarchy();
         tune_tiles();

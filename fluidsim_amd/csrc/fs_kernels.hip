@@ -695,6 +695,18 @@ __global__ __launch_bounds__(kNT) void k_pcg_ad(GridDev g)
         g.partials_ad[block_id()] = bs;
 }
 
+// Large grids: one workgroup reduces the per-block partials (stride components) into out[0 .. stride),
+// with the same fixed-order finish_sum every consumer workgroup would otherwise repeat over all blocks.
+__global__ __launch_bounds__(kNT) void k_fold_partials(const double* partials, int nb, int stride, double* out)
+{
+    __shared__ double lds[kNT / 64];
+    for (int c = 0; c < stride; ++c) {
+        const double v = finish_sum<kNT>(partials, nb, stride, c, lds);
+        if (threadIdx.x == 0)
+            out[c] = v;
+    }
+}
+
 // x += alpha d ; r -= alpha q ; residual sum and max-norm; convergence test of the recurrence.
 // Every workgroup first reduces the operator's d.q partials itself (same order, same value) and forms
 // alpha; workgroup 0 records it, clears the restart flag the previous direction consumed, and stops
@@ -714,7 +726,7 @@ __global__ __launch_bounds__(kNT) void k_pcg_update(GridDev g)
     }
     if (g.S->status != kRunning)
         return;
-    const double dq = finish_sum<kNT>(g.partials_ad, (int)block_count(), 1, 0, lds);
+    const double dq = finish_sum<kNT>(g.fold_ad, g.fold_ad_n, 1, 0, lds);
     const double rz = g.S->rz_slot[g.S->iterations & 1];
     bool stop = dq <= 0.0 || !isfinite(dq);
     const double alpha64 = stop ? 0.0 : rz / dq;
@@ -832,19 +844,18 @@ template <bool FLEX> __global__ __launch_bounds__(kNT) void k_pcg_direction(Grid
     }
     if (g.S->status != kRunning)
         return;
-    const int nb = (int)block_count();
     constexpr int NP = FLEX ? 5 : 3;
-    const double tz = finish_sum<kNT>(g.partials, nb, NP, 0, lds);
-    const double trz = finish_sum<kNT>(g.partials, nb, NP, 1, lds);
-    const double tr = finish_sum<kNT>(g.partials, nb, NP, 2, lds);
+    const double tz = finish_sum<kNT>(g.fold_rz, g.fold_rz_n, NP, 0, lds);
+    const double trz = finish_sum<kNT>(g.fold_rz, g.fold_rz_n, NP, 1, lds);
+    const double tr = finish_sum<kNT>(g.fold_rz, g.fold_rz_n, NP, 2, lds);
     const double n = (double)g.W * g.H;
     const double zmean = tz / n;
     const double next = trz - zmean * tr;
     const int c = g.S->iterations;
     double numerator = next;
     if (FLEX) {
-        const double tzq = finish_sum<kNT>(g.partials, nb, NP, 3, lds);
-        const double tq = finish_sum<kNT>(g.partials, nb, NP, 4, lds);
+        const double tzq = finish_sum<kNT>(g.fold_rz, g.fold_rz_n, NP, 3, lds);
+        const double tq = finish_sum<kNT>(g.fold_rz, g.fold_rz_n, NP, 4, lds);
         numerator = -g.S->alpha * (tzq - zmean * tq);
     }
     const double beta64 = g.S->restart ? 0.0 : numerator / g.S->rz_slot[(c + 1) & 1];
@@ -1170,6 +1181,9 @@ void launch_pad_copy(hipStream_t st, const double* src, double* dst, int W, int 
 void launch_pcg_ad(hipStream_t st, const GridDev& g)
 {
     hipLaunchKernelGGL(k_pcg_ad, cell_grid(g.W, g.H), dim3(kNT), 0, st, g);
+    if (g.fold_ad_n == 1)
+        hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(kNT), 0, st, (const double*)g.partials_ad,
+            (int)cell_blocks(g.W, g.H), 1, g.fold_ad);
 }
 
 void launch_pcg_update(hipStream_t st, const GridDev& g)
@@ -1184,6 +1198,9 @@ void launch_pcg_rz(hipStream_t st, const GridDev& g, const float* z, int flexibl
         hipLaunchKernelGGL(k_pcg_rz<true>, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
     else
         hipLaunchKernelGGL(k_pcg_rz<false>, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, z);
+    if (g.fold_rz_n == 1)
+        hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(kNT), 0, st, (const double*)g.partials,
+            (int)cell_blocks(g.W, g.H), flexible ? 5 : 3, g.fold_rz);
 }
 
 void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z, int flexible)

@@ -31,6 +31,14 @@ struct GridDev {
     PcgScalars* S;            // device scalar block
     double* partials;         // reduction partials (>= 4 * max blocks)
     double* partials_ad;      // d.q partials of the operator kernel (one per block)
+    // Where the update and direction kernels read the d.q and r.z partials, and how many blocks' worth:
+    // the producers' own per-block partials (every consumer workgroup reduces them: cheap while the
+    // grid has few blocks), or, on large grids, one pre-reduced entry written by k_fold_partials (the
+    // same fixed-order sum, so the result is identical).
+    double* fold_ad;
+    int fold_ad_n;
+    double* fold_rz;
+    int fold_rz_n;
     const unsigned char* solid;   // W * H, nonzero = solid cell (solid-mask extension); null: no mask
 };
 

@@ -181,3 +181,25 @@ def test_vortex_box_1024_reaches_relative_tolerance_in_every_mode():
         g.destroy()
     print("vortex box 1024^2 iterations:", its)
     assert its["fp16"] <= its["fp32"] + 2 and its["bf16"] <= its["fp32"] + 2 and its["bf16v"] <= 2 * its["fp32"] + 2
+
+
+def test_flexible_cg_reaches_the_same_answer(oracle):
+    """fs_set_flexible_cg swaps the Fletcher-Reeves beta for Polak-Ribiere: same stopping rule, same
+    answer (FP64 oracle within 1e-5), within a few iterations of the default."""
+    wl = WL.dam_break(128, 128, seed=1)
+    o = oracle_grid(oracle, wl)
+    o.apply_gravity(wl.dt)
+    ro = o.project(wl.dt, 500, 1e-6)
+    so = o.download()
+    its = []
+    for flexible in (True, False):
+        g = hip_grid(wl)
+        g.setFlexibleCG(flexible)
+        g.applyGravity(wl.dt)
+        r = g.project(wl.dt, 500, 1e-6)
+        assert r.converged
+        s = g.download()
+        assert rel_l2(s.u, so["u"]) < REL_L2 and rel_l2(s.v, so["v"]) < REL_L2
+        its.append(r.iterations)
+        g.destroy()
+    assert ro["converged"] and abs(its[0] - its[1]) <= 5, (its, ro["iterations"])
