@@ -97,16 +97,19 @@ def cpu_baseline(wl, tau, budget_s):
 def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048):
     """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
     [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each).
-    It is first served one request at a time on one stream (serial plain), then fed into the serving loop
-    (`lanes` concurrent streams, HBM-budget admission) at each arrival rate in `rates` (per second).
-    Reports aggregate steps/s, per-step latency p50/p99 and queueing delay of every run."""
+    It is first served one request at a time on one stream with FP32 multigrid levels (serial plain), then
+    fed into the serving loop (`lanes` concurrent streams, HBM-budget admission) with bfloat16 multigrid
+    conductances at each arrival rate in `rates` (per second).  Reports aggregate steps/s, per-step
+    latency p50/p99 and queueing delay of every run."""
+    from fluidsim_amd import FS_MG_BF16, FS_MG_FP32
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
 
-    def stream_at(rate):
-        return [(t, Request(**kw)) for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps)]
+    def stream_at(rate, precision=FS_MG_BF16):
+        return [(t, Request(multigrid_precision=precision, **kw))
+                for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps)]
 
-    base = stream_at(rates[0])
+    base = stream_at(rates[0], FS_MG_FP32)
     total_steps = sum(r.steps for _, r in base)
     with Server(lanes=1) as srv:
         t0 = time.perf_counter()
@@ -146,7 +149,8 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048):
         })
     return {
         "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each (PLIC transport, "
-                    "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to |div u| <= 1e-4), "
+                    "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to |div u| <= 1e-4; FP32 "
+                    "multigrid levels served one at a time, bfloat16 conductances in the serving loop), "
                     f"Poisson arrivals, seed {seed}",
         "lanes": lanes,
         "cells_mean": float(np.mean([r.width * r.height for _, r in base])),

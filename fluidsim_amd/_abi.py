@@ -102,12 +102,14 @@ class FsRequest(C.Structure):
         ("gas_density", C.POINTER(C.c_double)), ("gas_viscosity", C.POINTER(C.c_double)),
         ("liquid_fraction", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("liquid_mass", C.POINTER(C.c_double)), ("gas_mass", C.POINTER(C.c_double)),
+        ("multigrid_precision", C.c_int), ("solid", C.c_void_p),
     ]
 
 
 FS_REQUEST_QUEUED, FS_REQUEST_RUNNING, FS_REQUEST_DONE, FS_REQUEST_FAILED = 0, 1, 2, 3
 
 FS_FEATURE_TRANSPORT, FS_FEATURE_VISCOSITY, FS_FEATURE_SURFACE_TENSION = 1, 2, 4
+FS_FEATURE_SOLID_MASK, FS_FEATURE_MG16 = 8, 16
 
 
 class FsRequestReport(C.Structure):

@@ -184,6 +184,41 @@ thread_local std::string g_last_error;
 
 void set_last_error(const std::string& message) { g_last_error = message; }
 
+std::string solid_mask_problem(const unsigned char* mask, int W, int H)
+{
+    const size_t N = (size_t)W * H;
+    size_t fluid = 0, start = N;
+    for (size_t i = 0; i < N; ++i)
+        if (!mask[i]) {
+            ++fluid;
+            if (start == N)
+                start = i;
+        }
+    if (fluid == 0)
+        return "a solid mask must leave at least one fluid cell";
+    std::vector<unsigned char> seen(N, 0);
+    std::vector<size_t> stack{start};
+    seen[start] = 1;
+    size_t reached = 0;
+    while (!stack.empty()) {
+        const size_t i = stack.back();
+        stack.pop_back();
+        ++reached;
+        const int x = (int)(i % W), y = (int)(i / W);
+        auto visit = [&](size_t j) {
+            if (!mask[j] && !seen[j]) {
+                seen[j] = 1;
+                stack.push_back(j);
+            }
+        };
+        if (x > 0) visit(i - 1);
+        if (x + 1 < W) visit(i + 1);
+        if (y > 0) visit(i - W);
+        if (y + 1 < H) visit(i + W);
+    }
+    return reached == fluid ? std::string() : "the fluid cells of a solid mask must form one 4-connected region";
+}
+
 }  // namespace fsim
 
 using namespace fsim;
@@ -1502,39 +1537,13 @@ fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask)
         std::vector<unsigned char> m;
         long long solids = 0;
         if (mask) {
+            const std::string problem = solid_mask_problem(mask, W, H);
+            need(problem.empty(), FS_E_INVALID_ARGUMENT, problem.c_str());
             m.resize(N);
             for (size_t i = 0; i < N; ++i) {
                 m[i] = mask[i] ? 1 : 0;
                 solids += m[i];
             }
-            need(solids < (long long)N, FS_E_INVALID_ARGUMENT, "a solid mask must leave at least one fluid cell");
-            // the fluid cells must form one 4-connected region
-            std::vector<unsigned char> seen(N, 0);
-            std::vector<size_t> stack;
-            size_t start = 0;
-            while (m[start])
-                ++start;
-            stack.push_back(start);
-            seen[start] = 1;
-            size_t reached = 0;
-            while (!stack.empty()) {
-                const size_t i = stack.back();
-                stack.pop_back();
-                ++reached;
-                const int x = (int)(i % W), y = (int)(i / W);
-                auto visit = [&](size_t j) {
-                    if (!m[j] && !seen[j]) {
-                        seen[j] = 1;
-                        stack.push_back(j);
-                    }
-                };
-                if (x > 0) visit(i - 1);
-                if (x + 1 < W) visit(i + 1);
-                if (y > 0) visit(i - W);
-                if (y + 1 < H) visit(i + W);
-            }
-            need(reached == N - (size_t)solids, FS_E_INVALID_ARGUMENT,
-                "the fluid cells of a solid mask must form one 4-connected region");
         }
         set_device(grid->ctx);
         if (mask) {
@@ -2595,6 +2604,18 @@ fs_status fs_grid_estimate_bytes(int width, int height, int n_liquid, int n_gas,
         }
         if (features & FS_FEATURE_SURFACE_TENSION)
             b += 4 * 16 * N;
+        if (features & FS_FEATURE_SOLID_MASK)
+            b += N;
+        if (features & FS_FEATURE_MG16) {   // 16-bit conductances and (bfloat16 vectors) rhs + 2 solutions
+            size_t lw = W, lh = H;
+            for (;;) {
+                b += (lw + 2) * (lh + 2) * 2 * 5;
+                if (lw * lh <= 64)
+                    break;
+                lw = (lw + 1) / 2;
+                lh = (lh + 1) / 2;
+            }
+        }
         *bytes = b;
     });
 }

@@ -14,4 +14,8 @@ void set_last_error(const std::string& message);
 // implicitly synchronising call.
 std::mutex& device_setup_mutex();
 
+// Checks a W x H solid mask (nonzero = solid): at least one fluid cell, and the fluid cells one
+// 4-connected region.  Returns an empty string, or the reason it is refused.
+std::string solid_mask_problem(const unsigned char* mask, int W, int H);
+
 }  // namespace fsim

@@ -81,6 +81,7 @@ struct Request {
     long long id = 0;
     fs_request p{};   // scalar parameters; the pointers below replace the caller's
     std::vector<double> rho_l, mu_l, rho_g, mu_g, fraction, u, v, lmass, gmass;
+    std::vector<unsigned char> solid;
     size_t estimate = 0;
     int state = FS_REQUEST_QUEUED;
     int lane = -1;
@@ -209,7 +210,9 @@ struct fs_server {
                ok(fs_set_gravity(g, p.gravity_y)) && ok(fs_set_surface_tension(g, p.surface_tension)) &&
                ok(fs_set_transport(g, (fs_transport)p.transport)) &&
                ok(fs_set_preconditioner(g, (fs_preconditioner)p.preconditioner)) &&
-               ok(fs_upload(g, ptr(r->fraction), ptr(r->u), ptr(r->v), nullptr, ptr(r->lmass), ptr(r->gmass)));
+               ok(fs_upload(g, ptr(r->fraction), ptr(r->u), ptr(r->v), nullptr, ptr(r->lmass), ptr(r->gmass))) &&
+               (r->solid.empty() || ok(fs_set_solid_mask(g, r->solid.data()))) &&
+               ok(fs_set_multigrid_precision(g, (fs_mg_precision)p.multigrid_precision));
     }
 
     void run_lane(Lane& lane)
@@ -391,6 +394,12 @@ fs_status fs_server_submit(fs_server* server, const fs_request* request, long lo
             FS_E_INVALID_ARGUMENT, "unknown preconditioner");
         need(std::isfinite(q.surface_tension) && q.surface_tension >= 0.0, FS_E_INVALID_ARGUMENT,
             "Surface tension must be finite and nonnegative");
+        need(q.multigrid_precision >= FS_MG_FP32 && q.multigrid_precision <= FS_MG_BF16_VECTORS,
+            FS_E_INVALID_ARGUMENT, "unknown multigrid precision");
+        if (q.solid) {
+            const std::string problem = fsim::solid_mask_problem(q.solid, q.width, q.height);
+            need(problem.empty(), FS_E_INVALID_ARGUMENT, problem);
+        }
         auto r = std::make_unique<Request>();
         r->p = q;
         const size_t N = (size_t)q.width * q.height;
@@ -409,6 +418,9 @@ fs_status fs_server_submit(fs_server* server, const fs_request* request, long lo
         copy(r->gmass, q.gas_mass, (size_t)q.n_gas * N);
         r->p.liquid_density = r->p.liquid_viscosity = r->p.gas_density = r->p.gas_viscosity = nullptr;
         r->p.liquid_fraction = r->p.u = r->p.v = r->p.liquid_mass = r->p.gas_mass = nullptr;
+        if (q.solid)
+            r->solid.assign(q.solid, q.solid + N);
+        r->p.solid = nullptr;
         bool viscous = false;
         for (double mu : r->mu_l)
             viscous |= mu > 0.0;
@@ -416,7 +428,9 @@ fs_status fs_server_submit(fs_server* server, const fs_request* request, long lo
             viscous |= mu > 0.0;
         const int features = (q.transport == FS_TRANSPORT_REFERENCE ? FS_FEATURE_TRANSPORT : 0) |
                              (viscous ? FS_FEATURE_VISCOSITY : 0) |
-                             (q.surface_tension > 0.0 ? FS_FEATURE_SURFACE_TENSION : 0);
+                             (q.surface_tension > 0.0 ? FS_FEATURE_SURFACE_TENSION : 0) |
+                             (q.solid ? FS_FEATURE_SOLID_MASK : 0) |
+                             (q.multigrid_precision != FS_MG_FP32 ? FS_FEATURE_MG16 : 0);
         need(fs_grid_estimate_bytes(q.width, q.height, q.n_liquid, q.n_gas, features, &r->estimate) == FS_OK,
             FS_E_INVALID_ARGUMENT, "cannot size the request");
         need(r->estimate <= server->budget, FS_E_INVALID_ARGUMENT, "request is larger than the HBM budget");

@@ -39,12 +39,16 @@ class Request:
     v: Optional[np.ndarray] = None
     liquid_mass: Optional[np.ndarray] = None     # component-major, n_liquid * W * H
     gas_mass: Optional[np.ndarray] = None
+    multigrid_precision: int = _abi.FS_MG_FP32   # storage of the multigrid levels (fs_mg_precision)
+    solid: Optional[np.ndarray] = None           # W * H obstacle mask, nonzero = solid
 
     def estimate_bytes(self) -> int:
         viscous = any(m > 0 for m in (self.liquid_viscosity or []) + (self.gas_viscosity or []))
         features = ((_abi.FS_FEATURE_TRANSPORT if self.transport == _abi.FS_TRANSPORT_REFERENCE else 0)
                     | (_abi.FS_FEATURE_VISCOSITY if viscous else 0)
-                    | (_abi.FS_FEATURE_SURFACE_TENSION if self.surface_tension > 0 else 0))
+                    | (_abi.FS_FEATURE_SURFACE_TENSION if self.surface_tension > 0 else 0)
+                    | (_abi.FS_FEATURE_SOLID_MASK if self.solid is not None else 0)
+                    | (_abi.FS_FEATURE_MG16 if self.multigrid_precision != _abi.FS_MG_FP32 else 0))
         return estimate_bytes(self.width, self.height, len(self.liquid_density), len(self.gas_density), features)
 
 
@@ -125,9 +129,15 @@ class Server:
                 _arr(r.gas_viscosity, ng), _arr(r.liquid_fraction, N), _arr(r.u, (W + 1) * H),
                 _arr(r.v, W * (H + 1)), _arr(r.liquid_mass, nl * N), _arr(r.gas_mass, ng * N)]
         ptrs = [None if a is None else a.ctypes.data_as(C.POINTER(C.c_double)) for a in keep]
+        solid = None
+        if r.solid is not None:
+            solid = np.ascontiguousarray(r.solid, dtype=np.uint8).ravel()
+            if solid.size != N:
+                raise ValueError(f"solid mask has {solid.size} cells, expected {N}")
         req = FsRequest(W, H, float(r.cell_size), int(r.steps), float(r.dt), int(r.max_iterations),
                         float(r.divergence_tolerance), float(r.gravity_y), float(r.surface_tension), int(r.transport),
-                        int(r.preconditioner), nl, ng, *ptrs)
+                        int(r.preconditioner), nl, ng, *ptrs, int(r.multigrid_precision),
+                        None if solid is None else solid.ctypes.data)
         rid = C.c_longlong()
         call("fs_server_submit", self._h, C.byref(req), C.byref(rid))   # copies the arrays
         return rid.value

@@ -295,7 +295,9 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
 enum {
     FS_FEATURE_TRANSPORT = 1,          /* reference PLIC/VOF transport */
     FS_FEATURE_VISCOSITY = 2,          /* implicit viscosity solve */
-    FS_FEATURE_SURFACE_TENSION = 4     /* capillary geometry */
+    FS_FEATURE_SURFACE_TENSION = 4,    /* capillary geometry */
+    FS_FEATURE_SOLID_MASK = 8,         /* fs_set_solid_mask */
+    FS_FEATURE_MG16 = 16               /* 16-bit multigrid levels (any fs_mg_precision but FP32) */
 };
 fs_status fs_grid_estimate_bytes(int width, int height, int n_liquid, int n_gas, int features, size_t* bytes);
 
@@ -411,6 +413,9 @@ typedef struct fs_request {
     const double* v;
     const double* liquid_mass;
     const double* gas_mass;
+    /* extensions (zero: FP32 levels, no obstacles) */
+    int multigrid_precision;        /* fs_mg_precision */
+    const unsigned char* solid;     /* W*H solid mask (fs_set_solid_mask) or NULL; copied by fs_server_submit */
 } fs_request;
 
 enum { FS_REQUEST_QUEUED = 0, FS_REQUEST_RUNNING = 1, FS_REQUEST_DONE = 2, FS_REQUEST_FAILED = 3 };

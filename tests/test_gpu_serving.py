@@ -155,3 +155,44 @@ def test_requests_keep_arriving_while_others_run():
         assert rs.done and rs.lane == 0
         assert rl.steps_done < 40, "the short request waited for the long one to finish"
         assert srv.wait(long_id).done
+
+
+def test_requests_with_obstacles_and_16bit_levels():
+    """A request may carry an obstacle mask and a multigrid storage mode: its result equals the same steps
+    on a plain grid configured alike, bit for bit; an invalid mask or mode is refused at submission."""
+    from fluidsim_amd import FS_MG_BF16
+
+    W, H = 96, 80
+    wl = WL.obstacle_tank(W, H, seed=7)
+    mask = wl.meta["solid"]
+    probe = request(W, H, 40, steps=4, viscosity=0.01, sigma=0.02, multigrid_precision=FS_MG_BF16, solid=mask)
+    with Server(lanes=2) as srv:
+        with pytest.raises(ValueError):
+            srv.submit(request(W, H, 41, solid=np.ones(W * H, np.uint8)))
+        with pytest.raises(ValueError):
+            srv.submit(request(W, H, 41, multigrid_precision=9))
+        other = srv.submit(request(128, 64, 42, steps=3))
+        rid = srv.submit(probe)
+        rep = srv.wait(rid)
+        assert rep.done, rep.error
+        assert srv.wait(other).done
+        out = srv.result(rid, W, H)
+    g = FluidGridHIP(W, H, probe.cell_size)
+    g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0, probe.liquid_viscosity[0])]),
+                         FluidMixture([MixtureComponent("air", 0.1, probe.gas_viscosity[0])]))
+    g.setGravityY(probe.gravity_y)
+    g.setSurfaceTension(probe.surface_tension)
+    g.upload(FluidGridState(probe.liquid_fraction, probe.u, probe.v, np.zeros(0), [probe.liquid_mass],
+                            [probe.gas_mass]))
+    g.setSolidMask(mask)
+    g.setMultigridPrecision(FS_MG_BF16)
+    iterations = 0
+    for _ in range(probe.steps):
+        r = g.step(probe.dt, probe.max_iterations, probe.divergence_tolerance)
+        assert r.accepted()
+        iterations += r.iterations
+    s = g.download()
+    g.destroy()
+    assert iterations == rep.pressure_iterations
+    for k, key in (("u", "u"), ("v", "v"), ("pressure", "pressure"), ("liquid_fraction", "fraction")):
+        assert np.array_equal(getattr(s, k), out[key]), k
