@@ -442,8 +442,8 @@ def main():
         "converged_all": bool(ok),
         "max_rel_residual": float(max(rel)),
         "roofline": {
-            "kernel": f"k_rb_multi<{tx},{ty},4,{1 if (tx, ty) == (64, 32) else 0},1,{nt}> (finest-level "
-                      "prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
+            "kernel": f"k_rb_multi<{tx},{ty},4,{1 if (tx, ty) == (64, 32) else 0},1,{nt},float,float,float> "
+                      "(finest-level prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
