@@ -157,11 +157,32 @@ template <int NT> __device__ __forceinline__ double block_max(double v, double* 
 __device__ __forceinline__ double load_partial(const double* p, int i) { return p[i]; }
 
 // Sum of `n` partials with stride `k` starting at `offset`, in a fixed order; every thread returns it.
+// Loads run eight (then four) strides ahead of the additions (independent loads in flight instead of one memory
+// latency per partial); the additions keep the one-stride-at-a-time order, so the sum is unchanged.
 template <int NT>
 __device__ __forceinline__ double finish_sum(const double* partials, int n, int k, int offset, double* lds)
 {
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += NT)
+    int i = threadIdx.x;
+    for (; i + 7 * NT < n; i += 8 * NT) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            t[u] = load_partial(partials, (i + u * NT) * k + offset);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v += t[u];
+    }
+    for (; i + 3 * NT < n; i += 4 * NT) {
+        double t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            t[u] = load_partial(partials, (i + u * NT) * k + offset);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            v += t[u];
+    }
+    for (; i < n; i += NT)
         v += load_partial(partials, i * k + offset);
     double r = block_sum<NT>(v, lds);
     __shared__ double out;
@@ -177,7 +198,26 @@ template <int NT>
 __device__ __forceinline__ double finish_max(const double* partials, int n, int k, int offset, double* lds)
 {
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += NT)
+    int i = threadIdx.x;
+    for (; i + 7 * NT < n; i += 8 * NT) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            t[u] = load_partial(partials, (i + u * NT) * k + offset);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v = fmax(v, t[u]);
+    }
+    for (; i + 3 * NT < n; i += 4 * NT) {
+        double t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            t[u] = load_partial(partials, (i + u * NT) * k + offset);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            v = fmax(v, t[u]);
+    }
+    for (; i < n; i += NT)
         v = fmax(v, load_partial(partials, i * k + offset));
     double r = block_max<NT>(v, lds);
     __shared__ double out;
