@@ -2,7 +2,8 @@
 // Mirrors the contract checks of the reference's tests/FluidGridAPITests.cpp: deterministic exact
 // results with and without timings (:83-134), zero-time and zero-iteration steps reported, not thrown
 // (:186-229), upload validation, the hydrostatic closed form of tests/FluidGridTests.cpp:140-160,
-// checkpoint continuation (tests/FluidWebGPUAPITests.cpp:518-666) and the scene frame schedule.
+// checkpoint continuation (tests/FluidWebGPUAPITests.cpp:518-666), the scene frame schedule, and the
+// obstacle-mask and multigrid-storage extensions.
 // Needs a GPU; run by tests/test_cpp_adapter.py.
 #include "fluidsim_amd/host/FluidGridHIP.hpp"
 
@@ -213,6 +214,52 @@ void sceneFrame(HipContext& ctx)
     require(sameState(framed.download(), stepped.download()), "frame differs from its explicit steps");
 }
 
+// Obstacles and 16-bit multigrid levels through the adapter: solid faces stay closed, every storage mode
+// converges, an enclosed pocket is refused, and removing the mask restores the unmasked result.
+void obstaclesAndPrecision(HipContext& ctx)
+{
+    const int W = 160, H = 120;
+    std::vector<unsigned char> mask((size_t)W * H, 0);
+    for (int y = 40; y < 80; ++y)
+        for (int x = 60; x < 70; ++x)
+            mask[(size_t)y * W + x] = 1;
+    std::vector<double> u_after;
+    for (fs_mg_precision mode : {FS_MG_FP32, FS_MG_FP16, FS_MG_BF16, FS_MG_BF16_VECTORS}) {
+        FluidGridHIP grid(ctx, W, H, 1.0 / W);
+        configure(grid);
+        grid.setSolidMask(mask);
+        grid.setMultigridPrecision(mode);
+        require(grid.solidCells() == 400 && grid.getMultigridPrecision() == mode, "configuration");
+        grid.addVortex({-6.5, 11.35}, 0.1, 1.0);
+        const auto r = grid.project(0.01, 500, 1e-6);
+        require(r.converged, "masked projection did not converge");
+        const auto s = grid.download();
+        for (int y = 40; y < 80; ++y)
+            for (int x = 60; x <= 70; ++x)
+                require(s.u[(size_t)y * (W + 1) + x] == 0.0, "a solid face carries velocity");
+    }
+    FluidGridHIP plain(ctx, W, H, 1.0 / W), toggled(ctx, W, H, 1.0 / W);
+    configure(plain);
+    configure(toggled);
+    toggled.setSolidMask(mask);
+    toggled.setSolidMask({});
+    plain.addVortex({-6.5, 11.35}, 0.1, 1.0);
+    toggled.addVortex({-6.5, 11.35}, 0.1, 1.0);
+    require(plain.project(0.01, 500, 1e-6).iterations == toggled.project(0.01, 500, 1e-6).iterations,
+        "removing the mask changed the solve");
+    require(sameState(plain.download(), toggled.download()), "removing the mask changed the state");
+    std::vector<unsigned char> pocket((size_t)W * H, 0);
+    for (int x = 0; x < W; ++x)
+        pocket[(size_t)60 * W + x] = 1;
+    bool threw = false;
+    try {
+        plain.setSolidMask(pocket);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    require(threw && plain.solidCells() == 0, "a mask splitting the fluid was accepted");
+}
+
 }  // namespace
 
 int main()
@@ -224,7 +271,8 @@ int main()
     } tests[] = {{"timed and untimed steps are identical", timedAndUntimedStepsAreIdentical},
         {"zero time and zero iterations are reported", zeroIterationAndZeroTimeAreReported},
         {"upload validation", uploadIsValidated}, {"hydrostatic pressure", hydrostaticPressure},
-        {"checkpoint continuation", checkpointContinuation}, {"scene frame", sceneFrame}};
+        {"checkpoint continuation", checkpointContinuation}, {"scene frame", sceneFrame},
+        {"obstacles and multigrid precision", obstaclesAndPrecision}};
     int failures = 0;
     for (const auto& t : tests) {
         try {
