@@ -43,7 +43,9 @@ struct VisScalars {
     double truth;     // recomputed r M^-1 r
     int steps;        // completed alpha steps since the last (re)start
     int bad;          // breakdown or non-finite value
-    int pad[2];
+    float dt;         // time step of this solve: the iteration kernels read it here, so one captured
+                      // batch of iterations serves every dt
+    int pad;
 };
 
 struct VLev {
@@ -232,10 +234,11 @@ __global__ __launch_bounds__(kVT) void k_vis_initial(VLev L, const float2* v0, f
 // ---- V-cycle ----
 
 // "viscosity symmetric smoother" (ViscosityKernels.h:214-236), damped Jacobi with weight 1/2.
-__global__ __launch_bounds__(kVT) void k_vis_smooth(VLev L, const float2* b, int zero, float dt, const VisScalars* S)
+__global__ __launch_bounds__(kVT) void k_vis_smooth(VLev L, const float2* b, int zero, const VisScalars* S)
 {
     if (stopped(S))
         return;
+    const float dt = S->dt;
     VIS_SLOT(L)
     if (!inside)
         return;
@@ -255,10 +258,11 @@ __global__ __launch_bounds__(kVT) void k_vis_smooth(VLev L, const float2* b, int
 }
 
 // "viscosity true residual" on a level (ViscosityKernels.h:187-195)
-__global__ __launch_bounds__(kVT) void k_vis_residual(VLev L, const float2* b, float dt, const VisScalars* S)
+__global__ __launch_bounds__(kVT) void k_vis_residual(VLev L, const float2* b, const VisScalars* S)
 {
     if (stopped(S))
         return;
+    const float dt = S->dt;
     VIS_SLOT(L)
     if (!inside)
         return;
@@ -353,12 +357,13 @@ __global__ __launch_bounds__(kVT) void k_vis_prolong(VLev F, VLev C, const VisSc
 
 // "viscosity shared coarse solve" (ViscosityKernels.h:242-282): 16 Jacobi sweeps from zero in LDS.
 constexpr int kCoarseSlots = 256;
-__global__ __launch_bounds__(kCoarseSlots) void k_vis_coarse(VLev L, const float2* b, float dt, const VisScalars* S)
+__global__ __launch_bounds__(kCoarseSlots) void k_vis_coarse(VLev L, const float2* b, const VisScalars* S)
 {
     __shared__ float2 vel[kCoarseSlots];
     __shared__ float4 cof[kCoarseSlots];
     if (stopped(S))
         return;
+    const float dt = S->dt;
     const int lane = threadIdx.x;
     const int stride = L.w + 1;
     const int n = stride * (L.h + 1);
@@ -435,12 +440,13 @@ __global__ __launch_bounds__(kVT) void k_vis_direction(VLev L, const float2* z, 
 }
 
 // q = A d and the mode-2 reduction d . q
-__global__ __launch_bounds__(kVT) void k_vis_apply(VLev L, const float2* d, float2* q, float dt, const VisScalars* S,
+__global__ __launch_bounds__(kVT) void k_vis_apply(VLev L, const float2* d, float2* q, const VisScalars* S,
     double* partials)
 {
     __shared__ double lds[kVT / 64];
     if (stopped(S))
         return;
+    const float dt = S->dt;
     VIS_SLOT(L)
     double a = 0.0, bad = 0.0;
     if (inside) {
@@ -510,7 +516,8 @@ enum VisFinish : int { kVisInitial = 0, kVisRz, kVisDq, kVisTruth };
 
 // One workgroup reduces the partials in block order and updates the scalar block
 // (the reference's "viscosity CG scalars", ViscosityKernels.h:490-535).
-__global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double* partials, int nb, int op, double tol2)
+__global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double* partials, int nb, int op, double tol2,
+    float dt)
 {
     __shared__ double lds[kVT / 64];
     if ((op == kVisRz || op == kVisDq) && stopped(S))
@@ -526,6 +533,7 @@ __global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double*
     switch (op) {
     case kVisInitial:
         s = VisScalars{};
+        s.dt = dt;
         s.energy = b;
         s.mass_sum = m;
         s.target = tol2 * b + 1e-12 * m;
@@ -681,6 +689,7 @@ struct ViscositySolver {
 
     ~ViscositySolver()
     {
+        drop_batch_graph();
         for (Level* L : levels)
             delete L;
         if (pinned) {
@@ -704,38 +713,90 @@ struct ViscositySolver {
         return *pinned;
     }
 
-    void finish(hipStream_t st, int nb, int op, double tol2)
+    void finish(hipStream_t st, int nb, int op, double tol2, float dt = 0.0f)
     {
-        hipLaunchKernelGGL(k_vis_finish, dim3(1), dim3(kVT), 0, st, S.p, partials.p, nb, op, tol2);
+        hipLaunchKernelGGL(k_vis_finish, dim3(1), dim3(kVT), 0, st, S.p, partials.p, nb, op, tol2, dt);
     }
 
-    // cycle (ViscositySolverGPU.cpp:134-167); the result is levels[l]->x
-    void cycle(hipStream_t st, int l, const float2* b, float dt)
+    // cycle (ViscositySolverGPU.cpp:134-167); the result is levels[l]->x.  Every level swaps its x / xn
+    // pair five times per cycle, so four cycles leave the pointers where they started.
+    void cycle(hipStream_t st, int l, const float2* b)
     {
         VLev L = view(l);
         if (l + 1 == (int)levels.size()) {
-            hipLaunchKernelGGL(k_vis_coarse, dim3(1), dim3(kCoarseSlots), 0, st, L, b, dt, S.p);
+            hipLaunchKernelGGL(k_vis_coarse, dim3(1), dim3(kCoarseSlots), 0, st, L, b, S.p);
             return;
         }
         const dim3 g = slot_grid(L.w, L.h);
-        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 1, dt, S.p);
+        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 1, S.p);
         swap_x(l);
         L = view(l);
-        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, dt, S.p);
+        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, S.p);
         swap_x(l);
         L = view(l);
-        hipLaunchKernelGGL(k_vis_residual, g, dim3(kVT), 0, st, L, b, dt, S.p);
+        hipLaunchKernelGGL(k_vis_residual, g, dim3(kVT), 0, st, L, b, S.p);
         VLev C = view(l + 1);
         hipLaunchKernelGGL(k_vis_restrict, slot_grid(C.w, C.h), dim3(kVT), 0, st, L, C, S.p);
-        cycle(st, l + 1, C.b, dt);
+        cycle(st, l + 1, C.b);
         C = view(l + 1);
         hipLaunchKernelGGL(k_vis_prolong, g, dim3(kVT), 0, st, L, C, S.p);
         swap_x(l);
         for (int k = 0; k < 2; ++k) {
             L = view(l);
-            hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, dt, S.p);
+            hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, S.p);
             swap_x(l);
         }
+    }
+
+    // One CG iteration (ViscositySolverGPU.cpp:238-262): V-cycle, r.z and beta, direction, A d and alpha,
+    // update of the iterate `delta` and the residual.
+    void iteration(hipStream_t st, float2* delta)
+    {
+        const VLev L0 = view(0);
+        const dim3 g0 = slot_grid(W, H);
+        const int nb = (int)slot_blocks(W, H);
+        cycle(st, 0, r.p);
+        const VLev Lc = view(0);
+        hipLaunchKernelGGL(k_vis_rz, g0, dim3(kVT), 0, st, Lc, (const float2*)r.p, (const float2*)Lc.x, S.p,
+            partials.p);
+        finish(st, nb, kVisRz, 0.0);
+        hipLaunchKernelGGL(k_vis_direction, g0, dim3(kVT), 0, st, Lc, (const float2*)Lc.x, d.p, S.p);
+        hipLaunchKernelGGL(k_vis_apply, g0, dim3(kVT), 0, st, L0, (const float2*)d.p, q.p, S.p, partials.p);
+        finish(st, nb, kVisDq, 0.0);
+        hipLaunchKernelGGL(k_vis_update, g0, dim3(kVT), 0, st, L0, delta, r.p, (const float2*)d.p,
+            (const float2*)q.p, S.p);
+    }
+
+    // A batch of four iterations captured once into a hipGraph: the kernels take every changing value
+    // (dt, alpha, beta, the stop state) from the scalar block, and the x / xn swaps of four cycles return
+    // to their starting pointers, so one graph replays every full batch of every solve on this solver.
+    hipGraph_t batch_graph = nullptr;
+    hipGraphExec_t batch_exec = nullptr;
+    hipStream_t batch_stream = nullptr;
+
+    void run_batch(hipStream_t st, float2* delta)
+    {
+        if (!batch_exec || batch_stream != st) {
+            drop_batch_graph();
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            vhip(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+            for (int i = 0; i < 4; ++i)
+                iteration(st, delta);
+            vhip(hipStreamEndCapture(st, &batch_graph), "hipStreamEndCapture");
+            vhip(hipGraphInstantiate(&batch_exec, batch_graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            batch_stream = st;
+        }
+        vhip(hipGraphLaunch(batch_exec, st), "hipGraphLaunch");
+    }
+
+    void drop_batch_graph()
+    {
+        if (batch_exec)
+            (void)hipGraphExecDestroy(batch_exec);
+        if (batch_graph)
+            (void)hipGraphDestroy(batch_graph);
+        batch_exec = nullptr;
+        batch_graph = nullptr;
     }
 
     // ViscositySolverGPU::Impl::solve (ViscositySolverGPU.cpp:169-294) on device FP64 u, v and an FP64
@@ -765,7 +826,7 @@ struct ViscositySolver {
         const int nb = (int)slot_blocks(W, H);
         hipLaunchKernelGGL(k_vis_pack, g0, dim3(kVT), 0, st, L0, (const double*)u, (const double*)v, v0.p);
         hipLaunchKernelGGL(k_vis_initial, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p, r0.p, fdt, partials.p);
-        finish(st, nb, kVisInitial, tol * tol);
+        finish(st, nb, kVisInitial, tol * tol, fdt);
         VisScalars s = read(st);
         const double denom = std::max(s.energy, 1e-30);
         *relative_out = std::sqrt(std::max(s.rnorm, 0.0) / denom);
@@ -781,20 +842,14 @@ struct ViscositySolver {
         int iterations = 0;
         while (maximum < 0 || iterations < maximum) {
             const int batch = maximum < 0 ? 4 : std::min(4, maximum - iterations);
-            for (int i = 0; i < batch; ++i) {
-                cycle(st, 0, r.p, fdt);
-                L0 = view(0);
-                hipLaunchKernelGGL(k_vis_rz, g0, dim3(kVT), 0, st, L0, (const float2*)r.p, (const float2*)L0.x, S.p,
-                    partials.p);
-                finish(st, nb, kVisRz, 0.0);
-                hipLaunchKernelGGL(k_vis_direction, g0, dim3(kVT), 0, st, L0, (const float2*)L0.x, d.p, S.p);
-                hipLaunchKernelGGL(k_vis_apply, g0, dim3(kVT), 0, st, L0, (const float2*)d.p, q.p, fdt, S.p,
-                    partials.p);
-                finish(st, nb, kVisDq, 0.0);
-                hipLaunchKernelGGL(k_vis_update, g0, dim3(kVT), 0, st, L0, delta, r.p, (const float2*)d.p,
-                    (const float2*)q.p, S.p);
+            if (batch == 4) {
+                run_batch(st, delta);
+            } else {
+                for (int i = 0; i < batch; ++i)
+                    iteration(st, delta);
             }
             iterations += batch;
+            L0 = view(0);
             hipLaunchKernelGGL(k_vis_truth, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p, (const float2*)delta,
                 truth.p, fdt, partials.p);
             finish(st, nb, kVisTruth, 0.0);
