@@ -68,26 +68,34 @@ __device__ __forceinline__ bool valid_v(int w, int h, int x, int y) { return y >
 
 __device__ __forceinline__ bool stopped(const VisScalars* S) { return S->bad != 0 || S->rnorm <= S->target; }
 
+// Index of slot (x, y) in an array: the global slot field clamped like the reference's accessors, or an
+// LDS tile that holds the clamped values (see k_vis_smooth2).  The stencils below are written once over
+// it, so the fused and the per-sweep kernels evaluate the same expressions.
+struct GlobalSlots {
+    int w, h;
+    __device__ __forceinline__ int operator()(int x, int y) const { return sidx(w, h, x, y); }
+};
+
 // shear and stressProduct (ViscosityKernels.h:26-44)
-__device__ __forceinline__ float shear(const float4* c, const float2* s, int w, int h, int x, int y)
+template <class I> __device__ __forceinline__ float shear_at(const float4* c, const float2* s, I at, int x, int y)
 {
-    const float2 p = s[sidx(w, h, x, y)];
-    return c[sidx(w, h, x, y)].y * ((p.x - s[sidx(w, h, x, y - 1)].x) + (p.y - s[sidx(w, h, x - 1, y)].y));
+    const float2 p = s[at(x, y)];
+    return c[at(x, y)].y * ((p.x - s[at(x, y - 1)].x) + (p.y - s[at(x - 1, y)].y));
 }
 
 // product (ViscosityKernels.h:53-57): M s + dt K s on valid faces.
-__device__ float2 product(const float4* c, const float2* s, int w, int h, int x, int y, float dt)
+template <class I>
+__device__ __forceinline__ float2 product_at(const float4* c, const float2* s, I at, int w, int h, int x, int y,
+    float dt)
 {
-    const int p = sidx(w, h, x, y);
+    const int p = at(x, y);
     const float2 cv = s[p];
     const float4 a = c[p];
-    const float t = shear(c, s, w, h, x, y);
-    const float nu = 2.0f * (c[sidx(w, h, x - 1, y)].x * (cv.x - s[sidx(w, h, x - 1, y)].x) +
-                             a.x * (cv.x - s[sidx(w, h, x + 1, y)].x));
-    const float nv = 2.0f * (c[sidx(w, h, x, y - 1)].x * (cv.y - s[sidx(w, h, x, y - 1)].y) +
-                             a.x * (cv.y - s[sidx(w, h, x, y + 1)].y));
-    const float ku = nu + (t - shear(c, s, w, h, x, y + 1));
-    const float kv = nv + (t - shear(c, s, w, h, x + 1, y));
+    const float t = shear_at(c, s, at, x, y);
+    const float nu = 2.0f * (c[at(x - 1, y)].x * (cv.x - s[at(x - 1, y)].x) + a.x * (cv.x - s[at(x + 1, y)].x));
+    const float nv = 2.0f * (c[at(x, y - 1)].x * (cv.y - s[at(x, y - 1)].y) + a.x * (cv.y - s[at(x, y + 1)].y));
+    const float ku = nu + (t - shear_at(c, s, at, x, y + 1));
+    const float kv = nv + (t - shear_at(c, s, at, x + 1, y));
     float2 out;
     out.x = valid_u(w, h, x, y) ? a.z * cv.x + dt * ku : 0.0f;
     out.y = valid_v(w, h, x, y) ? a.w * cv.y + dt * kv : 0.0f;
@@ -95,13 +103,23 @@ __device__ float2 product(const float4* c, const float2* s, int w, int h, int x,
 }
 
 // diagonal (ViscosityKernels.h:17-24)
+template <class I> __device__ __forceinline__ float2 diagonal_at(const float4* c, I at, int x, int y, float dt)
+{
+    const float4 a = c[at(x, y)];
+    float2 d;
+    d.x = a.z + dt * (2.0f * (a.x + c[at(x - 1, y)].x) + a.y + c[at(x, y + 1)].y);
+    d.y = a.w + dt * (2.0f * (a.x + c[at(x, y - 1)].x) + a.y + c[at(x + 1, y)].y);
+    return d;
+}
+
+__device__ float2 product(const float4* c, const float2* s, int w, int h, int x, int y, float dt)
+{
+    return product_at(c, s, GlobalSlots{w, h}, w, h, x, y, dt);
+}
+
 __device__ __forceinline__ float2 diagonal(const float4* c, int w, int h, int x, int y, float dt)
 {
-    const float4 a = c[sidx(w, h, x, y)];
-    float2 d;
-    d.x = a.z + dt * (2.0f * (a.x + c[sidx(w, h, x - 1, y)].x) + a.y + c[sidx(w, h, x, y + 1)].y);
-    d.y = a.w + dt * (2.0f * (a.x + c[sidx(w, h, x, y - 1)].x) + a.y + c[sidx(w, h, x + 1, y)].y);
-    return d;
+    return diagonal_at(c, GlobalSlots{w, h}, x, y, dt);
 }
 
 #define VIS_SLOT(L)                                            \
@@ -233,30 +251,6 @@ __global__ __launch_bounds__(kVT) void k_vis_initial(VLev L, const float2* v0, f
 
 // ---- V-cycle ----
 
-// "viscosity symmetric smoother" (ViscosityKernels.h:214-236), damped Jacobi with weight 1/2.
-__global__ __launch_bounds__(kVT) void k_vis_smooth(VLev L, const float2* b, int zero, const VisScalars* S)
-{
-    if (stopped(S))
-        return;
-    const float dt = S->dt;
-    VIS_SLOT(L)
-    if (!inside)
-        return;
-    const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
-    const float2 rhs = b[p];
-    float2 v;
-    if (zero) {
-        v.x = 0.5f * rhs.x / d.x;
-        v.y = 0.5f * rhs.y / d.y;
-    } else {
-        const float2 q = product(L.coef, L.x, L.w, L.h, x, y, dt);
-        const float2 s = L.x[p];
-        v.x = s.x + 0.5f * (rhs.x - q.x) / d.x;
-        v.y = s.y + 0.5f * (rhs.y - q.y) / d.y;
-    }
-    L.xn[p] = make_float2(valid_u(L.w, L.h, x, y) ? v.x : 0.0f, valid_v(L.w, L.h, x, y) ? v.y : 0.0f);
-}
-
 // "viscosity true residual" on a level (ViscosityKernels.h:187-195)
 __global__ __launch_bounds__(kVT) void k_vis_residual(VLev L, const float2* b, const VisScalars* S)
 {
@@ -330,29 +324,102 @@ __global__ __launch_bounds__(kVT) void k_vis_restrict(VLev F, VLev C, const VisS
     C.b[p] = make_float2(val[0], val[1]);
 }
 
-// "viscosity prolongation" (ViscosityKernels.h:359-391): F.xn = F.x + P C.x on valid faces
-__global__ __launch_bounds__(kVT) void k_vis_prolong(VLev F, VLev C, const VisScalars* S)
+// Two sweeps of the "viscosity symmetric smoother" (ViscosityKernels.h:214-236, damped Jacobi with weight
+// 1/2) fused on an LDS tile: pre-smoothing from zero (result in L.x), or (POST) the "viscosity
+// prolongation" F.x + P C.x on valid faces (ViscosityKernels.h:359-391) followed by two sweeps (result in
+// L.xn).  The tile
+// holds a two-slot halo of coefficients, rhs and the starting solution, each slot loaded from its clamped
+// position; every stencil read maps through the same clamp to the tile, so each sweep reads exactly the
+// values a per-sweep kernel would read from the slot field (results identical to one launch per sweep).
+constexpr int kSX = 32, kSY = 16, kSH = 2;                   // interior slots per workgroup, halo
+constexpr int kSLX = kSX + 2 * kSH, kSLY = kSY + 2 * kSH;     // 36 x 20
+constexpr int kSN = kSLX * kSLY;
+
+template <bool POST>
+__global__ __launch_bounds__(kVT) void k_vis_smooth2(VLev L, VLev C, const float2* b, const VisScalars* S)
 {
+    __shared__ float4 sc[kSN];
+    __shared__ float2 sb[kSN];
+    __shared__ float2 s0[kSN];   // starting solution of the first sweep (POST: prolonged)
+    __shared__ float2 s1[kSN];   // after the first sweep
     if (stopped(S))
         return;
-    VIS_SLOT(F)
-    if (!inside)
-        return;
-    const float2 s = F.x[p];
-    float val[2] = {s.x, s.y};
-    for (int comp = 0; comp < 2; ++comp) {
-        int bx, by;
-        float tx, ty;
-        transfer_location(x, y, comp, bx, by, tx, ty);
-        for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx) {
-                int px, py;
-                coarse_point(bx + dx, by + dy, comp, C.w, C.h, px, py);
-                const float2 c = C.x[py * (C.w + 1) + px];
-                val[comp] += (dx == 1 ? tx : 1.0f - tx) * (dy == 1 ? ty : 1.0f - ty) * (comp == 0 ? c.x : c.y);
-            }
+    const float dt = S->dt;
+    const int w = L.w, h = L.h;
+    const int x0 = blockIdx.x * kSX - kSH, y0 = blockIdx.y * kSY - kSH;
+    const auto at = [=](int x, int y) { return (min(max(y, 0), h) - y0) * kSLX + (min(max(x, 0), w) - x0); };
+    for (int e = threadIdx.x; e < kSN; e += kVT) {
+        const int gx = x0 + e % kSLX, gy = y0 + e / kSLX;
+        const int g = sidx(w, h, gx, gy);
+        sc[e] = L.coef[g];
+        sb[e] = b[g];
+        if (POST)
+            s0[e] = L.x[g];
     }
-    F.xn[p] = make_float2(valid_u(F.w, F.h, x, y) ? val[0] : 0.0f, valid_v(F.w, F.h, x, y) ? val[1] : 0.0f);
+    __syncthreads();
+    if (POST) {
+        // prolongation on every in-field slot of the tile (interior and both halo rings)
+        for (int e = threadIdx.x; e < kSN; e += kVT) {
+            const int x = x0 + e % kSLX, y = y0 + e / kSLX;
+            if (x < 0 || y < 0 || x > w || y > h)
+                continue;
+            const float2 sv = s0[e];
+            float val[2] = {sv.x, sv.y};
+            for (int comp = 0; comp < 2; ++comp) {
+                int bx, by;
+                float tx, ty;
+                transfer_location(x, y, comp, bx, by, tx, ty);
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        int px, py;
+                        coarse_point(bx + dx, by + dy, comp, C.w, C.h, px, py);
+                        const float2 c = C.x[py * (C.w + 1) + px];
+                        val[comp] +=
+                            (dx == 1 ? tx : 1.0f - tx) * (dy == 1 ? ty : 1.0f - ty) * (comp == 0 ? c.x : c.y);
+                    }
+            }
+            s0[e] = make_float2(valid_u(w, h, x, y) ? val[0] : 0.0f, valid_v(w, h, x, y) ? val[1] : 0.0f);
+        }
+        __syncthreads();
+    }
+    // first sweep on the interior and the inner halo ring
+    constexpr int RX = kSX + 2, RY = kSY + 2;
+    for (int e = threadIdx.x; e < RX * RY; e += kVT) {
+        const int x = x0 + 1 + e % RX, y = y0 + 1 + e / RX;
+        if (x < 0 || y < 0 || x > w || y > h)
+            continue;
+        const int i = at(x, y);
+        const float2 d = diagonal_at(sc, at, x, y, dt);
+        const float2 rhs = sb[i];
+        float2 v;
+        if (!POST) {
+            v.x = 0.5f * rhs.x / d.x;
+            v.y = 0.5f * rhs.y / d.y;
+        } else {
+            const float2 q = product_at(sc, s0, at, w, h, x, y, dt);
+            const float2 sv = s0[i];
+            v.x = sv.x + 0.5f * (rhs.x - q.x) / d.x;
+            v.y = sv.y + 0.5f * (rhs.y - q.y) / d.y;
+        }
+        s1[i] = make_float2(valid_u(w, h, x, y) ? v.x : 0.0f, valid_v(w, h, x, y) ? v.y : 0.0f);
+    }
+    __syncthreads();
+    // second sweep on the interior, written out
+    float2* out = POST ? L.xn : L.x;
+    for (int e = threadIdx.x; e < kSX * kSY; e += kVT) {
+        const int x = x0 + kSH + e % kSX, y = y0 + kSH + e / kSX;
+        if (x > w || y > h)
+            continue;
+        const int i = at(x, y);
+        const float2 d = diagonal_at(sc, at, x, y, dt);
+        const float2 rhs = sb[i];
+        const float2 q = product_at(sc, s1, at, w, h, x, y, dt);
+        const float2 sv = s1[i];
+        float2 v;
+        v.x = sv.x + 0.5f * (rhs.x - q.x) / d.x;
+        v.y = sv.y + 0.5f * (rhs.y - q.y) / d.y;
+        out[y * (w + 1) + x] = make_float2(valid_u(w, h, x, y) ? v.x : 0.0f, valid_v(w, h, x, y) ? v.y : 0.0f);
+    }
 }
 
 // "viscosity shared coarse solve" (ViscosityKernels.h:242-282): 16 Jacobi sweeps from zero in LDS.
@@ -719,7 +786,7 @@ struct ViscositySolver {
     }
 
     // cycle (ViscositySolverGPU.cpp:134-167); the result is levels[l]->x.  Every level swaps its x / xn
-    // pair five times per cycle, so four cycles leave the pointers where they started.
+    // pair once per cycle, so four cycles leave the pointers where they started.
     void cycle(hipStream_t st, int l, const float2* b)
     {
         VLev L = view(l);
@@ -728,24 +795,18 @@ struct ViscositySolver {
             return;
         }
         const dim3 g = slot_grid(L.w, L.h);
-        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 1, S.p);
-        swap_x(l);
-        L = view(l);
-        hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, S.p);
-        swap_x(l);
-        L = view(l);
+        const dim3 g2((L.w + 1 + kSX - 1) / kSX, (L.h + 1 + kSY - 1) / kSY);
+        // pre-smoothing: two sweeps from zero in one launch (the result lands in L.x, as after two swaps)
+        hipLaunchKernelGGL(k_vis_smooth2<false>, g2, dim3(kVT), 0, st, L, L, b, S.p);
         hipLaunchKernelGGL(k_vis_residual, g, dim3(kVT), 0, st, L, b, S.p);
         VLev C = view(l + 1);
         hipLaunchKernelGGL(k_vis_restrict, slot_grid(C.w, C.h), dim3(kVT), 0, st, L, C, S.p);
         cycle(st, l + 1, C.b);
         C = view(l + 1);
-        hipLaunchKernelGGL(k_vis_prolong, g, dim3(kVT), 0, st, L, C, S.p);
+        // prolongation + two sweeps in one launch into L.xn: one swap, as after the three of the
+        // per-sweep kernels
+        hipLaunchKernelGGL(k_vis_smooth2<true>, g2, dim3(kVT), 0, st, L, C, b, S.p);
         swap_x(l);
-        for (int k = 0; k < 2; ++k) {
-            L = view(l);
-            hipLaunchKernelGGL(k_vis_smooth, g, dim3(kVT), 0, st, L, b, 0, S.p);
-            swap_x(l);
-        }
     }
 
     // One CG iteration (ViscositySolverGPU.cpp:238-262): V-cycle, r.z and beta, direction, A d and alpha,
