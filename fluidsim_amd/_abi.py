@@ -159,6 +159,7 @@ SIGNATURES = {
     "fs_set_multigrid_precision": [_vp, C.c_int],
     "fs_get_multigrid_precision": [_vp, C.POINTER(C.c_int)],
     "fs_set_flexible_cg": [_vp, C.c_int],
+    "fs_batch_apply_viscosity": [C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int)],
     "fs_solid_cells": [_vp, C.POINTER(C.c_longlong)],
     "fs_upload": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "fs_download": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],

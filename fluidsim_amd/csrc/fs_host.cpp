@@ -233,6 +233,7 @@ struct fs_context {
     double* pinned_value = nullptr; // host-pinned landing slot of single device reductions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf<BatchInstance> batch_desc;       // descriptors of the last batched solve
+    DevBuf<unsigned char> vis_batch_desc;   // descriptors of the last batched viscosity solve
     PcgScalars* batch_pinned = nullptr;     // host-pinned landing slots of a batch's scalar blocks
     size_t batch_pinned_n = 0;
 };
@@ -1890,12 +1891,10 @@ static void raise_viscosity(int status, double relative)
         fail(FS_E_RUNTIME, "Implicit viscosity did not converge; relative residual " + std::to_string(relative));
 }
 
-// FluidGridGPU::applyViscosity (FluidGridGPU.cpp:987-1046): blend, then solve to 1e-4.  Returns the
-// dispatched iteration count.
-static int apply_viscosity(fs_grid& g, double dt)
+// The volume-weighted (rho, mu) material of FluidGridGPU::applyViscosity (FluidGridGPU.cpp:995-1031),
+// blended on the grid's stream into the solver's material buffer.
+static double* blend_viscosity_material(fs_grid& g)
 {
-    if (dt <= 0.0 || !g.has_viscosity())
-        return 0;
     if (!g.visc) {
         g.visc = viscosity_create(g.W, g.H);
         g.bytes += viscosity_bytes(g.visc);
@@ -1912,6 +1911,16 @@ static int apply_viscosity(fs_grid& g, double dt)
     double* material = viscosity_material_buffer(g.visc);
     launch_viscosity_blend(g.st(), (int)g.N(), g.h, (int)g.rho_l.size(), (int)g.rho_g.size(), g.lmass.p,
         g.gmass.p, mix, material);
+    return material;
+}
+
+// FluidGridGPU::applyViscosity (FluidGridGPU.cpp:987-1046): blend, then solve to 1e-4.  Returns the
+// dispatched iteration count.
+static int apply_viscosity(fs_grid& g, double dt)
+{
+    if (dt <= 0.0 || !g.has_viscosity())
+        return 0;
+    double* material = blend_viscosity_material(g);
     int iterations = 0;
     double relative = 0.0;
     const int status = viscosity_solve(g.visc, g.st(), g.u.p, g.v.p, material, dt, g.h, 1e-4, kViscosityBudget,
@@ -1930,6 +1939,82 @@ fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations)
         HIPC(hipStreamSynchronize(grid->st()));
         if (iterations)
             *iterations = n;
+    });
+}
+
+fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, int* iterations)
+{
+    return guard([&] {
+        need(count >= 0 && (count == 0 || grids), FS_E_INVALID_ARGUMENT, "null pointer");
+        if (count == 0)
+            return;
+        fs_context* ctx = grids[0] ? grids[0]->ctx : nullptr;
+        for (int i = 0; i < count; ++i) {
+            need(grids[i] != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+            need(grids[i]->ctx == ctx, FS_E_INVALID_ARGUMENT, "batched grids must share one context");
+            need(grids[i]->uploaded, FS_E_LOGIC, "Upload the grid state before applying viscosity");
+            for (int j = 0; j < i; ++j)
+                need(grids[j] != grids[i], FS_E_INVALID_ARGUMENT, "a grid appears twice in the batch");
+            if (iterations)
+                iterations[i] = 0;
+        }
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        const size_t bytes = viscosity_batch_desc_bytes();
+        std::vector<unsigned char> descs;
+        std::vector<int> batched, alone;
+        for (int i = 0; i < count; ++i) {
+            fs_grid& g = *grids[i];
+            if (dt <= 0.0 || !g.has_viscosity())
+                continue;
+            double* material = blend_viscosity_material(g);
+            descs.resize((batched.size() + 1) * bytes);
+            if (viscosity_batch_prepare(g.visc, st, g.u.p, g.v.p, material, dt, g.h, 1e-4, kViscosityBudget,
+                    descs.data() + batched.size() * bytes))
+                batched.push_back(i);
+            else
+                alone.push_back(i);
+        }
+        if (!batched.empty()) {
+            const size_t need_bytes = batched.size() * bytes;
+            if (ctx->vis_batch_desc.n < need_bytes)
+                ctx->vis_batch_desc.alloc(need_bytes);
+            HIPC(hipMemcpyAsync(ctx->vis_batch_desc.p, descs.data(), need_bytes, hipMemcpyHostToDevice, st));
+            launch_batch_viscosity(st, ctx->vis_batch_desc.p, (int)batched.size());
+            for (int i : batched)
+                viscosity_batch_fetch(grids[i]->visc, st);
+        }
+        HIPC(hipStreamSynchronize(st));
+        std::string first_error;
+        auto attempt = [&](auto&& body) {
+            try {
+                body();
+            } catch (const std::exception& e) {
+                if (first_error.empty())
+                    first_error = e.what();
+            }
+        };
+        for (int i : batched) {
+            int it = 0;
+            double relative = 0.0;
+            const int status = viscosity_batch_result(grids[i]->visc, kViscosityBudget, &it, &relative);
+            if (status == 4) {
+                alone.push_back(i);   // more iterations than the batch runs: solved again on its own
+                continue;
+            }
+            if (iterations)
+                iterations[i] = it;
+            attempt([&] { raise_viscosity(status, relative); });
+        }
+        for (int i : alone)
+            attempt([&] {
+                const int it = apply_viscosity(*grids[i], dt);
+                if (iterations)
+                    iterations[i] = it;
+            });
+        HIPC(hipStreamSynchronize(st));
+        if (!first_error.empty())
+            fail(FS_E_RUNTIME, first_error);
     });
 }
 

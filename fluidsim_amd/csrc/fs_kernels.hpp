@@ -254,6 +254,18 @@ double* viscosity_material_buffer(ViscositySolver* s);   // 2 * W * H doubles
 // 2 breakdown, 3 not converged within `maximum`; u, v change only on 0.
 int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material, double dt,
     double spacing, double tol, int maximum, int* iterations, double* relative_residual);
+// Batched viscosity solves: one 1024-thread workgroup per instance runs its whole CG loop.
+// viscosity_batch_prepare launches an instance's set-up on `st` and writes its descriptor into `desc`
+// (viscosity_batch_desc_bytes() of host memory; false: too many levels, solve it on its own); after
+// launch_batch_viscosity over the descriptors (copied to device memory), viscosity_batch_fetch enqueues
+// the read-back and, after a synchronisation, viscosity_batch_result returns viscosity_solve's codes or
+// 4 when the instance needs more iterations than the batch runs (u, v untouched: solve it on its own).
+size_t viscosity_batch_desc_bytes();
+bool viscosity_batch_prepare(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material,
+    double dt, double spacing, double tol, int maximum, void* desc);
+void launch_batch_viscosity(hipStream_t st, const void* descs, int count);
+void viscosity_batch_fetch(ViscositySolver* s, hipStream_t st);
+int viscosity_batch_result(const ViscositySolver* s, int maximum, int* iterations, double* relative_residual);
 
 // ---- surface tension (FluidGridCPU::applySurfaceTension, FluidSurfaceTensionCPU) ----
 // Heights (x, y) and orientations per cell, raw and completed curvature (value, valid) per cell.

@@ -19,6 +19,8 @@
 
 #include <math.h>
 
+#include <cmath>
+
 #include <algorithm>
 #include <mutex>
 #include <stdexcept>
@@ -45,6 +47,8 @@ struct VisScalars {
     int bad;          // breakdown or non-finite value
     float dt;         // time step of this solve: the iteration kernels read it here, so one captured
                       // batch of iterations serves every dt
+    int status;       // batched solve: result code (see viscosity_solve)
+    int iterations;   // batched solve: iterations run
     int pad;
 };
 
@@ -703,6 +707,375 @@ template <class T> struct VBuf {
     }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Batched solve: one 1024-thread workgroup runs one instance's whole CG loop (the iteration body of
+// ViscositySolver::solve, every kernel a workgroup-strided loop followed by a barrier, the scalars
+// held uniformly in registers), so a batch of uneven instances is one launch.
+// ---------------------------------------------------------------------------------------------
+constexpr int kBVT = 1024;
+constexpr int kMaxVisLevels = 16;
+constexpr int kBatchVisBudget = 500;   // iterations inside the batch; an instance that needs more is
+                                       // solved again on its own by the caller
+
+struct BatchVisDesc {
+    int levels;
+    int W, H;
+    int maximum;          // the caller's budget
+    VLev lev[kMaxVisLevels];
+    const float2* v0;
+    float2* r;
+    float2* d;
+    float2* q;
+    float2* t;            // the iterate, starting at v0
+    double* u;
+    double* v;
+    VisScalars* S;        // after the initial reduction; receives status and iterations
+};
+
+// FP64 sum over the workgroup in a fixed order, returned to every thread.
+__device__ double wg_sum_all(double v, double* lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBVT / 64; ++i)
+        r += lds[i];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void slot_xy(int e, int w, int& x, int& y)
+{
+    y = e / (w + 1);
+    x = e - y * (w + 1);
+}
+
+// Two sweeps from zero (pre) on a level: xn = sweep(0), x = sweep(xn).
+__device__ void bv_pre(const VLev& L, const float2* b, float dt)
+{
+    const int n = (L.w + 1) * (L.h + 1);
+    for (int e = threadIdx.x; e < n; e += kBVT) {
+        int x, y;
+        slot_xy(e, L.w, x, y);
+        const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
+        const float2 rhs = b[e];
+        L.xn[e] = make_float2(valid_u(L.w, L.h, x, y) ? 0.5f * rhs.x / d.x : 0.0f,
+            valid_v(L.w, L.h, x, y) ? 0.5f * rhs.y / d.y : 0.0f);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += kBVT) {
+        int x, y;
+        slot_xy(e, L.w, x, y);
+        const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
+        const float2 rhs = b[e];
+        const float2 q = product(L.coef, L.xn, L.w, L.h, x, y, dt);
+        const float2 sv = L.xn[e];
+        L.x[e] = make_float2(valid_u(L.w, L.h, x, y) ? sv.x + 0.5f * (rhs.x - q.x) / d.x : 0.0f,
+            valid_v(L.w, L.h, x, y) ? sv.y + 0.5f * (rhs.y - q.y) / d.y : 0.0f);
+    }
+    __syncthreads();
+}
+
+// One sweep src -> dst.
+__device__ void bv_sweep(const VLev& L, const float2* b, const float2* src, float2* dst, float dt)
+{
+    const int n = (L.w + 1) * (L.h + 1);
+    for (int e = threadIdx.x; e < n; e += kBVT) {
+        int x, y;
+        slot_xy(e, L.w, x, y);
+        const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
+        const float2 rhs = b[e];
+        const float2 q = product(L.coef, src, L.w, L.h, x, y, dt);
+        const float2 sv = src[e];
+        dst[e] = make_float2(valid_u(L.w, L.h, x, y) ? sv.x + 0.5f * (rhs.x - q.x) / d.x : 0.0f,
+            valid_v(L.w, L.h, x, y) ? sv.y + 0.5f * (rhs.y - q.y) / d.y : 0.0f);
+    }
+    __syncthreads();
+}
+
+__device__ void bv_residual_restrict(const VLev& F, const float2* b, const VLev& C, float dt)
+{
+    const int n = (F.w + 1) * (F.h + 1);
+    for (int e = threadIdx.x; e < n; e += kBVT) {
+        int x, y;
+        slot_xy(e, F.w, x, y);
+        const float2 q = product(F.coef, F.x, F.w, F.h, x, y, dt);
+        const float2 rhs = b[e];
+        F.r[e] = make_float2(valid_u(F.w, F.h, x, y) ? rhs.x - q.x : 0.0f, valid_v(F.w, F.h, x, y) ? rhs.y - q.y : 0.0f);
+    }
+    __syncthreads();
+    const int nc = (C.w + 1) * (C.h + 1);
+    for (int e = threadIdx.x; e < nc; e += kBVT) {
+        int x, y;
+        slot_xy(e, C.w, x, y);
+        float val[2] = {0.0f, 0.0f};
+        for (int comp = 0; comp < 2; ++comp) {
+            if ((comp == 0 && (x == 0 || x == C.w || y == C.h)) || (comp == 1 && (y == 0 || y == C.h || x == C.w)))
+                continue;
+            const int ex = comp == 1 ? 2 : 1, ey = comp == 1 ? 1 : 2;
+            for (int dy = -1; dy <= ey; ++dy)
+                for (int dx = -1; dx <= ex; ++dx) {
+                    const int qx = 2 * x + dx, qy = 2 * y + dy;
+                    if (qx < 0 || qy < 0 || qx > F.w || qy > F.h)
+                        continue;
+                    const float2 r = F.r[qy * (F.w + 1) + qx];
+                    val[comp] += transfer_weight(qx, qy, x, y, comp, C.w, C.h) * (comp == 0 ? r.x : r.y);
+                }
+        }
+        C.b[e] = make_float2(val[0], val[1]);
+    }
+    __syncthreads();
+}
+
+// F.x += P C.x on valid faces (in place: each slot reads only its own fine value).
+__device__ void bv_prolong(const VLev& F, const VLev& C)
+{
+    const int n = (F.w + 1) * (F.h + 1);
+    for (int e = threadIdx.x; e < n; e += kBVT) {
+        int x, y;
+        slot_xy(e, F.w, x, y);
+        const float2 sv = F.x[e];
+        float val[2] = {sv.x, sv.y};
+        for (int comp = 0; comp < 2; ++comp) {
+            int bx, by;
+            float tx, ty;
+            transfer_location(x, y, comp, bx, by, tx, ty);
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    int px, py;
+                    coarse_point(bx + dx, by + dy, comp, C.w, C.h, px, py);
+                    const float2 c = C.x[py * (C.w + 1) + px];
+                    val[comp] += (dx == 1 ? tx : 1.0f - tx) * (dy == 1 ? ty : 1.0f - ty) * (comp == 0 ? c.x : c.y);
+                }
+        }
+        F.x[e] = make_float2(valid_u(F.w, F.h, x, y) ? val[0] : 0.0f, valid_v(F.w, F.h, x, y) ? val[1] : 0.0f);
+    }
+    __syncthreads();
+}
+
+// 16 Jacobi sweeps from zero on the coarsest level (k_vis_coarse), slots in LDS.
+__device__ void bv_coarse(const VLev& L, const float2* b, float dt, float2* vel, float4* cof)
+{
+    const int lane = threadIdx.x;
+    const int stride = L.w + 1;
+    const int n = stride * (L.h + 1);
+    const bool on = lane < kCoarseSlots;
+    const int x = lane % stride, y = lane / stride;
+    const int src = lane < n ? lane : n - 1;
+    float2 rhs = make_float2(0.0f, 0.0f);
+    if (on) {
+        vel[lane] = make_float2(0.0f, 0.0f);
+        cof[lane] = L.coef[src];
+        rhs = b[src];
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (on && lane < n) {
+            const float2 d = diagonal(cof, L.w, L.h, x, y, dt);
+            const float2 q = product(cof, vel, L.w, L.h, x, y, dt);
+            const float2 sv = vel[lane];
+            v.x = valid_u(L.w, L.h, x, y) ? sv.x + 0.5f * (rhs.x - q.x) / d.x : 0.0f;
+            v.y = valid_v(L.w, L.h, x, y) ? sv.y + 0.5f * (rhs.y - q.y) / d.y : 0.0f;
+        }
+        __syncthreads();
+        if (on)
+            vel[lane] = v;
+        __syncthreads();
+    }
+    if (on && lane < n)
+        L.x[lane] = vel[lane];
+    __syncthreads();
+}
+
+__device__ void bv_cycle(const BatchVisDesc& D, float dt, float2* vel, float4* cof)
+{
+    const int nl = D.levels;
+    for (int l = 0; l + 1 < nl; ++l) {
+        const float2* b = l == 0 ? (const float2*)D.r : D.lev[l].b;
+        bv_pre(D.lev[l], b, dt);
+        bv_residual_restrict(D.lev[l], b, D.lev[l + 1], dt);
+    }
+    bv_coarse(D.lev[nl - 1], nl == 1 ? (const float2*)D.r : D.lev[nl - 1].b, dt, vel, cof);
+    for (int l = nl - 2; l >= 0; --l) {
+        const float2* b = l == 0 ? (const float2*)D.r : D.lev[l].b;
+        bv_prolong(D.lev[l], D.lev[l + 1]);
+        bv_sweep(D.lev[l], b, D.lev[l].x, D.lev[l].xn, dt);
+        bv_sweep(D.lev[l], b, D.lev[l].xn, D.lev[l].x, dt);
+    }
+}
+
+__global__ __launch_bounds__(kBVT) void k_batch_vis(const BatchVisDesc* descs)
+{
+    __shared__ double lds[kBVT / 64];
+    __shared__ float2 vel[kCoarseSlots];
+    __shared__ float4 cof[kCoarseSlots];
+    const BatchVisDesc& D = descs[blockIdx.x];
+    VisScalars* S = D.S;
+    const VLev L0 = D.lev[0];
+    const int W = D.W, H = D.H;
+    const int n0 = (W + 1) * (H + 1);
+    const float dt = S->dt;
+    const double target = S->target;
+    double rnorm = S->rnorm;
+    int bad = S->bad;
+    int status;   // 0 converged (or nothing to do), 1 non-finite input, 2 breakdown, 3 over budget
+    int iterations = 0;
+    double truth = S->truth;
+    if (bad) {
+        status = 1;
+    } else if (rnorm <= target) {
+        status = 0;
+    } else {
+        double rz = 0.0, alpha = 0.0, beta = 0.0;
+        int steps = 0;
+        const int budget = D.maximum < 0 ? kBatchVisBudget : min(D.maximum, kBatchVisBudget);
+        status = 3;
+        while (iterations < budget) {
+            const int batch = min(4, budget - iterations);
+            for (int i = 0; i < batch; ++i) {
+                if (bad || rnorm <= target)
+                    continue;   // the iteration kernels exit at once once the solve has stopped
+                bv_cycle(D, dt, vel, cof);
+                // r . z, r M^-1 r
+                double a = 0.0, b2 = 0.0, f = 0.0;
+                for (int e = threadIdx.x; e < n0; e += kBVT) {
+                    const float4 c = L0.coef[e];
+                    const float2 rr = D.r[e], zz = L0.x[e];
+                    double ea = (double)rr.x * zz.x + (double)rr.y * zz.y;
+                    double eb = (double)rr.x / c.z * rr.x + (double)rr.y / c.w * rr.y;
+                    if (!isfinite(ea) || !isfinite(eb)) {
+                        ea = eb = 0.0;
+                        f = 1.0;
+                    }
+                    a += ea;
+                    b2 += eb;
+                }
+                a = wg_sum_all(a, lds);
+                b2 = wg_sum_all(b2, lds);
+                f = wg_sum_all(f, lds);
+                bad |= f != 0.0 ? 1 : 0;
+                rnorm = b2;
+                if (rnorm > target && !bad) {
+                    if (a <= 0.0 || !isfinite(a)) {
+                        bad = 1;
+                    } else {
+                        beta = steps > 0 ? a / rz : 0.0;
+                        rz = a;
+                    }
+                }
+                if (bad || rnorm <= target)
+                    continue;
+                // direction, q = A d and d . q
+                const float fb = (float)beta;
+                for (int e = threadIdx.x; e < n0; e += kBVT) {
+                    const float2 zz = L0.x[e];
+                    if (steps > 0) {
+                        const float2 dd = D.d[e];
+                        D.d[e] = make_float2(zz.x + fb * dd.x, zz.y + fb * dd.y);
+                    } else {
+                        D.d[e] = zz;
+                    }
+                }
+                __syncthreads();
+                double dq = 0.0;
+                f = 0.0;
+                for (int e = threadIdx.x; e < n0; e += kBVT) {
+                    int x, y;
+                    slot_xy(e, W, x, y);
+                    const float2 qv = product(L0.coef, D.d, W, H, x, y, dt);
+                    D.q[e] = qv;
+                    const float2 dd = D.d[e];
+                    double ea = (double)dd.x * qv.x + (double)dd.y * qv.y;
+                    if (!isfinite(ea)) {
+                        ea = 0.0;
+                        f = 1.0;
+                    }
+                    dq += ea;
+                }
+                dq = wg_sum_all(dq, lds);
+                f = wg_sum_all(f, lds);
+                if (f != 0.0 || dq <= 0.0 || !isfinite(dq)) {
+                    bad = 1;
+                    continue;
+                }
+                alpha = rz / dq;
+                steps += 1;
+                const float fa = (float)alpha;
+                for (int e = threadIdx.x; e < n0; e += kBVT) {
+                    const float2 dd = D.d[e], qq = D.q[e], xx = D.t[e], rr = D.r[e];
+                    D.t[e] = make_float2(xx.x + fa * dd.x, xx.y + fa * dd.y);
+                    D.r[e] = make_float2(rr.x - fa * qq.x, rr.y - fa * qq.y);
+                }
+                __syncthreads();
+            }
+            iterations += batch;
+            // recomputed residual q = M v0 - A x and q M^-1 q
+            double tr = 0.0, f = 0.0;
+            for (int e = threadIdx.x; e < n0; e += kBVT) {
+                int x, y;
+                slot_xy(e, W, x, y);
+                const float4 c = L0.coef[e];
+                const float2 qv = product(L0.coef, D.t, W, H, x, y, dt);
+                const float2 s0 = D.v0[e];
+                float2 vv;
+                vv.x = valid_u(W, H, x, y) ? c.z * s0.x - qv.x : 0.0f;
+                vv.y = valid_v(W, H, x, y) ? c.w * s0.y - qv.y : 0.0f;
+                D.q[e] = vv;
+                double ea = (double)vv.x / c.z * vv.x + (double)vv.y / c.w * vv.y;
+                if (!isfinite(ea)) {
+                    ea = 0.0;
+                    f = 1.0;
+                }
+                tr += ea;
+            }
+            tr = wg_sum_all(tr, lds);
+            f = wg_sum_all(f, lds);
+            truth = f != 0.0 ? INFINITY : tr;
+            if (f != 0.0)
+                bad = 1;
+            if (bad) {
+                status = 2;
+                break;
+            }
+            if (truth <= target) {
+                status = 0;
+                // the staggered result (k_vis_unpack)
+                for (int e = threadIdx.x; e < n0; e += kBVT) {
+                    int x, y;
+                    slot_xy(e, W, x, y);
+                    const float2 xs = D.t[e];
+                    if (y < H)
+                        D.u[y * (W + 1) + x] = valid_u(W, H, x, y) ? (double)xs.x : 0.0;
+                    if (x < W)
+                        D.v[y * W + x] = valid_v(W, H, x, y) ? (double)xs.y : 0.0;
+                }
+                break;
+            }
+            if (rnorm <= target) {
+                // the recurrence claims convergence the recomputed residual denies: restart from it
+                for (int e = threadIdx.x; e < n0; e += kBVT)
+                    D.r[e] = D.q[e];
+                __syncthreads();
+                rnorm = truth;
+                rz = alpha = beta = 0.0;
+                steps = 0;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        S->bad = bad;
+        S->rnorm = rnorm;
+        S->truth = truth;
+        S->status = status;
+        S->iterations = iterations;
+    }
+}
+
 }  // namespace
 
 struct ViscositySolver {
@@ -860,16 +1233,10 @@ struct ViscositySolver {
         batch_graph = nullptr;
     }
 
-    // ViscositySolverGPU::Impl::solve (ViscositySolverGPU.cpp:169-294) on device FP64 u, v and an FP64
-    // (rho, mu) material; u, v change only on success.
-    int solve(hipStream_t st, double* u, double* v, const double* mat64, double dt, double spacing, double tol,
-        int maximum, int* iterations_out, double* relative_out)
+    // Set-up of a solve (ViscositySolverGPU.cpp:169-205): material, level coefficients, v0, the rhs r0 and
+    // the initial reduction into the scalar block.
+    void prepare(hipStream_t st, double* u, double* v, const double* mat64, float fdt, double spacing, double tol)
     {
-        *iterations_out = 0;
-        *relative_out = 0.0;
-        if (dt <= 0.0)
-            return 0;
-        const float fdt = (float)dt;
         const int N = W * H;
         hipLaunchKernelGGL(k_vis_material, dim3(std::min((N + kVT - 1) / kVT, 4096)), dim3(kVT), 0, st, N, mat64,
             levels[0]->mat.p);
@@ -884,10 +1251,71 @@ struct ViscositySolver {
         }
         VLev L0 = view(0);
         const dim3 g0 = slot_grid(W, H);
-        const int nb = (int)slot_blocks(W, H);
         hipLaunchKernelGGL(k_vis_pack, g0, dim3(kVT), 0, st, L0, (const double*)u, (const double*)v, v0.p);
         hipLaunchKernelGGL(k_vis_initial, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p, r0.p, fdt, partials.p);
-        finish(st, nb, kVisInitial, tol * tol, fdt);
+        finish(st, (int)slot_blocks(W, H), kVisInitial, tol * tol, fdt);
+    }
+
+    // Batched solve (k_batch_vis): set-up, and the descriptor of this solver's instance.
+    bool batch_prepare(hipStream_t st, double* u, double* v, const double* mat64, double dt, double spacing,
+        double tol, int maximum, BatchVisDesc* desc)
+    {
+        if (levels.size() > (size_t)kMaxVisLevels)
+            return false;
+        prepare(st, u, v, mat64, (float)dt, spacing, tol);
+        const size_t ns = (size_t)(W + 1) * (H + 1);
+        vhip(hipMemcpyAsync(r.p, r0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
+        vhip(hipMemcpyAsync(t.p, v0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
+        *desc = BatchVisDesc{};
+        desc->levels = (int)levels.size();
+        desc->W = W;
+        desc->H = H;
+        desc->maximum = maximum;
+        for (size_t l = 0; l < levels.size(); ++l)
+            desc->lev[l] = view((int)l);
+        desc->v0 = v0.p;
+        desc->r = r.p;
+        desc->d = d.p;
+        desc->q = q.p;
+        desc->t = t.p;
+        desc->u = u;
+        desc->v = v;
+        desc->S = S.p;
+        return true;
+    }
+
+    void batch_fetch(hipStream_t st)
+    {
+        vhip(hipMemcpyAsync(pinned, S.p, sizeof(VisScalars), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    }
+
+    // After batch_fetch and a synchronisation: viscosity_solve's status code, or 4 when the instance needs
+    // more iterations than the batch runs.
+    int batch_result(int maximum, int* iterations, double* relative) const
+    {
+        const VisScalars& s = *pinned;
+        const double denom = std::max(s.energy, 1e-30);
+        *iterations = s.iterations;
+        *relative = std::sqrt(std::max(s.iterations > 0 ? s.truth : s.rnorm, 0.0) / denom);
+        if (s.status == 3 && (maximum < 0 || maximum > s.iterations))
+            return 4;
+        return s.status;
+    }
+
+    // ViscositySolverGPU::Impl::solve (ViscositySolverGPU.cpp:169-294) on device FP64 u, v and an FP64
+    // (rho, mu) material; u, v change only on success.
+    int solve(hipStream_t st, double* u, double* v, const double* mat64, double dt, double spacing, double tol,
+        int maximum, int* iterations_out, double* relative_out)
+    {
+        *iterations_out = 0;
+        *relative_out = 0.0;
+        if (dt <= 0.0)
+            return 0;
+        const float fdt = (float)dt;
+        prepare(st, u, v, mat64, fdt, spacing, tol);
+        VLev L0 = view(0);
+        const dim3 g0 = slot_grid(W, H);
+        const int nb = (int)slot_blocks(W, H);
         VisScalars s = read(st);
         const double denom = std::max(s.energy, 1e-30);
         *relative_out = std::sqrt(std::max(s.rnorm, 0.0) / denom);
@@ -943,6 +1371,27 @@ int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, co
     double spacing, double tol, int maximum, int* iterations, double* relative_residual)
 {
     return s->solve(st, u, v, material, dt, spacing, tol, maximum, iterations, relative_residual);
+}
+
+size_t viscosity_batch_desc_bytes() { return sizeof(BatchVisDesc); }
+
+bool viscosity_batch_prepare(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material,
+    double dt, double spacing, double tol, int maximum, void* desc)
+{
+    return s->batch_prepare(st, u, v, material, dt, spacing, tol, maximum, (BatchVisDesc*)desc);
+}
+
+void launch_batch_viscosity(hipStream_t st, const void* descs, int count)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_batch_vis, dim3(count), dim3(kBVT), 0, st, (const BatchVisDesc*)descs);
+}
+
+void viscosity_batch_fetch(ViscositySolver* s, hipStream_t st) { s->batch_fetch(st); }
+
+int viscosity_batch_result(const ViscositySolver* s, int maximum, int* iterations, double* relative_residual)
+{
+    return s->batch_result(maximum, iterations, relative_residual);
 }
 
 void launch_viscosity_blend(hipStream_t st, int N, double cell_size, int nl, int ng, const double* lm,

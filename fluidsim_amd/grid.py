@@ -191,6 +191,17 @@ def batch_project(grids, dt: float, max_iterations: int = 200, divergence_tolera
     return [ProjectionResult.from_c(res[i]) for i in range(n)]
 
 
+def batch_apply_viscosity(grids, dt: float):
+    """fs_batch_apply_viscosity: every grid's implicit viscosity step (all on one context) with one
+    persistent launch running all their CG loops.  Returns each grid's iteration count."""
+    grids = list(grids)
+    n = len(grids)
+    handles = (C.c_void_p * max(n, 1))(*[g.handle.value for g in grids])
+    its = (C.c_int * max(n, 1))()
+    call("fs_batch_apply_viscosity", n, handles, float(dt), its)
+    return [its[i] for i in range(n)]
+
+
 def _finite_json(value) -> bool:
     """True if every number in a JSON-like value is finite (JSON has no NaN or infinity)."""
     if isinstance(value, float):

@@ -276,6 +276,15 @@ fs_status fs_time_kernel(fs_grid* grid, int kernel, int reps, double* ms_per_lau
 /* Cumulative device bytes allocated by this grid. */
 fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes);
 
+/* Batched implicit viscosity (no reference counterpart; each instance is FluidGridGPU::applyViscosity,
+ * FluidGridGPU.cpp:987-1046: blend, backward-Euler solve to 1e-4): one persistent launch runs the CG loops
+ * of all `count` grids (one context), one workgroup per grid.  Grids without viscosity, or dt <= 0, are
+ * left as they are (0 iterations).  An instance that needs more than 500 iterations is solved again on
+ * its own.  iterations (may be NULL) receives each grid's count.  A failing instance (non-finite state,
+ * breakdown, no convergence) leaves its own velocities unchanged and makes the call return FS_E_RUNTIME
+ * with the first failure's message after every other instance has been applied. */
+fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, int* iterations);
+
 /* ---- batched projection (no reference counterpart: the reference projects one grid per call,
  * FluidGridCPU.cpp:1030-1124) ----
  * fs_project on `count` grids of one context at once: every grid's setup (coefficients, hierarchy, rhs,

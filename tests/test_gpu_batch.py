@@ -182,3 +182,58 @@ def test_instances_with_obstacles_batch_like_single_grids():
         assert np.array_equal(getattr(alone.download(), key), getattr(grids[k].download(), key)), key
     for g in grids + singles + [alone]:
         g.destroy()
+
+
+def viscous_grid(W, H, seed, nu, mask=None):
+    rng = WL.SplitMix64(seed)
+    h = 1.0 / max(W, H)
+    f = WL.pool_with_drop(W, H, h, 0.3 + 0.6 * rng.uniform(1)[0], 0.1, rng)
+    u = np.zeros((W + 1) * H)
+    v = np.zeros(W * (H + 1))
+    for _ in range(6):
+        WL.add_gaussian(u, v, W, H, h, rng.uniform(1, 0, W * h)[0], rng.uniform(1, 0, H * h)[0],
+                        rng.uniform(1, 0.02, 0.08)[0], 0.5 * rng.normal(1)[0])
+    WL.zero_walls(u, v, W, H)
+    lm, gm = WL.two_phase_masses(f, h)
+    g = FluidGridHIP(W, H, h)
+    g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0, nu * 100.0)]),
+                         FluidMixture([MixtureComponent("air", 0.1, nu * 0.1)]))
+    g.upload(FluidGridState(f, u, v, np.zeros(0), [lm], [gm]))
+    if mask is not None:
+        g.setSolidMask(mask)
+    return g
+
+
+VISC_SHAPES = [(17, 13, 0.5), (1, 129, 0.01), (131, 1, 0.1), (64, 64, 1.0), (65, 37, 0.003), (200, 150, 0.05),
+               (300, 200, 0.2), (2, 67, 0.3)]
+
+
+def test_batched_viscosity_matches_single_solves():
+    """fs_batch_apply_viscosity runs each grid's CG loop in one workgroup of one launch: each result meets
+    the same stopping rule as fs_apply_viscosity on the same grid (iteration counts within 2, velocities
+    within the 1e-4 solve tolerance), grids without viscosity are untouched, and a grid's result is the
+    same alone and in the batch, bit for bit."""
+    from fluidsim_amd import batch_apply_viscosity
+
+    dt = 0.01
+    grids = [viscous_grid(W, H, 60 + k, nu) for k, (W, H, nu) in enumerate(VISC_SHAPES)]
+    singles = [viscous_grid(W, H, 60 + k, nu) for k, (W, H, nu) in enumerate(VISC_SHAPES)]
+    inviscid = viscous_grid(40, 30, 99, 0.0)
+    before = inviscid.download()
+    its = batch_apply_viscosity(grids + [inviscid], dt)
+    assert its[-1] == 0
+    after = inviscid.download()
+    assert np.array_equal(after.u, before.u) and np.array_equal(after.v, before.v)
+    for k, (g, s) in enumerate(zip(grids, singles)):
+        n = s.applyViscosity(dt)
+        assert abs(its[k] - n) <= 2, (VISC_SHAPES[k], its[k], n)
+        a, b = g.download(), s.download()
+        scale = max(np.abs(b.u).max(), np.abs(b.v).max(), 1e-12)
+        assert np.abs(a.u - b.u).max() <= 1e-3 * scale and np.abs(a.v - b.v).max() <= 1e-3 * scale, VISC_SHAPES[k]
+    W, H, nu = VISC_SHAPES[5]
+    alone = viscous_grid(W, H, 65, nu)
+    assert batch_apply_viscosity([alone], dt) == [its[5]]
+    assert np.array_equal(alone.download().u, grids[5].download().u)
+    assert np.array_equal(alone.download().v, grids[5].download().v)
+    for g in grids + singles + [inviscid, alone]:
+        g.destroy()

@@ -203,3 +203,29 @@ def test_flexible_cg_reaches_the_same_answer(oracle):
         its.append(r.iterations)
         g.destroy()
     assert ro["converged"] and abs(its[0] - its[1]) <= 5, (its, ro["iterations"])
+
+
+def test_obstacle_tank_error_against_a_tight_fp64_solve(oracle):
+    """The C3 comparison at 512 x 512: every storage mode solved to 1e-6 of the initial divergence against
+    the FP64 oracle solved to 1e-12 -- the 16-bit levels cost no accuracy (errors printed)."""
+    wl = WL.obstacle_tank(512, 512, seed=3)
+    mask = wl.meta["solid"]
+    o = oracle_grid(oracle, wl, mask)
+    md = o.max_divergence()
+    ro = o.project(wl.dt, 5000, 1e-12 * md)
+    assert ro["converged"]
+    so = o.download()
+    fluid = mask == 0
+    po = so["pressure"][fluid]
+    errors = {}
+    for name, m in (("fp32", FS_MG_FP32), ("fp16", FS_MG_FP16), ("bf16", FS_MG_BF16)):
+        g = hip_grid(wl, mask, m)
+        r = g.project(wl.dt, 2000, 1e-6 * md)
+        assert r.converged
+        s = g.download()
+        ph = s.pressure[fluid]
+        errors[name] = (r.iterations, rel_l2(s.u, so["u"]), rel_l2(s.v, so["v"]), rel_l2(ph - ph.mean(), po - po.mean()))
+        g.destroy()
+    print("512^2 tank vs the 1e-12 FP64 solve (iterations, u, v, p rel L2):", errors, "oracle its", ro["iterations"])
+    for name, (_, eu, ev, ep) in errors.items():
+        assert eu < REL_L2 and ev < REL_L2 and ep < REL_L2, name
