@@ -7,11 +7,13 @@ one complete projection -- rhs formation, coefficient and multigrid-hierarchy re
 CG to max|b - A p| <= 1e-6 max|b|, velocity correction and FP64 divergence check -- starting from the
 same device-resident input every time (restored from an HBM snapshot, pressure history reset).
 
-Three further legs are appended to the same JSON line: "mixed_precision" (config C3: the 2048 x 2048
+Four further legs are appended to the same JSON line: "semi_lagrangian" (config C2's 200-step run with
+semi-Lagrangian advection), "mixed_precision" (config C3: the 2048 x 2048
 obstacle tank projected with FP32, binary16 and bfloat16 multigrid levels), "batched" (config C4: 256
 mixed-size grids projected one after another, then all at once with fs_batch_project) and "serving"
 (config C5: a Poisson stream of full simulations served one at a time, then through the serving loop
-at each arrival rate).  --precision-size 0 / --batch-instances 0 / --serving-requests 0 skip them.
+at each arrival rate).  --sl-steps 0 / --precision-size 0 / --batch-instances 0 / --serving-requests 0 skip
+them.
 
 Prints one JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
 """
@@ -161,6 +163,34 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048):
     }
 
 
+def semi_lagrangian_leg(size=1024, steps=200, seed=2):
+    """Config C2's second part: `steps` full steps (semi-Lagrangian advection, gravity, projection to 1e-6
+    of the initial max divergence) of the vortex box, timed end to end."""
+    from fluidsim_amd import FS_TRANSPORT_SEMI_LAGRANGIAN, FluidGridHIP
+    from fluidsim_amd import workloads as WL
+
+    wl = WL.vortex_box(size, size, seed=seed)
+    g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+    WL.load_into(g, wl)
+    g.setTransport(FS_TRANSPORT_SEMI_LAGRANGIAN)
+    tau = 1e-6 * g.maxDivergence()
+    g.step(wl.dt, 200, tau)   # warm-up: hierarchy, graphs
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = g.step(wl.dt, 200, tau)
+        if not r.accepted():
+            raise RuntimeError(f"semi-Lagrangian step rejected: {r}")
+        its.append(r.iterations)
+    elapsed = time.perf_counter() - t0
+    div = g.maxDivergence()
+    g.destroy()
+    return {"workload": f"vortex_box {size}x{size}, {steps} steps of semi-Lagrangian advection + gravity + "
+                        "projection to 1e-6 of the initial max divergence, seed 2",
+            "steps_per_s": steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+            "pcg_iterations_mean": float(np.mean(its)), "final_max_divergence_over_tol": div / tau}
+
+
 def smoother_model_bytes(coef_bytes, coarse_bytes):
     """Algorithmic bytes per fine cell of the finest post-smoothing launch: four sweeps of (read solution
     and rhs, FP32 on the finest level, read two conductances, write the solution) plus the bilinear
@@ -235,13 +265,15 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
     return out
 
 
-def batch_leg(count, seed=4, visc_sample=16):
+def batch_leg(count, seed=4):
     """Config C4 on one GPU: `count` seeded instances (sides log-uniform 64..512, 30-90% liquid plus a
     drop, vortex velocity, obstacles over 0-25% of the cells), each projected once to 1e-6 of its own initial max divergence.  Timed twice
     from the same device-resident inputs: one fs_project per grid in turn (the plain GPU MGPCG), then one
     fs_batch_project over all of them (one persistent launch for all PCG iterations).  Each instance's
-    implicit viscosity solve (nu log-uniform 1e-3..1) is timed on its own, instance by instance."""
-    from fluidsim_amd import FluidGridHIP, FluidMixture, MixtureComponent, batch_project, FS_TRANSPORT_NONE
+    implicit viscosity solve (nu log-uniform 1e-3..1) is timed the same two ways: fs_apply_viscosity one
+    grid after another, then fs_batch_apply_viscosity over all of them."""
+    from fluidsim_amd import (FluidGridHIP, FluidMixture, MixtureComponent, batch_apply_viscosity, batch_project,
+                              FS_TRANSPORT_NONE)
     from fluidsim_amd import workloads as WL
 
     t_start = time.perf_counter()
@@ -293,16 +325,23 @@ def batch_leg(count, seed=4, visc_sample=16):
     batch_s = time.perf_counter() - t0
     note(f"batched {batch_s * 1e3:.1f} ms")
     ok = all(r.converged for r in serial) and all(r.converged for r in batched)
-    # viscosity: a bounded sample of the instances, one solve each
+    # viscosity: every instance, one by one and then batched (warm-up of both first)
     restore()
-    sample = grids[:visc_sample]
+    for g in grids:
+        g.applyViscosity(dt)
+    restore()
+    batch_apply_viscosity(grids, dt)
+    restore()
     t0 = time.perf_counter()
-    vit = 0
-    for k, g in enumerate(sample):
-        vit += g.applyViscosity(dt)
-        if (k + 1) % 8 == 0:
-            note(f"viscosity {k + 1}/{len(sample)}: {vit} iterations so far")
+    vits = [g.applyViscosity(dt) for g in grids]
     visc_s = time.perf_counter() - t0
+    note(f"viscosity one by one {visc_s * 1e3:.1f} ms")
+    restore()
+    t0 = time.perf_counter()
+    bvits = batch_apply_viscosity(grids, dt)
+    grids[0].context.synchronize()
+    visc_batch_s = time.perf_counter() - t0
+    note(f"viscosity batched {visc_batch_s * 1e3:.1f} ms")
     for k, g in enumerate(grids):
         g.destroy()
         if (k + 1) % 32 == 0:
@@ -321,9 +360,11 @@ def batch_leg(count, seed=4, visc_sample=16):
         "converged_all": bool(ok),
         "pcg_iterations": {"serial_mean": float(np.mean([r.iterations for r in serial])),
                            "batched_mean": float(np.mean([r.iterations for r in batched]))},
-        "viscosity_sample": len(sample),
-        "viscosity_solves_per_s_serial": len(sample) / visc_s if sample else None,
-        "viscosity_iterations_mean": vit / max(len(sample), 1),
+        "viscosity_solves_per_s_serial": count / visc_s,
+        "viscosity_solves_per_s_batched": count / visc_batch_s,
+        "viscosity_speedup": visc_s / visc_batch_s,
+        "viscosity_iterations": {"serial_mean": float(np.mean(vits)), "batched_mean": float(np.mean(bvits))},
+        "projection_plus_viscosity_speedup": (serial_s + visc_s) / (batch_s + visc_batch_s),
     }
 
 
@@ -349,6 +390,7 @@ def main():
     ap.add_argument("--precision-size", type=int, default=2048, help="side of the C3 mixed-precision leg (0: skip)")
     ap.add_argument("--precision-steps", type=int, default=3)
     ap.add_argument("--precision-only", action="store_true")
+    ap.add_argument("--sl-steps", type=int, default=200, help="C2 semi-Lagrangian steps leg (0: skip)")
     ap.add_argument("--precision-modes", type=str, default="", help="subset of fp32,fp16,bf16,bf16_vectors")
     args = ap.parse_args()
 
@@ -457,6 +499,8 @@ def main():
     }
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+    if args.sl_steps > 0:
+        out["semi_lagrangian"] = semi_lagrangian_leg(args.size, args.sl_steps)
     if args.precision_size > 0:
         out["mixed_precision"] = precision_leg(args.precision_size, steps=args.precision_steps)
     if args.batch_instances > 0:
