@@ -1038,6 +1038,7 @@ constexpr int kMultiNS = 4;
 // reductions run in another order (thread-strided sums, then the 16 wavefronts in order).
 // ---------------------------------------------------------------------------------------------
 constexpr int kBT = kTailThreads;
+constexpr int kBU = 4;   // items per thread whose loads are issued together in the workgroup-strided loops
 constexpr int kBW = kBT / 64;
 
 // Workgroup sum / max, valid in every thread.
@@ -1081,17 +1082,39 @@ __device__ void wg_sweeps(const LevelView& L, float* sol, const float* rhs, cons
     for (int k = 0; k < 2 * kMultiNS; ++k) {
         const int color = first ^ (k & 1);
         const bool neighbours = !(from_zero && k == 0);
-        for (int t = threadIdx.x; t < items; t += kBT) {
-            const int y = t / pairs;
-            const int x = 2 * (t - y * pairs) + ((color + y) & 1);
-            if (x >= w)
-                continue;
-            const int i = (y + 1) * s + x + 1;
-            const float a = L.right[i - 1], b = L.right[i], u = L.down[i - s], d = L.down[i];
-            float v = shift ? (float)((double)rhs[i] - sh) : rhs[i];
-            if (neighbours)
-                v += (a * sol[i - 1] + b * sol[i + 1]) + (u * sol[i - s] + d * sol[i + s]);
-            sol[i] = v * inv_or_zero((a + b) + (u + d));
+        // kBU items per thread at a time: every load of the group issues before the first store (the
+        // items of one colour never read each other), through clamped valid indices
+        for (int t0 = threadIdx.x; t0 < items; t0 += kBU * kBT) {
+            int idx[kBU];
+            bool on[kBU];
+            float a[kBU], b[kBU], u[kBU], d[kBU], rv[kBU], n0[kBU], n1[kBU], n2[kBU], n3[kBU];
+#pragma unroll
+            for (int q = 0; q < kBU; ++q) {
+                const int t = t0 + q * kBT;
+                const int tt = min(t, items - 1);
+                const int y = tt / pairs;
+                const int x = 2 * (tt - y * pairs) + ((color + y) & 1);
+                on[q] = t < items && x < w;
+                const int i = (y + 1) * s + min(x, w - 1) + 1;
+                idx[q] = i;
+                a[q] = L.right[i - 1];
+                b[q] = L.right[i];
+                u[q] = L.down[i - s];
+                d[q] = L.down[i];
+                rv[q] = rhs[i];
+                n0[q] = sol[i - 1];
+                n1[q] = sol[i + 1];
+                n2[q] = sol[i - s];
+                n3[q] = sol[i + s];
+            }
+#pragma unroll
+            for (int q = 0; q < kBU; ++q) {
+                float v = shift ? (float)((double)rv[q] - sh) : rv[q];
+                if (neighbours)
+                    v += (a[q] * n0[q] + b[q] * n1[q]) + (u[q] * n2[q] + d[q] * n3[q]);
+                if (on[q])
+                    sol[idx[q]] = v * inv_or_zero((a[q] + b[q]) + (u[q] + d[q]));
+            }
         }
         __syncthreads();
     }
@@ -1105,30 +1128,54 @@ __device__ void wg_res_restrict(const LevelView& F, const float* rhs, const doub
     const int w = F.w, h = F.h, s = F.s;
     const float* sol = F.sol[1];
     const double sh = shift ? *shift : 0.0;
-    for (int e = threadIdx.x; e < w * h; e += kBT) {
-        const int y = e / w, x = e - y * w;
-        const int i = (y + 1) * s + x + 1;
-        const float c = sol[i];
-        const float product = (F.right[i - 1] * (c - sol[i - 1]) + F.right[i] * (c - sol[i + 1])) +
-                              (F.down[i - s] * (c - sol[i - s]) + F.down[i] * (c - sol[i + s]));
-        const float b = shift ? (float)((double)rhs[i] - sh) : rhs[i];
-        const bool active = (F.right[i - 1] + F.right[i]) + (F.down[i - s] + F.down[i]) > 0.f;
-        F.res[i] = active ? b - product : 0.f;   // cells without conductance are not restricted
+    const int cells = w * h;
+    for (int e0 = threadIdx.x; e0 < cells; e0 += kBU * kBT) {
+        int idx[kBU];
+        float val[kBU];
+#pragma unroll
+        for (int q = 0; q < kBU; ++q) {
+            const int e = min(e0 + q * kBT, cells - 1);
+            const int y = e / w, x = e - y * w;
+            const int i = (y + 1) * s + x + 1;
+            idx[q] = i;
+            const float c = sol[i];
+            const float product = (F.right[i - 1] * (c - sol[i - 1]) + F.right[i] * (c - sol[i + 1])) +
+                                  (F.down[i - s] * (c - sol[i - s]) + F.down[i] * (c - sol[i + s]));
+            const float b = shift ? (float)((double)rhs[i] - sh) : rhs[i];
+            const bool active = (F.right[i - 1] + F.right[i]) + (F.down[i - s] + F.down[i]) > 0.f;
+            val[q] = active ? b - product : 0.f;   // cells without conductance are not restricted
+        }
+#pragma unroll
+        for (int q = 0; q < kBU; ++q)
+            if (e0 + q * kBT < cells)
+                F.res[idx[q]] = val[q];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < C.w * C.h; e += kBT) {
-        const int y = e / C.w, x = e - y * C.w;
-        float value = 0.f;
+    const int ccells = C.w * C.h;
+    for (int e0 = threadIdx.x; e0 < ccells; e0 += kBU * kBT) {
+        int dst[kBU];
+        float val[kBU];
 #pragma unroll
-        for (int iy = 0; iy < 4; ++iy) {
-            const float* row = F.res + (ry.index[y * 4 + iy] + 1) * s + 1;
-            float horizontal = 0.f;
+        for (int q = 0; q < kBU; ++q) {
+            const int e = min(e0 + q * kBT, ccells - 1);
+            const int y = e / C.w, x = e - y * C.w;
+            float value = 0.f;
 #pragma unroll
-            for (int ix = 0; ix < 4; ++ix)
-                horizontal += rx.weight[x * 4 + ix] * row[rx.index[x * 4 + ix]];
-            value += ry.weight[y * 4 + iy] * horizontal;
+            for (int iy = 0; iy < 4; ++iy) {
+                const float* row = F.res + (ry.index[y * 4 + iy] + 1) * s + 1;
+                float horizontal = 0.f;
+#pragma unroll
+                for (int ix = 0; ix < 4; ++ix)
+                    horizontal += rx.weight[x * 4 + ix] * row[rx.index[x * 4 + ix]];
+                value += ry.weight[y * 4 + iy] * horizontal;
+            }
+            val[q] = value;
+            dst[q] = (y + 1) * C.s + x + 1;
         }
-        C.rhs[(y + 1) * C.s + x + 1] = value;
+#pragma unroll
+        for (int q = 0; q < kBU; ++q)
+            if (e0 + q * kBT < ccells)
+                C.rhs[dst[q]] = val[q];
     }
     __syncthreads();
 }
@@ -1137,18 +1184,30 @@ __device__ void wg_res_restrict(const LevelView& F, const float* rhs, const doub
 __device__ void wg_prolong(const LevelView& L, const CoarseLink& link)
 {
     const int w = L.w, h = L.h, s = L.s;
-    for (int e = threadIdx.x; e < w * h; e += kBT) {
-        const int y = e / w, x = e - y * w;
-        const int i = (y + 1) * s + x + 1;
-        const float wx = link.pxw[x], wy = link.pyw[y];
-        const float* top = link.csol + (link.pyf[y] + 1) * link.cs + 1;
-        const float* bot = link.csol + (link.pys[y] + 1) * link.cs + 1;
-        const int a0 = link.pxf[x], a1 = link.pxs[x];
-        const float a = (1.f - wx) * top[a0] + wx * top[a1];
-        const float b = (1.f - wx) * bot[a0] + wx * bot[a1];
-        float v = L.sol[1][i];
-        v += (1.f - wy) * a + wy * b;
-        L.sol[0][i] = v;
+    const int cells = w * h;
+    for (int e0 = threadIdx.x; e0 < cells; e0 += kBU * kBT) {
+        int idx[kBU];
+        float val[kBU];
+#pragma unroll
+        for (int q = 0; q < kBU; ++q) {
+            const int e = min(e0 + q * kBT, cells - 1);
+            const int y = e / w, x = e - y * w;
+            const int i = (y + 1) * s + x + 1;
+            idx[q] = i;
+            const float wx = link.pxw[x], wy = link.pyw[y];
+            const float* top = link.csol + (link.pyf[y] + 1) * link.cs + 1;
+            const float* bot = link.csol + (link.pys[y] + 1) * link.cs + 1;
+            const int a0 = link.pxf[x], a1 = link.pxs[x];
+            const float a = (1.f - wx) * top[a0] + wx * top[a1];
+            const float b = (1.f - wx) * bot[a0] + wx * bot[a1];
+            float v = L.sol[1][i];
+            v += (1.f - wy) * a + wy * b;
+            val[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < kBU; ++q)
+            if (e0 + q * kBT < cells)
+                L.sol[0][idx[q]] = val[q];
     }
     __syncthreads();
 }
@@ -1172,14 +1231,27 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
     while (S.status == kRunning) {
         // q = A d and d.q (k_pcg_ad), alpha (kFinAd)
         double dot = 0.0;
-        for (int e = threadIdx.x; e < N; e += kBT) {
-            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
-            const float c = g.d[i];
-            const float dl = g.d[i - 1], dr = g.d[i + 1], du = g.d[i - s], dd = g.d[i + s];
-            const float wl = g.right32[i - 1], wr = g.right32[i], wu = g.down32[i - s], wd = g.down32[i];
-            const float q = (wl * (c - dl) + wr * (c - dr)) + (wu * (c - du) + wd * (c - dd));
-            g.q[i] = q;
-            dot += (double)c * (double)q;
+        for (int e0 = threadIdx.x; e0 < N; e0 += kBU * kBT) {
+            int idx[kBU];
+            float cv[kBU], qv[kBU];
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                const int e = min(e0 + u * kBT, N - 1);
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                idx[u] = i;
+                const float c = g.d[i];
+                const float dl = g.d[i - 1], dr = g.d[i + 1], du = g.d[i - s], dd = g.d[i + s];
+                const float wl = g.right32[i - 1], wr = g.right32[i], wu = g.down32[i - s], wd = g.down32[i];
+                cv[u] = c;
+                qv[u] = (wl * (c - dl) + wr * (c - dr)) + (wu * (c - du) + wd * (c - dd));
+            }
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                if (e0 + u * kBT < N) {
+                    g.q[idx[u]] = qv[u];
+                    dot += (double)cv[u] * (double)qv[u];
+                }
+            }
         }
         dot = wg_sum(dot, slot[0]);
         if (threadIdx.x == 0) {
@@ -1199,13 +1271,26 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
         // x += alpha d, r -= alpha q (k_pcg_update); recurrence checks (kFinUpdate)
         const float alpha = (float)S.alpha;
         double sum = 0.0, m = 0.0;
-        for (int e = threadIdx.x; e < N; e += kBT) {
-            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
-            g.x[i] = g.x[i] + alpha * g.d[i];
-            const float r = g.r[i] - alpha * g.q[i];
-            g.r[i] = r;
-            sum += (double)r;
-            m = fmax(m, (double)fabsf(r));
+        for (int e0 = threadIdx.x; e0 < N; e0 += kBU * kBT) {
+            int idx[kBU];
+            float xv[kBU], rv[kBU];
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                const int e = min(e0 + u * kBT, N - 1);
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                idx[u] = i;
+                xv[u] = g.x[i] + alpha * g.d[i];
+                rv[u] = g.r[i] - alpha * g.q[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                if (e0 + u * kBT < N) {
+                    g.x[idx[u]] = xv[u];
+                    g.r[idx[u]] = rv[u];
+                    sum += (double)rv[u];
+                    m = fmax(m, (double)fabsf(rv[u]));
+                }
+            }
         }
         sum = wg_sum(sum, slot[0]);
         m = wg_max(m, slot[1]);
@@ -1293,12 +1378,24 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
         }
         // sum z, r.z, sum r (k_pcg_rz), beta (kFinRz)
         double sz = 0.0, srz = 0.0, sr = 0.0;
-        for (int e = threadIdx.x; e < N; e += kBT) {
-            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
-            const double zz = (double)z[i], rr = (double)g.r[i];
-            sz += zz;
-            srz += rr * zz;
-            sr += rr;
+        for (int e0 = threadIdx.x; e0 < N; e0 += kBU * kBT) {
+            float zv[kBU], rv[kBU];
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                const int e = min(e0 + u * kBT, N - 1);
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                zv[u] = z[i];
+                rv[u] = g.r[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                if (e0 + u * kBT < N) {
+                    const double zz = (double)zv[u], rr = (double)rv[u];
+                    sz += zz;
+                    srz += rr * zz;
+                    sr += rr;
+                }
+            }
         }
         sz = wg_sum(sz, slot[0]);
         srz = wg_sum(srz, slot[1]);
@@ -1325,10 +1422,21 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
         const float zmean = (float)S.z_mean;
         const double beta64 = S.beta;
         const float beta = (float)beta64;
-        for (int e = threadIdx.x; e < N; e += kBT) {
-            const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
-            const float zz = z[i] - zmean;
-            g.d[i] = beta64 == 0.0 ? zz : zz + beta * g.d[i];
+        for (int e0 = threadIdx.x; e0 < N; e0 += kBU * kBT) {
+            int idx[kBU];
+            float dv[kBU];
+#pragma unroll
+            for (int u = 0; u < kBU; ++u) {
+                const int e = min(e0 + u * kBT, N - 1);
+                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                idx[u] = i;
+                const float zz = z[i] - zmean;
+                dv[u] = beta64 == 0.0 ? zz : zz + beta * g.d[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kBU; ++u)
+                if (e0 + u * kBT < N)
+                    g.d[idx[u]] = dv[u];
         }
         __syncthreads();
     }
