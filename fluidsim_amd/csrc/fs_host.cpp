@@ -226,6 +226,7 @@ using namespace fsim;
 struct fs_context {
     int device = 0;
     size_t lds_per_block = 0;
+    int compute_units = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;          // second stream (default priority): hierarchy rebuild overlap
     hipEvent_t fork = nullptr, join = nullptr;
@@ -240,6 +241,7 @@ struct fs_context {
 
 constexpr int kPressureBudget = 100000;   // PCG iterations of an unbounded (negative) budget
 constexpr size_t kFoldBlocks = 1536;      // cell blocks (64 x 16) beyond which PCG partials are pre-reduced
+constexpr size_t kBatchSpreadLds = 84 * 1024;   // LDS of a batched-solve workgroup: one per compute unit
 
 struct fs_grid {
     fs_context* ctx = nullptr;
@@ -1358,6 +1360,9 @@ fs_status fs_context_create_ex(int device, int flags, fs_context** out)
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
             ctx->lds_per_block = (size_t)lds;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+            ctx->compute_units = cus;
         HIPC(hipStreamCreate(&ctx->stream));
         if (!(flags & FS_CONTEXT_SINGLE_STREAM)) {
             HIPC(hipStreamCreate(&ctx->aux));
@@ -1980,7 +1985,8 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
             if (ctx->vis_batch_desc.n < need_bytes)
                 ctx->vis_batch_desc.alloc(need_bytes);
             HIPC(hipMemcpyAsync(ctx->vis_batch_desc.p, descs.data(), need_bytes, hipMemcpyHostToDevice, st));
-            launch_batch_viscosity(st, ctx->vis_batch_desc.p, (int)batched.size());
+            const bool spread = (int)batched.size() <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds;
+            launch_batch_viscosity(st, ctx->vis_batch_desc.p, (int)batched.size(), spread ? kBatchSpreadLds : 0);
             for (int i : batched)
                 viscosity_batch_fetch(grids[i]->visc, st);
         }
@@ -2622,6 +2628,10 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             ctx->batch_desc.alloc((size_t)count);
         HIPC(hipMemcpyAsync(ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance),
             hipMemcpyHostToDevice, st));
+        // One workgroup is one instance's whole solve: while the instances fit on the compute units one
+        // each, reserve more than half a unit's 160 KB of LDS so that no two of them share a unit.
+        if (count <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds)
+            lds = std::max(lds, kBatchSpreadLds);
         if (lds > 65536)
             need(batch_allow_lds(lds), FS_E_DEVICE, "the device refused the batched solve's LDS request");
         launch_batch_pcg(st, ctx->batch_desc.p, count, lds);

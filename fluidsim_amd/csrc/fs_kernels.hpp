@@ -263,7 +263,8 @@ int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, co
 size_t viscosity_batch_desc_bytes();
 bool viscosity_batch_prepare(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material,
     double dt, double spacing, double tol, int maximum, void* desc);
-void launch_batch_viscosity(hipStream_t st, const void* descs, int count);
+// reserve_lds: dynamic LDS to reserve per workgroup (0: none), so that instances spread one per compute unit.
+void launch_batch_viscosity(hipStream_t st, const void* descs, int count, size_t reserve_lds);
 void viscosity_batch_fetch(ViscositySolver* s, hipStream_t st);
 int viscosity_batch_result(const ViscositySolver* s, int maximum, int* iterations, double* relative_residual);
 

@@ -1039,6 +1039,10 @@ constexpr int kMultiNS = 4;
 // ---------------------------------------------------------------------------------------------
 constexpr int kBT = kTailThreads;
 constexpr int kBU = 4;   // items per thread whose loads are issued together in the workgroup-strided loops
+
+// t / d in the workgroup-strided loops: the float-reciprocal quotient (tail_div) below 2^22 items, else
+// an integer division; `fast` is uniform across the workgroup.
+__device__ __forceinline__ int bdiv(int t, int d, float inv_d, bool fast) { return fast ? tail_div(t, inv_d) : t / d; }
 constexpr int kBW = kBT / 64;
 
 // Workgroup sum / max, valid in every thread.
@@ -1079,6 +1083,8 @@ __device__ void wg_sweeps(const LevelView& L, float* sol, const float* rhs, cons
     const int pairs = (w + 1) >> 1;
     const int items = pairs * h;
     const double sh = shift ? *shift : 0.0;
+    const float inv_pairs = 1.0f / (float)pairs;
+    const bool fast = items < (1 << 22);
     for (int k = 0; k < 2 * kMultiNS; ++k) {
         const int color = first ^ (k & 1);
         const bool neighbours = !(from_zero && k == 0);
@@ -1092,7 +1098,7 @@ __device__ void wg_sweeps(const LevelView& L, float* sol, const float* rhs, cons
             for (int q = 0; q < kBU; ++q) {
                 const int t = t0 + q * kBT;
                 const int tt = min(t, items - 1);
-                const int y = tt / pairs;
+                const int y = bdiv(tt, pairs, inv_pairs, fast);
                 const int x = 2 * (tt - y * pairs) + ((color + y) & 1);
                 on[q] = t < items && x < w;
                 const int i = (y + 1) * s + min(x, w - 1) + 1;
@@ -1129,13 +1135,15 @@ __device__ void wg_res_restrict(const LevelView& F, const float* rhs, const doub
     const float* sol = F.sol[1];
     const double sh = shift ? *shift : 0.0;
     const int cells = w * h;
+    const float inv_w = 1.0f / (float)w;
+    const bool fast = cells < (1 << 22);
     for (int e0 = threadIdx.x; e0 < cells; e0 += kBU * kBT) {
         int idx[kBU];
         float val[kBU];
 #pragma unroll
         for (int q = 0; q < kBU; ++q) {
             const int e = min(e0 + q * kBT, cells - 1);
-            const int y = e / w, x = e - y * w;
+            const int y = bdiv(e, w, inv_w, fast), x = e - y * w;
             const int i = (y + 1) * s + x + 1;
             idx[q] = i;
             const float c = sol[i];
@@ -1152,13 +1160,14 @@ __device__ void wg_res_restrict(const LevelView& F, const float* rhs, const doub
     }
     __syncthreads();
     const int ccells = C.w * C.h;
+    const float inv_cw = 1.0f / (float)C.w;
     for (int e0 = threadIdx.x; e0 < ccells; e0 += kBU * kBT) {
         int dst[kBU];
         float val[kBU];
 #pragma unroll
         for (int q = 0; q < kBU; ++q) {
             const int e = min(e0 + q * kBT, ccells - 1);
-            const int y = e / C.w, x = e - y * C.w;
+            const int y = bdiv(e, C.w, inv_cw, fast), x = e - y * C.w;
             float value = 0.f;
 #pragma unroll
             for (int iy = 0; iy < 4; ++iy) {
@@ -1185,13 +1194,15 @@ __device__ void wg_prolong(const LevelView& L, const CoarseLink& link)
 {
     const int w = L.w, h = L.h, s = L.s;
     const int cells = w * h;
+    const float inv_w = 1.0f / (float)w;
+    const bool fast = cells < (1 << 22);
     for (int e0 = threadIdx.x; e0 < cells; e0 += kBU * kBT) {
         int idx[kBU];
         float val[kBU];
 #pragma unroll
         for (int q = 0; q < kBU; ++q) {
             const int e = min(e0 + q * kBT, cells - 1);
-            const int y = e / w, x = e - y * w;
+            const int y = bdiv(e, w, inv_w, fast), x = e - y * w;
             const int i = (y + 1) * s + x + 1;
             idx[q] = i;
             const float wx = link.pxw[x], wy = link.pyw[y];
@@ -1221,6 +1232,8 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
     const GridDev& g = I.g;
     const int W = g.W, H = g.H, s = g.s, N = W * H;
     const double n = (double)N;
+    const float inv_W = 1.0f / (float)W;
+    const bool fast = N < (1 << 22);
     if (threadIdx.x == 0) {
         S = *g.S;
         S.rz = S.rz_slot[S.iterations & 1];   // r.z of the first direction (written by its kernel)
@@ -1237,7 +1250,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
 #pragma unroll
             for (int u = 0; u < kBU; ++u) {
                 const int e = min(e0 + u * kBT, N - 1);
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 idx[u] = i;
                 const float c = g.d[i];
                 const float dl = g.d[i - 1], dr = g.d[i + 1], du = g.d[i - s], dd = g.d[i + s];
@@ -1277,7 +1290,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
 #pragma unroll
             for (int u = 0; u < kBU; ++u) {
                 const int e = min(e0 + u * kBT, N - 1);
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 idx[u] = i;
                 xv[u] = g.x[i] + alpha * g.d[i];
                 rv[u] = g.r[i] - alpha * g.q[i];
@@ -1313,13 +1326,13 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
         if (S.verify) {
             double tsum = 0.0, tmax = 0.0;
             for (int e = threadIdx.x; e < N; e += kBT) {
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 g.p[i] = g.p[i] + (double)g.x[i];
                 g.x[i] = 0.0f;
             }
             __syncthreads();
             for (int e = threadIdx.x; e < N; e += kBT) {
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 const double c = g.p[i];
                 const double product = (g.right64[i - 1] * (c - g.p[i - 1]) + g.right64[i] * (c - g.p[i + 1])) +
                                        (g.down64[i - s] * (c - g.p[i - s]) + g.down64[i] * (c - g.p[i + s]));
@@ -1371,7 +1384,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
         } else {
             const double mean = S.r_mean;
             for (int e = threadIdx.x; e < N; e += kBT) {
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 g.zdiag[i] = (float)(((double)g.r[i] - mean) * (double)g.invd32[i]);
             }
             __syncthreads();
@@ -1383,7 +1396,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
 #pragma unroll
             for (int u = 0; u < kBU; ++u) {
                 const int e = min(e0 + u * kBT, N - 1);
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 zv[u] = z[i];
                 rv[u] = g.r[i];
             }
@@ -1428,7 +1441,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
 #pragma unroll
             for (int u = 0; u < kBU; ++u) {
                 const int e = min(e0 + u * kBT, N - 1);
-                const int y = e / W, i = (y + 1) * s + (e - y * W) + 1;
+                const int y = bdiv(e, W, inv_W, fast), i = (y + 1) * s + (e - y * W) + 1;
                 idx[u] = i;
                 const float zz = z[i] - zmean;
                 dv[u] = beta64 == 0.0 ? zz : zz + beta * g.d[i];

@@ -1381,10 +1381,16 @@ bool viscosity_batch_prepare(ViscositySolver* s, hipStream_t st, double* u, doub
     return s->batch_prepare(st, u, v, material, dt, spacing, tol, maximum, (BatchVisDesc*)desc);
 }
 
-void launch_batch_viscosity(hipStream_t st, const void* descs, int count)
+void launch_batch_viscosity(hipStream_t st, const void* descs, int count, size_t reserve_lds)
 {
-    if (count > 0)
-        hipLaunchKernelGGL(k_batch_vis, dim3(count), dim3(kBVT), 0, st, (const BatchVisDesc*)descs);
+    if (count <= 0)
+        return;
+    // reserved (unused) dynamic LDS keeps a second instance off a busy compute unit
+    if (reserve_lds > 65536 &&
+        hipFuncSetAttribute((const void*)k_batch_vis, hipFuncAttributeMaxDynamicSharedMemorySize, (int)reserve_lds) !=
+            hipSuccess)
+        reserve_lds = 0;
+    hipLaunchKernelGGL(k_batch_vis, dim3(count), dim3(kBVT), reserve_lds, st, (const BatchVisDesc*)descs);
 }
 
 void viscosity_batch_fetch(ViscositySolver* s, hipStream_t st) { s->batch_fetch(st); }
