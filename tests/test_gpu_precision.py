@@ -207,12 +207,13 @@ def test_flexible_cg_reaches_the_same_answer(oracle):
 
 def test_obstacle_tank_error_against_a_tight_fp64_solve(oracle):
     """The C3 comparison at 512 x 512: every storage mode solved to 1e-6 of the initial divergence against
-    the FP64 oracle solved to 1e-12 -- the 16-bit levels cost no accuracy (errors printed)."""
+    the FP64 oracle solved to 1e-11 (as tight as FP64 CG gets on this 1000:1 masked problem: at 1e-12 it
+    stalls near 3e-12) -- the 16-bit levels cost no accuracy (errors printed)."""
     wl = WL.obstacle_tank(512, 512, seed=3)
     mask = wl.meta["solid"]
     o = oracle_grid(oracle, wl, mask)
     md = o.max_divergence()
-    ro = o.project(wl.dt, 5000, 1e-12 * md)
+    ro = o.project(wl.dt, 5000, 1e-11 * md)
     assert ro["converged"]
     so = o.download()
     fluid = mask == 0
@@ -226,6 +227,6 @@ def test_obstacle_tank_error_against_a_tight_fp64_solve(oracle):
         ph = s.pressure[fluid]
         errors[name] = (r.iterations, rel_l2(s.u, so["u"]), rel_l2(s.v, so["v"]), rel_l2(ph - ph.mean(), po - po.mean()))
         g.destroy()
-    print("512^2 tank vs the 1e-12 FP64 solve (iterations, u, v, p rel L2):", errors, "oracle its", ro["iterations"])
+    print("512^2 tank vs the 1e-11 FP64 solve (iterations, u, v, p rel L2):", errors, "oracle its", ro["iterations"])
     for name, (_, eu, ev, ep) in errors.items():
         assert eu < REL_L2 and ev < REL_L2 and ep < REL_L2, name
