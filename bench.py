@@ -450,8 +450,10 @@ def main():
 
     # Dominant kernel: the finest-level post-smoothing launch, timed with HIP events on the grid's stream.
     variants = {0: (64, 32, 256), 1: (32, 32, 256), 2: (32, 16, 256), 3: (16, 16, 256), 4: (64, 32, 512),
-                5: (32, 32, 512)}
-    tx, ty, nt = variants[grid.multigridTiles(0)[1]]
+                5: (32, 32, 512), 6: (32, 32, 384), 7: (64, 32, 640), 8: (32, 64, 640),
+                9: (64, 32, 960), 10: (32, 64, 960)}
+    post_variant = grid.multigridTiles(0)[1]
+    tx, ty, nt = variants[post_variant]
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
     sweep_bytes = SMOOTH_BYTES_PER_CELL * wl.width * wl.height
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
@@ -484,7 +486,7 @@ def main():
         "converged_all": bool(ok),
         "max_rel_residual": float(max(rel)),
         "roofline": {
-            "kernel": f"k_rb_multi<{tx},{ty},4,{1 if (tx, ty) == (64, 32) else 0},1,{nt},float,float,float> "
+            "kernel": f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float> "
                       "(finest-level prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
             "bound": "hbm",
             "achieved": achieved,

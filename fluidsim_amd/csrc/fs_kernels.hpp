@@ -154,8 +154,9 @@ void launch_precond_diag(hipStream_t st, const GridDev& g);
 // Four red-black sweeps in one launch (reads sol[cur], writes sol[cur ^ 1]); with link.csol set the
 // coarse correction is prolonged and added first.  fine selects the finest-level symbol.
 // variant: interior tile and threads 0 = 64 x 32 / 256, 1 = 32 x 32 / 256, 2 = 32 x 16 / 256,
-// 3 = 16 x 16 / 256, 4 = 64 x 32 / 512, 5 = 32 x 32 / 512 (same results, different speed).
-constexpr int kSmoothVariants = 6;
+// 3 = 16 x 16 / 256, 4 = 64 x 32 / 512, 5 = 32 x 32 / 512, 6 = 32 x 32 / 384, 7 = 64 x 32 / 640,
+// 8 = 32 x 64 / 640, 9 = 64 x 32 / 960, 10 = 32 x 64 / 960 (same results, different speed).
+constexpr int kSmoothVariants = 11;
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
     int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant);
 // Residual of sol[cur] and its restriction into C.rhs in one launch; variant: coarse tile 0 = 32 x 16,

@@ -69,8 +69,12 @@ __device__ __forceinline__ void xcd_tile(int gx, int gy, int& bx, int& by)
 // registers for all eight half-sweeps, so an update reads four neighbours from LDS and writes one value.
 // MODE 0: pre-smoothing from a zero guess (no solution read); 1: post-smoothing, the prolonged coarse
 // correction added to the solution first; 2: plain sweeps of the stored solution.
-// 512-thread post-smoothing tiles are held to 128 VGPRs (four waves per SIMD, two workgroups per CU).
-template <int MODE, int NT> constexpr int rb_waves_per_eu() { return (MODE == 1 && NT == 512) ? 4 : 1; }
+// Post-smoothing tiles are held to the registers that keep their workgroups per CU: 128 VGPRs for 512
+// threads (four waves per SIMD, two workgroups) and 960 (one), 170 for 384 (two) and 640 threads (one).
+template <int MODE, int NT> constexpr int rb_waves_per_eu()
+{
+    return MODE != 1 ? 1 : (NT == 512 || NT == 960) ? 4 : (NT == 384 || NT == 640) ? 3 : 1;
+}
 
 // C: conductance storage, V: this level's vectors (rhs, solution), VC: the coarse solution (MODE 1).
 template <int TX, int TY, int NS, int FINE, int MODE, int NT, class C = float, class V = float, class VC = float>
@@ -133,7 +137,7 @@ void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, 
     if (MODE == 1) {
         // per chunk of PC cells, two batched phases: transfer tables, then the four coarse values (chunks
         // bound the registers this prologue adds to the kernel)
-        constexpr int PC = NT == 512 ? 1 : 4;   // 512-thread tiles keep to 128 VGPRs without spilling
+        constexpr int PC = NT >= 384 ? 1 : 4;   // large tiles keep to 128 VGPRs without spilling
 #pragma unroll
         for (int c0 = 0; c0 < LOADS; c0 += PC) {
             int a0[PC], a1[PC], b0[PC], b1[PC];
@@ -1492,7 +1496,10 @@ bool tail_allow_lds(size_t bytes)
 
 // Tile variants (interior cells per workgroup).  The fused sweeps compute every interior cell exactly as
 // global sweeps would, whatever the tile, so the choice changes time only; small levels use small
-// tiles to spread over more compute units.
+// tiles to spread over more compute units.  Variants 6-8 size the workgroup to the tile's red-black
+// slots (32 x 32: 1104 slots on 384 threads; 64 x 32 and 32 x 64: 1840 and 1872 on 640), three
+// rounds each with at most 4% of the lanes idle, against 28% for 32 x 32 on 512 threads; 9 runs the
+// 64 x 32 and 32 x 64 slots in two rounds of 960 threads.
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
     int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant)
 {
@@ -1520,6 +1527,11 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
             FS_RB_MODES(64, 32, 0, 512);
         break;
     case 5: FS_RB_MODES(32, 32, 0, 512); break;
+    case 6: FS_RB_MODES(32, 32, 0, 384); break;
+    case 7: FS_RB_MODES(64, 32, 0, 640); break;
+    case 8: FS_RB_MODES(32, 64, 0, 640); break;
+    case 9: FS_RB_MODES(64, 32, 0, 960); break;
+    case 10: FS_RB_MODES(32, 64, 0, 960); break;
     default:
         if (fine)
             FS_RB_MODES(64, 32, 1, 256);
@@ -1620,6 +1632,11 @@ void launch_rb_multi_t(hipStream_t st, const LevelViewT<C, V>& L, int cur, const
     case 3: FS_RBT_MODES(16, 16, 256); break;
     case 4: FS_RBT_MODES(64, 32, 512); break;
     case 5: FS_RBT_MODES(32, 32, 512); break;
+    case 6: FS_RBT_MODES(32, 32, 384); break;
+    case 7: FS_RBT_MODES(64, 32, 640); break;
+    case 8: FS_RBT_MODES(32, 64, 640); break;
+    case 9: FS_RBT_MODES(64, 32, 960); break;
+    case 10: FS_RBT_MODES(32, 64, 960); break;
     default: FS_RBT_MODES(64, 32, 256); break;
     }
 #undef FS_RBT_MODES

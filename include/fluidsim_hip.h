@@ -250,7 +250,8 @@ fs_status fs_mg_level_prolong(fs_grid* grid, int level, int axis, int* first, in
 fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, int* index, double* weight);
 /* Autotuned kernel tiles of a level (build's own): pre- and post-smoothing variants of the fused
  * four-sweep kernel (0 = 64x32/256 threads, 1 = 32x32/256, 2 = 32x16/256, 3 = 16x16/256, 4 = 64x32/512,
- * 5 = 32x32/512) and the residual-restriction variant (0 = 32x16, 1 = 16x16, 2 = 16x8, 3 = 8x8 coarse
+ * 5 = 32x32/512, 6 = 32x32/384, 7 = 64x32/640, 8 = 32x64/640, 9 = 64x32/960, 10 = 32x64/960) and the
+ * residual-restriction variant (0 = 32x16, 1 = 16x16, 2 = 16x8, 3 = 8x8 coarse
  * cells); -1 for levels solved inside the single-workgroup tail. */
 fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction);
 /* Overrides the autotuned variants of a tiled level (-1 keeps one): every variant computes the same
