@@ -265,13 +265,15 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
     return out
 
 
-def batch_leg(count, seed=4):
+def batch_leg(count, seed=4, splits=(0,)):
     """Config C4 on one GPU: `count` seeded instances (sides log-uniform 64..512, 30-90% liquid plus a
     drop, vortex velocity, obstacles over 0-25% of the cells), each projected once to 1e-6 of its own initial max divergence.  Timed twice
     from the same device-resident inputs: one fs_project per grid in turn (the plain GPU MGPCG), then one
     fs_batch_project over all of them (one persistent launch for all PCG iterations).  Each instance's
     implicit viscosity solve (nu log-uniform 1e-3..1) is timed the same two ways: fs_apply_viscosity one
-    grid after another, then fs_batch_apply_viscosity over all of them."""
+    grid after another, then fs_batch_apply_viscosity over all of them.  `splits` are the
+    fs_context_set_batch_split values timed for the projections (instances above the split run the
+    multi-kernel solve beside the persistent launch); the first one is the headline batched number."""
     from fluidsim_amd import (FluidGridHIP, FluidMixture, MixtureComponent, batch_apply_viscosity, batch_project,
                               FS_TRANSPORT_NONE)
     from fluidsim_amd import workloads as WL
@@ -319,29 +321,42 @@ def batch_leg(count, seed=4):
     serial = [g.project(dt, 2000, tau) for g, tau in zip(grids, taus)]
     serial_s = time.perf_counter() - t0
     note(f"serial {serial_s * 1e3:.1f} ms")
-    restore()
-    t0 = time.perf_counter()
-    batched = batch_project(grids, dt, 2000, taus)
-    batch_s = time.perf_counter() - t0
-    note(f"batched {batch_s * 1e3:.1f} ms")
-    ok = all(r.converged for r in serial) and all(r.converged for r in batched)
+    ctx = grids[0].context
+    by_split = {}
+    for split in splits:
+        ctx.set_batch_split(split)
+        restore()
+        batch_project(grids, dt, 2000, taus)
+        restore()
+        t0 = time.perf_counter()
+        res = batch_project(grids, dt, 2000, taus)
+        by_split[split] = (time.perf_counter() - t0, res)
+        note(f"batched (split {split} cells) {by_split[split][0] * 1e3:.1f} ms")
+    ctx.set_batch_split(0)
+    batch_s, batched = by_split[splits[0]]
+    ok = all(r.converged for r in serial) and all(all(r.converged for r in v[1]) for v in by_split.values())
     # viscosity: every instance, one by one and then batched (warm-up of both first)
     restore()
     for g in grids:
         g.applyViscosity(dt)
     restore()
-    batch_apply_viscosity(grids, dt)
-    restore()
     t0 = time.perf_counter()
     vits = [g.applyViscosity(dt) for g in grids]
     visc_s = time.perf_counter() - t0
     note(f"viscosity one by one {visc_s * 1e3:.1f} ms")
-    restore()
-    t0 = time.perf_counter()
-    bvits = batch_apply_viscosity(grids, dt)
-    grids[0].context.synchronize()
-    visc_batch_s = time.perf_counter() - t0
-    note(f"viscosity batched {visc_batch_s * 1e3:.1f} ms")
+    visc_by_split = {}
+    for split in splits:
+        ctx.set_batch_split(split)
+        restore()
+        batch_apply_viscosity(grids, dt)
+        restore()
+        t0 = time.perf_counter()
+        res = batch_apply_viscosity(grids, dt)
+        ctx.synchronize()
+        visc_by_split[split] = (time.perf_counter() - t0, res)
+        note(f"viscosity batched (split {split} cells) {visc_by_split[split][0] * 1e3:.1f} ms")
+    ctx.set_batch_split(0)
+    visc_batch_s, bvits = visc_by_split[splits[0]]
     for k, g in enumerate(grids):
         g.destroy()
         if (k + 1) % 32 == 0:
@@ -357,12 +372,15 @@ def batch_leg(count, seed=4):
         "batched_solves_per_s": count / batch_s,
         "speedup": serial_s / batch_s,
         "batched_ms": batch_s * 1e3,
+        "batch_split_cells": int(splits[0]),
+        "batched_ms_by_split": {str(k): v[0] * 1e3 for k, v in by_split.items()},
         "converged_all": bool(ok),
         "pcg_iterations": {"serial_mean": float(np.mean([r.iterations for r in serial])),
                            "batched_mean": float(np.mean([r.iterations for r in batched]))},
         "viscosity_solves_per_s_serial": count / visc_s,
         "viscosity_solves_per_s_batched": count / visc_batch_s,
         "viscosity_speedup": visc_s / visc_batch_s,
+        "viscosity_batched_ms_by_split": {str(k): v[0] * 1e3 for k, v in visc_by_split.items()},
         "viscosity_iterations": {"serial_mean": float(np.mean(vits)), "batched_mean": float(np.mean(bvits))},
         "projection_plus_viscosity_speedup": (serial_s + visc_s) / (batch_s + visc_batch_s),
     }
@@ -387,12 +405,15 @@ def main():
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=256, help="size of the C4 batched-solve leg (0: skip)")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--batch-splits", type=str, default="131072,0",
+                    help="fs_context_set_batch_split values to time (cells; the first is reported as batched)")
     ap.add_argument("--precision-size", type=int, default=2048, help="side of the C3 mixed-precision leg (0: skip)")
     ap.add_argument("--precision-steps", type=int, default=3)
     ap.add_argument("--precision-only", action="store_true")
     ap.add_argument("--sl-steps", type=int, default=200, help="C2 semi-Lagrangian steps leg (0: skip)")
     ap.add_argument("--precision-modes", type=str, default="", help="subset of fp32,fp16,bf16,bf16_vectors")
     args = ap.parse_args()
+    splits = tuple(int(v) for v in args.batch_splits.split(",") if v.strip())
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -405,7 +426,7 @@ def main():
 
     rates = [float(x) for x in args.serving_rates.split(",") if x]
     if args.batch_only:
-        print(json.dumps(batch_leg(args.batch_instances or 256)), flush=True)
+        print(json.dumps(batch_leg(args.batch_instances or 256, splits=splits)), flush=True)
         return
     if args.precision_only:
         only = [m for m in args.precision_modes.split(",") if m] or None
@@ -507,7 +528,7 @@ def main():
         out["mixed_precision"] = precision_leg(args.precision_size, steps=args.precision_steps)
     if args.batch_instances > 0:
         grid.destroy()
-        out["batched"] = batch_leg(args.batch_instances)
+        out["batched"] = batch_leg(args.batch_instances, splits=splits)
     if args.serving_requests > 0:
         grid.destroy()
         out["serving"] = serving_leg(args.serving_requests, rates, args.serving_steps,

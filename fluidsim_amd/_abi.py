@@ -202,6 +202,7 @@ SIGNATURES = {
     "fs_checkpoint_load": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_advance_frame": [_vp, C.POINTER(FsFrameSettings), C.POINTER(FsFrameStatistics)],
     "fs_batch_project": [C.c_int, C.POINTER(_vp), C.c_double, C.c_int, _dp, C.POINTER(FsProjectionResult)],
+    "fs_context_set_batch_split": [_vp, C.c_longlong],
     "fs_grid_estimate_bytes": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)],
     "fs_server_create": [C.POINTER(FsServerConfig), C.POINTER(_vp)],
     "fs_server_destroy": [_vp],

@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <type_traits>
 #include <memory>
@@ -237,11 +238,35 @@ struct fs_context {
     DevBuf<unsigned char> vis_batch_desc;   // descriptors of the last batched viscosity solve
     PcgScalars* batch_pinned = nullptr;     // host-pinned landing slots of a batch's scalar blocks
     size_t batch_pinned_n = 0;
+    long long batch_solo_cells = 0;         // fs_context_set_batch_split (0: every instance batched)
+    hipStream_t batch_stream = nullptr;     // the persistent launch's stream while solo solves run
+    hipEvent_t batch_in = nullptr, batch_out = nullptr;
 };
 
 constexpr int kPressureBudget = 100000;   // PCG iterations of an unbounded (negative) budget
 constexpr size_t kFoldBlocks = 1536;      // cell blocks (64 x 16) beyond which PCG partials are pre-reduced
 constexpr size_t kBatchSpreadLds = 84 * 1024;   // LDS of a batched-solve workgroup: one per compute unit
+
+// A batched solve split by fs_context_set_batch_split: grids above the split take their single-grid solve
+// on the context's stream while the persistent launch runs on a stream of its own.
+bool batch_solo(const fs_context* ctx, int W, int H)
+{
+    return ctx->aux && ctx->batch_solo_cells > 0 && (long long)W * H > ctx->batch_solo_cells;
+}
+
+// The persistent launch's stream of a split batch, ordered after the work already on the context's stream.
+hipStream_t batch_side_stream(fs_context* ctx)
+{
+    if (!ctx->batch_stream) {
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
+        HIPC(hipStreamCreate(&ctx->batch_stream));
+        HIPC(hipEventCreateWithFlags(&ctx->batch_in, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&ctx->batch_out, hipEventDisableTiming));
+    }
+    HIPC(hipEventRecord(ctx->batch_in, ctx->stream));
+    HIPC(hipStreamWaitEvent(ctx->batch_stream, ctx->batch_in, 0));
+    return ctx->batch_stream;
+}
 
 struct fs_grid {
     fs_context* ctx = nullptr;
@@ -1401,6 +1426,14 @@ fs_status fs_context_destroy(fs_context* ctx)
                 (void)hipStreamDestroy(ctx->stream);
             if (ctx->aux)
                 (void)hipStreamDestroy(ctx->aux);
+            if (ctx->batch_stream) {
+                (void)hipStreamSynchronize(ctx->batch_stream);
+                (void)hipStreamDestroy(ctx->batch_stream);
+            }
+            if (ctx->batch_in)
+                (void)hipEventDestroy(ctx->batch_in);
+            if (ctx->batch_out)
+                (void)hipEventDestroy(ctx->batch_out);
             if (ctx->fork)
                 (void)hipEventDestroy(ctx->fork);
             if (ctx->join)
@@ -1967,11 +2000,15 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
         hipStream_t st = ctx->stream;
         const size_t bytes = viscosity_batch_desc_bytes();
         std::vector<unsigned char> descs;
-        std::vector<int> batched, alone;
+        std::vector<int> batched, alone, solo;
         for (int i = 0; i < count; ++i) {
             fs_grid& g = *grids[i];
             if (dt <= 0.0 || !g.has_viscosity())
                 continue;
+            if (batch_solo(ctx, g.W, g.H)) {
+                solo.push_back(i);
+                continue;
+            }
             double* material = blend_viscosity_material(g);
             descs.resize((batched.size() + 1) * bytes);
             if (viscosity_batch_prepare(g.visc, st, g.u.p, g.v.p, material, dt, g.h, 1e-4, kViscosityBudget,
@@ -1986,11 +2023,13 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
                 ctx->vis_batch_desc.alloc(need_bytes);
             HIPC(hipMemcpyAsync(ctx->vis_batch_desc.p, descs.data(), need_bytes, hipMemcpyHostToDevice, st));
             const bool spread = (int)batched.size() <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds;
-            launch_batch_viscosity(st, ctx->vis_batch_desc.p, (int)batched.size(), spread ? kBatchSpreadLds : 0);
+            const hipStream_t bs = solo.empty() ? st : batch_side_stream(ctx);
+            launch_batch_viscosity(bs, ctx->vis_batch_desc.p, (int)batched.size(), spread ? kBatchSpreadLds : 0);
             for (int i : batched)
-                viscosity_batch_fetch(grids[i]->visc, st);
+                viscosity_batch_fetch(grids[i]->visc, bs);
+            if (!solo.empty())
+                HIPC(hipEventRecord(ctx->batch_out, bs));
         }
-        HIPC(hipStreamSynchronize(st));
         std::string first_error;
         auto attempt = [&](auto&& body) {
             try {
@@ -2000,6 +2039,16 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
                     first_error = e.what();
             }
         };
+        // grids above the batch split: their own solve, beside the persistent launch
+        for (int i : solo)
+            attempt([&] {
+                const int it = apply_viscosity(*grids[i], dt);
+                if (iterations)
+                    iterations[i] = it;
+            });
+        if (!solo.empty() && !batched.empty())
+            HIPC(hipStreamWaitEvent(st, ctx->batch_out, 0));
+        HIPC(hipStreamSynchronize(st));
         for (int i : batched) {
             int it = 0;
             double relative = 0.0;
@@ -2615,46 +2664,86 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             return;
         set_device(ctx);
         hipStream_t st = ctx->stream;
-        std::vector<BatchInstance> desc((size_t)count);
-        size_t lds = 0;
-        for (int i = 0; i < count; ++i) {
-            fs_grid& g = *grids[i];
-            g.project_begin(dt, fs_grid::budget(max_iterations), divergence_tolerances[i]);
-            desc[(size_t)i] = g.batch_instance();
-            if (g.precond == FS_PRECOND_MULTIGRID)
-                lds = std::max(lds, g.tail_lds);
-        }
-        if (ctx->batch_desc.n < (size_t)count)
-            ctx->batch_desc.alloc((size_t)count);
-        HIPC(hipMemcpyAsync(ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance),
-            hipMemcpyHostToDevice, st));
-        // One workgroup is one instance's whole solve: while the instances fit on the compute units one
-        // each, reserve more than half a unit's 160 KB of LDS so that no two of them share a unit.
-        if (count <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds)
-            lds = std::max(lds, kBatchSpreadLds);
-        if (lds > 65536)
-            need(batch_allow_lds(lds), FS_E_DEVICE, "the device refused the batched solve's LDS request");
-        launch_batch_pcg(st, ctx->batch_desc.p, count, lds);
-        HIPC(hipGetLastError());
+        // Instances above the split run fs_project's multi-kernel solve on the context's stream while the
+        // persistent launch runs the others on a stream of its own; the route depends on the instance's
+        // size only, so each result is the same alone and in any batch.
+        std::vector<int> batched, solo;
         for (int i = 0; i < count; ++i)
-            grids[i]->project_end(dt);
-        if (ctx->batch_pinned_n < (size_t)count) {
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            if (ctx->batch_pinned)
-                (void)hipHostFree(ctx->batch_pinned);
-            ctx->batch_pinned = nullptr;
-            ctx->batch_pinned_n = 0;
-            HIPC(hipHostMalloc((void**)&ctx->batch_pinned, (size_t)count * sizeof(PcgScalars), hipHostMallocDefault));
-            ctx->batch_pinned_n = (size_t)count;
+            (batch_solo(ctx, grids[i]->W, grids[i]->H) ? solo : batched).push_back(i);
+        const int nb = (int)batched.size();
+        const bool split = nb > 0 && !solo.empty();
+        if (nb > 0) {
+            std::vector<BatchInstance> desc((size_t)nb);
+            size_t lds = 0;
+            for (int k = 0; k < nb; ++k) {
+                fs_grid& g = *grids[batched[(size_t)k]];
+                g.project_begin(dt, fs_grid::budget(max_iterations), divergence_tolerances[batched[(size_t)k]]);
+                desc[(size_t)k] = g.batch_instance();
+                if (g.precond == FS_PRECOND_MULTIGRID)
+                    lds = std::max(lds, g.tail_lds);
+            }
+            if (ctx->batch_desc.n < (size_t)nb)
+                ctx->batch_desc.alloc((size_t)nb);
+            HIPC(hipMemcpyAsync(ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance),
+                hipMemcpyHostToDevice, st));
+            // One workgroup is one instance's whole solve: while the instances fit on the compute units one
+            // each, reserve more than half a unit's 160 KB of LDS so that no two of them share a unit.
+            if (nb <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds)
+                lds = std::max(lds, kBatchSpreadLds);
+            if (lds > 65536)
+                need(batch_allow_lds(lds), FS_E_DEVICE, "the device refused the batched solve's LDS request");
+            const hipStream_t bs = split ? batch_side_stream(ctx) : st;
+            launch_batch_pcg(bs, ctx->batch_desc.p, nb, lds);
+            HIPC(hipGetLastError());
+            if (split)
+                HIPC(hipEventRecord(ctx->batch_out, bs));
         }
-        for (int i = 0; i < count; ++i)
-            HIPC(hipMemcpyAsync(ctx->batch_pinned + i, grids[i]->S.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, st));
-        HIPC(hipStreamSynchronize(st));
-        for (int i = 0; i < count; ++i) {
-            const PcgScalars sc = ctx->batch_pinned[i];
-            results[i] = grids[i]->project_result(sc, divergence_tolerances[i]);
-            grids[i]->predicted = std::max(1, sc.iterations);
+        // the solo instances; a failure is reported after the batched instances have been finished
+        std::exception_ptr solo_error;
+        for (int i : solo) {
+            try {
+                results[i] = grids[i]->project(dt, max_iterations, divergence_tolerances[i]);
+            } catch (...) {
+                if (!solo_error)
+                    solo_error = std::current_exception();
+            }
         }
+        if (nb > 0) {
+            if (split)
+                HIPC(hipStreamWaitEvent(st, ctx->batch_out, 0));
+            for (int i : batched)
+                grids[i]->project_end(dt);
+            if (ctx->batch_pinned_n < (size_t)nb) {
+                std::lock_guard<std::mutex> lock(device_setup_mutex());
+                if (ctx->batch_pinned)
+                    (void)hipHostFree(ctx->batch_pinned);
+                ctx->batch_pinned = nullptr;
+                ctx->batch_pinned_n = 0;
+                HIPC(hipHostMalloc((void**)&ctx->batch_pinned, (size_t)nb * sizeof(PcgScalars), hipHostMallocDefault));
+                ctx->batch_pinned_n = (size_t)nb;
+            }
+            for (int k = 0; k < nb; ++k)
+                HIPC(hipMemcpyAsync(ctx->batch_pinned + k, grids[batched[(size_t)k]]->S.p, sizeof(PcgScalars),
+                    hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            for (int k = 0; k < nb; ++k) {
+                const int i = batched[(size_t)k];
+                const PcgScalars sc = ctx->batch_pinned[k];
+                results[i] = grids[i]->project_result(sc, divergence_tolerances[i]);
+                grids[i]->predicted = std::max(1, sc.iterations);
+            }
+        }
+        if (solo_error)
+            std::rethrow_exception(solo_error);
+    });
+}
+
+fs_status fs_context_set_batch_split(fs_context* ctx, long long solo_cells)
+{
+    return guard([&] {
+        need(ctx != nullptr, FS_E_INVALID_ARGUMENT, "null context");
+        need(solo_cells >= 0, FS_E_INVALID_ARGUMENT, "the split must be a non-negative cell count");
+        ctx->batch_solo_cells = solo_cells;
     });
 }
 

@@ -131,6 +131,11 @@ class Context:
     def synchronize(self):
         call("fs_context_synchronize", self._h)
 
+    def set_batch_split(self, solo_cells: int):
+        """fs_context_set_batch_split: batch_project sends grids of more than `solo_cells` cells through
+        the multi-kernel solve, concurrently with the persistent launch of the rest (0: batch all)."""
+        call("fs_context_set_batch_split", self._h, int(solo_cells))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().fs_context_destroy(self._h)

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(17, 13), (1, 129), (131, 1), (64, 64), (65, 37), (128, 96), (200, 150), (300, 200), (2, 67), (96, 257)]
 
 
-def pool_grid(W, H, seed, precond=None):
+def pool_grid(W, H, seed, precond=None, context=None):
     rng = WL.SplitMix64(seed)
     h = 1.0 / max(W, H)
     f = WL.pool_with_drop(W, H, h, 0.3 + 0.6 * rng.uniform(1)[0], 0.1, rng)
@@ -29,7 +29,7 @@ def pool_grid(W, H, seed, precond=None):
                         rng.uniform(1, 0.02, 0.08)[0], 0.5 * rng.normal(1)[0])
     WL.zero_walls(u, v, W, H)
     lm, gm = WL.two_phase_masses(f, h)
-    g = FluidGridHIP(W, H, h)
+    g = FluidGridHIP(W, H, h, context=context)
     g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0)]), FluidMixture([MixtureComponent("air", 0.1)]))
     g.setGravityY(9.81)
     g.setTransport(FS_TRANSPORT_NONE)
@@ -93,6 +93,38 @@ def test_alone_equals_in_any_batch():
         assert np.array_equal(getattr(s, k), getattr(ref_state, k)), k
     for g in others + [probe, again]:
         g.destroy()
+
+
+def test_split_batch_runs_large_grids_through_the_single_grid_solve():
+    """fs_context_set_batch_split: grids above the split get fs_project's own result bit for bit (the
+    multi-kernel solve, run beside the persistent launch), the others the persistent launch's result
+    bit for bit -- each grid's route depends only on its size."""
+    ctx = Context(0)
+    split = 10000
+    grids = [pool_grid(W, H, 300 + k, context=ctx)[0] for k, (W, H) in enumerate(SHAPES)]
+    taus = [tau_of(g) for g in grids]
+    large = [W * H > split for W, H in SHAPES]
+    assert 0 < sum(large) < len(SHAPES)
+    ctx.set_batch_split(split)
+    res = batch_project(grids, 0.002, 2000, taus)
+    ctx.set_batch_split(0)
+    for k, ((W, H), g, r, tau) in enumerate(zip(SHAPES, grids, res, taus)):
+        ref_grid, _ = pool_grid(W, H, 300 + k, context=ctx)
+        if large[k]:
+            rr = ref_grid.project(0.002, 2000, tau)
+        else:
+            rr = batch_project([ref_grid], 0.002, 2000, tau)[0]
+        assert r.converged and r.max_divergence <= tau, (W, H, r)
+        assert (r.iterations, r.max_divergence) == (rr.iterations, rr.max_divergence), (W, H, r, rr)
+        s, sr = g.download(), ref_grid.download()
+        for name in ("u", "v", "pressure"):
+            assert np.array_equal(getattr(s, name), getattr(sr, name)), (W, H, name)
+        ref_grid.destroy()
+    with pytest.raises(ValueError):
+        ctx.set_batch_split(-1)
+    for g in grids:
+        g.destroy()
+    ctx.close()
 
 
 def test_diagonal_and_multigrid_grids_mix():
@@ -184,7 +216,7 @@ def test_instances_with_obstacles_batch_like_single_grids():
         g.destroy()
 
 
-def viscous_grid(W, H, seed, nu, mask=None):
+def viscous_grid(W, H, seed, nu, mask=None, context=None):
     rng = WL.SplitMix64(seed)
     h = 1.0 / max(W, H)
     f = WL.pool_with_drop(W, H, h, 0.3 + 0.6 * rng.uniform(1)[0], 0.1, rng)
@@ -195,7 +227,7 @@ def viscous_grid(W, H, seed, nu, mask=None):
                         rng.uniform(1, 0.02, 0.08)[0], 0.5 * rng.normal(1)[0])
     WL.zero_walls(u, v, W, H)
     lm, gm = WL.two_phase_masses(f, h)
-    g = FluidGridHIP(W, H, h)
+    g = FluidGridHIP(W, H, h, context=context)
     g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0, nu * 100.0)]),
                          FluidMixture([MixtureComponent("air", 0.1, nu * 0.1)]))
     g.upload(FluidGridState(f, u, v, np.zeros(0), [lm], [gm]))
@@ -237,3 +269,30 @@ def test_batched_viscosity_matches_single_solves():
     assert np.array_equal(alone.download().v, grids[5].download().v)
     for g in grids + singles + [inviscid, alone]:
         g.destroy()
+
+
+def test_split_batched_viscosity_runs_large_grids_through_the_single_grid_solve():
+    """With fs_context_set_batch_split, grids above the split get fs_apply_viscosity's result bit for bit
+    (their own solve, beside the persistent launch) and the others the persistent launch's result bit
+    for bit."""
+    from fluidsim_amd import batch_apply_viscosity
+
+    dt = 0.01
+    ctx = Context(0)
+    split = 10000
+    grids = [viscous_grid(W, H, 60 + k, nu, context=ctx) for k, (W, H, nu) in enumerate(VISC_SHAPES)]
+    large = [W * H > split for W, H, _ in VISC_SHAPES]
+    assert 0 < sum(large) < len(VISC_SHAPES)
+    ctx.set_batch_split(split)
+    its = batch_apply_viscosity(grids, dt)
+    ctx.set_batch_split(0)
+    for k, ((W, H, nu), g) in enumerate(zip(VISC_SHAPES, grids)):
+        ref = viscous_grid(W, H, 60 + k, nu, context=ctx)
+        n = ref.applyViscosity(dt) if large[k] else batch_apply_viscosity([ref], dt)[0]
+        assert its[k] == n, (W, H, its[k], n)
+        a, b = g.download(), ref.download()
+        assert np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v), (W, H)
+        ref.destroy()
+    for g in grids:
+        g.destroy()
+    ctx.close()
