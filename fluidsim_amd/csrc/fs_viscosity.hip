@@ -909,6 +909,40 @@ __device__ void bv_cycle(const BatchVisDesc& D, float dt, float2* vel, float4* c
     }
 }
 
+// The coarse end of a V-cycle in one workgroup: levels whose sweeps are too small to fill the device cost
+// more in launches than in work, so cycle() hands every level from the first with at most kVisTailSlots
+// slots down to the coarsest to this kernel (the same sweeps, restriction, Jacobi coarse solve and
+// prolongation as the per-level kernels, each a workgroup-strided loop).  b0 is the first level's rhs;
+// the result lands in lev[0].x.
+constexpr int kVisTailSlots = 4096;
+
+struct VisTail {
+    int levels;
+    VLev lev[kMaxVisLevels];
+};
+
+__global__ __launch_bounds__(kBVT) void k_vis_tail(const VisTail T, const float2* b0, const VisScalars* S)
+{
+    __shared__ float2 vel[kCoarseSlots];
+    __shared__ float4 cof[kCoarseSlots];
+    if (stopped(S))
+        return;
+    const float dt = S->dt;
+    const int nl = T.levels;
+    for (int l = 0; l + 1 < nl; ++l) {
+        const float2* b = l == 0 ? b0 : T.lev[l].b;
+        bv_pre(T.lev[l], b, dt);
+        bv_residual_restrict(T.lev[l], b, T.lev[l + 1], dt);
+    }
+    bv_coarse(T.lev[nl - 1], nl == 1 ? b0 : T.lev[nl - 1].b, dt, vel, cof);
+    for (int l = nl - 2; l >= 0; --l) {
+        const float2* b = l == 0 ? b0 : T.lev[l].b;
+        bv_prolong(T.lev[l], T.lev[l + 1]);
+        bv_sweep(T.lev[l], b, T.lev[l].x, T.lev[l].xn, dt);
+        bv_sweep(T.lev[l], b, T.lev[l].xn, T.lev[l].x, dt);
+    }
+}
+
 __global__ __launch_bounds__(kBVT) void k_batch_vis(const BatchVisDesc* descs)
 {
     __shared__ double lds[kBVT / 64];
@@ -1160,11 +1194,31 @@ struct ViscositySolver {
 
     // cycle (ViscositySolverGPU.cpp:134-167); the result is levels[l]->x.  Every level swaps its x / xn
     // pair once per cycle, so four cycles leave the pointers where they started.
+    // First level of the one-workgroup tail (k_vis_tail): the first level below the finest with at most
+    // kVisTailSlots slots, when at least one level lies between it and the coarsest; else none (-1).
+    int tail_level() const
+    {
+        const int nl = (int)levels.size();
+        for (int l = 1; l + 1 < nl; ++l)
+            if ((size_t)(levels[l]->w + 1) * (levels[l]->h + 1) <= (size_t)kVisTailSlots)
+                return nl - l <= kMaxVisLevels ? l : -1;
+        return -1;
+    }
+
     void cycle(hipStream_t st, int l, const float2* b)
     {
         VLev L = view(l);
         if (l + 1 == (int)levels.size()) {
             hipLaunchKernelGGL(k_vis_coarse, dim3(1), dim3(kCoarseSlots), 0, st, L, b, S.p);
+            return;
+        }
+        if (l == tail_level()) {
+            // the tail leaves each of its levels' results in x: no swaps there
+            VisTail T{};
+            T.levels = (int)levels.size() - l;
+            for (int k = 0; k < T.levels; ++k)
+                T.lev[k] = view(l + k);
+            hipLaunchKernelGGL(k_vis_tail, dim3(1), dim3(kBVT), 0, st, T, b, S.p);
             return;
         }
         const dim3 g = slot_grid(L.w, L.h);
