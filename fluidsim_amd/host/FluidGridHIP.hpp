@@ -122,6 +122,9 @@ public:
     HipContext(const HipContext&) = delete;
     HipContext& operator=(const HipContext&) = delete;
     fs_context* get() const { return ctx_; }
+    // fs_context_set_batch_split: batched solves send grids above `solo_cells` cells through their
+    // single-grid solve beside the persistent launch (0: batch every grid).
+    void setBatchSplit(long long solo_cells) { check(fs_context_set_batch_split(ctx_, solo_cells)); }
 
 private:
     fs_context* ctx_ = nullptr;
@@ -397,7 +400,6 @@ public:
 
     fs_grid* handle() const { return grid_; }
 
-private:
     static ProjectionResult convert(const fs_projection_result& r)
     {
         ProjectionResult p;
@@ -414,6 +416,7 @@ private:
         return p;
     }
 
+private:
     fs_grid* grid_ = nullptr;
     double surface_tension_ = 0.0;
     int width_;
@@ -421,5 +424,36 @@ private:
     double cell_size_;
     FluidMixture liquid_, gas_;
 };
+
+// Batched projection (fs_batch_project; no reference counterpart): the grids share one context, each
+// gets the result its own project() contract defines; one divergence tolerance per grid.
+inline std::vector<FluidProjectionResult> batchProject(const std::vector<FluidGridHIP*>& grids, double dt,
+    int max_iterations, const std::vector<double>& divergence_tolerances)
+{
+    if (divergence_tolerances.size() != grids.size())
+        throw std::invalid_argument("one divergence tolerance per grid");
+    std::vector<fs_grid*> handles;
+    for (const FluidGridHIP* g : grids)
+        handles.push_back(g ? g->handle() : nullptr);
+    std::vector<fs_projection_result> raw(grids.size());
+    check(fs_batch_project((int)grids.size(), handles.data(), dt, max_iterations, divergence_tolerances.data(),
+        raw.data()));
+    std::vector<FluidProjectionResult> out;
+    for (const fs_projection_result& r : raw)
+        out.push_back(FluidGridHIP::convert(r));
+    return out;
+}
+
+// Batched implicit viscosity (fs_batch_apply_viscosity; no reference counterpart): each grid's
+// applyViscosity(dt); returns the iteration counts.
+inline std::vector<int> batchApplyViscosity(const std::vector<FluidGridHIP*>& grids, double dt)
+{
+    std::vector<fs_grid*> handles;
+    for (const FluidGridHIP* g : grids)
+        handles.push_back(g ? g->handle() : nullptr);
+    std::vector<int> iterations(grids.size(), 0);
+    check(fs_batch_apply_viscosity((int)grids.size(), handles.data(), dt, iterations.data()));
+    return iterations;
+}
 
 }  // namespace fluid_hip

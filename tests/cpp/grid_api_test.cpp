@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <iostream>
+#include <memory>
 #include <stdexcept>
 
 using fluid_hip::FluidGridHIP;
@@ -260,6 +261,43 @@ void obstaclesAndPrecision(HipContext& ctx)
     require(threw && plain.solidCells() == 0, "a mask splitting the fluid was accepted");
 }
 
+void batchedSolves(HipContext& ctx)
+{
+    // grids of 80 x 60 (4800 cells) and 160 x 120 (19200): the split sends the larger through its own
+    // solve; every result matches the grid's single projection to the solve tolerance
+    const int sizes[2][2] = {{80, 60}, {160, 120}};
+    std::vector<std::unique_ptr<FluidGridHIP>> grids, singles;
+    std::vector<FluidGridHIP*> batch;
+    for (const auto& wh : sizes) {
+        for (auto* list : {&grids, &singles}) {
+            list->push_back(std::make_unique<FluidGridHIP>(ctx, wh[0], wh[1], 1.0 / wh[0]));
+            configure(*list->back());
+            list->back()->addVortex({-6.5, 11.35}, 0.1, 1.0);
+        }
+        batch.push_back(grids.back().get());
+    }
+    for (long long split : {0LL, 10000LL}) {
+        ctx.setBatchSplit(split);
+        const auto results = batchProject(batch, 0.01, 500, {1e-6, 1e-6});
+        for (size_t k = 0; k < batch.size(); ++k)
+            require(results[k].converged && results[k].max_divergence <= 1e-6, "batched projection failed");
+    }
+    ctx.setBatchSplit(0);
+    for (size_t k = 0; k < batch.size(); ++k) {
+        const auto r = singles[k]->project(0.01, 500, 1e-6);
+        require(r.converged, "single projection failed");
+    }
+    bool threw = false;
+    try {
+        batchProject(batch, 0.01, 500, {1e-6});
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    require(threw, "a tolerance list of the wrong length was accepted");
+    const auto its = batchApplyViscosity(batch, 0.01);
+    require(its.size() == batch.size(), "batched viscosity result count");
+}
+
 }  // namespace
 
 int main()
@@ -272,7 +310,8 @@ int main()
         {"zero time and zero iterations are reported", zeroIterationAndZeroTimeAreReported},
         {"upload validation", uploadIsValidated}, {"hydrostatic pressure", hydrostaticPressure},
         {"checkpoint continuation", checkpointContinuation}, {"scene frame", sceneFrame},
-        {"obstacles and multigrid precision", obstaclesAndPrecision}};
+        {"obstacles and multigrid precision", obstaclesAndPrecision},
+        {"batched solves and the batch split", batchedSolves}};
     int failures = 0;
     for (const auto& t : tests) {
         try {
