@@ -416,8 +416,8 @@ def main():
     splits = tuple(int(v) for v in args.batch_splits.split(",") if v.strip())
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(multi-GPU sharding is design-only).", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the solver runs on a single GPU; multi-GPU sharding is a design, not code).", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 

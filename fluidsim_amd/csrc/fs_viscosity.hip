@@ -7,7 +7,7 @@
 // when the recomputed residual meets r M^-1 r <= tol^2 v0 M v0 + 1e-12 sum(M), checked every four
 // iterations; a recurrence that claims convergence before the recomputed residual restarts CG from it.
 //
-// This build's layout and numerics: every level stores (w+1) x (h+1) slots of float2 (u face left of
+// Layout and numerics: every level stores (w+1) x (h+1) slots of float2 (u face left of
 // cell p, v face on top of cell p) and float4 coefficients (cell mu, vertex mu, u mass, v mass); reads
 // clamp to the slot field like the reference's accessors.  The iterate x starts at v0 in FP32 (the
 // reference's formulation: iterating on v - v0 instead loses three digits to cancellation when dt K
