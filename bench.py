@@ -42,16 +42,20 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_rb_multi_fine_post_1024_traffic.md")
 
 
-def measured_traffic():
+def measured_traffic(kernel, size):
     """HBM bytes per launch of the fine post-smoother from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None when that summary covers
+    another kernel (the autotuner chose another tile) or another grid size."""
     try:
         with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+            lines = f.read().splitlines()
     except OSError:
-        pass
+        return None
+    if not lines or f"`{kernel}`" not in lines[0] or f"{size} x {size}" not in lines[0]:
+        return None
+    for line in lines:
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
     return None
 
 
@@ -475,6 +479,7 @@ def main():
                 9: (64, 32, 960), 10: (32, 64, 960)}
     post_variant = grid.multigridTiles(0)[1]
     tx, ty, nt = variants[post_variant]
+    fine_kernel = f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float>"
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
     sweep_bytes = SMOOTH_BYTES_PER_CELL * wl.width * wl.height
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
@@ -507,14 +512,13 @@ def main():
         "converged_all": bool(ok),
         "max_rel_residual": float(max(rel)),
         "roofline": {
-            "kernel": f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float> "
-                      "(finest-level prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
+            "kernel": fine_kernel + " (finest-level prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
+            "traffic": measured_traffic(fine_kernel, args.size) if wl.width == wl.height else None,
             "avg_launch_us": sweep_ms * 1e3,
             "algorithmic_bytes_per_launch": sweep_bytes,
             "copy_kernel_gbs": copy_bytes / (copy_ms * 1e-3) / 1e9,
