@@ -518,7 +518,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(fine_kernel, args.size) if wl.width == wl.height else None,
+            "traffic": measured_traffic(fine_kernel, args.size),
             "avg_launch_us": sweep_ms * 1e3,
             "algorithmic_bytes_per_launch": sweep_bytes,
             "copy_kernel_gbs": copy_bytes / (copy_ms * 1e-3) / 1e9,
