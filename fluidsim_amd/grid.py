@@ -116,11 +116,16 @@ class FluidMixture:
 # ---- devices ----
 
 class Context:
-    """One HIP device and one stream (fs_context)."""
+    """One HIP device and its streams (fs_context): the work stream plus a helper stream for hierarchy
+    rebuilds, or the work stream alone with single_stream=True (fs_context_create_ex,
+    FS_CONTEXT_SINGLE_STREAM)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, single_stream: bool = False):
         h = C.c_void_p()
-        call("fs_context_create", int(device), C.byref(h))
+        if single_stream:
+            call("fs_context_create_ex", int(device), 1, C.byref(h))
+        else:
+            call("fs_context_create", int(device), C.byref(h))
         self._h = h
         self.device = device
 

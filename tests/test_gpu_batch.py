@@ -127,6 +127,25 @@ def test_split_batch_runs_large_grids_through_the_single_grid_solve():
     ctx.close()
 
 
+def test_single_stream_context_batches_every_grid():
+    """A context without the helper stream (FS_CONTEXT_SINGLE_STREAM) ignores the batch split: a grid above
+    it gets the persistent launch's result, bit for bit the one it gets alone in an unsplit batch."""
+    ctx = Context(0, single_stream=True)
+    W, H = SHAPES[6]                                  # 200 x 150 = 30000 cells, above the split below
+    a, _ = pool_grid(W, H, 500, context=ctx)
+    b, _ = pool_grid(W, H, 500, context=ctx)
+    small = [pool_grid(w, h, 600 + k, context=ctx)[0] for k, (w, h) in enumerate(SHAPES[:3])]
+    ctx.set_batch_split(10000)
+    r = batch_project(small + [a], 0.002, 2000, [tau_of(g) for g in small + [a]])[-1]
+    ctx.set_batch_split(0)
+    rb = batch_project([b], 0.002, 2000, tau_of(b))[0]
+    assert r.converged and (r.iterations, r.max_divergence) == (rb.iterations, rb.max_divergence)
+    assert np.array_equal(a.download().pressure, b.download().pressure)
+    for g in small + [a, b]:
+        g.destroy()
+    ctx.close()
+
+
 def test_diagonal_and_multigrid_grids_mix():
     grids = [pool_grid(W, H, 7 + k, FS_PRECOND_DIAGONAL if k % 2 else None)[0] for k, (W, H) in
              enumerate([(40, 30), (33, 20), (64, 48), (50, 50)])]
