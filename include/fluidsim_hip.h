@@ -300,9 +300,9 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
 /* Split of fs_batch_project and fs_batch_apply_viscosity (no reference counterpart): grids of more than
  * `solo_cells` cells are solved by fs_project's / fs_apply_viscosity's own multi-kernel solve, one after
  * another on the context's stream, while the persistent launch runs the smaller ones on a second stream
- * -- one workgroup is slow on a large grid, and the batch lasts as long as its slowest instance.  The route depends on a grid's size only, so a
- * grid's result is the same alone and in any batch.  0 (the default) batches every grid; contexts
- * created with FS_CONTEXT_SINGLE_STREAM always do. */
+ * -- one workgroup is slow on a large grid, and the batch lasts as long as its slowest instance.  The
+ * route depends on a grid's size only, so a grid's result is the same alone and in any batch.  0 (the
+ * default) batches every grid; contexts created with FS_CONTEXT_SINGLE_STREAM always do. */
 fs_status fs_context_set_batch_split(fs_context* ctx, long long solo_cells);
 
 /* ---- device footprint (no reference counterpart) ----
