@@ -911,10 +911,11 @@ __device__ void bv_cycle(const BatchVisDesc& D, float dt, float2* vel, float4* c
 
 // The coarse end of a V-cycle in one workgroup: levels whose sweeps are too small to fill the device cost
 // more in launches than in work, so cycle() hands every level from the first with at most kVisTailSlots
-// slots down to the coarsest to this kernel (the same sweeps, restriction, Jacobi coarse solve and
+// slots down to the coarsest to this kernel (1024: one level of 1196 slots more made a 1417 x 787 solve
+// slower, one of 4590 slots 20% slower) (the same sweeps, restriction, Jacobi coarse solve and
 // prolongation as the per-level kernels, each a workgroup-strided loop).  b0 is the first level's rhs;
 // the result lands in lev[0].x.
-constexpr int kVisTailSlots = 4096;
+constexpr int kVisTailSlots = 1024;
 
 struct VisTail {
     int levels;
