@@ -238,6 +238,8 @@ struct fs_context {
     DevBuf<unsigned char> vis_batch_desc;   // descriptors of the last batched viscosity solve
     PcgScalars* batch_pinned = nullptr;     // host-pinned landing slots of a batch's scalar blocks
     size_t batch_pinned_n = 0;
+    unsigned char* staging = nullptr;       // host-pinned source of the batch descriptors' upload
+    size_t staging_n = 0;
     long long batch_solo_cells = 0;         // fs_context_set_batch_split (0: every instance batched)
     hipStream_t batch_stream = nullptr;     // the persistent launch's stream while solo solves run
     hipEvent_t batch_in = nullptr, batch_out = nullptr;
@@ -266,6 +268,35 @@ hipStream_t batch_side_stream(fs_context* ctx)
     HIPC(hipEventRecord(ctx->batch_in, ctx->stream));
     HIPC(hipStreamWaitEvent(ctx->batch_stream, ctx->batch_in, 0));
     return ctx->batch_stream;
+}
+
+// Waits for a stream when it goes out of scope, also when an exception leaves the scope: host memory
+// that an enqueued copy reads must outlive the copy.
+struct SyncOnExit {
+    hipStream_t stream;
+    ~SyncOnExit() { (void)hipStreamSynchronize(stream); }
+};
+
+// Copies `bytes` of host memory to device memory on the context's stream through the context's pinned
+// staging buffer, so the copy neither depends on the caller's memory staying alive nor reads pageable
+// memory after this returns.  The caller synchronises the stream before the staging buffer is reused
+// (every batched entry point does before it returns).
+void stage_upload(fs_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0)
+        return;
+    if (ctx->staging_n < bytes) {
+        HIPC(hipStreamSynchronize(ctx->stream));
+        std::lock_guard<std::mutex> lock(device_setup_mutex());
+        if (ctx->staging)
+            (void)hipHostFree(ctx->staging);
+        ctx->staging = nullptr;
+        ctx->staging_n = 0;
+        HIPC(hipHostMalloc((void**)&ctx->staging, bytes, hipHostMallocDefault));
+        ctx->staging_n = bytes;
+    }
+    std::memcpy(ctx->staging, src, bytes);
+    HIPC(hipMemcpyAsync(dst, ctx->staging, bytes, hipMemcpyHostToDevice, ctx->stream));
 }
 
 struct fs_grid {
@@ -1422,6 +1453,8 @@ fs_status fs_context_destroy(fs_context* ctx)
                 (void)hipHostFree(ctx->pinned_value);
             if (ctx->batch_pinned)
                 (void)hipHostFree(ctx->batch_pinned);
+            if (ctx->staging)
+                (void)hipHostFree(ctx->staging);
             if (ctx->stream)
                 (void)hipStreamDestroy(ctx->stream);
             if (ctx->aux)
@@ -1998,6 +2031,7 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
         }
         set_device(ctx);
         hipStream_t st = ctx->stream;
+        const SyncOnExit drain{st};   // the staged descriptors and host results stay valid until then
         const size_t bytes = viscosity_batch_desc_bytes();
         std::vector<unsigned char> descs;
         std::vector<int> batched, alone, solo;
@@ -2021,7 +2055,7 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
             const size_t need_bytes = batched.size() * bytes;
             if (ctx->vis_batch_desc.n < need_bytes)
                 ctx->vis_batch_desc.alloc(need_bytes);
-            HIPC(hipMemcpyAsync(ctx->vis_batch_desc.p, descs.data(), need_bytes, hipMemcpyHostToDevice, st));
+            stage_upload(ctx, ctx->vis_batch_desc.p, descs.data(), need_bytes);
             const bool spread = (int)batched.size() <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds;
             const hipStream_t bs = solo.empty() ? st : batch_side_stream(ctx);
             launch_batch_viscosity(bs, ctx->vis_batch_desc.p, (int)batched.size(), spread ? kBatchSpreadLds : 0);
@@ -2664,6 +2698,7 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             return;
         set_device(ctx);
         hipStream_t st = ctx->stream;
+        const SyncOnExit drain{st};   // the staged descriptors and host results stay valid until then
         // Instances above the split run fs_project's multi-kernel solve on the context's stream while the
         // persistent launch runs the others on a stream of its own; the route depends on the instance's
         // size only, so each result is the same alone and in any batch.
@@ -2684,8 +2719,9 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             }
             if (ctx->batch_desc.n < (size_t)nb)
                 ctx->batch_desc.alloc((size_t)nb);
-            HIPC(hipMemcpyAsync(ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance),
-                hipMemcpyHostToDevice, st));
+            // the descriptors go through pinned staging: `desc` dies at the end of this block, before the
+            // stream is synchronised
+            stage_upload(ctx, ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance));
             // One workgroup is one instance's whole solve: while the instances fit on the compute units one
             // each, reserve more than half a unit's 160 KB of LDS so that no two of them share a unit.
             if (nb <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds)

@@ -134,6 +134,15 @@ void orc_capillary_geometry(int width, int height, double cell_size, const doubl
     double* raw, double* curvature);
 /* faceCurvature (FluidSurfaceTensionCPU.h:378-386) of two cells of a (value, valid) array. */
 double orc_face_curvature(const double* curvature, size_t left, size_t right);
+/* The quadratic fit (Fit::add + Fit::evaluate, FluidSurfaceTensionCPU.h:204-254) of n weighted points
+ * (weight may be NULL: 1 each) about a normal; out = (curvature, valid). */
+void orc_fit_curvature(int n, const double* px, const double* py, const double* weight, double nx, double ny,
+    double cell_size, double* out);
+/* segment (FluidSurfaceTensionCPU.h:287-324) of one cell: out = (centroid x, y relative to the cell
+ * centre, length). */
+void orc_interface_segment(double nx, double ny, double fraction, double* out);
+/* interface_geometry::offset (FluidInterfaceGeometry.h:51-61). */
+double orc_line_offset(double nx, double ny, double fraction);
 
 /* Derived fields (read-only views of cached data, rebuilt on demand). */
 void orc_density(orc_grid* g, double* density);

@@ -407,3 +407,41 @@ double orc_face_curvature(const double* curvature, size_t left, size_t right)
         return bv ? b : 0.0;
     return bv ? 0.5 * (a + b) : a;
 }
+
+/* Test hooks: the quadratic fit and the interface segment on their own, so that each is checked against
+ * closed forms (exact parabolas, straight chords) rather than only through a whole geometry pass. */
+
+/* Fit::add over n points, then Fit::evaluate (FluidSurfaceTensionCPU.h:204-254); out = (value, valid). */
+void orc_fit_curvature(int n, const double* px, const double* py, const double* weight, double nx, double ny,
+    double cell_size, double* out)
+{
+    tfit fit;
+    memset(&fit, 0, sizeof(fit));
+    const tvec normal = {nx, ny};
+    for (int i = 0; i < n; ++i)
+        fit_add(&fit, px[i], py[i], normal, weight ? weight[i] : 1.0);
+    const tcurv k = fit_evaluate(&fit, cell_size);
+    out[0] = k.value;
+    out[1] = k.valid ? 1.0 : 0.0;
+}
+
+/* FluidSurfaceTensionCPU::segment (:287-324) of a cell with liquid fraction `fraction` cut by a line of
+ * normal (nx, ny): out = (centroid x, centroid y relative to the cell centre, length). */
+void orc_interface_segment(double nx, double ny, double fraction, double* out)
+{
+    tgeom G;
+    memset(&G, 0, sizeof(G));
+    G.W = 1;
+    G.H = 1;
+    G.h = 1.0;
+    G.liquid = &fraction;
+    const tvec n = {nx, ny};
+    segment(&G, 0, 0, n, out);
+}
+
+/* interface_geometry::offset (FluidInterfaceGeometry.h:51-61) */
+double orc_line_offset(double nx, double ny, double fraction)
+{
+    const tvec n = {nx, ny};
+    return line_offset(n, fraction);
+}

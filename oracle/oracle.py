@@ -102,6 +102,9 @@ def lib():
         "orc_capillary_step_limit": (C.c_double, [vp]),
         "orc_apply_surface_tension": (None, [vp, C.c_double]),
         "orc_capillary_geometry": (None, [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
+        "orc_fit_curvature": (None, [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
+        "orc_interface_segment": (None, [C.c_double, C.c_double, C.c_double, _dp]),
+        "orc_line_offset": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "orc_density": (None, [vp, _dp]),
         "orc_face_weights": (None, [vp, _dp, _dp]),
         "orc_apply_operator": (C.c_double, [vp, _dp, _dp]),
@@ -392,6 +395,30 @@ def capillary_geometry(width, height, cell_size, fraction):
     heights, raw, curv = np.zeros(4 * n), np.zeros(2 * n), np.zeros(2 * n)
     lib().orc_capillary_geometry(width, height, cell_size, _ptr(f), _ptr(heights), _ptr(raw), _ptr(curv))
     return heights.reshape(n, 4), raw.reshape(n, 2), curv.reshape(n, 2)
+
+
+def fit_curvature(px, py, normal, cell_size=1.0, weight=None):
+    """The capillary quadratic fit of points (px, py) about `normal` (FluidSurfaceTensionCPU.h:204-254):
+    (curvature, valid)."""
+    px, py = _f64(px), _f64(py)
+    w = None if weight is None else _f64(weight)
+    out = np.zeros(2)
+    lib().orc_fit_curvature(px.size, _ptr(px), _ptr(py), None if w is None else _ptr(w), float(normal[0]),
+                            float(normal[1]), float(cell_size), _ptr(out))
+    return float(out[0]), bool(out[1])
+
+
+def interface_segment(normal, fraction):
+    """Centroid (relative to the cell centre) and length of a cell's interface segment
+    (FluidSurfaceTensionCPU.h:287-324)."""
+    out = np.zeros(3)
+    lib().orc_interface_segment(float(normal[0]), float(normal[1]), float(fraction), _ptr(out))
+    return out
+
+
+def line_offset(normal, fraction):
+    """interface_geometry::offset (FluidInterfaceGeometry.h:51-61)."""
+    return float(lib().orc_line_offset(float(normal[0]), float(normal[1]), float(fraction)))
 
 
 # ---- scene frame (FluidSim/FluidScene.inl:328-410) ----
