@@ -50,6 +50,10 @@ class FsProjectionResult(C.Structure):
     ]
 
 
+class FsCellEdit(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("component", C.c_int), ("mass", C.c_double)]
+
+
 class FsStepTimings(C.Structure):
     _fields_ = [
         ("transport_ms", C.c_double),
@@ -169,6 +173,8 @@ SIGNATURES = {
     "fs_apply_gravity": [_vp, C.c_double],
     "fs_add_vortex": [_vp, C.c_double, C.c_double, C.c_double, C.c_double],
     "fs_max_divergence": [_vp, _dp],
+    "fs_color": [_vp, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)],
+    "fs_commit_edits": [_vp, C.c_int, C.POINTER(FsCellEdit), C.POINTER(C.c_double)],
     "fs_advect": [_vp, C.c_double, _ip],
     "fs_apply_viscosity": [_vp, C.c_double, _ip],
     "fs_apply_surface_tension": [_vp, C.c_double],

@@ -331,6 +331,8 @@ struct fs_grid {
     DevBuf<double2> st_heights, st_orient, st_raw, st_curv;   // capillary geometry (first use)
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<unsigned char> solid;   // solid-mask extension (empty: no mask)
+    DevBuf<float> color_px;        // writeColorPixels output, 4 floats per cell (first use)
+    DevBuf<EditCell> edit_cells;   // resolved material edits of the last commit (first use)
     long long solid_cells = 0;
     DevBuf<PcgScalars> S;
     DevBuf<PcgScalars> S_aux;   // failure flag of a hierarchy rebuilt on the second stream
@@ -1777,6 +1779,85 @@ fs_status fs_invalidate_caches(fs_grid* grid)
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
         grid->coeff_valid = false;
         grid->mg_valid = false;
+    });
+}
+
+// writeColorPixels (FluidGridBase.h:139, FluidGridCPU.cpp:1826-1860)
+fs_status fs_color(fs_grid* grid, const float* component_rgb, int liquid_fraction_alpha, float* pixels)
+{
+    return guard([&] {
+        need(grid != nullptr && pixels != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        fs_grid& g = *grid;
+        need(g.uploaded, FS_E_LOGIC, "Upload the grid state before rendering it");
+        const size_t nl = g.rho_l.size(), ng = g.rho_g.size(), N = g.N();
+        need(g.lmass.n == nl * N && g.gmass.n == ng * N, FS_E_LOGIC,
+            "Upload the configured fluid mixtures before rendering them");
+        need(component_rgb != nullptr || nl + ng == 0, FS_E_INVALID_ARGUMENT, "null component colours");
+        ColorTable table{};
+        for (size_t k = 0; k < nl + ng; ++k) {
+            for (int c = 0; c < 3; ++c)
+                table.rgb[k][c] = component_rgb[3 * k + c];
+            table.density[k] = k < nl ? g.rho_l[k] : g.rho_g[k - nl];
+        }
+        set_device(g.ctx);
+        if (g.color_px.n != 4 * N)
+            g.bytes += g.color_px.alloc(4 * N);
+        launch_color(g.st(), g.dev(), table, liquid_fraction_alpha != 0, g.color_px.p);
+        HIPC(hipMemcpyAsync(pixels, g.color_px.p, 4 * N * sizeof(float), hipMemcpyDeviceToHost, g.st()));
+        HIPC(hipStreamSynchronize(g.st()));
+    });
+}
+
+// commitEdits (FluidGridBase.h:142, FluidGridCPU.cpp:253-284) on resolved edits; duplicates are reduced as
+// CellEditBatch::prepare does (FluidGridEdit.h:37-95): raster order, the last edit of a cell wins.
+fs_status fs_commit_edits(fs_grid* grid, int count, const fs_cell_edit* cells, const double* velocity)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(velocity == nullptr, FS_E_UNSUPPORTED, "Material edit velocity requires uniform WebGPU");
+        need(count >= 0 && (count == 0 || cells != nullptr), FS_E_INVALID_ARGUMENT, "null edit list");
+        fs_grid& g = *grid;
+        const int components = (int)(g.rho_l.size() + g.rho_g.size());
+        std::vector<EditCell> list((size_t)count);
+        for (int i = 0; i < count; ++i) {
+            const fs_cell_edit& c = cells[i];
+            need(c.x >= 0 && c.x < g.W && c.y >= 0 && c.y < g.H, FS_E_INVALID_ARGUMENT,
+                "Fluid grid edit cell is outside the grid");
+            need(c.component >= 0 && c.component < components, FS_E_INVALID_ARGUMENT,
+                "Fluid grid edit component is not configured");
+            need(std::isfinite(c.mass) && c.mass >= 0.0, FS_E_INVALID_ARGUMENT,
+                "Fluid grid edit mass must be finite and non-negative");
+            list[(size_t)i] = EditCell{c.x, c.y, c.component, 0, c.mass};
+        }
+        const size_t N = g.N();
+        need(g.uploaded && g.lmass.n == g.rho_l.size() * N && g.gmass.n == g.rho_g.size() * N, FS_E_LOGIC,
+            "Upload the configured fluid mixtures before committing edits");
+        if (count == 0)
+            return;
+        std::stable_sort(list.begin(), list.end(), [](const EditCell& a, const EditCell& b) {
+            return a.y < b.y || (a.y == b.y && a.x < b.x);
+        });
+        size_t kept = 0;
+        for (size_t first = 0; first < list.size();) {
+            size_t last = first + 1;
+            while (last < list.size() && list[last].x == list[first].x && list[last].y == list[first].y)
+                ++last;
+            list[kept++] = list[last - 1];
+            first = last;
+        }
+        list.resize(kept);
+        set_device(g.ctx);
+        if (g.edit_cells.n < kept) {
+            if (g.edit_cells.n)
+                g.bytes -= g.edit_cells.n * sizeof(EditCell);
+            g.bytes += g.edit_cells.alloc(kept);
+        }
+        const SyncOnExit drain{g.st()};
+        stage_upload(g.ctx, g.edit_cells.p, list.data(), kept * sizeof(EditCell));
+        launch_commit_edits(g.st(), g.dev(), g.edit_cells.p, (int)kept);
+        HIPC(hipStreamSynchronize(g.st()));
+        g.coeff_valid = false;   // invalidateCaches (FluidGridCPU.cpp:282)
+        g.mg_valid = false;
     });
 }
 

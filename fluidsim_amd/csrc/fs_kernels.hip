@@ -1044,6 +1044,43 @@ __global__ void k_copy_f32(const float* src, float* dst, size_t n)
         dst[i] = src[i];
 }
 
+// writeColorPixels (FluidGridCPU.cpp:1826-1860): per cell, RGBA with the component colours weighted by
+// their volume fractions mass / (density h^2), in the reference's precision and order (FP64 fraction,
+// each channel term rounded to FP32 and summed in FP32, liquid components before gas components).
+__global__ __launch_bounds__(kNT) void k_color(GridDev g, ColorTable t, int alpha_from_fraction, float4* out)
+{
+    const size_t N = (size_t)g.W * g.H;
+    const double cell_volume = g.h * g.h;
+    for (size_t i = (size_t)blockIdx.x * kNT + threadIdx.x; i < N; i += (size_t)gridDim.x * kNT) {
+        float4 px = make_float4(0.0f, 0.0f, 0.0f, alpha_from_fraction ? (float)g.fraction[i] : 1.0f);
+        for (int k = 0; k < g.nl + g.ng; ++k) {
+            const double mass = k < g.nl ? g.lmass[(size_t)k * N + i] : g.gmass[(size_t)(k - g.nl) * N + i];
+            const double volume_fraction = mass / (t.density[k] * cell_volume);
+            px.x += (float)(volume_fraction * (double)t.rgb[k][0]);
+            px.y += (float)(volume_fraction * (double)t.rgb[k][1]);
+            px.z += (float)(volume_fraction * (double)t.rgb[k][2]);
+        }
+        out[i] = px;
+    }
+}
+
+// commitEdits (FluidGridCPU.cpp:253-284) for resolved, duplicate-free cells: the cell becomes pure liquid
+// or pure gas holding one component's mass, every other component's mass zero.
+__global__ __launch_bounds__(kNT) void k_commit_edits(GridDev g, const EditCell* cells, int n)
+{
+    const size_t N = (size_t)g.W * g.H;
+    for (int e = blockIdx.x * kNT + threadIdx.x; e < n; e += gridDim.x * kNT) {
+        const EditCell c = cells[e];
+        const size_t i = (size_t)c.y * g.W + c.x;
+        const bool liquid = c.component < g.nl;
+        g.fraction[i] = liquid ? 1.0 : 0.0;
+        for (int k = 0; k < g.nl; ++k)
+            g.lmass[(size_t)k * N + i] = k == c.component ? c.mass : 0.0;
+        for (int k = 0; k < g.ng; ++k)
+            g.gmass[(size_t)k * N + i] = g.nl + k == c.component ? c.mass : 0.0;
+    }
+}
+
 inline unsigned linear_blocks(size_t n)
 {
     size_t b = (n + kNT - 1) / kNT;
@@ -1140,6 +1177,18 @@ void launch_final_flush_sum(hipStream_t st, const GridDev& g)
 void launch_correct_velocity(hipStream_t st, const GridDev& g, double dt)
 {
     hipLaunchKernelGGL(k_correct_velocity, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, dt);
+}
+
+void launch_color(hipStream_t st, const GridDev& g, const ColorTable& t, int alpha_from_fraction, float* rgba)
+{
+    hipLaunchKernelGGL(k_color, dim3(linear_blocks((size_t)g.W * g.H)), dim3(kNT), 0, st, g, t, alpha_from_fraction,
+        (float4*)rgba);
+}
+
+void launch_commit_edits(hipStream_t st, const GridDev& g, const EditCell* cells, int n)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_commit_edits, dim3(linear_blocks((size_t)n)), dim3(kNT), 0, st, g, cells, n);
 }
 
 void launch_max_divergence(hipStream_t st, const GridDev& g)

@@ -141,6 +141,25 @@ void launch_copy_many(hipStream_t st, const CopyList& list);
 
 void launch_pad_copy(hipStream_t st, const double* src, double* dst, int W, int H, int to_padded);
 
+// ---- rendering and material edits (FluidGridBase.h:139-142) ----
+constexpr int kMaxColorComponents = 128;   // liquid + gas components of one grid
+// Colour and density of each component, liquid components first.
+struct ColorTable {
+    float rgb[kMaxColorComponents][3];
+    double density[kMaxColorComponents];
+};
+// writeColorPixels into a device RGBA buffer (4 floats per cell).
+void launch_color(hipStream_t st, const GridDev& g, const ColorTable& t, int alpha_from_fraction, float* rgba);
+// One resolved material edit (detail::ResolvedCellEdit, FluidGridEdit.h:22-28).
+struct EditCell {
+    int x, y;
+    int component;     // liquid components first, then gas
+    int pad;
+    double mass;
+};
+// commitEdits on duplicate-free cells (each cell at most once).
+void launch_commit_edits(hipStream_t st, const GridDev& g, const EditCell* cells, int n);
+
 // ---- PCG loop (FP32 vectors, FP64 reductions) ----
 void launch_pcg_ad(hipStream_t st, const GridDev& g);
 void launch_pcg_update(hipStream_t st, const GridDev& g);

@@ -113,6 +113,71 @@ class FluidMixture:
         return any(c.dynamic_viscosity != 0.0 for c in self.components)
 
 
+@dataclass
+class Substance:
+    """Substance (SubstanceCatalogue.h:14-21): a name, an RGB colour and an optional composition."""
+    name: str
+    color: tuple = (1.0, 1.0, 1.0)
+    composition: list = field(default_factory=list)
+
+
+class SubstanceCatalogue:
+    """SubstanceCatalogue (SubstanceCatalogue.h:23-59): substances addressed by SubstanceId (their index)."""
+
+    def __init__(self, substances=()):
+        self._items: List[Substance] = list(substances)
+
+    def add(self, substance: Substance) -> int:
+        self._items.append(substance)
+        return len(self._items) - 1
+
+    def __getitem__(self, substance_id: int) -> Substance:
+        return self._items[substance_id]
+
+    def __len__(self) -> int:
+        return len(self._items)
+
+    def size(self) -> int:
+        return len(self._items)
+
+    def find(self, name: str) -> int:
+        for k, s in enumerate(self._items):
+            if s.name == name:
+                return k
+        raise KeyError(f"no substance named {name!r} in the catalogue")
+
+
+LIQUID, GAS = "liquid", "gas"   # Phase (FluidMixture.h:8-12)
+
+
+class FluidGridEdit:
+    """FluidGridEdit<Grid> (FluidGridEdit.h:97-149): staged material replacements; only commit() touches
+    the grid.  setCell(x, y, phase, substance) stages one change (the last one of a cell wins); commit()
+    resolves them (CellEditBatch::prepare) and applies them through fs_commit_edits, then clears."""
+
+    def __init__(self, grid: "FluidGridHIP"):
+        self.grid = grid
+        self.changes: list = []
+
+    def size(self) -> int:
+        return len(self.changes)
+
+    def empty(self) -> bool:
+        return not self.changes
+
+    def clear(self):
+        self.changes.clear()
+
+    def setCell(self, x: int, y: int, phase: str, substance):
+        self.changes.append((int(x), int(y), phase, substance))
+
+    def commit(self, velocity=None):
+        if self.empty():
+            return
+        self.grid.commitEdits(self.changes, velocity)
+        self.clear()
+
+
 # ---- devices ----
 
 class Context:
@@ -289,6 +354,65 @@ class FluidGridHIP:
 
     def getSubstances(self):
         return self.substances
+
+    def _substance_id(self, substance):
+        """A component's SubstanceId; a name is looked up in the catalogue (mixtures built by name)."""
+        if isinstance(substance, str):
+            if isinstance(self.substances, SubstanceCatalogue):
+                return self.substances.find(substance)
+            for k, s in enumerate(self.substances):
+                if getattr(s, "name", s) == substance:
+                    return k
+            raise KeyError(f"no substance named {substance!r} in the catalogue")
+        return int(substance)
+
+    def _component_colors(self) -> np.ndarray:
+        rgb = []
+        for comp in self.liquid_mixture.components + self.gas_mixture.components:
+            sub = self.substances[self._substance_id(comp.substance)]
+            rgb.extend(float(c) for c in sub.color)
+        return np.asarray(rgb, dtype=np.float32)
+
+    # -- rendering and edits (FluidGridBase.h:139-142) --
+    def writeColorPixels(self, pixels: Optional[np.ndarray] = None, liquid_fraction_alpha: bool = False) -> np.ndarray:
+        """writeColorPixels (FluidGridCPU.cpp:1826-1860): RGBA float32 per cell, the component colours of the
+        substance catalogue weighted by volume fraction.  Reuses `pixels` when it has 4*W*H float32 slots."""
+        n = 4 * self.width * self.height
+        if pixels is None or pixels.dtype != np.float32 or pixels.size != n or not pixels.flags.c_contiguous:
+            pixels = np.empty(n, dtype=np.float32)
+        rgb = self._component_colors()
+        call("fs_color", self._h, rgb.ctypes.data_as(C.POINTER(C.c_float)) if rgb.size else None,
+             int(bool(liquid_fraction_alpha)), pixels.ctypes.data_as(C.POINTER(C.c_float)))
+        return pixels
+
+    def edit(self) -> FluidGridEdit:
+        return FluidGridEdit(self)
+
+    def commitEdits(self, changes, velocity=None):
+        """commitEdits (FluidGridCPU.cpp:253-284) of [(x, y, phase, substance)] changes, resolved as
+        CellEditBatch::prepare does (FluidGridEdit.h:37-95): IndexError for a cell outside the grid,
+        ValueError for a substance the phase does not hold -- both before anything changes."""
+        if velocity is not None:
+            raise ValueError("Material edit velocity requires uniform WebGPU")
+        nl = len(self.liquid_mixture.components)
+        cell_volume = self.cell_size * self.cell_size
+        cells = (_abi.FsCellEdit * max(1, len(changes)))()
+        last_key, component, mass = None, 0, 0.0
+        for i, (x, y, phase, substance) in enumerate(changes):
+            if not (0 <= x < self.width and 0 <= y < self.height):
+                raise IndexError("Fluid grid edit cell is outside the grid")
+            key = (phase, substance)
+            if key != last_key:
+                mixture = self.liquid_mixture if phase == LIQUID else self.gas_mixture
+                want = self._substance_id(substance)
+                found = [k for k, c in enumerate(mixture.components) if self._substance_id(c.substance) == want]
+                if not found:
+                    raise ValueError("Fluid grid edit substance is not configured in the selected phase")
+                component = found[0] + (0 if phase == LIQUID else nl)
+                mass = mixture.components[found[0]].density * cell_volume
+                last_key = key
+            cells[i].x, cells[i].y, cells[i].component, cells[i].mass = x, y, component, mass
+        call("fs_commit_edits", self._h, len(changes), cells, None)
 
     def getLiquidMixture(self) -> FluidMixture:
         return self.liquid_mixture

@@ -14,7 +14,10 @@
 
 #include "../../include/fluidsim_hip.h"
 
+#include <algorithm>
+#include <array>
 #include <cstddef>
+#include <cstdint>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -58,8 +61,32 @@ struct FluidProjectionResult {
 
 enum class Phase { Liquid, Gas };
 
+using SubstanceId = std::uint32_t;
+
+// Substance and SubstanceCatalogue (FluidSim/SubstanceCatalogue.h:14-59): the colours writeColorPixels
+// reads; SubstanceId is a substance's index in the catalogue.  Chemistry (elements, composition) is not
+// used on this path and not mirrored.
+struct Substance {
+    std::string name;
+    std::array<float, 3> color{1.0f, 1.0f, 1.0f};
+};
+
+class SubstanceCatalogue {
+public:
+    SubstanceId add(Substance substance)
+    {
+        definitions_.push_back(std::move(substance));
+        return static_cast<SubstanceId>(definitions_.size() - 1);
+    }
+    const Substance& operator[](SubstanceId id) const { return definitions_.at(id); }
+    std::size_t size() const { return definitions_.size(); }
+
+private:
+    std::vector<Substance> definitions_;
+};
+
 struct MixtureComponent {
-    int substance = 0;
+    SubstanceId substance = 0;
     double density = 0.0;
     double dynamic_viscosity = 0.0;
 };
@@ -100,6 +127,63 @@ struct FrameStatistics {
 struct DVec2 {
     double x = 0.0, y = 0.0;
 };
+
+// Staged material edits: detail::CellEdit / ResolvedCellEdit / CellEditBatch (FluidSim/FluidGridEdit.h:14-95).
+struct CellEdit {
+    int x;
+    int y;
+    Phase phase;
+    SubstanceId substance;
+};
+
+struct CellEditBatch {
+    std::vector<CellEdit> changes;
+    std::vector<fs_cell_edit> cells;   // resolved: component (liquids, then gases) and mass
+    bool has_velocity = false;
+    DVec2 velocity;
+
+    // Resolves every change to a mixture component and its full-cell mass (density * cell volume), checking
+    // the whole batch before anything is written: std::out_of_range for a cell outside the grid,
+    // std::invalid_argument for a substance the phase does not hold.  Cells end in raster order with the
+    // last change of a cell kept.
+    void prepare(int width, int height, double cell_volume, const FluidMixture& liquid, const FluidMixture& gas)
+    {
+        cells.clear();
+        cells.reserve(changes.size());
+        int component = 0;
+        double mass = 0.0;
+        for (std::size_t i = 0; i < changes.size(); ++i) {
+            const CellEdit& c = changes[i];
+            if (c.x < 0 || c.x >= width || c.y < 0 || c.y >= height)
+                throw std::out_of_range("Fluid grid edit cell is outside the grid");
+            if (i == 0 || c.phase != changes[i - 1].phase || c.substance != changes[i - 1].substance) {
+                const FluidMixture& mixture = c.phase == Phase::Liquid ? liquid : gas;
+                std::size_t k = 0;
+                while (k < mixture.components.size() && mixture.components[k].substance != c.substance)
+                    ++k;
+                if (k == mixture.components.size())
+                    throw std::invalid_argument("Fluid grid edit substance is not configured in the selected phase");
+                component = (int)(k + (c.phase == Phase::Liquid ? 0 : liquid.components.size()));
+                mass = mixture.components[k].density * cell_volume;
+            }
+            cells.push_back(fs_cell_edit{c.x, c.y, component, mass});
+        }
+        auto raster = [](const fs_cell_edit& a, const fs_cell_edit& b) {
+            return a.y < b.y || (a.y == b.y && a.x < b.x);
+        };
+        std::stable_sort(cells.begin(), cells.end(), raster);
+        std::size_t kept = 0;
+        for (std::size_t i = 0; i < cells.size(); ++i) {
+            if (kept > 0 && cells[kept - 1].x == cells[i].x && cells[kept - 1].y == cells[i].y)
+                cells[kept - 1] = cells[i];
+            else
+                cells[kept++] = cells[i];
+        }
+        cells.resize(kept);
+    }
+};
+
+template <class Grid> class FluidGridEdit;
 
 inline void check(fs_status status)
 {
@@ -160,7 +244,9 @@ public:
     void setOrigin(DVec2 origin) { check(fs_set_origin(grid_, origin.x, origin.y)); }
     bool requiresRenderThread() const { return false; }
 
-    void configureMaterials(FluidMixture liquid, FluidMixture gas)
+    // configureMaterials(SubstanceCatalogue, FluidMixture, FluidMixture) (FluidGridBase.h:42): the
+    // catalogue supplies the component colours of writeColorPixels.
+    void configureMaterials(SubstanceCatalogue substances, FluidMixture liquid, FluidMixture gas)
     {
         std::vector<double> ld, lv, gd, gv;
         for (const auto& c : liquid.components) {
@@ -173,9 +259,11 @@ public:
         }
         check(fs_configure_materials(grid_, (int)ld.size(), ld.data(), lv.data(), (int)gd.size(), gd.data(),
             gv.data()));
+        substances_ = std::move(substances);
         liquid_ = std::move(liquid);
         gas_ = std::move(gas);
     }
+    const SubstanceCatalogue& getSubstances() const { return substances_; }
     const FluidMixture& getLiquidMixture() const { return liquid_; }
     const FluidMixture& getGasMixture() const { return gas_; }
     bool hasViscosity() const { return liquid_.hasViscosity() || gas_.hasViscosity(); }
@@ -295,6 +383,32 @@ public:
     }
 
     void invalidateCaches() { check(fs_invalidate_caches(grid_)); }
+
+    // writeColorPixels (FluidGridBase.h:139, FluidGridCPU.cpp:1826-1860): RGBA per cell into caller storage.
+    void writeColorPixels(std::vector<float>& pixels, bool liquid_fraction_alpha = false) const
+    {
+        std::vector<float> rgb;
+        for (const FluidMixture* m : {&liquid_, &gas_})
+            for (const auto& c : m->components) {
+                const Substance& sub = substances_[c.substance];
+                rgb.insert(rgb.end(), sub.color.begin(), sub.color.end());
+            }
+        pixels.resize((size_t)width_ * height_ * 4);
+        check(fs_color(grid_, rgb.empty() ? nullptr : rgb.data(), liquid_fraction_alpha ? 1 : 0, pixels.data()));
+    }
+
+    // commitEdits (FluidGridBase.h:142, FluidGridCPU.cpp:253-284).  Velocity edits are a uniform-WebGPU
+    // feature: std::invalid_argument, as UniformCPU.
+    void commitEdits(CellEditBatch& batch)
+    {
+        if (batch.has_velocity)
+            throw std::invalid_argument("Material edit velocity requires uniform WebGPU");
+        batch.prepare(width_, height_, cell_size_ * cell_size_, liquid_, gas_);
+        check(fs_commit_edits(grid_, (int)batch.cells.size(), batch.cells.empty() ? nullptr : batch.cells.data(),
+            nullptr));
+    }
+
+    FluidGridEdit<FluidGridHIP> edit();
 
     void destroy()
     {
@@ -422,8 +536,49 @@ private:
     int width_;
     int height_;
     double cell_size_;
+    SubstanceCatalogue substances_;
     FluidMixture liquid_, gas_;
 };
+
+// FluidGridEdit<Grid> (FluidSim/FluidGridEdit.h:97-149): a session of staged edits; only commit() touches
+// the grid, and a successful commit clears the session.
+template <class Grid> class FluidGridEdit {
+public:
+    explicit FluidGridEdit(Grid& grid) : grid_(&grid) {}
+    void reserve(std::size_t count) { batch_.changes.reserve(count); }
+    std::size_t size() const { return batch_.changes.size(); }
+    bool empty() const { return batch_.changes.empty(); }
+    void clear()
+    {
+        batch_.changes.clear();
+        batch_.cells.clear();
+        batch_.has_velocity = false;
+    }
+    void setCell(int x, int y, Phase phase, SubstanceId substance) { batch_.changes.push_back({x, y, phase, substance}); }
+    void commit()
+    {
+        if (empty())
+            return;
+        batch_.has_velocity = false;
+        grid_->commitEdits(batch_);
+        clear();
+    }
+    void commit(DVec2 velocity)
+    {
+        if (empty())
+            return;
+        batch_.has_velocity = true;
+        batch_.velocity = velocity;
+        grid_->commitEdits(batch_);
+        clear();
+    }
+
+private:
+    Grid* grid_;
+    CellEditBatch batch_;
+};
+
+inline FluidGridEdit<FluidGridHIP> FluidGridHIP::edit() { return FluidGridEdit<FluidGridHIP>(*this); }
 
 // Batched projection (fs_batch_project; no reference counterpart): the grids share one context, each
 // gets the result its own project() contract defines; one divergence tolerance per grid.

@@ -231,6 +231,31 @@ fs_status fs_capillary_geometry(fs_grid* grid, double* heights, double* curvatur
 /* Semi-Lagrangian transport alone (build-defined mode, no reference counterpart). */
 fs_status fs_advect_semi_lagrangian(fs_grid* grid, double dt);
 
+/* writeColorPixels(std::vector<float>& pixels, bool liquid_fraction_alpha) (FluidGridBase.h:139,
+ * FluidGridCPU.cpp:1826-1860): pixels[4*W*H] receives RGBA per cell: RGB = sum over the components (liquid
+ * first, then gas, in mixture order) of (mass / (density h^2)) * colour, each term rounded to float and
+ * summed in float; A = the liquid fraction when liquid_fraction_alpha, else 1.  component_rgb holds 3
+ * floats per component in the same order: the colours of the SubstanceCatalogue entries the mixtures
+ * name (SubstanceCatalogue.h:14-21), looked up by the adapter. */
+fs_status fs_color(fs_grid* grid, const float* component_rgb, int liquid_fraction_alpha, float* pixels);
+
+/* One resolved material edit: detail::ResolvedCellEdit (FluidGridEdit.h:22-28).  component indexes the
+ * liquid components, then the gas components; mass is the component's density times h^2 as
+ * CellEditBatch::prepare computes it (FluidGridEdit.h:37-95). */
+typedef struct fs_cell_edit {
+    int x, y;
+    int component;
+    double mass;
+} fs_cell_edit;
+/* commitEdits(CellEditBatch&) (FluidGridBase.h:142, FluidGridCPU.cpp:253-284): each edited cell becomes
+ * pure liquid (fraction 1) or pure gas (fraction 0) holding only its component's mass; face velocities
+ * are kept.  The edits are applied in raster order and the last edit of a cell wins (as prepare() leaves
+ * them).  A cell outside the grid, an unconfigured component or a non-finite/negative mass returns
+ * FS_E_INVALID_ARGUMENT before anything changes; before an upload of the configured mixtures,
+ * FS_E_LOGIC.  velocity (2 doubles, the optional edit velocity) must be NULL: like UniformCPU, this
+ * backend returns FS_E_UNSUPPORTED ("requires uniform WebGPU") for it. */
+fs_status fs_commit_edits(fs_grid* grid, int count, const fs_cell_edit* cells, const double* velocity);
+
 /* ---- device-resident snapshots (no reference counterpart): keep the inputs in HBM and restore
  * them between timed solves or serving requests without a host round trip. ---- */
 fs_status fs_snapshot_save(fs_grid* grid);

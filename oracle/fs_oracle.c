@@ -1862,3 +1862,44 @@ int orc_apply_viscosity(orc_grid* g, double dt, int* iterations)
     free(material);
     return status;
 }
+
+/* writeColorPixels, FluidGridCPU.cpp:1826-1860: rgb holds 3 floats per component (liquid, then gas). */
+void orc_color_pixels(const orc_grid* g, const float* rgb, int liquid_fraction_alpha, float* pixels)
+{
+    const size_t N = (size_t)g->W * g->H;
+    const double cell_volume = g->h * g->h;
+    for (size_t i = 0; i < N; ++i) {
+        float* pixel = pixels + i * 4;
+        pixel[0] = pixel[1] = pixel[2] = 0.0f;
+        pixel[3] = liquid_fraction_alpha ? (float)g->fraction[i] : 1.0f;
+        for (int k = 0; k < g->nl + g->ng; ++k) {
+            const int liquid = k < g->nl;
+            const double density = liquid ? g->rho_l[k] : g->rho_g[k - g->nl];
+            const double mass = liquid ? g->lmass[(size_t)k * N + i] : g->gmass[(size_t)(k - g->nl) * N + i];
+            const double volume_fraction = mass / (density * cell_volume);
+            for (int channel = 0; channel < 3; ++channel)
+                pixel[channel] += (float)(volume_fraction * (double)rgb[3 * k + channel]);
+        }
+    }
+}
+
+/* commitEdits, FluidGridCPU.cpp:253-284, on cells already resolved and reduced by CellEditBatch::prepare
+ * (FluidGridEdit.h:37-95): component indexes liquids, then gases. */
+void orc_commit_edits(orc_grid* g, int count, const int* x, const int* y, const int* component, const double* mass)
+{
+    const size_t N = (size_t)g->W * g->H;
+    for (int n = 0; n < count; ++n) {
+        const size_t i = (size_t)y[n] * g->W + x[n];
+        const int liquid = component[n] < g->nl;
+        g->fraction[i] = liquid ? 1.0 : 0.0;
+        for (int k = 0; k < g->nl; ++k)
+            g->lmass[(size_t)k * N + i] = 0.0;
+        for (int k = 0; k < g->ng; ++k)
+            g->gmass[(size_t)k * N + i] = 0.0;
+        if (liquid)
+            g->lmass[(size_t)component[n] * N + i] = mass[n];
+        else
+            g->gmass[(size_t)(component[n] - g->nl) * N + i] = mass[n];
+    }
+    orc_invalidate_caches(g);
+}

@@ -88,6 +88,10 @@ orc_mg* orc_grid_multigrid(orc_grid* g);
 void orc_apply_gravity(orc_grid* g, double dt);
 void orc_add_vortex(orc_grid* g, double cx, double cy, double radius, double strength);
 double orc_max_divergence(const orc_grid* g);
+/* writeColorPixels (FluidGridCPU.cpp:1826-1860): pixels[4N] RGBA; rgb = 3 floats per component. */
+void orc_color_pixels(const orc_grid* g, const float* rgb, int liquid_fraction_alpha, float* pixels);
+/* commitEdits (FluidGridCPU.cpp:253-284) on resolved, reduced cells (CellEditBatch::prepare). */
+void orc_commit_edits(orc_grid* g, int count, const int* x, const int* y, const int* component, const double* mass);
 /* project(dt, max_iterations, divergence_tolerance) -- FluidGridCPU.cpp:1030-1124. */
 orc_result orc_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
 /* Gravity followed by projection: step() with transport and surface tension off. */

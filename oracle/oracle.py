@@ -105,6 +105,8 @@ def lib():
         "orc_fit_curvature": (None, [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
         "orc_interface_segment": (None, [C.c_double, C.c_double, C.c_double, _dp]),
         "orc_line_offset": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "orc_color_pixels": (None, [vp, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+        "orc_commit_edits": (None, [vp, C.c_int, _ip, _ip, _ip, _dp]),
         "orc_density": (None, [vp, _dp]),
         "orc_face_weights": (None, [vp, _dp, _dp]),
         "orc_apply_operator": (C.c_double, [vp, _dp, _dp]),
@@ -331,6 +333,27 @@ class OracleGrid:
     def capillary_step_limit(self):
         return lib().orc_capillary_step_limit(self._h)
 
+    def color_pixels(self, rgb, liquid_fraction_alpha=False):
+        """writeColorPixels (FluidGridCPU.cpp:1826-1860): float32 RGBA per cell; rgb = 3 per component."""
+        c = np.ascontiguousarray(rgb, dtype=np.float32).ravel()
+        out = np.zeros(4 * self.N, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        lib().orc_color_pixels(self._h, c.ctypes.data_as(fp), int(bool(liquid_fraction_alpha)),
+                               out.ctypes.data_as(fp))
+        return out
+
+    def commit_edits(self, cells):
+        """commitEdits (FluidGridCPU.cpp:253-284) on resolved cells [(x, y, component, mass)] as
+        prepare_edits returns them."""
+        n = len(cells)
+        xs = np.array([c[0] for c in cells], dtype=np.int32)
+        ys = np.array([c[1] for c in cells], dtype=np.int32)
+        ks = np.array([c[2] for c in cells], dtype=np.int32)
+        ms = np.array([c[3] for c in cells], dtype=np.float64)
+        ip = C.POINTER(C.c_int)
+        lib().orc_commit_edits(self._h, n, xs.ctypes.data_as(ip), ys.ctypes.data_as(ip), ks.ctypes.data_as(ip),
+                               _ptr(ms))
+
     def apply_surface_tension(self, dt):
         lib().orc_apply_surface_tension(self._h, dt)
 
@@ -419,6 +442,31 @@ def interface_segment(normal, fraction):
 def line_offset(normal, fraction):
     """interface_geometry::offset (FluidInterfaceGeometry.h:51-61)."""
     return float(lib().orc_line_offset(float(normal[0]), float(normal[1]), float(fraction)))
+
+
+def prepare_edits(changes, width, height, cell_volume, liquid, gas):
+    """CellEditBatch::prepare (FluidGridEdit.h:37-95): changes [(x, y, phase, substance)] with phase
+    "liquid" or "gas"; liquid / gas are the mixtures as [(substance, density)].  Returns the resolved cells
+    [(x, y, component, mass)] in raster order, the last change of a cell kept.  Raises IndexError for a
+    cell outside the grid and ValueError for a substance the phase does not hold."""
+    cells = []
+    for x, y, phase, substance in changes:
+        if not (0 <= x < width and 0 <= y < height):
+            raise IndexError("Fluid grid edit cell is outside the grid")
+        mixture = liquid if phase == "liquid" else gas
+        found = [k for k, (sub, _) in enumerate(mixture) if sub == substance]
+        if not found:
+            raise ValueError("Fluid grid edit substance is not configured in the selected phase")
+        k = found[0]
+        cells.append((x, y, k + (0 if phase == "liquid" else len(liquid)), mixture[k][1] * cell_volume))
+    cells.sort(key=lambda c: (c[1], c[0]))   # stable: equal cells keep their order
+    out = []
+    for c in cells:
+        if out and (out[-1][0], out[-1][1]) == (c[0], c[1]):
+            out[-1] = c
+        else:
+            out.append(c)
+    return out
 
 
 # ---- scene frame (FluidSim/FluidScene.inl:328-410) ----

@@ -31,12 +31,22 @@ bool sameState(const FluidGridHIP::State& a, const FluidGridHIP::State& b)
            a.liquid_mass == b.liquid_mass && a.gas_mass == b.gas_mass;
 }
 
+fluid_hip::SubstanceCatalogue catalogue()
+{
+    fluid_hip::SubstanceCatalogue c;
+    c.add({"water", {0.1f, 0.4f, 0.95f}});
+    c.add({"wine", {0.6f, 0.05f, 0.2f}});
+    c.add({"air", {0.3f, 0.3f, 0.3f}});
+    c.add({"smoke", {0.05f, 0.05f, 0.05f}});
+    return c;
+}
+
 FluidGridHIP::State configure(FluidGridHIP& grid)
 {
     FluidMixture liquid, gas;
     liquid.components = {{0, 100.0}, {1, 300.0}};
     gas.components = {{2, 0.1}, {3, 0.1}};
-    grid.configureMaterials(liquid, gas);
+    grid.configureMaterials(catalogue(), liquid, gas);
     grid.setOrigin({-7.0, 11.0});
     grid.setGravityY(80.0);
     const int width = grid.getWidth(), height = grid.getHeight();
@@ -133,7 +143,7 @@ void hydrostaticPressure(HipContext& ctx)
     FluidMixture liquid, gas;
     liquid.components = {{0, 100.0}};
     gas.components = {{1, 0.1}};
-    grid.configureMaterials(liquid, gas);
+    grid.configureMaterials(catalogue(), liquid, gas);
     grid.setGravityY(100.0);
     grid.setTransport(FS_TRANSPORT_NONE);
     FluidGridHIP::State s;
@@ -298,6 +308,62 @@ void batchedSolves(HipContext& ctx)
     require(its.size() == batch.size(), "batched viscosity result count");
 }
 
+// writeColorPixels and commitEdits through the adapter (FluidGridBase.h:139-142): caller storage is
+// resized and filled, pure cells show their substance colour, edits replace material only, the last edit
+// of a cell wins, a bad batch changes nothing, and velocity edits are refused as on UniformCPU.
+void colorsAndEdits(HipContext& ctx)
+{
+    FluidGridHIP grid(ctx, 24, 16, 0.5);
+    const FluidGridHIP::State before = configure(grid);
+    std::vector<float> pixels(3, -1.0f);
+    grid.writeColorPixels(pixels);
+    require(pixels.size() == 24 * 16 * 4, "writeColorPixels resizes the caller's storage");
+    auto edit = grid.edit();
+    edit.setCell(3, 2, fluid_hip::Phase::Liquid, 1);   // wine
+    edit.setCell(5, 9, fluid_hip::Phase::Gas, 3);      // smoke
+    edit.setCell(3, 2, fluid_hip::Phase::Liquid, 0);   // water: the last edit of (3, 2) wins
+    edit.commit();
+    require(edit.empty(), "a successful commit clears the session");
+    const FluidGridHIP::State after = grid.download();
+    const int a = 2 * 24 + 3, b = 9 * 24 + 5;
+    require(after.liquid_fraction[a] == 1.0 && after.liquid_fraction[b] == 0.0, "edited fractions");
+    require(after.liquid_mass[0][a] == 100.0 * 0.25 && after.liquid_mass[1][a] == 0.0, "edited liquid masses");
+    require(after.gas_mass[1][b] == 0.1 * 0.25 && after.gas_mass[0][b] == 0.0 && after.liquid_mass[0][b] == 0.0,
+        "edited gas masses");
+    require(after.u == before.u && after.v == before.v, "edits keep the face velocities");
+    grid.writeColorPixels(pixels, true);
+    require(pixels[4 * a] == 0.1f && pixels[4 * a + 1] == 0.4f && pixels[4 * a + 2] == 0.95f && pixels[4 * a + 3] == 1.0f,
+        "a pure water cell shows the water colour");
+    require(pixels[4 * b] == 0.05f && pixels[4 * b + 3] == 0.0f, "a pure smoke cell shows the smoke colour");
+    bool threw = false;
+    try {
+        edit.setCell(0, 0, fluid_hip::Phase::Liquid, 0);
+        edit.setCell(24, 0, fluid_hip::Phase::Liquid, 0);
+        edit.commit();
+    } catch (const std::out_of_range&) {
+        threw = true;
+    }
+    require(threw && grid.download().liquid_mass == after.liquid_mass, "an edit outside the grid rejects the batch");
+    edit.clear();
+    threw = false;
+    try {
+        edit.setCell(1, 1, fluid_hip::Phase::Gas, 0);   // water is not a gas component
+        edit.commit();
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    require(threw, "an unconfigured substance is rejected");
+    edit.clear();
+    threw = false;
+    try {
+        edit.setCell(1, 1, fluid_hip::Phase::Liquid, 0);
+        edit.commit({1.0, 0.0});
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    require(threw, "velocity edits require uniform WebGPU");
+}
+
 }  // namespace
 
 int main()
@@ -311,7 +377,7 @@ int main()
         {"upload validation", uploadIsValidated}, {"hydrostatic pressure", hydrostaticPressure},
         {"checkpoint continuation", checkpointContinuation}, {"scene frame", sceneFrame},
         {"obstacles and multigrid precision", obstaclesAndPrecision},
-        {"batched solves and the batch split", batchedSolves}};
+        {"batched solves and the batch split", batchedSolves}, {"colours and edits", colorsAndEdits}};
     int failures = 0;
     for (const auto& t : tests) {
         try {

@@ -32,4 +32,4 @@ def test_cpp_adapter_contract():
     proc = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
     print(proc.stdout, proc.stderr)
     assert proc.returncode == 0, proc.stderr
-    assert proc.stdout.count("PASS") == 8
+    assert proc.stdout.count("PASS") == 9
