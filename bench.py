@@ -165,7 +165,8 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
         })
     return {
         "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each (PLIC transport, "
-                    "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to |div u| <= 1e-4; FP32 "
+                    "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to a 1e-6 relative residual "
+                    "max|b - A p| <= 1e-6 max|b| in every step; FP32 "
                     "multigrid levels served one at a time, bfloat16 conductances in the serving loop), "
                     f"Poisson arrivals, seed {seed}",
         "lanes": lanes,

@@ -106,7 +106,7 @@ class FsRequest(C.Structure):
         ("gas_density", C.POINTER(C.c_double)), ("gas_viscosity", C.POINTER(C.c_double)),
         ("liquid_fraction", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("liquid_mass", C.POINTER(C.c_double)), ("gas_mass", C.POINTER(C.c_double)),
-        ("multigrid_precision", C.c_int), ("solid", C.c_void_p),
+        ("multigrid_precision", C.c_int), ("solid", C.c_void_p), ("relative_tolerance", C.c_int),
     ]
 
 
@@ -163,6 +163,8 @@ SIGNATURES = {
     "fs_set_multigrid_precision": [_vp, C.c_int],
     "fs_get_multigrid_precision": [_vp, C.POINTER(C.c_int)],
     "fs_set_flexible_cg": [_vp, C.c_int],
+    "fs_set_relative_tolerance": [_vp, C.c_int],
+    "fs_get_relative_tolerance": [_vp, _ip],
     "fs_batch_apply_viscosity": [C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int)],
     "fs_solid_cells": [_vp, C.POINTER(C.c_longlong)],
     "fs_upload": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],

@@ -66,6 +66,8 @@ struct PcgScalars {
     // the direction kernel reads the previous slot and writes the next one while other workgroups of
     // the same launch still read the previous value.
     double rz_slot[2];
+    // > 0: the tolerance is relative, tol = rel_tol * bmax once the rhs is formed (max-norm relative residual)
+    double rel_tol;
 };
 
 // 16-bit storage formats of multigrid levels (arithmetic stays FP32): IEEE binary16 and bfloat16

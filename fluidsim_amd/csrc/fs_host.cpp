@@ -311,6 +311,7 @@ struct fs_grid {
     fs_transport transport = FS_TRANSPORT_REFERENCE;
     fs_mg_precision mg_prec = FS_MG_FP32;
     bool flexible_cg = false;   // fs_set_flexible_cg
+    bool relative_tol = false;  // fs_set_relative_tolerance
     bool uploaded = false;
     bool coeff_valid = false;
     bool mg_valid = false;
@@ -1035,7 +1036,10 @@ struct fs_grid {
     void project_begin(double dt, int max_iterations, double tol)
     {
         const double scale = h * h / dt;
-        launch_init_scalars(st(), S.p, tol * scale, max_iterations);
+        if (relative_tol)
+            launch_init_scalars(st(), S.p, 0.0, max_iterations, tol);   // tol * max|b|, set on the device
+        else
+            launch_init_scalars(st(), S.p, tol * scale, max_iterations);
         if (!coeff_valid)
             rebuild_coefficients();
         GridDev g = dev();
@@ -1081,8 +1085,12 @@ struct fs_grid {
         launch_max_divergence(st(), g);
     }
 
-    fs_projection_result project_result(const PcgScalars& s, double tol)
+    // `tol` is the caller's divergence tolerance; in the relative mode the accepted divergence is the
+    // residual tolerance the device derived from max|b|, in divergence units (tol * max|b| * dt / h^2).
+    fs_projection_result project_result(const PcgScalars& s, double tol, double dt)
     {
+        if (relative_tol)
+            tol = s.tol / (h * h / dt);
         fs_projection_result res{};
         res.max_divergence = std::numeric_limits<double>::infinity();
         res.physical_steps = 1;
@@ -1130,7 +1138,7 @@ struct fs_grid {
             predicted = std::max(1, s.iterations);
         }
         project_end(dt);
-        return project_result(read_scalars(), tol);
+        return project_result(read_scalars(), tol, dt);
     }
 
     // This grid's entry of a batched solve (the iteration body of iteration_kernels()).
@@ -1659,6 +1667,22 @@ fs_status fs_set_flexible_cg(fs_grid* grid, int enabled)
             grid->flexible_cg = enabled != 0;
             grid->drop_graph();
         }
+    });
+}
+
+fs_status fs_set_relative_tolerance(fs_grid* grid, int enabled)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        grid->relative_tol = enabled != 0;
+    });
+}
+
+fs_status fs_get_relative_tolerance(const fs_grid* grid, int* enabled)
+{
+    return guard([&] {
+        need(grid != nullptr && enabled != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
+        *enabled = grid->relative_tol ? 1 : 0;
     });
 }
 
@@ -2846,7 +2870,7 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             for (int k = 0; k < nb; ++k) {
                 const int i = batched[(size_t)k];
                 const PcgScalars sc = ctx->batch_pinned[k];
-                results[i] = grids[i]->project_result(sc, divergence_tolerances[i]);
+                results[i] = grids[i]->project_result(sc, divergence_tolerances[i], dt);
                 grids[i]->predicted = std::max(1, sc.iterations);
             }
         }

@@ -67,11 +67,12 @@ __device__ __forceinline__ Tile4 tile4(int W, int H, int s)
 // ------------------------------------------------------------------------------------------
 // Scalar block initialisation (a kernel, so it is safe inside captured graphs).
 // ------------------------------------------------------------------------------------------
-__global__ void k_init_scalars(PcgScalars* S, double tol, int max_iterations)
+__global__ void k_init_scalars(PcgScalars* S, double tol, int max_iterations, double rel_tol)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         PcgScalars z{};
         z.tol = tol;
+        z.rel_tol = rel_tol;
         z.max_iterations = max_iterations;
         z.status = kRunning;
         z.restart = 1;
@@ -918,6 +919,8 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
         const double t = finish_max<kNT>(P, nb, 1, 0, lds);
         if (threadIdx.x == 0) {
             S->bmax = t;
+            if (S->rel_tol > 0.0)
+                S->tol = S->rel_tol * t;
             if (t <= S->tol)
                 S->status = kZeroRhs;
         }
@@ -1095,9 +1098,9 @@ inline unsigned linear_blocks(size_t n)
 // launchers
 // ------------------------------------------------------------------------------------------
 
-void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations)
+void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations, double rel_tol)
 {
-    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S, tol, max_iterations);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S, tol, max_iterations, rel_tol);
 }
 
 void launch_merge_failure(hipStream_t st, const PcgScalars* from, PcgScalars* S)

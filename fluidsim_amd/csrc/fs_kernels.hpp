@@ -91,7 +91,8 @@ size_t tail_layout(TailDesc& t);
 // Raises the dynamic-LDS limit of the tail kernel; false if the device refuses.
 bool tail_allow_lds(size_t bytes);
 
-void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations);
+// rel_tol > 0 replaces tol by rel_tol * max|b| once the rhs b is formed (a relative residual).
+void launch_init_scalars(hipStream_t st, PcgScalars* S, double tol, int max_iterations, double rel_tol = 0.0);
 // Copies a coarse-factorisation failure flagged in `from` into the solve's scalar block.
 void launch_merge_failure(hipStream_t st, const PcgScalars* from, PcgScalars* S);
 

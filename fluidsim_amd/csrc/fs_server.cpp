@@ -212,7 +212,8 @@ struct fs_server {
                ok(fs_set_preconditioner(g, (fs_preconditioner)p.preconditioner)) &&
                ok(fs_upload(g, ptr(r->fraction), ptr(r->u), ptr(r->v), nullptr, ptr(r->lmass), ptr(r->gmass))) &&
                (r->solid.empty() || ok(fs_set_solid_mask(g, r->solid.data()))) &&
-               ok(fs_set_multigrid_precision(g, (fs_mg_precision)p.multigrid_precision));
+               ok(fs_set_multigrid_precision(g, (fs_mg_precision)p.multigrid_precision)) &&
+               ok(fs_set_relative_tolerance(g, p.relative_tolerance));
     }
 
     void run_lane(Lane& lane)

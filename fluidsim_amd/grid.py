@@ -505,6 +505,17 @@ class FluidGridHIP:
         """Polak-Ribiere (flexible) CG beta instead of the reference's Fletcher-Reeves one."""
         call("fs_set_flexible_cg", self._h, int(bool(enabled)))
 
+    def setRelativeTolerance(self, enabled: bool):
+        """Read step/project tolerances as relative residuals: each projection's PCG stops at
+        max|b - A p| <= tol max|b| and is accepted at max|div u| <= tol max|b| dt / h^2
+        (fs_set_relative_tolerance)."""
+        call("fs_set_relative_tolerance", self._h, int(bool(enabled)))
+
+    def getRelativeTolerance(self) -> bool:
+        m = C.c_int()
+        call("fs_get_relative_tolerance", self._h, C.byref(m))
+        return bool(m.value)
+
     def getMultigridPrecision(self) -> int:
         m = C.c_int()
         call("fs_get_multigrid_precision", self._h, C.byref(m))

@@ -318,6 +318,8 @@ public:
     }
     // Multigrid level storage (no reference counterpart): FS_MG_FP32, FS_MG_FP16 or FS_MG_BF16.
     void setMultigridPrecision(fs_mg_precision mode) { check(fs_set_multigrid_precision(grid_, mode)); }
+    // step/project tolerances as relative residuals (fs_set_relative_tolerance)
+    void setRelativeTolerance(bool enabled) { check(fs_set_relative_tolerance(grid_, enabled ? 1 : 0)); }
     fs_mg_precision getMultigridPrecision() const
     {
         fs_mg_precision m = FS_MG_FP32;

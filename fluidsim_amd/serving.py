@@ -41,6 +41,7 @@ class Request:
     gas_mass: Optional[np.ndarray] = None
     multigrid_precision: int = _abi.FS_MG_FP32   # storage of the multigrid levels (fs_mg_precision)
     solid: Optional[np.ndarray] = None           # W * H obstacle mask, nonzero = solid
+    relative_tolerance: bool = False             # divergence_tolerance relative to each projection's max|b|
 
     def estimate_bytes(self) -> int:
         viscous = any(m > 0 for m in (self.liquid_viscosity or []) + (self.gas_viscosity or []))
@@ -137,7 +138,7 @@ class Server:
         req = FsRequest(W, H, float(r.cell_size), int(r.steps), float(r.dt), int(r.max_iterations),
                         float(r.divergence_tolerance), float(r.gravity_y), float(r.surface_tension), int(r.transport),
                         int(r.preconditioner), nl, ng, *ptrs, int(r.multigrid_precision),
-                        None if solid is None else solid.ctypes.data)
+                        None if solid is None else solid.ctypes.data, int(bool(r.relative_tolerance)))
         rid = C.c_longlong()
         call("fs_server_submit", self._h, C.byref(req), C.byref(rid))   # copies the arrays
         return rid.value

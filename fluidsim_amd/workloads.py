@@ -275,9 +275,9 @@ def serving_stream(seed: int = 5, count: int = 24, min_size: int = 128, max_size
     log-uniform in [min_size, max_size] (independent aspect), cell size 1/max(W, H), water 100 / air
     0.1, liquid filling 30-90% plus a drop, random vortex velocity, gravity 9.81, kinematic viscosity
     log-uniform in [1e-3, 1] (mu = nu rho per phase), surface tension U[0.01, 0.07], `steps` steps of
-    dt = half the capillary step limit (one physical substep per step), each projection to a maximum
-    divergence of 1e-4 (about 1e-6 of the initial divergence of these flows).  Returns [(arrival_s, kwargs)]
-    where kwargs build a `fluidsim_amd.serving.Request`."""
+    dt = half the capillary step limit (one physical substep per step), each projection to a relative
+    residual of 1e-6 (max|b - A p| <= 1e-6 max|b|, fs_set_relative_tolerance).  Returns
+    [(arrival_s, kwargs)] where kwargs build a `fluidsim_amd.serving.Request`."""
     rng = SplitMix64(seed)
     out = []
     t = 0.0
@@ -302,7 +302,7 @@ def serving_stream(seed: int = 5, count: int = 24, min_size: int = 128, max_size
         lm, gm = two_phase_masses(fraction, h)
         limit = 0.5 * math.sqrt((100.0 + 0.1) * h ** 3 / (2.0 * math.pi * sigma))
         out.append((t, dict(width=W, height=H, cell_size=h, steps=steps, dt=0.5 * limit, max_iterations=-1,
-                            divergence_tolerance=1e-4,
+                            divergence_tolerance=1e-6, relative_tolerance=True,
                             gravity_y=9.81, surface_tension=sigma, liquid_density=[100.0], gas_density=[0.1],
                             liquid_viscosity=[nu * 100.0], gas_viscosity=[nu * 0.1], liquid_fraction=fraction,
                             u=u, v=v, liquid_mass=lm, gas_mass=gm)))

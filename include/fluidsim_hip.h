@@ -178,6 +178,13 @@ fs_status fs_get_multigrid_precision(const fs_grid* grid, fs_mg_precision* mode)
 fs_status fs_set_flexible_cg(fs_grid* grid, int enabled);
 /* Number of solid cells of the current mask (0 without one). */
 fs_status fs_solid_cells(const fs_grid* grid, long long* count);
+/* Relative stopping rule (extension, no reference counterpart: the reference's tolerance is absolute,
+ * FluidGridCPU.cpp:1030-1124).  When enabled, the divergence_tolerance of fs_step / fs_project /
+ * fs_batch_project / fs_advance_frame is a relative residual: each projection's PCG stops at
+ * max|b - A p| <= tol * max|b| (b the mean-free pressure rhs) and the projection is accepted when
+ * max|div u| <= tol * max|b| * dt / h^2, i.e. tol times the divergence the projection started from. */
+fs_status fs_set_relative_tolerance(fs_grid* grid, int enabled);
+fs_status fs_get_relative_tolerance(const fs_grid* grid, int* enabled);
 
 /* upload(const State&) (FluidGridBase.h:118, FluidGridCPU.cpp:198-251).  NULL means an empty
  * vector (zeros).  Sizes are implied by the grid and the configured mixtures.  Resets the pressure
@@ -458,6 +465,7 @@ typedef struct fs_request {
     /* extensions (zero: FP32 levels, no obstacles) */
     int multigrid_precision;        /* fs_mg_precision */
     const unsigned char* solid;     /* W*H solid mask (fs_set_solid_mask) or NULL; copied by fs_server_submit */
+    int relative_tolerance;         /* nonzero: divergence_tolerance is relative (fs_set_relative_tolerance) */
 } fs_request;
 
 enum { FS_REQUEST_QUEUED = 0, FS_REQUEST_RUNNING = 1, FS_REQUEST_DONE = 2, FS_REQUEST_FAILED = 3 };

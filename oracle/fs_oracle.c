@@ -566,6 +566,7 @@ struct orc_grid {
     unsigned char* solid;  /* solid-mask extension: W*H, nonzero = solid cell (NULL: none) */
     orc_mg* mg;
     int density_valid, coeff_valid, mg_valid;
+    int relative_tol;      /* extension: tolerances are relative to max|rhs| (orc_set_relative_tolerance) */
 };
 
 orc_grid* orc_grid_create(int W, int H, double cell_size)
@@ -1101,6 +1102,14 @@ orc_result orc_project(orc_grid* g, double dt, int max_iterations, double tol)
             }
     }
     form_rhs(g, dt);
+    if (g->relative_tol) {
+        /* extension (no reference counterpart): stop at max|b - A p| <= tol max|b| and accept the
+         * divergence that residual bound allows */
+        double bmax = 0.0;
+        for (size_t i = 0; i < (size_t)W * H; ++i)
+            bmax = dmax(bmax, fabs(g->rhs[i]));
+        tol = tol * bmax / scale;
+    }
     result.iterations = solve_pressure(g, max_iterations, tol * scale, &result);
     if (result.iterations < 0) {
         orc_result failed;
@@ -1137,6 +1146,8 @@ orc_result orc_project(orc_grid* g, double dt, int max_iterations, double tol)
     result.converged = result.max_divergence <= tol;
     return result;
 }
+
+void orc_set_relative_tolerance(orc_grid* g, int enabled) { g->relative_tol = enabled != 0; }
 
 orc_result orc_force_project(orc_grid* g, double dt, int max_iterations, double tol)
 {

@@ -96,6 +96,9 @@ void orc_commit_edits(orc_grid* g, int count, const int* x, const int* y, const 
 orc_result orc_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
 /* Gravity followed by projection: step() with transport and surface tension off. */
 orc_result orc_force_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
+/* Extension matching fs_set_relative_tolerance: tolerances become relative, the PCG stopping at
+ * max|b - A p| <= tol max|b| and the projection accepted at max|div u| <= tol max|b| dt / h^2. */
+void orc_set_relative_tolerance(orc_grid* g, int enabled);
 /* Semi-Lagrangian transport mode (build-defined; the reference has none), then gravity and projection. */
 void orc_advect_semi_lagrangian(orc_grid* g, double dt);
 orc_result orc_step_semi_lagrangian(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);

@@ -73,6 +73,7 @@ def lib():
         "orc_grid_destroy": (None, [vp]),
         "orc_set_origin": (None, [vp, C.c_double, C.c_double]),
         "orc_set_gravity": (None, [vp, C.c_double]),
+        "orc_set_relative_tolerance": (None, [vp, C.c_int]),
         "orc_set_preconditioner": (None, [vp, C.c_int]),
         "orc_configure": (C.c_int, [vp, C.c_int, _dp, C.c_int, _dp]),
         "orc_upload": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -238,6 +239,9 @@ class OracleGrid:
 
     def set_gravity(self, gy):
         lib().orc_set_gravity(self._h, gy)
+
+    def set_relative_tolerance(self, enabled=True):
+        lib().orc_set_relative_tolerance(self._h, int(bool(enabled)))
 
     def set_preconditioner(self, kind):
         lib().orc_set_preconditioner(self._h, {"diagonal": 0, "multigrid": 1}[kind] if isinstance(kind, str) else int(kind))
