@@ -3,10 +3,13 @@
  *
  * TEST INFRASTRUCTURE ONLY (the parity checker; never the product path).
  *
- * Every function runs the reference's single-thread order: the reference splits passes over
- * worker batches only when a pass has >= 65536 items and more than one thread is allowed
+ * Every function runs the reference's single-thread order by default: the reference splits passes
+ * over worker batches only when a pass has >= 65536 items and more than one thread is allowed
  * (FluidGridCPU.cpp:25-57, PressureMultigrid.cpp:13-21); with one thread each pass is one
- * batch, so this file is the bitwise-reproducible mode.  Sums that the reference forms with
+ * batch, so that is the bitwise-reproducible mode.  orc_set_threads(n > 1) runs the projection's
+ * passes on the reference's worker counts (OpenMP; used to time the CPU baseline on the host's
+ * cores): element-wise passes give the same values, and reductions merge per-batch partials in batch
+ * order as reduceBatches does, over this file's own batch split.  Sums that the reference forms with
  * std::reduce follow libstdc++'s order (groups of four, ((a0+a1)+(a2+a3)) added to the
  * running total); 4-lane partial sums are reproduced where the reference writes them out.
  *
@@ -29,6 +32,27 @@ static double dclamp(double v, double lo, double hi) { return v < lo ? lo : (hi 
 
 static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
 
+/* ---- worker batches (Thread::forEachBatch) ---- */
+static int g_threads = 1;
+
+void orc_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
+int orc_get_threads(void) { return g_threads; }
+
+/* batchWorkers, FluidGridCPU.cpp:25-31: one batch below 65536 items of work, else up to 8 workers. */
+static int batch_workers(long long work) { return work < 65536 || g_threads <= 1 ? 1 : imin(8, g_threads); }
+
+/* forRows, PressureMultigrid.cpp:13-21: levels of at least 65536 cells on 4 workers (8 from 1M cells). */
+static int row_workers(int w, int h)
+{
+    const long long cells = (long long)w * h;
+    const int workers = imin(cells >= 1048576 ? 8 : 4, g_threads);
+    return cells >= 65536 && workers > 1 ? workers : 1;
+}
+
+/* First item of batch b when `count` items are split over `workers` batches (bitloop's own split is not
+ * available; with one worker the single batch is [0, count)). */
+static int batch_first(int count, int b, int workers) { return (int)((long long)count * b / workers); }
+
 /* std::reduce(first, last, 0.0) as libstdc++ evaluates it. */
 static double reduce_sum(const double* a, size_t n)
 {
@@ -44,14 +68,39 @@ static double reduce_sum(const double* a, size_t n)
     return init;
 }
 
+/* sumValues, FluidGridCPU.cpp:59-65: std::reduce per batch, batches merged in order. */
+static double sum_values(const double* a, size_t n)
+{
+    const int workers = batch_workers((long long)n);
+    double part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
+        const int first = batch_first((int)n, b, workers), last = batch_first((int)n, b + 1, workers);
+        part[b] = reduce_sum(a + first, (size_t)(last - first));
+    }
+    double total = part[0];
+    for (int b = 1; b < workers; ++b)
+        total = total + part[b];
+    return total;
+}
+
 /* FluidGridCPU.cpp:1488-1505 maxAbs: non-finite maps to +inf. */
 static double max_abs(const double* a, size_t n)
 {
-    double m = 0.0;
-    for (size_t i = 0; i < n; ++i) {
-        const double v = isfinite(a[i]) ? fabs(a[i]) : INFINITY;
-        m = dmax(m, v);
+    const int workers = batch_workers((long long)n);
+    double part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
+        double m = 0.0;
+        for (int i = batch_first((int)n, b, workers); i < batch_first((int)n, b + 1, workers); ++i) {
+            const double v = isfinite(a[i]) ? fabs(a[i]) : INFINITY;
+            m = dmax(m, v);
+        }
+        part[b] = m;
     }
+    double m = part[0];
+    for (int b = 1; b < workers; ++b)
+        m = dmax(m, part[b]);
     return m;
 }
 
@@ -227,7 +276,10 @@ void orc_mg_destroy(orc_mg* mg)
 static void rebuild_diagonal(orc_level* L)
 {
     const int s = L->stride;
-    for (int y = 0; y < L->h; ++y) {
+    const int workers = row_workers(L->w, L->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(L->h, b, workers); y < batch_first(L->h, b + 1, workers); ++y) {
         for (int x = 0, i = (y + 1) * s + 1; x < L->w; ++x, ++i) {
             const double diagonal = L->right[i - 1] + L->right[i] + L->down[i - s] + L->down[i];
             L->inv_diag[i] = diagonal > 0.0 ? 1.0 / diagonal : 0.0;
@@ -238,7 +290,10 @@ static void rebuild_diagonal(orc_level* L)
 /* coarsen, PressureMultigrid.cpp:142-202: parallel paths add; series resistances across density jumps. */
 static void coarsen(const orc_level* F, orc_level* C)
 {
-    for (int y = 0; y < C->h; ++y) {
+    int workers = row_workers(C->w, C->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(C->h, b, workers); y < batch_first(C->h, b + 1, workers); ++y) {
         for (int x = 0; x + 1 < C->w; ++x) {
             const int edge = 2 * x + 2;
             const double left = (F->cx[edge - 1] - C->cx[x]) / (F->cx[edge - 1] - F->cx[edge - 2]);
@@ -255,7 +310,10 @@ static void coarsen(const orc_level* F, orc_level* C)
             C->right[(y + 1) * C->stride + x + 1] = conductance;
         }
     }
-    for (int y = 0; y < C->h - 1; ++y) {
+    workers = row_workers(C->w, C->h - 1);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(C->h - 1, b, workers); y < batch_first(C->h - 1, b + 1, workers); ++y) {
         for (int x = 0; x < C->w; ++x) {
             const int edge = 2 * y + 2;
             const double up = (F->cy[edge - 1] - C->cy[y]) / (F->cy[edge - 1] - F->cy[edge - 2]);
@@ -324,7 +382,10 @@ static int factor_coarsest(orc_mg* mg)
 int orc_mg_rebuild(orc_mg* mg, const double* inv_u, const double* inv_v)
 {
     orc_level* F = &mg->lev[0];
-    for (int y = 0; y < F->h; ++y) {
+    const int workers = row_workers(F->w, F->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(F->h, b, workers); y < batch_first(F->h, b + 1, workers); ++y) {
         for (int x = 0, i = (y + 1) * F->stride + 1; x < F->w; ++x, ++i) {
             F->right[i] = inv_u[y * (F->w + 1) + x + 1];
             F->down[i] = inv_v[(y + 1) * F->w + x];
@@ -340,9 +401,12 @@ int orc_mg_rebuild(orc_mg* mg, const double* inv_u, const double* inv_v)
 static void smooth(orc_level* L, int reverse, int from_zero)
 {
     const int s = L->stride;
+    const int workers = row_workers(L->w, L->h);
     for (int pass = 0; pass < 2; ++pass) {
         const int color = reverse ? 1 - pass : pass;
-        for (int y = 0; y < L->h; ++y) {
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+        for (int b = 0; b < workers; ++b)
+        for (int y = batch_first(L->h, b, workers); y < batch_first(L->h, b + 1, workers); ++y) {
             for (int x = (color ^ (y & 1)), i = (y + 1) * s + x + 1; x < L->w; x += 2, i += 2) {
                 double value = L->rhs[i];
                 if (!(from_zero && pass == 0)) {
@@ -359,7 +423,10 @@ static void smooth(orc_level* L, int reverse, int from_zero)
 static void restrict_residual(orc_level* F, orc_level* C)
 {
     const int s = F->stride;
-    for (int y = 0; y < F->h; ++y) {
+    int workers = row_workers(F->w, F->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(F->h, b, workers); y < batch_first(F->h, b + 1, workers); ++y) {
         for (int x = 0, i = (y + 1) * s + 1; x < F->w; ++x, ++i) {
             const double center = F->sol[i];
             const double product = (F->right[i - 1] * (center - F->sol[i - 1]) +
@@ -371,7 +438,10 @@ static void restrict_residual(orc_level* F, orc_level* C)
             F->res[i] = F->inv_diag[i] > 0.0 ? F->rhs[i] - product : 0.0;
         }
     }
-    for (int y = 0; y < C->h; ++y) {
+    workers = row_workers(C->w, C->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(C->h, b, workers); y < batch_first(C->h, b + 1, workers); ++y) {
         for (int x = 0; x < C->w; ++x) {
             const orc_restr* rx = &F->rx[x];
             const orc_restr* ry = &F->ry[y];
@@ -391,7 +461,10 @@ static void restrict_residual(orc_level* F, orc_level* C)
 /* prolongCorrection, PressureMultigrid.cpp:278-298 */
 static void prolong_correction(const orc_level* C, orc_level* F)
 {
-    for (int y = 0; y < F->h; ++y) {
+    const int workers = row_workers(F->w, F->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(F->h, b, workers); y < batch_first(F->h, b + 1, workers); ++y) {
         const orc_interp* py = &F->py[y];
         const double* top = C->sol + (py->first + 1) * C->stride + 1;
         const double* bottom = C->sol + (py->second + 1) * C->stride + 1;
@@ -463,12 +536,17 @@ static void cycle(orc_mg* mg, int index)
 void orc_mg_apply(orc_mg* mg, const double* residual, double* correction, double residual_mean)
 {
     orc_level* F = &mg->lev[0];
-    for (int y = 0; y < F->h; ++y)
-        for (int x = 0; x < F->w; ++x)
-            F->rhs[(y + 1) * F->stride + x + 1] = residual[y * F->w + x] - residual_mean;
+    const int workers = row_workers(F->w, F->h);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+        for (int y = batch_first(F->h, b, workers); y < batch_first(F->h, b + 1, workers); ++y)
+            for (int x = 0; x < F->w; ++x)
+                F->rhs[(y + 1) * F->stride + x + 1] = residual[y * F->w + x] - residual_mean;
     cycle(mg, 0);
-    for (int y = 0; y < F->h; ++y)
-        memcpy(correction + (size_t)y * F->w, F->sol + (y + 1) * F->stride + 1, sizeof(double) * (size_t)F->w);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+        for (int y = batch_first(F->h, b, workers); y < batch_first(F->h, b + 1, workers); ++y)
+            memcpy(correction + (size_t)y * F->w, F->sol + (y + 1) * F->stride + 1, sizeof(double) * (size_t)F->w);
 }
 
 int orc_mg_levels(const orc_mg* mg) { return mg->nlev; }
@@ -703,12 +781,15 @@ static void rebuild_density(orc_grid* g)
         return;
     const size_t N = (size_t)g->W * g->H;
     const double inverse_cell_volume = 1.0 / (g->h * g->h);
-    for (size_t i = 0; i < N; ++i) {
-        double total = 0.0;
-        for (int k = 0; k < g->nl; ++k) total += g->lmass[(size_t)k * N + i];
-        for (int k = 0; k < g->ng; ++k) total += g->gmass[(size_t)k * N + i];
-        g->density[i] = total * inverse_cell_volume;
-    }
+    const int workers = batch_workers((long long)N);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+        for (size_t i = (size_t)batch_first((int)N, b, workers); i < (size_t)batch_first((int)N, b + 1, workers); ++i) {
+            double total = 0.0;
+            for (int k = 0; k < g->nl; ++k) total += g->lmass[(size_t)k * N + i];
+            for (int k = 0; k < g->ng; ++k) total += g->gmass[(size_t)k * N + i];
+            g->density[i] = total * inverse_cell_volume;
+        }
     g->density_valid = 1;
 }
 
@@ -719,7 +800,10 @@ static void rebuild_coefficients(orc_grid* g)
         return;
     rebuild_density(g);
     const int W = g->W, H = g->H;
-    for (int y = 0; y < H; ++y) {
+    const int workers = batch_workers((long long)H * W);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(H, b, workers); y < batch_first(H, b + 1, workers); ++y) {
         const double* row = g->density + (size_t)y * W;
         double* hz = g->inv_u + (size_t)y * (W + 1);
         double* vt = g->inv_v + (size_t)y * W;
@@ -744,7 +828,9 @@ static void rebuild_coefficients(orc_grid* g)
                     g->inv_v[(size_t)y * W + x] = 0.0;
             }
     }
-    for (int y = 0; y < H; ++y) {
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(H, b, workers); y < batch_first(H, b + 1, workers); ++y) {
         const double* hz = g->inv_u + (size_t)y * (W + 1);
         const double* vt = g->inv_v + (size_t)y * W;
         double* d = g->inv_diag + (size_t)y * W;
@@ -813,26 +899,40 @@ void orc_add_vortex(orc_grid* g, double cx, double cy, double radius, double str
 double orc_max_divergence(const orc_grid* g)
 {
     const int W = g->W, H = g->H;
-    double maximum = 0.0;
-    for (int y = 0; y < H; ++y) {
-        const double* hz = g->u + (size_t)y * (W + 1);
-        const double* vt = g->v + (size_t)y * W;
-        for (int x = 0; x < W; ++x) {
-            const double divergence = ((hz[x + 1] - hz[x]) + (vt[x + W] - vt[x])) / g->h;
-            if (!isfinite(divergence))
-                return INFINITY;
-            maximum = dmax(maximum, fabs(divergence));
+    const int workers = batch_workers((long long)H * W);
+    double part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
+        double maximum = 0.0;
+        for (int y = batch_first(H, b, workers); y < batch_first(H, b + 1, workers) && isfinite(maximum); ++y) {
+            const double* hz = g->u + (size_t)y * (W + 1);
+            const double* vt = g->v + (size_t)y * W;
+            for (int x = 0; x < W; ++x) {
+                const double divergence = ((hz[x + 1] - hz[x]) + (vt[x + W] - vt[x])) / g->h;
+                if (!isfinite(divergence)) {
+                    maximum = INFINITY;
+                    break;
+                }
+                maximum = dmax(maximum, fabs(divergence));
+            }
         }
+        part[b] = maximum;
     }
+    double maximum = part[0];
+    for (int b = 1; b < workers; ++b)
+        maximum = dmax(maximum, part[b]);
     return maximum;
 }
 
 /* removeMean, FluidGridCPU.cpp:1478-1486 */
 static void remove_mean(double* a, size_t n)
 {
-    const double mean = reduce_sum(a, n) / (double)n;
-    for (size_t i = 0; i < n; ++i)
-        a[i] -= mean;
+    const double mean = sum_values(a, n) / (double)n;
+    const int workers = batch_workers((long long)n);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+        for (int i = batch_first((int)n, b, workers); i < batch_first((int)n, b + 1, workers); ++i)
+            a[i] -= mean;
 }
 
 /* applyPressureOperator, FluidGridCPU.cpp:1507-1598: q = A p, returns p.q in the reference's order. */
@@ -859,8 +959,14 @@ static double apply_operator(const orc_grid* g, const double* in, double* out)
         return dot;
     }
     const int stride = W;
+    /* reduceBatches(height - 2, applyRows, plus, width): rows 1 .. H-2 in batches, each with its own
+     * four lanes, merged in batch order */
+    const int workers = batch_workers((long long)(H - 2) * W);
+    double part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
     double dots[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int y = 1; y < H - 1; ++y) {
+    for (int y = batch_first(H - 2, b, workers) + 1; y < batch_first(H - 2, b + 1, workers) + 1; ++y) {
         const double* row = in + (size_t)y * stride;
         const double* above = row - stride;
         const double* below = row + stride;
@@ -884,7 +990,11 @@ static double apply_operator(const orc_grid* g, const double* in, double* out)
             dots[0] += ORC_APPLY(x);
 #undef ORC_APPLY
     }
-    double dot = (dots[0] + dots[1]) + (dots[2] + dots[3]);
+    part[b] = (dots[0] + dots[1]) + (dots[2] + dots[3]);
+    }
+    double dot = part[0];
+    for (int b = 1; b < workers; ++b)
+        dot = dot + part[b];
     for (int x = 0; x < W; ++x) {
         dot += ORC_CELL(x, 0);
         dot += ORC_CELL(x, H - 1);
@@ -905,12 +1015,13 @@ double orc_apply_operator(orc_grid* g, const double* p, double* q)
 
 typedef struct { double sum, maximum; } orc_stats;
 
-/* 4-lane (sum, max) accumulation shared by updatePressureAndResidual and rebuildPressureResidual. */
-#define ORC_LANE_STATS(N, BODY)                                                          \
+/* 4-lane (sum, max) accumulation over [FIRST, LAST) shared by updatePressureAndResidual and
+ * rebuildPressureResidual (one batch of their reduceBatches). */
+#define ORC_LANE_STATS(FIRST, LAST, BODY)                                                \
     do {                                                                                 \
         double sums_[4] = {0, 0, 0, 0}, maxs_[4] = {0, 0, 0, 0};                         \
-        size_t i = 0;                                                                    \
-        for (; i + 3 < (N); i += 4) {                                                    \
+        size_t i = (FIRST);                                                              \
+        for (; i + 3 < (LAST); i += 4) {                                                 \
             for (int lane_ = 0; lane_ < 4; ++lane_) {                                    \
                 const size_t j = i + lane_;                                              \
                 const double value = (BODY);                                             \
@@ -920,7 +1031,7 @@ typedef struct { double sum, maximum; } orc_stats;
         }                                                                                \
         st.sum = (sums_[0] + sums_[1]) + (sums_[2] + sums_[3]);                          \
         st.maximum = dmax(dmax(maxs_[0], maxs_[1]), dmax(maxs_[2], maxs_[3]));           \
-        for (; i < (N); ++i) {                                                           \
+        for (; i < (LAST); ++i) {                                                        \
             const size_t j = i;                                                          \
             const double value = (BODY);                                                 \
             st.sum += value;                                                             \
@@ -937,7 +1048,20 @@ static orc_stats update_pressure_residual(orc_grid* g, double alpha)
     double* r = g->res;
     const double* d = g->dir;
     const double* q = g->prod;
-    ORC_LANE_STATS(N, (p[j] += alpha * d[j], r[j] -= alpha * q[j]));
+    const int workers = batch_workers((long long)N);
+    orc_stats part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
+        orc_stats st;
+        ORC_LANE_STATS((size_t)batch_first((int)N, b, workers), (size_t)batch_first((int)N, b + 1, workers),
+            (p[j] += alpha * d[j], r[j] -= alpha * q[j]));
+        part[b] = st;
+    }
+    st = part[0];
+    for (int b = 1; b < workers; ++b) {
+        st.sum = st.sum + part[b].sum;
+        st.maximum = dmax(st.maximum, part[b].maximum);
+    }
     if (!isfinite(st.sum))
         st.maximum = INFINITY;
     return st;
@@ -952,7 +1076,20 @@ static orc_stats rebuild_residual(orc_grid* g)
     double* r = g->res;
     const double* b = g->rhs;
     const double* q = g->prod;
-    ORC_LANE_STATS(N, (r[j] = b[j] - q[j]));
+    const int workers = batch_workers((long long)N);
+    orc_stats part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int k = 0; k < workers; ++k) {
+        orc_stats st;
+        ORC_LANE_STATS((size_t)batch_first((int)N, k, workers), (size_t)batch_first((int)N, k + 1, workers),
+            (r[j] = b[j] - q[j]));
+        part[k] = st;
+    }
+    st = part[0];
+    for (int k = 1; k < workers; ++k) {
+        st.sum = st.sum + part[k].sum;
+        st.maximum = dmax(st.maximum, part[k].maximum);
+    }
     if (!isfinite(st.sum))
         st.maximum = INFINITY;
     return st;
@@ -962,25 +1099,38 @@ static orc_stats rebuild_residual(orc_grid* g)
 static double apply_preconditioner(orc_grid* g, double residual_mean)
 {
     const size_t N = (size_t)g->W * g->H;
+    const int workers = batch_workers((long long)N);
     if (g->preconditioner == 1)
         orc_mg_apply(g->mg, g->res, g->cor, residual_mean);
-    else
-        for (size_t i = 0; i < N; ++i)
-            g->cor[i] = (g->res[i] - residual_mean) * g->inv_diag[i];
-    const double mean = reduce_sum(g->cor, N) / (double)N;
-    double dots[4] = {0, 0, 0, 0};
-    size_t i = 0;
-    for (; i + 3 < N; i += 4) {
-        for (int lane = 0; lane < 4; ++lane) {
-            g->cor[i + lane] -= mean;
-            dots[lane] += g->res[i + lane] * g->cor[i + lane];
+    else {
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+        for (int b = 0; b < workers; ++b)
+            for (int i = batch_first((int)N, b, workers); i < batch_first((int)N, b + 1, workers); ++i)
+                g->cor[i] = (g->res[i] - residual_mean) * g->inv_diag[i];
+    }
+    const double mean = sum_values(g->cor, N) / (double)N;
+    double part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
+        const size_t last = (size_t)batch_first((int)N, b + 1, workers);
+        double dots[4] = {0, 0, 0, 0};
+        size_t i = (size_t)batch_first((int)N, b, workers);
+        for (; i + 3 < last; i += 4) {
+            for (int lane = 0; lane < 4; ++lane) {
+                g->cor[i + lane] -= mean;
+                dots[lane] += g->res[i + lane] * g->cor[i + lane];
+            }
         }
+        double dot = (dots[0] + dots[1]) + (dots[2] + dots[3]);
+        for (; i < last; ++i) {
+            g->cor[i] -= mean;
+            dot += g->res[i] * g->cor[i];
+        }
+        part[b] = dot;
     }
-    double dot = (dots[0] + dots[1]) + (dots[2] + dots[3]);
-    for (; i < N; ++i) {
-        g->cor[i] -= mean;
-        dot += g->res[i] * g->cor[i];
-    }
+    double dot = part[0];
+    for (int b = 1; b < workers; ++b)
+        dot = dot + part[b];
     return dot;
 }
 
@@ -993,11 +1143,14 @@ static int solve_pressure(orc_grid* g, int max_iterations, double tol, orc_resul
         memset(g->p_prev, 0, sizeof(double) * N);
         return 0;
     }
-    for (size_t i = 0; i < N; ++i) {
-        const double latest = g->p[i];
-        g->p[i] = 2.0 * latest - g->p_prev[i];
-        g->p_prev[i] = latest;
-    }
+    const int workers = batch_workers((long long)N);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+        for (int i = batch_first((int)N, b, workers); i < batch_first((int)N, b + 1, workers); ++i) {
+            const double latest = g->p[i];
+            g->p[i] = 2.0 * latest - g->p_prev[i];
+            g->p_prev[i] = latest;
+        }
     remove_mean(g->p, N);
     const orc_stats initial = rebuild_residual(g);
     info->initial_residual = initial.maximum;
@@ -1035,8 +1188,10 @@ static int solve_pressure(orc_grid* g, int max_iterations, double tol, orc_resul
         const double beta = restart ? 0.0 : next / rz;
         if (!isfinite(beta))
             break;
-        for (size_t i = 0; i < N; ++i)
-            g->dir[i] = g->cor[i] + beta * g->dir[i];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+        for (int b = 0; b < workers; ++b)
+            for (int i = batch_first((int)N, b, workers); i < batch_first((int)N, b + 1, workers); ++i)
+                g->dir[i] = g->cor[i] + beta * g->dir[i];
         rz = next;
     }
     remove_mean(g->p, N);
@@ -1049,21 +1204,34 @@ static void form_rhs(orc_grid* g, double dt)
     const int W = g->W, H = g->H;
     const size_t N = (size_t)W * H;
     const double scale = g->h * g->h / dt;
-    double sum = 0.0;
-    for (int y = 0; y < H; ++y) {
-        const double* hz = g->u + (size_t)y * (W + 1);
-        const double* vt = g->v + (size_t)y * W;
-        double* rhs = g->rhs + (size_t)y * W;
-        for (int x = 0; x < W; ++x) {
-            const double du = hz[x + 1] - hz[x];
-            const double dv = vt[x + W] - vt[x];
-            rhs[x] = -(du + dv) / g->h * scale;
-            sum += rhs[x];
+    /* reduceBatches(height, ..., width): a running sum per batch of rows, merged in batch order */
+    int workers = batch_workers((long long)H * W);
+    double part[8];
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b) {
+        double sum = 0.0;
+        for (int y = batch_first(H, b, workers); y < batch_first(H, b + 1, workers); ++y) {
+            const double* hz = g->u + (size_t)y * (W + 1);
+            const double* vt = g->v + (size_t)y * W;
+            double* rhs = g->rhs + (size_t)y * W;
+            for (int x = 0; x < W; ++x) {
+                const double du = hz[x + 1] - hz[x];
+                const double dv = vt[x + W] - vt[x];
+                rhs[x] = -(du + dv) / g->h * scale;
+                sum += rhs[x];
+            }
         }
+        part[b] = sum;
     }
+    double sum = part[0];
+    for (int b = 1; b < workers; ++b)
+        sum = sum + part[b];
     const double mean = sum / (double)N;
-    for (size_t i = 0; i < N; ++i)
-        g->rhs[i] -= mean;
+    workers = batch_workers((long long)N);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+        for (int i = batch_first((int)N, b, workers); i < batch_first((int)N, b + 1, workers); ++i)
+            g->rhs[i] -= mean;
 }
 
 void orc_pressure_rhs(orc_grid* g, double dt, double* rhs)
@@ -1118,7 +1286,10 @@ orc_result orc_project(orc_grid* g, double dt, int max_iterations, double tol)
         failed.physical_steps = 1;
         return failed;
     }
-    for (int y = 0; y < H; ++y) {
+    const int workers = batch_workers((long long)H * W);
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(H, b, workers); y < batch_first(H, b + 1, workers); ++y) {
         const double* row = g->p + (size_t)y * W;
         double* hz = g->u + (size_t)y * (W + 1);
         const double* hw = g->inv_u + (size_t)y * (W + 1);

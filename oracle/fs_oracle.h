@@ -96,6 +96,12 @@ void orc_commit_edits(orc_grid* g, int count, const int* x, const int* y, const 
 orc_result orc_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
 /* Gravity followed by projection: step() with transport and surface tension off. */
 orc_result orc_force_project(orc_grid* g, double dt, int max_iterations, double divergence_tolerance);
+/* Worker threads for the projection's passes (default 1: the reference's single-thread, bitwise
+ * reproducible order).  With n > 1 the passes use the reference's worker counts (up to 8 on passes of at
+ * least 65536 items, FluidGridCPU.cpp:25-31; 4 or 8 on multigrid levels of at least 65536 cells,
+ * PressureMultigrid.cpp:13-21), at most n.  Process-wide. */
+void orc_set_threads(int threads);
+int orc_get_threads(void);
 /* Extension matching fs_set_relative_tolerance: tolerances become relative, the PCG stopping at
  * max|b - A p| <= tol max|b| and the projection accepted at max|div u| <= tol max|b| dt / h^2. */
 void orc_set_relative_tolerance(orc_grid* g, int enabled);

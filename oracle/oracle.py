@@ -74,6 +74,8 @@ def lib():
         "orc_set_origin": (None, [vp, C.c_double, C.c_double]),
         "orc_set_gravity": (None, [vp, C.c_double]),
         "orc_set_relative_tolerance": (None, [vp, C.c_int]),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_get_threads": (C.c_int, []),
         "orc_set_preconditioner": (None, [vp, C.c_int]),
         "orc_configure": (C.c_int, [vp, C.c_int, _dp, C.c_int, _dp]),
         "orc_upload": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -390,6 +392,15 @@ class OracleGrid:
 
 
 VISC_OK, VISC_NONFINITE, VISC_BREAKDOWN, VISC_NOT_CONVERGED = 0, 1, 2, 3
+
+
+def set_threads(threads):
+    """Worker threads of the projection's passes (1: the reference's single-thread order)."""
+    lib().orc_set_threads(int(threads))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def viscosity_solve(width, height, cell_size, material, u, v, dt, tolerance=1e-4, maximum=-1):
