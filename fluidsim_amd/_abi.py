@@ -184,6 +184,8 @@ SIGNATURES = {
     "fs_set_component_viscosity": [_vp, C.c_int, C.c_int, C.c_double],
     "fs_viscosity_solve": [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _ip,
                            _dp],
+    "fs_viscosity_solve_noalloc": [_vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _ip,
+                           _dp],
     "fs_advect_semi_lagrangian": [_vp, C.c_double],
     "fs_snapshot_save": [_vp],
     "fs_snapshot_restore": [_vp],

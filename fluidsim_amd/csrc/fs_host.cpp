@@ -1437,6 +1437,7 @@ fs_status fs_context_create_ex(int device, int flags, fs_context** out)
         }
         HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
         HIPC(hipHostMalloc((void**)&ctx->pinned_value, sizeof(double), hipHostMallocDefault));
+        viscosity_scratch_setup(device);   // this device's copy of the allocation-free solve's scratch
         HIPC(hipEventCreate(&ctx->ev0));
         HIPC(hipEventCreate(&ctx->ev1));
         *out = ctx.release();
@@ -2258,6 +2259,31 @@ fs_status fs_viscosity_solve(fs_context* ctx, int width, int height, double cell
         HIPC(hipMemcpyAsync(u, du.p, NU * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPC(hipMemcpyAsync(v, dv.p, NV * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPC(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+fs_status fs_viscosity_solve_noalloc(fs_context* ctx, int width, int height, double cell_size,
+    const double* material, double* u, double* v, double dt, double tolerance, int max_iterations, int* iterations,
+    double* relative_residual)
+{
+    return guard([&] {
+        need(ctx != nullptr && material != nullptr && u != nullptr && v != nullptr, FS_E_INVALID_ARGUMENT,
+            "null pointer");
+        need(width > 0 && height > 0 && cell_size > 0.0, FS_E_INVALID_ARGUMENT, "invalid grid shape");
+        need(std::isfinite(tolerance) && tolerance >= 0.0, FS_E_INVALID_ARGUMENT, "invalid tolerance");
+        need((long long)width * height <= (1LL << 26) && viscosity_scratch_bytes(width, height) <= kViscosityScratchBytes,
+            FS_E_INVALID_ARGUMENT, "grid too large for the allocation-free viscosity solve's 64 MiB scratch");
+        set_device(ctx);
+        int it = 0;
+        double rel = 0.0;
+        const int budget = max_iterations < 0 ? kViscosityBudget : max_iterations;
+        const int status = viscosity_solve_scratch(ctx->device, ctx->stream, width, height, u, v, material, dt,
+            cell_size, tolerance, budget, &it, &rel);
+        if (iterations)
+            *iterations = it;
+        if (relative_residual)
+            *relative_residual = rel;
+        raise_viscosity(status, rel);
     });
 }
 

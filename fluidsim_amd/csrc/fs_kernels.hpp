@@ -266,6 +266,17 @@ void launch_viscosity_blend(hipStream_t st, int N, double cell_size, int nl, int
     const double* gm, const VisMixture& mix, double* material);
 struct ViscositySolver;   // device workspace of one grid size
 ViscositySolver* viscosity_create(int width, int height);
+// Allocation-free solve (fs_viscosity_solve_noalloc): the workspace is a file-scope device array of
+// kViscosityScratchBytes per device.  viscosity_scratch_setup (with `device` current) resolves that
+// device's copy once; viscosity_scratch_bytes is the workspace a W x H solve needs;
+// viscosity_solve_scratch runs viscosity_solve's algorithm in it on device u, v and material, one call
+// at a time per device, and returns after the stream has drained.
+constexpr size_t kViscosityScratchBytes = (size_t)64 << 20;
+void* viscosity_scratch_setup(int device);
+size_t viscosity_scratch_bytes(int width, int height);
+int viscosity_solve_scratch(int device, hipStream_t st, int width, int height, double* u, double* v,
+    const double* material, double dt, double spacing, double tol, int maximum, int* iterations,
+    double* relative_residual);
 void viscosity_destroy(ViscositySolver* s);
 size_t viscosity_bytes(const ViscositySolver* s);
 int viscosity_levels(const ViscositySolver* s);

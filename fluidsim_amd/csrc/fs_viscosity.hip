@@ -681,18 +681,27 @@ void vhip(hipError_t e, const char* what)
 template <class T> struct VBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;   // false: placed in memory the buffer does not own (a scratch arena)
     VBuf() = default;
     VBuf(const VBuf&) = delete;
     VBuf& operator=(const VBuf&) = delete;
     ~VBuf() { release(); }
     void release()
     {
-        if (p) {
+        if (p && owned) {
             std::lock_guard<std::mutex> lock(device_setup_mutex());
             (void)hipFree(p);
         }
         p = nullptr;
         n = 0;
+        owned = true;
+    }
+    void borrow(T* ptr, size_t count)
+    {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
     }
     size_t alloc(size_t count)
     {
@@ -1113,50 +1122,89 @@ __global__ __launch_bounds__(kBVT) void k_batch_vis(const BatchVisDesc* descs)
 
 }  // namespace
 
+// Device memory handed out in 256-byte aligned pieces from one block the arena does not own.  With a
+// null base it only measures (every piece is null).
+struct ScratchArena {
+    unsigned char* base = nullptr;
+    size_t capacity = 0;
+    size_t used = 0;
+    template <class T> T* take(size_t count)
+    {
+        const size_t offset = (used + 255) & ~(size_t)255;
+        used = offset + count * sizeof(T);
+        if (base && used > capacity)
+            throw std::runtime_error("viscosity scratch arena exhausted");
+        return base ? (T*)(base + offset) : nullptr;
+    }
+};
+
 struct ViscositySolver {
     struct Level {
         int w, h;
         VBuf<float4> coef;
         VBuf<float2> mat, b, x, xn, r;
     };
+    // A grid halves until w * h <= 64: at most 31 times for int dimensions.  Fixed storage, so a solver
+    // placed in a scratch arena makes no allocation at all.
+    struct LevelList {
+        static constexpr int kCapacity = 32;
+        Level items[kCapacity];
+        int count = 0;
+        size_t size() const { return (size_t)count; }
+        Level* operator[](size_t i) { return &items[i]; }
+        const Level* operator[](size_t i) const { return &items[i]; }
+        Level* add() { return &items[count++]; }
+    };
     int W, H;
-    std::vector<Level*> levels;
+    LevelList levels;
     VBuf<float2> v0, r0, r, d, q, t;
     VBuf<double> material, partials;
     VBuf<VisScalars> S;
     VisScalars* pinned = nullptr;
+    VisScalars landing{};           // read-back slot of a solver placed in an arena (no pinned memory)
+    ScratchArena* arena = nullptr;  // non-null: every device buffer is a piece of the arena
     size_t bytes = 0;
 
-    ViscositySolver(int w, int h) : W(w), H(h)
+    template <class T> void place(VBuf<T>& b, size_t count)
+    {
+        if (arena)
+            b.borrow(arena->template take<T>(count), count);
+        else
+            bytes += b.alloc(count);
+    }
+
+    // With an arena: the buffers are pieces of it (zeroed by the caller), no per-grid material buffer, no
+    // pinned landing slot and no captured graph.
+    ViscositySolver(int w, int h, ScratchArena* in = nullptr) : W(w), H(h), arena(in)
     {
         for (;;) {
-            Level* L = new Level();
+            Level* L = levels.add();
             L->w = w;
             L->h = h;
             const size_t ns = (size_t)(w + 1) * (h + 1);
-            bytes += L->coef.alloc(ns);
-            bytes += L->mat.alloc((size_t)w * h);
-            bytes += L->b.alloc(ns);
-            bytes += L->x.alloc(ns);
-            bytes += L->xn.alloc(ns);
-            bytes += L->r.alloc(ns);
-            levels.push_back(L);
+            place(L->coef, ns);
+            place(L->mat, (size_t)w * h);
+            place(L->b, ns);
+            place(L->x, ns);
+            place(L->xn, ns);
+            place(L->r, ns);
             if (w * h <= 64)
                 break;
             w = (w + 1) / 2;
             h = (h + 1) / 2;
         }
         const size_t ns = (size_t)(W + 1) * (H + 1);
-        bytes += v0.alloc(ns);
-        bytes += r0.alloc(ns);
-        bytes += r.alloc(ns);
-        bytes += d.alloc(ns);
-        bytes += q.alloc(ns);
-        bytes += t.alloc(ns);
-        bytes += material.alloc(2 * (size_t)W * H);
-        bytes += partials.alloc(4 * (size_t)std::max(slot_blocks(W, H), 1u));
-        bytes += S.alloc(1);
-        {
+        place(v0, ns);
+        place(r0, ns);
+        place(r, ns);
+        place(d, ns);
+        place(q, ns);
+        place(t, ns);
+        if (!arena)
+            place(material, 2 * (size_t)W * H);
+        place(partials, 4 * (size_t)std::max(slot_blocks(W, H), 1u));
+        place(S, 1);
+        if (!arena) {
             std::lock_guard<std::mutex> lock(device_setup_mutex());
             vhip(hipHostMalloc((void**)&pinned, sizeof(VisScalars), hipHostMallocDefault), "hipHostMalloc");
         }
@@ -1165,8 +1213,6 @@ struct ViscositySolver {
     ~ViscositySolver()
     {
         drop_batch_graph();
-        for (Level* L : levels)
-            delete L;
         if (pinned) {
             std::lock_guard<std::mutex> lock(device_setup_mutex());
             (void)hipHostFree(pinned);
@@ -1183,9 +1229,10 @@ struct ViscositySolver {
 
     VisScalars read(hipStream_t st)
     {
-        vhip(hipMemcpyAsync(pinned, S.p, sizeof(VisScalars), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        VisScalars* slot = pinned ? pinned : &landing;
+        vhip(hipMemcpyAsync(slot, S.p, sizeof(VisScalars), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
         vhip(hipStreamSynchronize(st), "hipStreamSynchronize");
-        return *pinned;
+        return *slot;
     }
 
     void finish(hipStream_t st, int nb, int op, double tol2, float dt = 0.0f)
@@ -1265,6 +1312,11 @@ struct ViscositySolver {
 
     void run_batch(hipStream_t st, float2* delta)
     {
+        if (arena) {   // instantiating a graph allocates: the same four iterations, launched one by one
+            for (int i = 0; i < 4; ++i)
+                iteration(st, delta);
+            return;
+        }
         if (!batch_exec || batch_stream != st) {
             drop_batch_graph();
             std::lock_guard<std::mutex> lock(device_setup_mutex());
@@ -1415,6 +1467,75 @@ struct ViscositySolver {
         return 3;   // not converged within the caller's budget
     }
 };
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free solve: the solver's workspace lives in a file-scope device array, one per GPU.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// Each device that loads this library holds its own copy until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it names a stand-in, not device memory): it uses the
+// per-device address hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[kViscosityScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;                         // guards the address table
+void* g_scratch_address[kScratchDevices] = {};
+std::mutex g_scratch_use[kScratchDevices];          // one solve at a time per device
+
+__global__ __launch_bounds__(kVT) void k_zero_words(unsigned* p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kVT + threadIdx.x; i < n; i += (size_t)gridDim.x * kVT)
+        p[i] = 0u;
+}
+
+}  // namespace
+
+void* viscosity_scratch_setup(int device)
+{
+    if (device < 0 || device >= kScratchDevices)
+        throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        void* address = nullptr;
+        vhip(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+        if (!address)
+            throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_address[device] = address;
+    }
+    return g_scratch_address[device];
+}
+
+size_t viscosity_scratch_bytes(int width, int height)
+{
+    ScratchArena measure;
+    ViscositySolver probe(width, height, &measure);
+    return measure.used;
+}
+
+int viscosity_solve_scratch(int device, hipStream_t st, int width, int height, double* u, double* v,
+    const double* material, double dt, double spacing, double tol, int maximum, int* iterations,
+    double* relative_residual)
+{
+    void* base = viscosity_scratch_setup(device);
+    const size_t need = viscosity_scratch_bytes(width, height);
+    if (need > kViscosityScratchBytes)
+        throw std::invalid_argument("grid too large for the allocation-free viscosity solve's scratch");
+    std::lock_guard<std::mutex> serial(g_scratch_use[device]);
+    // every call starts from zeroed memory, as freshly allocated buffers do (the last call's values
+    // stay in the array otherwise)
+    const size_t words = (need + 3) / 4;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((words + kVT - 1) / kVT, 4096)), dim3(kVT), 0,
+        st, (unsigned*)base, words);
+    ScratchArena arena;
+    arena.base = (unsigned char*)base;
+    arena.capacity = kViscosityScratchBytes;
+    ViscositySolver solver(width, height, &arena);
+    const int status = solver.solve(st, u, v, material, dt, spacing, tol, maximum, iterations, relative_residual);
+    // the last kernels (velocity unpack) read the scratch: done before another call may reuse it
+    vhip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return status;
+}
 
 ViscositySolver* viscosity_create(int width, int height) { return new ViscositySolver(width, height); }
 void viscosity_destroy(ViscositySolver* s) { delete s; }

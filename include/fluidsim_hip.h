@@ -229,6 +229,19 @@ fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations);
 fs_status fs_viscosity_solve(fs_context* ctx, int width, int height, double cell_size, const double* material,
     double* u, double* v, double dt, double tolerance, int max_iterations, int* iterations,
     double* relative_residual);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code that must not
+ * call hipMalloc, tight serving loops, harnesses that forbid allocation): fs_viscosity_solve on DEVICE
+ * arrays material (2*W*H), u ((W+1)*H) and v (W*(H+1)) with the same algorithm, stopping rule and results,
+ * making no allocation of any kind.  Its workspace is a static device array of 64 MiB per GPU (held until
+ * the process ends): about 123 bytes per cell, up to about 737 x 737 cells; larger grids return
+ * FS_E_INVALID_ARGUMENT.  The
+ * work runs on the context's stream, which the call drains before returning (so the caller orders its own
+ * writes of the inputs before the call).  Every call on a GPU shares that GPU's array, which keeps the last
+ * call's values (each call clears what it uses first); calls that overlapped on one GPU would race on it,
+ * so the library serialises them within a process. */
+fs_status fs_viscosity_solve_noalloc(fs_context* ctx, int width, int height, double cell_size,
+    const double* material, double* u, double* v, double dt, double tolerance, int max_iterations, int* iterations,
+    double* relative_residual);
 /* FluidGridCPU::applySurfaceTension(dt) (FluidGridCPU.cpp:862-926; private in the reference, exposed for
  * parity tests): capillary geometry of the current fractions, then u, v += dt w sigma kappa df / h. */
 fs_status fs_apply_surface_tension(fs_grid* grid, double dt);
