@@ -40,6 +40,15 @@ METRIC = "MGPCG solves/s to 1e-6 rel. residual (1024², mixed batch); p50/p99 st
 SMOOTH_BYTES_PER_CELL = 89
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_rb_multi_fine_post_1024_traffic.md")
+TRACE_PROFILE = os.path.join(HERE, "profiles", "bench_1024_kernel_trace.md")
+TAIL_KERNEL = 6   # fs_time_kernel id of the single-workgroup coarse tail
+# Algorithmic bytes per fine cell of one whole projection, by the per-iteration model: 15 FP32 CG vector
+# passes (operator + dot 4, update + max 6, dot + direction 5; s = 4 B) plus the V(4,4) cycle's
+# (10 nu + 5.5) c = 45.5 c per level over a hierarchy summing to 4/3 of the fine level (c = 4 B):
+# 60 + 242.7 = 302.7 B per PCG iteration and per V-cycle; one extra V-cycle, rz and direction start
+# the solve, and set-up, rhs, correction and the FP64 divergence check add about 65 B.
+SOLVE_BYTES_PER_CELL_ITERATION = 302.7
+SOLVE_SETUP_BYTES_PER_CELL = 65.0
 
 
 def measured_traffic(kernel, size):
@@ -59,6 +68,25 @@ def measured_traffic(kernel, size):
     return None
 
 
+def traced_kernels():
+    """(kernel, share of traced kernel time) rows of the committed kernel-trace summary of this benchmark,
+    largest first, or [] when there is none."""
+    try:
+        with open(TRACE_PROFILE) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return []
+    rows = []
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].startswith("`"):
+            try:
+                rows.append((cells[0].strip("`"), float(cells[4]) / 100.0))
+            except ValueError:
+                pass
+    return rows
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -70,33 +98,98 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(wl, tau, budget_s):
-    """The FP64 CPU oracle (single-threaded restatement of UniformCPU) on the same projection."""
+    """The FP64 CPU oracle (the restatement of UniformCPU) on the same projection, timed as the reference
+    runs it: its passes on min(8, host cores) workers (FluidGridCPU.cpp:25-31, PressureMultigrid.cpp:13-21),
+    and again on one thread.  Median of the timed projections after one warm-up, half the budget each."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O  # noqa: E402  (checker / baseline only)
 
     g = O.OracleGrid(wl.width, wl.height, wl.cell_size)
     g.configure(wl.liquid_density, wl.gas_density)
     g.set_gravity(wl.gravity_y)
-    times = []
-    iters = []
-    t_start = time.perf_counter()
-    while True:
-        g.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
-        t0 = time.perf_counter()
-        r = g.project(wl.dt, 200, tau)
-        times.append(time.perf_counter() - t0)
-        iters.append(r["iterations"])
-        if time.perf_counter() - t_start >= budget_s or len(times) >= 200:
-            break
-    med = float(np.median(times))
+
+    def timed(threads, budget):
+        O.set_threads(threads)
+        times, iters = [], []
+        t_start = time.perf_counter()
+        for k in range(201):
+            g.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+            t0 = time.perf_counter()
+            r = g.project(wl.dt, 200, tau)
+            if k > 0:   # the first projection is the warm-up
+                times.append(time.perf_counter() - t0)
+                iters.append(r["iterations"])
+            if len(times) >= 5 and time.perf_counter() - t_start >= budget:
+                break
+        O.set_threads(1)
+        return float(np.median(times)), len(times), int(np.median(iters))
+
+    cores = host_cores()
+    workers = min(8, cores)
+    med_n, count_n, it_n = timed(workers, 0.5 * budget_s)
+    med_1, count_1, it_1 = timed(1, 0.5 * budget_s)
     return {
-        "value": 1.0 / med,
+        "value": 1.0 / med_n,
         "unit": "solves/s",
-        "cores": 1,
+        "cores": workers,
         "kind": "port",
-        "sample": f"{len(times)} single-threaded FP64 projections of the same 1024x1024 input "
-                  f"(median {med * 1e3:.1f} ms, {int(np.median(iters))} PCG iterations) on {cpu_model()}",
+        "sample": f"{count_n} FP64 projections of the same {wl.width}x{wl.height} input on {workers} threads "
+                  f"(the reference's worker count min(8, cores); median {med_n * 1e3:.1f} ms, {it_n} PCG iterations) "
+                  f"on {cpu_model()}, {cores} cores available",
+        "single_thread": {"value": 1.0 / med_1, "unit": "solves/s", "cores": 1,
+                          "sample": f"{count_1} projections, median {med_1 * 1e3:.1f} ms, {it_1} PCG iterations"},
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
+    }
+
+
+def dam_break_leg(steps=100, seed=1):
+    """Config C1: the 128 x 128 dam break, `steps` full steps (reference transport, gravity, projection to
+    |div u| <= 1e-6), on the GPU and on the FP64 CPU oracle (16384 cells: the reference runs every pass
+    of this size on one worker), each timed end to end from the same initial state."""
+    from fluidsim_amd import workloads as WL
+    from fluidsim_amd import FluidGridHIP
+
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O  # noqa: E402  (baseline only)
+
+    wl = WL.dam_break(128, 128, seed=seed)
+    g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+    WL.load_into(g, wl)
+    g.step(wl.dt, 200, 1e-6)   # warm-up: hierarchy, graphs, tile choice
+    WL.load_into(g, wl)
+    iters = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = g.step(wl.dt, 200, 1e-6)
+        if not r.converged:
+            raise RuntimeError("C1 GPU step did not converge")
+        iters += r.iterations
+    gpu_s = time.perf_counter() - t0
+    g.destroy()
+    o = O.OracleGrid(wl.width, wl.height, wl.cell_size)
+    o.configure(wl.liquid_density, wl.gas_density)
+    o.set_gravity(wl.gravity_y)
+    o.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(wl.dt, 200, 1e-6)
+    cpu_s = time.perf_counter() - t0
+    return {
+        "workload": f"dam_break 128x128, {steps} steps (PLIC transport, gravity, MG-PCG to |div u| <= 1e-6), seed {seed}",
+        "gpu_steps_per_s": steps / gpu_s,
+        "gpu_pcg_iterations_mean": iters / steps,
+        "cpu_steps_per_s": steps / cpu_s,
+        "cpu_kind": "port (FP64 oracle, one thread)",
     }
 
 
@@ -151,8 +244,11 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
             if failed:
                 raise RuntimeError(f"served request failed: {failed[0]}")
             samples.append((served_s, st, [r.queue_ms for r in reps]))
+            setup_ms = sum(r.setup_ms for r in reps)
+            teardown_ms = sum(r.teardown_ms for r in reps)
+            samples[-1] = samples[-1] + (setup_ms, teardown_ms)
         samples.sort(key=lambda v: v[0])
-        served_s, st, queue = samples[len(samples) // 2]
+        served_s, st, queue, setup_ms, teardown_ms = samples[len(samples) // 2]
         runs.append({
             "rate_per_s": rate,
             "arrival_span_s": stream[-1][0],
@@ -162,6 +258,8 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
             "step_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
             "queue_ms": {"p50": float(np.percentile(queue, 50)), "p99": float(np.percentile(queue, 99))},
             "peak_resident_bytes": st["peak_resident_bytes"],
+            "lane_ms_creating_instances": setup_ms,
+            "lane_ms_retiring_instances": teardown_ms,
         })
     return {
         "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each (PLIC transport, "
@@ -496,11 +594,15 @@ def main():
     tx, ty, nt = variants[post_variant]
     fine_kernel = f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float>"
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
+    tail_ms = grid.timeKernel(TAIL_KERNEL, args.kernel_reps)
     sweep_bytes = SMOOTH_BYTES_PER_CELL * wl.width * wl.height
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     # Reference point: a plain FP32 copy of 64 MiB into another 64 MiB buffer on the same stream.
     copy_ms = grid.timeKernel(10, 20)
     copy_bytes = 2 * 4 * (16 << 20)
+    traffic = measured_traffic(fine_kernel, args.size)
+    cells = wl.width * wl.height
+    solve_bytes = cells * (SOLVE_BYTES_PER_CELL_ITERATION * (float(np.median(iters)) + 1.0) + SOLVE_SETUP_BYTES_PER_CELL)
 
     out = {
         "metric": METRIC,
@@ -533,12 +635,44 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(fine_kernel, args.size),
+            "traffic": traffic,
             "avg_launch_us": sweep_ms * 1e3,
             "algorithmic_bytes_per_launch": sweep_bytes,
             "copy_kernel_gbs": copy_bytes / (copy_ms * 1e-3) / 1e9,
         },
+        "solve_roofline": {
+            "bound": "hbm",
+            "algorithmic_bytes_per_solve": solve_bytes,
+            "achieved": solve_bytes / (elapsed / args.steps) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": solve_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "model": f"{SOLVE_BYTES_PER_CELL_ITERATION} B per fine cell per PCG iteration (and per starting "
+                     f"V-cycle) + {SOLVE_SETUP_BYTES_PER_CELL:.0f} B set-up",
+        },
     }
+    if traffic:
+        # the fused smoother moves far fewer bytes than its model: what limits it is instruction issue
+        out["roofline"]["frac_of_measured_traffic"] = traffic / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+        out["roofline"]["limiter"] = (f"instruction issue: measured HBM traffic is {traffic / sweep_bytes:.2f} of the "
+                                      "model bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE)")
+    top = traced_kernels()
+    if top:
+        dom = {"kernel": top[0][0], "share_of_kernel_time": top[0][1], "source": os.path.relpath(TRACE_PROFILE, HERE)}
+        if top[0][0].startswith("k_mg_tail"):
+            # the single-workgroup tail: levels from the first untiled one down, V(4,4) at the multigrid
+            # model's (10 nu + 5.5) c = 182 B per level cell, plus the FP32 coarse inverse
+            shapes = [grid.multigridShape(l) for l in range(grid.multigridLevels())]
+            tail = [s for l, s in enumerate(shapes) if grid.multigridTiles(l)[0] < 0]
+            last = shapes[-1][0] * shapes[-1][1] - 1
+            tail_bytes = 182 * sum(w * h for w, h in tail[:-1]) + 4 * last * last
+            tail_us = tail_ms * 1e3
+            dom.update({"avg_launch_us": tail_us, "algorithmic_bytes_per_launch": tail_bytes,
+                        "achieved_gbs": tail_bytes / (tail_us * 1e-6) / 1e9,
+                        "frac": tail_bytes / (tail_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                        "limiter": "latency: one 1024-thread workgroup on one compute unit, about 60 "
+                                   "barrier-separated phases per launch"})
+        out["roofline"]["dominant_kernel_in_trace"] = dom
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
     if args.sl_steps > 0:

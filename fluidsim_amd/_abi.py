@@ -122,7 +122,7 @@ class FsRequestReport(C.Structure):
         ("viscosity_iterations", C.c_int), ("physical_steps", C.c_int), ("max_divergence", C.c_double),
         ("queue_ms", C.c_double), ("run_ms", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
         ("step_max_ms", C.c_double), ("estimated_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-        ("error", C.c_char * 256),
+        ("setup_ms", C.c_double), ("teardown_ms", C.c_double), ("error", C.c_char * 256),
     ]
 
 
@@ -132,7 +132,7 @@ class FsServerStats(C.Structure):
         ("failed", C.c_longlong), ("steps", C.c_longlong), ("queued", C.c_int), ("resident", C.c_int),
         ("budget_bytes", C.c_size_t), ("resident_bytes", C.c_size_t), ("peak_resident_bytes", C.c_size_t),
         ("elapsed_s", C.c_double), ("steps_per_s", C.c_double), ("step_p50_ms", C.c_double),
-        ("step_p99_ms", C.c_double),
+        ("step_p99_ms", C.c_double), ("setup_ms_total", C.c_double), ("teardown_ms_total", C.c_double),
     ]
 
 

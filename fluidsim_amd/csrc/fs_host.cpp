@@ -90,17 +90,20 @@ template <class T> struct DevBuf {
         p = nullptr;
         n = 0;
     }
-    // Allocates and zero-fills (halos of padded fields must start, and stay, zero).
-    size_t alloc(size_t count)
+    // Allocates and zero-fills (halos of padded fields must start, and stay, zero), releasing any previous
+    // allocation.  Returns the change in allocated bytes (negative when a larger buffer is replaced), which
+    // callers add to their byte count (unsigned wrap-around gives the right total).
+    long long alloc(size_t count)
     {
+        const long long before = (long long)(n * sizeof(T));
         release();
         n = count;
         if (count == 0)
-            return 0;
+            return -before;
         std::lock_guard<std::mutex> lock(device_setup_mutex());
         HIPC(hipMalloc(&p, count * sizeof(T)));
         HIPC(hipMemset(p, 0, count * sizeof(T)));
-        return count * sizeof(T);
+        return (long long)(count * sizeof(T)) - before;
     }
 };
 
@@ -1872,11 +1875,8 @@ fs_status fs_commit_edits(fs_grid* grid, int count, const fs_cell_edit* cells, c
         }
         list.resize(kept);
         set_device(g.ctx);
-        if (g.edit_cells.n < kept) {
-            if (g.edit_cells.n)
-                g.bytes -= g.edit_cells.n * sizeof(EditCell);
+        if (g.edit_cells.n < kept)
             g.bytes += g.edit_cells.alloc(kept);
-        }
         const SyncOnExit drain{g.st()};
         stage_upload(g.ctx, g.edit_cells.p, list.data(), kept * sizeof(EditCell));
         launch_commit_edits(g.st(), g.dev(), g.edit_cells.p, (int)kept);

@@ -903,11 +903,22 @@ __device__ void tail_coarse_solve(const TailLevel& T, const TailArrays& A, const
     const double mean = wave_sum((double)b) / cells;
     // y_j = rhs(cell j + 1) - mean, held by lane j + 1
     const float y = (float)((double)b - mean);
+    // c = G y in the order j = 0 .. n-1, eight terms at a time: their G loads and y reads (v_readlane of a
+    // uniform lane, no LDS) issue before the additions
     float c = 0.f;
-    for (int j = 0; j < n; ++j) {
-        const float yj = __shfl(y, j + 1, 64);
-        if (lane < n)
-            c += G[lane * n + j] * yj;
+    const int grow = min(lane, max(n - 1, 0)) * n;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        float g[8], yv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int j = min(j0 + q, n - 1);
+            g[q] = G[grow + j];
+            yv[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y), j + 1));
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (j0 + q < n && lane < n)
+                c += g[q] * yv[q];
     }
     const double gauge = wave_sum(lane < n ? (double)c : 0.0) / cells;
     if (lane == 0)
@@ -925,63 +936,77 @@ template <class TV = float> __device__ void tail_run(const TailDesc& D, const do
     float* lds)
 {
     const int nl = D.levels;
-    // Flat index over every level's padded arrays; all loads issue before the LDS stores.
+    // Flat index over every level's padded arrays; all loads issue before the LDS stores.  The level of an
+    // element, its source pointers and its LDS slot are picked by comparisons against uniform level
+    // offsets (no per-lane reads of the descriptor).
     constexpr int TL = 8;
-    int total = 0;
-    for (int l = 0; l < nl; ++l)
-        total += D.lev[l].s * (D.lev[l].h + 2);
+    int start[kMaxTailLevels + 1];
+    start[0] = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxTailLevels; ++k)
+        start[k + 1] = start[k] + (k < nl ? D.lev[k].s * (D.lev[k].h + 2) : 0);
+    const int total = start[kMaxTailLevels];
+    const TailLevel& T0 = D.lev[0];
+    const float inv_s0 = 1.0f / (float)T0.s;
     float vr[TL], vd[TL], vb[TL];
-    int lv[TL], li[TL];
+    int slot[TL], size[TL];
     bool bin[TL];
 #pragma unroll
     for (int it = 0; it < TL; ++it) {
         const int e0 = threadIdx.x + it * kTailThreads;
-        int e = min(e0, total - 1);
-        int l = 0;
-        while (e >= D.lev[l].s * (D.lev[l].h + 2)) {
-            e -= D.lev[l].s * (D.lev[l].h + 2);
-            ++l;
+        const int e = min(e0, total - 1);
+        const float* pr = T0.right;
+        const float* pd = T0.down;
+        int first = 0, lds_at = T0.lds_arrays, n_at = start[1];
+#pragma unroll
+        for (int k = 1; k < kMaxTailLevels; ++k) {
+            if (k < nl && e >= start[k]) {
+                pr = D.lev[k].right;
+                pd = D.lev[k].down;
+                first = start[k];
+                lds_at = D.lev[k].lds_arrays;
+                n_at = start[k + 1] - start[k];
+            }
         }
-        const TailLevel& T = D.lev[l];
-        lv[it] = e0 < total ? l : -1;
-        li[it] = e;
-        vr[it] = T.right[e];
-        vd[it] = T.down[e];
-        const int y = e / T.s - 1, x = e % T.s - 1;
-        const bool interior0 = l == 0 && x >= 0 && x < T.w && y >= 0 && y < T.h;
-        const float b = ldv(((const TV*)D.top_rhs)[min(e, D.lev[0].s * (D.lev[0].h + 2) - 1)]);
+        const int le = e - first;
+        vr[it] = pr[le];
+        vd[it] = pd[le];
+        const int y = tail_div(e, inv_s0) - 1, x = e - (y + 1) * T0.s - 1;   // meaningful on level 0 only
+        const bool interior0 = first == 0 && x >= 0 && x < T0.w && y >= 0 && y < T0.h;
+        const float b = ldv(((const TV*)D.top_rhs)[min(e, start[1] - 1)]);
         vb[it] = interior0 ? b : 0.f;
         bin[it] = interior0;
+        slot[it] = e0 < total ? lds_at + le : -1;
+        size[it] = n_at;
     }
+    // transfer tables (host-built, already in LDS layout) and the coarse inverse: flat copies with
+    // clamped, unconditional loads, issued with the level loads above
+    constexpr int TT = 2, TG = 4;
+    unsigned tv[TT];
+    float gv[TG];
+    const int nn = D.coarse_n * D.coarse_n;
+#pragma unroll
+    for (int it = 0; it < TT; ++it)
+        tv[it] = D.tables[min(threadIdx.x + it * kTailThreads, (unsigned)max(D.table_words - 1, 0))];
+#pragma unroll
+    for (int it = 0; it < TG; ++it)
+        gv[it] = D.ginv[min(threadIdx.x + it * kTailThreads, (unsigned)max(nn - 1, 0))];
     if (status && *status != kRunning)
         return;
     const double sh = shift ? *shift : 0.0;
 #pragma unroll
     for (int it = 0; it < TL; ++it) {
-        if (lv[it] < 0)
+        if (slot[it] < 0)
             continue;
-        const TailArrays A = tail_arrays(lds, D.lev[lv[it]]);
-        const int e = li[it];
-        A.right[e] = vr[it];
-        A.down[e] = vd[it];
-        A.rhs[e] = (shift && bin[it]) ? (float)((double)vb[it] - sh) : vb[it];
-        A.sol[e] = 0.f;
-        A.res[e] = 0.f;
+        float* p = lds + slot[it];
+        const int ps = size[it];
+        p[0] = vr[it];                 // right
+        p[ps] = vd[it];                // down
+        p[2 * ps] = (shift && bin[it]) ? (float)((double)vb[it] - sh) : vb[it];   // rhs
+        p[3 * ps] = 0.f;               // sol
+        p[4 * ps] = 0.f;               // res
     }
     {
-        // transfer tables (host-built, already in LDS layout) and the coarse inverse: flat copies with
-        // clamped, unconditional loads
-        constexpr int TT = 2, TG = 4;
-        unsigned tv[TT];
-        float gv[TG];
-        const int n = D.coarse_n;
-        const int nn = n * n;
-#pragma unroll
-        for (int it = 0; it < TT; ++it)
-            tv[it] = D.tables[min(threadIdx.x + it * kTailThreads, (unsigned)max(D.table_words - 1, 0))];
-#pragma unroll
-        for (int it = 0; it < TG; ++it)
-            gv[it] = D.ginv[min(threadIdx.x + it * kTailThreads, (unsigned)max(nn - 1, 0))];
         unsigned* tables = (unsigned*)(lds + D.lds_tables_base);
 #pragma unroll
         for (int it = 0; it < TT; ++it) {
@@ -1015,10 +1040,10 @@ template <class TV = float> __device__ void tail_run(const TailDesc& D, const do
         tail_prolong(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
         tail_smooth4(T, A, 1, 0);
     }
-    const TailLevel& T0 = D.lev[0];
     const TailArrays A0 = tail_arrays(lds, T0);
+    const float inv_w0 = 1.0f / (float)T0.w;
     for (int e = threadIdx.x; e < T0.w * T0.h; e += kTailThreads) {
-        const int y = e / T0.w, x = e - y * T0.w;
+        const int y = tail_div(e, inv_w0), x = e - y * T0.w;
         const int i = (y + 1) * T0.s + x + 1;
         ((TV*)D.top_sol)[i] = stv<TV>(A0.sol[i]);
     }

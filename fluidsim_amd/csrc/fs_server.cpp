@@ -91,6 +91,8 @@ struct Request {
     Clock::time_point submitted, admitted, finished;
     std::vector<float> step_ms;
     size_t device_bytes = 0;
+    double setup_ms = 0.0;      // materialise: create, configure, upload
+    double teardown_ms = 0.0;   // retire: download the final state, free the instance
     std::string error;
     // final state, kept on the host once the instance has left the device
     std::vector<double> out_fraction, out_u, out_v, out_p, out_lmass, out_gmass;
@@ -121,6 +123,7 @@ struct fs_server {
     Clock::time_point window_start = Clock::now();
     long long window_steps = 0;
     std::vector<float> window_step_ms;
+    double window_setup_ms = 0.0, window_teardown_ms = 0.0;
 
     // Admits waiting requests in arrival order while they fit.  Caller holds m.
     void admit_locked()
@@ -149,6 +152,7 @@ struct fs_server {
     void retire(Lane& lane, Request* r, bool ok, std::string error)
     {
         size_t bytes = 0;
+        const auto t0 = Clock::now();
         if (r->grid) {
             (void)fs_grid_device_bytes(r->grid, &bytes);
             if (ok) {
@@ -170,8 +174,11 @@ struct fs_server {
             fs_grid_destroy(r->grid);
             r->grid = nullptr;
         }
+        const double teardown = ms_between(t0, Clock::now());
         std::lock_guard<std::mutex> lock(m);
         r->device_bytes = bytes;
+        r->teardown_ms = teardown;
+        window_teardown_ms += teardown;
         r->finished = Clock::now();
         r->state = ok ? FS_REQUEST_DONE : FS_REQUEST_FAILED;
         r->error = ok ? "" : error;
@@ -230,9 +237,19 @@ struct fs_server {
             }
             for (Request* r : round) {
                 std::string error;
-                if (!r->grid && !materialise(lane, r, error)) {
-                    retire(lane, r, false, error);
-                    continue;
+                if (!r->grid) {
+                    const auto t0 = Clock::now();
+                    const bool made = materialise(lane, r, error);
+                    const double setup = ms_between(t0, Clock::now());
+                    {
+                        std::lock_guard<std::mutex> lock(m);
+                        r->setup_ms = setup;
+                        window_setup_ms += setup;
+                    }
+                    if (!made) {
+                        retire(lane, r, false, error);
+                        continue;
+                    }
                 }
                 if (r->steps_done >= r->p.steps) {
                     retire(lane, r, true, "");
@@ -306,6 +323,8 @@ struct fs_server {
         rep.step_max_ms = r->step_ms.empty() ? 0.0 : *std::max_element(r->step_ms.begin(), r->step_ms.end());
         rep.estimated_bytes = r->estimate;
         rep.device_bytes = r->device_bytes;
+        rep.setup_ms = r->setup_ms;
+        rep.teardown_ms = r->teardown_ms;
         std::snprintf(rep.error, sizeof rep.error, "%s", r->error.c_str());
         *out = rep;
     }
@@ -539,6 +558,8 @@ fs_status fs_server_get_stats(fs_server* server, fs_server_stats* stats)
         s.steps_per_s = s.elapsed_s > 0.0 ? (double)server->window_steps / s.elapsed_s : 0.0;
         s.step_p50_ms = percentile(server->window_step_ms, 0.5);
         s.step_p99_ms = percentile(server->window_step_ms, 0.99);
+        s.setup_ms_total = server->window_setup_ms;
+        s.teardown_ms_total = server->window_teardown_ms;
         *stats = s;
     });
 }
@@ -551,6 +572,7 @@ fs_status fs_server_reset_stats(fs_server* server)
         server->window_start = Clock::now();
         server->window_steps = 0;
         server->window_step_ms.clear();
+        server->window_setup_ms = server->window_teardown_ms = 0.0;
         server->peak_bytes = server->resident_bytes;
     });
 }

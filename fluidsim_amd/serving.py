@@ -70,6 +70,8 @@ class Report:
     step_max_ms: float
     estimated_bytes: int
     device_bytes: int
+    setup_ms: float
+    teardown_ms: float
     error: str
 
     @property

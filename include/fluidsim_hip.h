@@ -496,6 +496,8 @@ typedef struct fs_request_report {
     double step_p50_ms, step_p99_ms, step_max_ms;
     size_t estimated_bytes;       /* admission footprint */
     size_t device_bytes;          /* allocated by the instance at its end */
+    double setup_ms;              /* wall time to create, configure and upload the instance on its lane */
+    double teardown_ms;           /* wall time to download its final state and free it */
     char error[256];
 } fs_request_report;
 
@@ -506,6 +508,7 @@ typedef struct fs_server_stats {
     double elapsed_s;             /* since creation or the last fs_server_reset_stats */
     double steps_per_s;           /* steps completed in that window / elapsed_s */
     double step_p50_ms, step_p99_ms;   /* over the steps of that window */
+    double setup_ms_total, teardown_ms_total;   /* lane time spent creating and retiring instances */
 } fs_server_stats;
 
 fs_status fs_server_create(const fs_server_config* config, fs_server** out);

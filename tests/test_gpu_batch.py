@@ -315,3 +315,39 @@ def test_split_batched_viscosity_runs_large_grids_through_the_single_grid_solve(
     for g in grids:
         g.destroy()
     ctx.close()
+
+
+def test_batched_viscosity_of_c4_instances_matches_the_oracle(oracle):
+    """Config C4's instances (sides 64..512, nu log-uniform 1e-3..1): every instance's batched viscosity
+    step agrees with the FP64 oracle's FluidGridGPU::applyViscosity restatement on the same state within
+    3e-4 of the velocity scale (the solve stops at a 1e-4 relative residual in the mass norm; the GPU
+    iterates in FP32), the bar of the single-grid viscosity tests."""
+    from fluidsim_amd import batch_apply_viscosity
+
+    wls = [wl for wl in WL.batch_instances(4, 40) if wl.width * wl.height <= 120_000][:12]
+    assert len(wls) == 12 and max(wl.width * wl.height for wl in wls) > 60_000
+    dt = wls[0].dt
+    grids = []
+    for wl in wls:
+        nu = wl.meta["nu"]
+        g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+        g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0, nu * 100.0)]),
+                             FluidMixture([MixtureComponent("air", 0.1, nu * 0.1)]))
+        g.upload(FluidGridState(wl.fraction, wl.u, wl.v, np.zeros(0), [wl.liquid_mass], [wl.gas_mass]))
+        grids.append(g)
+    its = batch_apply_viscosity(grids, dt)
+    for wl, g, n in zip(wls, grids, its):
+        nu = wl.meta["nu"]
+        o = oracle.OracleGrid(wl.width, wl.height, wl.cell_size)
+        o.configure([100.0], [0.1])
+        o.configure_viscosity([nu * 100.0], [nu * 0.1])
+        o.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+        st, oit = o.apply_viscosity(dt)
+        assert st == oracle.VISC_OK and n > 0, (wl.width, wl.height, st, n)
+        so, sg = o.download(), g.download()
+        scale = max(np.abs(wl.u).max(), np.abs(wl.v).max())
+        err = max(np.abs(sg.u - so["u"]).max(), np.abs(sg.v - so["v"]).max())
+        print(f"{wl.width}x{wl.height} nu {nu:.3g}: {n} GPU / {oit} oracle iterations, max error {err / scale:.2e} of scale",
+              flush=True)
+        assert err <= 3e-4 * scale, (wl.width, wl.height, err, scale)
+        g.destroy()

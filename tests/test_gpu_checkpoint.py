@@ -209,3 +209,20 @@ def test_checkpoint_contract_errors():
         g.loadCheckpoint(FluidCheckpoint({"backend": "hip-uniform"}, b""))
     with pytest.raises(FluidSimError):         # the checkpoint of another geometry
         g.loadCheckpoint(saved(make_grid()))
+
+
+def test_device_bytes_follow_a_load_with_fewer_components():
+    """A checkpoint with two gases loaded into a grid with three replaces the gas masses by a smaller
+    buffer: fs_grid_device_bytes drops by exactly one gas field (W*H doubles)."""
+    source = make_grid()
+    target = make_grid(extra_gas=True)
+    for g in (source, target):
+        assert g.step(0.004, 2000, 0.01).converged
+    before = target.deviceBytes()
+    assert target.loadCheckpoint(saved(source))
+    assert len(target.getGasMixture().components) == 2
+    assert target.deviceBytes() == before - W * H * 8
+    # and a load that needs a third gas grows a two-gas grid by the same amount
+    grown = source.deviceBytes()
+    assert source.loadCheckpoint(saved(make_grid(extra_gas=True)))
+    assert source.deviceBytes() == grown + W * H * 8
