@@ -18,6 +18,7 @@
 #include <cstring>
 #include <exception>
 #include <limits>
+#include <map>
 #include <type_traits>
 #include <memory>
 #include <mutex>
@@ -57,6 +58,153 @@ std::mutex& device_setup_mutex()
 
 namespace {
 
+// Size classes of the device cache: multiples of 4 KiB up to 64 KiB, then four classes per power of two
+// (at most 25% over the request).
+size_t size_class(size_t bytes)
+{
+    if (bytes <= 65536)
+        return (bytes + 4095) & ~(size_t)4095;
+    size_t top = 1;
+    while (top < bytes)
+        top <<= 1;
+    const size_t step = top / 8;   // classes top/2 + k top/8, k = 1..4
+    return ((bytes + step - 1) / step) * step;
+}
+
+struct DeviceCache {
+    std::mutex m;
+    std::map<std::pair<int, size_t>, std::vector<void*>> free_blocks;   // (device, class) -> blocks
+    size_t cached = 0;
+    std::map<int, hipStream_t> setup_streams;
+    std::mutex setup_m;   // one set-up copy or fill at a time per process
+    std::map<size_t, std::vector<void*>> pinned_free_blocks;
+};
+
+DeviceCache& cache()
+{
+    static DeviceCache* c = new DeviceCache();   // outlives static destructors that may still release
+    return *c;
+}
+
+// Cached bytes beyond which released blocks are freed for real.
+constexpr size_t kCacheLimit = (size_t)64 << 30;
+
+int current_device()
+{
+    int d = 0;
+    HIPC(hipGetDevice(&d));
+    return d;
+}
+
+hipStream_t setup_stream_locked(int device)
+{
+    DeviceCache& c = cache();
+    auto it = c.setup_streams.find(device);
+    if (it != c.setup_streams.end())
+        return it->second;
+    hipStream_t st = nullptr;
+    HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    c.setup_streams[device] = st;
+    return st;
+}
+
+}  // namespace
+
+void* device_alloc(size_t bytes)
+{
+    if (bytes == 0)
+        return nullptr;
+    const int device = current_device();
+    const size_t cls = size_class(bytes);
+    DeviceCache& c = cache();
+    {
+        std::lock_guard<std::mutex> lock(c.m);
+        auto it = c.free_blocks.find({device, cls});
+        if (it != c.free_blocks.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            c.cached -= cls;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
+    HIPC(hipMalloc(&p, cls));
+    return p;
+}
+
+void device_free(void* p, size_t bytes)
+{
+    if (!p)
+        return;
+    const size_t cls = size_class(bytes);
+    DeviceCache& c = cache();
+    {
+        std::lock_guard<std::mutex> lock(c.m);
+        if (c.cached + cls <= kCacheLimit) {
+            int device = 0;
+            if (hipGetDevice(&device) == hipSuccess) {
+                c.free_blocks[{device, cls}].push_back(p);
+                c.cached += cls;
+                return;
+            }
+        }
+    }
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
+    (void)hipFree(p);
+}
+
+void device_zero(void* p, size_t bytes)
+{
+    if (!p || bytes == 0)
+        return;
+    DeviceCache& c = cache();
+    std::lock_guard<std::mutex> lock(c.setup_m);
+    const hipStream_t st = setup_stream_locked(current_device());
+    HIPC(hipMemsetAsync(p, 0, bytes, st));
+    HIPC(hipStreamSynchronize(st));
+}
+
+void device_upload(void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0)
+        return;
+    DeviceCache& c = cache();
+    std::lock_guard<std::mutex> lock(c.setup_m);
+    const hipStream_t st = setup_stream_locked(current_device());
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+}
+
+void* pinned_alloc(size_t bytes)
+{
+    DeviceCache& c = cache();
+    {
+        std::lock_guard<std::mutex> lock(c.m);
+        auto it = c.pinned_free_blocks.find(bytes);
+        if (it != c.pinned_free_blocks.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
+    HIPC(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    return p;
+}
+
+void pinned_free(void* p, size_t bytes)
+{
+    if (!p)
+        return;
+    DeviceCache& c = cache();
+    std::lock_guard<std::mutex> lock(c.m);
+    c.pinned_free_blocks[bytes].push_back(p);
+}
+
+namespace {
+
 template <class T> struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
@@ -81,12 +229,10 @@ template <class T> struct DevBuf {
         std::swap(p, o.p);
         std::swap(n, o.n);
     }
+    // Returns the memory to the device cache: the owner has drained every stream that used it.
     void release()
     {
-        if (p) {
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            (void)hipFree(p);
-        }
+        device_free(p, n * sizeof(T));
         p = nullptr;
         n = 0;
     }
@@ -100,9 +246,8 @@ template <class T> struct DevBuf {
         n = count;
         if (count == 0)
             return -before;
-        std::lock_guard<std::mutex> lock(device_setup_mutex());
-        HIPC(hipMalloc(&p, count * sizeof(T)));
-        HIPC(hipMemset(p, 0, count * sizeof(T)));
+        p = (T*)device_alloc(count * sizeof(T));
+        device_zero(p, count * sizeof(T));
         return (long long)(count * sizeof(T)) - before;
     }
 };
@@ -169,9 +314,7 @@ void build_axis(const std::vector<double>& fc, const std::vector<double>& cc, st
 template <class T> void upload_vec(DevBuf<T>& d, const std::vector<T>& h, size_t& bytes)
 {
     bytes += d.alloc(h.size());
-    std::lock_guard<std::mutex> lock(device_setup_mutex());
-    if (!h.empty())
-        HIPC(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    device_upload(d.p, h.data(), h.size() * sizeof(T));
 }
 
 std::vector<float> to_float(const std::vector<double>& v)
@@ -408,6 +551,17 @@ struct fs_grid {
     size_t NU() const { return (size_t)(W + 1) * H; }
     size_t NV() const { return (size_t)W * (H + 1); }
     hipStream_t st() const { return ctx->stream; }
+
+    // Waits for every stream of the context that may still use this grid's buffers, before any of them is
+    // released or replaced (released memory goes straight back to the device cache).
+    void drain() const
+    {
+        HIPC(hipStreamSynchronize(ctx->stream));
+        if (ctx->aux)
+            HIPC(hipStreamSynchronize(ctx->aux));
+        if (ctx->batch_stream)
+            HIPC(hipStreamSynchronize(ctx->batch_stream));
+    }
 
     void drop_graph()
     {
@@ -671,10 +825,7 @@ struct fs_grid {
             limit = 65536 - 4096;  // the device refused a larger dynamic LDS request
         }
         bytes += tail_dev.alloc(1);
-        {
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            HIPC(hipMemcpy(tail_dev.p, &tail, sizeof(TailDesc), hipMemcpyHostToDevice));
-        }
+        device_upload(tail_dev.p, &tail, sizeof(TailDesc));
     }
 
     // ---------------------------------------------------------------------------------------
@@ -1525,7 +1676,7 @@ fs_status fs_grid_destroy(fs_grid* grid)
         if (!grid)
             return;
         set_device(grid->ctx);
-        (void)hipStreamSynchronize(grid->st());
+        grid->drain();
         delete grid;
     });
 }
@@ -1632,11 +1783,11 @@ fs_status fs_set_solid_mask(fs_grid* grid, const unsigned char* mask)
             }
         }
         set_device(grid->ctx);
+        grid->drain();
         if (mask) {
             if (grid->solid.n != N)
                 grid->bytes += grid->solid.alloc(N);
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            HIPC(hipMemcpy(grid->solid.p, m.data(), N, hipMemcpyHostToDevice));
+            device_upload(grid->solid.p, m.data(), N);
         } else if (grid->solid.n) {
             grid->bytes -= grid->solid.n;
             grid->solid.release();
@@ -1657,6 +1808,7 @@ fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode)
         if (mode == grid->mg_prec)
             return;
         set_device(grid->ctx);
+        grid->drain();
         grid->mg_prec = mode;
         grid->apply_precision();
         grid->tune_for_precision();
@@ -1743,6 +1895,7 @@ fs_status fs_upload(fs_grid* grid, const double* liquid_fraction, const double* 
         fs_grid& g = *grid;
         const size_t N = g.N();
         const size_t nl = g.rho_l.size(), ng = g.rho_g.size();
+        g.drain();
         if (g.lmass.n != nl * N)
             g.bytes += g.lmass.alloc(nl * N);
         if (g.gmass.n != ng * N)
@@ -1875,6 +2028,7 @@ fs_status fs_commit_edits(fs_grid* grid, int count, const fs_cell_edit* cells, c
         }
         list.resize(kept);
         set_device(g.ctx);
+        g.drain();
         if (g.edit_cells.n < kept)
             g.bytes += g.edit_cells.alloc(kept);
         const SyncOnExit drain{g.st()};
@@ -3077,6 +3231,7 @@ fs_status fs_checkpoint_load(fs_grid* grid, const char* metadata, size_t metadat
             bad_checkpoint("fluid payload has the wrong size");
         // Validated: replace the live configuration and state.
         set_device(g.ctx);
+        g.drain();
         const size_t N = g.N();
         if (g.lmass.n != c.rho_l.size() * N)
             g.bytes += g.lmass.alloc(c.rho_l.size() * N);

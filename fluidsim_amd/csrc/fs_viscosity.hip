@@ -686,12 +686,11 @@ template <class T> struct VBuf {
     VBuf(const VBuf&) = delete;
     VBuf& operator=(const VBuf&) = delete;
     ~VBuf() { release(); }
+    // Owned memory goes back to the device cache: the solver's owner drains its streams first.
     void release()
     {
-        if (p && owned) {
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            (void)hipFree(p);
-        }
+        if (owned)
+            device_free(p, n * sizeof(T));
         p = nullptr;
         n = 0;
         owned = true;
@@ -709,9 +708,8 @@ template <class T> struct VBuf {
         n = count;
         if (!count)
             return 0;
-        std::lock_guard<std::mutex> lock(device_setup_mutex());
-        vhip(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
-        vhip(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+        p = (T*)device_alloc(count * sizeof(T));
+        device_zero(p, count * sizeof(T));
         return count * sizeof(T);
     }
 };
@@ -1204,19 +1202,14 @@ struct ViscositySolver {
             place(material, 2 * (size_t)W * H);
         place(partials, 4 * (size_t)std::max(slot_blocks(W, H), 1u));
         place(S, 1);
-        if (!arena) {
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            vhip(hipHostMalloc((void**)&pinned, sizeof(VisScalars), hipHostMallocDefault), "hipHostMalloc");
-        }
+        if (!arena)
+            pinned = (VisScalars*)pinned_alloc(sizeof(VisScalars));
     }
 
     ~ViscositySolver()
     {
         drop_batch_graph();
-        if (pinned) {
-            std::lock_guard<std::mutex> lock(device_setup_mutex());
-            (void)hipHostFree(pinned);
-        }
+        pinned_free(pinned, sizeof(VisScalars));
     }
 
     VLev view(int l)
