@@ -41,6 +41,7 @@ SMOOTH_BYTES_PER_CELL = 89
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_rb_multi_fine_post_1024_traffic.md")
 TRACE_PROFILE = os.path.join(HERE, "profiles", "bench_1024_kernel_trace.md")
+SQ_PROFILE = os.path.join(HERE, "profiles", "k_rb_multi_sq_counters.md")
 TAIL_KERNEL = 6   # fs_time_kernel id of the single-workgroup coarse tail
 # Algorithmic bytes per fine cell of one whole projection, by the per-iteration model: 15 FP32 CG vector
 # passes (operator + dot 4, update + max 6, dot + direction 5; s = 4 B) plus the V(4,4) cycle's
@@ -136,7 +137,8 @@ def cpu_baseline(wl, tau, budget_s):
     workers = min(8, cores)
     med_n, count_n, it_n = timed(workers, 0.5 * budget_s)
     med_1, count_1, it_1 = timed(1, 0.5 * budget_s)
-    return {
+    state = g.download()   # the oracle's projection of the input, for the GPU result's error
+    return state, {
         "value": 1.0 / med_n,
         "unit": "solves/s",
         "cores": workers,
@@ -149,6 +151,19 @@ def cpu_baseline(wl, tau, budget_s):
         "nproc": os.cpu_count(),
         "cpu_model": cpu_model(),
     }
+
+
+def oracle_error(gpu, oracle):
+    """Relative L2 distance of a GPU projection from the FP64 oracle's projection of the same input to the
+    same tolerance: u, v, and the pressure with each field's mean removed (the pressure is defined up to a
+    constant)."""
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    pg = gpu.pressure - gpu.pressure.mean()
+    po = oracle["pressure"] - oracle["pressure"].mean()
+    return {"u": rel(gpu.u, oracle["u"]), "v": rel(gpu.v, oracle["v"]), "pressure": rel(pg, po),
+            "against": "FP64 CPU oracle, same input and tolerance"}
 
 
 def dam_break_leg(steps=100, seed=1):
@@ -317,9 +332,11 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
     """Config C3 on one GPU: the obstacle tank (40 seeded circles and boxes over 15% of the cells, liquid
     in the bottom 70%, vortex velocity) projected to 1e-6 of its initial max divergence with each
     multigrid storage mode, from the same device-resident input every solve.  Reports solves/s, PCG
-    iterations, the finest post-smoother's launch time and model bytes, and each result's distance from
-    the FP32 result (the FP64-oracle errors are checked by the GPU tests).  The bfloat16-vector mode gets
-    `stall_budget` iterations."""
+    iterations, the finest post-smoother's launch time and model bytes, each result's distance from the
+    FP32 result, and each result's error against the FP64 CPU oracle's projection of the same input to the
+    same tolerance (relative L2 of u, v and of the pressure over the fluid cells, each pressure taken
+    relative to its fluid-cell mean; the oracle runs on min(8, host cores) workers).  The bfloat16-vector
+    mode gets `stall_budget` iterations."""
     from fluidsim_amd import FS_MG_BF16, FS_MG_BF16_VECTORS, FS_MG_FP16, FS_MG_FP32, FluidGridHIP
     from fluidsim_amd import workloads as WL
 
@@ -338,6 +355,31 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
     out = {"workload": f"obstacle_tank {size}x{size}: 40 circles/boxes over {mask.mean():.3f} of the cells, "
                        f"liquid in the bottom 70% (1000:1), vortex velocity, seed {seed}; one projection per "
                        "solve to 1e-6 of the initial max divergence", "modes": {}}
+    # the FP64 oracle's projection of the same input (checker only)
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O  # noqa: E402
+
+    fluid = mask.ravel() == 0
+    o = O.OracleGrid(wl.width, wl.height, wl.cell_size)
+    o.configure(wl.liquid_density, wl.gas_density)
+    o.set_gravity(wl.gravity_y)
+    o.set_solid(mask)
+    o.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+    tau_o = 1e-6 * o.max_divergence()
+    O.set_threads(min(8, host_cores()))
+    t_o = time.perf_counter()
+    ro = o.project(wl.dt, 2000, tau_o)
+    t_o = time.perf_counter() - t_o
+    O.set_threads(1)
+    so = o.download()
+    po = so["pressure"][fluid] - so["pressure"][fluid].mean()
+    out["fp64_oracle"] = {"pcg_iterations": ro["iterations"], "converged": bool(ro["converged"]),
+                          "seconds": t_o, "threads": min(8, host_cores())}
+    note(f"FP64 oracle: {ro['iterations']} iterations in {t_o:.1f} s")
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
     ref_u = None
     for name, mode, coef_bytes, coarse_bytes in modes:
         g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
@@ -355,7 +397,9 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
             g.snapshotRestore()
             results.append(g.project(wl.dt, budget, tau))
         elapsed = time.perf_counter() - t0
-        u = g.download().u
+        st = g.download()
+        u = st.u
+        pg = st.pressure[fluid] - st.pressure[fluid].mean()
         post_ms = g.timeKernel(0, kernel_reps)
         model = smoother_model_bytes(coef_bytes, coarse_bytes)
         r = results[-1]
@@ -368,6 +412,7 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
             "fine_post_smoother_us": post_ms * 1e3,
             "fine_post_smoother_model_bytes_per_cell": model,
             "fine_post_smoother_model_gbs": model * size * size / (post_ms * 1e-3) / 1e9,
+            "rel_l2_vs_fp64_oracle": {"u": rel(st.u, so["u"]), "v": rel(st.v, so["v"]), "pressure": rel(pg, po)},
         }
         if ref_u is None:
             ref_u = u
@@ -527,6 +572,7 @@ def main():
     ap.add_argument("--precision-steps", type=int, default=3)
     ap.add_argument("--precision-only", action="store_true")
     ap.add_argument("--sl-steps", type=int, default=200, help="C2 semi-Lagrangian steps leg (0: skip)")
+    ap.add_argument("--c1-steps", type=int, default=100, help="C1 dam-break steps leg, GPU and CPU (0: skip)")
     ap.add_argument("--precision-modes", type=str, default="", help="subset of fp32,fp16,bf16,bf16_vectors")
     args = ap.parse_args()
     splits = tuple(int(v) for v in args.batch_splits.split(",") if v.strip())
@@ -654,8 +700,12 @@ def main():
     if traffic:
         # the fused smoother moves far fewer bytes than its model: what limits it is instruction issue
         out["roofline"]["frac_of_measured_traffic"] = traffic / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-        out["roofline"]["limiter"] = (f"instruction issue: measured HBM traffic is {traffic / sweep_bytes:.2f} of the "
-                                      "model bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE)")
+        out["roofline"]["limiter"] = (f"instruction issue and latency, not HBM: measured HBM traffic is "
+                                      f"{traffic / sweep_bytes:.2f} of the model bytes (rocprofv3 FETCH_SIZE + "
+                                      "WRITE_SIZE); SQ counters show VALU the busiest pipe and waves waiting "
+                                      "on latency at 4 waves per SIMD")
+        if os.path.exists(SQ_PROFILE):
+            out["roofline"]["limiter_source"] = os.path.relpath(SQ_PROFILE, HERE)
     top = traced_kernels()
     if top:
         dom = {"kernel": top[0][0], "share_of_kernel_time": top[0][1], "source": os.path.relpath(TRACE_PROFILE, HERE)}
@@ -674,9 +724,13 @@ def main():
                                    "barrier-separated phases per launch"})
         out["roofline"]["dominant_kernel_in_trace"] = dom
     if rank == 0 and not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+        gpu_state = grid.download()   # the last timed projection
+        oracle_state, out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+        out["oracle_error"] = oracle_error(gpu_state, oracle_state)
     if args.sl_steps > 0:
         out["semi_lagrangian"] = semi_lagrangian_leg(args.size, args.sl_steps)
+    if args.c1_steps > 0:
+        out["dam_break"] = dam_break_leg(args.c1_steps)
     if args.precision_size > 0:
         out["mixed_precision"] = precision_leg(args.precision_size, steps=args.precision_steps)
     if args.batch_instances > 0:
