@@ -208,22 +208,30 @@ def dam_break_leg(steps=100, seed=1):
     }
 
 
-def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
+def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, batch_cells=0,
+                compare_batch_cells=0):
     """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
     [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each).
     It is first served one request at a time on one stream with FP32 multigrid levels (serial plain), then
-    fed into the serving loop (`lanes` concurrent streams, HBM-budget admission) with bfloat16 multigrid
-    conductances at each arrival rate in `rates` (per second).  Reports aggregate steps/s, per-step
-    latency p50/p99 and queueing delay.  The leg varies from run to run (which large grids step side by
-    side depends on arrival and thread timing), so the serial pass and each rate run `repeats` times and
-    the median run is reported, with every run's rate listed."""
+    fed into the serving loop (`lanes` concurrent streams, HBM-budget admission) at each arrival rate in
+    `rates` (per second): requests of at most `batch_cells` cells step together in batched launches with
+    FP32 levels (the batched solves' format), larger ones one by one with bfloat16 multigrid conductances.
+    Reports aggregate steps/s, per-step latency p50/p99 and queueing delay.  The leg varies from run to run
+    (which large grids step side by side depends on arrival and thread timing), so the serial pass and each
+    rate run `repeats` times and the median run is reported, with every run's rate listed.  With
+    `compare_batch_cells`, one more run at the last rate batches the requests of at most that many cells
+    (fs_batch_step: one workgroup per instance in the persistent solves)."""
     from fluidsim_amd import FS_MG_BF16, FS_MG_FP32
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
 
-    def stream_at(rate, precision=FS_MG_BF16):
-        return [(t, Request(multigrid_precision=precision, **kw))
-                for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps)]
+    def stream_at(rate, precision=None, cells=batch_cells):
+        out = []
+        for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps):
+            small = kw["width"] * kw["height"] <= cells
+            p = precision if precision is not None else (FS_MG_FP32 if small else FS_MG_BF16)
+            out.append((t, Request(multigrid_precision=p, **kw)))
+        return out
 
     base = stream_at(rates[0], FS_MG_FP32)
     total_steps = sum(r.steps for _, r in base)
@@ -238,30 +246,31 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
             serial_runs.append((time.perf_counter() - t0, srv.stats()))
     serial_runs.sort(key=lambda v: v[0])
     serial_s, serial = serial_runs[len(serial_runs) // 2]
+    def serve(stream, cells):
+        with Server(lanes=lanes, batch_cells=cells) as srv:
+            ids = []
+            t0 = time.perf_counter()
+            for t, r in stream:
+                delay = t0 + t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(r))
+            srv.drain()
+            served_s = time.perf_counter() - t0
+            st = srv.stats()
+            reps = [srv.wait(i) for i in ids]
+        failed = [r.error for r in reps if not r.done]
+        if failed:
+            raise RuntimeError(f"served request failed: {failed[0]}")
+        return (served_s, st, [r.queue_ms for r in reps], sum(r.setup_ms for r in reps),
+                sum(r.teardown_ms for r in reps))
+
     runs = []
     for rate in rates:
         stream = stream_at(rate)
         samples = []
         for _ in range(repeats):
-            with Server(lanes=lanes) as srv:
-                ids = []
-                t0 = time.perf_counter()
-                for t, r in stream:
-                    delay = t0 + t - time.perf_counter()
-                    if delay > 0:
-                        time.sleep(delay)
-                    ids.append(srv.submit(r))
-                srv.drain()
-                served_s = time.perf_counter() - t0
-                st = srv.stats()
-                reps = [srv.wait(i) for i in ids]
-            failed = [r.error for r in reps if not r.done]
-            if failed:
-                raise RuntimeError(f"served request failed: {failed[0]}")
-            samples.append((served_s, st, [r.queue_ms for r in reps]))
-            setup_ms = sum(r.setup_ms for r in reps)
-            teardown_ms = sum(r.teardown_ms for r in reps)
-            samples[-1] = samples[-1] + (setup_ms, teardown_ms)
+            samples.append(serve(stream, batch_cells))
         samples.sort(key=lambda v: v[0])
         served_s, st, queue, setup_ms, teardown_ms = samples[len(samples) // 2]
         runs.append({
@@ -276,13 +285,23 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
             "lane_ms_creating_instances": setup_ms,
             "lane_ms_retiring_instances": teardown_ms,
         })
+    compare = None
+    if compare_batch_cells > 0:
+        served_s, st = serve(stream_at(rates[-1], cells=compare_batch_cells), compare_batch_cells)[:2]
+        compare = {"batch_cells": compare_batch_cells, "rate_per_s": rates[-1],
+                   "batched_requests": sum(r.width * r.height <= compare_batch_cells for _, r in base),
+                   "steps_per_s": total_steps / served_s, "speedup_vs_serial": serial_s / served_s,
+                   "step_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]}}
     return {
         "workload": f"{count} requests, sides log-uniform 128..{max_size}, {steps} steps each (PLIC transport, "
                     "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to a 1e-6 relative residual "
                     "max|b - A p| <= 1e-6 max|b| in every step; FP32 "
-                    "multigrid levels served one at a time, bfloat16 conductances in the serving loop), "
+                    "multigrid levels served one at a time; in the serving loop, requests of at most "
+                    f"{batch_cells} cells batched with FP32 levels, larger ones with bfloat16 conductances), "
                     f"Poisson arrivals, seed {seed}",
         "lanes": lanes,
+        "batch_cells": batch_cells,
+        "batched_requests": sum(r.width * r.height <= batch_cells for _, r in base),
         "cells_mean": float(np.mean([r.width * r.height for _, r in base])),
         "steps": total_steps,
         "serial_steps_per_s": total_steps / serial_s,
@@ -290,6 +309,7 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3):
         "repeats": repeats,
         "serial_step_ms": {"p50": serial["step_p50_ms"], "p99": serial["step_p99_ms"]},
         "served": runs,
+        "served_batched": compare,
     }
 
 
@@ -561,6 +581,10 @@ def main():
     ap.add_argument("--serving-rates", type=str, default="8,64", help="arrival rates to sweep (per second)")
     ap.add_argument("--serving-steps", type=int, default=50)
     ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--serving-batch-cells", type=int, default=0,
+                    help="C5: requests of at most this many cells step batched (0: every request alone)")
+    ap.add_argument("--serving-compare-batch-cells", type=int, default=65536,
+                    help="C5: one extra run at the last rate with this batch size, for comparison (0: skip)")
     ap.add_argument("--serving-repeats", type=int, default=3, help="runs per pass; the median is reported")
     ap.add_argument("--serving-max-size", type=int, default=2048)
     ap.add_argument("--serving-only", action="store_true")
@@ -598,6 +622,8 @@ def main():
     if args.serving_only:
         print(json.dumps(serving_leg(args.serving_requests or 24, rates, args.serving_steps,
                                      args.serving_lanes, max_size=args.serving_max_size,
+                                     batch_cells=args.serving_batch_cells,
+                                     compare_batch_cells=args.serving_compare_batch_cells,
                                      repeats=max(1, args.serving_repeats))), flush=True)
         return
 
@@ -740,6 +766,8 @@ def main():
         grid.destroy()
         out["serving"] = serving_leg(args.serving_requests, rates, args.serving_steps,
                                      args.serving_lanes, max_size=args.serving_max_size,
+                                     batch_cells=args.serving_batch_cells,
+                                     compare_batch_cells=args.serving_compare_batch_cells,
                                      repeats=max(1, args.serving_repeats))
     if rank == 0:
         print(json.dumps(out), flush=True)

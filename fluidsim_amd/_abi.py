@@ -93,7 +93,8 @@ class FsFrameStatistics(C.Structure):
 
 
 class FsServerConfig(C.Structure):
-    _fields_ = [("device", C.c_int), ("lanes", C.c_int), ("hbm_budget_bytes", C.c_size_t)]
+    _fields_ = [("device", C.c_int), ("lanes", C.c_int), ("hbm_budget_bytes", C.c_size_t),
+                ("batch_cells", C.c_longlong)]
 
 
 class FsRequest(C.Structure):
@@ -212,6 +213,8 @@ SIGNATURES = {
     "fs_checkpoint_load": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_advance_frame": [_vp, C.POINTER(FsFrameSettings), C.POINTER(FsFrameStatistics)],
     "fs_batch_project": [C.c_int, C.POINTER(_vp), C.c_double, C.c_int, _dp, C.POINTER(FsProjectionResult)],
+    "fs_batch_step": [C.c_int, C.POINTER(_vp), _dp, C.POINTER(C.c_int), _dp, C.POINTER(FsProjectionResult),
+                      C.POINTER(C.c_int)],
     "fs_context_set_batch_split": [_vp, C.c_longlong],
     "fs_grid_estimate_bytes": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)],
     "fs_server_create": [C.POINTER(FsServerConfig), C.POINTER(_vp)],

@@ -2273,97 +2273,118 @@ fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations)
     });
 }
 
+// Every grid of a batch shares `ctx`, is uploaded and appears once.
+static fs_context* batch_context(int count, fs_grid* const* grids, const char* upload_message)
+{
+    fs_context* ctx = grids[0] ? grids[0]->ctx : nullptr;
+    for (int i = 0; i < count; ++i) {
+        need(grids[i] != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grids[i]->ctx == ctx, FS_E_INVALID_ARGUMENT, "batched grids must share one context");
+        need(grids[i]->uploaded, FS_E_LOGIC, upload_message);
+        for (int j = 0; j < i; ++j)
+            need(grids[j] != grids[i], FS_E_INVALID_ARGUMENT, "a grid appears twice in the batch");
+    }
+    return ctx;
+}
+
+// The batched viscosity step of the grids listed in `which` (each with its own dt), one persistent launch
+// for those at or under the context's split.  A grid's failure lands in errors[i] (its velocities unchanged)
+// and does not stop the others.  Returns with the context's stream drained.
+static void batch_viscosity(fs_context* ctx, fs_grid* const* grids, const std::vector<int>& which, const double* dts,
+    int* iterations, std::vector<std::string>& errors)
+{
+    hipStream_t st = ctx->stream;
+    const SyncOnExit drain{st};   // the staged descriptors and host results stay valid until then
+    const size_t bytes = viscosity_batch_desc_bytes();
+    std::vector<unsigned char> descs;
+    std::vector<int> batched, alone, solo;
+    for (int i : which) {
+        fs_grid& g = *grids[i];
+        const double dt = dts[i];
+        if (dt <= 0.0 || !g.has_viscosity())
+            continue;
+        if (batch_solo(ctx, g.W, g.H)) {
+            solo.push_back(i);
+            continue;
+        }
+        double* material = blend_viscosity_material(g);
+        descs.resize((batched.size() + 1) * bytes);
+        if (viscosity_batch_prepare(g.visc, st, g.u.p, g.v.p, material, dt, g.h, 1e-4, kViscosityBudget,
+                descs.data() + batched.size() * bytes))
+            batched.push_back(i);
+        else
+            alone.push_back(i);
+    }
+    if (!batched.empty()) {
+        const size_t need_bytes = batched.size() * bytes;
+        if (ctx->vis_batch_desc.n < need_bytes)
+            ctx->vis_batch_desc.alloc(need_bytes);
+        stage_upload(ctx, ctx->vis_batch_desc.p, descs.data(), need_bytes);
+        const bool spread = (int)batched.size() <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds;
+        const hipStream_t bs = solo.empty() ? st : batch_side_stream(ctx);
+        launch_batch_viscosity(bs, ctx->vis_batch_desc.p, (int)batched.size(), spread ? kBatchSpreadLds : 0);
+        for (int i : batched)
+            viscosity_batch_fetch(grids[i]->visc, bs);
+        if (!solo.empty())
+            HIPC(hipEventRecord(ctx->batch_out, bs));
+    }
+    auto attempt = [&](int i, auto&& body) {
+        try {
+            body();
+        } catch (const std::exception& e) {
+            errors[(size_t)i] = e.what();
+        }
+    };
+    // grids above the batch split: their own solve, beside the persistent launch
+    for (int i : solo)
+        attempt(i, [&] {
+            const int it = apply_viscosity(*grids[i], dts[i]);
+            if (iterations)
+                iterations[i] = it;
+        });
+    if (!solo.empty() && !batched.empty())
+        HIPC(hipStreamWaitEvent(st, ctx->batch_out, 0));
+    HIPC(hipStreamSynchronize(st));
+    for (int i : batched) {
+        int it = 0;
+        double relative = 0.0;
+        const int status = viscosity_batch_result(grids[i]->visc, kViscosityBudget, &it, &relative);
+        if (status == 4) {
+            alone.push_back(i);   // more iterations than the batch runs: solved again on its own
+            continue;
+        }
+        if (iterations)
+            iterations[i] = it;
+        attempt(i, [&] { raise_viscosity(status, relative); });
+    }
+    for (int i : alone)
+        attempt(i, [&] {
+            const int it = apply_viscosity(*grids[i], dts[i]);
+            if (iterations)
+                iterations[i] = it;
+        });
+    HIPC(hipStreamSynchronize(st));
+}
+
 fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, int* iterations)
 {
     return guard([&] {
         need(count >= 0 && (count == 0 || grids), FS_E_INVALID_ARGUMENT, "null pointer");
         if (count == 0)
             return;
-        fs_context* ctx = grids[0] ? grids[0]->ctx : nullptr;
-        for (int i = 0; i < count; ++i) {
-            need(grids[i] != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
-            need(grids[i]->ctx == ctx, FS_E_INVALID_ARGUMENT, "batched grids must share one context");
-            need(grids[i]->uploaded, FS_E_LOGIC, "Upload the grid state before applying viscosity");
-            for (int j = 0; j < i; ++j)
-                need(grids[j] != grids[i], FS_E_INVALID_ARGUMENT, "a grid appears twice in the batch");
-            if (iterations)
-                iterations[i] = 0;
-        }
+        fs_context* ctx = batch_context(count, grids, "Upload the grid state before applying viscosity");
+        if (iterations)
+            std::fill(iterations, iterations + count, 0);
         set_device(ctx);
-        hipStream_t st = ctx->stream;
-        const SyncOnExit drain{st};   // the staged descriptors and host results stay valid until then
-        const size_t bytes = viscosity_batch_desc_bytes();
-        std::vector<unsigned char> descs;
-        std::vector<int> batched, alone, solo;
-        for (int i = 0; i < count; ++i) {
-            fs_grid& g = *grids[i];
-            if (dt <= 0.0 || !g.has_viscosity())
-                continue;
-            if (batch_solo(ctx, g.W, g.H)) {
-                solo.push_back(i);
-                continue;
-            }
-            double* material = blend_viscosity_material(g);
-            descs.resize((batched.size() + 1) * bytes);
-            if (viscosity_batch_prepare(g.visc, st, g.u.p, g.v.p, material, dt, g.h, 1e-4, kViscosityBudget,
-                    descs.data() + batched.size() * bytes))
-                batched.push_back(i);
-            else
-                alone.push_back(i);
-        }
-        if (!batched.empty()) {
-            const size_t need_bytes = batched.size() * bytes;
-            if (ctx->vis_batch_desc.n < need_bytes)
-                ctx->vis_batch_desc.alloc(need_bytes);
-            stage_upload(ctx, ctx->vis_batch_desc.p, descs.data(), need_bytes);
-            const bool spread = (int)batched.size() <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds;
-            const hipStream_t bs = solo.empty() ? st : batch_side_stream(ctx);
-            launch_batch_viscosity(bs, ctx->vis_batch_desc.p, (int)batched.size(), spread ? kBatchSpreadLds : 0);
-            for (int i : batched)
-                viscosity_batch_fetch(grids[i]->visc, bs);
-            if (!solo.empty())
-                HIPC(hipEventRecord(ctx->batch_out, bs));
-        }
-        std::string first_error;
-        auto attempt = [&](auto&& body) {
-            try {
-                body();
-            } catch (const std::exception& e) {
-                if (first_error.empty())
-                    first_error = e.what();
-            }
-        };
-        // grids above the batch split: their own solve, beside the persistent launch
-        for (int i : solo)
-            attempt([&] {
-                const int it = apply_viscosity(*grids[i], dt);
-                if (iterations)
-                    iterations[i] = it;
-            });
-        if (!solo.empty() && !batched.empty())
-            HIPC(hipStreamWaitEvent(st, ctx->batch_out, 0));
-        HIPC(hipStreamSynchronize(st));
-        for (int i : batched) {
-            int it = 0;
-            double relative = 0.0;
-            const int status = viscosity_batch_result(grids[i]->visc, kViscosityBudget, &it, &relative);
-            if (status == 4) {
-                alone.push_back(i);   // more iterations than the batch runs: solved again on its own
-                continue;
-            }
-            if (iterations)
-                iterations[i] = it;
-            attempt([&] { raise_viscosity(status, relative); });
-        }
-        for (int i : alone)
-            attempt([&] {
-                const int it = apply_viscosity(*grids[i], dt);
-                if (iterations)
-                    iterations[i] = it;
-            });
-        HIPC(hipStreamSynchronize(st));
-        if (!first_error.empty())
-            fail(FS_E_RUNTIME, first_error);
+        std::vector<int> all((size_t)count);
+        for (int i = 0; i < count; ++i)
+            all[(size_t)i] = i;
+        const std::vector<double> dts((size_t)count, dt);
+        std::vector<std::string> errors((size_t)count);
+        batch_viscosity(ctx, grids, all, dts.data(), iterations, errors);
+        for (const std::string& e : errors)
+            if (!e.empty())
+                fail(FS_E_RUNTIME, e);
     });
 }
 
@@ -2956,6 +2977,87 @@ fs_status fs_grid_device_bytes(const fs_grid* grid, size_t* bytes)
     });
 }
 
+// The batched projection of the grids listed in `which`, each with its own dt, budget and tolerance: their
+// set-up on the context's stream, one persistent launch for those at or under the context's split (the others
+// run fs_project's multi-kernel solve beside it), then every velocity correction.  A solo grid's failure
+// lands in errors[i].  Returns with the context's stream drained.
+static void batch_projection(fs_context* ctx, fs_grid* const* grids, const std::vector<int>& which, const double* dts,
+    const int* max_iterations, const double* tolerances, fs_projection_result* results, std::vector<std::string>& errors)
+{
+    hipStream_t st = ctx->stream;
+    const SyncOnExit drain{st};   // the staged descriptors and host results stay valid until then
+    // Instances above the split run fs_project's multi-kernel solve on the context's stream while the
+    // persistent launch runs the others on a stream of its own; the route depends on the instance's
+    // size only, so each result is the same alone and in any batch.
+    std::vector<int> batched, solo;
+    for (int i : which)
+        if (dts[i] > 0.0)
+            (batch_solo(ctx, grids[i]->W, grids[i]->H) ? solo : batched).push_back(i);
+    const int nb = (int)batched.size();
+    const bool split = nb > 0 && !solo.empty();
+    if (nb > 0) {
+        std::vector<BatchInstance> desc((size_t)nb);
+        size_t lds = 0;
+        for (int k = 0; k < nb; ++k) {
+            const int i = batched[(size_t)k];
+            fs_grid& g = *grids[i];
+            g.project_begin(dts[i], fs_grid::budget(max_iterations[i]), tolerances[i]);
+            desc[(size_t)k] = g.batch_instance();
+            if (g.precond == FS_PRECOND_MULTIGRID)
+                lds = std::max(lds, g.tail_lds);
+        }
+        if (ctx->batch_desc.n < (size_t)nb)
+            ctx->batch_desc.alloc((size_t)nb);
+        // the descriptors go through pinned staging: `desc` dies at the end of this block, before the
+        // stream is synchronised
+        stage_upload(ctx, ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance));
+        // One workgroup is one instance's whole solve: while the instances fit on the compute units one
+        // each, reserve more than half a unit's 160 KB of LDS so that no two of them share a unit.
+        if (nb <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds)
+            lds = std::max(lds, kBatchSpreadLds);
+        if (lds > 65536)
+            need(batch_allow_lds(lds), FS_E_DEVICE, "the device refused the batched solve's LDS request");
+        const hipStream_t bs = split ? batch_side_stream(ctx) : st;
+        launch_batch_pcg(bs, ctx->batch_desc.p, nb, lds);
+        HIPC(hipGetLastError());
+        if (split)
+            HIPC(hipEventRecord(ctx->batch_out, bs));
+    }
+    // the solo instances; a failure is kept for its grid while the batched instances are finished
+    for (int i : solo) {
+        try {
+            results[i] = grids[i]->project(dts[i], max_iterations[i], tolerances[i]);
+        } catch (const std::exception& e) {
+            errors[(size_t)i] = e.what();
+        }
+    }
+    if (nb > 0) {
+        if (split)
+            HIPC(hipStreamWaitEvent(st, ctx->batch_out, 0));
+        for (int i : batched)
+            grids[i]->project_end(dts[i]);
+        if (ctx->batch_pinned_n < (size_t)nb) {
+            std::lock_guard<std::mutex> lock(device_setup_mutex());
+            if (ctx->batch_pinned)
+                (void)hipHostFree(ctx->batch_pinned);
+            ctx->batch_pinned = nullptr;
+            ctx->batch_pinned_n = 0;
+            HIPC(hipHostMalloc((void**)&ctx->batch_pinned, (size_t)nb * sizeof(PcgScalars), hipHostMallocDefault));
+            ctx->batch_pinned_n = (size_t)nb;
+        }
+        for (int k = 0; k < nb; ++k)
+            HIPC(hipMemcpyAsync(ctx->batch_pinned + k, grids[batched[(size_t)k]]->S.p, sizeof(PcgScalars),
+                hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        for (int k = 0; k < nb; ++k) {
+            const int i = batched[(size_t)k];
+            const PcgScalars sc = ctx->batch_pinned[k];
+            results[i] = grids[i]->project_result(sc, tolerances[i], dts[i]);
+            grids[i]->predicted = std::max(1, sc.iterations);
+        }
+    }
+}
+
 fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_iterations,
     const double* divergence_tolerances, fs_projection_result* results)
 {
@@ -2964,16 +3066,10 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
             "null pointer");
         if (count == 0)
             return;
-        fs_context* ctx = grids[0] ? grids[0]->ctx : nullptr;
-        for (int i = 0; i < count; ++i) {
-            need(grids[i] != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
-            need(grids[i]->ctx == ctx, FS_E_INVALID_ARGUMENT, "batched grids must share one context");
-            need(grids[i]->uploaded, FS_E_LOGIC, "Upload the grid state before stepping or projecting");
+        fs_context* ctx = batch_context(count, grids, "Upload the grid state before stepping or projecting");
+        for (int i = 0; i < count; ++i)
             need(!grids[i]->active16(), FS_E_UNSUPPORTED,
                 "batched solves run FP32 multigrid levels: set the grid's multigrid precision to FS_MG_FP32");
-            for (int j = 0; j < i; ++j)
-                need(grids[j] != grids[i], FS_E_INVALID_ARGUMENT, "a grid appears twice in the batch");
-        }
         for (int i = 0; i < count; ++i) {
             results[i] = fs_projection_result{};
             results[i].max_divergence = std::numeric_limits<double>::infinity();
@@ -2982,80 +3078,143 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
         if (dt <= 0.0)
             return;
         set_device(ctx);
-        hipStream_t st = ctx->stream;
-        const SyncOnExit drain{st};   // the staged descriptors and host results stay valid until then
-        // Instances above the split run fs_project's multi-kernel solve on the context's stream while the
-        // persistent launch runs the others on a stream of its own; the route depends on the instance's
-        // size only, so each result is the same alone and in any batch.
-        std::vector<int> batched, solo;
+        std::vector<int> all((size_t)count);
         for (int i = 0; i < count; ++i)
-            (batch_solo(ctx, grids[i]->W, grids[i]->H) ? solo : batched).push_back(i);
-        const int nb = (int)batched.size();
-        const bool split = nb > 0 && !solo.empty();
-        if (nb > 0) {
-            std::vector<BatchInstance> desc((size_t)nb);
-            size_t lds = 0;
-            for (int k = 0; k < nb; ++k) {
-                fs_grid& g = *grids[batched[(size_t)k]];
-                g.project_begin(dt, fs_grid::budget(max_iterations), divergence_tolerances[batched[(size_t)k]]);
-                desc[(size_t)k] = g.batch_instance();
-                if (g.precond == FS_PRECOND_MULTIGRID)
-                    lds = std::max(lds, g.tail_lds);
+            all[(size_t)i] = i;
+        const std::vector<double> dts((size_t)count, dt);
+        const std::vector<int> budgets((size_t)count, max_iterations);
+        std::vector<std::string> errors((size_t)count);
+        batch_projection(ctx, grids, all, dts.data(), budgets.data(), divergence_tolerances, results, errors);
+        for (const std::string& e : errors)
+            if (!e.empty())
+                fail(FS_E_RUNTIME, e);
+    });
+}
+
+fs_status fs_batch_step(int count, fs_grid* const* grids, const double* dts, const int* max_iterations,
+    const double* divergence_tolerances, fs_projection_result* results, fs_status* statuses)
+{
+    return guard([&] {
+        need(count >= 0 && (count == 0 || (grids && dts && max_iterations && divergence_tolerances && results)),
+            FS_E_INVALID_ARGUMENT, "null pointer");
+        if (count == 0)
+            return;
+        fs_context* ctx = batch_context(count, grids, "Upload the grid state before stepping");
+        const size_t n = (size_t)count;
+        // fs_step's plan per grid: `count` capillary substeps of dt / count (FluidGridCPU.cpp:319-357)
+        std::vector<int> substeps(n, 0);
+        std::vector<double> step(n, 0.0);
+        std::vector<std::string> errors(n);
+        std::vector<char> alive(n, 0);
+        for (size_t i = 0; i < n; ++i) {
+            fs_projection_result& r = results[i];
+            r = fs_projection_result{};
+            r.max_divergence = std::numeric_limits<double>::infinity();
+            if (statuses)
+                statuses[i] = FS_OK;
+            const fs_grid& g = *grids[i];
+            if (!(dts[i] > 0.0))   // FluidGridCPU::step returns {0, inf, false, 0, 0} for dt <= 0
+                continue;
+            const double subdivisions = g.sigma > 0.0 ? std::max(1.0, std::ceil(dts[i] / g.capillary_step_limit())) : 1.0;
+            if (subdivisions > (double)std::numeric_limits<int>::max()) {
+                errors[i] = "Surface tension requires too many physical substeps for this timestep";
+                continue;
             }
-            if (ctx->batch_desc.n < (size_t)nb)
-                ctx->batch_desc.alloc((size_t)nb);
-            // the descriptors go through pinned staging: `desc` dies at the end of this block, before the
-            // stream is synchronised
-            stage_upload(ctx, ctx->batch_desc.p, desc.data(), desc.size() * sizeof(BatchInstance));
-            // One workgroup is one instance's whole solve: while the instances fit on the compute units one
-            // each, reserve more than half a unit's 160 KB of LDS so that no two of them share a unit.
-            if (nb <= ctx->compute_units && ctx->lds_per_block >= kBatchSpreadLds)
-                lds = std::max(lds, kBatchSpreadLds);
-            if (lds > 65536)
-                need(batch_allow_lds(lds), FS_E_DEVICE, "the device refused the batched solve's LDS request");
-            const hipStream_t bs = split ? batch_side_stream(ctx) : st;
-            launch_batch_pcg(bs, ctx->batch_desc.p, nb, lds);
-            HIPC(hipGetLastError());
-            if (split)
-                HIPC(hipEventRecord(ctx->batch_out, bs));
+            substeps[i] = (int)subdivisions;
+            step[i] = dts[i] / substeps[i];
+            alive[i] = 1;
         }
-        // the solo instances; a failure is reported after the batched instances have been finished
-        std::exception_ptr solo_error;
-        for (int i : solo) {
-            try {
-                results[i] = grids[i]->project(dt, max_iterations, divergence_tolerances[i]);
-            } catch (...) {
-                if (!solo_error)
-                    solo_error = std::current_exception();
+        set_device(ctx);
+        std::vector<int> viscosity_iterations(n, 0);
+        std::vector<fs_projection_result> proj(n);
+        for (int k = 0;; ++k) {
+            std::vector<int> active;
+            for (size_t i = 0; i < n; ++i)
+                if (alive[i] && k < substeps[i])
+                    active.push_back((int)i);
+            if (active.empty())
+                break;
+            auto keep = [&](std::vector<int>& list) {
+                list.erase(std::remove_if(list.begin(), list.end(), [&](int i) { return !alive[(size_t)i]; }),
+                    list.end());
+            };
+            auto attempt = [&](int i, auto&& body) {
+                try {
+                    body();
+                } catch (const std::exception& e) {
+                    errors[(size_t)i] = e.what();
+                    alive[(size_t)i] = 0;
+                }
+            };
+            // transport, one grid after another (each reads its CFL bound back)
+            for (int i : active)
+                attempt(i, [&] {
+                    fs_grid& g = *grids[i];
+                    if (g.transport == FS_TRANSPORT_REFERENCE) {
+                        advect_reference(g, step[(size_t)i]);
+                    } else if (g.transport == FS_TRANSPORT_SEMI_LAGRANGIAN) {
+                        advect_sl(g, step[(size_t)i]);
+                        results[i].transport_steps += 1;
+                    }
+                });
+            keep(active);
+            // every viscous grid's implicit solve, batched
+            std::fill(viscosity_iterations.begin(), viscosity_iterations.end(), 0);
+            batch_viscosity(ctx, grids, active, step.data(), viscosity_iterations.data(), errors);
+            for (int i : active) {
+                if (!errors[(size_t)i].empty())
+                    alive[(size_t)i] = 0;
+                results[i].viscosity_iterations += viscosity_iterations[(size_t)i];
+            }
+            keep(active);
+            // body forces
+            for (int i : active)
+                attempt(i, [&] {
+                    fs_grid& g = *grids[i];
+                    launch_gravity(g.st(), g.dev(), g.gravity_y, step[(size_t)i]);
+                    if (g.sigma > 0.0)
+                        g.apply_surface_tension(step[(size_t)i]);
+                });
+            keep(active);
+            // the projections: FP32-level grids in one batched solve, 16-bit-level grids one by one
+            std::vector<int> batched, alone;
+            for (int i : active)
+                (grids[i]->active16() ? alone : batched).push_back(i);
+            batch_projection(ctx, grids, batched, step.data(), max_iterations, divergence_tolerances, proj.data(), errors);
+            for (int i : alone)
+                attempt(i, [&] {
+                    proj[(size_t)i] = grids[i]->project(step[(size_t)i], max_iterations[i], divergence_tolerances[i]);
+                });
+            for (int i : active) {
+                if (!errors[(size_t)i].empty()) {
+                    alive[(size_t)i] = 0;
+                    continue;
+                }
+                const fs_projection_result& p = proj[(size_t)i];
+                fs_projection_result& r = results[i];
+                r.iterations += p.iterations;
+                r.max_divergence = p.max_divergence;
+                r.converged = p.converged;
+                r.restarts = p.restarts;
+                r.initial_residual = p.initial_residual;
+                r.final_residual = p.final_residual;
+                r.rhs_max = p.rhs_max;
+                r.physical_steps += 1;
+                if (!r.converged)
+                    alive[(size_t)i] = 0;   // stop after the first substep that does not converge
             }
         }
-        if (nb > 0) {
-            if (split)
-                HIPC(hipStreamWaitEvent(st, ctx->batch_out, 0));
-            for (int i : batched)
-                grids[i]->project_end(dt);
-            if (ctx->batch_pinned_n < (size_t)nb) {
-                std::lock_guard<std::mutex> lock(device_setup_mutex());
-                if (ctx->batch_pinned)
-                    (void)hipHostFree(ctx->batch_pinned);
-                ctx->batch_pinned = nullptr;
-                ctx->batch_pinned_n = 0;
-                HIPC(hipHostMalloc((void**)&ctx->batch_pinned, (size_t)nb * sizeof(PcgScalars), hipHostMallocDefault));
-                ctx->batch_pinned_n = (size_t)nb;
-            }
-            for (int k = 0; k < nb; ++k)
-                HIPC(hipMemcpyAsync(ctx->batch_pinned + k, grids[batched[(size_t)k]]->S.p, sizeof(PcgScalars),
-                    hipMemcpyDeviceToHost, st));
-            HIPC(hipStreamSynchronize(st));
-            for (int k = 0; k < nb; ++k) {
-                const int i = batched[(size_t)k];
-                const PcgScalars sc = ctx->batch_pinned[k];
-                results[i] = grids[i]->project_result(sc, divergence_tolerances[i], dt);
-                grids[i]->predicted = std::max(1, sc.iterations);
-            }
+        std::string first;
+        for (size_t i = 0; i < n; ++i) {
+            if (errors[i].empty())
+                continue;
+            if (statuses)
+                statuses[i] = FS_E_RUNTIME;
+            if (first.empty())
+                first = errors[i];
         }
-        if (solo_error)
-            std::rethrow_exception(solo_error);
+        if (!first.empty())
+            fail(FS_E_RUNTIME, first);
     });
 }
 

@@ -2,8 +2,9 @@
 //
 // Built only on the grid ABI (include/fluidsim_hip.h).  `lanes` worker threads each own a single-stream
 // fs_context (one HIP stream -- one hardware queue -- and pinned read-back words); a lane steps its resident
-// instances in rotation, one fs_step each per round, so instances on different lanes overlap on the
-// device while each instance's kernels stay ordered on its lane's stream.  Admission is first come,
+// instances in rotation, one step each per round, so instances on different lanes overlap on the device
+// while each instance's kernels stay ordered on its lane's stream.  With a batch size set, a round steps
+// the lane's small instances together (one fs_batch_step) and each larger one with its own fs_step.  Admission is first come,
 // first served against an HBM budget counted with fs_grid_estimate_bytes; an instance's grid is created
 // and uploaded by its lane when it is first stepped, and destroyed (its device memory freed) when it
 // finishes, after its final state has been copied to host memory for fs_server_result.
@@ -111,6 +112,7 @@ struct Lane {
 struct fs_server {
     int device = 0;
     size_t budget = 0;
+    long long batch_cells = 0;   // instances of at most this many cells step batched
     std::mutex m;
     std::condition_variable finished_cv;
     std::deque<Request*> queue;
@@ -133,9 +135,17 @@ struct fs_server {
             if (resident_bytes + r->estimate > budget)
                 break;
             queue.pop_front();
+            // small (batched) and larger instances are balanced across the lanes separately
+            const bool small = batched(r);
+            auto load = [&](const Lane& l) {
+                size_t n = 0;
+                for (const Request* a : l.active)
+                    n += batched(a) == small;
+                return n;
+            };
             Lane* best = lanes.front().get();
             for (auto& l : lanes)
-                if (l->active.size() < best->active.size())
+                if (load(*l) < load(*best))
                     best = l.get();
             resident_bytes += r->estimate;
             peak_bytes = std::max(peak_bytes, resident_bytes);
@@ -146,6 +156,11 @@ struct fs_server {
             best->active.push_back(r);
             best->wake.notify_one();
         }
+    }
+
+    bool batched(const Request* r) const
+    {
+        return batch_cells > 0 && (long long)r->p.width * r->p.height <= batch_cells;
     }
 
     // Ends a request on its lane: keeps the final state (if done), frees the grid, admits more.
@@ -223,10 +238,40 @@ struct fs_server {
                ok(fs_set_relative_tolerance(g, p.relative_tolerance));
     }
 
+    // Records one finished step of `r` (ms of lane time); ends the request when it failed, was not
+    // accepted, or has run all its steps.
+    void finish_step(Lane& lane, Request* r, fs_status st, const fs_projection_result& res, float ms)
+    {
+        if (st != FS_OK) {
+            retire(lane, r, false, fs_last_error());
+            return;
+        }
+        {
+            // counters are read by reporting threads under the same lock
+            std::lock_guard<std::mutex> lock(m);
+            r->step_ms.push_back(ms);
+            ++r->steps_done;
+            r->pressure_iterations += res.iterations;
+            r->viscosity_iterations += res.viscosity_iterations;
+            r->physical_steps += res.physical_steps;
+            r->max_divergence = res.max_divergence;
+            ++window_steps;
+            window_step_ms.push_back(ms);
+        }
+        if (!(res.converged || res.relaxed || res.precision_limited)) {
+            retire(lane, r, false,
+                "projection not accepted at step " + std::to_string(r->steps_done) + " (max divergence " +
+                    std::to_string(res.max_divergence) + ")");
+            return;
+        }
+        if (r->steps_done >= r->p.steps)
+            retire(lane, r, true, "");
+    }
+
     void run_lane(Lane& lane)
     {
         (void)hipSetDevice(device);
-        std::vector<Request*> round;
+        std::vector<Request*> round, group, solo;
         for (;;) {
             {
                 std::unique_lock<std::mutex> lock(m);
@@ -235,6 +280,8 @@ struct fs_server {
                     break;
                 round = lane.active;   // instances admitted meanwhile join the next round
             }
+            group.clear();
+            solo.clear();
             for (Request* r : round) {
                 std::string error;
                 if (!r->grid) {
@@ -255,35 +302,48 @@ struct fs_server {
                     retire(lane, r, true, "");
                     continue;
                 }
+                (batched(r) ? group : solo).push_back(r);
+            }
+            if (!group.empty()) {
+                const size_t n = group.size();
+                std::vector<fs_grid*> grids(n);
+                std::vector<double> dts(n), tolerances(n);
+                std::vector<int> budgets(n);
+                std::vector<fs_projection_result> results(n);
+                std::vector<fs_status> statuses(n, FS_OK);
+                for (size_t k = 0; k < n; ++k) {
+                    grids[k] = group[k]->grid;
+                    dts[k] = group[k]->p.dt;
+                    budgets[k] = group[k]->p.max_iterations;
+                    tolerances[k] = group[k]->p.divergence_tolerance;
+                }
+                const auto t0 = Clock::now();
+                const fs_status st = fs_batch_step((int)n, grids.data(), dts.data(), budgets.data(), tolerances.data(),
+                    results.data(), statuses.data());
+                const float ms = (float)ms_between(t0, Clock::now());
+                const std::string message = fs_last_error();   // the lowest failing instance's message
+                if (st != FS_OK && st != FS_E_RUNTIME) {
+                    // the batch itself was refused: every instance of it fails with that message
+                    for (Request* r : group)
+                        retire(lane, r, false, message);
+                } else {
+                    bool first = true;
+                    for (size_t k = 0; k < n; ++k) {
+                        if (statuses[k] != FS_OK) {
+                            retire(lane, group[k], false, first ? message : "step failed inside a batched step");
+                            first = false;
+                            continue;
+                        }
+                        finish_step(lane, group[k], FS_OK, results[k], ms);
+                    }
+                }
+            }
+            for (Request* r : solo) {
                 fs_projection_result res{};
                 const auto t0 = Clock::now();
                 const fs_status st = fs_step(r->grid, r->p.dt, r->p.max_iterations, r->p.divergence_tolerance,
                     nullptr, &res);
-                const float ms = (float)ms_between(t0, Clock::now());
-                if (st != FS_OK) {
-                    retire(lane, r, false, fs_last_error());
-                    continue;
-                }
-                {
-                    // counters are read by reporting threads under the same lock
-                    std::lock_guard<std::mutex> lock(m);
-                    r->step_ms.push_back(ms);
-                    ++r->steps_done;
-                    r->pressure_iterations += res.iterations;
-                    r->viscosity_iterations += res.viscosity_iterations;
-                    r->physical_steps += res.physical_steps;
-                    r->max_divergence = res.max_divergence;
-                    ++window_steps;
-                    window_step_ms.push_back(ms);
-                }
-                if (!(res.converged || res.relaxed || res.precision_limited)) {
-                    retire(lane, r, false,
-                        "projection not accepted at step " + std::to_string(r->steps_done) +
-                            " (max divergence " + std::to_string(res.max_divergence) + ")");
-                    continue;
-                }
-                if (r->steps_done >= r->p.steps)
-                    retire(lane, r, true, "");
+                finish_step(lane, r, st, res, (float)ms_between(t0, Clock::now()));
             }
         }
         // shutting down: unfinished instances end as failed
@@ -338,8 +398,10 @@ fs_status fs_server_create(const fs_server_config* config, fs_server** out)
         need(config != nullptr && out != nullptr, FS_E_INVALID_ARGUMENT, "null pointer");
         need(config->lanes >= 1 && config->lanes <= 16, FS_E_INVALID_ARGUMENT, "lanes must be in 1..16");
         auto s = std::make_unique<fs_server>();
+        need(config->batch_cells >= 0, FS_E_INVALID_ARGUMENT, "batch_cells must be non-negative");
         s->device = config->device;
         s->budget = config->hbm_budget_bytes;
+        s->batch_cells = config->batch_cells;
         if (s->budget == 0) {
             need(hipSetDevice(config->device) == hipSuccess, FS_E_DEVICE, "no such HIP device");
             size_t free_bytes = 0, total = 0;

@@ -266,6 +266,26 @@ def batch_project(grids, dt: float, max_iterations: int = 200, divergence_tolera
     return [ProjectionResult.from_c(res[i]) for i in range(n)]
 
 
+def batch_step(grids, dt, max_iterations=200, divergence_tolerance=1e-6, raise_on_error=True):
+    """fs_batch_step: fs_step on every grid (all on one context) at once, each with its own dt, budget and
+    tolerance (one value or one per grid); viscosity solves and projections run batched.  Returns
+    (results, statuses): one ProjectionResult per grid with fs_step's contract and one fs_status per grid.
+    With raise_on_error, a failing grid raises after every other grid has been stepped."""
+    grids = list(grids)
+    n = len(grids)
+    dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)).copy()
+    tol = np.broadcast_to(np.asarray(divergence_tolerance, dtype=np.float64), (n,)).copy()
+    budgets = (C.c_int * max(n, 1))(*np.broadcast_to(np.asarray(max_iterations, dtype=np.int64), (n,)).tolist())
+    handles = (C.c_void_p * max(n, 1))(*[g.handle.value for g in grids])
+    res = (FsProjectionResult * max(n, 1))()
+    statuses = (C.c_int * max(n, 1))()
+    if raise_on_error:
+        call("fs_batch_step", n, handles, _dp(dts), budgets, _dp(tol), res, statuses)
+    else:
+        lib().fs_batch_step(n, handles, _dp(dts), budgets, _dp(tol), res, statuses)
+    return [ProjectionResult.from_c(res[i]) for i in range(n)], [statuses[i] for i in range(n)]
+
+
 def batch_apply_viscosity(grids, dt: float):
     """fs_batch_apply_viscosity: every grid's implicit viscosity step (all on one context) with one
     persistent launch running all their CG loops.  Returns each grid's iteration count."""

@@ -613,4 +613,26 @@ inline std::vector<int> batchApplyViscosity(const std::vector<FluidGridHIP*>& gr
     return iterations;
 }
 
+// Batched timestep (fs_batch_step; no reference counterpart): each grid's step(dt[i], max_iterations[i],
+// divergence_tolerance[i]) with the viscosity solves and projections batched.  A failing grid throws
+// std::runtime_error after every other grid has been stepped.
+inline std::vector<FluidProjectionResult> batchStep(const std::vector<FluidGridHIP*>& grids,
+    const std::vector<double>& dts, const std::vector<int>& max_iterations,
+    const std::vector<double>& divergence_tolerances)
+{
+    if (dts.size() != grids.size() || max_iterations.size() != grids.size() ||
+        divergence_tolerances.size() != grids.size())
+        throw std::invalid_argument("one dt, iteration budget and divergence tolerance per grid");
+    std::vector<fs_grid*> handles;
+    for (const FluidGridHIP* g : grids)
+        handles.push_back(g ? g->handle() : nullptr);
+    std::vector<fs_projection_result> raw(grids.size());
+    check(fs_batch_step((int)grids.size(), handles.data(), dts.data(), max_iterations.data(),
+        divergence_tolerances.data(), raw.data(), nullptr));
+    std::vector<FluidProjectionResult> out;
+    for (const fs_projection_result& r : raw)
+        out.push_back(FluidGridHIP::convert(r));
+    return out;
+}
+
 }  // namespace fluid_hip

@@ -99,10 +99,11 @@ def _arr(a, n=None):
 
 
 class Server:
-    """fs_server: lanes worker streams, admission within hbm_budget bytes (0 = 80% of free memory)."""
+    """fs_server: lanes worker streams, admission within hbm_budget bytes (0 = 80% of free memory); instances
+    of at most batch_cells cells (0: none) take their steps together in one fs_batch_step per lane round."""
 
-    def __init__(self, lanes: int = 4, hbm_budget: int = 0, device: int = 0):
-        cfg = FsServerConfig(int(device), int(lanes), int(hbm_budget))
+    def __init__(self, lanes: int = 4, hbm_budget: int = 0, device: int = 0, batch_cells: int = 0):
+        cfg = FsServerConfig(int(device), int(lanes), int(hbm_budget), int(batch_cells))
         h = C.c_void_p()
         call("fs_server_create", C.byref(cfg), C.byref(h))
         self._h = h

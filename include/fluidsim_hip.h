@@ -350,6 +350,21 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
  * default) batches every grid; contexts created with FS_CONTEXT_SINGLE_STREAM always do. */
 fs_status fs_context_set_batch_split(fs_context* ctx, long long solo_cells);
 
+/* Batched timestep (no reference counterpart: each instance is FluidGridCPU::step, FluidGridCPU.cpp:300-360,
+ * as fs_step runs it): fs_step on `count` grids of one context at once, each with its own dts[i],
+ * max_iterations[i] and divergence_tolerances[i].  Per capillary substep, every grid's transport, gravity and
+ * surface tension are launched one grid after another on the context's stream, while the implicit viscosity
+ * solves and the projections run batched (fs_batch_apply_viscosity's and fs_batch_project's persistent
+ * launches, with the context's split; grids with 16-bit multigrid levels project one by one).  results[i]
+ * follows fs_step's contract: a grid stops after its first substep whose projection does not converge, and
+ * dts[i] <= 0 leaves it untouched with physical_steps = 0.  Values agree with fs_step to rounding (the batched
+ * solves reduce in another order), and a grid's result does not depend on the other grids of the batch.  A
+ * grid whose step fails (transport, viscosity or solver error) stops there; statuses[i] (may be NULL)
+ * receives FS_OK or FS_E_RUNTIME per grid, and the call returns FS_E_RUNTIME with the lowest failing index's
+ * message after every other grid has been stepped. */
+fs_status fs_batch_step(int count, fs_grid* const* grids, const double* dts, const int* max_iterations,
+    const double* divergence_tolerances, fs_projection_result* results, fs_status* statuses);
+
 /* ---- device footprint (no reference counterpart) ----
  * Upper bound of the device bytes a grid of this shape allocates (fs_grid_device_bytes reads the actual
  * sum once it exists): the always-present state, pressure solver and hierarchy, plus the stages named
@@ -442,15 +457,22 @@ fs_status fs_advance_frame(fs_grid* grid, const fs_frame_settings* settings, fs_
  * admitted in arrival order while the sum of the admitted instances' fs_grid_estimate_bytes stays within
  * the HBM budget; finishing an instance frees its device memory and admits the next waiting request.
  * Each step is fs_step(dt, max_iterations, divergence_tolerance) on the instance's own grid, so a
- * request's results do not depend on what else is resident.  A step that fails, or whose projection is
- * not accepted, ends that request as failed (the scene's simulation_failed rule) without affecting others.
- * Step latency is the wall time of one step's execution on its lane. */
+ * request's results do not depend on what else is resident.  With batch_cells > 0, the instances of at
+ * most batch_cells cells take each round's step together instead: one fs_batch_step over the lane's small
+ * instances (batched viscosity and projection in persistent launches), then fs_step for each larger one.
+ * The route depends only on the request's own size, and a batched step's result does not depend on the
+ * other instances, so the results still do not depend on what else is resident; they agree with fs_step's
+ * to rounding.  Small instances are placed on the lane with the fewest resident small instances, larger ones
+ * on the lane with the fewest larger ones.  A step that fails, or whose projection is not accepted, ends
+ * that request as failed (the scene's simulation_failed rule) without affecting others.  Step latency is the
+ * wall time of one step's execution on its lane: for a batched instance, its batch's fs_batch_step. */
 typedef struct fs_server fs_server;
 
 typedef struct fs_server_config {
     int device;
     int lanes;                    /* concurrent worker streams, 1..16 */
     size_t hbm_budget_bytes;      /* admission budget; 0 = 80% of the device memory free at creation */
+    long long batch_cells;        /* instances of at most this many cells step together, 0 = none */
 } fs_server_config;
 
 typedef struct fs_request {
