@@ -94,7 +94,7 @@ class FsFrameStatistics(C.Structure):
 
 class FsServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("lanes", C.c_int), ("hbm_budget_bytes", C.c_size_t),
-                ("batch_cells", C.c_longlong)]
+                ("batch_cells", C.c_longlong), ("evict_after_steps", C.c_int)]
 
 
 class FsRequest(C.Structure):
@@ -123,7 +123,7 @@ class FsRequestReport(C.Structure):
         ("viscosity_iterations", C.c_int), ("physical_steps", C.c_int), ("max_divergence", C.c_double),
         ("queue_ms", C.c_double), ("run_ms", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
         ("step_max_ms", C.c_double), ("estimated_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-        ("setup_ms", C.c_double), ("teardown_ms", C.c_double), ("error", C.c_char * 256),
+        ("setup_ms", C.c_double), ("teardown_ms", C.c_double), ("evictions", C.c_int), ("error", C.c_char * 256),
     ]
 
 
@@ -134,6 +134,7 @@ class FsServerStats(C.Structure):
         ("budget_bytes", C.c_size_t), ("resident_bytes", C.c_size_t), ("peak_resident_bytes", C.c_size_t),
         ("elapsed_s", C.c_double), ("steps_per_s", C.c_double), ("step_p50_ms", C.c_double),
         ("step_p99_ms", C.c_double), ("setup_ms_total", C.c_double), ("teardown_ms_total", C.c_double),
+        ("evictions", C.c_longlong),
     ]
 
 

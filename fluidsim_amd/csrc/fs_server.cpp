@@ -95,6 +95,13 @@ struct Request {
     double setup_ms = 0.0;      // materialise: create, configure, upload
     double teardown_ms = 0.0;   // retire: download the final state, free the instance
     std::string error;
+    // eviction: marked by admission, carried out by the lane at the next step boundary
+    bool evict = false;
+    bool live = false;          // its grid exists (set and cleared under the server's lock)
+    int steps_since_admit = 0;
+    int evictions = 0;
+    std::string checkpoint_metadata;            // empty unless the instance was evicted
+    std::vector<unsigned char> checkpoint_payload;
     // final state, kept on the host once the instance has left the device
     std::vector<double> out_fraction, out_u, out_v, out_p, out_lmass, out_gmass;
 };
@@ -113,6 +120,9 @@ struct fs_server {
     int device = 0;
     size_t budget = 0;
     long long batch_cells = 0;   // instances of at most this many cells step batched
+    int evict_after = 0;         // resident steps before an instance may be evicted (0: never)
+    size_t evicting_bytes = 0;   // estimates of the instances marked for eviction
+    long long evictions = 0;
     std::mutex m;
     std::condition_variable finished_cv;
     std::deque<Request*> queue;
@@ -127,13 +137,16 @@ struct fs_server {
     std::vector<float> window_step_ms;
     double window_setup_ms = 0.0, window_teardown_ms = 0.0;
 
-    // Admits waiting requests in arrival order while they fit.  Caller holds m.
+    // Admits waiting requests in arrival order while they fit; when the head does not, may mark resident
+    // instances for eviction.  Caller holds m.
     void admit_locked()
     {
         while (!queue.empty() && !stopping) {
             Request* r = queue.front();
-            if (resident_bytes + r->estimate > budget)
+            if (resident_bytes + r->estimate > budget) {
+                mark_evictions_locked(r->estimate);
                 break;
+            }
             queue.pop_front();
             // small (batched) and larger instances are balanced across the lanes separately
             const bool small = batched(r);
@@ -151,11 +164,83 @@ struct fs_server {
             peak_bytes = std::max(peak_bytes, resident_bytes);
             r->state = FS_REQUEST_RUNNING;
             r->lane = best->index;
-            r->admitted = Clock::now();
-            ++admitted;
+            r->steps_since_admit = 0;
+            if (r->evictions == 0) {   // a readmitted instance keeps its first admission time
+                r->admitted = Clock::now();
+                ++admitted;
+            }
             best->active.push_back(r);
             best->wake.notify_one();
         }
+    }
+
+    // Marks the longest-resident instances that have run evict_after steps since their admission, enough
+    // of them to make room for `need` bytes -- or none, if all of them together would not.  Caller holds m.
+    void mark_evictions_locked(size_t need)
+    {
+        if (evict_after <= 0)
+            return;
+        size_t room = budget - resident_bytes + evicting_bytes;   // free once the marked ones have left
+        if (room >= need)
+            return;
+        std::vector<Request*> candidates;
+        size_t total = 0;
+        for (auto& l : lanes)
+            for (Request* a : l->active)
+                if (!a->evict && a->live && a->steps_since_admit >= evict_after && a->steps_done < a->p.steps) {
+                    candidates.push_back(a);
+                    total += a->estimate;
+                }
+        if (room + total < need)
+            return;
+        std::sort(candidates.begin(), candidates.end(),
+            [](const Request* a, const Request* b) { return a->admitted < b->admitted; });
+        for (Request* a : candidates) {
+            if (room >= need)
+                break;
+            a->evict = true;
+            evicting_bytes += a->estimate;
+            room += a->estimate;
+        }
+    }
+
+    // Checkpoints a marked instance to host memory, frees its grid and queues it again.  On a failure to
+    // save, the request ends as failed.
+    void evict(Lane& lane, Request* r)
+    {
+        const auto t0 = Clock::now();
+        size_t meta_bytes = 0, payload_bytes = 0;
+        std::string meta;
+        std::vector<unsigned char> payload;
+        fs_status st = fs_checkpoint_size(r->grid, &meta_bytes, &payload_bytes);
+        if (st == FS_OK) {
+            meta.assign(meta_bytes, '\0');
+            payload.resize(payload_bytes);
+            st = fs_checkpoint_save(r->grid, &meta[0], meta_bytes, payload.data(), payload_bytes);
+        }
+        if (st != FS_OK) {
+            retire(lane, r, false, std::string("eviction failed: ") + fs_last_error());
+            return;
+        }
+        meta.resize(std::strlen(meta.c_str()));
+        fs_grid_destroy(r->grid);
+        const double ms = ms_between(t0, Clock::now());
+        std::lock_guard<std::mutex> lock(m);
+        r->grid = nullptr;
+        r->live = false;
+        r->teardown_ms += ms;
+        window_teardown_ms += ms;
+        r->checkpoint_metadata = std::move(meta);
+        r->checkpoint_payload = std::move(payload);
+        r->evict = false;
+        evicting_bytes -= r->estimate;
+        r->state = FS_REQUEST_QUEUED;
+        ++r->evictions;
+        ++evictions;
+        lane.active.erase(std::find(lane.active.begin(), lane.active.end(), r));
+        resident_bytes -= r->estimate;
+        queue.push_back(r);
+        admit_locked();
     }
 
     bool batched(const Request* r) const
@@ -187,12 +272,20 @@ struct fs_server {
             if (!ok && error.empty())
                 error = fs_last_error();
             fs_grid_destroy(r->grid);
-            r->grid = nullptr;
         }
         const double teardown = ms_between(t0, Clock::now());
         std::lock_guard<std::mutex> lock(m);
+        r->grid = nullptr;
+        r->live = false;
+        if (r->evict) {   // marked, but it ended first
+            r->evict = false;
+            evicting_bytes -= r->estimate;
+        }
+        r->checkpoint_metadata.clear();
+        r->checkpoint_payload.clear();
+        r->checkpoint_payload.shrink_to_fit();
         r->device_bytes = bytes;
-        r->teardown_ms = teardown;
+        r->teardown_ms += teardown;
         window_teardown_ms += teardown;
         r->finished = Clock::now();
         r->state = ok ? FS_REQUEST_DONE : FS_REQUEST_FAILED;
@@ -232,7 +325,10 @@ struct fs_server {
                ok(fs_set_gravity(g, p.gravity_y)) && ok(fs_set_surface_tension(g, p.surface_tension)) &&
                ok(fs_set_transport(g, (fs_transport)p.transport)) &&
                ok(fs_set_preconditioner(g, (fs_preconditioner)p.preconditioner)) &&
-               ok(fs_upload(g, ptr(r->fraction), ptr(r->u), ptr(r->v), nullptr, ptr(r->lmass), ptr(r->gmass))) &&
+               (r->checkpoint_metadata.empty()
+                       ? ok(fs_upload(g, ptr(r->fraction), ptr(r->u), ptr(r->v), nullptr, ptr(r->lmass), ptr(r->gmass)))
+                       : ok(fs_checkpoint_load(g, r->checkpoint_metadata.c_str(), r->checkpoint_metadata.size(),
+                             r->checkpoint_payload.data(), r->checkpoint_payload.size()))) &&
                (r->solid.empty() || ok(fs_set_solid_mask(g, r->solid.data()))) &&
                ok(fs_set_multigrid_precision(g, (fs_mg_precision)p.multigrid_precision)) &&
                ok(fs_set_relative_tolerance(g, p.relative_tolerance));
@@ -251,12 +347,16 @@ struct fs_server {
             std::lock_guard<std::mutex> lock(m);
             r->step_ms.push_back(ms);
             ++r->steps_done;
+            ++r->steps_since_admit;
             r->pressure_iterations += res.iterations;
             r->viscosity_iterations += res.viscosity_iterations;
             r->physical_steps += res.physical_steps;
             r->max_divergence = res.max_divergence;
             ++window_steps;
             window_step_ms.push_back(ms);
+            // an instance that has now run its quantum may make room for the waiting head
+            if (evict_after > 0 && r->steps_since_admit >= evict_after && !queue.empty())
+                admit_locked();
         }
         if (!(res.converged || res.relaxed || res.precision_limited)) {
             retire(lane, r, false,
@@ -290,16 +390,30 @@ struct fs_server {
                     const double setup = ms_between(t0, Clock::now());
                     {
                         std::lock_guard<std::mutex> lock(m);
-                        r->setup_ms = setup;
+                        r->setup_ms += setup;
                         window_setup_ms += setup;
                     }
                     if (!made) {
                         retire(lane, r, false, error);
                         continue;
                     }
+                    std::lock_guard<std::mutex> lock(m);
+                    r->live = true;
+                    r->checkpoint_metadata.clear();
+                    r->checkpoint_payload.clear();
+                    r->checkpoint_payload.shrink_to_fit();
                 }
                 if (r->steps_done >= r->p.steps) {
                     retire(lane, r, true, "");
+                    continue;
+                }
+                bool leave;
+                {
+                    std::lock_guard<std::mutex> lock(m);
+                    leave = r->evict;
+                }
+                if (leave) {
+                    evict(lane, r);
                     continue;
                 }
                 (batched(r) ? group : solo).push_back(r);
@@ -374,7 +488,7 @@ struct fs_server {
         rep.physical_steps = r->physical_steps;
         rep.max_divergence = r->max_divergence;
         const auto now = Clock::now();
-        const bool admitted_yet = r->state != FS_REQUEST_QUEUED;
+        const bool admitted_yet = r->state != FS_REQUEST_QUEUED || r->evictions > 0;
         const bool ended = r->state == FS_REQUEST_DONE || r->state == FS_REQUEST_FAILED;
         rep.queue_ms = ms_between(r->submitted, admitted_yet ? r->admitted : now);
         rep.run_ms = admitted_yet ? ms_between(r->admitted, ended ? r->finished : now) : 0.0;
@@ -385,6 +499,7 @@ struct fs_server {
         rep.device_bytes = r->device_bytes;
         rep.setup_ms = r->setup_ms;
         rep.teardown_ms = r->teardown_ms;
+        rep.evictions = r->evictions;
         std::snprintf(rep.error, sizeof rep.error, "%s", r->error.c_str());
         *out = rep;
     }
@@ -399,9 +514,11 @@ fs_status fs_server_create(const fs_server_config* config, fs_server** out)
         need(config->lanes >= 1 && config->lanes <= 16, FS_E_INVALID_ARGUMENT, "lanes must be in 1..16");
         auto s = std::make_unique<fs_server>();
         need(config->batch_cells >= 0, FS_E_INVALID_ARGUMENT, "batch_cells must be non-negative");
+        need(config->evict_after_steps >= 0, FS_E_INVALID_ARGUMENT, "evict_after_steps must be non-negative");
         s->device = config->device;
         s->budget = config->hbm_budget_bytes;
         s->batch_cells = config->batch_cells;
+        s->evict_after = config->evict_after_steps;
         if (s->budget == 0) {
             need(hipSetDevice(config->device) == hipSuccess, FS_E_DEVICE, "no such HIP device");
             size_t free_bytes = 0, total = 0;
@@ -622,6 +739,7 @@ fs_status fs_server_get_stats(fs_server* server, fs_server_stats* stats)
         s.step_p99_ms = percentile(server->window_step_ms, 0.99);
         s.setup_ms_total = server->window_setup_ms;
         s.teardown_ms_total = server->window_teardown_ms;
+        s.evictions = server->evictions;
         *stats = s;
     });
 }

@@ -72,6 +72,7 @@ class Report:
     device_bytes: int
     setup_ms: float
     teardown_ms: float
+    evictions: int
     error: str
 
     @property
@@ -100,10 +101,13 @@ def _arr(a, n=None):
 
 class Server:
     """fs_server: lanes worker streams, admission within hbm_budget bytes (0 = 80% of free memory); instances
-    of at most batch_cells cells (0: none) take their steps together in one fs_batch_step per lane round."""
+    of at most batch_cells cells (0: none) take their steps together in one fs_batch_step per lane round;
+    with evict_after_steps > 0, instances resident for that many steps are checkpointed to host memory when
+    a waiting request needs their room, and resume from the checkpoint when readmitted."""
 
-    def __init__(self, lanes: int = 4, hbm_budget: int = 0, device: int = 0, batch_cells: int = 0):
-        cfg = FsServerConfig(int(device), int(lanes), int(hbm_budget), int(batch_cells))
+    def __init__(self, lanes: int = 4, hbm_budget: int = 0, device: int = 0, batch_cells: int = 0,
+                 evict_after_steps: int = 0):
+        cfg = FsServerConfig(int(device), int(lanes), int(hbm_budget), int(batch_cells), int(evict_after_steps))
         h = C.c_void_p()
         call("fs_server_create", C.byref(cfg), C.byref(h))
         self._h = h

@@ -463,7 +463,16 @@ fs_status fs_advance_frame(fs_grid* grid, const fs_frame_settings* settings, fs_
  * The route depends only on the request's own size, and a batched step's result does not depend on the
  * other instances, so the results still do not depend on what else is resident; they agree with fs_step's
  * to rounding.  Small instances are placed on the lane with the fewest resident small instances, larger ones
- * on the lane with the fewest larger ones.  A step that fails, or whose projection is not accepted, ends
+ * on the lane with the fewest larger ones.
+ * Eviction under HBM pressure (FluidCheckpoint's metadata + payload, FluidGridGPU.cpp:567-695): with
+ * evict_after_steps > 0, when the request at the head of the queue does not fit in the budget and the
+ * instances that have run at least evict_after_steps steps since their admission would make room, the
+ * longest-resident of them are marked; at its next step boundary each marked instance is saved with
+ * fs_checkpoint_save into host memory, destroyed (its device memory freed), and queued again at the tail.
+ * When readmitted it is rebuilt from the checkpoint (fs_checkpoint_load in place of the upload, then the
+ * request's mask, precision and stopping rule) and continues: the checkpoint holds the whole stepping state
+ * (fractions, velocities, both pressure histories, masses, the previous solve's iteration count), so an
+ * evicted request's results equal the uninterrupted ones bit for bit.  A step that fails, or whose projection is not accepted, ends
  * that request as failed (the scene's simulation_failed rule) without affecting others.  Step latency is the
  * wall time of one step's execution on its lane: for a batched instance, its batch's fs_batch_step. */
 typedef struct fs_server fs_server;
@@ -473,6 +482,7 @@ typedef struct fs_server_config {
     int lanes;                    /* concurrent worker streams, 1..16 */
     size_t hbm_budget_bytes;      /* admission budget; 0 = 80% of the device memory free at creation */
     long long batch_cells;        /* instances of at most this many cells step together, 0 = none */
+    int evict_after_steps;        /* eviction under HBM pressure after this many resident steps, 0 = never */
 } fs_server_config;
 
 typedef struct fs_request {
@@ -520,6 +530,7 @@ typedef struct fs_request_report {
     size_t device_bytes;          /* allocated by the instance at its end */
     double setup_ms;              /* wall time to create, configure and upload the instance on its lane */
     double teardown_ms;           /* wall time to download its final state and free it */
+    int evictions;                /* times the instance was checkpointed to host memory and left the device */
     char error[256];
 } fs_request_report;
 
@@ -531,6 +542,7 @@ typedef struct fs_server_stats {
     double steps_per_s;           /* steps completed in that window / elapsed_s */
     double step_p50_ms, step_p99_ms;   /* over the steps of that window */
     double setup_ms_total, teardown_ms_total;   /* lane time spent creating and retiring instances */
+    long long evictions;          /* instances checkpointed out of the device (since creation) */
 } fs_server_stats;
 
 fs_status fs_server_create(const fs_server_config* config, fs_server** out);
