@@ -34,38 +34,54 @@ sys.path.insert(0, HERE)
 METRIC = "MGPCG solves/s to 1e-6 rel. residual (1024², mixed batch); p50/p99 step latency"
 # Algorithmic bytes per fine cell of the finest post-smoothing launch, by the per-level multigrid model
 # (c = 4 bytes per FP32 level value): bilinear prolongation 2.25c (read the coarse correction, read and
-# write the fine solution) plus four red-black Gauss-Seidel sweeps of 5c each (read solution, rhs and
-# the two face conductances, write solution) = 22.25c = 89 B.  The kernel fuses all five passes through
-# LDS, so its HBM traffic (measured with rocprofv3 counters) is far lower than this.
-SMOOTH_BYTES_PER_CELL = 89
+# write the fine solution) plus nu sweeps of 5c each (read solution, rhs and the two face conductances,
+# write solution; a red-black sweep or a Jacobi sweep alike) = 22.25c = 89 B at nu = 4.  The kernels fuse
+# all passes through LDS, so their HBM traffic (measured with rocprofv3 counters) is far lower than this.
+def smooth_bytes_per_cell(nu=4):
+    return (2.25 + 5.0 * nu) * 4.0
+
+
+SMOOTH_BYTES_PER_CELL = smooth_bytes_per_cell(4)
+# Tile variants (tx, ty, threads) of the fused smoothers, as fs_mg_level_tiles numbers them.
+RB_VARIANTS = {0: (64, 32, 256), 1: (32, 32, 256), 2: (32, 16, 256), 3: (16, 16, 256), 4: (64, 32, 512),
+               5: (32, 32, 512), 6: (32, 32, 384), 7: (64, 32, 640), 8: (32, 64, 640), 9: (64, 32, 960),
+               10: (32, 64, 960)}
+JACOBI_VARIANTS = {0: (64, 32, 512), 1: (32, 32, 256), 2: (32, 16, 256), 3: (16, 16, 256), 4: (32, 32, 512),
+                   5: (64, 64, 1024), 6: (64, 32, 1024), 7: (32, 64, 1024)}
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_rb_multi_fine_post_1024_traffic.md")
+PROFILES = os.path.join(HERE, "profiles")
 TRACE_PROFILE = os.path.join(HERE, "profiles", "bench_1024_kernel_trace.md")
 SQ_PROFILE = os.path.join(HERE, "profiles", "k_rb_multi_sq_counters.md")
 TAIL_KERNEL = 6   # fs_time_kernel id of the single-workgroup coarse tail
 # Algorithmic bytes per fine cell of one whole projection, by the per-iteration model: 15 FP32 CG vector
-# passes (operator + dot 4, update + max 6, dot + direction 5; s = 4 B) plus the V(4,4) cycle's
-# (10 nu + 5.5) c = 45.5 c per level over a hierarchy summing to 4/3 of the fine level (c = 4 B):
+# passes (operator + dot 4, update + max 6, dot + direction 5; s = 4 B) plus the V(nu,nu) cycle's
+# (10 nu + 5.5) c per level over a hierarchy summing to 4/3 of the fine level (c = 4 B): at nu = 4,
 # 60 + 242.7 = 302.7 B per PCG iteration and per V-cycle; one extra V-cycle, rz and direction start
 # the solve, and set-up, rhs, correction and the FP64 divergence check add about 65 B.
-SOLVE_BYTES_PER_CELL_ITERATION = 302.7
+def solve_bytes_per_cell_iteration(nu=4):
+    return round(60.0 + (10.0 * nu + 5.5) * 4.0 * 4.0 / 3.0, 1)
+
+
+SOLVE_BYTES_PER_CELL_ITERATION = solve_bytes_per_cell_iteration(4)
 SOLVE_SETUP_BYTES_PER_CELL = 65.0
 
 
 def measured_traffic(kernel, size):
-    """HBM bytes per launch of the fine post-smoother from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None when that summary covers
-    another kernel (the autotuner chose another tile) or another grid size."""
+    """HBM bytes per launch of the fine post-smoother from a committed rocprofv3 counter summary
+    (profiles/*_traffic.md: FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size) whose title names
+    this kernel and grid size, or None when none does (the autotuner chose another tile or smoother)."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
-            lines = f.read().splitlines()
+        names = sorted(n for n in os.listdir(PROFILES) if n.endswith("_traffic.md"))
     except OSError:
         return None
-    if not lines or f"`{kernel}`" not in lines[0] or f"{size} x {size}" not in lines[0]:
-        return None
-    for line in lines:
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
+    for name in names:
+        with open(os.path.join(PROFILES, name)) as f:
+            lines = f.read().splitlines()
+        if not lines or f"`{kernel}`" not in lines[0] or f"{size} x {size}" not in lines[0]:
+            continue
+        for line in lines:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
     return None
 
 
@@ -341,11 +357,11 @@ def semi_lagrangian_leg(size=1024, steps=200, seed=2):
             "pcg_iterations_mean": float(np.mean(its)), "final_max_divergence_over_tol": div / tau}
 
 
-def smoother_model_bytes(coef_bytes, coarse_bytes):
-    """Algorithmic bytes per fine cell of the finest post-smoothing launch: four sweeps of (read solution
+def smoother_model_bytes(coef_bytes, coarse_bytes, nu=4):
+    """Algorithmic bytes per fine cell of the finest post-smoothing launch: nu sweeps of (read solution
     and rhs, FP32 on the finest level, read two conductances, write the solution) plus the bilinear
     prolongation (read a quarter coarse value, read and write the fine solution)."""
-    return 4 * (12 + 2 * coef_bytes) + 8 + 0.25 * coarse_bytes
+    return nu * (12 + 2 * coef_bytes) + 8 + 0.25 * coarse_bytes
 
 
 def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, only=None):
@@ -368,8 +384,9 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
     wl = WL.obstacle_tank(size, size, seed=seed)
     mask = wl.meta["solid"]
     note(f"generated the {size}^2 obstacle tank ({mask.mean():.3f} solid)")
-    modes = [("fp32", FS_MG_FP32, 4, 4), ("fp16", FS_MG_FP16, 2, 4), ("bf16", FS_MG_BF16, 2, 4),
-             ("bf16_vectors", FS_MG_BF16_VECTORS, 2, 2)]
+    # (name, storage, conductance bytes, coarse-vector bytes); fp32_tuned adds the smoother the tuner picks
+    modes = [("fp32", FS_MG_FP32, 4, 4), ("fp32_tuned", FS_MG_FP32, 4, 4), ("fp16", FS_MG_FP16, 2, 4),
+             ("bf16", FS_MG_BF16, 2, 4), ("bf16_vectors", FS_MG_BF16_VECTORS, 2, 2)]
     if only:
         modes = [m for m in modes if m[0] in only]
     out = {"workload": f"obstacle_tank {size}x{size}: 40 circles/boxes over {mask.mean():.3f} of the cells, "
@@ -408,6 +425,11 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
         g.setMultigridPrecision(mode)
         tau = 1e-6 * g.maxDivergence()
         budget = stall_budget if mode == FS_MG_BF16_VECTORS else 2000
+        tuned = None
+        if name == "fp32_tuned":
+            kind, nu, _ = g.tunePressureSmoother(wl.dt, budget, tau)
+            tuned = {"smoother": "chebyshev" if kind == 1 else "red_black", "sweeps": nu}
+            note(f"tuned smoother: {tuned}")
         g.snapshotSave()
         g.snapshotRestore()
         g.project(wl.dt, budget, tau)        # warm-up: hierarchy, graph capture
@@ -421,7 +443,7 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
         u = st.u
         pg = st.pressure[fluid] - st.pressure[fluid].mean()
         post_ms = g.timeKernel(0, kernel_reps)
-        model = smoother_model_bytes(coef_bytes, coarse_bytes)
+        model = smoother_model_bytes(coef_bytes, coarse_bytes, tuned["sweeps"] if tuned else 4)
         r = results[-1]
         entry = {
             "solves_per_s": steps / elapsed,
@@ -434,6 +456,8 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
             "fine_post_smoother_model_gbs": model * size * size / (post_ms * 1e-3) / 1e9,
             "rel_l2_vs_fp64_oracle": {"u": rel(st.u, so["u"]), "v": rel(st.v, so["v"]), "pressure": rel(pg, po)},
         }
+        if tuned:
+            entry["smoother"] = tuned
         if ref_u is None:
             ref_u = u
         else:
@@ -575,6 +599,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--smoother", type=str, default="tuned",
+                    help="tuned (fs_tune_pressure_smoother), red_black:N (N = 1..4) or chebyshev:N (N = 1..5)")
     ap.add_argument("--fine-tiles", type=str, default="", help="force the finest level's tile variants "
                     "PRE,POST,RESTRICT (profiling runs: the autotuner's choice can shift under a profiler)")
     ap.add_argument("--serving-requests", type=int, default=24, help="size of the C5 serving stream (0: skip)")
@@ -597,7 +623,7 @@ def main():
     ap.add_argument("--precision-only", action="store_true")
     ap.add_argument("--sl-steps", type=int, default=200, help="C2 semi-Lagrangian steps leg (0: skip)")
     ap.add_argument("--c1-steps", type=int, default=100, help="C1 dam-break steps leg, GPU and CPU (0: skip)")
-    ap.add_argument("--precision-modes", type=str, default="", help="subset of fp32,fp16,bf16,bf16_vectors")
+    ap.add_argument("--precision-modes", type=str, default="", help="subset of fp32,fp32_tuned,fp16,bf16,bf16_vectors")
     args = ap.parse_args()
     splits = tuple(int(v) for v in args.batch_splits.split(",") if v.strip())
 
@@ -607,7 +633,7 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
-    from fluidsim_amd import FluidGridHIP
+    from fluidsim_amd import FS_SMOOTHER_CHEBYSHEV, FS_SMOOTHER_RED_BLACK, FluidGridHIP
     from fluidsim_amd import workloads as WL
 
     rates = [float(x) for x in args.serving_rates.split(",") if x]
@@ -630,9 +656,18 @@ def main():
     wl = WL.vortex_box(args.size, args.size, seed=2)
     grid = FluidGridHIP(wl.width, wl.height, wl.cell_size)
     WL.load_into(grid, wl)
+    tau = 1e-6 * grid.maxDivergence()  # 1e-6 of max |div u*| = 1e-6 relative residual (max-norm)
+    # The smoother of the levels above 4096 cells: chosen by fs_tune_pressure_smoother on this input (each
+    # candidate projects the same state, which the tuner restores), or fixed on the command line.
+    tuned_ms = None
+    if args.smoother == "tuned":
+        kind, nu, tuned_ms = grid.tunePressureSmoother(wl.dt, 200, tau)
+    else:
+        name, _, count = args.smoother.partition(":")
+        kind, nu = (FS_SMOOTHER_CHEBYSHEV if name == "chebyshev" else FS_SMOOTHER_RED_BLACK), int(count or 4)
+        grid.setPressureSmoother(kind, nu)
     if args.fine_tiles:
         grid.setMultigridTiles(0, *[int(v) for v in args.fine_tiles.split(",")])
-    tau = 1e-6 * grid.maxDivergence()  # 1e-6 of max |div u*| = 1e-6 relative residual (max-norm)
     grid.snapshotSave()
 
     def one_step():
@@ -659,22 +694,30 @@ def main():
     iters = [r.iterations for r in results]
 
     # Dominant kernel: the finest-level post-smoothing launch, timed with HIP events on the grid's stream.
-    variants = {0: (64, 32, 256), 1: (32, 32, 256), 2: (32, 16, 256), 3: (16, 16, 256), 4: (64, 32, 512),
-                5: (32, 32, 512), 6: (32, 32, 384), 7: (64, 32, 640), 8: (32, 64, 640),
-                9: (64, 32, 960), 10: (32, 64, 960)}
     post_variant = grid.multigridTiles(0)[1]
-    tx, ty, nt = variants[post_variant]
-    fine_kernel = f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float>"
+    chebyshev = kind == FS_SMOOTHER_CHEBYSHEV
+    if chebyshev:
+        tx, ty, nt = JACOBI_VARIANTS[post_variant]
+        fine_kernel = f"k_jacobi_multi<{tx},{ty},1,{nt}>"
+        smoother = (f"Chebyshev-weighted damped Jacobi, {nu} sweeps per side, on the levels above 4096 cells "
+                    "(red-black Gauss-Seidel V(4,4) below)")
+        fine_what = f"finest-level prolongation + {nu} Chebyshev-weighted Jacobi sweeps, autotuned tile"
+    else:
+        tx, ty, nt = RB_VARIANTS[post_variant]
+        fine_kernel = f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float>"
+        smoother = f"red-black Gauss-Seidel, {nu} sweeps per side on the levels above 4096 cells (4 below)"
+        fine_what = f"finest-level prolongation + {nu} red-black Gauss-Seidel sweeps, autotuned tile"
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
     tail_ms = grid.timeKernel(TAIL_KERNEL, args.kernel_reps)
-    sweep_bytes = SMOOTH_BYTES_PER_CELL * wl.width * wl.height
+    sweep_bytes = smooth_bytes_per_cell(nu) * wl.width * wl.height
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     # Reference point: a plain FP32 copy of 64 MiB into another 64 MiB buffer on the same stream.
     copy_ms = grid.timeKernel(10, 20)
     copy_bytes = 2 * 4 * (16 << 20)
     traffic = measured_traffic(fine_kernel, args.size)
     cells = wl.width * wl.height
-    solve_bytes = cells * (SOLVE_BYTES_PER_CELL_ITERATION * (float(np.median(iters)) + 1.0) + SOLVE_SETUP_BYTES_PER_CELL)
+    per_iteration = solve_bytes_per_cell_iteration(nu)
+    solve_bytes = cells * (per_iteration * (float(np.median(iters)) + 1.0) + SOLVE_SETUP_BYTES_PER_CELL)
 
     out = {
         "metric": METRIC,
@@ -693,7 +736,9 @@ def main():
             "workload": f"vortex_box {wl.width}x{wl.height}: one projection per step to 1e-6 relative "
                         "max-norm residual (64 vortices + 16 sources, seed 2, fully liquid, closed box)",
             "grid": [wl.width, wl.height],
-            "preconditioner": "multigrid V(4,4) red-black Gauss-Seidel, FP32 levels",
+            "preconditioner": f"multigrid V-cycle, FP32 levels: {smoother}",
+            "smoother_choice": ("fs_tune_pressure_smoother on this input (fastest converging of red-black 2-4 and "
+                                f"Chebyshev 3-5 sweeps: {tuned_ms:.3f} ms per projection)") if tuned_ms else "fixed",
             "parallelism": "single GPU",
         },
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
@@ -701,7 +746,7 @@ def main():
         "converged_all": bool(ok),
         "max_rel_residual": float(max(rel)),
         "roofline": {
-            "kernel": fine_kernel + " (finest-level prolongation + 4 red-black Gauss-Seidel sweeps, autotuned tile)",
+            "kernel": f"{fine_kernel} ({fine_what})",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -711,6 +756,9 @@ def main():
             "avg_launch_us": sweep_ms * 1e3,
             "algorithmic_bytes_per_launch": sweep_bytes,
             "copy_kernel_gbs": copy_bytes / (copy_ms * 1e-3) / 1e9,
+            "model_note": "algorithmic bytes count every sweep's pass through HBM, as unfused sweeps would make "
+                          "it; the fused kernel reads each value once, so frac can exceed 1 (frac_of_measured_"
+                          "traffic is the HBM figure)",
         },
         "solve_roofline": {
             "bound": "hbm",
@@ -719,18 +767,23 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": solve_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
-            "model": f"{SOLVE_BYTES_PER_CELL_ITERATION} B per fine cell per PCG iteration (and per starting "
+            "model": f"{per_iteration} B per fine cell per PCG iteration (and per starting "
                      f"V-cycle) + {SOLVE_SETUP_BYTES_PER_CELL:.0f} B set-up",
         },
     }
     if traffic:
         # the fused smoother moves far fewer bytes than its model: what limits it is instruction issue
         out["roofline"]["frac_of_measured_traffic"] = traffic / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-        out["roofline"]["limiter"] = (f"instruction issue and latency, not HBM: measured HBM traffic is "
-                                      f"{traffic / sweep_bytes:.2f} of the model bytes (rocprofv3 FETCH_SIZE + "
-                                      "WRITE_SIZE); SQ counters show VALU the busiest pipe and waves waiting "
-                                      "on latency at 4 waves per SIMD")
-        if os.path.exists(SQ_PROFILE):
+        out["roofline"]["limiter"] = (f"not HBM: measured HBM traffic is {traffic / sweep_bytes:.2f} of the model "
+                                      "bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE)")
+        try:
+            with open(SQ_PROFILE) as f:
+                covered = f"`<{fine_kernel.split('<', 1)[1].split(',float')[0]}>`" in f.read().replace(" ", "")
+        except OSError:
+            covered = False
+        if covered:
+            out["roofline"]["limiter"] += ("; SQ counters show instruction issue and latency: VALU the busiest pipe, "
+                                           "waves waiting a third or more of their time")
             out["roofline"]["limiter_source"] = os.path.relpath(SQ_PROFILE, HERE)
     top = traced_kernels()
     if top:

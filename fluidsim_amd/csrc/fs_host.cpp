@@ -272,6 +272,8 @@ struct Level {
     int smooth_variant = 0;     // tile of k_rb_multi pre-smoothing on this level (autotuned)
     int post_variant = 0;       // tile of k_rb_multi prolongation + post-smoothing (autotuned)
     int restrict_variant = 0;   // tile of k_res_restrict from this level (autotuned)
+    int jacobi_pre = 0;         // tiles of k_jacobi_multi pre- and post-smoothing (autotuned on first use)
+    int jacobi_post = 0;
 
     size_t padded() const { return (size_t)s * (h + 2); }
 };
@@ -393,6 +395,18 @@ struct fs_context {
 
 constexpr int kPressureBudget = 100000;   // PCG iterations of an unbounded (negative) budget
 constexpr size_t kFoldBlocks = 1536;      // cell blocks (64 x 16) beyond which PCG partials are pre-reduced
+// Multigrid levels above this many cells take the grid's chosen smoother (fs_set_pressure_smoother); smaller
+// ones, the single-workgroup tail's levels among them, always run red-black V(4,4).
+constexpr long long kSmootherMinCells = 4096;
+
+// Weight i of `sweeps` Chebyshev-weighted Jacobi sweeps: PressureMultigridGPU's constructor
+// (PressureMultigridGPU.cpp:24-28), 1 / (1.1 + 0.9 cos(pi (i + 1/2) / sweeps)) -- the reciprocals of the
+// Chebyshev nodes of [0.2, 2], the spectrum it assumes for D^-1 A -- rounded to float.
+float chebyshev_weight(int i, int sweeps)
+{
+    const double pi = 3.14159265358979323846;
+    return (float)(1.0 / (1.1 + 0.9 * std::cos(pi * (i + 0.5) / sweeps)));
+}
 constexpr size_t kBatchSpreadLds = 84 * 1024;   // LDS of a batched-solve workgroup: one per compute unit
 
 // A batched solve split by fs_context_set_batch_split: grids above the split take their single-grid solve
@@ -458,6 +472,9 @@ struct fs_grid {
     fs_mg_precision mg_prec = FS_MG_FP32;
     bool flexible_cg = false;   // fs_set_flexible_cg
     bool relative_tol = false;  // fs_set_relative_tolerance
+    fs_pressure_smoother smoother = FS_SMOOTHER_RED_BLACK;   // fs_set_pressure_smoother: levels above
+    int smooth_sweeps = 4;                                    // kSmootherMinCells cells
+    bool jacobi_tuned = false;
     bool uploaded = false;
     bool coeff_valid = false;
     bool mg_valid = false;
@@ -661,6 +678,8 @@ struct fs_grid {
         bytes += partials.alloc(6 * blocks + 64);
         build_hierarchy();
         tune_tiles();
+        if (smoother == FS_SMOOTHER_CHEBYSHEV)
+            tune_jacobi();
     }
 
     void fill_tail(int start)
@@ -862,6 +881,11 @@ struct fs_grid {
 
     // ---- 16-bit multigrid levels ----
     bool active16() const { return mg_prec != FS_MG_FP32 && tail_start >= 1; }
+    // The batched solve runs the reference's red-black V(4,4) on every level.
+    bool default_cycle() const
+    {
+        return (smoother == FS_SMOOTHER_RED_BLACK && smooth_sweeps == 4) || (long long)W * H <= kSmootherMinCells;
+    }
 
     // Allocates (or releases) the 16-bit arrays of the current mode and points the tail's top level at
     // the storage its neighbouring tiled level uses.
@@ -949,10 +973,11 @@ struct fs_grid {
         const CoarseLinkT<VL> link = from_zero ? CoarseLinkT<VL>{} : link_t<VL>(l);
         if (l == 0)
             launch_rb_multi_t<C, float, VL>(st(), view_t<C, float>(0), cur, r.p, rmean_ptr(), first_color, from_zero,
-                link, status, variant);
+                link, status, variant, sweeps_on(l));
         else {
             const LevelViewT<C, VL> L = view_t<C, VL>(l);
-            launch_rb_multi_t<C, VL, VL>(st(), L, cur, L.rhs, nullptr, first_color, from_zero, link, status, variant);
+            launch_rb_multi_t<C, VL, VL>(st(), L, cur, L.rhs, nullptr, first_color, from_zero, link, status, variant,
+                sweeps_on(l));
         }
     }
 
@@ -1038,12 +1063,10 @@ struct fs_grid {
             smooth16<bf16, float>(0, cur, 1, 0, nullptr, levels[0].post_variant);
         else if (active16())
             smooth16<bf16, bf16>(0, cur, 1, 0, nullptr, levels[0].post_variant);
-        else {
-            CoarseLink link{};
-            if (tail_start > 0)
-                link = link_t<float>(0);
-            launch_rb_multi(st(), view(0), cur, r.p, rmean_ptr(), 1, 0, link, nullptr, 1, levels[0].post_variant);
-        }
+        else if (tail_start > 0)
+            smooth_level(0, cur, r.p, rmean_ptr(), false, link_t<float>(0), nullptr, -1);
+        else
+            launch_rb_multi(st(), view(0), cur, r.p, rmean_ptr(), 1, 0, CoarseLink{}, nullptr, 1, levels[0].post_variant);
     }
 
     // One V(4,4) cycle on the residual r (minus its mean) into levels[0].sol0.
@@ -1065,7 +1088,7 @@ struct fs_grid {
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
             const Level& F = levels[l];
-            launch_rb_multi(st(), L, 0, rhs, shift, 0, 1, none, status, l == 0, F.smooth_variant);
+            smooth_level(l, 0, rhs, shift, true, none, status, -1);
             AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
             AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
             launch_res_restrict(st(), L, 1, rhs, shift, view(l + 1), rx, ry, status, F.restrict_variant);
@@ -1073,13 +1096,86 @@ struct fs_grid {
         launch_mg_tail(st(), tail, tail_lds, T == 0 ? rmean_ptr() : nullptr, status);
         for (int l = T - 1; l >= 0; --l) {
             LevelView L = view(l);
-            const Level& F = levels[l];
-            const Level& Cl = levels[l + 1];
-            CoarseLink link{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
-            launch_rb_multi(st(), L, 1, rhs, shift, 1, 0, link, status, l == 0, F.post_variant);
+            smooth_level(l, 1, rhs, shift, false, link_t<float>(l), status, -1);
         }
+    }
+
+    // Smoother of tiled level l: the chosen one above kSmootherMinCells cells, red-black V(4,4) below.
+    bool chebyshev_on(int l) const
+    {
+        return smoother == FS_SMOOTHER_CHEBYSHEV && (long long)levels[l].w * levels[l].h > kSmootherMinCells;
+    }
+    int sweeps_on(int l) const
+    {
+        return (long long)levels[l].w * levels[l].h > kSmootherMinCells ? smooth_sweeps : 4;
+    }
+    // Weights of the Chebyshev sweeps in the order a pre- (or, reversed, a post-) smoothing applies them.
+    SmoothWeights chebyshev_weights(bool pre) const
+    {
+        SmoothWeights wt{};
+        for (int k = 0; k < smooth_sweeps; ++k)
+            wt.w[pre ? k : smooth_sweeps - 1 - k] = chebyshev_weight(k, smooth_sweeps);
+        return wt;
+    }
+    // Pre-smoothing (from zero: reads nothing, writes sol[cur ^ 1]) or prolongation + post-smoothing of
+    // tiled level l in FP32 storage; variant -1 takes the level's tuned tile.
+    void smooth_level(int l, int cur, const float* rhs, const double* shift, bool pre, const CoarseLink& link,
+        const int* status, int variant)
+    {
+        const Level& F = levels[l];
+        if (chebyshev_on(l)) {
+            launch_jacobi_multi(st(), view(l), cur, rhs, shift, pre ? 1 : 0, chebyshev_weights(pre), smooth_sweeps,
+                link, status, variant >= 0 ? variant : pre ? F.jacobi_pre : F.jacobi_post);
+        } else {
+            launch_rb_multi(st(), view(l), cur, rhs, shift, pre ? 0 : 1, pre ? 1 : 0, link, status, l == 0,
+                variant >= 0 ? variant : pre ? F.smooth_variant : F.post_variant, sweeps_on(l));
+        }
+    }
+
+    // Autotuner of the Chebyshev smoother's tiles (first use): every variant on each level's buffers.
+    void tune_jacobi()
+    {
+        auto time = [&](auto&& launch) {
+            launch();
+            HIPC(hipEventRecord(ctx->ev0, st()));
+            for (int k = 0; k < 3; ++k)
+                launch();
+            HIPC(hipEventRecord(ctx->ev1, st()));
+            HIPC(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            return ms;
+        };
+        const CoarseLink none{};
+        const SmoothWeights pre = chebyshev_weights(true), post = chebyshev_weights(false);
+        for (int l = 0; l < tail_start; ++l) {
+            Level& F = levels[l];
+            if ((long long)F.w * F.h <= kSmootherMinCells)
+                continue;
+            const LevelView L = view(l);
+            const float* rhs = l == 0 ? r.p : L.rhs;
+            const CoarseLink link = link_t<float>(l);
+            float best = std::numeric_limits<float>::infinity();
+            for (int v = 0; v < kJacobiVariants; ++v) {
+                const float ms = time([&] { launch_jacobi_multi(st(), L, 0, rhs, nullptr, 1, pre, smooth_sweeps, none, nullptr, v); });
+                if (ms < best) {
+                    best = ms;
+                    F.jacobi_pre = v;
+                }
+            }
+            best = std::numeric_limits<float>::infinity();
+            for (int v = 0; v < kJacobiVariants; ++v) {
+                const float ms = time([&] { launch_jacobi_multi(st(), L, 1, rhs, nullptr, 0, post, smooth_sweeps, link, nullptr, v); });
+                if (ms < best) {
+                    best = ms;
+                    F.jacobi_post = v;
+                }
+            }
+        }
+        jacobi_tuned = true;
+        drop_graph();
     }
 
     // Autotuner: times every tile variant of the level smoother (plain sweeps) and of the residual +
@@ -1807,11 +1903,139 @@ fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode)
             FS_E_INVALID_ARGUMENT, "unknown multigrid precision");
         if (mode == grid->mg_prec)
             return;
+        need(mode == FS_MG_FP32 || grid->smoother == FS_SMOOTHER_RED_BLACK, FS_E_UNSUPPORTED,
+            "16-bit multigrid levels run the red-black smoother: select FS_SMOOTHER_RED_BLACK first");
         set_device(grid->ctx);
         grid->drain();
         grid->mg_prec = mode;
         grid->apply_precision();
         grid->tune_for_precision();
+    });
+}
+
+fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(kind == FS_SMOOTHER_RED_BLACK || kind == FS_SMOOTHER_CHEBYSHEV, FS_E_INVALID_ARGUMENT,
+            "unknown pressure smoother");
+        need(sweeps >= 1 && sweeps <= (kind == FS_SMOOTHER_RED_BLACK ? 4 : kJacobiMaxSweeps), FS_E_INVALID_ARGUMENT,
+            "sweeps must be 1..4 (red-black) or 1..5 (Chebyshev)");
+        need(kind == FS_SMOOTHER_RED_BLACK || grid->mg_prec == FS_MG_FP32, FS_E_UNSUPPORTED,
+            "the Chebyshev smoother runs FP32 multigrid levels: set the multigrid precision to FS_MG_FP32");
+        if (kind == grid->smoother && sweeps == grid->smooth_sweeps)
+            return;
+        set_device(grid->ctx);
+        grid->drain();
+        grid->smoother = kind;
+        grid->smooth_sweeps = sweeps;
+        grid->drop_graph();
+        if (kind == FS_SMOOTHER_CHEBYSHEV && !grid->levels.empty())
+            grid->tune_jacobi();
+    });
+}
+
+fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
+    fs_pressure_smoother* kind, int* sweeps, double* ms)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before tuning");
+        need(std::isfinite(dt) && dt > 0.0, FS_E_INVALID_ARGUMENT, "dt must be finite and positive");
+        fs_grid& g = *grid;
+        set_device(g.ctx);
+        g.drain();
+        // the state a projection changes: velocities, both pressure histories, the iteration prediction
+        DevBuf<double> su, sv, sp, spp;
+        su.alloc(g.u.n);
+        sv.alloc(g.v.n);
+        sp.alloc(g.p.n);
+        spp.alloc(g.p_prev.n);
+        auto copy = [&](double* dst, const double* src, size_t n) {
+            if (n)
+                HIPC(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
+        };
+        auto save = [&] {
+            copy(su.p, g.u.p, g.u.n);
+            copy(sv.p, g.v.p, g.v.n);
+            copy(sp.p, g.p.p, g.p.n);
+            copy(spp.p, g.p_prev.p, g.p_prev.n);
+        };
+        const int predicted = g.predicted;
+        auto restore = [&] {
+            copy(g.u.p, su.p, g.u.n);
+            copy(g.v.p, sv.p, g.v.n);
+            copy(g.p.p, sp.p, g.p.n);
+            copy(g.p_prev.p, spp.p, g.p_prev.n);
+            g.predicted = predicted;
+        };
+        auto choose = [&](fs_pressure_smoother k, int n) {
+            g.smoother = k;
+            g.smooth_sweeps = n;
+            g.drop_graph();
+            if (k == FS_SMOOTHER_CHEBYSHEV && !g.jacobi_tuned)
+                g.tune_jacobi();
+        };
+        save();
+        struct Candidate {
+            fs_pressure_smoother kind;
+            int sweeps;
+        };
+        std::vector<Candidate> candidates = {{FS_SMOOTHER_RED_BLACK, 4}, {FS_SMOOTHER_RED_BLACK, 3},
+            {FS_SMOOTHER_RED_BLACK, 2}};
+        if (g.mg_prec == FS_MG_FP32)
+            candidates.insert(candidates.end(),
+                {{FS_SMOOTHER_CHEBYSHEV, 3}, {FS_SMOOTHER_CHEBYSHEV, 4}, {FS_SMOOTHER_CHEBYSHEV, 5}});
+        const fs_pressure_smoother kind0 = g.smoother;
+        const int sweeps0 = g.smooth_sweeps;
+        Candidate best{kind0, sweeps0};
+        double best_ms = std::numeric_limits<double>::infinity();
+        try {
+            for (const Candidate& c : candidates) {
+                choose(c.kind, c.sweeps);
+                double fastest = std::numeric_limits<double>::infinity();
+                bool converged = true;
+                for (int rep = 0; rep < 3; ++rep) {   // the first is a warm-up (graph capture, first replays)
+                    restore();
+                    HIPC(hipStreamSynchronize(g.st()));
+                    const auto t0 = std::chrono::steady_clock::now();
+                    const fs_projection_result res = g.project(dt, max_iterations, divergence_tolerance);
+                    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    converged = converged && res.converged;
+                    if (rep > 0)
+                        fastest = std::min(fastest, t);
+                }
+                if (converged && fastest < best_ms) {
+                    best_ms = fastest;
+                    best = c;
+                }
+            }
+        } catch (...) {
+            choose(kind0, sweeps0);
+            restore();
+            HIPC(hipStreamSynchronize(g.st()));
+            throw;
+        }
+        choose(best.kind, best.sweeps);
+        restore();
+        HIPC(hipStreamSynchronize(g.st()));
+        if (kind)
+            *kind = best.kind;
+        if (sweeps)
+            *sweeps = best.sweeps;
+        if (ms)
+            *ms = best_ms;
+    });
+}
+
+fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        if (kind)
+            *kind = grid->smoother;
+        if (sweeps)
+            *sweeps = grid->smooth_sweeps;
     });
 }
 
@@ -2791,10 +3015,11 @@ fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* 
         need(level >= 0 && level < (int)grid->levels.size(), FS_E_INVALID_ARGUMENT, "no such level");
         const Level& L = grid->levels[(size_t)level];
         const bool fused = level < grid->tail_start;
+        const bool jacobi = fused && grid->chebyshev_on(level);
         if (pre)
-            *pre = fused ? L.smooth_variant : -1;
+            *pre = fused ? (jacobi ? L.jacobi_pre : L.smooth_variant) : -1;
         if (post)
-            *post = fused ? L.post_variant : -1;
+            *post = fused ? (jacobi ? L.jacobi_post : L.post_variant) : -1;
         if (restriction)
             *restriction = fused ? L.restrict_variant : -1;
     });
@@ -2805,14 +3030,16 @@ fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int
     return guard([&] {
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
         need(level >= 0 && level < grid->tail_start, FS_E_INVALID_ARGUMENT, "no tiled level with this index");
-        need(pre >= -1 && pre < kSmoothVariants && post >= -1 && post < kSmoothVariants && restriction >= -1 &&
+        const bool jacobi = grid->chebyshev_on(level);
+        const int smooth_variants = jacobi ? kJacobiVariants : kSmoothVariants;
+        need(pre >= -1 && pre < smooth_variants && post >= -1 && post < smooth_variants && restriction >= -1 &&
                  restriction < kRestrictVariants,
             FS_E_INVALID_ARGUMENT, "unknown tile variant");
         Level& L = grid->levels[(size_t)level];
         if (pre >= 0)
-            L.smooth_variant = pre;
+            (jacobi ? L.jacobi_pre : L.smooth_variant) = pre;
         if (post >= 0)
-            L.post_variant = post;
+            (jacobi ? L.jacobi_post : L.post_variant) = post;
         if (restriction >= 0)
             L.restrict_variant = restriction;
         grid->drop_graph();   // the captured iteration holds the old tiles
@@ -3067,9 +3294,12 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
         if (count == 0)
             return;
         fs_context* ctx = batch_context(count, grids, "Upload the grid state before stepping or projecting");
-        for (int i = 0; i < count; ++i)
+        for (int i = 0; i < count; ++i) {
             need(!grids[i]->active16(), FS_E_UNSUPPORTED,
                 "batched solves run FP32 multigrid levels: set the grid's multigrid precision to FS_MG_FP32");
+            need(grids[i]->default_cycle(), FS_E_UNSUPPORTED,
+                "batched solves run the red-black V(4,4) cycle: set the grid's smoother to FS_SMOOTHER_RED_BLACK, 4");
+        }
         for (int i = 0; i < count; ++i) {
             results[i] = fs_projection_result{};
             results[i].max_divergence = std::numeric_limits<double>::infinity();
@@ -3176,10 +3406,10 @@ fs_status fs_batch_step(int count, fs_grid* const* grids, const double* dts, con
                         g.apply_surface_tension(step[(size_t)i]);
                 });
             keep(active);
-            // the projections: FP32-level grids in one batched solve, 16-bit-level grids one by one
+            // the projections: FP32-level grids with the default cycle in one batched solve, the others one by one
             std::vector<int> batched, alone;
             for (int i : active)
-                (grids[i]->active16() ? alone : batched).push_back(i);
+                (grids[i]->active16() || !grids[i]->default_cycle() ? alone : batched).push_back(i);
             batch_projection(ctx, grids, batched, step.data(), max_iterations, divergence_tolerances, proj.data(), errors);
             for (int i : alone)
                 attempt(i, [&] {

@@ -171,14 +171,29 @@ void launch_pcg_direction(hipStream_t st, const GridDev& g, const float* z, int 
 void launch_precond_diag(hipStream_t st, const GridDev& g);
 
 // ---- multigrid V-cycle pieces (FP32) ----
-// Four red-black sweeps in one launch (reads sol[cur], writes sol[cur ^ 1]); with link.csol set the
-// coarse correction is prolonged and added first.  fine selects the finest-level symbol.
+// nu (1..4, default 4) red-black sweeps in one launch (reads sol[cur], writes sol[cur ^ 1]); with link.csol
+// set the coarse correction is prolonged and added first.  fine selects the finest-level symbol.
 // variant: interior tile and threads 0 = 64 x 32 / 256, 1 = 32 x 32 / 256, 2 = 32 x 16 / 256,
 // 3 = 16 x 16 / 256, 4 = 64 x 32 / 512, 5 = 32 x 32 / 512, 6 = 32 x 32 / 384, 7 = 64 x 32 / 640,
 // 8 = 32 x 64 / 640, 9 = 64 x 32 / 960, 10 = 32 x 64 / 960 (same results, different speed).
 constexpr int kSmoothVariants = 11;
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
-    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant);
+    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant, int nu = 4);
+// Chebyshev-weighted damped Jacobi (the reference WebGPU multigrid's polynomial smoother,
+// PressureMultigridGPU.cpp:24-28, 619-631; PressureSmoothingKernels.h:91-176): nu (1..kJacobiMaxSweeps)
+// sweeps x <- x + w_k (b - A x) / diag in one launch on an LDS tile with an nu-cell halo, reading sol[cur]
+// (from_zero: x = 0) and writing sol[cur ^ 1]; with link.csol set the coarse correction is prolonged and
+// added first.  A cell without conductance is set to zero.  weights.w[k] is sweep k's weight (the caller
+// reverses them for post-smoothing).  variant: 0 = 64 x 32 / 512, 1 = 32 x 32 / 256, 2 = 32 x 16 / 256,
+// 3 = 16 x 16 / 256, 4 = 32 x 32 / 512, 5 = 64 x 64 / 1024, 6 = 64 x 32 / 1024, 7 = 32 x 64 / 1024 (same
+// results, different speed).
+constexpr int kJacobiMaxSweeps = 5;
+constexpr int kJacobiVariants = 8;
+struct SmoothWeights {
+    float w[kJacobiMaxSweeps];
+};
+void launch_jacobi_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
+    int from_zero, const SmoothWeights& weights, int nu, const CoarseLink& link, const int* status, int variant);
 // Residual of sol[cur] and its restriction into C.rhs in one launch; variant: coarse tile 0 = 32 x 16,
 // 1 = 16 x 16, 2 = 16 x 8, 3 = 8 x 8.
 constexpr int kRestrictVariants = 4;
@@ -198,7 +213,7 @@ template <class C> void launch_level_pack(hipStream_t st, int w, int h, const do
 // (bf16, float, bf16) and (bf16, bf16, bf16).
 template <class C, class V, class VC>
 void launch_rb_multi_t(hipStream_t st, const LevelViewT<C, V>& L, int cur, const V* rhs, const double* shift,
-    int first_color, int from_zero, const CoarseLinkT<VC>& link, const int* status, int variant);
+    int first_color, int from_zero, const CoarseLinkT<VC>& link, const int* status, int variant, int nu = 4);
 template <class C, class V, class VC>
 void launch_res_restrict_t(hipStream_t st, const LevelViewT<C, V>& F, int cur, const V* rhs, const double* shift,
     const LevelViewT<C, VC>& Cl, AxisRestrict rx, AxisRestrict ry, const int* status, int variant);

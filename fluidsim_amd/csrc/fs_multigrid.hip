@@ -79,7 +79,7 @@ template <int MODE, int NT> constexpr int rb_waves_per_eu()
 // C: conductance storage, V: this level's vectors (rhs, solution), VC: the coarse solution (MODE 1).
 template <int TX, int TY, int NS, int FINE, int MODE, int NT, class C = float, class V = float, class VC = float>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(rb_waves_per_eu<MODE, NT>())))
-void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, int c0, CoarseLinkT<VC> link,
+void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, int c0, int nu, CoarseLinkT<VC> link,
     const int* status)
 {
     constexpr bool from_zero = MODE == 0;
@@ -212,13 +212,15 @@ void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, 
         }
         sreach[r] = reach;
     }
-    // ---- eight half-sweeps, each region one cell smaller than the last ----
+    // ---- 2 nu half-sweeps (nu <= NS), each region one cell smaller than the last; with nu < NS the
+    // outer 2 (NS - nu) rings of the halo are loaded but never updated ----
+    const int skip = 2 * (NS - nu);
     // A half-sweep reads only the other colour, so all neighbour values are gathered first and the
     // updates written afterwards: one LDS latency per half-sweep instead of one per round.  The colour
     // is a compile-time constant inside a half-sweep, so per-colour registers are addressed directly.
     auto half = [&](auto color_tag, int k) {
         constexpr int color = decltype(color_tag)::value;
-        const int lo = k + 1;
+        const int lo = k + 1 + skip;
         const bool neighbours = !(from_zero && k == 0);
         float* mine = s_col[color];
         const float* other = s_col[color ^ 1];
@@ -242,7 +244,7 @@ void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, 
         __syncthreads();
     };
 #pragma unroll 1
-    for (int k = 0; k < 2 * NS; k += 2) {
+    for (int k = 0; k < 2 * nu; k += 2) {
         if (c0 == 0) {
             half(std::integral_constant<int, 0>{}, k);
             half(std::integral_constant<int, 1>{}, k + 1);
@@ -257,6 +259,122 @@ void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, 
         const int x = x0 + lx, y = y0 + ly;
         if (x < w && y < h)
             sol_out[(y + 1) * s + (x + 1)] = stv<V>(s_col[(lx + ly) & 1][ly * HX + (lx >> 1)]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Chebyshev-weighted damped Jacobi with optional fused prolongation (the reference WebGPU multigrid's
+// polynomial smoother, PressureMultigridGPU.cpp:24-28, 619-631; PressureSmoothingKernels.h:91-176).
+// ---------------------------------------------------------------------------------------------
+// The tile and a HALO-cell ring live in two LDS buffers (sweep k reads buffer k & 1 and writes the other).
+// Each thread owns fixed cells; their four conductances, rhs and 1/diagonal stay in registers for all
+// sweeps.  Sweep k updates the cells at least k + 1 + (HALO - nu) cells from the tile edge, so after nu
+// sweeps the interior holds exactly what nu global sweeps compute.  Cells outside the grid hold zero in both
+// buffers and are never updated.  MODE 0: from zero (the first sweep is w b / diag); 1: post-smoothing,
+// the prolonged coarse correction added to the solution first.
+template <int TX, int TY, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_jacobi_multi(LevelView L, int cur, const float* rhs, const double* shift,
+    SmoothWeights wt, int nu, CoarseLink link, const int* status)
+{
+    constexpr int HALO = kJacobiMaxSweeps;
+    constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
+    constexpr int NE = LX * LY;
+    constexpr int ROUNDS = (NE + NT - 1) / NT;
+    __shared__ float s_x[2][NE];
+    const int w = L.w, h = L.h, s = L.s;
+    int bx, by;
+    xcd_tile((w + TX - 1) / TX, (h + TY - 1) / TY, bx, by);
+    const int x0 = bx * TX - HALO, y0 = by * TY - HALO;
+    const float* sol_in = L.sol[cur];
+    // every global load goes through a clamped (valid) interior index and is masked afterwards
+    float cl[ROUNDS], cr[ROUNDS], cu[ROUNDS], cd[ROUNDS], cb[ROUNDS], xv[ROUNDS];
+    int ix[ROUNDS], iy[ROUNDS];
+    bool cin[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int e = threadIdx.x + r * NT;
+        const int ly = e / LX, lx = e - ly * LX;
+        const int x = x0 + lx, y = y0 + ly;
+        cin[r] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
+        ix[r] = min(max(x, 0), w - 1);
+        iy[r] = min(max(y, 0), h - 1);
+        const int P = (iy[r] + 1) * s + (ix[r] + 1);
+        cl[r] = L.right[P - 1];
+        cr[r] = L.right[P];
+        cu[r] = L.down[P - s];
+        cd[r] = L.down[P];
+        cb[r] = rhs[P];
+        xv[r] = MODE == 0 ? 0.f : sol_in[P];
+    }
+    if (MODE == 1) {
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const float wx = link.pxw[ix[r]], wy = link.pyw[iy[r]];
+            const float* top = link.csol + (link.pyf[iy[r]] + 1) * link.cs + 1;
+            const float* bot = link.csol + (link.pys[iy[r]] + 1) * link.cs + 1;
+            const int a0 = link.pxf[ix[r]], a1 = link.pxs[ix[r]];
+            const float a = (1.f - wx) * top[a0] + wx * top[a1];
+            const float b = (1.f - wx) * bot[a0] + wx * bot[a1];
+            xv[r] += (1.f - wy) * a + wy * b;
+        }
+    }
+    if (status && *status != kRunning)
+        return;
+    const double sh = shift ? *shift : 0.0;
+    float inv[ROUNDS];
+    int reach[ROUNDS];   // distance to the tile edge + 1; 0 for a cell outside the grid
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int e = threadIdx.x + r * NT;
+        const int ly = e / LX, lx = e - ly * LX;
+        if (!cin[r]) {
+            cl[r] = cr[r] = cu[r] = cd[r] = cb[r] = xv[r] = 0.f;
+        } else if (shift) {
+            cb[r] = (float)((double)cb[r] - sh);
+        }
+        inv[r] = inv_or_zero((cl[r] + cr[r]) + (cu[r] + cd[r]));
+        reach[r] = cin[r] ? min(min(ly, LY - 1 - ly), min(lx, LX - 1 - lx)) + 1 : 0;
+        if (e < NE) {
+            s_x[0][e] = xv[r];
+            s_x[1][e] = xv[r];
+        }
+    }
+    __syncthreads();
+    const int skip = HALO - nu;
+#pragma unroll 1
+    for (int k = 0; k < nu; ++k) {
+        float wk = wt.w[0];   // uniform select: no dynamic indexing of the argument block
+#pragma unroll
+        for (int j = 1; j < kJacobiMaxSweeps; ++j)
+            wk = k == j ? wt.w[j] : wk;
+        const float* src = s_x[k & 1];
+        float* dst = s_x[(k & 1) ^ 1];
+        const bool neighbours = !(MODE == 0 && k == 0);
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            if (reach[r] > k + 1 + skip) {
+                const int e = threadIdx.x + r * NT;
+                float v;
+                if (neighbours) {
+                    const float c = src[e];
+                    const float product = (cl[r] * (c - src[e - 1]) + cr[r] * (c - src[e + 1])) +
+                                          (cu[r] * (c - src[e - LX]) + cd[r] * (c - src[e + LX]));
+                    v = c + wk * (cb[r] - product) * inv[r];
+                } else {
+                    v = wk * cb[r] * inv[r];
+                }
+                dst[e] = inv[r] > 0.f ? v : 0.f;
+            }
+        }
+        __syncthreads();
+    }
+    const float* fin = s_x[nu & 1];
+    float* sol_out = L.sol[cur ^ 1];
+    for (int e = threadIdx.x; e < TX * TY; e += NT) {
+        const int ly = HALO + e / TX, lx = HALO + e % TX;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x < w && y < h)
+            sol_out[(y + 1) * s + (x + 1)] = fin[ly * LX + lx];
     }
 }
 
@@ -1526,12 +1644,13 @@ bool tail_allow_lds(size_t bytes)
 // rounds each with at most 4% of the lanes idle, against 28% for 32 x 32 on 512 threads; 9 runs the
 // 64 x 32 and 32 x 64 slots in two rounds of 960 threads.
 void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
-    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant)
+    int first_color, int from_zero, const CoarseLink& link, const int* status, int fine, int variant, int nu)
 {
     const int mode = from_zero ? 0 : (link.csol ? 1 : 2);
+    nu = std::min(std::max(nu, 1), kMultiNS);
 #define FS_RB(TX, TY, F, M, NT)                                                                               \
     hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, F, M, NT>), dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)), \
-        dim3(NT), 0, st, L, cur, rhs, shift, first_color, link, status)
+        dim3(NT), 0, st, L, cur, rhs, shift, first_color, nu, link, status)
 #define FS_RB_MODES(TX, TY, F, NT)      \
     do {                                \
         if (mode == 0)                  \
@@ -1566,6 +1685,34 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
     }
 #undef FS_RB_MODES
 #undef FS_RB
+}
+
+void launch_jacobi_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
+    int from_zero, const SmoothWeights& weights, int nu, const CoarseLink& link, const int* status, int variant)
+{
+    nu = std::min(std::max(nu, 1), kJacobiMaxSweeps);
+#define FS_JM(TX, TY, M, NT)                                                                                  \
+    hipLaunchKernelGGL((k_jacobi_multi<TX, TY, M, NT>), dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)),   \
+        dim3(NT), 0, st, L, cur, rhs, shift, weights, nu, link, status)
+#define FS_JM_MODES(TX, TY, NT)      \
+    do {                             \
+        if (from_zero)               \
+            FS_JM(TX, TY, 0, NT);    \
+        else                         \
+            FS_JM(TX, TY, 1, NT);    \
+    } while (0)
+    switch (variant) {
+    case 1: FS_JM_MODES(32, 32, 256); break;
+    case 2: FS_JM_MODES(32, 16, 256); break;
+    case 3: FS_JM_MODES(16, 16, 256); break;
+    case 4: FS_JM_MODES(32, 32, 512); break;
+    case 5: FS_JM_MODES(64, 64, 1024); break;
+    case 6: FS_JM_MODES(64, 32, 1024); break;
+    case 7: FS_JM_MODES(32, 64, 1024); break;
+    default: FS_JM_MODES(64, 32, 512); break;
+    }
+#undef FS_JM_MODES
+#undef FS_JM
 }
 
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,
@@ -1638,12 +1785,13 @@ template void launch_level_pack<bf16>(hipStream_t, int, int, const double*, cons
 
 template <class C, class V, class VC>
 void launch_rb_multi_t(hipStream_t st, const LevelViewT<C, V>& L, int cur, const V* rhs, const double* shift,
-    int first_color, int from_zero, const CoarseLinkT<VC>& link, const int* status, int variant)
+    int first_color, int from_zero, const CoarseLinkT<VC>& link, const int* status, int variant, int nu)
 {
+    nu = std::min(std::max(nu, 1), kMultiNS);
 #define FS_RBT(TX, TY, M, NT)                                                                                 \
     hipLaunchKernelGGL((k_rb_multi<TX, TY, kMultiNS, 0, M, NT, C, V, VC>),                                   \
-        dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)), dim3(NT), 0, st, L, cur, rhs, shift, first_color, link, \
-        status)
+        dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)), dim3(NT), 0, st, L, cur, rhs, shift, first_color, nu, \
+        link, status)
 #define FS_RBT_MODES(TX, TY, NT)      \
     do {                              \
         if (from_zero)                \
@@ -1686,7 +1834,7 @@ void launch_res_restrict_t(hipStream_t st, const LevelViewT<C, V>& F, int cur, c
 
 #define FS_MG16(C, V, VC)                                                                                      \
     template void launch_rb_multi_t<C, V, VC>(hipStream_t, const LevelViewT<C, V>&, int, const V*, const double*, \
-        int, int, const CoarseLinkT<VC>&, const int*, int);                                                    \
+        int, int, const CoarseLinkT<VC>&, const int*, int, int);                                               \
     template void launch_res_restrict_t<C, V, VC>(hipStream_t, const LevelViewT<C, V>&, int, const V*,          \
         const double*, const LevelViewT<C, VC>&, AxisRestrict, AxisRestrict, const int*, int);
 FS_MG16(half16, float, float)

@@ -107,6 +107,19 @@ typedef enum fs_mg_precision {
     FS_MG_BF16_VECTORS = 3
 } fs_mg_precision;
 
+/* Smoother of the pressure multigrid's levels above 4096 cells (no reference counterpart as a setting: the
+ * reference's CPU multigrid runs red-black Gauss-Seidel, PressureMultigrid.cpp:204-231, and its WebGPU
+ * multigrid Chebyshev-weighted damped Jacobi on levels above 65536 cells, PressureMultigridGPU.cpp:24-44,
+ * 619-631).  Both are symmetric, so the V-cycle stays an SPD preconditioner; the choice changes the
+ * preconditioner (iteration counts, results within the tolerance), not the equation.  Levels of at most
+ * 4096 cells always run red-black V(4,4).
+ *   FS_SMOOTHER_RED_BLACK  red-black Gauss-Seidel, 1..4 sweeps per side (default: 4, the reference's cycle);
+ *   FS_SMOOTHER_CHEBYSHEV  damped Jacobi with the weights 1 / (1.1 + 0.9 cos(pi (k + 1/2) / n)), k = 0..n-1,
+ *                          in order before the coarse correction and reversed after it, n = 1..5 sweeps per
+ *                          side; FP32 multigrid levels only.  A cell without conductance (solid masks) is
+ *                          set to zero, as the red-black sweep sets it. */
+typedef enum fs_pressure_smoother { FS_SMOOTHER_RED_BLACK = 0, FS_SMOOTHER_CHEBYSHEV = 1 } fs_pressure_smoother;
+
 /* ---- library / context ---- */
 const char* fs_version(void);
 /* Message of the last failed call on this thread ("" if none).  No reference counterpart: the C
@@ -184,6 +197,17 @@ fs_status fs_solid_cells(const fs_grid* grid, long long* count);
  * max|b - A p| <= tol * max|b| (b the mean-free pressure rhs) and the projection is accepted when
  * max|div u| <= tol * max|b| * dt / h^2, i.e. tol times the divergence the projection started from. */
 fs_status fs_set_relative_tolerance(fs_grid* grid, int enabled);
+/* Pressure smoother choice (see fs_pressure_smoother).  FS_E_UNSUPPORTED for Chebyshev on a grid with
+ * 16-bit multigrid levels.  Batched solves (fs_batch_project) need the default. */
+fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps);
+fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps);
+/* Autotuner of the smoother and its sweep count (no reference counterpart): projects the grid's current
+ * state with each candidate (red-black 2, 3, 4 and Chebyshev 3, 4, 5 sweeps, each from the same state,
+ * restored after every trial: the grid's state is unchanged on return) with fs_project's arguments, times
+ * each projection on the device after one warm-up, and selects the fastest candidate whose projection
+ * converges.  kind, sweeps and ms (each may be NULL) receive the choice and its time per projection. */
+fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
+    fs_pressure_smoother* kind, int* sweeps, double* ms);
 fs_status fs_get_relative_tolerance(const fs_grid* grid, int* enabled);
 
 /* upload(const State&) (FluidGridBase.h:118, FluidGridCPU.cpp:198-251).  NULL means an empty
@@ -297,7 +321,9 @@ fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, i
  * four-sweep kernel (0 = 64x32/256 threads, 1 = 32x32/256, 2 = 32x16/256, 3 = 16x16/256, 4 = 64x32/512,
  * 5 = 32x32/512, 6 = 32x32/384, 7 = 64x32/640, 8 = 32x64/640, 9 = 64x32/960, 10 = 32x64/960) and the
  * residual-restriction variant (0 = 32x16, 1 = 16x16, 2 = 16x8, 3 = 8x8 coarse
- * cells); -1 for levels solved inside the single-workgroup tail. */
+ * cells); -1 for levels solved inside the single-workgroup tail.  On a level that runs the Chebyshev smoother,
+ * pre and post are tiles of the Jacobi kernel instead (0 = 64x32/512, 1 = 32x32/256, 2 = 32x16/256,
+ * 3 = 16x16/256, 4 = 32x32/512, 5 = 64x64/1024, 6 = 64x32/1024, 7 = 32x64/1024). */
 fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction);
 /* Overrides the autotuned variants of a tiled level (-1 keeps one): every variant computes the same
  * values, so this changes time only -- for profiling one fixed configuration and for testing that claim. */
@@ -313,7 +339,7 @@ fs_status fs_face_weights(fs_grid* grid, double* inv_density_u, double* inv_dens
 /* ---- measurement (no reference counterpart) ---- */
 /* Times `reps` back-to-back launches of one hot kernel on the grid's stream with HIP events and
  * returns the mean milliseconds per launch.  kernel: 0 = finest-level post-smoothing (prolongation +
- * four sweeps, as in the V-cycle),
+ * the grid's smoother sweeps, as in the V-cycle),
  * 1 = operator + dot (A d), 2 = PCG update, 3 = FP32 copy of one padded vector (reference kernel),
  * 4 = second level's smoother, 5 = finest residual + restriction, 6 = coarse tail, 7 = coarse
  * inverse (sweep operator), 8 = direction update, 9 = r.z reduction, 10 = FP32 copy of a 64 MiB buffer

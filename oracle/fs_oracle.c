@@ -512,7 +512,62 @@ static void solve_coarsest(orc_mg* mg)
         L->sol[(cell / L->w + 1) * L->stride + cell % L->w + 1] = c[cell - 1] - mean;
 }
 
-/* cycle, PressureMultigrid.cpp:386-408: V(4,4) */
+/* Smoother choice of the levels above ORC_SMOOTH_MIN_CELLS cells (an execution choice: the reference's CPU
+ * multigrid runs red-black Gauss-Seidel, PressureMultigrid.cpp:204-231, 396; its WebGPU multigrid runs
+ * Chebyshev-weighted damped Jacobi on levels above 65536 cells, PressureMultigridGPU.cpp:24-28, 619-631).
+ * Smaller levels always keep V(4,4) red-black Gauss-Seidel.  Process-wide, like the worker count. */
+#define ORC_SMOOTH_MIN_CELLS 4096
+static int g_smooth_kind = ORC_SMOOTH_RBGS;
+static int g_smooth_sweeps = 4;
+
+int orc_set_smoother(int kind, int sweeps)
+{
+    if ((kind != ORC_SMOOTH_RBGS && kind != ORC_SMOOTH_CHEBYSHEV) || sweeps < 1 || sweeps > 8)
+        return -1;
+    g_smooth_kind = kind;
+    g_smooth_sweeps = sweeps;
+    return 0;
+}
+
+/* The polynomial weights of PressureMultigridGPU's constructor (PressureMultigridGPU.cpp:24-28):
+ * 1 / (1.1 + 0.9 cos(pi (i + 1/2) / sweeps)), the reciprocals of the Chebyshev nodes of [0.2, 2] (the
+ * spectrum of D^-1 A), rounded to float as the reference stores them. */
+double orc_chebyshev_weight(int i, int sweeps)
+{
+    const double pi = 3.14159265358979323846;
+    return (double)(float)(1.0 / (1.1 + 0.9 * cos(pi * (i + 0.5) / sweeps)));
+}
+
+/* One damped-Jacobi sweep x <- x + w (b - A x) / diag (polynomial_smooth, PressureSmoothingKernels.h:91-176):
+ * from zero it is w b / diag.  A cell without conductance (only solid masks make one; the reference has
+ * none) is set to zero, as the Gauss-Seidel sweep's value * inv_diag sets it: keeping a prolonged value
+ * there would leak into the mean removal and make the cycle unsymmetric.  The new values go to the
+ * level's residual array, which then swaps with the solution. */
+static void jacobi(orc_level* L, double weight, int from_zero)
+{
+    const int s = L->stride;
+    const int workers = row_workers(L->w, L->h);
+    const double* x = L->sol;
+    double* out = L->res;
+#pragma omp parallel for num_threads(workers) schedule(static, 1) if (workers > 1)
+    for (int b = 0; b < workers; ++b)
+    for (int y = batch_first(L->h, b, workers); y < batch_first(L->h, b + 1, workers); ++y) {
+        for (int x0 = 0, i = (y + 1) * s + 1; x0 < L->w; ++x0, ++i) {
+            if (from_zero) {
+                out[i] = weight * L->rhs[i] * L->inv_diag[i];
+                continue;
+            }
+            const double c = x[i];
+            const double product = (L->right[i - 1] * (c - x[i - 1]) + L->right[i] * (c - x[i + 1])) +
+                                   (L->down[i - s] * (c - x[i - s]) + L->down[i] * (c - x[i + s]));
+            out[i] = L->inv_diag[i] > 0.0 ? c + weight * (L->rhs[i] - product) * L->inv_diag[i] : 0.0;
+        }
+    }
+    L->res = L->sol;
+    L->sol = out;
+}
+
+/* cycle, PressureMultigrid.cpp:386-408: V(4,4) (or the chosen smoother on the large levels) */
 static void cycle(orc_mg* mg, int index)
 {
     if (index + 1 == mg->nlev) {
@@ -521,15 +576,27 @@ static void cycle(orc_mg* mg, int index)
     }
     orc_level* F = &mg->lev[index];
     orc_level* C = &mg->lev[index + 1];
-    const int sweeps = 4;
-    smooth(F, 0, 1);
-    for (int k = 1; k < sweeps; ++k)
-        smooth(F, 0, 0);
+    const int large = (long)F->w * F->h > ORC_SMOOTH_MIN_CELLS;
+    const int kind = large ? g_smooth_kind : ORC_SMOOTH_RBGS;
+    const int sweeps = large ? g_smooth_sweeps : 4;
+    if (kind == ORC_SMOOTH_CHEBYSHEV) {
+        for (int k = 0; k < sweeps; ++k)
+            jacobi(F, orc_chebyshev_weight(k, sweeps), k == 0);
+    } else {
+        smooth(F, 0, 1);
+        for (int k = 1; k < sweeps; ++k)
+            smooth(F, 0, 0);
+    }
     restrict_residual(F, C);
     cycle(mg, index + 1);
     prolong_correction(C, F);
-    for (int k = 0; k < sweeps; ++k)
-        smooth(F, 1, 0);
+    if (kind == ORC_SMOOTH_CHEBYSHEV) {
+        for (int k = sweeps - 1; k >= 0; --k)
+            jacobi(F, orc_chebyshev_weight(k, sweeps), 0);
+    } else {
+        for (int k = 0; k < sweeps; ++k)
+            smooth(F, 1, 0);
+    }
 }
 
 /* apply, PressureMultigrid.cpp:410-434 */

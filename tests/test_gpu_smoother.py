@@ -1,0 +1,180 @@
+"""The pressure smoother choice on the GPU (fs_set_pressure_smoother, fs_tune_pressure_smoother) against the
+FP64 oracle with the same choice.
+
+  - the V-cycle (fs_mg_apply) with Chebyshev-weighted Jacobi (3 and 5 sweeps) and with 2 or 3 red-black
+    sweeps matches the oracle's to FP32 rounding, on a pool and on a masked 1000:1 tank, and stays a valid
+    preconditioner (the FP32 bars of tests/FluidGridWebGPUTests.cpp:428-535);
+  - every Jacobi tile variant computes the same values, bit for bit;
+  - projections with each choice converge and match the oracle's projection with the same choice within
+    1e-4 of the velocity scale (iteration counts within a few on the pool, up to 2x on the masked tank,
+    where FP32 CG loses conjugacy);
+  - the tuner leaves the grid's state unchanged and selects a candidate that converges;
+  - batched solves keep the reference's cycle: fs_batch_project refuses another choice and fs_batch_step
+    projects such a grid on its own, bit for bit as fs_step does; 16-bit levels refuse Chebyshev.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fluidsim_amd import (FS_MG_BF16, FS_MG_FP32, FS_SMOOTHER_CHEBYSHEV, FS_SMOOTHER_RED_BLACK, FluidGridHIP,
+                          batch_project, batch_step)
+from fluidsim_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+CHOICES = [(FS_SMOOTHER_CHEBYSHEV, 3), (FS_SMOOTHER_CHEBYSHEV, 5), (FS_SMOOTHER_RED_BLACK, 2),
+           (FS_SMOOTHER_RED_BLACK, 3)]
+
+
+@pytest.fixture
+def O(oracle):
+    yield oracle
+    oracle.set_smoother(oracle.SMOOTH_RBGS, 4)
+
+
+def workload(kind, W, H):
+    return WL.obstacle_tank(W, H, seed=3) if kind == "tank" else WL.sloped_pool(W, H, 4.0)
+
+
+def hip_grid(wl):
+    g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+    WL.load_into(g, wl)
+    if wl.meta.get("solid") is not None:
+        g.setSolidMask(wl.meta["solid"])
+    return g
+
+
+def oracle_grid(O, wl):
+    o = O.OracleGrid(wl.width, wl.height, wl.cell_size)
+    o.configure(wl.liquid_density, wl.gas_density)
+    o.set_gravity(wl.gravity_y)
+    if wl.meta.get("solid") is not None:
+        o.set_solid(wl.meta["solid"])
+    o.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+    return o
+
+
+@pytest.mark.parametrize("kind,sweeps", CHOICES)
+@pytest.mark.parametrize("case", ["pool", "tank"])
+def test_vcycle_matches_the_oracle(O, case, kind, sweeps):
+    W, H = (300, 200) if case == "pool" else (256, 192)
+    wl = workload(case, W, H)
+    g = hip_grid(wl)
+    g.setPressureSmoother(kind, sweeps)
+    assert g.getPressureSmoother() == (kind, sweeps)
+    O.set_smoother(kind, sweeps)
+    o = oracle_grid(O, wl)
+    tau = 1e-6 * o.max_divergence()
+    o.project(wl.dt, 0, tau)   # the wall step closes solid faces in both
+    g.project(wl.dt, 0, tau)
+    mg = o.multigrid()
+    i = np.arange(W * H)
+    a = np.sin(i * 0.31) + np.cos(i * 0.073)
+    b = np.cos(i * 0.17) - np.sin(i * 0.019)
+    if case == "tank":
+        fluid = wl.meta["solid"] == 0
+        a, b = a * fluid, b * fluid
+    a -= a.mean()
+    b -= b.mean()
+    ho, hg = mg.apply(a, 0.0), g.multigridApply(a)
+    err = np.linalg.norm(hg - ho) / np.linalg.norm(ho)
+    print(f"{case} {kind}/{sweeps}: V-cycle rel L2 {err:.2e}")
+    assert err < 1e-5
+    ma, mb = hg - hg.mean(), g.multigridApply(b)
+    mb -= mb.mean()
+    mc = g.multigridApply(1.3 * a - 0.7 * b)
+    mc -= mc.mean()
+    assert np.abs(mc - (1.3 * ma - 0.7 * mb)).max() <= 5e-4 * np.abs(mc).max()
+    ea, eb = a @ ma, b @ mb
+    assert ea > 0 and eb > 0 and abs(a @ mb - b @ ma) <= 5e-4 * math.sqrt(ea * eb)
+    g.destroy()
+
+
+def test_every_jacobi_tile_computes_the_same_values():
+    wl = WL.sloped_pool(300, 200, 4.0)
+    g = hip_grid(wl)
+    g.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 5)
+    i = np.arange(300 * 200)
+    a = np.sin(i * 0.31) + np.cos(i * 0.073)
+    a -= a.mean()
+    levels = [l for l in range(g.multigridLevels()) if g.multigridTiles(l)[0] >= 0
+              and g.multigridShape(l)[0] * g.multigridShape(l)[1] > 4096]
+    assert levels, "no level runs the Chebyshev smoother"
+    g.setMultigridTiles(levels[0], 0, 0, 0)
+    ref = g.multigridApply(a)
+    for l in levels:
+        for v in range(8):
+            g.setMultigridTiles(l, v, 7 - v, v % 4)
+            assert g.multigridTiles(l) == (v, 7 - v, v % 4)
+            assert np.array_equal(g.multigridApply(a), ref), f"level {l} variant {v}"
+        g.setMultigridTiles(l, 0, 0, 0)
+    with pytest.raises(ValueError):
+        g.setMultigridTiles(levels[0], 8, -1, -1)
+    g.destroy()
+
+
+@pytest.mark.parametrize("kind,sweeps", CHOICES + [(FS_SMOOTHER_CHEBYSHEV, 4)])
+@pytest.mark.parametrize("case", ["pool", "tank"])
+def test_projection_matches_the_oracle(O, case, kind, sweeps):
+    W, H = (320, 240) if case == "pool" else (256, 256)
+    wl = workload(case, W, H)
+    g = hip_grid(wl)
+    g.setPressureSmoother(kind, sweeps)
+    O.set_smoother(kind, sweeps)
+    o = oracle_grid(O, wl)
+    tau = 1e-6 * o.max_divergence()
+    ro = o.project(wl.dt, 4000, tau)
+    rg = g.project(wl.dt, 4000, tau)
+    print(f"{case} {kind}/{sweeps}: iterations GPU {rg.iterations} oracle {ro['iterations']}")
+    assert ro["converged"] and rg.converged, (ro, rg)
+    # FP32 CG loses conjugacy on the masked 1000:1 tank (1.5-1.8x the FP64 iterations, the price of an FP32
+    # iterate and preconditioner); the pool stays within a few iterations
+    assert rg.iterations <= (2.0 if case == "tank" else 1.2) * ro["iterations"] + 3
+    so, sg = o.download(), g.download()
+    scale = max(np.abs(so["u"]).max(), np.abs(so["v"]).max())
+    assert np.abs(sg.u - so["u"]).max() < 1e-4 * scale and np.abs(sg.v - so["v"]).max() < 1e-4 * scale
+    g.destroy()
+
+
+def test_tuner_keeps_the_state_and_picks_a_converging_choice():
+    wl = WL.vortex_box(512, 512, seed=2)
+    g = hip_grid(wl)
+    before = g.download()
+    tau = 1e-6 * g.maxDivergence()
+    kind, sweeps, ms = g.tunePressureSmoother(wl.dt, 200, tau)
+    print(f"tuned: kind {kind}, {sweeps} sweeps, {ms:.3f} ms per projection")
+    assert (kind, sweeps) == g.getPressureSmoother() and ms > 0
+    after = g.download()
+    for key in ("liquid_fraction", "u", "v", "pressure"):
+        assert np.array_equal(getattr(after, key), getattr(before, key)), key
+    r = g.project(wl.dt, 200, tau)
+    assert r.converged
+    g.destroy()
+
+
+def test_batched_solves_keep_the_reference_cycle():
+    wl = WL.sloped_pool(160, 120, 4.0)
+    a, b = hip_grid(wl), hip_grid(wl)
+    a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 4)
+    b.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 4)
+    with pytest.raises(NotImplementedError):
+        batch_project([a], 0.002, 500, 1e-6)
+    (ra,), st = batch_step([a], 0.002, 500, 1e-6)
+    rb = b.step(0.002, 500, 1e-6)
+    assert st == [0] and ra.converged and (ra.iterations, ra.max_divergence) == (rb.iterations, rb.max_divergence)
+    assert np.array_equal(a.download().u, b.download().u)
+    # 16-bit levels and the Chebyshev smoother exclude each other
+    with pytest.raises(NotImplementedError):
+        a.setMultigridPrecision(FS_MG_BF16)
+    a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 4)
+    a.setMultigridPrecision(FS_MG_BF16)
+    with pytest.raises(NotImplementedError):
+        a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 4)
+    a.setMultigridPrecision(FS_MG_FP32)
+    with pytest.raises(ValueError):
+        a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 5)
+    with pytest.raises(ValueError):
+        a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 6)
+    a.destroy()
+    b.destroy()

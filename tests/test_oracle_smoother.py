@@ -1,0 +1,115 @@
+"""The oracle's smoother choice (the reference WebGPU multigrid's Chebyshev-weighted damped Jacobi and the
+red-black sweep count) pinned without the reference -- CPU only.
+
+  - the weights 1 / (1.1 + 0.9 cos(pi (k + 1/2) / n)) make the n-sweep error polynomial prod_k (1 - w_k l)
+    the Chebyshev polynomial T_n((1.1 - l) / 0.9) / T_n(11 / 9) of [0.2, 2] (PressureMultigridGPU.cpp:24-28):
+    the closed form holds at every l to float rounding of the weights;
+  - every choice keeps the V-cycle an SPD preconditioner on mean-free vectors (tests/FluidGridTests.cpp:538-625
+    bars: linear, symmetric to rounding, positive, constants to zero), with and without solid masks;
+  - MG-PCG with every choice reaches the same pressure and velocity as the reference's V(4,4) cycle (the
+    reference's MG-vs-diagonal bar, 2e-6 at tol 1e-8, tests/FluidGridTests.cpp:627-660), and the Chebyshev
+    cycle converges at least as fast as red-black at the same number of sweeps on the vortex box.
+A sign or term dropped from the sweep breaks symmetry or convergence; a smoother that kept a prolonged
+value in a cell without conductance breaks symmetry on the masked tank (measured: 0.29 relative).
+"""
+import numpy as np
+import pytest
+
+from fluidsim_amd import workloads as WL
+
+CHOICES = [(0, 4), (0, 3), (0, 2), (0, 1), (1, 1), (1, 3), (1, 4), (1, 5)]
+
+
+@pytest.fixture
+def O(oracle):
+    yield oracle
+    oracle.set_smoother(oracle.SMOOTH_RBGS, 4)
+
+
+def chebyshev_t(n, x):
+    return np.cosh(n * np.arccosh(x)) if x >= 1 else np.cos(n * np.arccos(x))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+def test_weights_are_the_chebyshev_polynomial_of_the_interval(O, n):
+    w = np.array([O.chebyshev_weight(k, n) for k in range(n)])
+    assert np.all((1.0 / w >= 0.2 - 1e-6) & (1.0 / w <= 2.0 + 1e-6)), "nodes outside [0.2, 2]"
+    bound = 1.0 / chebyshev_t(n, 1.1 / 0.9)
+    for lam in np.linspace(0.0, 2.0, 81):
+        p = np.prod(1.0 - w * lam)
+        q = chebyshev_t(n, (1.1 - lam) / 0.9) * bound
+        assert abs(p - q) <= 1e-5 * max(1.0, abs(q)), (n, lam, p, q)
+        if lam >= 0.2:
+            assert abs(p) <= bound * (1 + 1e-5)
+
+
+def test_invalid_choices_are_refused(O):
+    for kind, sweeps in [(2, 4), (0, 0), (1, 9), (-1, 3)]:
+        with pytest.raises(ValueError):
+            O.set_smoother(kind, sweeps)
+
+
+def tank(O, W, H, masked):
+    wl = WL.obstacle_tank(W, H, seed=3)
+    g = O.OracleGrid(W, H, wl.cell_size)
+    g.configure(wl.liquid_density, wl.gas_density)
+    g.set_gravity(wl.gravity_y)
+    if masked:
+        g.set_solid(wl.meta["solid"])
+    g.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+    g.project(wl.dt, 0, 1e-6 * g.max_divergence())   # the wall step closes the solid faces
+    return g, wl
+
+
+@pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("kind,sweeps", CHOICES)
+def test_every_choice_is_an_spd_preconditioner(O, masked, kind, sweeps):
+    O.set_smoother(kind, sweeps)
+    W, H = 160, 120     # levels 160 x 120 and 80 x 60 run the choice, the rest red-black V(4,4)
+    g, _ = tank(O, W, H, masked)
+    mg = g.multigrid()
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(W * H)
+    b = rng.standard_normal(W * H)
+    a -= a.mean()
+    b -= b.mean()
+    ma, mb = mg.apply(a, 0.0), mg.apply(b, 0.0)
+    mc = mg.apply(1.3 * a - 0.7 * b, 0.0)
+    assert np.abs(mc - (1.3 * ma - 0.7 * mb)).max() <= 1e-10 * np.abs(mc).max()
+    ea, eb = a @ ma, b @ mb
+    assert ea > 0 and eb > 0
+    assert abs(a @ mb - b @ ma) <= 1e-11 * np.sqrt(ea * eb)
+    const = mg.apply(np.full(W * H, 3.0), 3.0)
+    assert np.abs(const).max() == 0.0
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_every_choice_reaches_the_reference_cycles_solution(O, masked):
+    W, H = 144, 112
+    out = {}
+    for kind, sweeps in [(0, 4)] + [c for c in CHOICES if c != (0, 4)]:
+        O.set_smoother(kind, sweeps)
+        g, wl = tank(O, W, H, masked)
+        r = g.project(wl.dt, 4000, 1e-8)
+        assert r["converged"], (kind, sweeps, r)
+        s = g.download()
+        out[(kind, sweeps)] = (r["iterations"], s["u"], s["v"])
+    _, u0, v0 = out[(0, 4)]
+    for key, (it, u, v) in out.items():
+        assert np.abs(u - u0).max() < 2e-6 and np.abs(v - v0).max() < 2e-6, key
+
+
+def test_chebyshev_converges_at_least_as_fast_as_red_black_on_the_vortex_box(O):
+    wl = WL.vortex_box(256, 256, seed=2)
+    its = {}
+    for kind, sweeps in [(0, 4), (1, 4), (1, 5), (0, 2), (1, 2)]:
+        O.set_smoother(kind, sweeps)
+        g = O.OracleGrid(wl.width, wl.height, wl.cell_size)
+        g.configure(wl.liquid_density, wl.gas_density)
+        g.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+        r = g.project(wl.dt, 200, 1e-8 * g.max_divergence())
+        assert r["converged"]
+        its[(kind, sweeps)] = r["iterations"]
+    print(its)
+    assert its[(1, 4)] <= its[(0, 4)] and its[(1, 2)] <= its[(0, 2)] + 1
+    assert its[(1, 5)] <= its[(1, 4)]
