@@ -737,6 +737,7 @@ def main():
                         "max-norm residual (64 vortices + 16 sources, seed 2, fully liquid, closed box)",
             "grid": [wl.width, wl.height],
             "preconditioner": f"multigrid V-cycle, FP32 levels: {smoother}",
+            "smoother": f"{'chebyshev' if chebyshev else 'red_black'}:{nu}",
             "smoother_choice": ("fs_tune_pressure_smoother on this input (fastest converging of red-black 2-4 and "
                                 f"Chebyshev 3-5 sweeps: {tuned_ms:.3f} ms per projection)") if tuned_ms else "fixed",
             "parallelism": "single GPU",
