@@ -427,8 +427,8 @@ def precision_leg(size=2048, seed=3, steps=3, kernel_reps=50, stall_budget=400, 
         budget = stall_budget if mode == FS_MG_BF16_VECTORS else 2000
         tuned = None
         if name == "fp32_tuned":
-            kind, nu, _ = g.tunePressureSmoother(wl.dt, budget, tau)
-            tuned = {"smoother": "chebyshev" if kind == 1 else "red_black", "sweeps": nu}
+            kind, nu, coarse, _ = g.tunePressureSmoother(wl.dt, budget, tau)
+            tuned = {"smoother": "chebyshev" if kind == 1 else "red_black", "sweeps": nu, "coarse_sweeps": coarse}
             note(f"tuned smoother: {tuned}")
         g.snapshotSave()
         g.snapshotRestore()
@@ -600,7 +600,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--smoother", type=str, default="tuned",
-                    help="tuned (fs_tune_pressure_smoother), red_black:N (N = 1..4) or chebyshev:N (N = 1..5)")
+                    help="tuned (fs_tune_pressure_smoother), red_black:N[:C] (N = 1..4) or chebyshev:N[:C] "
+                         "(N = 1..5); C = red-black sweeps on the levels of at most 4096 cells (1..4, default 4)")
     ap.add_argument("--fine-tiles", type=str, default="", help="force the finest level's tile variants "
                     "PRE,POST,RESTRICT (profiling runs: the autotuner's choice can shift under a profiler)")
     ap.add_argument("--serving-requests", type=int, default=24, help="size of the C5 serving stream (0: skip)")
@@ -661,11 +662,13 @@ def main():
     # candidate projects the same state, which the tuner restores), or fixed on the command line.
     tuned_ms = None
     if args.smoother == "tuned":
-        kind, nu, tuned_ms = grid.tunePressureSmoother(wl.dt, 200, tau)
+        kind, nu, coarse, tuned_ms = grid.tunePressureSmoother(wl.dt, 200, tau)
     else:
-        name, _, count = args.smoother.partition(":")
-        kind, nu = (FS_SMOOTHER_CHEBYSHEV if name == "chebyshev" else FS_SMOOTHER_RED_BLACK), int(count or 4)
-        grid.setPressureSmoother(kind, nu)
+        name, *counts = args.smoother.split(":")
+        kind = FS_SMOOTHER_CHEBYSHEV if name == "chebyshev" else FS_SMOOTHER_RED_BLACK
+        nu = int(counts[0]) if counts else 4
+        coarse = int(counts[1]) if len(counts) > 1 else 4
+        grid.setPressureSmoother(kind, nu, coarse)
     if args.fine_tiles:
         grid.setMultigridTiles(0, *[int(v) for v in args.fine_tiles.split(",")])
     grid.snapshotSave()
@@ -700,12 +703,12 @@ def main():
         tx, ty, nt = JACOBI_VARIANTS[post_variant]
         fine_kernel = f"k_jacobi_multi<{tx},{ty},1,{nt}>"
         smoother = (f"Chebyshev-weighted damped Jacobi, {nu} sweeps per side, on the levels above 4096 cells "
-                    "(red-black Gauss-Seidel V(4,4) below)")
+                    f"(red-black Gauss-Seidel, {coarse} sweeps per side, below)")
         fine_what = f"finest-level prolongation + {nu} Chebyshev-weighted Jacobi sweeps, autotuned tile"
     else:
         tx, ty, nt = RB_VARIANTS[post_variant]
         fine_kernel = f"k_rb_multi<{tx},{ty},4,{1 if post_variant in (0, 4) else 0},1,{nt},float,float,float>"
-        smoother = f"red-black Gauss-Seidel, {nu} sweeps per side on the levels above 4096 cells (4 below)"
+        smoother = f"red-black Gauss-Seidel, {nu} sweeps per side on the levels above 4096 cells ({coarse} below)"
         fine_what = f"finest-level prolongation + {nu} red-black Gauss-Seidel sweeps, autotuned tile"
     sweep_ms = grid.timeKernel(0, args.kernel_reps)
     tail_ms = grid.timeKernel(TAIL_KERNEL, args.kernel_reps)
@@ -737,9 +740,10 @@ def main():
                         "max-norm residual (64 vortices + 16 sources, seed 2, fully liquid, closed box)",
             "grid": [wl.width, wl.height],
             "preconditioner": f"multigrid V-cycle, FP32 levels: {smoother}",
-            "smoother": f"{'chebyshev' if chebyshev else 'red_black'}:{nu}",
+            "smoother": f"{'chebyshev' if chebyshev else 'red_black'}:{nu}:{coarse}",
             "smoother_choice": ("fs_tune_pressure_smoother on this input (fastest converging of red-black 2-4 and "
-                                f"Chebyshev 3-5 sweeps: {tuned_ms:.3f} ms per projection)") if tuned_ms else "fixed",
+                                "Chebyshev 3-5 sweeps, each with 4 or 2 sweeps on the levels of at most 4096 cells: "
+                                f"{tuned_ms:.3f} ms per projection)") if tuned_ms else "fixed",
             "parallelism": "single GPU",
         },
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},

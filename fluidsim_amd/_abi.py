@@ -168,10 +168,10 @@ SIGNATURES = {
     "fs_get_multigrid_precision": [_vp, C.POINTER(C.c_int)],
     "fs_set_flexible_cg": [_vp, C.c_int],
     "fs_set_relative_tolerance": [_vp, C.c_int],
-    "fs_set_pressure_smoother": [_vp, C.c_int, C.c_int],
-    "fs_get_pressure_smoother": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "fs_set_pressure_smoother": [_vp, C.c_int, C.c_int, C.c_int],
+    "fs_get_pressure_smoother": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "fs_tune_pressure_smoother": [_vp, C.c_double, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                  C.POINTER(C.c_double)],
+                                  C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "fs_get_relative_tolerance": [_vp, _ip],
     "fs_batch_apply_viscosity": [C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int)],
     "fs_solid_cells": [_vp, C.POINTER(C.c_longlong)],

@@ -474,6 +474,7 @@ struct fs_grid {
     bool relative_tol = false;  // fs_set_relative_tolerance
     fs_pressure_smoother smoother = FS_SMOOTHER_RED_BLACK;   // fs_set_pressure_smoother: levels above
     int smooth_sweeps = 4;                                    // kSmootherMinCells cells
+    int coarse_sweeps = 4;                                    // red-black sweeps of the smaller levels
     bool jacobi_tuned = false;
     bool uploaded = false;
     bool coeff_valid = false;
@@ -686,6 +687,7 @@ struct fs_grid {
     {
         std::memset(&tail, 0, sizeof(tail));
         const int nlev = (int)levels.size();
+        tail.sweeps = coarse_sweeps;
         tail.levels = nlev - start;
         tail.coarse_n = coarse_n;
         tail.ginv = ginv.p;
@@ -884,7 +886,23 @@ struct fs_grid {
     // The batched solve runs the reference's red-black V(4,4) on every level.
     bool default_cycle() const
     {
-        return (smoother == FS_SMOOTHER_RED_BLACK && smooth_sweeps == 4) || (long long)W * H <= kSmootherMinCells;
+        return coarse_sweeps == 4 &&
+               ((smoother == FS_SMOOTHER_RED_BLACK && smooth_sweeps == 4) || (long long)W * H <= kSmootherMinCells);
+    }
+    // Applies a smoother choice (the caller has drained the grid's streams).
+    void set_smoother(fs_pressure_smoother kind, int sweeps, int coarse)
+    {
+        smoother = kind;
+        smooth_sweeps = sweeps;
+        if (coarse != coarse_sweeps) {
+            coarse_sweeps = coarse;
+            tail.sweeps = coarse;
+            if (tail_dev.p)
+                device_upload(tail_dev.p, &tail, sizeof(TailDesc));
+        }
+        drop_graph();
+        if (kind == FS_SMOOTHER_CHEBYSHEV && !levels.empty() && !jacobi_tuned)
+            tune_jacobi();
     }
 
     // Allocates (or releases) the 16-bit arrays of the current mode and points the tail's top level at
@@ -1109,7 +1127,7 @@ struct fs_grid {
     }
     int sweeps_on(int l) const
     {
-        return (long long)levels[l].w * levels[l].h > kSmootherMinCells ? smooth_sweeps : 4;
+        return (long long)levels[l].w * levels[l].h > kSmootherMinCells ? smooth_sweeps : coarse_sweeps;
     }
     // Weights of the Chebyshev sweeps in the order a pre- (or, reversed, a post-) smoothing applies them.
     SmoothWeights chebyshev_weights(bool pre) const
@@ -1913,7 +1931,7 @@ fs_status fs_set_multigrid_precision(fs_grid* grid, fs_mg_precision mode)
     });
 }
 
-fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps)
+fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps, int coarse_sweeps)
 {
     return guard([&] {
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
@@ -1921,22 +1939,19 @@ fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int
             "unknown pressure smoother");
         need(sweeps >= 1 && sweeps <= (kind == FS_SMOOTHER_RED_BLACK ? 4 : kJacobiMaxSweeps), FS_E_INVALID_ARGUMENT,
             "sweeps must be 1..4 (red-black) or 1..5 (Chebyshev)");
+        need(coarse_sweeps >= 1 && coarse_sweeps <= 4, FS_E_INVALID_ARGUMENT, "coarse sweeps must be 1..4");
         need(kind == FS_SMOOTHER_RED_BLACK || grid->mg_prec == FS_MG_FP32, FS_E_UNSUPPORTED,
             "the Chebyshev smoother runs FP32 multigrid levels: set the multigrid precision to FS_MG_FP32");
-        if (kind == grid->smoother && sweeps == grid->smooth_sweeps)
+        if (kind == grid->smoother && sweeps == grid->smooth_sweeps && coarse_sweeps == grid->coarse_sweeps)
             return;
         set_device(grid->ctx);
         grid->drain();
-        grid->smoother = kind;
-        grid->smooth_sweeps = sweeps;
-        grid->drop_graph();
-        if (kind == FS_SMOOTHER_CHEBYSHEV && !grid->levels.empty())
-            grid->tune_jacobi();
+        grid->set_smoother(kind, sweeps, coarse_sweeps);
     });
 }
 
 fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
-    fs_pressure_smoother* kind, int* sweeps, double* ms)
+    fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps, double* ms)
 {
     return guard([&] {
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
@@ -1969,30 +1984,29 @@ fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations
             copy(g.p_prev.p, spp.p, g.p_prev.n);
             g.predicted = predicted;
         };
-        auto choose = [&](fs_pressure_smoother k, int n) {
-            g.smoother = k;
-            g.smooth_sweeps = n;
-            g.drop_graph();
-            if (k == FS_SMOOTHER_CHEBYSHEV && !g.jacobi_tuned)
-                g.tune_jacobi();
+        auto choose = [&](fs_pressure_smoother k, int n, int c) {
+            HIPC(hipStreamSynchronize(g.st()));
+            g.set_smoother(k, n, c);
         };
         save();
         struct Candidate {
             fs_pressure_smoother kind;
-            int sweeps;
+            int sweeps, coarse;
         };
-        std::vector<Candidate> candidates = {{FS_SMOOTHER_RED_BLACK, 4}, {FS_SMOOTHER_RED_BLACK, 3},
-            {FS_SMOOTHER_RED_BLACK, 2}};
-        if (g.mg_prec == FS_MG_FP32)
-            candidates.insert(candidates.end(),
-                {{FS_SMOOTHER_CHEBYSHEV, 3}, {FS_SMOOTHER_CHEBYSHEV, 4}, {FS_SMOOTHER_CHEBYSHEV, 5}});
-        const fs_pressure_smoother kind0 = g.smoother;
-        const int sweeps0 = g.smooth_sweeps;
-        Candidate best{kind0, sweeps0};
+        std::vector<Candidate> candidates;
+        for (int coarse : {4, 2}) {
+            for (int n : {4, 3, 2})
+                candidates.push_back({FS_SMOOTHER_RED_BLACK, n, coarse});
+            if (g.mg_prec == FS_MG_FP32)
+                for (int n : {3, 4, 5})
+                    candidates.push_back({FS_SMOOTHER_CHEBYSHEV, n, coarse});
+        }
+        const Candidate original{g.smoother, g.smooth_sweeps, g.coarse_sweeps};
+        Candidate best = original;
         double best_ms = std::numeric_limits<double>::infinity();
         try {
             for (const Candidate& c : candidates) {
-                choose(c.kind, c.sweeps);
+                choose(c.kind, c.sweeps, c.coarse);
                 double fastest = std::numeric_limits<double>::infinity();
                 bool converged = true;
                 for (int rep = 0; rep < 3; ++rep) {   // the first is a warm-up (graph capture, first replays)
@@ -2011,24 +2025,26 @@ fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations
                 }
             }
         } catch (...) {
-            choose(kind0, sweeps0);
+            choose(original.kind, original.sweeps, original.coarse);
             restore();
             HIPC(hipStreamSynchronize(g.st()));
             throw;
         }
-        choose(best.kind, best.sweeps);
+        choose(best.kind, best.sweeps, best.coarse);
         restore();
         HIPC(hipStreamSynchronize(g.st()));
         if (kind)
             *kind = best.kind;
         if (sweeps)
             *sweeps = best.sweeps;
+        if (coarse_sweeps)
+            *coarse_sweeps = best.coarse;
         if (ms)
             *ms = best_ms;
     });
 }
 
-fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps)
+fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps)
 {
     return guard([&] {
         need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
@@ -2036,6 +2052,8 @@ fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* ki
             *kind = grid->smoother;
         if (sweeps)
             *sweeps = grid->smooth_sweeps;
+        if (coarse_sweeps)
+            *coarse_sweeps = grid->coarse_sweeps;
     });
 }
 

@@ -83,6 +83,7 @@ struct TailDesc {
     int table_words;
     int lds_tables_base;       // word offset of the table block in LDS
     int lds_words;             // total LDS words
+    int sweeps;                // red-black sweeps per side on every tail level (4: the reference's V(4,4))
     TailLevel lev[kMaxTailLevels];
 };
 

@@ -807,11 +807,11 @@ __device__ __forceinline__ bool wave_has_items(int items) { return (int)(threadI
 // integer, and the float product's error is below (t / d) 2^-22 < 1/(2d).
 __device__ __forceinline__ int tail_div(int t, float inv_d) { return (int)(((float)t + 0.5f) * inv_d); }
 
-// Four red-black sweeps of one tail level with each thread's cells, conductances, rhs and 1/diagonal
-// held in registers across all eight half-sweeps: a half-sweep is four LDS reads, one write and a
-// barrier per item.
+// `sweeps` (uniform, 1..4) red-black sweeps of one tail level with each thread's cells, conductances, rhs
+// and 1/diagonal held in registers across all the half-sweeps: a half-sweep is four LDS reads, one write
+// and a barrier per item.
 template <int ITER>
-__device__ void tail_smooth4_n(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
+__device__ void tail_smooth4_n(const TailLevel& T, const TailArrays& A, int reverse, int from_zero, int sweeps)
 {
     const int w = T.w, h = T.h, s = T.s;
     const int pairs = (w + 1) >> 1;
@@ -843,7 +843,7 @@ __device__ void tail_smooth4_n(const TailLevel& T, const TailArrays& A, int reve
 #pragma unroll
         for (int it = 0; it < ITER; ++it)
             inv[c][it] = inv_or_zero((wl[c][it] + wr[c][it]) + (wu[c][it] + wd[c][it]));
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < sweeps; ++k) {
         for (int pass = 0; pass < 2; ++pass) {
             const int color = reverse ? 1 - pass : pass;
             const bool neighbours = !(from_zero && k == 0 && pass == 0);
@@ -869,16 +869,17 @@ __device__ void tail_smooth4_n(const TailLevel& T, const TailArrays& A, int reve
     }
 }
 
-// Four sweeps (PressureMultigrid.cpp:204-231 applied four times); reverse = post-smoothing colour order.
-__device__ void tail_smooth4(const TailLevel& T, const TailArrays& A, int reverse, int from_zero)
+// `sweeps` sweeps (PressureMultigrid.cpp:204-231 applied four times in the reference's cycle); reverse =
+// post-smoothing colour order.
+__device__ void tail_smooth4(const TailLevel& T, const TailArrays& A, int reverse, int from_zero, int sweeps)
 {
     const int items = ((T.w + 1) >> 1) * T.h;
     if (items <= kTailThreads)
-        tail_smooth4_n<1>(T, A, reverse, from_zero);
+        tail_smooth4_n<1>(T, A, reverse, from_zero, sweeps);
     else if (items <= 2 * kTailThreads)
-        tail_smooth4_n<2>(T, A, reverse, from_zero);
+        tail_smooth4_n<2>(T, A, reverse, from_zero, sweeps);
     else
-        tail_smooth4_n<kTailIter>(T, A, reverse, from_zero);
+        tail_smooth4_n<kTailIter>(T, A, reverse, from_zero, sweeps);
 }
 
 template <int ITER> __device__ void tail_residual_n(const TailLevel& T, const TailArrays& F)
@@ -1146,7 +1147,7 @@ template <class TV = float> __device__ void tail_run(const TailDesc& D, const do
     for (int l = 0; l + 1 < nl; ++l) {
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
-        tail_smooth4(T, A, 0, 1);
+        tail_smooth4(T, A, 0, 1, D.sweeps);
         tail_residual_restrict(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
     }
     if (threadIdx.x < 64)
@@ -1156,7 +1157,7 @@ template <class TV = float> __device__ void tail_run(const TailDesc& D, const do
         const TailLevel& T = D.lev[l];
         const TailArrays A = tail_arrays(lds, T);
         tail_prolong(T, A, D.lev[l + 1], tail_arrays(lds, D.lev[l + 1]));
-        tail_smooth4(T, A, 1, 0);
+        tail_smooth4(T, A, 1, 0, D.sweeps);
     }
     const TailArrays A0 = tail_arrays(lds, T0);
     const float inv_w0 = 1.0f / (float)T0.w;

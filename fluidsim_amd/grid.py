@@ -536,23 +536,25 @@ class FluidGridHIP:
         call("fs_get_relative_tolerance", self._h, C.byref(m))
         return bool(m.value)
 
-    def setPressureSmoother(self, kind: int, sweeps: int):
+    def setPressureSmoother(self, kind: int, sweeps: int, coarse_sweeps: int = 4):
         """Smoother of the multigrid levels above 4096 cells: FS_SMOOTHER_RED_BLACK (1..4 sweeps per side, default
-        4) or FS_SMOOTHER_CHEBYSHEV (Chebyshev-weighted damped Jacobi, 1..5 sweeps; FP32 levels only)."""
-        call("fs_set_pressure_smoother", self._h, int(kind), int(sweeps))
+        4) or FS_SMOOTHER_CHEBYSHEV (Chebyshev-weighted damped Jacobi, 1..5 sweeps; FP32 levels only); the smaller
+        levels run `coarse_sweeps` (1..4) red-black sweeps."""
+        call("fs_set_pressure_smoother", self._h, int(kind), int(sweeps), int(coarse_sweeps))
 
     def getPressureSmoother(self):
-        k, n = C.c_int(), C.c_int()
-        call("fs_get_pressure_smoother", self._h, C.byref(k), C.byref(n))
-        return k.value, n.value
+        """(kind, sweeps, coarse_sweeps)."""
+        k, n, c = C.c_int(), C.c_int(), C.c_int()
+        call("fs_get_pressure_smoother", self._h, C.byref(k), C.byref(n), C.byref(c))
+        return k.value, n.value, c.value
 
     def tunePressureSmoother(self, dt: float, max_iterations: int = 200, divergence_tolerance: float = 1e-6):
         """Times a projection of the current state with each smoother candidate and keeps the fastest that
-        converges; the state is unchanged.  Returns (kind, sweeps, ms per projection)."""
-        k, n, ms = C.c_int(), C.c_int(), C.c_double()
+        converges; the state is unchanged.  Returns (kind, sweeps, coarse_sweeps, ms per projection)."""
+        k, n, c, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         call("fs_tune_pressure_smoother", self._h, float(dt), int(max_iterations), float(divergence_tolerance),
-             C.byref(k), C.byref(n), C.byref(ms))
-        return k.value, n.value, ms.value
+             C.byref(k), C.byref(n), C.byref(c), C.byref(ms))
+        return k.value, n.value, c.value, ms.value
 
     def getMultigridPrecision(self) -> int:
         m = C.c_int()

@@ -21,6 +21,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -320,15 +321,22 @@ public:
     void setMultigridPrecision(fs_mg_precision mode) { check(fs_set_multigrid_precision(grid_, mode)); }
     // step/project tolerances as relative residuals (fs_set_relative_tolerance)
     void setRelativeTolerance(bool enabled) { check(fs_set_relative_tolerance(grid_, enabled ? 1 : 0)); }
-    // Smoother of the multigrid levels above 4096 cells (fs_set_pressure_smoother; build's own setting).
-    void setPressureSmoother(fs_pressure_smoother kind, int sweeps) { check(fs_set_pressure_smoother(grid_, kind, sweeps)); }
-    // Keeps the fastest converging smoother for a projection of the current state (fs_tune_pressure_smoother).
-    std::pair<fs_pressure_smoother, int> tunePressureSmoother(double dt, int max_iterations, double divergence_tolerance)
+    // Smoother of the multigrid levels above 4096 cells and sweeps of the smaller ones
+    // (fs_set_pressure_smoother; build's own setting).
+    void setPressureSmoother(fs_pressure_smoother kind, int sweeps, int coarse_sweeps = 4)
+    {
+        check(fs_set_pressure_smoother(grid_, kind, sweeps, coarse_sweeps));
+    }
+    // Keeps the fastest converging smoother for a projection of the current state (fs_tune_pressure_smoother);
+    // returns (kind, sweeps, coarse sweeps).
+    std::tuple<fs_pressure_smoother, int, int> tunePressureSmoother(double dt, int max_iterations,
+        double divergence_tolerance)
     {
         fs_pressure_smoother kind = FS_SMOOTHER_RED_BLACK;
-        int sweeps = 4;
-        check(fs_tune_pressure_smoother(grid_, dt, max_iterations, divergence_tolerance, &kind, &sweeps, nullptr));
-        return {kind, sweeps};
+        int sweeps = 4, coarse = 4;
+        check(fs_tune_pressure_smoother(grid_, dt, max_iterations, divergence_tolerance, &kind, &sweeps, &coarse,
+            nullptr));
+        return {kind, sweeps, coarse};
     }
     fs_mg_precision getMultigridPrecision() const
     {

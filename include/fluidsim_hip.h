@@ -112,7 +112,7 @@ typedef enum fs_mg_precision {
  * multigrid Chebyshev-weighted damped Jacobi on levels above 65536 cells, PressureMultigridGPU.cpp:24-44,
  * 619-631).  Both are symmetric, so the V-cycle stays an SPD preconditioner; the choice changes the
  * preconditioner (iteration counts, results within the tolerance), not the equation.  Levels of at most
- * 4096 cells always run red-black V(4,4).
+ * 4096 cells run red-black Gauss-Seidel with their own sweep count (coarse sweeps, 1..4).
  *   FS_SMOOTHER_RED_BLACK  red-black Gauss-Seidel, 1..4 sweeps per side (default: 4, the reference's cycle);
  *   FS_SMOOTHER_CHEBYSHEV  damped Jacobi with the weights 1 / (1.1 + 0.9 cos(pi (k + 1/2) / n)), k = 0..n-1,
  *                          in order before the coarse correction and reversed after it, n = 1..5 sweeps per
@@ -197,17 +197,20 @@ fs_status fs_solid_cells(const fs_grid* grid, long long* count);
  * max|b - A p| <= tol * max|b| (b the mean-free pressure rhs) and the projection is accepted when
  * max|div u| <= tol * max|b| * dt / h^2, i.e. tol times the divergence the projection started from. */
 fs_status fs_set_relative_tolerance(fs_grid* grid, int enabled);
-/* Pressure smoother choice (see fs_pressure_smoother).  FS_E_UNSUPPORTED for Chebyshev on a grid with
- * 16-bit multigrid levels.  Batched solves (fs_batch_project) need the default. */
-fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps);
-fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps);
-/* Autotuner of the smoother and its sweep count (no reference counterpart): projects the grid's current
- * state with each candidate (red-black 2, 3, 4 and Chebyshev 3, 4, 5 sweeps, each from the same state,
- * restored after every trial: the grid's state is unchanged on return) with fs_project's arguments, times
- * each projection on the device after one warm-up, and selects the fastest candidate whose projection
- * converges.  kind, sweeps and ms (each may be NULL) receive the choice and its time per projection. */
+/* Pressure smoother choice (see fs_pressure_smoother): `kind` with `sweeps` per side on the levels above 4096
+ * cells, red-black with `coarse_sweeps` (1..4) per side on the smaller ones.  (FS_SMOOTHER_RED_BLACK, 4, 4)
+ * is the reference's V(4,4).  FS_E_UNSUPPORTED for Chebyshev on a grid with 16-bit multigrid levels.
+ * Batched solves (fs_batch_project) need the default. */
+fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps, int coarse_sweeps);
+fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps);
+/* Autotuner of the smoother and its sweep counts (no reference counterpart): projects the grid's current
+ * state with each candidate (red-black 2, 3, 4 and Chebyshev 3, 4, 5 sweeps, each with 4 and with 2 coarse
+ * sweeps; each from the same state, restored after every trial: the grid's state is unchanged on return)
+ * with fs_project's arguments, times each projection after one warm-up, and selects the fastest candidate
+ * whose projection converges.  kind, sweeps, coarse_sweeps and ms (each may be NULL) receive the choice and
+ * its time per projection. */
 fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
-    fs_pressure_smoother* kind, int* sweeps, double* ms);
+    fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps, double* ms);
 fs_status fs_get_relative_tolerance(const fs_grid* grid, int* enabled);
 
 /* upload(const State&) (FluidGridBase.h:118, FluidGridCPU.cpp:198-251).  NULL means an empty

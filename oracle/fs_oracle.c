@@ -515,17 +515,21 @@ static void solve_coarsest(orc_mg* mg)
 /* Smoother choice of the levels above ORC_SMOOTH_MIN_CELLS cells (an execution choice: the reference's CPU
  * multigrid runs red-black Gauss-Seidel, PressureMultigrid.cpp:204-231, 396; its WebGPU multigrid runs
  * Chebyshev-weighted damped Jacobi on levels above 65536 cells, PressureMultigridGPU.cpp:24-28, 619-631).
- * Smaller levels always keep V(4,4) red-black Gauss-Seidel.  Process-wide, like the worker count. */
+ * Smaller levels run red-black Gauss-Seidel with their own sweep count (4 in the reference's V(4,4)).
+ * Process-wide, like the worker count. */
 #define ORC_SMOOTH_MIN_CELLS 4096
 static int g_smooth_kind = ORC_SMOOTH_RBGS;
 static int g_smooth_sweeps = 4;
+static int g_coarse_sweeps = 4;
 
-int orc_set_smoother(int kind, int sweeps)
+int orc_set_smoother(int kind, int sweeps, int coarse_sweeps)
 {
-    if ((kind != ORC_SMOOTH_RBGS && kind != ORC_SMOOTH_CHEBYSHEV) || sweeps < 1 || sweeps > 8)
+    if ((kind != ORC_SMOOTH_RBGS && kind != ORC_SMOOTH_CHEBYSHEV) || sweeps < 1 || sweeps > 8 ||
+        coarse_sweeps < 1 || coarse_sweeps > 8)
         return -1;
     g_smooth_kind = kind;
     g_smooth_sweeps = sweeps;
+    g_coarse_sweeps = coarse_sweeps;
     return 0;
 }
 
@@ -578,7 +582,7 @@ static void cycle(orc_mg* mg, int index)
     orc_level* C = &mg->lev[index + 1];
     const int large = (long)F->w * F->h > ORC_SMOOTH_MIN_CELLS;
     const int kind = large ? g_smooth_kind : ORC_SMOOTH_RBGS;
-    const int sweeps = large ? g_smooth_sweeps : 4;
+    const int sweeps = large ? g_smooth_sweeps : g_coarse_sweeps;
     if (kind == ORC_SMOOTH_CHEBYSHEV) {
         for (int k = 0; k < sweeps; ++k)
             jacobi(F, orc_chebyshev_weight(k, sweeps), k == 0);

@@ -105,11 +105,11 @@ int orc_get_threads(void);
 /* Smoother of the multigrid levels above 4096 cells (an execution choice, matching fs_set_pressure_smoother):
  * ORC_SMOOTH_RBGS (the reference CPU's red-black Gauss-Seidel, PressureMultigrid.cpp:204-231) or
  * ORC_SMOOTH_CHEBYSHEV (the reference WebGPU's Chebyshev-weighted damped Jacobi,
- * PressureMultigridGPU.cpp:24-28, 619-631), with `sweeps` (1..8) per side.  Levels of at most 4096 cells keep
- * red-black V(4,4).  Default RBGS, 4 sweeps: the reference's cycle.  Returns -1 for an invalid choice.
- * Process-wide. */
+ * PressureMultigridGPU.cpp:24-28, 619-631), with `sweeps` (1..8) per side; levels of at most 4096 cells run
+ * `coarse_sweeps` (1..8) red-black sweeps.  Default RBGS 4, 4: the reference's cycle.  Returns -1 for an
+ * invalid choice.  Process-wide. */
 enum { ORC_SMOOTH_RBGS = 0, ORC_SMOOTH_CHEBYSHEV = 1 };
-int orc_set_smoother(int kind, int sweeps);
+int orc_set_smoother(int kind, int sweeps, int coarse_sweeps);
 /* Weight i of `sweeps` Chebyshev-weighted Jacobi sweeps (float-rounded, as the reference stores them). */
 double orc_chebyshev_weight(int i, int sweeps);
 /* Extension matching fs_set_relative_tolerance: tolerances become relative, the PCG stopping at

@@ -75,7 +75,7 @@ def lib():
         "orc_set_gravity": (None, [vp, C.c_double]),
         "orc_set_relative_tolerance": (None, [vp, C.c_int]),
         "orc_set_threads": (None, [C.c_int]),
-        "orc_set_smoother": (C.c_int, [C.c_int, C.c_int]),
+        "orc_set_smoother": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         "orc_chebyshev_weight": (C.c_double, [C.c_int, C.c_int]),
         "orc_get_threads": (C.c_int, []),
         "orc_set_preconditioner": (None, [vp, C.c_int]),
@@ -408,11 +408,11 @@ def get_threads():
 SMOOTH_RBGS, SMOOTH_CHEBYSHEV = 0, 1
 
 
-def set_smoother(kind, sweeps):
-    """Smoother of the multigrid levels above 4096 cells: SMOOTH_RBGS or SMOOTH_CHEBYSHEV, `sweeps` per side
-    (levels of at most 4096 cells keep red-black V(4,4)); process-wide."""
-    if lib().orc_set_smoother(int(kind), int(sweeps)) != 0:
-        raise ValueError(f"invalid smoother {kind} with {sweeps} sweeps")
+def set_smoother(kind, sweeps, coarse_sweeps=4):
+    """Smoother of the multigrid levels above 4096 cells: SMOOTH_RBGS or SMOOTH_CHEBYSHEV, `sweeps` per side;
+    levels of at most 4096 cells run `coarse_sweeps` red-black sweeps; process-wide."""
+    if lib().orc_set_smoother(int(kind), int(sweeps), int(coarse_sweeps)) != 0:
+        raise ValueError(f"invalid smoother {kind} with {sweeps} / {coarse_sweeps} sweeps")
 
 
 def chebyshev_weight(i, sweeps):

@@ -23,8 +23,9 @@ from fluidsim_amd import workloads as WL
 
 pytestmark = pytest.mark.gpu
 
-CHOICES = [(FS_SMOOTHER_CHEBYSHEV, 3), (FS_SMOOTHER_CHEBYSHEV, 5), (FS_SMOOTHER_RED_BLACK, 2),
-           (FS_SMOOTHER_RED_BLACK, 3)]
+# (kind, sweeps above 4096 cells, red-black sweeps below)
+CHOICES = [(FS_SMOOTHER_CHEBYSHEV, 3, 4), (FS_SMOOTHER_CHEBYSHEV, 5, 4), (FS_SMOOTHER_RED_BLACK, 2, 4),
+           (FS_SMOOTHER_RED_BLACK, 3, 4), (FS_SMOOTHER_CHEBYSHEV, 3, 2), (FS_SMOOTHER_RED_BLACK, 4, 1)]
 
 
 @pytest.fixture
@@ -55,15 +56,15 @@ def oracle_grid(O, wl):
     return o
 
 
-@pytest.mark.parametrize("kind,sweeps", CHOICES)
+@pytest.mark.parametrize("kind,sweeps,coarse", CHOICES)
 @pytest.mark.parametrize("case", ["pool", "tank"])
-def test_vcycle_matches_the_oracle(O, case, kind, sweeps):
+def test_vcycle_matches_the_oracle(O, case, kind, sweeps, coarse):
     W, H = (300, 200) if case == "pool" else (256, 192)
     wl = workload(case, W, H)
     g = hip_grid(wl)
-    g.setPressureSmoother(kind, sweeps)
-    assert g.getPressureSmoother() == (kind, sweeps)
-    O.set_smoother(kind, sweeps)
+    g.setPressureSmoother(kind, sweeps, coarse)
+    assert g.getPressureSmoother() == (kind, sweeps, coarse)
+    O.set_smoother(kind, sweeps, coarse)
     o = oracle_grid(O, wl)
     tau = 1e-6 * o.max_divergence()
     o.project(wl.dt, 0, tau)   # the wall step closes solid faces in both
@@ -79,7 +80,7 @@ def test_vcycle_matches_the_oracle(O, case, kind, sweeps):
     b -= b.mean()
     ho, hg = mg.apply(a, 0.0), g.multigridApply(a)
     err = np.linalg.norm(hg - ho) / np.linalg.norm(ho)
-    print(f"{case} {kind}/{sweeps}: V-cycle rel L2 {err:.2e}")
+    print(f"{case} {kind}/{sweeps}/{coarse}: V-cycle rel L2 {err:.2e}")
     assert err < 1e-5
     ma, mb = hg - hg.mean(), g.multigridApply(b)
     mb -= mb.mean()
@@ -114,19 +115,19 @@ def test_every_jacobi_tile_computes_the_same_values():
     g.destroy()
 
 
-@pytest.mark.parametrize("kind,sweeps", CHOICES + [(FS_SMOOTHER_CHEBYSHEV, 4)])
+@pytest.mark.parametrize("kind,sweeps,coarse", CHOICES + [(FS_SMOOTHER_CHEBYSHEV, 4, 4)])
 @pytest.mark.parametrize("case", ["pool", "tank"])
-def test_projection_matches_the_oracle(O, case, kind, sweeps):
+def test_projection_matches_the_oracle(O, case, kind, sweeps, coarse):
     W, H = (320, 240) if case == "pool" else (256, 256)
     wl = workload(case, W, H)
     g = hip_grid(wl)
-    g.setPressureSmoother(kind, sweeps)
-    O.set_smoother(kind, sweeps)
+    g.setPressureSmoother(kind, sweeps, coarse)
+    O.set_smoother(kind, sweeps, coarse)
     o = oracle_grid(O, wl)
     tau = 1e-6 * o.max_divergence()
     ro = o.project(wl.dt, 4000, tau)
     rg = g.project(wl.dt, 4000, tau)
-    print(f"{case} {kind}/{sweeps}: iterations GPU {rg.iterations} oracle {ro['iterations']}")
+    print(f"{case} {kind}/{sweeps}/{coarse}: iterations GPU {rg.iterations} oracle {ro['iterations']}")
     assert ro["converged"] and rg.converged, (ro, rg)
     # FP32 CG loses conjugacy on the masked 1000:1 tank (1.5-1.8x the FP64 iterations, the price of an FP32
     # iterate and preconditioner); the pool stays within a few iterations
@@ -142,9 +143,9 @@ def test_tuner_keeps_the_state_and_picks_a_converging_choice():
     g = hip_grid(wl)
     before = g.download()
     tau = 1e-6 * g.maxDivergence()
-    kind, sweeps, ms = g.tunePressureSmoother(wl.dt, 200, tau)
-    print(f"tuned: kind {kind}, {sweeps} sweeps, {ms:.3f} ms per projection")
-    assert (kind, sweeps) == g.getPressureSmoother() and ms > 0
+    kind, sweeps, coarse, ms = g.tunePressureSmoother(wl.dt, 200, tau)
+    print(f"tuned: kind {kind}, {sweeps} sweeps, {coarse} coarse sweeps, {ms:.3f} ms per projection")
+    assert (kind, sweeps, coarse) == g.getPressureSmoother() and ms > 0
     after = g.download()
     for key in ("liquid_fraction", "u", "v", "pressure"):
         assert np.array_equal(getattr(after, key), getattr(before, key)), key
@@ -176,5 +177,11 @@ def test_batched_solves_keep_the_reference_cycle():
         a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 5)
     with pytest.raises(ValueError):
         a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 6)
+    with pytest.raises(ValueError):
+        a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 4, 0)
+    # fewer coarse sweeps alone also leave the reference's cycle
+    a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 4, 2)
+    with pytest.raises(NotImplementedError):
+        batch_project([a], 0.002, 500, 1e-6)
     a.destroy()
     b.destroy()

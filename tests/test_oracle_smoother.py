@@ -4,7 +4,8 @@ red-black sweep count) pinned without the reference -- CPU only.
   - the weights 1 / (1.1 + 0.9 cos(pi (k + 1/2) / n)) make the n-sweep error polynomial prod_k (1 - w_k l)
     the Chebyshev polynomial T_n((1.1 - l) / 0.9) / T_n(11 / 9) of [0.2, 2] (PressureMultigridGPU.cpp:24-28):
     the closed form holds at every l to float rounding of the weights;
-  - every choice keeps the V-cycle an SPD preconditioner on mean-free vectors (tests/FluidGridTests.cpp:538-625
+  - every choice (smoother and sweeps above 4096 cells, red-black sweeps below) keeps the V-cycle an SPD
+    preconditioner on mean-free vectors (tests/FluidGridTests.cpp:538-625
     bars: linear, symmetric to rounding, positive, constants to zero), with and without solid masks;
   - MG-PCG with every choice reaches the same pressure and velocity as the reference's V(4,4) cycle (the
     reference's MG-vs-diagonal bar, 2e-6 at tol 1e-8, tests/FluidGridTests.cpp:627-660), and the Chebyshev
@@ -17,7 +18,9 @@ import pytest
 
 from fluidsim_amd import workloads as WL
 
-CHOICES = [(0, 4), (0, 3), (0, 2), (0, 1), (1, 1), (1, 3), (1, 4), (1, 5)]
+# (kind, sweeps on the levels above 4096 cells, red-black sweeps on the smaller ones)
+CHOICES = [(0, 4, 4), (0, 3, 4), (0, 2, 4), (0, 1, 4), (1, 1, 4), (1, 3, 4), (1, 4, 4), (1, 5, 4), (1, 3, 2),
+           (0, 4, 1)]
 
 
 @pytest.fixture
@@ -44,9 +47,9 @@ def test_weights_are_the_chebyshev_polynomial_of_the_interval(O, n):
 
 
 def test_invalid_choices_are_refused(O):
-    for kind, sweeps in [(2, 4), (0, 0), (1, 9), (-1, 3)]:
+    for kind, sweeps, coarse in [(2, 4, 4), (0, 0, 4), (1, 9, 4), (-1, 3, 4), (0, 4, 0), (1, 3, 9)]:
         with pytest.raises(ValueError):
-            O.set_smoother(kind, sweeps)
+            O.set_smoother(kind, sweeps, coarse)
 
 
 def tank(O, W, H, masked):
@@ -62,10 +65,10 @@ def tank(O, W, H, masked):
 
 
 @pytest.mark.parametrize("masked", [False, True])
-@pytest.mark.parametrize("kind,sweeps", CHOICES)
-def test_every_choice_is_an_spd_preconditioner(O, masked, kind, sweeps):
-    O.set_smoother(kind, sweeps)
-    W, H = 160, 120     # levels 160 x 120 and 80 x 60 run the choice, the rest red-black V(4,4)
+@pytest.mark.parametrize("kind,sweeps,coarse", CHOICES)
+def test_every_choice_is_an_spd_preconditioner(O, masked, kind, sweeps, coarse):
+    O.set_smoother(kind, sweeps, coarse)
+    W, H = 160, 120     # levels 160 x 120 and 80 x 60 run the choice, the rest the coarse sweeps
     g, _ = tank(O, W, H, masked)
     mg = g.multigrid()
     rng = np.random.default_rng(7)
@@ -87,14 +90,14 @@ def test_every_choice_is_an_spd_preconditioner(O, masked, kind, sweeps):
 def test_every_choice_reaches_the_reference_cycles_solution(O, masked):
     W, H = 144, 112
     out = {}
-    for kind, sweeps in [(0, 4)] + [c for c in CHOICES if c != (0, 4)]:
-        O.set_smoother(kind, sweeps)
+    for choice in CHOICES:
+        O.set_smoother(*choice)
         g, wl = tank(O, W, H, masked)
         r = g.project(wl.dt, 4000, 1e-8)
-        assert r["converged"], (kind, sweeps, r)
+        assert r["converged"], (choice, r)
         s = g.download()
-        out[(kind, sweeps)] = (r["iterations"], s["u"], s["v"])
-    _, u0, v0 = out[(0, 4)]
+        out[choice] = (r["iterations"], s["u"], s["v"])
+    _, u0, v0 = out[(0, 4, 4)]
     for key, (it, u, v) in out.items():
         assert np.abs(u - u0).max() < 2e-6 and np.abs(v - v0).max() < 2e-6, key
 
