@@ -122,10 +122,13 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(wl, tau, budget_s):
-    """The FP64 CPU oracle (the restatement of UniformCPU) on the same projection, timed as the reference
-    runs it: its passes on min(8, host cores) workers (FluidGridCPU.cpp:25-31, PressureMultigrid.cpp:13-21),
-    and again on one thread.  Median of the timed projections after one warm-up, half the budget each."""
+def cpu_baseline(wl, tau, budget_s, smoother=None):
+    """The FP64 CPU oracle (the restatement of UniformCPU, with its red-black V(4,4) cycle) on the same
+    projection, timed as the reference runs it: its passes on min(8, host cores) workers
+    (FluidGridCPU.cpp:25-31, PressureMultigrid.cpp:13-21), and again on one thread.  Median of the timed
+    projections after one warm-up, half the budget each.  Returns (the oracle's projected state, the baseline,
+    the oracle's projected state with `smoother` = (kind, sweeps, coarse sweeps), the GPU's choice, or
+    None)."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O  # noqa: E402  (checker / baseline only)
 
@@ -154,7 +157,18 @@ def cpu_baseline(wl, tau, budget_s):
     med_n, count_n, it_n = timed(workers, 0.5 * budget_s)
     med_1, count_1, it_1 = timed(1, 0.5 * budget_s)
     state = g.download()   # the oracle's projection of the input, for the GPU result's error
-    return state, {
+    same = None
+    if smoother is not None:
+        O.set_threads(min(8, cores))
+        O.set_smoother(*smoother)
+        try:
+            g.upload(wl.fraction, wl.u, wl.v, None, wl.liquid_mass, wl.gas_mass)
+            g.project(wl.dt, 200, tau)
+            same = g.download()
+        finally:
+            O.set_smoother(O.SMOOTH_RBGS, 4, 4)
+            O.set_threads(1)
+    return state, same, {
         "value": 1.0 / med_n,
         "unit": "solves/s",
         "cores": workers,
@@ -809,8 +823,12 @@ def main():
         out["roofline"]["dominant_kernel_in_trace"] = dom
     if rank == 0 and not args.no_cpu:
         gpu_state = grid.download()   # the last timed projection
-        oracle_state, out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget)
+        oracle_state, same_state, out["cpu_baseline"] = cpu_baseline(wl, tau, args.cpu_budget, (int(kind), nu, coarse))
+        # against UniformCPU's own cycle (what the reference computes), and against the same smoother
         out["oracle_error"] = oracle_error(gpu_state, oracle_state)
+        out["oracle_error"]["against"] = "FP64 CPU oracle with the reference's red-black V(4,4), same input and tolerance"
+        out["oracle_error_same_smoother"] = oracle_error(gpu_state, same_state)
+        out["oracle_error_same_smoother"]["against"] = "FP64 CPU oracle with the GPU's smoother, same input and tolerance"
     if args.sl_steps > 0:
         out["semi_lagrangian"] = semi_lagrangian_leg(args.size, args.sl_steps)
     if args.c1_steps > 0:
