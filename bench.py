@@ -183,6 +183,14 @@ def cpu_baseline(wl, tau, budget_s, smoother=None):
     }
 
 
+def parse_smoother(text):
+    """'red_black:N:C' / 'chebyshev:N:C' -> (kind, N, C); empty -> None."""
+    if not text:
+        return None
+    name, *counts = text.split(":")
+    return (1 if name == "chebyshev" else 0, int(counts[0]) if counts else 4, int(counts[1]) if len(counts) > 1 else 4)
+
+
 def oracle_error(gpu, oracle):
     """Relative L2 distance of a GPU projection from the FP64 oracle's projection of the same input to the
     same tolerance: u, v, and the pressure with each field's mean removed (the pressure is defined up to a
@@ -239,7 +247,7 @@ def dam_break_leg(steps=100, seed=1):
 
 
 def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, batch_cells=0,
-                compare_batch_cells=0):
+                compare_batch_cells=0, smoother=None):
     """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
     [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each).
     It is first served one request at a time on one stream with FP32 multigrid levels (serial plain), then
@@ -250,17 +258,22 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
     (which large grids step side by side depends on arrival and thread timing), so the serial pass and each
     rate run `repeats` times and the median run is reported, with every run's rate listed.  With
     `compare_batch_cells`, one more run at the last rate batches the requests of at most that many cells
-    (fs_batch_step: one workgroup per instance in the persistent solves)."""
+    (fs_batch_step: one workgroup per instance in the persistent solves).  `smoother` (kind, sweeps, coarse
+    sweeps) is the pressure smoother of the served FP32-level requests; the serial pass keeps the reference's
+    V(4,4) cycle."""
     from fluidsim_amd import FS_MG_BF16, FS_MG_FP32
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
 
-    def stream_at(rate, precision=None, cells=batch_cells):
+    def stream_at(rate, precision=None, cells=batch_cells, choice=None):
         out = []
         for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps):
             small = kw["width"] * kw["height"] <= cells
-            p = precision if precision is not None else (FS_MG_FP32 if small else FS_MG_BF16)
-            out.append((t, Request(multigrid_precision=p, **kw)))
+            p = precision if precision is not None else (FS_MG_FP32 if small or choice else FS_MG_BF16)
+            extra = {}
+            if choice and not small:
+                extra = dict(pressure_smoother=choice[0], smoother_sweeps=choice[1], coarse_sweeps=choice[2])
+            out.append((t, Request(multigrid_precision=p, **kw, **extra)))
         return out
 
     base = stream_at(rates[0], FS_MG_FP32)
@@ -297,7 +310,7 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
 
     runs = []
     for rate in rates:
-        stream = stream_at(rate)
+        stream = stream_at(rate, choice=smoother)
         samples = []
         for _ in range(repeats):
             samples.append(serve(stream, batch_cells))
@@ -327,10 +340,13 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
                     "viscosity nu 1e-3..1, surface tension 0.01..0.07, MG-PCG to a 1e-6 relative residual "
                     "max|b - A p| <= 1e-6 max|b| in every step; FP32 "
                     "multigrid levels served one at a time; in the serving loop, requests of at most "
-                    f"{batch_cells} cells batched with FP32 levels, larger ones with bfloat16 conductances), "
+                    f"{batch_cells} cells batched with FP32 levels, larger ones "
+                    + (f"with FP32 levels and the smoother {smoother}" if smoother else "with bfloat16 conductances")
+                    + "), "
                     f"Poisson arrivals, seed {seed}",
         "lanes": lanes,
         "batch_cells": batch_cells,
+        "served_smoother": list(smoother) if smoother else None,
         "batched_requests": sum(r.width * r.height <= batch_cells for _, r in base),
         "cells_mean": float(np.mean([r.width * r.height for _, r in base])),
         "steps": total_steps,
@@ -624,6 +640,9 @@ def main():
     ap.add_argument("--serving-lanes", type=int, default=4)
     ap.add_argument("--serving-batch-cells", type=int, default=0,
                     help="C5: requests of at most this many cells step batched (0: every request alone)")
+    ap.add_argument("--serving-smoother", type=str, default="",
+                    help="C5: pressure smoother of the served requests, red_black:N:C or chebyshev:N:C "
+                         "(empty: the reference's V(4,4) with bfloat16 conductances)")
     ap.add_argument("--serving-compare-batch-cells", type=int, default=65536,
                     help="C5: one extra run at the last rate with this batch size, for comparison (0: skip)")
     ap.add_argument("--serving-repeats", type=int, default=3, help="runs per pass; the median is reported")
@@ -665,6 +684,7 @@ def main():
                                      args.serving_lanes, max_size=args.serving_max_size,
                                      batch_cells=args.serving_batch_cells,
                                      compare_batch_cells=args.serving_compare_batch_cells,
+                                     smoother=parse_smoother(args.serving_smoother),
                                      repeats=max(1, args.serving_repeats))), flush=True)
         return
 
@@ -844,6 +864,7 @@ def main():
                                      args.serving_lanes, max_size=args.serving_max_size,
                                      batch_cells=args.serving_batch_cells,
                                      compare_batch_cells=args.serving_compare_batch_cells,
+                                     smoother=parse_smoother(args.serving_smoother),
                                      repeats=max(1, args.serving_repeats))
     if rank == 0:
         print(json.dumps(out), flush=True)

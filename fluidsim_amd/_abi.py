@@ -110,6 +110,7 @@ class FsRequest(C.Structure):
         ("liquid_fraction", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("liquid_mass", C.POINTER(C.c_double)), ("gas_mass", C.POINTER(C.c_double)),
         ("multigrid_precision", C.c_int), ("solid", C.c_void_p), ("relative_tolerance", C.c_int),
+        ("pressure_smoother", C.c_int), ("smoother_sweeps", C.c_int), ("coarse_sweeps", C.c_int),
     ]
 
 

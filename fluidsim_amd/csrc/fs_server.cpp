@@ -331,7 +331,10 @@ struct fs_server {
                              r->checkpoint_payload.data(), r->checkpoint_payload.size()))) &&
                (r->solid.empty() || ok(fs_set_solid_mask(g, r->solid.data()))) &&
                ok(fs_set_multigrid_precision(g, (fs_mg_precision)p.multigrid_precision)) &&
-               ok(fs_set_relative_tolerance(g, p.relative_tolerance));
+               ok(fs_set_relative_tolerance(g, p.relative_tolerance)) &&
+               (p.smoother_sweeps == 0 ||
+                   ok(fs_set_pressure_smoother(g, (fs_pressure_smoother)p.pressure_smoother, p.smoother_sweeps,
+                       p.coarse_sweeps ? p.coarse_sweeps : 4)));
     }
 
     // Records one finished step of `r` (ms of lane time); ends the request when it failed, was not
@@ -595,6 +598,15 @@ fs_status fs_server_submit(fs_server* server, const fs_request* request, long lo
             "Surface tension must be finite and nonnegative");
         need(q.multigrid_precision >= FS_MG_FP32 && q.multigrid_precision <= FS_MG_BF16_VECTORS,
             FS_E_INVALID_ARGUMENT, "unknown multigrid precision");
+        if (q.smoother_sweeps != 0) {
+            const bool cheb = q.pressure_smoother == FS_SMOOTHER_CHEBYSHEV;
+            need(q.pressure_smoother == FS_SMOOTHER_RED_BLACK || cheb, FS_E_INVALID_ARGUMENT, "unknown pressure smoother");
+            need(q.smoother_sweeps >= 1 && q.smoother_sweeps <= (cheb ? 5 : 4) && q.coarse_sweeps >= 0 &&
+                     q.coarse_sweeps <= 4,
+                FS_E_INVALID_ARGUMENT, "invalid smoother sweep counts");
+            need(!cheb || q.multigrid_precision == FS_MG_FP32, FS_E_INVALID_ARGUMENT,
+                "the Chebyshev smoother runs FP32 multigrid levels");
+        }
         if (q.solid) {
             const std::string problem = fsim::solid_mask_problem(q.solid, q.width, q.height);
             need(problem.empty(), FS_E_INVALID_ARGUMENT, problem);

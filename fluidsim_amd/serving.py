@@ -42,6 +42,9 @@ class Request:
     multigrid_precision: int = _abi.FS_MG_FP32   # storage of the multigrid levels (fs_mg_precision)
     solid: Optional[np.ndarray] = None           # W * H obstacle mask, nonzero = solid
     relative_tolerance: bool = False             # divergence_tolerance relative to each projection's max|b|
+    pressure_smoother: int = _abi.FS_SMOOTHER_RED_BLACK   # fs_set_pressure_smoother; smoother_sweeps 0 keeps
+    smoother_sweeps: int = 0                              # the reference's V(4,4)
+    coarse_sweeps: int = 0
 
     def estimate_bytes(self) -> int:
         viscous = any(m > 0 for m in (self.liquid_viscosity or []) + (self.gas_viscosity or []))
@@ -145,7 +148,8 @@ class Server:
         req = FsRequest(W, H, float(r.cell_size), int(r.steps), float(r.dt), int(r.max_iterations),
                         float(r.divergence_tolerance), float(r.gravity_y), float(r.surface_tension), int(r.transport),
                         int(r.preconditioner), nl, ng, *ptrs, int(r.multigrid_precision),
-                        None if solid is None else solid.ctypes.data, int(bool(r.relative_tolerance)))
+                        None if solid is None else solid.ctypes.data, int(bool(r.relative_tolerance)),
+                        int(r.pressure_smoother), int(r.smoother_sweeps), int(r.coarse_sweeps))
         rid = C.c_longlong()
         call("fs_server_submit", self._h, C.byref(req), C.byref(rid))   # copies the arrays
         return rid.value

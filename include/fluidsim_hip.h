@@ -540,6 +540,10 @@ typedef struct fs_request {
     int multigrid_precision;        /* fs_mg_precision */
     const unsigned char* solid;     /* W*H solid mask (fs_set_solid_mask) or NULL; copied by fs_server_submit */
     int relative_tolerance;         /* nonzero: divergence_tolerance is relative (fs_set_relative_tolerance) */
+    /* pressure smoother (fs_set_pressure_smoother); smoother_sweeps 0 keeps the reference's V(4,4) */
+    int pressure_smoother;          /* fs_pressure_smoother */
+    int smoother_sweeps;            /* 0 or 1..5 on the levels above 4096 cells */
+    int coarse_sweeps;              /* 0 (= 4) or 1..4 on the smaller levels */
 } fs_request;
 
 enum { FS_REQUEST_QUEUED = 0, FS_REQUEST_RUNNING = 1, FS_REQUEST_DONE = 2, FS_REQUEST_FAILED = 3 };

@@ -196,3 +196,41 @@ def test_requests_with_obstacles_and_16bit_levels():
     assert iterations == rep.pressure_iterations
     for k, key in (("u", "u"), ("v", "v"), ("pressure", "pressure"), ("liquid_fraction", "fraction")):
         assert np.array_equal(getattr(s, k), out[key]), k
+
+
+def test_requests_with_a_smoother_choice():
+    """A request may carry a pressure smoother choice: its result equals the same steps on a plain grid with
+    the same choice, bit for bit; an invalid choice is refused at submission."""
+    from fluidsim_amd import FS_MG_BF16, FS_SMOOTHER_CHEBYSHEV
+
+    probe = request(150, 110, 50, steps=4, viscosity=0.01, sigma=0.02, pressure_smoother=FS_SMOOTHER_CHEBYSHEV,
+                    smoother_sweeps=3, coarse_sweeps=2)
+    with Server(lanes=2) as srv:
+        for bad in (dict(smoother_sweeps=6), dict(coarse_sweeps=5), dict(multigrid_precision=FS_MG_BF16)):
+            with pytest.raises(ValueError):
+                srv.submit(request(64, 64, 51, pressure_smoother=FS_SMOOTHER_CHEBYSHEV,
+                                   **{"smoother_sweeps": 3, **bad}))
+        rid = srv.submit(probe)
+        other = srv.submit(request(96, 80, 52, steps=3))
+        rep = srv.wait(rid)
+        assert rep.done, rep.error
+        assert srv.wait(other).done
+        out = srv.result(rid, probe.width, probe.height)
+    g = FluidGridHIP(probe.width, probe.height, probe.cell_size)
+    g.configureMaterials([], FluidMixture([MixtureComponent("water", 100.0, probe.liquid_viscosity[0])]),
+                         FluidMixture([MixtureComponent("air", 0.1, probe.gas_viscosity[0])]))
+    g.setGravityY(probe.gravity_y)
+    g.setSurfaceTension(probe.surface_tension)
+    g.upload(FluidGridState(probe.liquid_fraction, probe.u, probe.v, np.zeros(0), [probe.liquid_mass],
+                            [probe.gas_mass]))
+    g.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 3, 2)
+    iterations = 0
+    for _ in range(probe.steps):
+        r = g.step(probe.dt, probe.max_iterations, probe.divergence_tolerance)
+        assert r.accepted()
+        iterations += r.iterations
+    s = g.download()
+    g.destroy()
+    assert iterations == rep.pressure_iterations
+    for k, key in (("u", "u"), ("v", "v"), ("pressure", "pressure"), ("liquid_fraction", "fraction")):
+        assert np.array_equal(getattr(s, k), out[key]), k
