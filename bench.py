@@ -359,32 +359,56 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
     }
 
 
-def semi_lagrangian_leg(size=1024, steps=200, seed=2):
-    """Config C2's second part: `steps` full steps (semi-Lagrangian advection, gravity, projection to 1e-6
-    of the initial max divergence) of the vortex box, timed end to end."""
+def semi_lagrangian_leg(size=1024, steps=200, seed=2, smoother=None):
+    """Config C2's second part: `steps` full steps (semi-Lagrangian advection, gravity, projection) of the
+    vortex box, timed end to end, twice: every projection to a 1e-6 relative residual of its own rhs
+    (fs_set_relative_tolerance: max|b - A p| <= 1e-6 max|b| in every step, C2's stopping rule), and with the
+    tolerance fixed at 1e-6 of the initial max divergence (after the first steps the warm-started pressure
+    nearly meets it, so that run is mostly advection).  `smoother` = (kind, sweeps, coarse sweeps) or None."""
     from fluidsim_amd import FS_TRANSPORT_SEMI_LAGRANGIAN, FluidGridHIP
     from fluidsim_amd import workloads as WL
 
     wl = WL.vortex_box(size, size, seed=seed)
-    g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
-    WL.load_into(g, wl)
-    g.setTransport(FS_TRANSPORT_SEMI_LAGRANGIAN)
-    tau = 1e-6 * g.maxDivergence()
-    g.step(wl.dt, 200, tau)   # warm-up: hierarchy, graphs
-    its = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        r = g.step(wl.dt, 200, tau)
-        if not r.accepted():
-            raise RuntimeError(f"semi-Lagrangian step rejected: {r}")
-        its.append(r.iterations)
-    elapsed = time.perf_counter() - t0
-    div = g.maxDivergence()
-    g.destroy()
-    return {"workload": f"vortex_box {size}x{size}, {steps} steps of semi-Lagrangian advection + gravity + "
-                        "projection to 1e-6 of the initial max divergence, seed 2",
-            "steps_per_s": steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
-            "pcg_iterations_mean": float(np.mean(its)), "final_max_divergence_over_tol": div / tau}
+    out = {"workload": f"vortex_box {size}x{size}, {steps} steps of semi-Lagrangian advection + gravity + "
+                       "projection, seed 2"}
+    for relative in (True, False):
+        g = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+        WL.load_into(g, wl)
+        g.setTransport(FS_TRANSPORT_SEMI_LAGRANGIAN)
+        if smoother:
+            g.setPressureSmoother(*smoother)
+        tau = 1e-6 * g.maxDivergence()
+        if relative:
+            g.setRelativeTolerance(True)
+            tol = 1e-6
+        else:
+            tol = tau
+        g.snapshotSave()
+        g.step(wl.dt, 200, tol)   # warm-up: hierarchy, graphs
+        g.snapshotRestore()
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = g.step(wl.dt, 200, tol)
+            if not r.accepted():
+                raise RuntimeError(f"semi-Lagrangian step rejected: {r}")
+            its.append(r.iterations)
+        elapsed = time.perf_counter() - t0
+        div = g.maxDivergence()
+        g.destroy()
+        entry = {"steps_per_s": steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+                 "pcg_iterations_mean": float(np.mean(its))}
+        if relative:
+            entry["stopping_rule"] = "1e-6 relative residual in every step"
+            out["relative_residual"] = entry
+        else:
+            entry["stopping_rule"] = "1e-6 of the initial max divergence"
+            entry["final_max_divergence_over_tol"] = div / tau
+            out["initial_divergence"] = entry
+    # the headline figure of the leg: C2's own rule
+    out["steps_per_s"] = out["relative_residual"]["steps_per_s"]
+    out["pcg_iterations_mean"] = out["relative_residual"]["pcg_iterations_mean"]
+    return out
 
 
 def smoother_model_bytes(coef_bytes, coarse_bytes, nu=4):
@@ -850,7 +874,7 @@ def main():
         out["oracle_error_same_smoother"] = oracle_error(gpu_state, same_state)
         out["oracle_error_same_smoother"]["against"] = "FP64 CPU oracle with the GPU's smoother, same input and tolerance"
     if args.sl_steps > 0:
-        out["semi_lagrangian"] = semi_lagrangian_leg(args.size, args.sl_steps)
+        out["semi_lagrangian"] = semi_lagrangian_leg(args.size, args.sl_steps, smoother=(int(kind), nu, coarse))
     if args.c1_steps > 0:
         out["dam_break"] = dam_break_leg(args.c1_steps)
     if args.precision_size > 0:
