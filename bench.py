@@ -533,7 +533,7 @@ def batch_leg(count, seed=4, splits=(0,)):
     fs_context_set_batch_split values timed for the projections (instances above the split run the
     multi-kernel solve beside the persistent launch); the first one is the headline batched number."""
     from fluidsim_amd import (FluidGridHIP, FluidMixture, MixtureComponent, batch_apply_viscosity, batch_project,
-                              FS_TRANSPORT_NONE)
+                              tune_batch_smoother, FS_TRANSPORT_NONE)
     from fluidsim_amd import workloads as WL
 
     t_start = time.perf_counter()
@@ -590,9 +590,31 @@ def batch_leg(count, seed=4, splits=(0,)):
         res = batch_project(grids, dt, 2000, taus)
         by_split[split] = (time.perf_counter() - t0, res)
         note(f"batched (split {split} cells) {by_split[split][0] * 1e3:.1f} ms")
-    ctx.set_batch_split(0)
     batch_s, batched = by_split[splits[0]]
     ok = all(r.converged for r in serial) and all(all(r.converged for r in v[1]) for v in by_split.values())
+    # the same projections with the red-black sweep counts fs_tune_batch_smoother picks for this batch (at
+    # the headline split), batched and one by one with that choice
+    ctx.set_batch_split(splits[0])
+    restore()
+    t0 = time.perf_counter()
+    tuned_choice = tune_batch_smoother(grids, dt, 2000, taus)
+    tune_s = time.perf_counter() - t0
+    note(f"batch tuner: red-black {tuned_choice[0]}/{tuned_choice[1]} in {tune_s:.2f} s")
+    restore()
+    t0 = time.perf_counter()
+    tuned = batch_project(grids, dt, 2000, taus)
+    tuned_s = time.perf_counter() - t0
+    note(f"batched with the tuned sweeps {tuned_s * 1e3:.1f} ms")
+    ctx.set_batch_split(0)
+    restore()
+    for g, tau in zip(grids, taus):
+        g.project(dt, 2000, tau)
+    restore()
+    t0 = time.perf_counter()
+    tuned_serial = [g.project(dt, 2000, tau) for g, tau in zip(grids, taus)]
+    tuned_serial_s = time.perf_counter() - t0
+    note(f"serial with the tuned sweeps {tuned_serial_s * 1e3:.1f} ms")
+    tuned_ok = all(r.converged for r in tuned) and all(r.converged for r in tuned_serial)
     # viscosity: every instance, one by one and then batched (warm-up of both first)
     restore()
     for g in grids:
@@ -635,6 +657,11 @@ def batch_leg(count, seed=4, splits=(0,)):
         "converged_all": bool(ok),
         "pcg_iterations": {"serial_mean": float(np.mean([r.iterations for r in serial])),
                            "batched_mean": float(np.mean([r.iterations for r in batched]))},
+        "tuned_sweeps": {"red_black_sweeps": tuned_choice[0], "coarse_sweeps": tuned_choice[1],
+                         "tuner_s": tune_s, "batched_solves_per_s": count / tuned_s,
+                         "serial_solves_per_s": count / tuned_serial_s, "speedup": tuned_serial_s / tuned_s,
+                         "converged_all": bool(tuned_ok),
+                         "pcg_iterations_batched_mean": float(np.mean([r.iterations for r in tuned]))},
         "viscosity_solves_per_s_serial": count / visc_s,
         "viscosity_solves_per_s_batched": count / visc_batch_s,
         "viscosity_speedup": visc_s / visc_batch_s,

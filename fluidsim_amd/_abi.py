@@ -221,6 +221,8 @@ SIGNATURES = {
     "fs_checkpoint_load": [_vp, C.c_char_p, C.c_size_t, _vp, C.c_size_t],
     "fs_advance_frame": [_vp, C.POINTER(FsFrameSettings), C.POINTER(FsFrameStatistics)],
     "fs_batch_project": [C.c_int, C.POINTER(_vp), C.c_double, C.c_int, _dp, C.POINTER(FsProjectionResult)],
+    "fs_tune_batch_smoother": [C.c_int, C.POINTER(_vp), C.c_double, C.c_int, _dp, C.POINTER(C.c_int),
+                               C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "fs_batch_step": [C.c_int, C.POINTER(_vp), _dp, C.POINTER(C.c_int), _dp, C.POINTER(FsProjectionResult),
                       C.POINTER(C.c_int)],
     "fs_context_set_batch_split": [_vp, C.c_longlong],

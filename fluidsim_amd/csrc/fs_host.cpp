@@ -883,11 +883,10 @@ struct fs_grid {
 
     // ---- 16-bit multigrid levels ----
     bool active16() const { return mg_prec != FS_MG_FP32 && tail_start >= 1; }
-    // The batched solve runs the reference's red-black V(4,4) on every level.
-    bool default_cycle() const
+    // The batched solve runs red-black sweeps (any count) on every level, not the Chebyshev smoother.
+    bool batch_cycle() const
     {
-        return coarse_sweeps == 4 &&
-               ((smoother == FS_SMOOTHER_RED_BLACK && smooth_sweeps == 4) || (long long)W * H <= kSmootherMinCells);
+        return smoother == FS_SMOOTHER_RED_BLACK || (long long)W * H <= kSmootherMinCells;
     }
     // Applies a smoother choice (the caller has drained the grid's streams).
     void set_smoother(fs_pressure_smoother kind, int sweeps, int coarse)
@@ -1428,6 +1427,7 @@ struct fs_grid {
             B.rx = AxisRestrict{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
             B.ry = AxisRestrict{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
             B.link = CoarseLink{Cl.sol0.p, Cl.s, F.d_pxf.p, F.d_pxs.p, F.d_pxw.p, F.d_pyf.p, F.d_pys.p, F.d_pyw.p};
+            B.sweeps = sweeps_on(l);
         }
         return b;
     }
@@ -1950,6 +1950,43 @@ fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int
     });
 }
 
+// The state a projection changes (velocities, both pressure histories, the iteration prediction), kept on
+// the device so that autotuning trials start from the same state and leave it unchanged.
+struct ProjectionState {
+    DevBuf<double> u, v, p, pp;
+    int predicted = 0;
+    static void copy(hipStream_t st, double* dst, const double* src, size_t n)
+    {
+        if (n)
+            HIPC(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    void save(fs_grid& g)
+    {
+        u.alloc(g.u.n);
+        v.alloc(g.v.n);
+        p.alloc(g.p.n);
+        pp.alloc(g.p_prev.n);
+        copy(g.st(), u.p, g.u.p, g.u.n);
+        copy(g.st(), v.p, g.v.p, g.v.n);
+        copy(g.st(), p.p, g.p.p, g.p.n);
+        copy(g.st(), pp.p, g.p_prev.p, g.p_prev.n);
+        predicted = g.predicted;
+    }
+    void restore(fs_grid& g) const
+    {
+        copy(g.st(), g.u.p, u.p, g.u.n);
+        copy(g.st(), g.v.p, v.p, g.v.n);
+        copy(g.st(), g.p.p, p.p, g.p.n);
+        copy(g.st(), g.p_prev.p, pp.p, g.p_prev.n);
+        g.predicted = predicted;
+    }
+};
+
+struct SmootherChoice {
+    fs_pressure_smoother kind;
+    int sweeps, coarse;
+};
+
 fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations, double divergence_tolerance,
     fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps, double* ms)
 {
@@ -1960,40 +1997,13 @@ fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations
         fs_grid& g = *grid;
         set_device(g.ctx);
         g.drain();
-        // the state a projection changes: velocities, both pressure histories, the iteration prediction
-        DevBuf<double> su, sv, sp, spp;
-        su.alloc(g.u.n);
-        sv.alloc(g.v.n);
-        sp.alloc(g.p.n);
-        spp.alloc(g.p_prev.n);
-        auto copy = [&](double* dst, const double* src, size_t n) {
-            if (n)
-                HIPC(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, g.st()));
-        };
-        auto save = [&] {
-            copy(su.p, g.u.p, g.u.n);
-            copy(sv.p, g.v.p, g.v.n);
-            copy(sp.p, g.p.p, g.p.n);
-            copy(spp.p, g.p_prev.p, g.p_prev.n);
-        };
-        const int predicted = g.predicted;
-        auto restore = [&] {
-            copy(g.u.p, su.p, g.u.n);
-            copy(g.v.p, sv.p, g.v.n);
-            copy(g.p.p, sp.p, g.p.n);
-            copy(g.p_prev.p, spp.p, g.p_prev.n);
-            g.predicted = predicted;
-        };
-        auto choose = [&](fs_pressure_smoother k, int n, int c) {
+        ProjectionState saved;
+        saved.save(g);
+        auto choose = [&](const SmootherChoice& c) {
             HIPC(hipStreamSynchronize(g.st()));
-            g.set_smoother(k, n, c);
+            g.set_smoother(c.kind, c.sweeps, c.coarse);
         };
-        save();
-        struct Candidate {
-            fs_pressure_smoother kind;
-            int sweeps, coarse;
-        };
-        std::vector<Candidate> candidates;
+        std::vector<SmootherChoice> candidates;
         for (int coarse : {4, 2}) {
             for (int n : {4, 3, 2})
                 candidates.push_back({FS_SMOOTHER_RED_BLACK, n, coarse});
@@ -2001,16 +2011,16 @@ fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations
                 for (int n : {3, 4, 5})
                     candidates.push_back({FS_SMOOTHER_CHEBYSHEV, n, coarse});
         }
-        const Candidate original{g.smoother, g.smooth_sweeps, g.coarse_sweeps};
-        Candidate best = original;
+        const SmootherChoice original{g.smoother, g.smooth_sweeps, g.coarse_sweeps};
+        SmootherChoice best = original;
         double best_ms = std::numeric_limits<double>::infinity();
         try {
-            for (const Candidate& c : candidates) {
-                choose(c.kind, c.sweeps, c.coarse);
+            for (const SmootherChoice& c : candidates) {
+                choose(c);
                 double fastest = std::numeric_limits<double>::infinity();
                 bool converged = true;
                 for (int rep = 0; rep < 3; ++rep) {   // the first is a warm-up (graph capture, first replays)
-                    restore();
+                    saved.restore(g);
                     HIPC(hipStreamSynchronize(g.st()));
                     const auto t0 = std::chrono::steady_clock::now();
                     const fs_projection_result res = g.project(dt, max_iterations, divergence_tolerance);
@@ -2025,13 +2035,13 @@ fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations
                 }
             }
         } catch (...) {
-            choose(original.kind, original.sweeps, original.coarse);
-            restore();
+            choose(original);
+            saved.restore(g);
             HIPC(hipStreamSynchronize(g.st()));
             throw;
         }
-        choose(best.kind, best.sweeps, best.coarse);
-        restore();
+        choose(best);
+        saved.restore(g);
         HIPC(hipStreamSynchronize(g.st()));
         if (kind)
             *kind = best.kind;
@@ -3315,8 +3325,8 @@ fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_
         for (int i = 0; i < count; ++i) {
             need(!grids[i]->active16(), FS_E_UNSUPPORTED,
                 "batched solves run FP32 multigrid levels: set the grid's multigrid precision to FS_MG_FP32");
-            need(grids[i]->default_cycle(), FS_E_UNSUPPORTED,
-                "batched solves run the red-black V(4,4) cycle: set the grid's smoother to FS_SMOOTHER_RED_BLACK, 4");
+            need(grids[i]->batch_cycle(), FS_E_UNSUPPORTED,
+                "batched solves run red-black smoothing: set the grid's smoother to FS_SMOOTHER_RED_BLACK");
         }
         for (int i = 0; i < count; ++i) {
             results[i] = fs_projection_result{};
@@ -3427,7 +3437,7 @@ fs_status fs_batch_step(int count, fs_grid* const* grids, const double* dts, con
             // the projections: FP32-level grids with the default cycle in one batched solve, the others one by one
             std::vector<int> batched, alone;
             for (int i : active)
-                (grids[i]->active16() || !grids[i]->default_cycle() ? alone : batched).push_back(i);
+                (grids[i]->active16() || !grids[i]->batch_cycle() ? alone : batched).push_back(i);
             batch_projection(ctx, grids, batched, step.data(), max_iterations, divergence_tolerances, proj.data(), errors);
             for (int i : alone)
                 attempt(i, [&] {
@@ -3463,6 +3473,88 @@ fs_status fs_batch_step(int count, fs_grid* const* grids, const double* dts, con
         }
         if (!first.empty())
             fail(FS_E_RUNTIME, first);
+    });
+}
+
+fs_status fs_tune_batch_smoother(int count, fs_grid* const* grids, double dt, int max_iterations,
+    const double* divergence_tolerances, int* sweeps, int* coarse_sweeps, double* ms)
+{
+    return guard([&] {
+        need(count > 0 && grids && divergence_tolerances, FS_E_INVALID_ARGUMENT, "null pointer or empty batch");
+        need(std::isfinite(dt) && dt > 0.0, FS_E_INVALID_ARGUMENT, "dt must be finite and positive");
+        fs_context* ctx = batch_context(count, grids, "Upload the grid state before tuning");
+        for (int i = 0; i < count; ++i)
+            need(!grids[i]->active16(), FS_E_UNSUPPORTED,
+                "batched solves run FP32 multigrid levels: set the grid's multigrid precision to FS_MG_FP32");
+        set_device(ctx);
+        const size_t n = (size_t)count;
+        std::vector<ProjectionState> saved(n);
+        std::vector<SmootherChoice> original(n);
+        for (size_t i = 0; i < n; ++i) {
+            fs_grid& g = *grids[i];
+            g.drain();
+            saved[i].save(g);
+            original[i] = {g.smoother, g.smooth_sweeps, g.coarse_sweeps};
+        }
+        HIPC(hipStreamSynchronize(ctx->stream));
+        auto choose_all = [&](const SmootherChoice& c) {
+            HIPC(hipStreamSynchronize(ctx->stream));
+            for (size_t i = 0; i < n; ++i)
+                grids[i]->set_smoother(c.kind, c.sweeps, c.coarse);
+        };
+        auto restore_all = [&] {
+            for (size_t i = 0; i < n; ++i)
+                saved[i].restore(*grids[i]);
+            HIPC(hipStreamSynchronize(ctx->stream));
+        };
+        std::vector<int> all(n);
+        for (size_t i = 0; i < n; ++i)
+            all[i] = (int)i;
+        const std::vector<double> dts(n, dt);
+        const std::vector<int> budgets(n, max_iterations);
+        std::vector<fs_projection_result> results(n);
+        std::vector<std::string> errors(n);
+        SmootherChoice best{FS_SMOOTHER_RED_BLACK, 4, 4};
+        double best_ms = std::numeric_limits<double>::infinity();
+        try {
+            for (int coarse : {4, 2})
+                for (int nu : {4, 3, 2}) {
+                    const SmootherChoice c{FS_SMOOTHER_RED_BLACK, nu, coarse};
+                    choose_all(c);
+                    double fastest = std::numeric_limits<double>::infinity();
+                    bool ok = true;
+                    for (int rep = 0; rep < 2; ++rep) {   // the first is a warm-up
+                        restore_all();
+                        const auto t0 = std::chrono::steady_clock::now();
+                        batch_projection(ctx, grids, all, dts.data(), budgets.data(), divergence_tolerances,
+                            results.data(), errors);
+                        const double t =
+                            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                        for (size_t i = 0; i < n; ++i)
+                            ok = ok && errors[i].empty() && results[i].converged;
+                        if (rep > 0)
+                            fastest = t;
+                    }
+                    if (ok && fastest < best_ms) {
+                        best_ms = fastest;
+                        best = c;
+                    }
+                }
+        } catch (...) {
+            HIPC(hipStreamSynchronize(ctx->stream));
+            for (size_t i = 0; i < n; ++i)
+                grids[i]->set_smoother(original[i].kind, original[i].sweeps, original[i].coarse);
+            restore_all();
+            throw;
+        }
+        choose_all(best);
+        restore_all();
+        if (sweeps)
+            *sweeps = best.sweeps;
+        if (coarse_sweeps)
+            *coarse_sweeps = best.coarse;
+        if (ms)
+            *ms = best_ms;
     });
 }
 

@@ -226,6 +226,7 @@ struct BatchLevel {
     LevelView C;        // level l + 1 (its rhs receives the restriction, its sol[0] the coarse correction)
     AxisRestrict rx, ry;  // restriction l -> l + 1
     CoarseLink link;    // prolongation l + 1 -> l
+    int sweeps;         // red-black sweeps per side on this level (1..4)
 };
 struct BatchInstance {
     GridDev g;

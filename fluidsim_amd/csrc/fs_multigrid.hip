@@ -1222,10 +1222,10 @@ __device__ __forceinline__ double wg_max(double v, double* slot)
     return r;
 }
 
-// Eight red-black half-sweeps of one level in place on `sol` (k_rb_multi's update): colours from
+// 2 * sweeps red-black half-sweeps of one level in place on `sol` (k_rb_multi's update): colours from
 // `first`, alternating; the first half-sweep of a pre-smoothing ignores the (zero) neighbours.
 __device__ void wg_sweeps(const LevelView& L, float* sol, const float* rhs, const double* shift, int first,
-    bool from_zero)
+    bool from_zero, int sweeps)
 {
     const int w = L.w, h = L.h, s = L.s;
     const int pairs = (w + 1) >> 1;
@@ -1233,7 +1233,7 @@ __device__ void wg_sweeps(const LevelView& L, float* sol, const float* rhs, cons
     const double sh = shift ? *shift : 0.0;
     const float inv_pairs = 1.0f / (float)pairs;
     const bool fast = items < (1 << 22);
-    for (int k = 0; k < 2 * kMultiNS; ++k) {
+    for (int k = 0; k < 2 * sweeps; ++k) {
         const int color = first ^ (k & 1);
         const bool neighbours = !(from_zero && k == 0);
         // kBU items per thread at a time: every load of the group issues before the first store (the
@@ -1519,7 +1519,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
             for (int l = 0; l < I.levels; ++l) {
                 const BatchLevel& B = I.lev[l];
                 const double* shift = l == 0 ? &S.r_mean : nullptr;
-                wg_sweeps(B.L, B.L.sol[1], B.L.rhs, shift, 0, true);
+                wg_sweeps(B.L, B.L.sol[1], B.L.rhs, shift, 0, true, B.sweeps);
                 wg_res_restrict(B.L, B.L.rhs, shift, B.C, B.rx, B.ry);
             }
             tail_run<float>(*I.tail, I.levels == 0 ? &S.r_mean : nullptr, nullptr, lds);
@@ -1527,7 +1527,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pcg(const BatchInstance* batch)
             for (int l = I.levels - 1; l >= 0; --l) {
                 const BatchLevel& B = I.lev[l];
                 wg_prolong(B.L, B.link);
-                wg_sweeps(B.L, B.L.sol[0], B.L.rhs, l == 0 ? &S.r_mean : nullptr, 1, false);
+                wg_sweeps(B.L, B.L.sol[0], B.L.rhs, l == 0 ? &S.r_mean : nullptr, 1, false, B.sweeps);
             }
         } else {
             const double mean = S.r_mean;

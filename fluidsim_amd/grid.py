@@ -266,6 +266,20 @@ def batch_project(grids, dt: float, max_iterations: int = 200, divergence_tolera
     return [ProjectionResult.from_c(res[i]) for i in range(n)]
 
 
+def tune_batch_smoother(grids, dt: float, max_iterations: int = 200, divergence_tolerance=1e-6):
+    """fs_tune_batch_smoother: time fs_batch_project on the grids with every grid at red-black (sweeps,
+    coarse) for sweeps 4, 3, 2 and coarse 4, 2, from their current state (unchanged on return), and set
+    every grid to the fastest choice under which all converge.  Returns (sweeps, coarse_sweeps, ms)."""
+    grids = list(grids)
+    n = len(grids)
+    tol = np.broadcast_to(np.asarray(divergence_tolerance, dtype=np.float64), (n,)).copy()
+    handles = (C.c_void_p * max(n, 1))(*[g.handle.value for g in grids])
+    sweeps, coarse, ms = C.c_int(), C.c_int(), C.c_double()
+    call("fs_tune_batch_smoother", n, handles, float(dt), int(max_iterations), _dp(tol), C.byref(sweeps),
+         C.byref(coarse), C.byref(ms))
+    return sweeps.value, coarse.value, ms.value
+
+
 def batch_step(grids, dt, max_iterations=200, divergence_tolerance=1e-6, raise_on_error=True):
     """fs_batch_step: fs_step on every grid (all on one context) at once, each with its own dt, budget and
     tolerance (one value or one per grid); viscosity solves and projections run batched.  Returns

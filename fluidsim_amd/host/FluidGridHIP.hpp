@@ -619,6 +619,23 @@ inline std::vector<FluidProjectionResult> batchProject(const std::vector<FluidGr
     return out;
 }
 
+// Red-black sweep counts for batched projections (fs_tune_batch_smoother; no reference counterpart): sets
+// every grid to the fastest converging (sweeps, coarse sweeps); returns them with the time per batch.
+inline std::tuple<int, int, double> tuneBatchSmoother(const std::vector<FluidGridHIP*>& grids, double dt,
+    int max_iterations, const std::vector<double>& divergence_tolerances)
+{
+    if (divergence_tolerances.size() != grids.size())
+        throw std::invalid_argument("one divergence tolerance per grid");
+    std::vector<fs_grid*> handles;
+    for (const FluidGridHIP* g : grids)
+        handles.push_back(g ? g->handle() : nullptr);
+    int sweeps = 0, coarse = 0;
+    double ms = 0.0;
+    check(fs_tune_batch_smoother((int)grids.size(), handles.data(), dt, max_iterations,
+        divergence_tolerances.data(), &sweeps, &coarse, &ms));
+    return {sweeps, coarse, ms};
+}
+
 // Batched implicit viscosity (fs_batch_apply_viscosity; no reference counterpart): each grid's
 // applyViscosity(dt); returns the iteration counts.
 inline std::vector<int> batchApplyViscosity(const std::vector<FluidGridHIP*>& grids, double dt)

@@ -200,7 +200,9 @@ fs_status fs_set_relative_tolerance(fs_grid* grid, int enabled);
 /* Pressure smoother choice (see fs_pressure_smoother): `kind` with `sweeps` per side on the levels above 4096
  * cells, red-black with `coarse_sweeps` (1..4) per side on the smaller ones.  (FS_SMOOTHER_RED_BLACK, 4, 4)
  * is the reference's V(4,4).  FS_E_UNSUPPORTED for Chebyshev on a grid with 16-bit multigrid levels.
- * Batched solves (fs_batch_project) need the default. */
+ * Batched solves (fs_batch_project) run red-black smoothing with the grid's sweep counts: a grid whose
+ * levels above 4096 cells run Chebyshev projects on its own in fs_batch_step and is refused by
+ * fs_batch_project. */
 fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps, int coarse_sweeps);
 fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps);
 /* Autotuner of the smoother and its sweep counts (no reference counterpart): projects the grid's current
@@ -371,6 +373,14 @@ fs_status fs_batch_apply_viscosity(int count, fs_grid* const* grids, double dt, 
  * rounding, not bit for bit.  tolerances[i] is grid i's divergence tolerance. */
 fs_status fs_batch_project(int count, fs_grid* const* grids, double dt, int max_iterations,
     const double* divergence_tolerances, fs_projection_result* results);
+/* Autotuner of the batched solves' red-black sweep counts (no reference counterpart): runs fs_batch_project
+ * on the `count` grids with every grid set to (FS_SMOOTHER_RED_BLACK, sweeps, coarse) for sweeps 4, 3, 2
+ * and coarse 4, 2 -- each from the grids' current state, restored after every trial: their state is
+ * unchanged on return -- times each batch after one warm-up, and sets every grid to the fastest choice
+ * under which every grid's projection converges.  sweeps, coarse_sweeps and ms (each may be NULL) receive
+ * the choice and its time per batch.  FS_E_UNSUPPORTED for grids with 16-bit multigrid levels. */
+fs_status fs_tune_batch_smoother(int count, fs_grid* const* grids, double dt, int max_iterations,
+    const double* divergence_tolerances, int* sweeps, int* coarse_sweeps, double* ms);
 /* Split of fs_batch_project and fs_batch_apply_viscosity (no reference counterpart): grids of more than
  * `solo_cells` cells are solved by fs_project's / fs_apply_viscosity's own multi-kernel solve, one after
  * another on the context's stream, while the persistent launch runs the smaller ones on a second stream

@@ -9,8 +9,12 @@ FP64 oracle with the same choice.
     1e-4 of the velocity scale (iteration counts within a few on the pool, up to 2x on the masked tank,
     where FP32 CG loses conjugacy);
   - the tuner leaves the grid's state unchanged and selects a candidate that converges;
-  - batched solves keep the reference's cycle: fs_batch_project refuses another choice and fs_batch_step
-    projects such a grid on its own, bit for bit as fs_step does; 16-bit levels refuse Chebyshev.
+  - batched solves run red-black smoothing with the grid's sweep counts: fs_batch_project refuses a
+    Chebyshev grid and fs_batch_step projects such a grid on its own, bit for bit as fs_step does; a batched
+    grid with fewer red-black sweeps matches fs_project with the same choice (iterations within one,
+    velocities to FP32 rounding of the solve); 16-bit levels refuse Chebyshev;
+  - the batch tuner (fs_tune_batch_smoother) leaves every grid's state unchanged, sets every grid to one
+    converging red-black choice, and batched projections with it converge.
 """
 import math
 
@@ -18,7 +22,7 @@ import numpy as np
 import pytest
 
 from fluidsim_amd import (FS_MG_BF16, FS_MG_FP32, FS_SMOOTHER_CHEBYSHEV, FS_SMOOTHER_RED_BLACK, FluidGridHIP,
-                          batch_project, batch_step)
+                          batch_project, batch_step, tune_batch_smoother)
 from fluidsim_amd import workloads as WL
 
 pytestmark = pytest.mark.gpu
@@ -154,7 +158,7 @@ def test_tuner_keeps_the_state_and_picks_a_converging_choice():
     g.destroy()
 
 
-def test_batched_solves_keep_the_reference_cycle():
+def test_batched_solves_run_red_black_smoothing():
     wl = WL.sloped_pool(160, 120, 4.0)
     a, b = hip_grid(wl), hip_grid(wl)
     a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 4)
@@ -179,9 +183,46 @@ def test_batched_solves_keep_the_reference_cycle():
         a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 6)
     with pytest.raises(ValueError):
         a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 4, 0)
-    # fewer coarse sweeps alone also leave the reference's cycle
-    a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 4, 2)
-    with pytest.raises(NotImplementedError):
-        batch_project([a], 0.002, 500, 1e-6)
     a.destroy()
     b.destroy()
+
+
+@pytest.mark.parametrize("sweeps,coarse", [(3, 2), (2, 4), (4, 1)])
+def test_batched_red_black_sweeps_match_fs_project(sweeps, coarse):
+    wl = WL.vortex_box(192, 160, seed=5)
+    grids = [hip_grid(wl) for _ in range(3)]
+    for g in grids:
+        g.setPressureSmoother(FS_SMOOTHER_RED_BLACK, sweeps, coarse)
+    tau = 1e-6 * grids[0].maxDivergence()
+    ra = batch_project(grids[:2], wl.dt, 300, tau)
+    rb = grids[2].project(wl.dt, 300, tau)
+    print(f"red-black {sweeps}/{coarse}: batched {[r.iterations for r in ra]} fs_project {rb.iterations}")
+    assert all(r.converged for r in ra) and rb.converged
+    assert ra[0].iterations == ra[1].iterations and abs(ra[0].iterations - rb.iterations) <= 1
+    s0, s1, s2 = (g.download() for g in grids)
+    assert np.array_equal(s0.u, s1.u) and np.array_equal(s0.v, s1.v)
+    scale = max(np.abs(s2.u).max(), np.abs(s2.v).max())
+    assert np.abs(s0.u - s2.u).max() < 1e-5 * scale and np.abs(s0.v - s2.v).max() < 1e-5 * scale
+    for g in grids:
+        g.destroy()
+
+
+def test_batch_tuner_keeps_the_state_and_picks_a_converging_choice():
+    wls = [WL.vortex_box(256, 256, seed=s) for s in (2, 3)] + [WL.sloped_pool(192, 128, 4.0)]
+    grids = [hip_grid(wl) for wl in wls]
+    before = [g.download() for g in grids]
+    tau = [1e-6 * g.maxDivergence() for g in grids]
+    sweeps, coarse, ms = tune_batch_smoother(grids, wls[0].dt, 300, tau)
+    print(f"batch tuned: red-black {sweeps} sweeps, {coarse} coarse sweeps, {ms:.3f} ms per batch")
+    assert sweeps in (2, 3, 4) and coarse in (2, 4) and ms > 0
+    for g, b in zip(grids, before):
+        assert g.getPressureSmoother() == (FS_SMOOTHER_RED_BLACK, sweeps, coarse)
+        after = g.download()
+        for key in ("liquid_fraction", "u", "v", "pressure"):
+            assert np.array_equal(getattr(after, key), getattr(b, key)), key
+    assert all(r.converged for r in batch_project(grids, wls[0].dt, 300, tau))
+    grids[0].setMultigridPrecision(FS_MG_BF16)
+    with pytest.raises(NotImplementedError):
+        tune_batch_smoother(grids, wls[0].dt, 300, tau)
+    for g in grids:
+        g.destroy()
