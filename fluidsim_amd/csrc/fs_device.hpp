@@ -104,6 +104,57 @@ struct AxisRestrict {
 
 __device__ __forceinline__ int pidx(int x, int y, int s) { return (y + 1) * s + (x + 1); }
 
+// PressureMultigrid::coarsen (PressureMultigrid.cpp:142-202) for coarse cell (x, y): parallel paths add,
+// series resistances combine across the two or three fine faces between coarse centres.  Writes the
+// coarse right face (x + 1 < cw) and down face (y + 1 < ch) of the cell.
+__device__ __forceinline__ void coarsen_cell(int x, int y, int fw, int fh, const double* fcx, const double* fcy,
+    const double* fright, const double* fdown, int cw, int ch, const double* ccx, const double* ccy,
+    double* cright, double* cdown)
+{
+    const int fs = fw + 2, cs = cw + 2;
+    if (x + 1 < cw) {
+        const int edge = 2 * x + 2;
+        const double left = (fcx[edge - 1] - ccx[x]) / (fcx[edge - 1] - fcx[edge - 2]);
+        const double right = edge + 1 < fw ? (ccx[x + 1] - fcx[edge]) / (fcx[edge + 1] - fcx[edge]) : 0.0;
+        double conductance = 0.0;
+        const int fy_end = min(2 * y + 2, fh);
+        for (int fy = 2 * y; fy < fy_end; ++fy) {
+            const int i = (fy + 1) * fs + edge;
+            double resistance = left / fright[i - 1] + 1.0 / fright[i];
+            if (right != 0.0)
+                resistance += right / fright[i + 1];
+            conductance += 1.0 / resistance;
+        }
+        cright[(y + 1) * cs + x + 1] = conductance;
+    }
+    if (y + 1 < ch) {
+        const int edge = 2 * y + 2;
+        const double up = (fcy[edge - 1] - ccy[y]) / (fcy[edge - 1] - fcy[edge - 2]);
+        const double down = edge + 1 < fh ? (ccy[y + 1] - fcy[edge]) / (fcy[edge + 1] - fcy[edge]) : 0.0;
+        double conductance = 0.0;
+        const int fx_end = min(2 * x + 2, fw);
+        for (int fx = 2 * x; fx < fx_end; ++fx) {
+            const int i = edge * fs + fx + 1;
+            double resistance = up / fdown[i - fs] + 1.0 / fdown[i];
+            if (down != 0.0)
+                resistance += down / fdown[i + fs];
+            conductance += 1.0 / resistance;
+        }
+        cdown[(y + 1) * cs + x + 1] = conductance;
+    }
+}
+
+// Level::rebuildDiagonal (PressureMultigrid.cpp:104-117) of one cell from its four FP64 face
+// conductances, and the FP32 copies the smoothers read.
+__device__ __forceinline__ void level_diag_values(double rl, double rr, double du, double dd, float& invd32,
+    float& right32, float& down32)
+{
+    const double diagonal = rl + rr + du + dd;
+    invd32 = (float)(diagonal > 0.0 ? 1.0 / diagonal : 0.0);
+    right32 = (float)rr;
+    down32 = (float)dd;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

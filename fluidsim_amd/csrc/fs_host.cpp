@@ -379,6 +379,7 @@ struct fs_context {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;          // second stream (default priority): hierarchy rebuild overlap
     hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t join_levels = nullptr;   // the rebuild's tiled levels are done (the coarse chain may still run)
     PcgScalars* pinned = nullptr;   // host-pinned mirror of a grid's scalar block
     double* pinned_value = nullptr; // host-pinned landing slot of single device reductions
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -398,6 +399,8 @@ constexpr size_t kFoldBlocks = 1536;      // cell blocks (64 x 16) beyond which 
 // Multigrid levels above this many cells take the grid's chosen smoother (fs_set_pressure_smoother); smaller
 // ones, the single-workgroup tail's levels among them, always run red-black V(4,4).
 constexpr long long kSmootherMinCells = 4096;
+// Levels of at most this many cells are coarsened by the hierarchy rebuild's single-workgroup launch.
+constexpr long long kChainCells = 4096;
 
 // Weight i of `sweeps` Chebyshev-weighted Jacobi sweeps: PressureMultigridGPU's constructor
 // (PressureMultigridGPU.cpp:24-28), 1 / (1.1 + 0.9 cos(pi (i + 1/2) / sweeps)) -- the reciprocals of the
@@ -863,9 +866,20 @@ struct fs_grid {
 
     // PressureMultigrid::rebuild (PressureMultigrid.cpp:119-140) on the device, FP64, on `stream`; a
     // failed coarse factorisation is flagged in `flag`.
-    void rebuild_hierarchy(hipStream_t stream, PcgScalars* flag)
+    // The levels of more than kChainCells cells are built one launch pair each; the smaller ones (at most
+    // kMaxTailLevels of them) and the coarse inverse in one single-workgroup launch, whose phases are
+    // too small to fill the device -- the same values either way.
+    // `levels_done` (optional) is recorded on `stream` once the levels the tiled V-cycle kernels read are
+    // built and packed; returns true when the coarse chain (tail levels and coarse inverse) is launched
+    // after it, i.e. only the tail launch has to wait for the rest.
+    bool rebuild_hierarchy(hipStream_t stream, PcgScalars* flag, hipEvent_t levels_done = nullptr)
     {
-        for (size_t l = 0; l + 1 < levels.size(); ++l) {
+        const size_t nlev = levels.size();
+        size_t chain_from = nlev;   // first level built by the chain launch
+        while (chain_from > 1 && nlev - (chain_from - 1) <= (size_t)kMaxTailLevels &&
+               (long long)levels[chain_from - 1].w * levels[chain_from - 1].h <= kChainCells)
+            --chain_from;
+        for (size_t l = 0; l + 1 < chain_from; ++l) {
             Level& F = levels[l];
             Level& Cl = levels[l + 1];
             launch_coarsen(stream, F.w, F.h, F.d_cx.p, F.d_cy.p, F.right64.p, F.down64.p, Cl.w, Cl.h, Cl.d_cx.p,
@@ -873,11 +887,31 @@ struct fs_grid {
             launch_level_diag(stream, Cl.w, Cl.h, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p,
                 Cl.invd32.p);
         }
-        pack_levels(stream);
         const Level& last = levels.back();
-        launch_coarse_inverse(stream, last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, flag,
-            solid.n > 0);
+        // the chain builds tail levels only: pack the tiled levels and let the V-cycle start
+        const bool early = levels_done && chain_from < nlev && (int)chain_from >= tail_start;
+        if (early) {
+            pack_levels(stream);
+            HIPC(hipEventRecord(levels_done, stream));
+        }
+        if (chain_from < nlev) {
+            ChainDesc chain{};
+            for (size_t l = chain_from; l < nlev; ++l) {
+                const Level& F = levels[l - 1];
+                Level& Cl = levels[l];
+                chain.lev[chain.levels++] = ChainLevel{F.w, F.h, Cl.w, Cl.h, F.d_cx.p, F.d_cy.p, F.right64.p,
+                    F.down64.p, Cl.d_cx.p, Cl.d_cy.p, Cl.right64.p, Cl.down64.p, Cl.right32.p, Cl.down32.p,
+                    Cl.invd32.p};
+            }
+            launch_coarse_chain(stream, chain, ginv.p, coarse_n, flag, solid.n > 0);
+        } else {
+            launch_coarse_inverse(stream, last.w, last.h, last.right64.p, last.down64.p, ginv.p, coarse_n, flag,
+                solid.n > 0);
+        }
+        if (!early)
+            pack_levels(stream);
         mg_valid = true;
+        return early;
     }
     void rebuild_hierarchy() { rebuild_hierarchy(st(), S.p); }
 
@@ -1020,6 +1054,7 @@ struct fs_grid {
             smooth16<C, VL>(l, 0, 0, 1, status, levels[l].smooth_variant);
             restrict16<C, VL>(l, 1, status, levels[l].restrict_variant);
         }
+        join_before_tail();
         launch_mg_tail(st(), tail, tail_lds, nullptr, status, std::is_same<VL, bf16>::value);
         for (int l = T - 1; l >= 0; --l)
             smooth16<C, VL>(l, 1, 1, 0, status, levels[l].post_variant);
@@ -1110,6 +1145,7 @@ struct fs_grid {
             AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
             launch_res_restrict(st(), L, 1, rhs, shift, view(l + 1), rx, ry, status, F.restrict_variant);
         }
+        join_before_tail();
         launch_mg_tail(st(), tail, tail_lds, T == 0 ? rmean_ptr() : nullptr, status);
         for (int l = T - 1; l >= 0; --l) {
             LevelView L = view(l);
@@ -1257,6 +1293,18 @@ struct fs_grid {
         return flexible_cg || (precond == FS_PRECOND_MULTIGRID && mg_prec == FS_MG_BF16_VECTORS && active16());
     }
 
+    // A rebuild whose coarse chain still runs on the helper stream: the first tail launch waits for it
+    // and takes over its failure flag (the kernels before it read only the tiled levels).
+    hipEvent_t tail_join = nullptr;
+    void join_before_tail()
+    {
+        if (!tail_join)
+            return;
+        HIPC(hipStreamWaitEvent(st(), tail_join, 0));
+        launch_merge_failure(st(), S_aux.p, S.p);
+        tail_join = nullptr;
+    }
+
     void precondition(const int* status)
     {
         if (precond == FS_PRECOND_MULTIGRID)
@@ -1315,13 +1363,14 @@ struct fs_grid {
         const bool rebuild = precond == FS_PRECOND_MULTIGRID && !mg_valid;
         // A context without the helper stream rebuilds in line.
         hipStream_t side = ctx->aux ? ctx->aux : st();
+        bool late_tail = false;   // the first V-cycle's tail waits for the coarse chain
         if (rebuild) {
             if (ctx->aux) {
                 HIPC(hipEventRecord(ctx->fork, st()));
                 HIPC(hipStreamWaitEvent(side, ctx->fork, 0));
             }
             launch_init_scalars(side, S_aux.p, 0.0, 0);
-            rebuild_hierarchy(side, S_aux.p);
+            late_tail = rebuild_hierarchy(side, S_aux.p, ctx->aux ? ctx->join_levels : nullptr);
             if (ctx->aux)
                 HIPC(hipEventRecord(ctx->join, side));
         }
@@ -1332,11 +1381,17 @@ struct fs_grid {
         launch_sub_mean_p(st(), g, 0);
         launch_true_residual(st(), g, 0);
         if (rebuild) {
-            if (ctx->aux)
-                HIPC(hipStreamWaitEvent(st(), ctx->join, 0));
-            launch_merge_failure(st(), S_aux.p, S.p);
+            if (late_tail) {
+                HIPC(hipStreamWaitEvent(st(), ctx->join_levels, 0));
+                tail_join = ctx->join;   // consumed by the tail launch of the V-cycle below
+            } else {
+                if (ctx->aux)
+                    HIPC(hipStreamWaitEvent(st(), ctx->join, 0));
+                launch_merge_failure(st(), S_aux.p, S.p);
+            }
         }
         precondition(status_ptr());
+        tail_join = nullptr;
         launch_pcg_rz(st(), g, z_ptr(), flexible());
         launch_pcg_direction(st(), g, z_ptr(), flexible());
     }
@@ -1702,6 +1757,7 @@ fs_status fs_context_create_ex(int device, int flags, fs_context** out)
             HIPC(hipStreamCreate(&ctx->aux));
             HIPC(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
             HIPC(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&ctx->join_levels, hipEventDisableTiming));
         }
         HIPC(hipHostMalloc((void**)&ctx->pinned, sizeof(PcgScalars), hipHostMallocDefault));
         HIPC(hipHostMalloc((void**)&ctx->pinned_value, sizeof(double), hipHostMallocDefault));
@@ -1750,6 +1806,8 @@ fs_status fs_context_destroy(fs_context* ctx)
                 (void)hipEventDestroy(ctx->fork);
             if (ctx->join)
                 (void)hipEventDestroy(ctx->join);
+            if (ctx->join_levels)
+                (void)hipEventDestroy(ctx->join_levels);
         }
         // outside the lock: the descriptor buffer's release takes it
         delete ctx;

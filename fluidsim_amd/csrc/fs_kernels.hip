@@ -161,55 +161,21 @@ __global__ __launch_bounds__(kNT) void k_level_diag(int w, int h, const double* 
     for (int k = 0; k < 4; ++k) {
         if (T.in[k]) {
             const int i = T.P[k];
-            const double diagonal = rl[k] + rr[k] + du[k] + dd[k];
-            invd32[i] = (float)(diagonal > 0.0 ? 1.0 / diagonal : 0.0);
-            right32[i] = (float)rr[k];
-            down32[i] = (float)dd[k];
+            level_diag_values(rl[k], rr[k], du[k], dd[k], invd32[i], right32[i], down32[i]);
         }
     }
 }
 
-// PressureMultigrid::coarsen (PressureMultigrid.cpp:142-202): parallel paths add, series resistances
-// combine across the two or three fine faces between coarse centres.  One thread per coarse cell.
+// PressureMultigrid::coarsen (PressureMultigrid.cpp:142-202), one thread per coarse cell.
 __global__ __launch_bounds__(kNT) void k_coarsen(int fw, int fh, const double* fcx, const double* fcy,
     const double* fright, const double* fdown, int cw, int ch, const double* ccx, const double* ccy,
     double* cright, double* cdown)
 {
-    const int fs = fw + 2, cs = cw + 2;
     const int t = blockIdx.x * kNT + threadIdx.x;
     if (t >= cw * ch)
         return;
     const int y = t / cw, x = t - y * cw;
-    if (x + 1 < cw) {
-        const int edge = 2 * x + 2;
-        const double left = (fcx[edge - 1] - ccx[x]) / (fcx[edge - 1] - fcx[edge - 2]);
-        const double right = edge + 1 < fw ? (ccx[x + 1] - fcx[edge]) / (fcx[edge + 1] - fcx[edge]) : 0.0;
-        double conductance = 0.0;
-        const int fy_end = min(2 * y + 2, fh);
-        for (int fy = 2 * y; fy < fy_end; ++fy) {
-            const int i = (fy + 1) * fs + edge;
-            double resistance = left / fright[i - 1] + 1.0 / fright[i];
-            if (right != 0.0)
-                resistance += right / fright[i + 1];
-            conductance += 1.0 / resistance;
-        }
-        cright[(y + 1) * cs + x + 1] = conductance;
-    }
-    if (y + 1 < ch) {
-        const int edge = 2 * y + 2;
-        const double up = (fcy[edge - 1] - ccy[y]) / (fcy[edge - 1] - fcy[edge - 2]);
-        const double down = edge + 1 < fh ? (ccy[y + 1] - fcy[edge]) / (fcy[edge + 1] - fcy[edge]) : 0.0;
-        double conductance = 0.0;
-        const int fx_end = min(2 * x + 2, fw);
-        for (int fx = 2 * x; fx < fx_end; ++fx) {
-            const int i = edge * fs + fx + 1;
-            double resistance = up / fdown[i - fs] + 1.0 / fdown[i];
-            if (down != 0.0)
-                resistance += down / fdown[i + fs];
-            conductance += 1.0 / resistance;
-        }
-        cdown[(y + 1) * cs + x + 1] = conductance;
-    }
+    coarsen_cell(x, y, fw, fh, fcx, fcy, fright, fdown, cw, ch, ccx, ccy, cright, cdown);
 }
 
 // ------------------------------------------------------------------------------------------

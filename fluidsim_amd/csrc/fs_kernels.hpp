@@ -112,6 +112,21 @@ void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, c
 // FP32 inverse of the pinned coarse matrix by the in-register sweep operator (the projection's path).
 void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, const double* down64, float* ginv,
     int n, PcgScalars* S, int pin);
+// The coarse end of the hierarchy rebuild in one single-workgroup launch: for each listed level, the
+// coarsening from its finer level and its diagonal (the values of launch_coarsen + launch_level_diag),
+// then the inverse of the last level's pinned matrix (the values of launch_coarse_inverse).
+struct ChainLevel {
+    int fw, fh, cw, ch;
+    const double *fcx, *fcy, *fright, *fdown;   // finer level: centres and FP64 conductances
+    const double *ccx, *ccy;                    // this level's centres
+    double *cright, *cdown;                     // this level's FP64 conductances (written)
+    float *right32, *down32, *invd32;           // and their FP32 copies and 1/diagonal (written)
+};
+struct ChainDesc {
+    int levels;
+    ChainLevel lev[kMaxTailLevels];
+};
+void launch_coarse_chain(hipStream_t st, const ChainDesc& chain, float* ginv, int n, PcgScalars* S, int pin);
 
 // ---- projection set-up and finish (FP64) ----
 void launch_rhs(hipStream_t st, const GridDev& g, double scale);

@@ -655,13 +655,14 @@ __global__ __launch_bounds__(kFactorThreads) void k_coarse_factor(int w, int h, 
 // ---------------------------------------------------------------------------------------------
 // Inverse of the pinned coarse matrix for the per-cycle coarse solve (the hot path).  Goodnight's
 // sweep operator, in place, one pivot per step: the matrix lives in registers, column j in lane j
-// and rows split over the four wavefronts (16 per wavefront).  A step broadcasts the pivot row
-// through LDS (double-buffered, one barrier) and the pivot column inside each wavefront by lane
-// reads, so the 63 steps cost 63 barriers and no LDS matrix traffic.  Pivots are the Schur
-// complement diagonals (the squared Cholesky pivots); a non-positive or non-finite one marks the
-// hierarchy failed exactly where the Cholesky factorisation would.
+// and rows split over sixteen wavefronts (4 per wavefront, so that a step's row updates are a short
+// chain).  A step broadcasts the pivot row through LDS (double-buffered, one barrier) and the pivot
+// column inside each wavefront by lane reads, so the 63 steps cost 63 barriers and no LDS matrix
+// traffic.  Pivots are the Schur complement diagonals (the squared Cholesky pivots); a non-positive or
+// non-finite one marks the hierarchy failed exactly where the Cholesky factorisation would.
 // ---------------------------------------------------------------------------------------------
-constexpr int kSweepRows = 16;   // rows per wavefront (4 x 16 >= kMaxCoarse)
+constexpr int kSweepRows = 4;     // rows per wavefront
+constexpr int kSweepWaves = 16;   // kSweepWaves x kSweepRows >= kMaxCoarse
 
 // v as held by lane k (k uniform across the wavefront).
 __device__ __forceinline__ double lane_value(double v, int k)
@@ -672,8 +673,8 @@ __device__ __forceinline__ double lane_value(double v, int k)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-__global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const double* right, const double* down,
-    float* ginv, int n, PcgScalars* S, int pin)
+__device__ void coarse_inverse(int w, int h, const double* right, const double* down, float* ginv, int n,
+    PcgScalars* S, int pin)
 {
     __shared__ double rowbuf[2][64];
     __shared__ double assembled[64];   // diagonal before elimination (pinning bound)
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const doub
     // Solid masks: an unknown whose Schur pivot vanishes is pinned -- never swept, and its row and
     // column of the inverse are zero (the swept block is then the inverse of the other unknowns' matrix)
     unsigned long long pinned = 0ull;   // the same in every thread
-    for (int kb = 0; kb < 4 && !failed; ++kb) {
+    for (int kb = 0; kb < kSweepWaves && !failed; ++kb) {
 #pragma unroll
         for (int r0 = 0; r0 < kSweepRows; ++r0) {
             const int k = kb * kSweepRows + r0;
@@ -749,6 +750,42 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int w, int h, const doub
         const int i = wv * kSweepRows + r;
         if (i < n && lane < n)
             ginv[i * n + lane] = ((pinned >> i) & 1ull) || ((pinned >> lane) & 1ull) ? 0.f : (float)a[r];
+    }
+}
+
+__global__ __launch_bounds__(64 * kSweepWaves) void k_coarse_inverse(int w, int h, const double* right,
+    const double* down, float* ginv, int n, PcgScalars* S, int pin)
+{
+    coarse_inverse(w, h, right, down, ginv, n, S, pin);
+}
+
+// Coarse end of the hierarchy rebuild: each level's cells coarsened, then its diagonal and FP32 copies,
+// each phase workgroup-strided and closed by a barrier (which also makes this workgroup's global
+// writes visible to its next phase); then the coarse inverse.
+__global__ __launch_bounds__(64 * kSweepWaves) void k_coarse_chain(const ChainDesc C, float* ginv, int n,
+    PcgScalars* S, int pin)
+{
+    for (int l = 0; l < C.levels; ++l) {
+        const ChainLevel& L = C.lev[l];
+        const int cells = L.cw * L.ch;
+        for (int t = threadIdx.x; t < cells; t += 64 * kSweepWaves) {
+            const int y = t / L.cw, x = t - y * L.cw;
+            coarsen_cell(x, y, L.fw, L.fh, L.fcx, L.fcy, L.fright, L.fdown, L.cw, L.ch, L.ccx, L.ccy, L.cright,
+                L.cdown);
+        }
+        __syncthreads();
+        const int s = L.cw + 2;
+        for (int t = threadIdx.x; t < cells; t += 64 * kSweepWaves) {
+            const int y = t / L.cw, x = t - y * L.cw;
+            const int i = (y + 1) * s + x + 1;
+            level_diag_values(L.cright[i - 1], L.cright[i], L.cdown[i - s], L.cdown[i], L.invd32[i], L.right32[i],
+                L.down32[i]);
+        }
+        __syncthreads();
+    }
+    if (n > 0) {
+        const ChainLevel& L = C.lev[C.levels - 1];
+        coarse_inverse(L.cw, L.ch, L.cright, L.cdown, ginv, n, S, pin);
     }
 }
 
@@ -1740,12 +1777,18 @@ void launch_coarse_factor(hipStream_t st, int w, int h, const double* right64, c
         S, pin);
 }
 
+void launch_coarse_chain(hipStream_t st, const ChainDesc& chain, float* ginv, int n, PcgScalars* S, int pin)
+{
+    hipLaunchKernelGGL(k_coarse_chain, dim3(1), dim3(64 * kSweepWaves), 0, st, chain, ginv, n, S, pin);
+}
+
 void launch_coarse_inverse(hipStream_t st, int w, int h, const double* right64, const double* down64, float* ginv,
     int n, PcgScalars* S, int pin)
 {
     if (n <= 0)
         return;
-    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, st, w, h, right64, down64, ginv, n, S, pin);
+    hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(64 * kSweepWaves), 0, st, w, h, right64, down64, ginv, n, S,
+        pin);
 }
 
 void launch_mg_tail(hipStream_t st, const TailDesc& desc, size_t lds_bytes, const double* shift, const int* status,
