@@ -1397,14 +1397,16 @@ struct fs_grid {
     }
 
     // Last phase: fold the correction into the pressure, remove its mean, correct the velocity and
-    // measure the divergence -- launches only.
+    // measure the divergence -- launches only.  Every kernel of it checks on the device that the solve
+    // has stopped and is a no-op otherwise, so it may be queued behind iterations that might not finish
+    // the solve.
     void project_end(double dt)
     {
         GridDev g = dev();
         launch_final_flush_sum(st(), g);
         launch_sub_mean_p(st(), g, 1);
         launch_correct_velocity(st(), g, dt);
-        launch_max_divergence(st(), g);
+        launch_max_divergence(st(), g, true);
     }
 
     // `tol` is the caller's divergence tolerance; in the relative mode the accepted divergence is the
@@ -1446,21 +1448,22 @@ struct fs_grid {
             return res;
         ensure_graph();
         project_begin(dt, budget(max_iterations), tol);
-        PcgScalars s = read_scalars();
-        if (s.status == kRunning) {
-            int batch = std::max(1, std::min(predicted, 256));
-            for (;;) {
-                for (int i = 0; i < batch; ++i)
-                    HIPC(hipGraphLaunch(iter_exec, st()));
-                s = read_scalars();
-                if (s.status != kRunning)
-                    break;
-                batch = 2;
-            }
-            predicted = std::max(1, s.iterations);
+        // The set-up, the iterations the previous solve needed and the closing phase are queued without a
+        // read-back in between (iterations past the solve's end and a closing phase before it are
+        // device-side no-ops); more iterations follow, two at a time, while the solve still runs.
+        int batch = std::min(predicted, 256);
+        PcgScalars s{};
+        for (;;) {
+            for (int i = 0; i < batch; ++i)
+                HIPC(hipGraphLaunch(iter_exec, st()));
+            project_end(dt);
+            s = read_scalars();
+            if (s.status != kRunning)
+                break;
+            batch = 2;
         }
-        project_end(dt);
-        return project_result(read_scalars(), tol, dt);
+        predicted = s.iterations;
+        return project_result(s, tol, dt);
     }
 
     // This grid's entry of a batched solve (the iteration body of iteration_kernels()).
@@ -1719,8 +1722,7 @@ CheckpointConfig parse_checkpoint(const fs_grid& g, const char* text, size_t len
     };
     mixture("liquid_mixture", c.rho_l, c.mu_l);
     mixture("gas_mixture", c.rho_g, c.mu_g);
-    c.predicted = (int)std::max<long long>(
-        1, integer_of(member(meta, "previous_solve_iterations"), "previous_solve_iterations", 0, 1 << 30));
+    c.predicted = (int)integer_of(member(meta, "previous_solve_iterations"), "previous_solve_iterations", 0, 1 << 30);
     return c;
 }
 
@@ -3366,7 +3368,7 @@ static void batch_projection(fs_context* ctx, fs_grid* const* grids, const std::
             const int i = batched[(size_t)k];
             const PcgScalars sc = ctx->batch_pinned[k];
             results[i] = grids[i]->project_result(sc, tolerances[i], dts[i]);
-            grids[i]->predicted = std::max(1, sc.iterations);
+            grids[i]->predicted = sc.iterations;
         }
     }
 }

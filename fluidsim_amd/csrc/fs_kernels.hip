@@ -442,7 +442,10 @@ __global__ __launch_bounds__(kNT) void k_correct_velocity(GridDev g, double dt)
         uv[k] = g.u[(size_t)yc * (W + 1) + xc];
         vv[k] = g.v[(size_t)yc * W + xc];
     }
-    if (g.S->status == kFailed)
+    // after the solve only: a failed hierarchy leaves the velocity alone, and a solve still running (the
+    // correction launched behind a batch of iterations that did not finish it) is corrected later
+    const int st = g.S->status;
+    if (st == kFailed || st == kRunning)
         return;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -458,9 +461,12 @@ __global__ __launch_bounds__(kNT) void k_correct_velocity(GridDev g, double dt)
 }
 
 // maxDivergence (FluidGridCPU.cpp:1126-1152): non-finite divergence reports infinity.
-__global__ __launch_bounds__(kNT) void k_max_divergence(GridDev g)
+// when_stopped: only once the solve has stopped (the measurement closing a projection).
+__global__ __launch_bounds__(kNT) void k_max_divergence(GridDev g, int when_stopped)
 {
     __shared__ double lds[kNT / 64];
+    if (when_stopped && g.S->status == kRunning)
+        return;
     const int W = g.W, H = g.H;
     const int x = blockIdx.x * kTX + (threadIdx.x & 63);
     const int xc = min(x, W - 1);
@@ -855,7 +861,7 @@ template <bool FLEX> __global__ __launch_bounds__(kNT) void k_pcg_direction(Grid
 // ------------------------------------------------------------------------------------------
 enum FinishOp : int {
     kFinRhs = 0, kFinRhsCenter, kFinPredict, kFinTrueRes0, kFinTrueRes1, kFinFinalSum, kFinDiv, kFinAd,
-    kFinUpdate, kFinRz
+    kFinUpdate, kFinRz, kFinDivDone
 };
 
 __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
@@ -868,6 +874,7 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
     case kFinTrueRes1: run = st == kRunning && S->verify; break;
     case kFinFinalSum: run = st == kConverged || st == kStopped; break;
     case kFinDiv: run = true; break;
+    case kFinDivDone: run = st != kRunning; break;
     default: run = st == kRunning; break;
     }
     if (!run)
@@ -933,7 +940,8 @@ __global__ __launch_bounds__(kNT) void k_finish(GridDev g, int op, int nb)
         }
         break;
     }
-    case kFinDiv: {
+    case kFinDiv:
+    case kFinDivDone: {
         const double t = finish_max<kNT>(P, nb, 1, 0, lds);
         if (threadIdx.x == 0)
             S->div_max = t;
@@ -1160,10 +1168,11 @@ void launch_commit_edits(hipStream_t st, const GridDev& g, const EditCell* cells
         hipLaunchKernelGGL(k_commit_edits, dim3(linear_blocks((size_t)n)), dim3(kNT), 0, st, g, cells, n);
 }
 
-void launch_max_divergence(hipStream_t st, const GridDev& g)
+void launch_max_divergence(hipStream_t st, const GridDev& g, bool when_stopped)
 {
-    hipLaunchKernelGGL(k_max_divergence, cell_grid(g.W, g.H), dim3(kNT), 0, st, g);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kNT), 0, st, g, (int)kFinDiv, (int)cell_blocks(g.W, g.H));
+    hipLaunchKernelGGL(k_max_divergence, cell_grid(g.W, g.H), dim3(kNT), 0, st, g, (int)when_stopped);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kNT), 0, st, g, (int)(when_stopped ? kFinDivDone : kFinDiv),
+        (int)cell_blocks(g.W, g.H));
 }
 
 void launch_gravity(hipStream_t st, const GridDev& g, double gy, double dt)

@@ -138,7 +138,8 @@ void launch_true_residual(hipStream_t st, const GridDev& g, int mode);  // 0 ini
 void launch_flush(hipStream_t st, const GridDev& g);                     // only when S->verify
 void launch_final_flush_sum(hipStream_t st, const GridDev& g);
 void launch_correct_velocity(hipStream_t st, const GridDev& g, double dt);
-void launch_max_divergence(hipStream_t st, const GridDev& g);
+// when_stopped: measure only once the grid's solve has stopped (otherwise a no-op).
+void launch_max_divergence(hipStream_t st, const GridDev& g, bool when_stopped = false);
 void launch_gravity(hipStream_t st, const GridDev& g, double gy, double dt);
 void launch_add_vortex(hipStream_t st, const GridDev& g, double ox, double oy, double cx, double cy, double radius,
     double strength);
