@@ -53,7 +53,8 @@ def test_dam_break_projection_matches_oracle(oracle):
     h.destroy()
 
 
-@pytest.mark.parametrize("w,h", [(128, 128), (65, 37), (1, 129), (131, 1), (13, 9), (300, 200), (2, 67), (1, 1)])
+@pytest.mark.parametrize("w,h", [(128, 128), (65, 37), (1, 129), (131, 1), (13, 9), (300, 200), (2, 67), (1, 1),
+                                 (4000, 3), (2, 3000), (1000, 700)])
 def test_hierarchy_is_bit_exact(oracle, w, h):
     wl = WL.sloped_pool(w, h, 4.0)
     o = oracle_grid(oracle, wl)
