@@ -1390,8 +1390,15 @@ struct fs_grid {
                 launch_merge_failure(st(), S_aux.p, S.p);
             }
         }
-        precondition(status_ptr());
-        tail_join = nullptr;
+        {
+            // a pending join never outlives this V-cycle, not even when a launch in it throws (a later
+            // graph capture would record it)
+            struct ClearJoin {
+                hipEvent_t& e;
+                ~ClearJoin() { e = nullptr; }
+            } clear{tail_join};
+            precondition(status_ptr());
+        }
         launch_pcg_rz(st(), g, z_ptr(), flexible());
         launch_pcg_direction(st(), g, z_ptr(), flexible());
     }

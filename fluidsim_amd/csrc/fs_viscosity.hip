@@ -48,7 +48,10 @@ struct VisScalars {
     float dt;         // time step of this solve: the iteration kernels read it here, so one captured
                       // batch of iterations serves every dt
     int status;       // batched solve: result code (see viscosity_solve)
-    int iterations;   // batched solve: iterations run
+    int iterations;   // iterations run (batched solve; single solve: counted by k_vis_decide)
+    int done;         // single solve, set on the device: 0 running, 1 converged (4 once unpacked),
+                      // 2 breakdown, 3 budget spent, 5 no iteration needed, 6 non-finite input
+    int restart;      // single solve: restart from the recomputed residual before the next batch
     int pad;
 };
 
@@ -70,7 +73,10 @@ __device__ __forceinline__ int sidx(int w, int h, int x, int y)
 __device__ __forceinline__ bool valid_u(int w, int h, int x, int y) { return x > 0 && x < w && y < h; }
 __device__ __forceinline__ bool valid_v(int w, int h, int x, int y) { return y > 0 && y < h && x < w; }
 
-__device__ __forceinline__ bool stopped(const VisScalars* S) { return S->bad != 0 || S->rnorm <= S->target; }
+__device__ __forceinline__ bool stopped(const VisScalars* S)
+{
+    return S->bad != 0 || S->rnorm <= S->target || S->done != 0;
+}
 
 // Index of slot (x, y) in an array: the global slot field clamped like the reference's accessors, or an
 // LDS tile that holds the clamped values (see k_vis_smooth2).  The stencils below are written once over
@@ -555,9 +561,11 @@ __global__ __launch_bounds__(kVT) void k_vis_update(VLev L, float2* xs, float2* 
 
 // recomputed residual t = M v0 - A x and t M^-1 t (mode 3)
 __global__ __launch_bounds__(kVT) void k_vis_truth(VLev L, const float2* v0, const float2* xs, float2* t, float dt,
-    double* partials)
+    double* partials, const VisScalars* S)
 {
     __shared__ double lds[kVT / 64];
+    if (S && S->done != 0)
+        return;
     VIS_SLOT(L)
     double a = 0.0, bad = 0.0;
     if (inside) {
@@ -593,6 +601,8 @@ __global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double*
     __shared__ double lds[kVT / 64];
     if ((op == kVisRz || op == kVisDq) && stopped(S))
         return;
+    if (op == kVisTruth && S->done != 0)
+        return;
     const double a = finish_sum<kVT>(partials, nb, 1, 0, lds);
     const double b = (op == kVisInitial || op == kVisRz) ? finish_sum<kVT>(partials + nb, nb, 1, 0, lds) : 0.0;
     const double m = op == kVisInitial ? finish_sum<kVT>(partials + 2 * nb, nb, 1, 0, lds) : 0.0;
@@ -611,6 +621,7 @@ __global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double*
         s.rnorm = a;
         s.truth = a;
         s.bad = bad ? 1 : 0;
+        s.done = s.bad ? 6 : (s.rnorm <= s.target ? 5 : 0);
         break;
     case kVisRz:
         s.bad |= bad ? 1 : 0;
@@ -641,23 +652,71 @@ __global__ __launch_bounds__(kVT) void k_vis_finish(VisScalars* S, const double*
     *S = s;
 }
 
-// Restart from the recomputed residual ("viscosity refresh residual", ViscosityKernels.h:607-620).
+// Restart from the recomputed residual ("viscosity refresh residual", ViscosityKernels.h:607-620), when
+// k_vis_decide asked for it.
 __global__ void k_vis_refresh(VisScalars* S)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         VisScalars s = *S;
+        if (!s.restart)
+            return;
         s.rnorm = s.truth;
         s.rz = s.alpha = s.beta = 0.0;
         s.steps = 0;
+        s.restart = 0;
         *S = s;
     }
+}
+
+// The decision after a batch of `batch` iterations and the recomputed residual (ViscositySolverGPU.cpp:
+// 264-294, taken on the device so that batches queue without a read-back): breakdown, convergence of the
+// recomputed residual, a restart when only the recurrence claims convergence, and the iteration budget
+// (`maximum` < 0: none).  A batch queued after the solve ended changes nothing (its kernels exit at once);
+// the first such batch marks the converged result as unpacked.
+__global__ void k_vis_decide(VisScalars* S, int batch, int maximum)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    VisScalars s = *S;
+    if (s.done != 0) {
+        if (s.done == 1) {
+            s.done = 4;
+            *S = s;
+        }
+        return;
+    }
+    s.iterations += batch;
+    if (s.bad) {
+        s.done = 2;
+    } else if (s.truth <= s.target) {
+        s.done = 1;
+    } else {
+        if (s.rnorm <= s.target)
+            s.restart = 1;
+        if (maximum >= 0 && s.iterations >= maximum)
+            s.done = 3;
+    }
+    *S = s;
+}
+
+// r = the recomputed residual t, when k_vis_decide asked for a restart.
+__global__ __launch_bounds__(kVT) void k_vis_restart_copy(const VisScalars* S, float2* r, const float2* t, size_t n)
+{
+    if (!S->restart)
+        return;
+    for (size_t i = (size_t)blockIdx.x * kVT + threadIdx.x; i < n; i += (size_t)gridDim.x * kVT)
+        r[i] = t[i];
 }
 
 // "viscosity staggered result" (ViscosityKernels.h:165-185): every face of u and v takes the solution
 // (walls 0).  The output is the FP32 iterate, as in the reference: writing u0 + (x - v0) instead would
 // keep the FP32 rounding noise of u0, which the stiff operator dt K amplifies in the residual.
-__global__ __launch_bounds__(kVT) void k_vis_unpack(int W, int H, const float2* xs, double* u, double* v)
+// S (single solve): only once k_vis_decide has found the solve converged, and only the first time.
+__global__ __launch_bounds__(kVT) void k_vis_unpack(int W, int H, const float2* xs, double* u, double* v,
+    const VisScalars* S)
 {
+    if (S && S->done != 1)
+        return;
     const int x = blockIdx.x * kVX + (threadIdx.x & 63);
     const int y = blockIdx.y * kVY + (threadIdx.x >> 6);
     if (x <= W && y < H) {
@@ -1153,6 +1212,7 @@ struct ViscositySolver {
         const Level* operator[](size_t i) const { return &items[i]; }
         Level* add() { return &items[count++]; }
     };
+    int last_rounds = 0;   // batches of four iterations the last solve ran (0: none, or no solve yet)
     int W, H;
     LevelList levels;
     VBuf<float2> v0, r0, r, d, q, t;
@@ -1416,48 +1476,60 @@ struct ViscositySolver {
         VLev L0 = view(0);
         const dim3 g0 = slot_grid(W, H);
         const int nb = (int)slot_blocks(W, H);
-        VisScalars s = read(st);
-        const double denom = std::max(s.energy, 1e-30);
-        *relative_out = std::sqrt(std::max(s.rnorm, 0.0) / denom);
-        if (s.bad)
-            return 1;   // non-finite velocity or material
-        if (s.rnorm <= s.target)
-            return 0;
         const size_t ns = (size_t)(W + 1) * (H + 1);
         vhip(hipMemcpyAsync(r.p, r0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
         vhip(hipMemcpyAsync(t.p, v0.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
         float2* delta = t.p;   // the iterate x, starting from v0 (the reference's formulation)
         VBuf<float2>& truth = q;   // q is free between iterations
-        int iterations = 0;
-        while (maximum < 0 || iterations < maximum) {
-            const int batch = maximum < 0 ? 4 : std::min(4, maximum - iterations);
-            if (batch == 4) {
-                run_batch(st, delta);
-            } else {
-                for (int i = 0; i < batch; ++i)
-                    iteration(st, delta);
-            }
-            iterations += batch;
-            L0 = view(0);
-            hipLaunchKernelGGL(k_vis_truth, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p, (const float2*)delta,
-                truth.p, fdt, partials.p);
-            finish(st, nb, kVisTruth, 0.0);
-            s = read(st);
-            *iterations_out = iterations;
-            *relative_out = std::sqrt(std::max(s.truth, 0.0) / denom);
-            if (s.bad)
-                return 2;   // breakdown
-            if (s.truth <= s.target) {
-                hipLaunchKernelGGL(k_vis_unpack, g0, dim3(kVT), 0, st, W, H, (const float2*)delta, u, v);
-                return 0;
-            }
-            if (s.rnorm <= s.target) {
+        // Batches of four iterations, each followed by the recomputed residual and k_vis_decide, are queued
+        // without a read-back in between: one batch fewer than the previous solve needed (a batch past the
+        // solve's end costs its launches, a missing one a read-back), then one at a time.  Every decision
+        // is taken on the device, so the iterations and values are those of reading back after each batch.
+        int rounds = last_rounds > 0 ? std::max(1, last_rounds - 1) : 0;
+        int queued = 0;
+        VisScalars s{};
+        for (;;) {
+            const int queued_before = queued;
+            for (int k = 0; k < rounds && (maximum < 0 || queued < maximum); ++k) {
+                const int batch = maximum < 0 ? 4 : std::min(4, maximum - queued);
+                if (batch == 4) {
+                    run_batch(st, delta);
+                } else {
+                    for (int i = 0; i < batch; ++i)
+                        iteration(st, delta);
+                }
+                queued += batch;
+                L0 = view(0);
+                hipLaunchKernelGGL(k_vis_truth, g0, dim3(kVT), 0, st, L0, (const float2*)v0.p,
+                    (const float2*)delta, truth.p, fdt, partials.p, (const VisScalars*)S.p);
+                finish(st, nb, kVisTruth, 0.0);
+                hipLaunchKernelGGL(k_vis_decide, dim3(1), dim3(64), 0, st, S.p, batch, maximum);
                 // the recurrence claims convergence the recomputed residual denies: restart from it
-                vhip(hipMemcpyAsync(r.p, truth.p, ns * sizeof(float2), hipMemcpyDeviceToDevice, st), "copy");
+                hipLaunchKernelGGL(k_vis_restart_copy, dim3((unsigned)std::min<size_t>((ns + kVT - 1) / kVT, 2048)),
+                    dim3(kVT), 0, st, (const VisScalars*)S.p, r.p, (const float2*)truth.p, ns);
                 hipLaunchKernelGGL(k_vis_refresh, dim3(1), dim3(64), 0, st, S.p);
+                hipLaunchKernelGGL(k_vis_unpack, g0, dim3(kVT), 0, st, W, H, (const float2*)delta, u, v,
+                    (const VisScalars*)S.p);
             }
+            s = read(st);
+            // (k_vis_decide ends the solve once the budget is spent: nothing left to queue means done)
+            if (s.done != 0 || maximum == 0 || (rounds > 0 && queued == queued_before))
+                break;
+            rounds = 1;
         }
-        return 3;   // not converged within the caller's budget
+        last_rounds = (s.iterations + 3) / 4;
+        const double denom = std::max(s.energy, 1e-30);
+        *iterations_out = s.iterations;
+        const bool iterated = s.done >= 1 && s.done <= 4;
+        *relative_out = std::sqrt(std::max(iterated ? s.truth : s.rnorm, 0.0) / denom);
+        switch (s.done) {
+        case 5:
+        case 1:
+        case 4: return 0;
+        case 6: return 1;   // non-finite velocity or material
+        case 2: return 2;   // breakdown
+        default: return 3;  // not converged within the caller's budget
+        }
     }
 };
 
