@@ -1746,7 +1746,8 @@ fs_status fs_context_create(int device, fs_context** out) { return fs_context_cr
 fs_status fs_context_create_ex(int device, int flags, fs_context** out)
 {
     return guard([&] {
-        need((flags & ~FS_CONTEXT_SINGLE_STREAM) == 0, FS_E_INVALID_ARGUMENT, "fs_context_create_ex: unknown flags");
+        need((flags & ~(FS_CONTEXT_SINGLE_STREAM | FS_CONTEXT_PRIVATE_QUEUE)) == 0, FS_E_INVALID_ARGUMENT,
+            "fs_context_create_ex: unknown flags");
         need(out != nullptr, FS_E_INVALID_ARGUMENT, "fs_context_create: null output pointer");
         int count = 0;
         HIPC(hipGetDeviceCount(&count));
@@ -1761,7 +1762,17 @@ fs_status fs_context_create_ex(int device, int flags, fs_context** out)
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
             ctx->compute_units = cus;
-        HIPC(hipStreamCreate(&ctx->stream));
+        if ((flags & FS_CONTEXT_PRIVATE_QUEUE) && ctx->compute_units > 0) {
+            // a CU mask over every compute unit: same placement as an unmasked stream, on a queue of its own
+            const int cus = ctx->compute_units;
+            const int words = (cus + 31) / 32;
+            std::vector<uint32_t> mask((size_t)words, 0xffffffffu);
+            if (cus % 32 != 0)
+                mask.back() = (1u << (cus % 32)) - 1u;
+            HIPC(hipExtStreamCreateWithCUMask(&ctx->stream, (uint32_t)words, mask.data()));
+        } else {
+            HIPC(hipStreamCreate(&ctx->stream));
+        }
         if (!(flags & FS_CONTEXT_SINGLE_STREAM)) {
             HIPC(hipStreamCreate(&ctx->aux));
             HIPC(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));

@@ -531,7 +531,8 @@ fs_status fs_server_create(const fs_server_config* config, fs_server** out)
         for (int i = 0; i < config->lanes; ++i) {
             auto lane = std::make_unique<Lane>();
             lane->index = i;
-            if (fs_context_create_ex(config->device, FS_CONTEXT_SINGLE_STREAM, &lane->ctx) != FS_OK) {
+            if (fs_context_create_ex(config->device, FS_CONTEXT_SINGLE_STREAM | FS_CONTEXT_PRIVATE_QUEUE, &lane->ctx) !=
+                FS_OK) {
                 const std::string message = fs_last_error();
                 for (auto& l : s->lanes)
                     fs_context_destroy(l->ctx);

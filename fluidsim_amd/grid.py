@@ -183,12 +183,14 @@ class FluidGridEdit:
 class Context:
     """One HIP device and its streams (fs_context): the work stream plus a helper stream for hierarchy
     rebuilds, or the work stream alone with single_stream=True (fs_context_create_ex,
-    FS_CONTEXT_SINGLE_STREAM)."""
+    FS_CONTEXT_SINGLE_STREAM); private_queue=True puts the work stream on a hardware queue of its own
+    (FS_CONTEXT_PRIVATE_QUEUE)."""
 
-    def __init__(self, device: int = 0, single_stream: bool = False):
+    def __init__(self, device: int = 0, single_stream: bool = False, private_queue: bool = False):
         h = C.c_void_p()
-        if single_stream:
-            call("fs_context_create_ex", int(device), 1, C.byref(h))
+        flags = (1 if single_stream else 0) | (2 if private_queue else 0)
+        if flags:
+            call("fs_context_create_ex", int(device), flags, C.byref(h))
         else:
             call("fs_context_create", int(device), C.byref(h))
         self._h = h

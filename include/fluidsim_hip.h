@@ -128,10 +128,16 @@ const char* fs_last_error(void);
 /* Opens HIP device `device` with its work stream and a helper stream (the multigrid hierarchy is rebuilt
  * on the helper while the work stream forms the right-hand side). */
 fs_status fs_context_create(int device, fs_context** out);
-/* As fs_context_create; FS_CONTEXT_SINGLE_STREAM creates the work stream only (the rebuild then runs in
- * line), so that N such contexts occupy N of the process's hardware queues (GPU_MAX_HW_QUEUES) -- the
- * serving loop's lanes use it.  No reference counterpart. */
-enum { FS_CONTEXT_SINGLE_STREAM = 1 };
+/* As fs_context_create, with flags (no reference counterpart):
+ *   FS_CONTEXT_SINGLE_STREAM  the work stream only (the hierarchy rebuild then runs in line);
+ *   FS_CONTEXT_PRIVATE_QUEUE  the work stream on a hardware queue of its own: it is created with a CU
+ *                             mask that covers every compute unit (hipExtStreamCreateWithCUMask), and
+ *                             the HIP runtime does not share such a queue with other streams.  Without
+ *                             it, streams share the process's pool of GPU_MAX_HW_QUEUES queues in an
+ *                             order set by which streams the process created and used before, so two
+ *                             busy streams can end up on one queue and run one after the other.
+ * The serving loop's lanes use both. */
+enum { FS_CONTEXT_SINGLE_STREAM = 1, FS_CONTEXT_PRIVATE_QUEUE = 2 };
 fs_status fs_context_create_ex(int device, int flags, fs_context** out);
 fs_status fs_context_destroy(fs_context* ctx);
 fs_status fs_context_synchronize(fs_context* ctx);
