@@ -146,6 +146,32 @@ def test_single_stream_context_batches_every_grid():
     ctx.close()
 
 
+@pytest.mark.parametrize("single,private", [(False, True), (True, True), (True, False)])
+def test_context_flags_change_no_result(single, private):
+    """FS_CONTEXT_SINGLE_STREAM (rebuild in line) and FS_CONTEXT_PRIVATE_QUEUE (the work stream on a
+    hardware queue of its own, as the serving loop's lanes use) place work differently but compute the
+    same: steps on such a context equal steps on the default context bit for bit; unknown flags are
+    refused."""
+    wl = WL.dam_break(96, 80, seed=1)
+    ctx = Context(0, single_stream=single, private_queue=private)
+    a = FluidGridHIP(wl.width, wl.height, wl.cell_size, context=ctx)
+    b = FluidGridHIP(wl.width, wl.height, wl.cell_size)
+    for g in (a, b):
+        WL.load_into(g, wl)
+    for _ in range(4):
+        ra, rb = a.step(wl.dt, 300, 1e-6), b.step(wl.dt, 300, 1e-6)
+        assert (ra.iterations, ra.max_divergence) == (rb.iterations, rb.max_divergence)
+    sa, sb = a.download(), b.download()
+    assert np.array_equal(sa.u, sb.u) and np.array_equal(sa.v, sb.v) and np.array_equal(sa.pressure, sb.pressure)
+    a.destroy()
+    b.destroy()
+    ctx.close()
+    import ctypes as C
+    from fluidsim_amd._abi import lib
+    h = C.c_void_p()
+    assert lib().fs_context_create_ex(0, 4, C.byref(h)) != 0
+
+
 def test_diagonal_and_multigrid_grids_mix():
     grids = [pool_grid(W, H, 7 + k, FS_PRECOND_DIAGONAL if k % 2 else None)[0] for k, (W, H) in
              enumerate([(40, 30), (33, 20), (64, 48), (50, 50)])]
