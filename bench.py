@@ -246,8 +246,12 @@ def dam_break_leg(steps=100, seed=1):
     }
 
 
+# fs_mg_precision values of --serving-precision (None: the leg's own choice)
+SERVING_PRECISIONS = {"default": None, "fp32": 0, "fp16": 1, "bf16": 2}
+
+
 def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, batch_cells=0,
-                compare_batch_cells=0, smoother=None):
+                compare_batch_cells=0, smoother=None, precision=None):
     """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
     [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each).
     It is first served one request at a time on one stream with FP32 multigrid levels (serial plain), then
@@ -260,7 +264,7 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
     `compare_batch_cells`, one more run at the last rate batches the requests of at most that many cells
     (fs_batch_step: one workgroup per instance in the persistent solves).  `smoother` (kind, sweeps, coarse
     sweeps) is the pressure smoother of the served FP32-level requests; the serial pass keeps the reference's
-    V(4,4) cycle."""
+    V(4,4) cycle.  `precision` (an fs_mg_precision) overrides the served requests' multigrid storage."""
     from fluidsim_amd import FS_MG_BF16, FS_MG_FP32
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
@@ -310,7 +314,7 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
 
     runs = []
     for rate in rates:
-        stream = stream_at(rate, choice=smoother)
+        stream = stream_at(rate, precision=precision, choice=smoother)
         samples = []
         for _ in range(repeats):
             samples.append(serve(stream, batch_cells))
@@ -699,6 +703,9 @@ def main():
     ap.add_argument("--serving-repeats", type=int, default=3, help="runs per pass; the median is reported")
     ap.add_argument("--serving-max-size", type=int, default=2048)
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--serving-precision", choices=sorted(SERVING_PRECISIONS), default="default",
+                    help="multigrid storage of the served requests (default: bfloat16 conductances above "
+                         "the batch size, FP32 below)")
     ap.add_argument("--batch-instances", type=int, default=256, help="size of the C4 batched-solve leg (0: skip)")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-splits", type=str, default="131072,0",
@@ -736,6 +743,7 @@ def main():
                                      batch_cells=args.serving_batch_cells,
                                      compare_batch_cells=args.serving_compare_batch_cells,
                                      smoother=parse_smoother(args.serving_smoother),
+                                     precision=SERVING_PRECISIONS[args.serving_precision],
                                      repeats=max(1, args.serving_repeats))), flush=True)
         return
 
@@ -916,6 +924,7 @@ def main():
                                      batch_cells=args.serving_batch_cells,
                                      compare_batch_cells=args.serving_compare_batch_cells,
                                      smoother=parse_smoother(args.serving_smoother),
+                                     precision=SERVING_PRECISIONS[args.serving_precision],
                                      repeats=max(1, args.serving_repeats))
     if rank == 0:
         print(json.dumps(out), flush=True)
