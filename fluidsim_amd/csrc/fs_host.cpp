@@ -274,6 +274,7 @@ struct Level {
     int restrict_variant = 0;   // tile of k_res_restrict from this level (autotuned)
     int jacobi_pre = 0;         // tiles of k_jacobi_multi pre- and post-smoothing (autotuned on first use)
     int jacobi_post = 0;
+    int jacobi_fused = -1;      // k_jacobi_res_restrict tile for pre-smoothing + restriction; -1: two launches
 
     size_t padded() const { return (size_t)s * (h + 2); }
 };
@@ -1140,9 +1141,14 @@ struct fs_grid {
             const float* rhs = l == 0 ? r.p : L.rhs;
             const double* shift = l == 0 ? rmean_ptr() : nullptr;
             const Level& F = levels[l];
-            smooth_level(l, 0, rhs, shift, true, none, status, -1);
             AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
             AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+            if (chebyshev_on(l) && F.jacobi_fused >= 0) {
+                launch_jacobi_res_restrict(st(), L, 0, rhs, shift, chebyshev_weights(true), smooth_sweeps, view(l + 1),
+                    rx, ry, status, F.jacobi_fused);
+                continue;
+            }
+            smooth_level(l, 0, rhs, shift, true, none, status, -1);
             launch_res_restrict(st(), L, 1, rhs, shift, view(l + 1), rx, ry, status, F.restrict_variant);
         }
         join_before_tail();
@@ -1224,6 +1230,27 @@ struct fs_grid {
                 if (ms < best) {
                     best = ms;
                     F.jacobi_post = v;
+                }
+            }
+            // pre-smoothing + residual + restriction: the two launches against the fused one
+            F.jacobi_fused = -1;
+            const Level& Cn = levels[l + 1];
+            if (2 * (long long)Cn.w >= F.w && 2 * (long long)Cn.h >= F.h) {
+                const LevelView C = view(l + 1);
+                const AxisRestrict rx{F.d_rxc.p, F.d_rxi.p, F.d_rxw.p};
+                const AxisRestrict ry{F.d_ryc.p, F.d_ryi.p, F.d_ryw.p};
+                best = time([&] {
+                    launch_jacobi_multi(st(), L, 0, rhs, nullptr, 1, pre, smooth_sweeps, none, nullptr, F.jacobi_pre);
+                    launch_res_restrict(st(), L, 1, rhs, nullptr, C, rx, ry, nullptr, F.restrict_variant);
+                });
+                for (int v = 0; v < kJacobiRestrictVariants; ++v) {
+                    const float ms = time([&] {
+                        launch_jacobi_res_restrict(st(), L, 0, rhs, nullptr, pre, smooth_sweeps, C, rx, ry, nullptr, v);
+                    });
+                    if (ms < best) {
+                        best = ms;
+                        F.jacobi_fused = v;
+                    }
                 }
             }
         }
@@ -3142,6 +3169,8 @@ fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int
                  restriction < kRestrictVariants,
             FS_E_INVALID_ARGUMENT, "unknown tile variant");
         Level& L = grid->levels[(size_t)level];
+        if (pre >= 0 && jacobi)
+            L.jacobi_fused = -1;   // an explicit pre-smoothing tile selects the separate launches
         if (pre >= 0)
             (jacobi ? L.jacobi_pre : L.smooth_variant) = pre;
         if (post >= 0)

@@ -211,6 +211,13 @@ struct SmoothWeights {
 };
 void launch_jacobi_multi(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
     int from_zero, const SmoothWeights& weights, int nu, const CoarseLink& link, const int* status, int variant);
+// Pre-smoothing from zero (launch_jacobi_multi with from_zero) and the residual + restriction into C
+// (launch_res_restrict) in one launch, same values; tile variants 0 = 16x8 coarse cells on 512 threads,
+// 1 = 16x16/512, 2 = 32x16/1024.  C must cover the fine level (2 C.w >= L.w, 2 C.h >= L.h).
+constexpr int kJacobiRestrictVariants = 3;
+void launch_jacobi_res_restrict(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
+    const SmoothWeights& weights, int nu, const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status,
+    int variant);
 // Residual of sol[cur] and its restriction into C.rhs in one launch; variant: coarse tile 0 = 32 x 16,
 // 1 = 16 x 16, 2 = 16 x 8, 3 = 8 x 8.
 constexpr int kRestrictVariants = 4;

@@ -512,6 +512,194 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelViewT<C, V> F, int cu
 }
 
 // ---------------------------------------------------------------------------------------------
+// Pre-smoothing from zero with the Chebyshev-weighted Jacobi sweeps, fused with the residual and the
+// restriction that follow it, for one coarse tile of CTX x CTY cells.  The smoothing tile is the coarse
+// tile's fine image widened by the restriction's reach and the residual's ring (RING = 3: taps in
+// [2c - 2, 2c + 3], one more cell for the operator) and by kJacobiMaxSweeps cells of smoothing halo; with
+// k_jacobi_multi's update rule its cells at least that halo from the edge hold exactly what nu global
+// sweeps compute.  The fine image's cells are written as the pre-smoothed solution, the residual of the
+// restriction window is formed from LDS with k_res_restrict's expression and restricted with its sums in
+// the same order (a missing tap points at the same cell) -- the values of the two separate launches, in
+// one.  It pays on the levels where launches, not bytes, set the time; the autotuner decides per level.
+// ---------------------------------------------------------------------------------------------
+template <int CTX, int CTY, int NT>
+__global__ __launch_bounds__(NT) void k_jacobi_res_restrict(LevelView L, int cur, const float* rhs,
+    const double* shift, SmoothWeights wt, int nu, LevelView Cl, AxisRestrict rx, AxisRestrict ry,
+    const int* status)
+{
+    constexpr int HALO = kJacobiMaxSweeps;
+    constexpr int RING = 3;
+    constexpr int EDGE = RING + HALO;
+    constexpr int LX = 2 * CTX + 2 * EDGE, LY = 2 * CTY + 2 * EDGE;
+    constexpr int NE = LX * LY;
+    constexpr int ROUNDS = (NE + NT - 1) / NT;
+    static_assert(CTX + CTY <= NT, "one thread per restriction table row");
+    __shared__ float s_x[2][NE];
+    __shared__ int s_rxi[CTX * 4], s_ryi[CTY * 4];
+    __shared__ float s_rxw[CTX * 4], s_ryw[CTY * 4];
+    const int w = L.w, h = L.h, s = L.s;
+    int bx, by;
+    xcd_tile((Cl.w + CTX - 1) / CTX, (Cl.h + CTY - 1) / CTY, bx, by);
+    const int cx0 = bx * CTX, cy0 = by * CTY;
+    const int x0 = 2 * cx0 - EDGE, y0 = 2 * cy0 - EDGE;
+    // every global load goes through a clamped (valid) interior index and is masked afterwards
+    float cl[ROUNDS], cr[ROUNDS], cu[ROUNDS], cd[ROUNDS], cb[ROUNDS];
+    bool cin[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int e = threadIdx.x + r * NT;
+        const int ly = e / LX, lx = e - ly * LX;
+        const int x = x0 + lx, y = y0 + ly;
+        cin[r] = e < NE && x >= 0 && x < w && y >= 0 && y < h;
+        const int P = (min(max(y, 0), h - 1) + 1) * s + (min(max(x, 0), w - 1) + 1);
+        cl[r] = L.right[P - 1];
+        cr[r] = L.right[P];
+        cu[r] = L.down[P - s];
+        cd[r] = L.down[P];
+        cb[r] = rhs[P];
+    }
+    // restriction tables of this coarse tile, as k_res_restrict stages them (indices relative to x0, y0;
+    // a missing tap: weight 0 at the cell k_res_restrict's missing taps read)
+    int tc[2], ti[2][4];
+    float tw[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int e = threadIdx.x - a * CTX;  // threads [0, CTX) take x, [CTX, CTX + CTY) take y
+        const bool isx = a == 0;
+        const int n_axis = isx ? CTX : CTY;
+        const int c = (isx ? cx0 : cy0) + min(max(e, 0), n_axis - 1);
+        const int limit = isx ? Cl.w : Cl.h;
+        const AxisRestrict& R = isx ? rx : ry;
+        const int cc = min(c, limit - 1);
+        const int n = R.count[cc];
+        tc[a] = c < limit ? n : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ti[a][k] = R.index[cc * 4 + k];
+            tw[a][k] = R.weight[cc * 4 + k];
+        }
+    }
+    if (status && *status != kRunning)
+        return;
+    const double sh = shift ? *shift : 0.0;
+    float inv[ROUNDS];
+    int reach[ROUNDS];   // distance to the tile edge + 1; 0 for a cell outside the grid
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int e = threadIdx.x + r * NT;
+        const int ly = e / LX, lx = e - ly * LX;
+        if (!cin[r]) {
+            cl[r] = cr[r] = cu[r] = cd[r] = cb[r] = 0.f;
+        } else if (shift) {
+            cb[r] = (float)((double)cb[r] - sh);
+        }
+        inv[r] = inv_or_zero((cl[r] + cr[r]) + (cu[r] + cd[r]));
+        reach[r] = cin[r] ? min(min(ly, LY - 1 - ly), min(lx, LX - 1 - lx)) + 1 : 0;
+        if (e < NE) {
+            s_x[0][e] = 0.f;
+            s_x[1][e] = 0.f;
+        }
+    }
+    constexpr int MISS = EDGE - 2;   // k_res_restrict's missing-tap cell, 2c0 - 2 on each axis
+    if (threadIdx.x < CTX) {
+        const int e = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = k < tc[0];
+            s_rxi[e * 4 + k] = on ? ti[0][k] - x0 : MISS;
+            s_rxw[e * 4 + k] = on ? tw[0][k] : 0.f;
+        }
+    } else if (threadIdx.x < CTX + CTY) {
+        const int e = threadIdx.x - CTX;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = k < tc[1];
+            s_ryi[e * 4 + k] = on ? ti[1][k] - y0 : MISS;
+            s_ryw[e * 4 + k] = on ? tw[1][k] : 0.f;
+        }
+    }
+    __syncthreads();
+    const int skip = HALO - nu;
+#pragma unroll 1
+    for (int k = 0; k < nu; ++k) {
+        float wk = wt.w[0];   // uniform select: no dynamic indexing of the argument block
+#pragma unroll
+        for (int j = 1; j < kJacobiMaxSweeps; ++j)
+            wk = k == j ? wt.w[j] : wk;
+        const float* src = s_x[k & 1];
+        float* dst = s_x[(k & 1) ^ 1];
+        const bool neighbours = k != 0;
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            if (reach[r] > k + 1 + skip) {
+                const int e = threadIdx.x + r * NT;
+                float v;
+                if (neighbours) {
+                    const float c = src[e];
+                    const float product = (cl[r] * (c - src[e - 1]) + cr[r] * (c - src[e + 1])) +
+                                          (cu[r] * (c - src[e - LX]) + cd[r] * (c - src[e + LX]));
+                    v = c + wk * (cb[r] - product) * inv[r];
+                } else {
+                    v = wk * cb[r] * inv[r];
+                }
+                dst[e] = inv[r] > 0.f ? v : 0.f;
+            }
+        }
+        __syncthreads();
+    }
+    const float* fin = s_x[nu & 1];
+    float* res = s_x[(nu & 1) ^ 1];
+    float* sol_out = L.sol[cur ^ 1];
+    // the pre-smoothed solution of the coarse tile's fine image, and the residual of the restriction
+    // window ([2 c0 - 2, 2 c1 + 2) on each axis; zero outside the grid and at cells without conductance)
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int e = threadIdx.x + r * NT;
+        const int ly = e / LX, lx = e - ly * LX;
+        if (e >= NE)
+            continue;
+        if (cin[r] && lx >= EDGE && lx < EDGE + 2 * CTX && ly >= EDGE && ly < EDGE + 2 * CTY)
+            sol_out[(y0 + ly + 1) * s + (x0 + lx + 1)] = fin[e];
+        if (lx >= EDGE - 2 && lx < EDGE + 2 * CTX + 2 && ly >= EDGE - 2 && ly < EDGE + 2 * CTY + 2) {
+            float v = 0.f;
+            if (cin[r]) {
+                const float c = fin[e];
+                const float product = (cl[r] * (c - fin[e - 1]) + cr[r] * (c - fin[e + 1])) +
+                                      (cu[r] * (c - fin[e - LX]) + cd[r] * (c - fin[e + LX]));
+                const bool active = (cl[r] + cr[r]) + (cu[r] + cd[r]) > 0.f;
+                v = active ? cb[r] - product : 0.f;
+            }
+            res[e] = v;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < CTX * CTY; e += NT) {
+        const int ly = e / CTX, lx = e - ly * CTX;
+        const int x = cx0 + lx, y = cy0 + ly;
+        if (x >= Cl.w || y >= Cl.h)
+            continue;
+        float tap[4][4];
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            const float* row = res + s_ryi[ly * 4 + iy] * LX;
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+                tap[iy][ix] = row[s_rxi[lx * 4 + ix]];
+        }
+        float value = 0.f;
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+            float horizontal = 0.f;
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+                horizontal += s_rxw[lx * 4 + ix] * tap[iy][ix];
+            value += s_ryw[ly * 4 + iy] * horizontal;
+        }
+        Cl.rhs[(y + 1) * Cl.s + x + 1] = value;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Coarse factorisation (factorCoarsest, PressureMultigrid.cpp:300-345) and inverse.
 // Right-looking updates applied column by column: every entry receives its subtractions in the
 // same k order as the reference's row loop, so the factor is bit-identical to it.
@@ -1751,6 +1939,22 @@ void launch_jacobi_multi(hipStream_t st, const LevelView& L, int cur, const floa
     }
 #undef FS_JM_MODES
 #undef FS_JM
+}
+
+void launch_jacobi_res_restrict(hipStream_t st, const LevelView& L, int cur, const float* rhs, const double* shift,
+    const SmoothWeights& weights, int nu, const LevelView& C, AxisRestrict rx, AxisRestrict ry, const int* status,
+    int variant)
+{
+    nu = std::min(std::max(nu, 1), kJacobiMaxSweeps);
+#define FS_JRR(CX, CY, NT)                                                                                    \
+    hipLaunchKernelGGL((k_jacobi_res_restrict<CX, CY, NT>), dim3(((C.w + CX - 1) / CX) * ((C.h + CY - 1) / CY)), \
+        dim3(NT), 0, st, L, cur, rhs, shift, weights, nu, C, rx, ry, status)
+    switch (variant) {
+    case 1: FS_JRR(16, 16, 512); break;
+    case 2: FS_JRR(32, 16, 1024); break;
+    default: FS_JRR(16, 8, 512); break;
+    }
+#undef FS_JRR
 }
 
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,

@@ -337,7 +337,9 @@ fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, i
  * 3 = 16x16/256, 4 = 32x32/512, 5 = 64x64/1024, 6 = 64x32/1024, 7 = 32x64/1024). */
 fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction);
 /* Overrides the autotuned variants of a tiled level (-1 keeps one): every variant computes the same
- * values, so this changes time only -- for profiling one fixed configuration and for testing that claim. */
+ * values, so this changes time only -- for profiling one fixed configuration and for testing that claim.
+ * The autotuner may run a Chebyshev level's pre-smoothing fused with its residual and restriction (one
+ * launch, the same values); an explicit pre variant selects the separate launches on that level. */
 fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int restriction);
 /* Coarsest dense Cholesky factor, n*n row-major (lower triangle); *n receives the unknown count. */
 fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n);
