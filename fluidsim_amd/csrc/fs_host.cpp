@@ -3150,7 +3150,9 @@ fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* 
         const bool fused = level < grid->tail_start;
         const bool jacobi = fused && grid->chebyshev_on(level);
         if (pre)
-            *pre = fused ? (jacobi ? L.jacobi_pre : L.smooth_variant) : -1;
+            *pre = fused ? (jacobi ? (L.jacobi_fused >= 0 ? kJacobiVariants + L.jacobi_fused : L.jacobi_pre)
+                                   : L.smooth_variant)
+                         : -1;
         if (post)
             *post = fused ? (jacobi ? L.jacobi_post : L.post_variant) : -1;
         if (restriction)
@@ -3165,14 +3167,20 @@ fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int
         need(level >= 0 && level < grid->tail_start, FS_E_INVALID_ARGUMENT, "no tiled level with this index");
         const bool jacobi = grid->chebyshev_on(level);
         const int smooth_variants = jacobi ? kJacobiVariants : kSmoothVariants;
-        need(pre >= -1 && pre < smooth_variants && post >= -1 && post < smooth_variants && restriction >= -1 &&
+        Level& L = grid->levels[(size_t)level];
+        const Level& Cn = grid->levels[(size_t)level + 1];
+        const bool can_fuse = jacobi && 2 * (long long)Cn.w >= L.w && 2 * (long long)Cn.h >= L.h;
+        const int pre_variants = smooth_variants + (can_fuse ? kJacobiRestrictVariants : 0);
+        need(pre >= -1 && pre < pre_variants && post >= -1 && post < smooth_variants && restriction >= -1 &&
                  restriction < kRestrictVariants,
             FS_E_INVALID_ARGUMENT, "unknown tile variant");
-        Level& L = grid->levels[(size_t)level];
-        if (pre >= 0 && jacobi)
-            L.jacobi_fused = -1;   // an explicit pre-smoothing tile selects the separate launches
-        if (pre >= 0)
-            (jacobi ? L.jacobi_pre : L.smooth_variant) = pre;
+        if (pre >= kJacobiVariants && jacobi)
+            L.jacobi_fused = pre - kJacobiVariants;   // the fused pre-smoothing + restriction launch
+        else if (pre >= 0 && jacobi) {
+            L.jacobi_fused = -1;   // the separate launches with this Jacobi tile
+            L.jacobi_pre = pre;
+        } else if (pre >= 0)
+            L.smooth_variant = pre;
         if (post >= 0)
             (jacobi ? L.jacobi_post : L.post_variant) = post;
         if (restriction >= 0)

@@ -334,12 +334,15 @@ fs_status fs_mg_level_restrict(fs_grid* grid, int level, int axis, int* count, i
  * residual-restriction variant (0 = 32x16, 1 = 16x16, 2 = 16x8, 3 = 8x8 coarse
  * cells); -1 for levels solved inside the single-workgroup tail.  On a level that runs the Chebyshev smoother,
  * pre and post are tiles of the Jacobi kernel instead (0 = 64x32/512, 1 = 32x32/256, 2 = 32x16/256,
- * 3 = 16x16/256, 4 = 32x32/512, 5 = 64x64/1024, 6 = 64x32/1024, 7 = 32x64/1024). */
+ * 3 = 16x16/256, 4 = 32x32/512, 5 = 64x64/1024, 6 = 64x32/1024, 7 = 32x64/1024), and pre 8, 9, 10 mean
+ * that the pre-smoothing runs fused with the residual and restriction (16x8/512, 16x16/512, 32x16/1024
+ * coarse cells per workgroup; same values). */
 fs_status fs_mg_level_tiles(fs_grid* grid, int level, int* pre, int* post, int* restriction);
 /* Overrides the autotuned variants of a tiled level (-1 keeps one): every variant computes the same
  * values, so this changes time only -- for profiling one fixed configuration and for testing that claim.
  * The autotuner may run a Chebyshev level's pre-smoothing fused with its residual and restriction (one
- * launch, the same values); an explicit pre variant selects the separate launches on that level. */
+ * launch, the same values): pre 0..7 selects the separate launches with that Jacobi tile, 8..10 the fused
+ * launch with the tiles fs_mg_level_tiles lists. */
 fs_status fs_mg_set_level_tiles(fs_grid* grid, int level, int pre, int post, int restriction);
 /* Coarsest dense Cholesky factor, n*n row-major (lower triangle); *n receives the unknown count. */
 fs_status fs_mg_coarse_factor(fs_grid* grid, double* factor, int* n);

@@ -4,7 +4,8 @@ FP64 oracle with the same choice.
   - the V-cycle (fs_mg_apply) with Chebyshev-weighted Jacobi (3 and 5 sweeps) and with 2 or 3 red-black
     sweeps matches the oracle's to FP32 rounding, on a pool and on a masked 1000:1 tank, and stays a valid
     preconditioner (the FP32 bars of tests/FluidGridWebGPUTests.cpp:428-535);
-  - every Jacobi tile variant computes the same values, bit for bit;
+  - every Jacobi tile variant computes the same values, bit for bit, and so does the pre-smoothing fused
+    with the residual and restriction in each of its tiles;
   - projections with each choice converge and match the oracle's projection with the same choice within
     1e-4 of the velocity scale (iteration counts within a few on the pool, up to 2x on the masked tank,
     where FP32 CG loses conjugacy);
@@ -106,16 +107,20 @@ def test_every_jacobi_tile_computes_the_same_values():
     levels = [l for l in range(g.multigridLevels()) if g.multigridTiles(l)[0] >= 0
               and g.multigridShape(l)[0] * g.multigridShape(l)[1] > 4096]
     assert levels, "no level runs the Chebyshev smoother"
-    g.setMultigridTiles(levels[0], 0, 0, 0)
+    for l in levels:
+        g.setMultigridTiles(l, 0, 0, 0)   # separate launches everywhere
     ref = g.multigridApply(a)
     for l in levels:
-        for v in range(8):
-            g.setMultigridTiles(l, v, 7 - v, v % 4)
-            assert g.multigridTiles(l) == (v, 7 - v, v % 4)
+        # 0..7: Jacobi tiles with the separate residual + restriction launch; 8..10: the fused launch
+        for v in range(11):
+            g.setMultigridTiles(l, v, 7 - v % 8, v % 4)
+            assert g.multigridTiles(l) == (v, 7 - v % 8, v % 4)
             assert np.array_equal(g.multigridApply(a), ref), f"level {l} variant {v}"
         g.setMultigridTiles(l, 0, 0, 0)
     with pytest.raises(ValueError):
-        g.setMultigridTiles(levels[0], 8, -1, -1)
+        g.setMultigridTiles(levels[0], 11, -1, -1)
+    with pytest.raises(ValueError):
+        g.setMultigridTiles(levels[0], -1, 8, -1)
     g.destroy()
 
 
