@@ -708,10 +708,10 @@ def main():
     ap.add_argument("--serving-lanes", type=int, default=4)
     ap.add_argument("--serving-batch-cells", type=int, default=0,
                     help="C5: requests of at most this many cells step batched (0: every request alone)")
-    ap.add_argument("--serving-smoother", type=str, default="chebyshev:5:4",
+    ap.add_argument("--serving-smoother", type=str, default="chebyshev:8:3",
                     help="C5: pressure smoother of the served requests (FP32 levels), red_black:N:C or "
                          "chebyshev:N:C; empty: the reference's V(4,4) with bfloat16 conductances.  The default, "
-                         "5 Chebyshev sweeps and 4 coarse sweeps, is the fastest measured on this stream")
+                         "8 Chebyshev sweeps and 3 coarse sweeps, is the fastest measured on this stream")
     ap.add_argument("--serving-compare-batch-cells", type=int, default=65536,
                     help="C5: one extra run at the last rate with this batch size, for comparison (0: skip)")
     ap.add_argument("--serving-repeats", type=int, default=3, help="runs per pass; the median is reported")

@@ -2043,7 +2043,7 @@ fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int
         need(kind == FS_SMOOTHER_RED_BLACK || kind == FS_SMOOTHER_CHEBYSHEV, FS_E_INVALID_ARGUMENT,
             "unknown pressure smoother");
         need(sweeps >= 1 && sweeps <= (kind == FS_SMOOTHER_RED_BLACK ? 4 : kJacobiMaxSweeps), FS_E_INVALID_ARGUMENT,
-            "sweeps must be 1..4 (red-black) or 1..5 (Chebyshev)");
+            "sweeps must be 1..4 (red-black) or 1..8 (Chebyshev)");
         need(coarse_sweeps >= 1 && coarse_sweeps <= 4, FS_E_INVALID_ARGUMENT, "coarse sweeps must be 1..4");
         need(kind == FS_SMOOTHER_RED_BLACK || grid->mg_prec == FS_MG_FP32, FS_E_UNSUPPORTED,
             "the Chebyshev smoother runs FP32 multigrid levels: set the multigrid precision to FS_MG_FP32");
@@ -2113,7 +2113,7 @@ fs_status fs_tune_pressure_smoother(fs_grid* grid, double dt, int max_iterations
             for (int n : {4, 3, 2})
                 candidates.push_back({FS_SMOOTHER_RED_BLACK, n, coarse});
             if (g.mg_prec == FS_MG_FP32)
-                for (int n : {3, 4, 5})
+                for (int n : {3, 4, 5, 6, 8})
                     candidates.push_back({FS_SMOOTHER_CHEBYSHEV, n, coarse});
         }
         const SmootherChoice original{g.smoother, g.smooth_sweeps, g.coarse_sweeps};

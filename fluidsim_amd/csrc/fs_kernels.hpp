@@ -204,7 +204,8 @@ void launch_rb_multi(hipStream_t st, const LevelView& L, int cur, const float* r
 // reverses them for post-smoothing).  variant: 0 = 64 x 32 / 512, 1 = 32 x 32 / 256, 2 = 32 x 16 / 256,
 // 3 = 16 x 16 / 256, 4 = 32 x 32 / 512, 5 = 64 x 64 / 1024, 6 = 64 x 32 / 1024, 7 = 32 x 64 / 1024 (same
 // results, different speed).
-constexpr int kJacobiMaxSweeps = 5;
+constexpr int kJacobiMaxSweeps = 8;
+constexpr int kJacobiShortHalo = 5;   // tile halo of the kernels used for up to this many sweeps
 constexpr int kJacobiVariants = 8;
 struct SmoothWeights {
     float w[kJacobiMaxSweeps];

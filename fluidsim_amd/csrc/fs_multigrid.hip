@@ -272,11 +272,10 @@ void k_rb_multi(LevelViewT<C, V> L, int cur, const V* rhs, const double* shift, 
 // sweeps the interior holds exactly what nu global sweeps compute.  Cells outside the grid hold zero in both
 // buffers and are never updated.  MODE 0: from zero (the first sweep is w b / diag); 1: post-smoothing,
 // the prolonged coarse correction added to the solution first.
-template <int TX, int TY, int MODE, int NT>
+template <int TX, int TY, int MODE, int NT, int HALO>
 __global__ __launch_bounds__(NT) void k_jacobi_multi(LevelView L, int cur, const float* rhs, const double* shift,
     SmoothWeights wt, int nu, CoarseLink link, const int* status)
 {
-    constexpr int HALO = kJacobiMaxSweeps;
     constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
     constexpr int NE = LX * LY;
     constexpr int ROUNDS = (NE + NT - 1) / NT;
@@ -515,19 +514,18 @@ __global__ __launch_bounds__(kNT) void k_res_restrict(LevelViewT<C, V> F, int cu
 // Pre-smoothing from zero with the Chebyshev-weighted Jacobi sweeps, fused with the residual and the
 // restriction that follow it, for one coarse tile of CTX x CTY cells.  The smoothing tile is the coarse
 // tile's fine image widened by the restriction's reach and the residual's ring (RING = 3: taps in
-// [2c - 2, 2c + 3], one more cell for the operator) and by kJacobiMaxSweeps cells of smoothing halo; with
+// [2c - 2, 2c + 3], one more cell for the operator) and by HALO (>= nu) cells of smoothing halo; with
 // k_jacobi_multi's update rule its cells at least that halo from the edge hold exactly what nu global
 // sweeps compute.  The fine image's cells are written as the pre-smoothed solution, the residual of the
 // restriction window is formed from LDS with k_res_restrict's expression and restricted with its sums in
 // the same order (a missing tap points at the same cell) -- the values of the two separate launches, in
 // one.  It pays on the levels where launches, not bytes, set the time; the autotuner decides per level.
 // ---------------------------------------------------------------------------------------------
-template <int CTX, int CTY, int NT>
+template <int CTX, int CTY, int NT, int HALO>
 __global__ __launch_bounds__(NT) void k_jacobi_res_restrict(LevelView L, int cur, const float* rhs,
     const double* shift, SmoothWeights wt, int nu, LevelView Cl, AxisRestrict rx, AxisRestrict ry,
     const int* status)
 {
-    constexpr int HALO = kJacobiMaxSweeps;
     constexpr int RING = 3;
     constexpr int EDGE = RING + HALO;
     constexpr int LX = 2 * CTX + 2 * EDGE, LY = 2 * CTY + 2 * EDGE;
@@ -1917,15 +1915,23 @@ void launch_jacobi_multi(hipStream_t st, const LevelView& L, int cur, const floa
     int from_zero, const SmoothWeights& weights, int nu, const CoarseLink& link, const int* status, int variant)
 {
     nu = std::min(std::max(nu, 1), kJacobiMaxSweeps);
-#define FS_JM(TX, TY, M, NT)                                                                                  \
-    hipLaunchKernelGGL((k_jacobi_multi<TX, TY, M, NT>), dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)),   \
-        dim3(NT), 0, st, L, cur, rhs, shift, weights, nu, link, status)
-#define FS_JM_MODES(TX, TY, NT)      \
-    do {                             \
-        if (from_zero)               \
-            FS_JM(TX, TY, 0, NT);    \
-        else                         \
-            FS_JM(TX, TY, 1, NT);    \
+    // the halo the sweeps need: 5 cells up to 5 sweeps (the common case), 8 beyond
+#define FS_JM(TX, TY, M, NT, HALO)                                                                            \
+    hipLaunchKernelGGL((k_jacobi_multi<TX, TY, M, NT, HALO>),                                                  \
+        dim3(((L.w + TX - 1) / TX) * ((L.h + TY - 1) / TY)), dim3(NT), 0, st, L, cur, rhs, shift, weights, nu,  \
+        link, status)
+#define FS_JM_MODES(TX, TY, NT)                  \
+    do {                                         \
+        if (nu <= kJacobiShortHalo) {            \
+            if (from_zero)                       \
+                FS_JM(TX, TY, 0, NT, kJacobiShortHalo); \
+            else                                 \
+                FS_JM(TX, TY, 1, NT, kJacobiShortHalo); \
+        } else if (from_zero) {                  \
+            FS_JM(TX, TY, 0, NT, kJacobiMaxSweeps); \
+        } else {                                 \
+            FS_JM(TX, TY, 1, NT, kJacobiMaxSweeps); \
+        }                                        \
     } while (0)
     switch (variant) {
     case 1: FS_JM_MODES(32, 32, 256); break;
@@ -1946,15 +1952,24 @@ void launch_jacobi_res_restrict(hipStream_t st, const LevelView& L, int cur, con
     int variant)
 {
     nu = std::min(std::max(nu, 1), kJacobiMaxSweeps);
-#define FS_JRR(CX, CY, NT)                                                                                    \
-    hipLaunchKernelGGL((k_jacobi_res_restrict<CX, CY, NT>), dim3(((C.w + CX - 1) / CX) * ((C.h + CY - 1) / CY)), \
-        dim3(NT), 0, st, L, cur, rhs, shift, weights, nu, C, rx, ry, status)
+#define FS_JRR_H(CX, CY, NT, HALO)                                                                            \
+    hipLaunchKernelGGL((k_jacobi_res_restrict<CX, CY, NT, HALO>),                                              \
+        dim3(((C.w + CX - 1) / CX) * ((C.h + CY - 1) / CY)), dim3(NT), 0, st, L, cur, rhs, shift, weights, nu, C, \
+        rx, ry, status)
+#define FS_JRR(CX, CY, NT)                      \
+    do {                                        \
+        if (nu <= kJacobiShortHalo)             \
+            FS_JRR_H(CX, CY, NT, kJacobiShortHalo); \
+        else                                    \
+            FS_JRR_H(CX, CY, NT, kJacobiMaxSweeps); \
+    } while (0)
     switch (variant) {
     case 1: FS_JRR(16, 16, 512); break;
     case 2: FS_JRR(32, 16, 1024); break;
     default: FS_JRR(16, 8, 512); break;
     }
 #undef FS_JRR
+#undef FS_JRR_H
 }
 
 void launch_res_restrict(hipStream_t st, const LevelView& F, int cur, const float* rhs, const double* shift,

@@ -602,7 +602,7 @@ fs_status fs_server_submit(fs_server* server, const fs_request* request, long lo
         if (q.smoother_sweeps != 0) {
             const bool cheb = q.pressure_smoother == FS_SMOOTHER_CHEBYSHEV;
             need(q.pressure_smoother == FS_SMOOTHER_RED_BLACK || cheb, FS_E_INVALID_ARGUMENT, "unknown pressure smoother");
-            need(q.smoother_sweeps >= 1 && q.smoother_sweeps <= (cheb ? 5 : 4) && q.coarse_sweeps >= 0 &&
+            need(q.smoother_sweeps >= 1 && q.smoother_sweeps <= (cheb ? 8 : 4) && q.coarse_sweeps >= 0 &&
                      q.coarse_sweeps <= 4,
                 FS_E_INVALID_ARGUMENT, "invalid smoother sweep counts");
             need(!cheb || q.multigrid_precision == FS_MG_FP32, FS_E_INVALID_ARGUMENT,

@@ -554,7 +554,7 @@ class FluidGridHIP:
 
     def setPressureSmoother(self, kind: int, sweeps: int, coarse_sweeps: int = 4):
         """Smoother of the multigrid levels above 4096 cells: FS_SMOOTHER_RED_BLACK (1..4 sweeps per side, default
-        4) or FS_SMOOTHER_CHEBYSHEV (Chebyshev-weighted damped Jacobi, 1..5 sweeps; FP32 levels only); the smaller
+        4) or FS_SMOOTHER_CHEBYSHEV (Chebyshev-weighted damped Jacobi, 1..8 sweeps; FP32 levels only); the smaller
         levels run `coarse_sweeps` (1..4) red-black sweeps."""
         call("fs_set_pressure_smoother", self._h, int(kind), int(sweeps), int(coarse_sweeps))
 

@@ -115,7 +115,7 @@ typedef enum fs_mg_precision {
  * 4096 cells run red-black Gauss-Seidel with their own sweep count (coarse sweeps, 1..4).
  *   FS_SMOOTHER_RED_BLACK  red-black Gauss-Seidel, 1..4 sweeps per side (default: 4, the reference's cycle);
  *   FS_SMOOTHER_CHEBYSHEV  damped Jacobi with the weights 1 / (1.1 + 0.9 cos(pi (k + 1/2) / n)), k = 0..n-1,
- *                          in order before the coarse correction and reversed after it, n = 1..5 sweeps per
+ *                          in order before the coarse correction and reversed after it, n = 1..8 sweeps per
  *                          side; FP32 multigrid levels only.  A cell without conductance (solid masks) is
  *                          set to zero, as the red-black sweep sets it. */
 typedef enum fs_pressure_smoother { FS_SMOOTHER_RED_BLACK = 0, FS_SMOOTHER_CHEBYSHEV = 1 } fs_pressure_smoother;
@@ -212,7 +212,7 @@ fs_status fs_set_relative_tolerance(fs_grid* grid, int enabled);
 fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int sweeps, int coarse_sweeps);
 fs_status fs_get_pressure_smoother(const fs_grid* grid, fs_pressure_smoother* kind, int* sweeps, int* coarse_sweeps);
 /* Autotuner of the smoother and its sweep counts (no reference counterpart): projects the grid's current
- * state with each candidate (red-black 2, 3, 4 and Chebyshev 3, 4, 5 sweeps, each with 4 and with 2 coarse
+ * state with each candidate (red-black 2, 3, 4 and Chebyshev 3, 4, 5, 6, 8 sweeps, each with 4 and with 2 coarse
  * sweeps; each from the same state, restored after every trial: the grid's state is unchanged on return)
  * with fs_project's arguments, times each projection after one warm-up, and selects the fastest candidate
  * whose projection converges.  kind, sweeps, coarse_sweeps and ms (each may be NULL) receive the choice and
@@ -563,7 +563,7 @@ typedef struct fs_request {
     int relative_tolerance;         /* nonzero: divergence_tolerance is relative (fs_set_relative_tolerance) */
     /* pressure smoother (fs_set_pressure_smoother); smoother_sweeps 0 keeps the reference's V(4,4) */
     int pressure_smoother;          /* fs_pressure_smoother */
-    int smoother_sweeps;            /* 0 or 1..5 on the levels above 4096 cells */
+    int smoother_sweeps;            /* 0 or 1..4 (red-black) / 1..8 (Chebyshev) on the levels above 4096 cells */
     int coarse_sweeps;              /* 0 (= 4) or 1..4 on the smaller levels */
 } fs_request;
 

@@ -206,7 +206,7 @@ def test_requests_with_a_smoother_choice():
     probe = request(150, 110, 50, steps=4, viscosity=0.01, sigma=0.02, pressure_smoother=FS_SMOOTHER_CHEBYSHEV,
                     smoother_sweeps=3, coarse_sweeps=2)
     with Server(lanes=2) as srv:
-        for bad in (dict(smoother_sweeps=6), dict(coarse_sweeps=5), dict(multigrid_precision=FS_MG_BF16)):
+        for bad in (dict(smoother_sweeps=9), dict(coarse_sweeps=5), dict(multigrid_precision=FS_MG_BF16)):
             with pytest.raises(ValueError):
                 srv.submit(request(64, 64, 51, pressure_smoother=FS_SMOOTHER_CHEBYSHEV,
                                    **{"smoother_sweeps": 3, **bad}))

@@ -30,7 +30,8 @@ pytestmark = pytest.mark.gpu
 
 # (kind, sweeps above 4096 cells, red-black sweeps below)
 CHOICES = [(FS_SMOOTHER_CHEBYSHEV, 3, 4), (FS_SMOOTHER_CHEBYSHEV, 5, 4), (FS_SMOOTHER_RED_BLACK, 2, 4),
-           (FS_SMOOTHER_RED_BLACK, 3, 4), (FS_SMOOTHER_CHEBYSHEV, 3, 2), (FS_SMOOTHER_RED_BLACK, 4, 1)]
+           (FS_SMOOTHER_RED_BLACK, 3, 4), (FS_SMOOTHER_CHEBYSHEV, 3, 2), (FS_SMOOTHER_RED_BLACK, 4, 1),
+           (FS_SMOOTHER_CHEBYSHEV, 6, 4), (FS_SMOOTHER_CHEBYSHEV, 8, 4)]
 
 
 @pytest.fixture
@@ -97,10 +98,11 @@ def test_vcycle_matches_the_oracle(O, case, kind, sweeps, coarse):
     g.destroy()
 
 
-def test_every_jacobi_tile_computes_the_same_values():
+@pytest.mark.parametrize("sweeps", [5, 8])   # the kernels with a 5-cell and with an 8-cell halo
+def test_every_jacobi_tile_computes_the_same_values(sweeps):
     wl = WL.sloped_pool(300, 200, 4.0)
     g = hip_grid(wl)
-    g.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 5)
+    g.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, sweeps)
     i = np.arange(300 * 200)
     a = np.sin(i * 0.31) + np.cos(i * 0.073)
     a -= a.mean()
@@ -185,7 +187,7 @@ def test_batched_solves_run_red_black_smoothing():
     with pytest.raises(ValueError):
         a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 5)
     with pytest.raises(ValueError):
-        a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 6)
+        a.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 9)
     with pytest.raises(ValueError):
         a.setPressureSmoother(FS_SMOOTHER_RED_BLACK, 4, 0)
     a.destroy()
