@@ -252,7 +252,7 @@ SERVING_PRECISIONS = {"default": None, "fp32": 0, "fp16": 1, "bf16": 2}
 
 
 def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, batch_cells=0,
-                compare_batch_cells=0, smoother=None, precision=None):
+                compare_batch_cells=0, smoother=None, precision=None, viscosity_sweeps=0):
     """Config C5 on one GPU: a seeded Poisson stream of `count` simulations (sides log-uniform in
     [128, max_size], transport + viscosity + surface tension + MG-PCG, `steps` resident steps each).
     It is first served one request at a time on one stream with FP32 multigrid levels (serial plain), then
@@ -266,7 +266,8 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
     (fs_batch_step: one workgroup per instance in the persistent solves).  `smoother` (kind, sweeps, coarse
     sweeps) is the pressure smoother of the served FP32-level requests (one more run at the last rate then
     serves them with the reference's cycle and bfloat16 conductances); the serial pass keeps the
-    reference's V(4,4) cycle.  `precision` (an fs_mg_precision) overrides the served requests' multigrid storage."""
+    reference's V(4,4) cycle.  `precision` (an fs_mg_precision) overrides the served requests' multigrid storage.
+    `viscosity_sweeps` (fs_set_viscosity_smoother) is the viscosity smoother of the same requests."""
     from fluidsim_amd import FS_MG_BF16, FS_MG_FP32
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
@@ -278,7 +279,8 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
             p = precision if precision is not None else (FS_MG_FP32 if small or choice else FS_MG_BF16)
             extra = {}
             if choice and not small:
-                extra = dict(pressure_smoother=choice[0], smoother_sweeps=choice[1], coarse_sweeps=choice[2])
+                extra = dict(pressure_smoother=choice[0], smoother_sweeps=choice[1], coarse_sweeps=choice[2],
+                             viscosity_sweeps=viscosity_sweeps)
             out.append((t, Request(multigrid_precision=p, **kw, **extra)))
         return out
 
@@ -362,6 +364,7 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
         "lanes": lanes,
         "batch_cells": batch_cells,
         "served_smoother": list(smoother) if smoother else None,
+        "served_viscosity_sweeps": viscosity_sweeps if smoother else 0,
         "batched_requests": sum(r.width * r.height <= batch_cells for _, r in base),
         "cells_mean": float(np.mean([r.width * r.height for _, r in base])),
         "steps": total_steps,
@@ -712,6 +715,9 @@ def main():
                     help="C5: pressure smoother of the served requests (FP32 levels), red_black:N:C or "
                          "chebyshev:N:C; empty: the reference's V(4,4) with bfloat16 conductances.  The default, "
                          "8 Chebyshev sweeps and 3 coarse sweeps, is the fastest measured on this stream")
+    ap.add_argument("--serving-viscosity-sweeps", type=int, default=0, choices=(0, 2, 4, 6),
+                    help="C5: viscosity smoother of the served requests with a pressure smoother choice "
+                         "(fs_set_viscosity_smoother: 0 the reference's V(2,2), else Chebyshev sweeps per side)")
     ap.add_argument("--serving-compare-batch-cells", type=int, default=65536,
                     help="C5: one extra run at the last rate with this batch size, for comparison (0: skip)")
     ap.add_argument("--serving-repeats", type=int, default=3, help="runs per pass; the median is reported")
@@ -758,6 +764,7 @@ def main():
                                      compare_batch_cells=args.serving_compare_batch_cells,
                                      smoother=parse_smoother(args.serving_smoother),
                                      precision=SERVING_PRECISIONS[args.serving_precision],
+                                     viscosity_sweeps=args.serving_viscosity_sweeps,
                                      repeats=max(1, args.serving_repeats))), flush=True)
         return
 
@@ -941,6 +948,7 @@ def main():
                                      compare_batch_cells=args.serving_compare_batch_cells,
                                      smoother=parse_smoother(args.serving_smoother),
                                      precision=SERVING_PRECISIONS[args.serving_precision],
+                                     viscosity_sweeps=args.serving_viscosity_sweeps,
                                      repeats=max(1, args.serving_repeats))
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -111,6 +111,7 @@ class FsRequest(C.Structure):
         ("liquid_mass", C.POINTER(C.c_double)), ("gas_mass", C.POINTER(C.c_double)),
         ("multigrid_precision", C.c_int), ("solid", C.c_void_p), ("relative_tolerance", C.c_int),
         ("pressure_smoother", C.c_int), ("smoother_sweeps", C.c_int), ("coarse_sweeps", C.c_int),
+        ("viscosity_sweeps", C.c_int),
     ]
 
 
@@ -188,6 +189,7 @@ SIGNATURES = {
     "fs_commit_edits": [_vp, C.c_int, C.POINTER(FsCellEdit), C.POINTER(C.c_double)],
     "fs_advect": [_vp, C.c_double, _ip],
     "fs_apply_viscosity": [_vp, C.c_double, _ip],
+    "fs_set_viscosity_smoother": [_vp, C.c_int],
     "fs_apply_surface_tension": [_vp, C.c_double],
     "fs_capillary_geometry": [_vp, _dp, _dp],
     "fs_set_component_viscosity": [_vp, C.c_int, C.c_int, C.c_double],

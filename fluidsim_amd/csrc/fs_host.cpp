@@ -497,6 +497,7 @@ struct fs_grid {
     DevBuf<unsigned char> tr_split;
     DevBuf<float> calib_src, calib_dst;   // 64 MiB copy-bandwidth reference (allocated on first use)
     ViscositySolver* visc = nullptr;       // implicit viscosity workspace (created on first use)
+    int vis_sweeps = 0;                    // fs_set_viscosity_smoother (0: the reference's V(2,2))
     DevBuf<double2> st_heights, st_orient, st_raw, st_curv;   // capillary geometry (first use)
     DevBuf<float> x, r, d, q, zdiag;
     DevBuf<unsigned char> solid;   // solid-mask extension (empty: no mask)
@@ -2055,6 +2056,22 @@ fs_status fs_set_pressure_smoother(fs_grid* grid, fs_pressure_smoother kind, int
     });
 }
 
+fs_status fs_set_viscosity_smoother(fs_grid* grid, int chebyshev_sweeps)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(chebyshev_sweeps == 0 || chebyshev_sweeps == 2 || chebyshev_sweeps == 4 || chebyshev_sweeps == 6,
+            FS_E_INVALID_ARGUMENT, "viscosity smoother sweeps must be 0, 2, 4 or 6");
+        if (chebyshev_sweeps == grid->vis_sweeps)
+            return;
+        set_device(grid->ctx);
+        grid->drain();
+        grid->vis_sweeps = chebyshev_sweeps;
+        if (grid->visc)
+            viscosity_set_smoother(grid->visc, chebyshev_sweeps);
+    });
+}
+
 // The state a projection changes (velocities, both pressure histories, the iteration prediction), kept on
 // the device so that autotuning trials start from the same state and leave it unchanged.
 struct ProjectionState {
@@ -2586,6 +2603,7 @@ static double* blend_viscosity_material(fs_grid& g)
     if (!g.visc) {
         g.visc = viscosity_create(g.W, g.H);
         g.bytes += viscosity_bytes(g.visc);
+        viscosity_set_smoother(g.visc, g.vis_sweeps);
     }
     VisMixture mix{};
     for (size_t k = 0; k < g.rho_l.size(); ++k) {

@@ -321,6 +321,8 @@ int viscosity_solve_scratch(int device, hipStream_t st, int width, int height, d
 void viscosity_destroy(ViscositySolver* s);
 size_t viscosity_bytes(const ViscositySolver* s);
 int viscosity_levels(const ViscositySolver* s);
+// 0: the reference's V(2,2) damped Jacobi; 2, 4, 6: Chebyshev-weighted sweeps per side (throws otherwise)
+void viscosity_set_smoother(ViscositySolver* s, int chebyshev_sweeps);
 double* viscosity_material_buffer(ViscositySolver* s);   // 2 * W * H doubles
 // Backward-Euler viscosity on device FP64 u ((W+1) x H) and v (W x (H+1)) with a device FP64 (rho, mu)
 // material; maximum < 0 is unbounded.  Returns 0 converged (or nothing to do), 1 non-finite input,

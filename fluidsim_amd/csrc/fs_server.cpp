@@ -334,7 +334,8 @@ struct fs_server {
                ok(fs_set_relative_tolerance(g, p.relative_tolerance)) &&
                (p.smoother_sweeps == 0 ||
                    ok(fs_set_pressure_smoother(g, (fs_pressure_smoother)p.pressure_smoother, p.smoother_sweeps,
-                       p.coarse_sweeps ? p.coarse_sweeps : 4)));
+                       p.coarse_sweeps ? p.coarse_sweeps : 4))) &&
+               ok(fs_set_viscosity_smoother(g, p.viscosity_sweeps));
     }
 
     // Records one finished step of `r` (ms of lane time); ends the request when it failed, was not
@@ -608,6 +609,8 @@ fs_status fs_server_submit(fs_server* server, const fs_request* request, long lo
             need(!cheb || q.multigrid_precision == FS_MG_FP32, FS_E_INVALID_ARGUMENT,
                 "the Chebyshev smoother runs FP32 multigrid levels");
         }
+        need(q.viscosity_sweeps == 0 || q.viscosity_sweeps == 2 || q.viscosity_sweeps == 4 || q.viscosity_sweeps == 6,
+            FS_E_INVALID_ARGUMENT, "viscosity smoother sweeps must be 0, 2, 4 or 6");
         if (q.solid) {
             const std::string problem = fsim::solid_mask_problem(q.solid, q.width, q.height);
             need(problem.empty(), FS_E_INVALID_ARGUMENT, problem);

@@ -65,6 +65,15 @@ struct VLev {
     float2* r;
 };
 
+// Jacobi weights of a level's smoothing, per side: the reference's two sweeps of weight 1/2, or `sweeps`
+// (2, 4 or 6) Chebyshev-weighted sweeps (fs_set_viscosity_smoother).  Pre-smoothing takes w[0..sweeps-1],
+// post-smoothing the same weights in reverse.
+constexpr int kVisMaxSweeps = 6;
+struct VisSmooth {
+    int sweeps;
+    float w[kVisMaxSweeps];
+};
+
 __device__ __forceinline__ int sidx(int w, int h, int x, int y)
 {
     return min(max(y, 0), h) * (w + 1) + min(max(x, 0), w);
@@ -79,7 +88,7 @@ __device__ __forceinline__ bool stopped(const VisScalars* S)
 }
 
 // Index of slot (x, y) in an array: the global slot field clamped like the reference's accessors, or an
-// LDS tile that holds the clamped values (see k_vis_smooth2).  The stencils below are written once over
+// LDS tile that holds the clamped values (see k_vis_smooth).  The stencils below are written once over
 // it, so the fused and the per-sweep kernels evaluate the same expressions.
 struct GlobalSlots {
     int w, h;
@@ -334,32 +343,33 @@ __global__ __launch_bounds__(kVT) void k_vis_restrict(VLev F, VLev C, const VisS
     C.b[p] = make_float2(val[0], val[1]);
 }
 
-// Two sweeps of the "viscosity symmetric smoother" (ViscosityKernels.h:214-236, damped Jacobi with weight
-// 1/2) fused on an LDS tile: pre-smoothing from zero (result in L.x), or (POST) the "viscosity
-// prolongation" F.x + P C.x on valid faces (ViscosityKernels.h:359-391) followed by two sweeps (result in
-// L.xn).  The tile
-// holds a two-slot halo of coefficients, rhs and the starting solution, each slot loaded from its clamped
-// position; every stencil read maps through the same clamp to the tile, so each sweep reads exactly the
-// values a per-sweep kernel would read from the slot field (results identical to one launch per sweep).
-constexpr int kSX = 32, kSY = 16, kSH = 2;                   // interior slots per workgroup, halo
-constexpr int kSLX = kSX + 2 * kSH, kSLY = kSY + 2 * kSH;     // 36 x 20
-constexpr int kSN = kSLX * kSLY;
+// NS sweeps of the "viscosity symmetric smoother" (ViscosityKernels.h:214-236: damped Jacobi, weight 1/2 in
+// the reference's two sweeps per side; the weights of VisSmooth otherwise) fused on an LDS tile:
+// pre-smoothing from zero (result in L.x), or (POST) the "viscosity prolongation" F.x + P C.x on valid
+// faces (ViscosityKernels.h:359-391) followed by the sweeps (result in L.xn).  The tile holds an NS-slot
+// halo of coefficients, rhs and the starting solution, each slot loaded from its clamped position; sweep k
+// covers the tile less k + 1 rings, the last one the interior, and every stencil read maps through the same
+// clamp to the tile, so each sweep reads exactly the values a per-sweep kernel would read from the slot
+// field (results identical to one launch per sweep).
+constexpr int kSX = 32, kSY = 16;   // interior slots per workgroup
 
-template <bool POST>
-__global__ __launch_bounds__(kVT) void k_vis_smooth2(VLev L, VLev C, const float2* b, const VisScalars* S)
+template <bool POST, int NS>
+__global__ __launch_bounds__(kVT) void k_vis_smooth(VLev L, VLev C, const float2* b, const VisScalars* S,
+    const VisSmooth M)
 {
-    __shared__ float4 sc[kSN];
-    __shared__ float2 sb[kSN];
-    __shared__ float2 s0[kSN];   // starting solution of the first sweep (POST: prolonged)
-    __shared__ float2 s1[kSN];   // after the first sweep
+    constexpr int LX = kSX + 2 * NS, LY = kSY + 2 * NS, LN = LX * LY;   // NS 2: 36 x 20, NS 6: 44 x 28
+    __shared__ float4 sc[LN];
+    __shared__ float2 sb[LN];
+    __shared__ float2 s0[LN];   // starting solution of the first sweep (POST: prolonged); even sweeps' input
+    __shared__ float2 s1[LN];   // odd sweeps' input
     if (stopped(S))
         return;
     const float dt = S->dt;
     const int w = L.w, h = L.h;
-    const int x0 = blockIdx.x * kSX - kSH, y0 = blockIdx.y * kSY - kSH;
-    const auto at = [=](int x, int y) { return (min(max(y, 0), h) - y0) * kSLX + (min(max(x, 0), w) - x0); };
-    for (int e = threadIdx.x; e < kSN; e += kVT) {
-        const int gx = x0 + e % kSLX, gy = y0 + e / kSLX;
+    const int x0 = blockIdx.x * kSX - NS, y0 = blockIdx.y * kSY - NS;
+    const auto at = [=](int x, int y) { return (min(max(y, 0), h) - y0) * LX + (min(max(x, 0), w) - x0); };
+    for (int e = threadIdx.x; e < LN; e += kVT) {
+        const int gx = x0 + e % LX, gy = y0 + e / LX;
         const int g = sidx(w, h, gx, gy);
         sc[e] = L.coef[g];
         sb[e] = b[g];
@@ -368,9 +378,9 @@ __global__ __launch_bounds__(kVT) void k_vis_smooth2(VLev L, VLev C, const float
     }
     __syncthreads();
     if (POST) {
-        // prolongation on every in-field slot of the tile (interior and both halo rings)
-        for (int e = threadIdx.x; e < kSN; e += kVT) {
-            const int x = x0 + e % kSLX, y = y0 + e / kSLX;
+        // prolongation on every in-field slot of the tile (interior and every halo ring)
+        for (int e = threadIdx.x; e < LN; e += kVT) {
+            const int x = x0 + e % LX, y = y0 + e / LX;
             if (x < 0 || y < 0 || x > w || y > h)
                 continue;
             const float2 sv = s0[e];
@@ -392,42 +402,50 @@ __global__ __launch_bounds__(kVT) void k_vis_smooth2(VLev L, VLev C, const float
         }
         __syncthreads();
     }
-    // first sweep on the interior and the inner halo ring
-    constexpr int RX = kSX + 2, RY = kSY + 2;
-    for (int e = threadIdx.x; e < RX * RY; e += kVT) {
-        const int x = x0 + 1 + e % RX, y = y0 + 1 + e / RX;
-        if (x < 0 || y < 0 || x > w || y > h)
-            continue;
-        const int i = at(x, y);
-        const float2 d = diagonal_at(sc, at, x, y, dt);
-        const float2 rhs = sb[i];
-        float2 v;
-        if (!POST) {
-            v.x = 0.5f * rhs.x / d.x;
-            v.y = 0.5f * rhs.y / d.y;
-        } else {
-            const float2 q = product_at(sc, s0, at, w, h, x, y, dt);
-            const float2 sv = s0[i];
-            v.x = sv.x + 0.5f * (rhs.x - q.x) / d.x;
-            v.y = sv.y + 0.5f * (rhs.y - q.y) / d.y;
+    // sweeps 0..NS-2 on the tile less k + 1 rings
+#pragma unroll
+    for (int k = 0; k + 1 < NS; ++k) {
+        const float wk = M.w[POST ? NS - 1 - k : k];
+        const float2* src = (k & 1) ? s1 : s0;
+        float2* dst = (k & 1) ? s0 : s1;
+        const int RX = LX - 2 * (k + 1), RY = LY - 2 * (k + 1);
+        for (int e = threadIdx.x; e < RX * RY; e += kVT) {
+            const int x = x0 + k + 1 + e % RX, y = y0 + k + 1 + e / RX;
+            if (x < 0 || y < 0 || x > w || y > h)
+                continue;
+            const int i = at(x, y);
+            const float2 d = diagonal_at(sc, at, x, y, dt);
+            const float2 rhs = sb[i];
+            float2 v;
+            if (!POST && k == 0) {
+                v.x = wk * rhs.x / d.x;
+                v.y = wk * rhs.y / d.y;
+            } else {
+                const float2 q = product_at(sc, src, at, w, h, x, y, dt);
+                const float2 sv = src[i];
+                v.x = sv.x + wk * (rhs.x - q.x) / d.x;
+                v.y = sv.y + wk * (rhs.y - q.y) / d.y;
+            }
+            dst[i] = make_float2(valid_u(w, h, x, y) ? v.x : 0.0f, valid_v(w, h, x, y) ? v.y : 0.0f);
         }
-        s1[i] = make_float2(valid_u(w, h, x, y) ? v.x : 0.0f, valid_v(w, h, x, y) ? v.y : 0.0f);
+        __syncthreads();
     }
-    __syncthreads();
-    // second sweep on the interior, written out
+    // the last sweep on the interior, written out
+    const float wl = M.w[POST ? 0 : NS - 1];
+    const float2* src = ((NS - 1) & 1) ? s1 : s0;
     float2* out = POST ? L.xn : L.x;
     for (int e = threadIdx.x; e < kSX * kSY; e += kVT) {
-        const int x = x0 + kSH + e % kSX, y = y0 + kSH + e / kSX;
+        const int x = x0 + NS + e % kSX, y = y0 + NS + e / kSX;
         if (x > w || y > h)
             continue;
         const int i = at(x, y);
         const float2 d = diagonal_at(sc, at, x, y, dt);
         const float2 rhs = sb[i];
-        const float2 q = product_at(sc, s1, at, w, h, x, y, dt);
-        const float2 sv = s1[i];
+        const float2 q = product_at(sc, src, at, w, h, x, y, dt);
+        const float2 sv = src[i];
         float2 v;
-        v.x = sv.x + 0.5f * (rhs.x - q.x) / d.x;
-        v.y = sv.y + 0.5f * (rhs.y - q.y) / d.y;
+        v.x = sv.x + wl * (rhs.x - q.x) / d.x;
+        v.y = sv.y + wl * (rhs.y - q.y) / d.y;
         out[y * (w + 1) + x] = make_float2(valid_u(w, h, x, y) ? v.x : 0.0f, valid_v(w, h, x, y) ? v.y : 0.0f);
     }
 }
@@ -796,6 +814,7 @@ struct BatchVisDesc {
     double* u;
     double* v;
     VisScalars* S;        // after the initial reduction; receives status and iterations
+    VisSmooth M;
 };
 
 // FP64 sum over the workgroup in a fixed order, returned to every thread.
@@ -820,34 +839,8 @@ __device__ __forceinline__ void slot_xy(int e, int w, int& x, int& y)
     x = e - y * (w + 1);
 }
 
-// Two sweeps from zero (pre) on a level: xn = sweep(0), x = sweep(xn).
-__device__ void bv_pre(const VLev& L, const float2* b, float dt)
-{
-    const int n = (L.w + 1) * (L.h + 1);
-    for (int e = threadIdx.x; e < n; e += kBVT) {
-        int x, y;
-        slot_xy(e, L.w, x, y);
-        const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
-        const float2 rhs = b[e];
-        L.xn[e] = make_float2(valid_u(L.w, L.h, x, y) ? 0.5f * rhs.x / d.x : 0.0f,
-            valid_v(L.w, L.h, x, y) ? 0.5f * rhs.y / d.y : 0.0f);
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < n; e += kBVT) {
-        int x, y;
-        slot_xy(e, L.w, x, y);
-        const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
-        const float2 rhs = b[e];
-        const float2 q = product(L.coef, L.xn, L.w, L.h, x, y, dt);
-        const float2 sv = L.xn[e];
-        L.x[e] = make_float2(valid_u(L.w, L.h, x, y) ? sv.x + 0.5f * (rhs.x - q.x) / d.x : 0.0f,
-            valid_v(L.w, L.h, x, y) ? sv.y + 0.5f * (rhs.y - q.y) / d.y : 0.0f);
-    }
-    __syncthreads();
-}
-
 // One sweep src -> dst.
-__device__ void bv_sweep(const VLev& L, const float2* b, const float2* src, float2* dst, float dt)
+__device__ void bv_sweep(const VLev& L, const float2* b, const float2* src, float2* dst, float dt, float wt)
 {
     const int n = (L.w + 1) * (L.h + 1);
     for (int e = threadIdx.x; e < n; e += kBVT) {
@@ -857,10 +850,36 @@ __device__ void bv_sweep(const VLev& L, const float2* b, const float2* src, floa
         const float2 rhs = b[e];
         const float2 q = product(L.coef, src, L.w, L.h, x, y, dt);
         const float2 sv = src[e];
-        dst[e] = make_float2(valid_u(L.w, L.h, x, y) ? sv.x + 0.5f * (rhs.x - q.x) / d.x : 0.0f,
-            valid_v(L.w, L.h, x, y) ? sv.y + 0.5f * (rhs.y - q.y) / d.y : 0.0f);
+        dst[e] = make_float2(valid_u(L.w, L.h, x, y) ? sv.x + wt * (rhs.x - q.x) / d.x : 0.0f,
+            valid_v(L.w, L.h, x, y) ? sv.y + wt * (rhs.y - q.y) / d.y : 0.0f);
     }
     __syncthreads();
+}
+
+// Pre-smoothing from zero on a level (an even number of sweeps): xn = sweep(0), x = sweep(xn), xn = ...,
+// ending in x.
+__device__ void bv_pre(const VLev& L, const float2* b, float dt, const VisSmooth& M)
+{
+    const int n = (L.w + 1) * (L.h + 1);
+    for (int e = threadIdx.x; e < n; e += kBVT) {
+        int x, y;
+        slot_xy(e, L.w, x, y);
+        const float2 d = diagonal(L.coef, L.w, L.h, x, y, dt);
+        const float2 rhs = b[e];
+        L.xn[e] = make_float2(valid_u(L.w, L.h, x, y) ? M.w[0] * rhs.x / d.x : 0.0f,
+            valid_v(L.w, L.h, x, y) ? M.w[0] * rhs.y / d.y : 0.0f);
+    }
+    __syncthreads();
+    for (int k = 1; k < M.sweeps; ++k)
+        bv_sweep(L, b, (k & 1) ? L.xn : L.x, (k & 1) ? L.x : L.xn, dt, M.w[k]);
+}
+
+// Post-smoothing after the prolongation into x (an even number of sweeps, the weights in reverse): xn =
+// sweep(x), x = sweep(xn), ..., ending in x.
+__device__ void bv_post(const VLev& L, const float2* b, float dt, const VisSmooth& M)
+{
+    for (int k = 0; k < M.sweeps; ++k)
+        bv_sweep(L, b, (k & 1) ? L.xn : L.x, (k & 1) ? L.x : L.xn, dt, M.w[M.sweeps - 1 - k]);
 }
 
 __device__ void bv_residual_restrict(const VLev& F, const float2* b, const VLev& C, float dt)
@@ -963,15 +982,14 @@ __device__ void bv_cycle(const BatchVisDesc& D, float dt, float2* vel, float4* c
     const int nl = D.levels;
     for (int l = 0; l + 1 < nl; ++l) {
         const float2* b = l == 0 ? (const float2*)D.r : D.lev[l].b;
-        bv_pre(D.lev[l], b, dt);
+        bv_pre(D.lev[l], b, dt, D.M);
         bv_residual_restrict(D.lev[l], b, D.lev[l + 1], dt);
     }
     bv_coarse(D.lev[nl - 1], nl == 1 ? (const float2*)D.r : D.lev[nl - 1].b, dt, vel, cof);
     for (int l = nl - 2; l >= 0; --l) {
         const float2* b = l == 0 ? (const float2*)D.r : D.lev[l].b;
         bv_prolong(D.lev[l], D.lev[l + 1]);
-        bv_sweep(D.lev[l], b, D.lev[l].x, D.lev[l].xn, dt);
-        bv_sweep(D.lev[l], b, D.lev[l].xn, D.lev[l].x, dt);
+        bv_post(D.lev[l], b, dt, D.M);
     }
 }
 
@@ -986,6 +1004,7 @@ constexpr int kVisTailSlots = 1024;
 struct VisTail {
     int levels;
     VLev lev[kMaxVisLevels];
+    VisSmooth M;
 };
 
 __global__ __launch_bounds__(kBVT) void k_vis_tail(const VisTail T, const float2* b0, const VisScalars* S)
@@ -998,15 +1017,14 @@ __global__ __launch_bounds__(kBVT) void k_vis_tail(const VisTail T, const float2
     const int nl = T.levels;
     for (int l = 0; l + 1 < nl; ++l) {
         const float2* b = l == 0 ? b0 : T.lev[l].b;
-        bv_pre(T.lev[l], b, dt);
+        bv_pre(T.lev[l], b, dt, T.M);
         bv_residual_restrict(T.lev[l], b, T.lev[l + 1], dt);
     }
     bv_coarse(T.lev[nl - 1], nl == 1 ? b0 : T.lev[nl - 1].b, dt, vel, cof);
     for (int l = nl - 2; l >= 0; --l) {
         const float2* b = l == 0 ? b0 : T.lev[l].b;
         bv_prolong(T.lev[l], T.lev[l + 1]);
-        bv_sweep(T.lev[l], b, T.lev[l].x, T.lev[l].xn, dt);
-        bv_sweep(T.lev[l], b, T.lev[l].xn, T.lev[l].x, dt);
+        bv_post(T.lev[l], b, dt, T.M);
     }
 }
 
@@ -1306,6 +1324,40 @@ struct ViscositySolver {
         return -1;
     }
 
+    // Smoothing weights of every level (fs_set_viscosity_smoother): the reference's two sweeps of weight 1/2,
+    // or 2, 4 or 6 Chebyshev-weighted sweeps, 1 / (2.125 + 1.875 cos(pi (k + 1/2) / n)) rounded to float --
+    // the reciprocals of the Chebyshev nodes of [0.25, 4], 4 bounding the spectrum of D^-1 A (Gershgorin).
+    VisSmooth smooth{2, {0.5f, 0.5f, 0.0f, 0.0f, 0.0f, 0.0f}};
+    int smooth_chebyshev = 0;
+
+    void set_smoother(int chebyshev_sweeps)
+    {
+        if (chebyshev_sweeps != 0 && chebyshev_sweeps != 2 && chebyshev_sweeps != 4 && chebyshev_sweeps != 6)
+            throw std::invalid_argument("viscosity smoother sweeps must be 0, 2, 4 or 6");
+        if (chebyshev_sweeps == smooth_chebyshev)
+            return;
+        VisSmooth m{2, {0.5f, 0.5f, 0.0f, 0.0f, 0.0f, 0.0f}};
+        if (chebyshev_sweeps) {
+            const double pi = 3.14159265358979323846;
+            m.sweeps = chebyshev_sweeps;
+            for (int k = 0; k < chebyshev_sweeps; ++k)
+                m.w[k] = (float)(1.0 / (2.125 + 1.875 * std::cos(pi * (k + 0.5) / chebyshev_sweeps)));
+        }
+        smooth = m;
+        smooth_chebyshev = chebyshev_sweeps;
+        drop_batch_graph();   // the captured launches carry the weights
+    }
+
+    template <bool POST> void launch_smooth(hipStream_t st, dim3 g2, const VLev& L, const VLev& C, const float2* b)
+    {
+        if (smooth.sweeps == 6)
+            hipLaunchKernelGGL((k_vis_smooth<POST, 6>), g2, dim3(kVT), 0, st, L, C, b, S.p, smooth);
+        else if (smooth.sweeps == 4)
+            hipLaunchKernelGGL((k_vis_smooth<POST, 4>), g2, dim3(kVT), 0, st, L, C, b, S.p, smooth);
+        else
+            hipLaunchKernelGGL((k_vis_smooth<POST, 2>), g2, dim3(kVT), 0, st, L, C, b, S.p, smooth);
+    }
+
     void cycle(hipStream_t st, int l, const float2* b)
     {
         VLev L = view(l);
@@ -1319,21 +1371,23 @@ struct ViscositySolver {
             T.levels = (int)levels.size() - l;
             for (int k = 0; k < T.levels; ++k)
                 T.lev[k] = view(l + k);
+            T.M = smooth;
             hipLaunchKernelGGL(k_vis_tail, dim3(1), dim3(kBVT), 0, st, T, b, S.p);
             return;
         }
         const dim3 g = slot_grid(L.w, L.h);
         const dim3 g2((L.w + 1 + kSX - 1) / kSX, (L.h + 1 + kSY - 1) / kSY);
-        // pre-smoothing: two sweeps from zero in one launch (the result lands in L.x, as after two swaps)
-        hipLaunchKernelGGL(k_vis_smooth2<false>, g2, dim3(kVT), 0, st, L, L, b, S.p);
+        // pre-smoothing: every sweep from zero in one launch (the result lands in L.x, as after an even
+        // number of swaps)
+        launch_smooth<false>(st, g2, L, L, b);
         hipLaunchKernelGGL(k_vis_residual, g, dim3(kVT), 0, st, L, b, S.p);
         VLev C = view(l + 1);
         hipLaunchKernelGGL(k_vis_restrict, slot_grid(C.w, C.h), dim3(kVT), 0, st, L, C, S.p);
         cycle(st, l + 1, C.b);
         C = view(l + 1);
-        // prolongation + two sweeps in one launch into L.xn: one swap, as after the three of the
+        // prolongation + every sweep in one launch into L.xn: one swap, as after the odd number of the
         // per-sweep kernels
-        hipLaunchKernelGGL(k_vis_smooth2<true>, g2, dim3(kVT), 0, st, L, C, b, S.p);
+        launch_smooth<true>(st, g2, L, C, b);
         swap_x(l);
     }
 
@@ -1433,6 +1487,7 @@ struct ViscositySolver {
         desc->maximum = maximum;
         for (size_t l = 0; l < levels.size(); ++l)
             desc->lev[l] = view((int)l);
+        desc->M = smooth;
         desc->v0 = v0.p;
         desc->r = r.p;
         desc->d = d.p;
@@ -1606,6 +1661,7 @@ ViscositySolver* viscosity_create(int width, int height) { return new ViscosityS
 void viscosity_destroy(ViscositySolver* s) { delete s; }
 size_t viscosity_bytes(const ViscositySolver* s) { return s ? s->bytes : 0; }
 int viscosity_levels(const ViscositySolver* s) { return s ? (int)s->levels.size() : 0; }
+void viscosity_set_smoother(ViscositySolver* s, int chebyshev_sweeps) { s->set_smoother(chebyshev_sweeps); }
 double* viscosity_material_buffer(ViscositySolver* s) { return s->material.p; }
 
 int viscosity_solve(ViscositySolver* s, hipStream_t st, double* u, double* v, const double* material, double dt,

@@ -558,6 +558,11 @@ class FluidGridHIP:
         levels run `coarse_sweeps` (1..4) red-black sweeps."""
         call("fs_set_pressure_smoother", self._h, int(kind), int(sweeps), int(coarse_sweeps))
 
+    def setViscositySmoother(self, chebyshev_sweeps: int):
+        """fs_set_viscosity_smoother: 0 restores the reference's V(2,2) damped-Jacobi viscosity smoother; 2, 4 or 6
+        smooths every viscosity multigrid level with that many Chebyshev-weighted Jacobi sweeps per side."""
+        call("fs_set_viscosity_smoother", self._h, int(chebyshev_sweeps))
+
     def getPressureSmoother(self):
         """(kind, sweeps, coarse_sweeps)."""
         k, n, c = C.c_int(), C.c_int(), C.c_int()

@@ -45,6 +45,7 @@ class Request:
     pressure_smoother: int = _abi.FS_SMOOTHER_RED_BLACK   # fs_set_pressure_smoother; smoother_sweeps 0 keeps
     smoother_sweeps: int = 0                              # the reference's V(4,4)
     coarse_sweeps: int = 0
+    viscosity_sweeps: int = 0                             # fs_set_viscosity_smoother (0: the reference's)
 
     def estimate_bytes(self) -> int:
         viscous = any(m > 0 for m in (self.liquid_viscosity or []) + (self.gas_viscosity or []))
@@ -149,7 +150,8 @@ class Server:
                         float(r.divergence_tolerance), float(r.gravity_y), float(r.surface_tension), int(r.transport),
                         int(r.preconditioner), nl, ng, *ptrs, int(r.multigrid_precision),
                         None if solid is None else solid.ctypes.data, int(bool(r.relative_tolerance)),
-                        int(r.pressure_smoother), int(r.smoother_sweeps), int(r.coarse_sweeps))
+                        int(r.pressure_smoother), int(r.smoother_sweeps), int(r.coarse_sweeps),
+                        int(r.viscosity_sweeps))
         rid = C.c_longlong()
         call("fs_server_submit", self._h, C.byref(req), C.byref(rid))   # copies the arrays
         return rid.value

@@ -257,6 +257,14 @@ fs_status fs_advect(fs_grid* grid, double dt, int* substeps);
  * no component is viscous).  Breakdown, non-finite input or non-convergence return FS_E_RUNTIME with the
  * velocities unchanged, as the reference throws. */
 fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations);
+/* Viscosity smoother (no reference counterpart; the reference's solve smooths every multigrid level with two
+ * damped-Jacobi sweeps of weight 1/2 per side, ViscosityKernels.h:214-236, ViscositySolverGPU.cpp:134-167).
+ * chebyshev_sweeps 2, 4 or 6 smooths with that many Jacobi sweeps per side weighted
+ * 1 / (2.125 + 1.875 cos(pi (k + 1/2) / n)) -- the reciprocals of the Chebyshev nodes of [0.25, 4], 4
+ * bounding the spectrum of D^-1 A -- so that the CG needs fewer iterations; 0 (default) restores the
+ * reference's smoother.  Applies to fs_apply_viscosity, fs_step and fs_batch_apply_viscosity on this grid.
+ * FS_E_INVALID_ARGUMENT for another value. */
+fs_status fs_set_viscosity_smoother(fs_grid* grid, int chebyshev_sweeps);
 /* ViscositySolverGPU::solve(u, v, material, dt, cell_size, tolerance, maximum)
  * (ViscositySolverGPU.cpp:169-294, .h:12-28) on host arrays: material is (rho, mu) per cell (2*W*H),
  * u ((W+1)*H) and v (W*(H+1)) are updated in place on success only.  max_iterations < 0 is the
@@ -565,6 +573,7 @@ typedef struct fs_request {
     int pressure_smoother;          /* fs_pressure_smoother */
     int smoother_sweeps;            /* 0 or 1..4 (red-black) / 1..8 (Chebyshev) on the levels above 4096 cells */
     int coarse_sweeps;              /* 0 (= 4) or 1..4 on the smaller levels */
+    int viscosity_sweeps;           /* fs_set_viscosity_smoother: 0 (the reference's) or 2, 4, 6 Chebyshev sweeps */
 } fs_request;
 
 enum { FS_REQUEST_QUEUED = 0, FS_REQUEST_RUNNING = 1, FS_REQUEST_DONE = 2, FS_REQUEST_FAILED = 3 };

@@ -707,6 +707,7 @@ struct orc_grid {
     int nl, ng;
     double *rho_l, *rho_g;
     double *mu_l, *mu_g;   /* component dynamic viscosities */
+    int vis_sweeps;        /* orc_set_viscosity_smoother (0: the reference's smoother) */
     double sigma;          /* surface tension coefficient */
     double *fraction, *u, *v, *p, *p_prev;
     double *lmass, *gmass;
@@ -2101,6 +2102,14 @@ void orc_configure_viscosity(orc_grid* g, const double* mu_l, const double* mu_g
         g->mu_g[k] = mu_g ? mu_g[k] : 0.0;
 }
 
+int orc_set_viscosity_smoother(orc_grid* g, int chebyshev_sweeps)
+{
+    if (chebyshev_sweeps != 0 && chebyshev_sweeps != 2 && chebyshev_sweeps != 4 && chebyshev_sweeps != 6)
+        return -1;
+    g->vis_sweeps = chebyshev_sweeps;
+    return 0;
+}
+
 int orc_apply_viscosity(orc_grid* g, double dt, int* iterations)
 {
     if (iterations)
@@ -2111,7 +2120,8 @@ int orc_apply_viscosity(orc_grid* g, double dt, int* iterations)
     double* material = dalloc(2 * N);
     orc_viscosity_material((int)N, g->h, g->nl, g->rho_l, g->mu_l, g->lmass, g->ng, g->rho_g, g->mu_g, g->gmass,
         material);
-    const int status = orc_viscosity_solve(g->W, g->H, g->h, material, g->u, g->v, dt, 1e-4, -1, iterations, NULL);
+    const int status = orc_viscosity_solve_smoothed(g->W, g->H, g->h, material, g->u, g->v, dt, 1e-4, -1,
+        g->vis_sweeps, iterations, NULL);
     free(material);
     return status;
 }

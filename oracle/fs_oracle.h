@@ -130,11 +130,18 @@ orc_result orc_step(orc_grid* g, double dt, int max_iterations, double divergenc
 void orc_sample_velocity(const orc_grid* g, double px, double py, double* vx, double* vy);
 
 /* ---- implicit viscosity (WebGPU-only in the reference; fs_viscosity.c) ---- */
-enum { ORC_VISC_OK = 0, ORC_VISC_NONFINITE = 1, ORC_VISC_BREAKDOWN = 2, ORC_VISC_NOT_CONVERGED = 3 };
+enum { ORC_VISC_OK = 0, ORC_VISC_NONFINITE = 1, ORC_VISC_BREAKDOWN = 2, ORC_VISC_NOT_CONVERGED = 3,
+    ORC_VISC_BAD_SMOOTHER = 4 };
 /* ViscositySolverGPU::solve (ViscositySolverGPU.cpp:169-294): material is (rho, mu) per cell; u, v are
  * updated in place only on success.  maximum < 0 means unbounded.  Returns an ORC_VISC_* status. */
 int orc_viscosity_solve(int width, int height, double cell_size, const double* material, double* u, double* v,
     double dt, double tolerance, int maximum, int* iterations, double* relative_residual);
+/* The same solve with chebyshev_sweeps (2, 4 or 6) Chebyshev-weighted Jacobi sweeps per side in place of the
+ * V(2,2) damped Jacobi (0: the reference's smoother); no reference counterpart. */
+int orc_viscosity_solve_smoothed(int width, int height, double cell_size, const double* material, double* u,
+    double* v, double dt, double tolerance, int maximum, int chebyshev_sweeps, int* iterations,
+    double* relative_residual);
+double orc_viscosity_chebyshev_weight(int k, int n);
 int orc_viscosity_levels(int width, int height);
 /* Material blend of FluidGridGPU::applyViscosity (FluidGridGPU.cpp:173-218). */
 void orc_viscosity_material(int cells, double cell_size, int n_liquid, const double* liquid_density,
@@ -146,6 +153,9 @@ void orc_configure_viscosity(orc_grid* g, const double* liquid_viscosity, const 
  * Returns an ORC_VISC_* status; *iterations receives the dispatched iteration count. */
 int orc_apply_viscosity(orc_grid* g, double dt, int* iterations);
 int orc_has_viscosity(const orc_grid* g);
+/* Viscosity smoother of orc_apply_viscosity (orc_viscosity_solve_smoothed's chebyshev_sweeps); returns 0, or
+ * -1 for a value other than 0, 2, 4, 6. */
+int orc_set_viscosity_smoother(orc_grid* g, int chebyshev_sweeps);
 
 /* ---- surface tension (FluidGridCPU.cpp:835-926, FluidSurfaceTensionCPU.h; fs_tension.c) ---- */
 int orc_set_surface_tension(orc_grid* g, double coefficient);

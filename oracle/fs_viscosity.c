@@ -16,6 +16,13 @@
  *   stops when the recomputed residual satisfies r M^-1 r <= tol^2 v0 M v0 + 1e-12 sum(M), checked
  *   every four iterations.
  *
+ * orc_viscosity_solve_smoothed replaces the V(2,2) damped-Jacobi smoother by n = 2, 4 or 6 Jacobi sweeps
+ * per side with the weights 1 / (2.125 + 1.875 cos(pi (k + 1/2) / n)) -- the reciprocals of the Chebyshev
+ * nodes of [0.25, 4] (pre-smoothing k = 0..n-1, post-smoothing in reverse).  This option has no
+ * counterpart in the reference; 4 bounds the spectrum of D^-1 A by Gershgorin's theorem (each row's
+ * off-diagonal magnitudes sum to at most three times its viscous diagonal part), so the smoother never
+ * amplifies an error component and the V-cycle stays a symmetric positive definite preconditioner.
+ *
  * Storage follows the shaders: every level keeps (w+1) x (h+1) "slots", slot p holding the u face
  * on the left of cell p and the v face on top of cell p; reads clamp to the field like the WGSL
  * accessors (FluidSim/Shared/WebGPU/FluidWebGPUCompute.h:241-262).
@@ -129,8 +136,8 @@ static void diagonal(const vlevel* L, double dt, int x, int y, double* du, doubl
     *dv = L->mv_m[p] + dt * (2.0 * (L->mc[p] + L->mc[slot(L, x, y - 1)]) + L->mv[p] + L->mv[slot(L, x + 1, y)]);
 }
 
-/* "viscosity symmetric smoother" (ViscosityKernels.h:214-236): damped Jacobi, weight 1/2. */
-static void smooth(vlevel* L, const double* bu, const double* bv, double dt, int zero)
+/* "viscosity symmetric smoother" (ViscosityKernels.h:214-236): damped Jacobi, weight 1/2 (w = 0.5). */
+static void smooth(vlevel* L, const double* bu, const double* bv, double dt, int zero, double w)
 {
     for (int y = 0; y <= L->h; ++y)
         for (int x = 0; x <= L->w; ++x) {
@@ -139,12 +146,12 @@ static void smooth(vlevel* L, const double* bu, const double* bv, double dt, int
             diagonal(L, dt, x, y, &du, &dv);
             double vu, vv;
             if (zero) {
-                vu = 0.5 * bu[p] / du;
-                vv = 0.5 * bv[p] / dv;
+                vu = w * bu[p] / du;
+                vv = w * bv[p] / dv;
             } else {
                 product(L, L->xu, L->xv, dt, x, y, &au, &av);
-                vu = L->xu[p] + 0.5 * (bu[p] - au) / du;
-                vv = L->xv[p] + 0.5 * (bv[p] - av) / dv;
+                vu = L->xu[p] + w * (bu[p] - au) / du;
+                vv = L->xv[p] + w * (bv[p] - av) / dv;
             }
             L->nu[p] = valid_u(L, x, y) ? vu : 0.0;
             L->nv[p] = valid_v(L, x, y) ? vv : 0.0;
@@ -260,14 +267,22 @@ static void coarse_solve(vlevel* L, const double* bu, const double* bv, double d
     memset(L->xu, 0, n * sizeof(double));
     memset(L->xv, 0, n * sizeof(double));
     for (int sweep = 0; sweep < 16; ++sweep)
-        smooth(L, bu, bv, dt, 0);
+        smooth(L, bu, bv, dt, 0, 0.5);
 }
 
 typedef struct {
     int nlev;
     vlevel* lev;
     double dt;
+    int sweeps;         /* per side */
+    double weight[6];   /* pre-smoothing weights; post-smoothing takes them in reverse */
 } vsolver;
+
+/* Weight k of n Chebyshev-weighted sweeps: 1 / (2.125 + 1.875 cos(pi (k + 1/2) / n)). */
+double orc_viscosity_chebyshev_weight(int k, int n)
+{
+    return 1.0 / (2.125 + 1.875 * cos(3.14159265358979323846 * (k + 0.5) / n));
+}
 
 /* cycle (ViscositySolverGPU.cpp:134-167) */
 static void cycle(vsolver* S, int level, const double* bu, const double* bv)
@@ -277,15 +292,15 @@ static void cycle(vsolver* S, int level, const double* bu, const double* bv)
         coarse_solve(L, bu, bv, S->dt);
         return;
     }
-    smooth(L, bu, bv, S->dt, 1);
-    smooth(L, bu, bv, S->dt, 0);
+    for (int k = 0; k < S->sweeps; ++k)
+        smooth(L, bu, bv, S->dt, k == 0, S->weight[k]);
     residual(L, L->xu, L->xv, bu, bv, S->dt, L->ru, L->rv);
     vlevel* C = &S->lev[level + 1];
     restrict_residual(L, C);
     cycle(S, level + 1, C->bu, C->bv);
     prolong(L, C);
-    smooth(L, bu, bv, S->dt, 0);
-    smooth(L, bu, bv, S->dt, 0);
+    for (int k = S->sweeps - 1; k >= 0; --k)
+        smooth(L, bu, bv, S->dt, 0, S->weight[k]);
 }
 
 static void level_alloc(vlevel* L, int w, int h, int fine)
@@ -334,6 +349,16 @@ static int finite3(double a, double b, double c) { return isfinite(a) && isfinit
 int orc_viscosity_solve(int W, int H, double spacing, const double* material, double* u, double* v, double dt,
     double tolerance, int maximum, int* iterations_out, double* relative_residual)
 {
+    return orc_viscosity_solve_smoothed(W, H, spacing, material, u, v, dt, tolerance, maximum, 0, iterations_out,
+        relative_residual);
+}
+
+/* chebyshev_sweeps 0: the reference's V(2,2) damped Jacobi; 2, 4 or 6: the Chebyshev-weighted sweeps. */
+int orc_viscosity_solve_smoothed(int W, int H, double spacing, const double* material, double* u, double* v,
+    double dt, double tolerance, int maximum, int chebyshev_sweeps, int* iterations_out, double* relative_residual)
+{
+    if (chebyshev_sweeps != 0 && chebyshev_sweeps != 2 && chebyshev_sweeps != 4 && chebyshev_sweeps != 6)
+        return ORC_VISC_BAD_SMOOTHER;
     if (iterations_out)
         *iterations_out = 0;
     if (relative_residual)
@@ -343,6 +368,9 @@ int orc_viscosity_solve(int W, int H, double spacing, const double* material, do
     const size_t NC = (size_t)W * H;
     vsolver S;
     S.dt = dt;
+    S.sweeps = chebyshev_sweeps ? chebyshev_sweeps : 2;
+    for (int k = 0; k < S.sweeps; ++k)
+        S.weight[k] = chebyshev_sweeps ? orc_viscosity_chebyshev_weight(k, chebyshev_sweeps) : 0.5;
     S.nlev = orc_viscosity_levels(W, H);
     S.lev = (vlevel*)calloc((size_t)S.nlev, sizeof(vlevel));
     double* rho0 = vz(NC);

@@ -97,6 +97,10 @@ def lib():
         "orc_sample_velocity": (None, [vp, C.c_double, C.c_double, _dp, _dp]),
         "orc_viscosity_solve": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double, C.c_double,
                                           C.c_int, _ip, _dp]),
+        "orc_viscosity_solve_smoothed": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double,
+                                                   C.c_double, C.c_int, C.c_int, _ip, _dp]),
+        "orc_viscosity_chebyshev_weight": (C.c_double, [C.c_int, C.c_int]),
+        "orc_set_viscosity_smoother": (C.c_int, [vp, C.c_int]),
         "orc_viscosity_levels": (C.c_int, [C.c_int, C.c_int]),
         "orc_viscosity_material": (None, [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
                                           _dp]),
@@ -328,6 +332,11 @@ class OracleGrid:
         self._mu = (_f64(liquid_viscosity), _f64(gas_viscosity))
         lib().orc_configure_viscosity(self._h, _ptr(self._mu[0]), _ptr(self._mu[1]))
 
+    def set_viscosity_smoother(self, chebyshev_sweeps):
+        """orc_set_viscosity_smoother: 0 (the reference's V(2,2) damped Jacobi) or 2, 4, 6 Chebyshev sweeps."""
+        if lib().orc_set_viscosity_smoother(self._h, chebyshev_sweeps) != 0:
+            raise ValueError("viscosity smoother sweeps must be 0, 2, 4 or 6")
+
     def apply_viscosity(self, dt):
         """FluidGridGPU::applyViscosity(dt); returns (status, iterations)."""
         it = C.c_int()
@@ -393,7 +402,7 @@ class OracleGrid:
         return r
 
 
-VISC_OK, VISC_NONFINITE, VISC_BREAKDOWN, VISC_NOT_CONVERGED = 0, 1, 2, 3
+VISC_OK, VISC_NONFINITE, VISC_BREAKDOWN, VISC_NOT_CONVERGED, VISC_BAD_SMOOTHER = 0, 1, 2, 3, 4
 
 
 def set_threads(threads):
@@ -419,16 +428,19 @@ def chebyshev_weight(i, sweeps):
     return lib().orc_chebyshev_weight(int(i), int(sweeps))
 
 
-def viscosity_solve(width, height, cell_size, material, u, v, dt, tolerance=1e-4, maximum=-1):
+def viscosity_solve(width, height, cell_size, material, u, v, dt, tolerance=1e-4, maximum=-1,
+                    chebyshev_sweeps=0):
     """ViscositySolverGPU::solve on host arrays (FP64 restatement).  material: (rho, mu) per cell.
-    Returns (status, iterations, relative_residual, u, v); u and v change only on success."""
+    chebyshev_sweeps 2, 4 or 6 replaces the V(2,2) damped-Jacobi smoother by that many Chebyshev-weighted
+    sweeps per side (no reference counterpart).  Returns (status, iterations, relative_residual, u, v); u
+    and v change only on success."""
     material = _f64(material)
     u = _f64(u).copy()
     v = _f64(v).copy()
     it = C.c_int()
     rel = C.c_double()
-    st = lib().orc_viscosity_solve(width, height, cell_size, _ptr(material), _ptr(u), _ptr(v), dt, tolerance,
-                                   maximum, C.byref(it), C.byref(rel))
+    st = lib().orc_viscosity_solve_smoothed(width, height, cell_size, _ptr(material), _ptr(u), _ptr(v), dt,
+                                            tolerance, maximum, chebyshev_sweeps, C.byref(it), C.byref(rel))
     return st, it.value, rel.value, u, v
 
 

@@ -199,14 +199,15 @@ def test_requests_with_obstacles_and_16bit_levels():
 
 
 def test_requests_with_a_smoother_choice():
-    """A request may carry a pressure smoother choice: its result equals the same steps on a plain grid with
-    the same choice, bit for bit; an invalid choice is refused at submission."""
+    """A request may carry pressure and viscosity smoother choices: its result equals the same steps on a plain
+    grid with the same choices, bit for bit; an invalid choice is refused at submission."""
     from fluidsim_amd import FS_MG_BF16, FS_SMOOTHER_CHEBYSHEV
 
     probe = request(150, 110, 50, steps=4, viscosity=0.01, sigma=0.02, pressure_smoother=FS_SMOOTHER_CHEBYSHEV,
-                    smoother_sweeps=3, coarse_sweeps=2)
+                    smoother_sweeps=3, coarse_sweeps=2, viscosity_sweeps=4)
     with Server(lanes=2) as srv:
-        for bad in (dict(smoother_sweeps=9), dict(coarse_sweeps=5), dict(multigrid_precision=FS_MG_BF16)):
+        for bad in (dict(smoother_sweeps=9), dict(coarse_sweeps=5), dict(multigrid_precision=FS_MG_BF16),
+                    dict(viscosity_sweeps=3)):
             with pytest.raises(ValueError):
                 srv.submit(request(64, 64, 51, pressure_smoother=FS_SMOOTHER_CHEBYSHEV,
                                    **{"smoother_sweeps": 3, **bad}))
@@ -224,13 +225,16 @@ def test_requests_with_a_smoother_choice():
     g.upload(FluidGridState(probe.liquid_fraction, probe.u, probe.v, np.zeros(0), [probe.liquid_mass],
                             [probe.gas_mass]))
     g.setPressureSmoother(FS_SMOOTHER_CHEBYSHEV, 3, 2)
-    iterations = 0
+    g.setViscositySmoother(4)
+    iterations = viscous = 0
     for _ in range(probe.steps):
         r = g.step(probe.dt, probe.max_iterations, probe.divergence_tolerance)
         assert r.accepted()
         iterations += r.iterations
+        viscous += r.viscosity_iterations
     s = g.download()
     g.destroy()
     assert iterations == rep.pressure_iterations
+    assert viscous == rep.viscosity_iterations > 0
     for k, key in (("u", "u"), ("v", "v"), ("pressure", "pressure"), ("liquid_fraction", "fraction")):
         assert np.array_equal(getattr(s, k), out[key]), k

@@ -67,6 +67,25 @@ class Case:
 
 def strain_residual(old, new, dt):
     """Independent strain-energy residual (tests/FluidViscosityTests.cpp:82-138)."""
+    ku, kv = strain_forces(new)
+    err = 0.0
+    for y in range(new.h):
+        for x in range(1, new.w):
+            i = new.ui(x, y)
+            m = new.um(x, y)
+            r = m * (new.u[i] - old.u[i]) + dt * ku[i]
+            err += r * r / m
+    for y in range(1, new.h):
+        for x in range(new.w):
+            i = new.vi(x, y)
+            m = new.vm(x, y)
+            r = m * (new.v[i] - old.v[i]) + dt * kv[i]
+            err += r * r / m
+    return math.sqrt(err / max(2 * old.energy(), 1e-30))
+
+
+def strain_forces(new):
+    """K v from the cell and vertex strain energies (tests/FluidViscosityTests.cpp:82-122)."""
     ku = np.zeros(new.u.size)
     kv = np.zeros(new.v.size)
     for y in range(new.h):
@@ -85,20 +104,7 @@ def strain_residual(old, new, dt):
             ku[new.ui(x, y - 1)] -= t
             kv[new.vi(x, y)] += t
             kv[new.vi(x - 1, y)] -= t
-    err = 0.0
-    for y in range(new.h):
-        for x in range(1, new.w):
-            i = new.ui(x, y)
-            m = new.um(x, y)
-            r = m * (new.u[i] - old.u[i]) + dt * ku[i]
-            err += r * r / m
-    for y in range(1, new.h):
-        for x in range(new.w):
-            i = new.vi(x, y)
-            m = new.vm(x, y)
-            r = m * (new.v[i] - old.v[i]) + dt * kv[i]
-            err += r * r / m
-    return math.sqrt(err / max(2 * old.energy(), 1e-30))
+    return ku, kv
 
 
 def solve(O, case, dt, tol=1e-4, maximum=-1):
@@ -278,3 +284,102 @@ def test_grid_step_with_viscosity_dissipates(oracle):
         assert (U[:, 0] == 0).all() and (U[:, W] == 0).all() and (V[0] == 0).all() and (V[H] == 0).all()
         energies.append(float((s["u"] ** 2).sum() + (s["v"] ** 2).sum()))
     assert energies[1] < energies[0]
+
+
+# ---- Chebyshev-weighted viscosity smoother (orc_viscosity_solve_smoothed; no reference counterpart) ----
+def operator_matrix(case, dt):
+    """M + dt K on the interior faces, column by column from strain_forces (independent of the oracle)."""
+    faces = [("u", case.ui(x, y), case.um(x, y)) for y in range(case.h) for x in range(1, case.w)]
+    faces += [("v", case.vi(x, y), case.vm(x, y)) for y in range(1, case.h) for x in range(case.w)]
+    n = len(faces)
+    A = np.zeros((n, n))
+    probe = case.copy()
+    for j, (comp, idx, mass) in enumerate(faces):
+        probe.u[:] = 0.0
+        probe.v[:] = 0.0
+        (probe.u if comp == "u" else probe.v)[idx] = 1.0
+        ku, kv = strain_forces(probe)
+        for i, (ci, ii, _) in enumerate(faces):
+            A[i, j] = dt * (ku[ii] if ci == "u" else kv[ii])
+        A[j, j] += mass
+    return A
+
+
+def test_viscosity_spectrum_is_bounded_by_four():
+    """The Chebyshev interval's upper end: every eigenvalue of D^-1 (M + dt K) lies in (0, 4] for any
+    material (Gershgorin: each row's off-diagonal magnitudes sum to at most three times its viscous
+    diagonal part), and exceeds 2 -- so the pressure smoother's interval [0.2, 2] would not do here."""
+    rng = np.random.default_rng(11)
+    top = 0.0
+    for trial in range(6):
+        c = Case(6, 5, 1.0, 0.0, dx=0.5)
+        c.material[0::2] = rng.uniform(0.1, 10.0, c.w * c.h)
+        mu = rng.uniform(0.0, 1e4, c.w * c.h)
+        if trial % 2:
+            mu *= (np.indices((c.h, c.w)).sum(axis=0).ravel() % 2)   # checkerboard of inviscid cells
+        c.material[1::2] = mu
+        A = operator_matrix(c, 1.0 if trial < 3 else 1e-3)
+        assert np.allclose(A, A.T, rtol=1e-12, atol=1e-9)
+        d = np.sqrt(np.diag(A))
+        lam = np.linalg.eigvalsh(A / d[:, None] / d[None, :])
+        assert lam.min() > 0.0
+        assert lam.max() <= 4.0 + 1e-9
+        top = max(top, lam.max())
+    assert top > 2.2
+
+
+@pytest.mark.parametrize("n", [2, 4, 6])
+def test_chebyshev_viscosity_weights_give_the_chebyshev_residual_polynomial(oracle, n):
+    """prod_k (1 - w_k lam) = T_n((4.25 - 2 lam) / 3.75) / T_n(4.25 / 3.75): the weights are the reciprocals of
+    the Chebyshev nodes of [0.25, 4], so the error factor is at most 1 / T_n(4.25 / 3.75) there."""
+    w = [oracle.lib().orc_viscosity_chebyshev_weight(k, n) for k in range(n)]
+    cheb = np.polynomial.chebyshev.Chebyshev.basis(n)
+    lam = np.linspace(0.0, 4.0, 81)
+    prod = np.prod([1.0 - wk * lam for wk in w], axis=0)
+    expect = cheb((4.25 - 2.0 * lam) / 3.75) / cheb(4.25 / 3.75)
+    assert np.allclose(prod, expect, rtol=0, atol=1e-12)
+    inside = lam >= 0.25
+    assert np.abs(prod[inside]).max() <= 1.0 / cheb(4.25 / 3.75) + 1e-12
+    assert np.all(prod[~inside] > 0) and np.all(prod[~inside] <= 1.0)
+
+
+def smoothed(O, case, dt, n, tol=1e-4):
+    st, it, rel, u, v = O.viscosity_solve(case.w, case.h, case.dx, case.material, case.u, case.v, dt, tol, -1, n)
+    return st, it, u, v
+
+
+@pytest.mark.parametrize("n", [2, 4, 6])
+def test_chebyshev_viscosity_smoother_solves_to_the_same_bar(oracle, n):
+    """The decay and variable-viscosity checks above, with n Chebyshev-weighted sweeps per side: the same
+    analytic factor and independent strain residual, and no more iterations than the reference's smoother
+    from four sweeps on."""
+    c, sx, sy = decay_case(31, 23, 1000, True)
+    old = c.copy()
+    st, it, c.u, c.v = smoothed(oracle, c, 0.02, n)
+    assert st == oracle.VISC_OK and it > 0 and it % 4 == 0
+    c.walls()
+    factor = 1.0 / (1.0 + 0.02 * 1000 * (sx * sx + sy * sy) / (2 * c.dx * c.dx))
+    assert max(np.abs(c.u - factor * old.u).max(), np.abs(c.v - factor * old.v).max()) < 3e-4
+    assert strain_residual(old, c, 0.02) < 1.5e-4
+    c = variable_case(64, 48)
+    for _ in range(3):
+        old = c.copy()
+        st, it, c.u, c.v = smoothed(oracle, c, 0.02, n)
+        assert st == oracle.VISC_OK
+        assert c.energy() <= old.energy() * (1 + 2e-6)
+        assert strain_residual(old, c, 0.02) < 1.8e-4
+        st0, it0, _, _ = solve(oracle, old, 0.02)
+        if n >= 4:
+            assert it <= it0
+
+
+def test_chebyshev_viscosity_smoother_rejects_other_counts(oracle):
+    c = variable_case(9, 7)
+    for n in (1, 3, 5, 8, -2):
+        st, it, u, v = smoothed(oracle, c, 0.02, n)
+        assert st == oracle.VISC_BAD_SMOOTHER and np.array_equal(u, c.u)
+    g = oracle.OracleGrid(9, 7, 0.1)
+    with pytest.raises(ValueError):
+        g.set_viscosity_smoother(3)
+    g.set_viscosity_smoother(4)
+    g.set_viscosity_smoother(0)
