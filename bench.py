@@ -272,6 +272,11 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
     from fluidsim_amd import workloads as WL
     from fluidsim_amd.serving import Request, Server
 
+    t_start = time.perf_counter()
+
+    def note(msg):
+        print(f"[serving leg {time.perf_counter() - t_start:7.2f} s] {msg}", file=sys.stderr, flush=True)
+
     def stream_at(rate, precision=None, cells=batch_cells, choice=None):
         out = []
         for t, kw in WL.serving_stream(seed, count, 128, max_size, rate, steps):
@@ -297,6 +302,8 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
             serial_runs.append((time.perf_counter() - t0, srv.stats()))
     serial_runs.sort(key=lambda v: v[0])
     serial_s, serial = serial_runs[len(serial_runs) // 2]
+    note(f"served one at a time: {total_steps / serial_s:.1f} steps/s")
+
     def serve(stream, cells):
         with Server(lanes=lanes, batch_cells=cells) as srv:
             ids = []
@@ -324,6 +331,7 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
             samples.append(serve(stream, batch_cells))
         samples.sort(key=lambda v: v[0])
         served_s, st, queue, setup_ms, teardown_ms = samples[len(samples) // 2]
+        note(f"rate {rate}/s: {total_steps / served_s:.1f} steps/s")
         runs.append({
             "rate_per_s": rate,
             "arrival_span_s": stream[-1][0],
@@ -715,9 +723,10 @@ def main():
                     help="C5: pressure smoother of the served requests (FP32 levels), red_black:N:C or "
                          "chebyshev:N:C; empty: the reference's V(4,4) with bfloat16 conductances.  The default, "
                          "8 Chebyshev sweeps and 3 coarse sweeps, is the fastest measured on this stream")
-    ap.add_argument("--serving-viscosity-sweeps", type=int, default=0, choices=(0, 2, 4, 6),
+    ap.add_argument("--serving-viscosity-sweeps", type=int, default=4, choices=(0, 2, 4, 6),
                     help="C5: viscosity smoother of the served requests with a pressure smoother choice "
-                         "(fs_set_viscosity_smoother: 0 the reference's V(2,2), else Chebyshev sweeps per side)")
+                         "(fs_set_viscosity_smoother: 0 the reference's V(2,2), else Chebyshev sweeps per side; "
+                         "the default, 4, is the fastest measured on this stream)")
     ap.add_argument("--serving-compare-batch-cells", type=int, default=65536,
                     help="C5: one extra run at the last rate with this batch size, for comparison (0: skip)")
     ap.add_argument("--serving-repeats", type=int, default=3, help="runs per pass; the median is reported")
