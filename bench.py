@@ -303,6 +303,18 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
     serial_runs.sort(key=lambda v: v[0])
     serial_s, serial = serial_runs[len(serial_runs) // 2]
     note(f"served one at a time: {total_steps / serial_s:.1f} steps/s")
+    # one more serial pass with the served requests' smoothers, to separate what the smoothers give from
+    # what serving several requests at once gives
+    serial_same = None
+    if smoother:
+        with Server(lanes=1) as srv:
+            t0 = time.perf_counter()
+            for _, r in stream_at(rates[0], FS_MG_FP32, choice=smoother):
+                rep = srv.wait(srv.submit(r))
+                if not rep.done:
+                    raise RuntimeError(f"serial request failed: {rep.error}")
+            serial_same = time.perf_counter() - t0
+        note(f"served one at a time with the served smoothers: {total_steps / serial_same:.1f} steps/s")
 
     def serve(stream, cells):
         with Server(lanes=lanes, batch_cells=cells) as srv:
@@ -382,6 +394,9 @@ def serving_leg(count, rates, steps, lanes, seed=5, max_size=2048, repeats=3, ba
         "serial_step_ms": {"p50": serial["step_p50_ms"], "p99": serial["step_p99_ms"]},
         "served": runs,
         "served_reference_cycle": reference,
+        "serial_with_served_smoothers": None if serial_same is None else {
+            "steps_per_s": total_steps / serial_same,
+            "served_speedup_over_it": [serial_same / (total_steps / r["steps_per_s"]) for r in runs]},
         "served_batched": compare,
     }
 
