@@ -285,3 +285,34 @@ def test_batched_chebyshev_viscosity_matches_oracle(oracle):
         scale = max(np.abs(u).max(), np.abs(v).max())
         assert max(np.abs(got.u - so["u"]).max(), np.abs(got.v - so["v"]).max()) <= 3e-4 * scale, n
         g.destroy()
+
+
+@pytest.mark.parametrize("w,h,n", [(47, 1, 6), (2, 333, 4), (1100, 2, 6), (3, 700, 2)])
+def test_chebyshev_viscosity_smoother_on_thin_grids(oracle, w, h, n):
+    """Thin grids with the Chebyshev smoother: one-row and two-column fields through the LDS-tile kernel (the
+    1100 x 2 and 3 x 700 fields' finest levels exceed the one-workgroup tail) and the tail, against the FP64
+    oracle with the same sweeps, to the 3e-4 bar."""
+    c, sx, sy = decay_case(w, h, 1000, h > 1 and w > 1)
+    g = FluidGridHIP(w, h, c.dx)
+    g.configureMaterials([], FluidMixture([MixtureComponent("l", 2.0, 1000.0)]),
+                         FluidMixture([MixtureComponent("g", 2.0, 1000.0)]))
+    g.setGravityY(0.0)
+    f = np.full(w * h, 0.5)
+    area = c.dx * c.dx
+    lm, gm = [area * f * 2.0], [area * (1 - f) * 2.0]
+    g.upload(FluidGridState(f, c.u, c.v, np.zeros(0), lm, gm))
+    g.setViscositySmoother(n)
+    it = g.applyViscosity(0.02)
+    got = g.download()
+    g.destroy()
+    o = oracle.OracleGrid(w, h, c.dx)
+    o.configure([2.0], [2.0])
+    o.configure_viscosity([1000.0], [1000.0])
+    o.set_viscosity_smoother(n)
+    o.upload(f, c.u, c.v, None, lm[0], gm[0])
+    st, oit = o.apply_viscosity(0.02)
+    assert st == oracle.VISC_OK and abs(it - oit) <= 4, (it, oit)
+    so = o.download()
+    scale = max(np.abs(c.u).max(), np.abs(c.v).max(), 1e-12)
+    err = max(np.abs(got.u - so["u"]).max(), np.abs(got.v - so["v"]).max())
+    assert err <= 3e-4 * scale, f"GPU vs FP64 oracle {err:.3e} (scale {scale:.3e})"
