@@ -190,6 +190,7 @@ SIGNATURES = {
     "fs_advect": [_vp, C.c_double, _ip],
     "fs_apply_viscosity": [_vp, C.c_double, _ip],
     "fs_set_viscosity_smoother": [_vp, C.c_int],
+    "fs_tune_viscosity_smoother": [_vp, C.c_double, _ip, _dp],
     "fs_apply_surface_tension": [_vp, C.c_double],
     "fs_capillary_geometry": [_vp, _dp, _dp],
     "fs_set_component_viscosity": [_vp, C.c_int, C.c_int, C.c_double],

@@ -2648,6 +2648,68 @@ fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations)
     });
 }
 
+fs_status fs_tune_viscosity_smoother(fs_grid* grid, double dt, int* chebyshev_sweeps, double* ms)
+{
+    return guard([&] {
+        need(grid != nullptr, FS_E_INVALID_ARGUMENT, "null grid");
+        need(grid->uploaded, FS_E_LOGIC, "Upload the grid state before tuning");
+        need(std::isfinite(dt) && dt > 0.0, FS_E_INVALID_ARGUMENT, "dt must be finite and positive");
+        fs_grid& g = *grid;
+        need(g.has_viscosity(), FS_E_LOGIC, "The grid has no viscosity to tune");
+        set_device(g.ctx);
+        g.drain();
+        ProjectionState saved;
+        saved.save(g);
+        auto choose = [&](int n) {
+            HIPC(hipStreamSynchronize(g.st()));
+            g.vis_sweeps = n;
+            if (g.visc)
+                viscosity_set_smoother(g.visc, n);
+        };
+        const int original = g.vis_sweeps;
+        int best = original;
+        double best_ms = std::numeric_limits<double>::infinity();
+        try {
+            for (int n : {0, 2, 4, 6}) {
+                choose(n);
+                double fastest = std::numeric_limits<double>::infinity();
+                bool converged = true;
+                for (int rep = 0; rep < 3; ++rep) {   // the first is a warm-up (graph capture, first replays)
+                    saved.restore(g);
+                    HIPC(hipStreamSynchronize(g.st()));
+                    const auto t0 = std::chrono::steady_clock::now();
+                    double* material = blend_viscosity_material(g);   // creates the solver with g.vis_sweeps
+                    int iterations = 0;
+                    double relative = 0.0;
+                    const int status = viscosity_solve(g.visc, g.st(), g.u.p, g.v.p, material, dt, g.h, 1e-4,
+                        kViscosityBudget, &iterations, &relative);
+                    HIPC(hipStreamSynchronize(g.st()));
+                    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    converged = converged && status != 1 && status != 2 && status != 3;
+                    if (rep > 0)
+                        fastest = std::min(fastest, t);
+                }
+                if (converged && fastest < best_ms) {
+                    best_ms = fastest;
+                    best = n;
+                }
+            }
+        } catch (...) {
+            choose(original);
+            saved.restore(g);
+            HIPC(hipStreamSynchronize(g.st()));
+            throw;
+        }
+        choose(best);
+        saved.restore(g);
+        HIPC(hipStreamSynchronize(g.st()));
+        if (chebyshev_sweeps)
+            *chebyshev_sweeps = best;
+        if (ms)
+            *ms = best_ms;
+    });
+}
+
 // Every grid of a batch shares `ctx`, is uploaded and appears once.
 static fs_context* batch_context(int count, fs_grid* const* grids, const char* upload_message)
 {

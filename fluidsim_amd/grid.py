@@ -563,6 +563,13 @@ class FluidGridHIP:
         smooths every viscosity multigrid level with that many Chebyshev-weighted Jacobi sweeps per side."""
         call("fs_set_viscosity_smoother", self._h, int(chebyshev_sweeps))
 
+    def tuneViscositySmoother(self, dt: float):
+        """fs_tune_viscosity_smoother: keeps the fastest converging viscosity smoother (0, 2, 4, 6) for a
+        viscosity solve of the current state, which stays unchanged; returns (sweeps, ms)."""
+        n, ms = C.c_int(), C.c_double()
+        call("fs_tune_viscosity_smoother", self._h, float(dt), C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
     def getPressureSmoother(self):
         """(kind, sweeps, coarse_sweeps)."""
         k, n, c = C.c_int(), C.c_int(), C.c_int()

@@ -329,6 +329,13 @@ public:
     }
     // Viscosity smoother (fs_set_viscosity_smoother): 0 the reference's V(2,2) damped Jacobi, 2/4/6 Chebyshev sweeps.
     void setViscositySmoother(int chebyshev_sweeps) { check(fs_set_viscosity_smoother(grid_, chebyshev_sweeps)); }
+    // Keeps the fastest converging viscosity smoother for a solve of the current state; returns its sweeps.
+    int tuneViscositySmoother(double dt)
+    {
+        int n = 0;
+        check(fs_tune_viscosity_smoother(grid_, dt, &n, nullptr));
+        return n;
+    }
     // Keeps the fastest converging smoother for a projection of the current state (fs_tune_pressure_smoother);
     // returns (kind, sweeps, coarse sweeps).
     std::tuple<fs_pressure_smoother, int, int> tunePressureSmoother(double dt, int max_iterations,

@@ -265,6 +265,11 @@ fs_status fs_apply_viscosity(fs_grid* grid, double dt, int* iterations);
  * reference's smoother.  Applies to fs_apply_viscosity, fs_step and fs_batch_apply_viscosity on this grid.
  * FS_E_INVALID_ARGUMENT for another value. */
 fs_status fs_set_viscosity_smoother(fs_grid* grid, int chebyshev_sweeps);
+/* Viscosity smoother autotuning (no reference counterpart): times fs_apply_viscosity(dt) of the current state
+ * with each smoother (0, 2, 4, 6; best of two runs after a warm-up), keeps the fastest that converges and
+ * leaves the velocities as they were.  *chebyshev_sweeps and *ms (may be NULL) receive the choice and its
+ * time.  FS_E_LOGIC when the grid has no viscosity or no uploaded state. */
+fs_status fs_tune_viscosity_smoother(fs_grid* grid, double dt, int* chebyshev_sweeps, double* ms);
 /* ViscositySolverGPU::solve(u, v, material, dt, cell_size, tolerance, maximum)
  * (ViscositySolverGPU.cpp:169-294, .h:12-28) on host arrays: material is (rho, mu) per cell (2*W*H),
  * u ((W+1)*H) and v (W*(H+1)) are updated in place on success only.  max_iterations < 0 is the

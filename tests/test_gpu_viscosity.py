@@ -316,3 +316,30 @@ def test_chebyshev_viscosity_smoother_on_thin_grids(oracle, w, h, n):
     scale = max(np.abs(c.u).max(), np.abs(c.v).max(), 1e-12)
     err = max(np.abs(got.u - so["u"]).max(), np.abs(got.v - so["v"]).max())
     assert err <= 3e-4 * scale, f"GPU vs FP64 oracle {err:.3e} (scale {scale:.3e})"
+
+
+def test_viscosity_smoother_tuner_keeps_state_and_sets_its_choice():
+    """fs_tune_viscosity_smoother: picks one of 0, 2, 4, 6, leaves the velocities bit for bit as they were,
+    and leaves the grid set to its choice (the next solve equals a grid set to it by hand, bit for bit); a
+    grid without viscosity is refused."""
+    W, H, h, dt = 150, 110, 0.05, 0.05
+    g, *_ = _mixture_grid(2, 2, W, H, h)
+    before = g.download()
+    n, ms = g.tuneViscositySmoother(dt)
+    assert n in (0, 2, 4, 6) and ms > 0.0
+    after = g.download()
+    assert np.array_equal(after.u, before.u) and np.array_equal(after.v, before.v)
+    it = g.applyViscosity(dt)
+    tuned = g.download()
+    ref, *_ = _mixture_grid(2, 2, W, H, h)
+    ref.setViscositySmoother(n)
+    assert ref.applyViscosity(dt) == it
+    same = ref.download()
+    assert np.array_equal(same.u, tuned.u) and np.array_equal(same.v, tuned.v)
+    g.destroy()
+    ref.destroy()
+    dry, *_ = _mixture_grid(1, 1, 24, 18, 0.7)
+    dry.setComponentViscosity("liquid", 0, 0.0)
+    with pytest.raises(RuntimeError):
+        dry.tuneViscositySmoother(dt)
+    dry.destroy()
